@@ -1,0 +1,323 @@
+// C entry points of the oracle for the test suite and the CPU baseline in
+// bench.py.  TEST INFRASTRUCTURE: only tests/, the smoke check and the CPU
+// baseline leg load this library.
+#include <cstring>
+#include <string>
+
+#include "orc.h"
+
+using namespace orc;
+
+extern "C" {
+
+// Geometry block shared with the product C-ABI (include/fr_amd.h).
+typedef struct orc_geom {
+  int Nx, Ny, Nz, bs;
+  double dx, dy, dz, Lx, Ly, Lz;
+  int is_box;
+} orc_geom;
+
+static Geom mk(const orc_geom* gg) {
+  Geom g;
+  g.setup(gg->Nx - 2 * gg->bs, gg->Ny - 2 * gg->bs, gg->Nz - 2 * gg->bs, gg->bs,
+          gg->Lx, gg->Ly, gg->Lz, gg->is_box != 0);
+  return g;
+}
+static void load(F0& f, const double* p) { std::memcpy(f.v.data(), p, f.v.size() * 8); }
+static void store(const F0& f, double* p) { std::memcpy(p, f.v.data(), f.v.size() * 8); }
+
+// ---- tensor0/tensor1 operators (each includes its reference halo) ---------
+void orc_div_f2c(const orc_geom* gg, double* out, const double* tx,
+                 const double* ty, const double* tz) {
+  Geom g = mk(gg);
+  F1 T;
+  T.init(&g);
+  load(T.x, tx);
+  load(T.y, ty);
+  load(T.z, tz);
+  F0 o;
+  o.init(&g);
+  load(o, out);
+  o.Equal_Div_F2C(T);
+  store(o, out);
+}
+void orc_del2(const orc_geom* gg, double* out, const double* in) {
+  Geom g = mk(gg);
+  F0 a, o;
+  a.init(&g);
+  o.init(&g);
+  load(a, in);
+  load(o, out);
+  o.Equal_Del2(a);
+  store(o, out);
+}
+void orc_grad_c2f(const orc_geom* gg, double* ox, double* oy, double* oz,
+                  const double* in) {
+  Geom g = mk(gg);
+  F0 a;
+  a.init(&g);
+  load(a, in);
+  F1 o;
+  o.init(&g);
+  load(o.x, ox);
+  load(o.y, oy);
+  load(o.z, oz);
+  o.Equal_Grad_C2F(a);
+  store(o.x, ox);
+  store(o.y, oy);
+  store(o.z, oz);
+}
+// axis 0/1/2, dir 0 = C2F, 1 = F2C
+void orc_interp(const orc_geom* gg, double* out, const double* in, int axis,
+                int dir) {
+  Geom g = mk(gg);
+  F0 a, o;
+  a.init(&g);
+  o.init(&g);
+  load(a, in);
+  load(o, out);
+  if (dir == 0) {
+    if (axis == 0) o.Equal_Ix_C2F(a);
+    if (axis == 1) o.Equal_Iy_C2F(a);
+    if (axis == 2) o.Equal_Iz_C2F(a);
+  } else {
+    if (axis == 0) o.Equal_Ix_F2C(a);
+    if (axis == 1) o.Equal_Iy_F2C(a);
+    if (axis == 2) o.Equal_Iz_F2C(a);
+  }
+  store(o, out);
+}
+void orc_i_f2c(const orc_geom* gg, double* out, const double* tx,
+               const double* ty, const double* tz) {
+  Geom g = mk(gg);
+  F1 T;
+  T.init(&g);
+  load(T.x, tx);
+  load(T.y, ty);
+  load(T.z, tz);
+  F0 o;
+  o.init(&g);
+  load(o, out);
+  o.Equal_I_F2C(T);
+  store(o, out);
+}
+void orc_update_ghosts(const orc_geom* gg, double* f, int cumulative) {
+  Geom g = mk(gg);
+  F0 a;
+  a.init(&g);
+  load(a, f);
+  if (cumulative)
+    a.Update_Ghosts_CUM();
+  else
+    a.Update_Ghosts();
+  store(a, f);
+}
+// op: 0 sum, 1 mean, 2 max, 3 min, 4 max_abs, 5 sum_squares
+double orc_reduce(const orc_geom* gg, const double* f, int op) {
+  Geom g = mk(gg);
+  F0 a;
+  a.init(&g);
+  load(a, f);
+  switch (op) {
+    case 0: return a.sum();
+    case 1: return a.mean();
+    case 2: return a.max();
+    case 3: return a.min();
+    case 4: return a.max_abs();
+    case 5: return a.sum_squares();
+  }
+  return 0;
+}
+void orc_reducex(const orc_geom* gg, const double* f, double* out) {
+  Geom g = mk(gg);
+  F0 a;
+  a.init(&g);
+  load(a, f);
+  a.reduceX();
+  std::memcpy(out, a.red.data(), a.red.size() * 8);
+}
+
+// ---- FFT and pressure solves -----------------------------------------------
+void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign) {
+  size_t n = (size_t)nx * ny * nz;
+  std::vector<double> r(re, re + n), i(im, im + n);
+  fft3d(r, i, nx, ny, nz, sign);
+  std::memcpy(re, r.data(), n * 8);
+  std::memcpy(im, i.data(), n * 8);
+}
+void orc_fft2d(int ny, int nz, double* re, double* im, int sign) {
+  fft2d(re, im, ny, nz, sign);
+}
+// Box (is_box=1, poisson.cpp:391-462) or duct (is_box=0, poisson_IO.cpp:
+// 81-452) defect-correction solve of div(C grad P) = RHS.  Returns the
+// iteration count.
+int orc_poisson(const orc_geom* gg, const double* cx, const double* cy,
+                const double* cz, double* P, const double* RHS, int iterations,
+                double epsilon) {
+  Geom g = mk(gg);
+  Params par;
+  par.Iteration = iterations;
+  par.epsilon = epsilon;
+  F1 C;
+  C.init(&g);
+  load(C.x, cx);
+  load(C.y, cy);
+  load(C.z, cz);
+  F0 p, r;
+  p.init(&g);
+  r.init(&g);
+  load(p, P);
+  load(r, RHS);
+  int it;
+  if (g.box) {
+    BoxPoisson s;
+    s.init(&par, &g);
+    s.Solve(C, p, r);
+    it = s.num_iteration;
+  } else {
+    DuctPoisson s;
+    s.init(&par, &g);
+    s.Solve(C, p, r);
+    it = s.num_iteration1;
+  }
+  store(p, P);
+  return it;
+}
+void orc_glibc_rand(int n, int* out) {
+  GlibcRand r;
+  r.seed(1);
+  for (int i = 0; i < n; i++) out[i] = r.next();
+}
+
+// ---- whole grid (box) ---------------------------------------------------------
+void* orc_grid_create(const char* params_path) {
+  Params p;
+  if (!p.parse(params_path)) return nullptr;
+  Grid* g = new Grid();
+  g->setup(p);
+  return g;
+}
+void orc_grid_destroy(void* h) { delete (Grid*)h; }
+void orc_grid_initialize(void* h, const double* U, const double* V,
+                         const double* W) {
+  ((Grid*)h)->Initialize(U, V, W);
+}
+void orc_grid_finish_initialize(void* h) { ((Grid*)h)->Finish_Initialize(); }
+void orc_grid_step(void* h) { ((Grid*)h)->box_step(); }
+// Run one named grid method (the names of grid.h:93-114).
+int orc_grid_call(void* h, const char* name, int rk4_count) {
+  Grid* g = (Grid*)h;
+  std::string n(name);
+  g->RK4_count = rk4_count;
+  if (n == "update") g->par.update(g->T_cur);
+  else if (n == "Update_Rho") g->Update_Rho();
+  else if (n == "Update_P0") g->Update_P0();
+  else if (n == "Update_Particle") g->Update_Particle();
+  else if (n == "Update_RU_WOP") g->Update_RU_WOP();
+  else if (n == "Compute_Div_U_new") g->Compute_Div_U_new();
+  else if (n == "Compute_RHS_Pois") g->Compute_RHS_Pois();
+  else if (n == "Solve_Poisson") g->Solve_Poisson();
+  else if (n == "Update_RU_WP") g->Update_RU_WP();
+  else if (n == "TimeAdvance_RK4") g->TimeAdvance_RK4();
+  else if (n == "Correct_mean_RU") g->Correct_mean_RU();
+  else if (n == "TimeAdvance") g->TimeAdvance();
+  else if (n == "Send_Recv") g->part.Send_Recv();
+  else return -1;
+  return 0;
+}
+// Field access: comp 0..2 selects a tensor1 component, -1 a tensor0.
+double* orc_grid_field(void* h, const char* name, int comp) {
+  Grid* g = (Grid*)h;
+  if (comp < 0) {
+    F0* f = g->field0(name);
+    return f ? f->v.data() : nullptr;
+  }
+  F1* f = g->field1(name);
+  if (!f) return nullptr;
+  return comp == 0 ? f->x.v.data() : comp == 1 ? f->y.v.data() : f->z.v.data();
+}
+// scalars: P0, P0_int, P0_new, P0_np1, T_cur, C_BC, num_timestep
+double* orc_grid_scalar(void* h, const char* name) {
+  Grid* g = (Grid*)h;
+  std::string n(name);
+  if (n == "P0") return &g->P0;
+  if (n == "P0_int") return &g->P0_int;
+  if (n == "P0_new") return &g->P0_new;
+  if (n == "P0_np1") return &g->P0_np1;
+  if (n == "T_cur") return &g->T_cur;
+  if (n == "C_BC") return &g->C_BC;
+  return nullptr;
+}
+int orc_grid_np(void* h) { return ((Grid*)h)->part.Np; }
+void orc_grid_set_np(void* h, int np) {
+  Grid* g = (Grid*)h;
+  g->part.Np = np;
+  g->part.RESIZE();
+}
+// Particle stage arrays: q 0..6 = x,y,z,u,v,w,T; s 0..3 = base,_int,_new,_np1;
+// q = 7 returns RHS_T (s ignored).
+double* orc_grid_part(void* h, int q, int s) {
+  Grid* g = (Grid*)h;
+  if (q == 7) return g->part.RHS_T.data();
+  return g->part.stage(q, s)->data();
+}
+int* orc_grid_flag(void* h) { return ((Grid*)h)->part.flag.data(); }
+void orc_grid_diagnostics(void* h, double* out) { ((Grid*)h)->diagnostics(out); }
+int orc_grid_num_timestep(void* h) { return ((Grid*)h)->num_timestep; }
+void orc_grid_geom(void* h, orc_geom* out) {
+  Grid* g = (Grid*)h;
+  out->Nx = g->g.Nx;
+  out->Ny = g->g.Ny;
+  out->Nz = g->g.Nz;
+  out->bs = g->g.bs;
+  out->dx = g->g.dx;
+  out->dy = g->g.dy;
+  out->dz = g->g.dz;
+  out->Lx = g->g.Lx;
+  out->Ly = g->g.Ly;
+  out->Lz = g->g.Lz;
+  out->is_box = g->g.box;
+}
+
+// Particle physics on the grid's particle set, same op codes as the partial
+// reference build's harness (oracle/ref_harness.cpp ref_part_op).
+void orc_grid_part_op(void* h, int op, int k, double pre, double post,
+                      double* f0, double* f1, double* f2) {
+  Grid* gr = (Grid*)h;
+  Particles& p = gr->part;
+  const Geom* g = &gr->g;
+  F1 U;
+  F0 A;
+  U.init(g);
+  A.init(g);
+  switch (op) {
+    case 0:
+      load(U.x, f0); load(U.y, f1); load(U.z, f2);
+      p.gas2part_velocity(U);
+      break;
+    case 1: p.update_position(k, pre, post); break;
+    case 2: p.update_velocity(k, pre, post); break;
+    case 3: p.update_Temp(k, pre, post); break;
+    case 4: load(A, f0); p.gas2part_Temp(A, true); break;
+    case 5: p.Compute_RHS_Temp(true); break;
+    case 6: p.part2gas_concentration(A); store(A, f0); break;
+    case 7:
+      p.part2gas_velocity(U);
+      store(U.x, f0); store(U.y, f1); store(U.z, f2);
+      break;
+    case 8: p.part2gas_Temp(A, true); store(A, f0); break;
+    case 9: p.Send_Recv(); break;
+    case 10: load(A, f0); p.gas2part_Temp(A, false); break;
+    case 11: p.Compute_RHS_Temp(false); break;
+    case 12: p.part2gas_Temp(A, false); store(A, f0); break;
+    case 13: p.load_random(); break;
+  }
+}
+// Gathered values and momentum RHS: which 0..3 = ug,vg,wg,Tg; 4..6 = RHS_u,v,w.
+double* orc_grid_part_aux(void* h, int which) {
+  Particles& p = ((Grid*)h)->part;
+  std::vector<double>* t[7] = {&p.ug, &p.vg, &p.wg, &p.Tg, &p.RHS_u, &p.RHS_v, &p.RHS_w};
+  return t[which]->data();
+}
+}
+

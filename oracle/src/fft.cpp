@@ -1,0 +1,110 @@
+// Oracle FFT: unnormalised complex DFT with sign convention
+//   X_k = sum_n x_n exp(sign * 2 pi i n k / N).
+// TEST INFRASTRUCTURE.  Stands in for the Sandia fft_3d/fft_2d package over
+// FFTW 2.1.5 that the reference links (poisson.cpp:87,431,453;
+// poisson_IO.cpp:41,168,196); that library is absent, so this transform is
+// pinned by the DFT definition (brute force and numpy fixtures in tests/).
+#include <vector>
+
+#include "orc.h"
+
+namespace orc {
+
+static void twiddles(int n, int sign, std::vector<double>& c,
+                     std::vector<double>& s) {
+  c.resize(n / 2 + 1);
+  s.resize(n / 2 + 1);
+  for (int k = 0; k <= n / 2; k++) {
+    double a = 2.0 * PI * (double)k / (double)n;
+    c[k] = cos(a);
+    s[k] = sign * sin(a);
+  }
+}
+
+// In-place transform of n elements spaced by `stride`.
+void fft1d(double* re, double* im, int n, long stride, int sign) {
+  if (n <= 1) return;
+  std::vector<double> xr(n), xi(n);
+  for (int i = 0; i < n; i++) {
+    xr[i] = re[i * stride];
+    xi[i] = im[i * stride];
+  }
+  bool pow2 = (n & (n - 1)) == 0;
+  if (pow2) {
+    // bit reversal
+    for (int i = 1, j = 0; i < n; i++) {
+      int bit = n >> 1;
+      for (; j & bit; bit >>= 1) j ^= bit;
+      j ^= bit;
+      if (i < j) {
+        std::swap(xr[i], xr[j]);
+        std::swap(xi[i], xi[j]);
+      }
+    }
+    std::vector<double> c, s;
+    twiddles(n, sign, c, s);
+    for (int len = 2; len <= n; len <<= 1) {
+      int half = len >> 1, step = n / len;
+      for (int b = 0; b < n; b += len)
+        for (int m = 0; m < half; m++) {
+          double wr = c[m * step], wi = s[m * step];
+          double ur = xr[b + m], ui = xi[b + m];
+          double tr = xr[b + m + half], ti = xi[b + m + half];
+          double vr = tr * wr - ti * wi, vi = tr * wi + ti * wr;
+          xr[b + m] = ur + vr;
+          xi[b + m] = ui + vi;
+          xr[b + m + half] = ur - vr;
+          xi[b + m + half] = ui - vi;
+        }
+    }
+  } else {
+    std::vector<double> yr(n), yi(n), c(n), s(n);
+    for (int k = 0; k < n; k++) {
+      double a = 2.0 * PI * (double)k / (double)n;
+      c[k] = cos(a);
+      s[k] = sign * sin(a);
+    }
+    for (int k = 0; k < n; k++) {
+      double ar = 0, ai = 0;
+      for (int m = 0; m < n; m++) {
+        int t = (int)(((long)m * k) % n);
+        ar += xr[m] * c[t] - xi[m] * s[t];
+        ai += xr[m] * s[t] + xi[m] * c[t];
+      }
+      yr[k] = ar;
+      yi[k] = ai;
+    }
+    xr.swap(yr);
+    xi.swap(yi);
+  }
+  for (int i = 0; i < n; i++) {
+    re[i * stride] = xr[i];
+    im[i * stride] = xi[i];
+  }
+}
+
+// 3-D transform of an [nz][ny][nx] array (x fastest).
+void fft3d(std::vector<double>& re, std::vector<double>& im, int nx, int ny,
+           int nz, int sign) {
+  const long sx = 1, sy = nx, sz = (long)nx * ny;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      fft1d(&re[k * sz + j * sy], &im[k * sz + j * sy], nx, sx, sign);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int i = 0; i < nx; i++)
+      fft1d(&re[k * sz + i], &im[k * sz + i], ny, sy, sign);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++)
+      fft1d(&re[j * sy + i], &im[j * sy + i], nz, sz, sign);
+}
+
+// 2-D transform of an [nz][ny] plane (y fastest).
+void fft2d(double* re, double* im, int ny, int nz, int sign) {
+  for (int k = 0; k < nz; k++) fft1d(re + (long)k * ny, im + (long)k * ny, ny, 1, sign);
+  for (int j = 0; j < ny; j++) fft1d(re + j, im + j, nz, ny, sign);
+}
+
+}  // namespace orc

@@ -1,0 +1,212 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) and of the partial
+reference build (oracle/_ref/libref.so).  Test infrastructure: the oracle is
+the checker, never the thing measured or shipped."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+MPI_LIB = "/opt/conda/lib"
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class Geom(C.Structure):
+    """Same layout as fr_geom in include/fr_amd.h."""
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("bs", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+                ("is_box", C.c_int)]
+
+
+def make_geom(nx, ny, nz, L=(2 * np.pi, 2 * np.pi, 2 * np.pi), box=True, bs=1):
+    g = Geom()
+    g.Nx, g.Ny, g.Nz, g.bs = nx + 2 * bs, ny + 2 * bs, nz + 2 * bs, bs
+    g.Lx, g.Ly, g.Lz = L
+    g.dx, g.dy, g.dz = L[0] / nx, L[1] / ny, L[2] / nz
+    g.is_box = 1 if box else 0
+    return g
+
+
+def shape(g):
+    return (g.Nz, g.Ny, g.Nx)
+
+
+_orc = None
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        L = C.CDLL(ORACLE_SO)
+        G = C.POINTER(Geom)
+        L.orc_div_f2c.argtypes = [G, dp, dp, dp, dp]
+        L.orc_del2.argtypes = [G, dp, dp]
+        L.orc_grad_c2f.argtypes = [G, dp, dp, dp, dp]
+        L.orc_interp.argtypes = [G, dp, dp, C.c_int, C.c_int]
+        L.orc_i_f2c.argtypes = [G, dp, dp, dp, dp]
+        L.orc_update_ghosts.argtypes = [G, dp, C.c_int]
+        L.orc_reduce.argtypes = [G, dp, C.c_int]
+        L.orc_reduce.restype = C.c_double
+        L.orc_reducex.argtypes = [G, dp, dp]
+        L.orc_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, C.c_int]
+        L.orc_fft2d.argtypes = [C.c_int, C.c_int, dp, dp, C.c_int]
+        L.orc_poisson.argtypes = [G, dp, dp, dp, dp, dp, C.c_int, C.c_double]
+        L.orc_poisson.restype = C.c_int
+        L.orc_glibc_rand.argtypes = [C.c_int, ip]
+        L.orc_grid_create.argtypes = [C.c_char_p]
+        L.orc_grid_create.restype = C.c_void_p
+        L.orc_grid_destroy.argtypes = [C.c_void_p]
+        L.orc_grid_initialize.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_grid_finish_initialize.argtypes = [C.c_void_p]
+        L.orc_grid_step.argtypes = [C.c_void_p]
+        L.orc_grid_call.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_grid_call.restype = C.c_int
+        L.orc_grid_field.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_grid_field.restype = C.POINTER(C.c_double)
+        L.orc_grid_scalar.argtypes = [C.c_void_p, C.c_char_p]
+        L.orc_grid_scalar.restype = C.POINTER(C.c_double)
+        L.orc_grid_np.argtypes = [C.c_void_p]
+        L.orc_grid_np.restype = C.c_int
+        L.orc_grid_set_np.argtypes = [C.c_void_p, C.c_int]
+        L.orc_grid_part.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_grid_part.restype = C.POINTER(C.c_double)
+        L.orc_grid_flag.argtypes = [C.c_void_p]
+        L.orc_grid_flag.restype = C.POINTER(C.c_int)
+        L.orc_grid_diagnostics.argtypes = [C.c_void_p, dp]
+        L.orc_grid_geom.argtypes = [C.c_void_p, C.POINTER(Geom)]
+        L.orc_grid_part_op.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_grid_part_aux.argtypes = [C.c_void_p, C.c_int]
+        L.orc_grid_part_aux.restype = C.POINTER(C.c_double)
+        _orc = L
+    return _orc
+
+
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO) and os.path.exists(os.path.join(MPI_LIB, "libmpi.so.12"))
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        # load the oracle first: libmpi would otherwise pull an older libgomp
+        # from /opt/conda/lib that the oracle's OpenMP loops cannot use
+        orc()
+        C.CDLL(os.path.join(MPI_LIB, "libmpi.so.12"), mode=C.RTLD_GLOBAL)
+        L = C.CDLL(REF_SO)
+        vp = C.c_void_p
+        L.ref_create.argtypes = [C.c_char_p]
+        L.ref_create.restype = vp
+        L.ref_size.argtypes = [vp]
+        L.ref_size.restype = C.c_int
+        L.ref_op.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.ref_reduce.argtypes = [vp, dp, C.c_int]
+        L.ref_reduce.restype = C.c_double
+        L.ref_reducex.argtypes = [vp, dp, dp]
+        L.ref_part_set.argtypes = [vp, C.c_int, dp, ip, dp]
+        L.ref_part_np.argtypes = [vp]
+        L.ref_part_np.restype = C.c_int
+        L.ref_part_get.argtypes = [vp, dp, ip, dp, vp, vp]
+        L.ref_part_op.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, vp, vp, vp]
+        L.ref_part_load_random.argtypes = [vp]
+        _ref = L
+    return _ref
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OracleGrid:
+    """The oracle's box grid (grid.cpp restated) driven from a params file."""
+
+    def __init__(self, params_path, ic_dir=None):
+        L = orc()
+        self.L = L
+        self.h = L.orc_grid_create(params_path.encode())
+        if not self.h:
+            raise RuntimeError("oracle: cannot parse " + params_path)
+        self.g = Geom()
+        L.orc_grid_geom(self.h, C.byref(self.g))
+        self.ic_dir = ic_dir
+
+    def initialize(self):
+        g = self.g
+        n = (g.Nz - 2) * (g.Ny - 2) * (g.Nx - 2)
+        if self.ic_dir:
+            blks = [np.fromfile(os.path.join(self.ic_dir, f), dtype="<f8", count=n)
+                    for f in ("U.bin", "V.bin", "W.bin")]
+            self._ic = blks
+            self.L.orc_grid_initialize(self.h, *[ptr(b) for b in blks])
+        else:
+            self.L.orc_grid_initialize(self.h, None, None, None)
+
+    def step(self):
+        self.L.orc_grid_step(self.h)
+
+    def call(self, name, k=0):
+        r = self.L.orc_grid_call(self.h, name.encode(), k)
+        assert r == 0, name
+
+    def field(self, name, comp=-1):
+        p = self.L.orc_grid_field(self.h, name.encode(), comp)
+        if not p:
+            raise KeyError(name)
+        return np.ctypeslib.as_array(p, shape=shape(self.g))
+
+    def scalar(self, name):
+        return self.L.orc_grid_scalar(self.h, name.encode())
+
+    @property
+    def np_(self):
+        return self.L.orc_grid_np(self.h)
+
+    def set_np(self, n):
+        self.L.orc_grid_set_np(self.h, n)
+
+    def part(self, q, s):
+        n = self.np_
+        if n == 0:
+            return np.zeros(0)
+        return np.ctypeslib.as_array(self.L.orc_grid_part(self.h, q, s), shape=(n,))
+
+    def flag(self):
+        n = self.np_
+        if n == 0:
+            return np.zeros(0, np.int32)
+        return np.ctypeslib.as_array(self.L.orc_grid_flag(self.h), shape=(n,))
+
+    def part_op(self, op, k=0, pre=0.0, post=0.0, f0=None, f1=None, f2=None):
+        self.L.orc_grid_part_op(self.h, op, k, pre, post, ptr(f0), ptr(f1), ptr(f2))
+
+    def part_aux(self, which):
+        n = self.np_
+        return np.ctypeslib.as_array(self.L.orc_grid_part_aux(self.h, which), shape=(n,))
+
+    def diagnostics(self):
+        out = np.zeros(10)
+        self.L.orc_grid_diagnostics(self.h, out)
+        return out
+
+    def __del__(self):
+        try:
+            self.L.orc_grid_destroy(self.h)
+        except Exception:
+            pass
