@@ -1,0 +1,138 @@
+"""Throughput of the full RK4 time step with particles on one MI355X.
+
+Workload (the plain FP64 GPU configuration): triply periodic box 256^3,
+4,000,000 point particles, two-way momentum and heat coupling, radiative
+heating, FFT pressure solve to max|dp| <= 1e-10 (synthetic seeded solenoidal
+initial field; see first_repo_amd/configs.py).  A "step" is one full RK4 step
+(4 substeps: density, particles, momentum, divergence, pressure solve,
+projection) on device-resident state; statistics and I/O are excluded.
+
+Prints one JSON line (metric Mcell*steps/s, higher is better), with the
+roofline of the dominant stencil kernel measured with HIP events inside the
+library and the CPU oracle timed on a bounded sample on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(params, d, threads):
+    """Oracle (serial restatement, OpenMP over field loops) on the same
+    workload: one RK4 step after initialisation."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    og = OL.OracleGrid(params, d)
+    og.initialize()
+    t = time.perf_counter()
+    og.step()
+    wall = time.perf_counter() - t
+    return wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--particles", type=int, default=4_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import first_repo_amd as fr
+    from first_repo_amd.configs import BoxWorkload
+
+    tmp = tempfile.mkdtemp(prefix="frbench_")
+    w = BoxWorkload(n=args.n, n_particles=args.particles, steps=200, mass_loading=0.5)
+    print(f"bench: building {args.n}^3 workload with {args.particles} particles", file=sys.stderr,
+          flush=True)
+    w.build(tmp)
+    params = os.path.join(tmp, "params.in")
+    sim = fr.Sim(params, tmp)
+    sim.initialize()
+    print("bench: initialised", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        sim.step()
+    t_poisson = t_part = iters = 0.0
+    kern_ms = kern_n = 0.0
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        sim.step()
+        tm = sim.timing()
+        t_poisson += tm[1]
+        t_part += tm[2]
+        iters += tm[3]
+        kern_ms += tm[4]
+        kern_n += tm[5]
+    wall = time.perf_counter() - t0
+    ncell = args.n ** 3
+    ms_per_step = wall / args.steps * 1e3
+    value = ncell * args.steps / wall / 1e6
+    diag = sim.diagnostics()
+    # dominant stencil kernel: the fused momentum update (grid::Update_RU_WOP);
+    # algorithmic bytes per launch = 21 doubles per ghosted cell (reads U, RU_int,
+    # S, RU, RU_np1: 15; writes RU_np1, RU_new: 6)
+    g = sim.g
+    ghosted = g.Nx * g.Ny * g.Nz
+    alg_bytes = 21 * 8 * ghosted
+    per_launch_ms = kern_ms / max(kern_n, 1)
+    achieved = alg_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
+    out = {
+        "metric": "Mcell*steps/s full RK4 step w/ particles",
+        "value": round(value, 3),
+        "unit": "Mcell*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"box {args.n}^3, {args.particles} particles, plain FP64, "
+                               "FFT pressure solve to max|dp|<=1e-10",
+                   "cells": ncell, "particles": args.particles,
+                   "particle_steps_per_s": round(args.particles * args.steps / wall, 1),
+                   "poisson_iters_per_substep": round(iters / args.steps / 4, 3),
+                   "poisson_ms_per_step": round(t_poisson / args.steps, 3),
+                   "particle_ms_per_step": round(t_part / args.steps, 3),
+                   "TKE": float(diag[0]), "Rho_Mean": float(diag[1]),
+                   "mean_particle_T": float(diag[8])},
+        "roofline": {"kernel": "k_update_ru_wop", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": alg_bytes,
+                     "avg_launch_ms": round(per_launch_ms, 4)},
+    }
+    sim.close()
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        print(f"bench: CPU oracle sample, 1 step, {threads} threads", file=sys.stderr, flush=True)
+        cw = cpu_baseline(params, tmp, threads)
+        out["cpu_baseline"] = {"value": round(ncell / cw / 1e6, 4), "unit": "Mcell*steps/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"oracle, 1 RK4 step of the same {args.n}^3 / "
+                                         f"{args.particles}-particle workload after "
+                                         f"initialisation ({cw:.1f} s), OpenMP field loops"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

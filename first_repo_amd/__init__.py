@@ -1,0 +1,192 @@
+"""MI355X (gfx950) implementation of the ggeraci/first_repo time-step path.
+
+The product is the C-ABI library ``libfr_amd.so`` (include/fr_amd.h): HIP
+kernels for the RK4 staggered-grid stencils, the FFT pressure solve and the
+Lagrangian particle update, plus host classes mirroring the reference's
+tensor0/tensor1/particle/poisson/grid.  This module only loads it with ctypes.
+Loading fails loudly when the library has not been built: there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfr_amd.so")
+
+_lib = None
+
+
+class Geom(C.Structure):
+    """fr_geom of include/fr_amd.h."""
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("bs", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+                ("is_box", C.c_int)]
+
+
+def make_geom(nx, ny, nz, L=(2 * np.pi, 2 * np.pi, 2 * np.pi), box=True):
+    g = Geom()
+    g.Nx, g.Ny, g.Nz, g.bs = nx + 2, ny + 2, nz + 2, 1
+    g.Lx, g.Ly, g.Lz = L
+    g.dx, g.dy, g.dz = L[0] / nx, L[1] / ny, L[2] / nz
+    g.is_box = 1 if box else 0
+    return g
+
+
+class FrError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded libfr_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FrError(f"{LIB_PATH} not built: run python -m first_repo_amd.build")
+        L = C.CDLL(LIB_PATH)
+        vp, G = C.c_void_p, C.POINTER(Geom)
+        i, d = C.c_int, C.c_double
+        sig = {
+            "fr_version": ([], C.c_char_p),
+            "fr_device_count": ([], i),
+            "fr_t0_div_f2c": ([G, vp, vp, vp, vp, vp], i),
+            "fr_t0_del2": ([G, vp, vp, vp], i),
+            "fr_t1_grad_c2f": ([G, vp, vp, vp, vp, vp], i),
+            "fr_t0_interp": ([G, vp, vp, i, i, vp], i),
+            "fr_t0_i_f2c": ([G, vp, vp, vp, vp, vp], i),
+            "fr_t0_update_ghosts": ([G, vp, i, vp], i),
+            "fr_t0_elementwise": ([G, i, vp, d, vp, d, vp, d, vp, vp], i),
+            "fr_t0_reduce": ([G, vp, i, C.POINTER(d), vp], i),
+            "fr_t0_reducex": ([G, vp, vp, vp], i),
+            "fr_box_poisson_create": ([G, C.POINTER(vp)], i),
+            "fr_box_poisson_destroy": ([vp], i),
+            "fr_box_poisson_solve": ([vp, vp, vp, vp, vp, vp, i, d, C.POINTER(i), vp], i),
+            "fr_sim_create": ([C.c_char_p, C.c_char_p, C.POINTER(vp)], i),
+            "fr_sim_destroy": ([vp], i),
+            "fr_sim_initialize": ([vp], i),
+            "fr_sim_step": ([vp], i),
+            "fr_sim_call": ([vp, C.c_char_p, i], i),
+            "fr_sim_get_field": ([vp, C.c_char_p, i, vp], i),
+            "fr_sim_set_field": ([vp, C.c_char_p, i, vp], i),
+            "fr_sim_get_scalar": ([vp, C.c_char_p, C.POINTER(d)], i),
+            "fr_sim_set_scalar": ([vp, C.c_char_p, d], i),
+            "fr_sim_geom": ([vp, G], i),
+            "fr_sim_np": ([vp, C.POINTER(i)], i),
+            "fr_sim_get_particles": ([vp, i, i, vp], i),
+            "fr_sim_set_particles": ([vp, i, i, i, vp], i),
+            "fr_sim_get_flags": ([vp, vp], i),
+            "fr_sim_set_flags": ([vp, i, vp], i),
+            "fr_sim_particles_changed": ([vp], i),
+            "fr_sim_diagnostics": ([vp, vp], i),
+            "fr_sim_timing": ([vp, vp], i),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise FrError(f"{what} failed with status {rc}")
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Sim:
+    """The box solver held on the device (fr_sim_* of include/fr_amd.h)."""
+
+    def __init__(self, params_path: str, ic_dir: str | None = None):
+        self.L = lib()
+        h = C.c_void_p()
+        check(self.L.fr_sim_create(params_path.encode(), (ic_dir or ".").encode(), C.byref(h)),
+              "fr_sim_create")
+        self.h = h
+        self.g = Geom()
+        check(self.L.fr_sim_geom(self.h, C.byref(self.g)), "fr_sim_geom")
+
+    @property
+    def shape(self):
+        return (self.g.Nz, self.g.Ny, self.g.Nx)
+
+    def initialize(self):
+        check(self.L.fr_sim_initialize(self.h), "fr_sim_initialize")
+
+    def step(self):
+        check(self.L.fr_sim_step(self.h), "fr_sim_step")
+
+    def call(self, name, k=0):
+        check(self.L.fr_sim_call(self.h, name.encode(), k), name)
+
+    def field(self, name, comp=-1):
+        a = np.empty(self.shape)
+        check(self.L.fr_sim_get_field(self.h, name.encode(), comp, _p(a)), "get " + name)
+        return a
+
+    def set_field(self, name, a, comp=-1):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape)
+        check(self.L.fr_sim_set_field(self.h, name.encode(), comp, _p(a)), "set " + name)
+
+    def scalar(self, name):
+        v = C.c_double()
+        check(self.L.fr_sim_get_scalar(self.h, name.encode(), C.byref(v)), name)
+        return v.value
+
+    def set_scalar(self, name, v):
+        check(self.L.fr_sim_set_scalar(self.h, name.encode(), float(v)), name)
+
+    @property
+    def np_(self):
+        n = C.c_int()
+        check(self.L.fr_sim_np(self.h, C.byref(n)))
+        return n.value
+
+    def part(self, q, s=0):
+        a = np.empty(self.np_)
+        check(self.L.fr_sim_get_particles(self.h, q, s, _p(a)), "get particles")
+        return a
+
+    def set_part(self, q, s, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(self.L.fr_sim_set_particles(self.h, len(a), q, s, _p(a)), "set particles")
+
+    def flags(self):
+        a = np.empty(self.np_, dtype=np.int32)
+        check(self.L.fr_sim_get_flags(self.h, _p(a)))
+        return a
+
+    def set_flags(self, a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        check(self.L.fr_sim_set_flags(self.h, len(a), _p(a)))
+
+    def particles_changed(self):
+        check(self.L.fr_sim_particles_changed(self.h))
+
+    def diagnostics(self):
+        out = np.zeros(10)
+        check(self.L.fr_sim_diagnostics(self.h, _p(out)), "diagnostics")
+        return out
+
+    def timing(self):
+        out = np.zeros(6)
+        check(self.L.fr_sim_timing(self.h, _p(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fr_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,440 @@
+// tensor0/tensor1 operators on gfx950: stencils with the reference write
+// ranges and single-rank halo semantics, element-wise arithmetic, cumulative
+// halos and deterministic reductions.
+//
+// Halo semantics folded into each stencil (communicator.cpp:177-396): after
+// an operator with x write range [xlo,xhi) and its Update_Ghosts, the value
+// at ghosted cell (i,j,k) equals the operator evaluated at
+//   box : (img(i), img(j), img(k))            (periodic in x, y, z)
+//   duct: (i, img(j), img(k)) when xlo <= i < xhi, else the prior content of
+//         (i, img(j), img(k))                  (y, z wrap; x ghosts kept)
+// so every kernel writes the whole ghosted array in one pass and reads only
+// cells the reference reads.
+#include "fr_internal.hpp"
+
+namespace fr {
+
+enum StencilOp {
+  ST_DIV = 0,   // tensor0::Equal_Div_F2C  tensor0.cpp:303-310
+  ST_DEL2,      // tensor0::Equal_Del2     tensor0.cpp:312-319
+  ST_IX_C2F,    // tensor0.cpp:353-360
+  ST_IY_C2F,    // tensor0.cpp:362-369
+  ST_IZ_C2F,    // tensor0.cpp:371-378
+  ST_IX_F2C,    // tensor0.cpp:381-388
+  ST_IY_F2C,    // tensor0.cpp:390-397
+  ST_IZ_F2C,    // tensor0.cpp:399-406
+  ST_I_F2C,     // tensor0::Equal_I_F2C    tensor0.cpp:408-416
+  ST_GX,        // tensor1::Equal_Grad_C2F x  tensor1.cpp:244-247
+  ST_GY,        //                        y  tensor1.cpp:249-252
+  ST_GZ,        //                        z  tensor1.cpp:254-257
+  ST_COPY,      // Update_Ghosts alone
+};
+
+template <int OP>
+__device__ __forceinline__ double stencil_at(const Dev& d, const double* __restrict__ a,
+                                             const double* __restrict__ b,
+                                             const double* __restrict__ c, int i,
+                                             int j, int k) {
+  const long n = at(d, i, j, k);
+  const long sx = 1, sy = d.Nx, sz = d.NxNy;
+  if (OP == ST_DIV)
+    return (a[n + sx] - a[n]) * d.idx + (b[n + sy] - b[n]) * d.idy +
+           (c[n + sz] - c[n]) * d.idz;
+  if (OP == ST_DEL2)
+    return (a[n + sx] - 2 * a[n] + a[n - sx]) * d.idx2 +
+           (a[n + sy] - 2 * a[n] + a[n - sy]) * d.idy2 +
+           (a[n + sz] - 2 * a[n] + a[n - sz]) * d.idz2;
+  if (OP == ST_IX_C2F) return (a[n] + a[n - sx]) * 0.5;
+  if (OP == ST_IY_C2F) return (a[n] + a[n - sy]) * 0.5;
+  if (OP == ST_IZ_C2F) return (a[n] + a[n - sz]) * 0.5;
+  if (OP == ST_IX_F2C) return (a[n] + a[n + sx]) * 0.5;
+  if (OP == ST_IY_F2C) return (a[n] + a[n + sy]) * 0.5;
+  if (OP == ST_IZ_F2C) return (a[n] + a[n + sz]) * 0.5;
+  if (OP == ST_I_F2C) {
+    const double OneThird = 1. / 3.;
+    return OneThird * ((a[n] + a[n + sx]) * 0.5 + (b[n] + b[n + sy]) * 0.5 +
+                       (c[n] + c[n + sz]) * 0.5);
+  }
+  if (OP == ST_GX) return (a[n] - a[n - sx]) * d.idx;
+  if (OP == ST_GY) return (a[n] - a[n - sy]) * d.idy;
+  if (OP == ST_GZ) return (a[n] - a[n - sz]) * d.idz;
+  return 0.0;
+}
+
+template <int OP>
+__global__ void __launch_bounds__(256) k_stencil(Dev d, double* __restrict__ out,
+                                                 const double* __restrict__ a,
+                                                 const double* __restrict__ b,
+                                                 const double* __restrict__ c,
+                                                 int xlo, int xhi) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.Nx || j >= d.Ny) return;
+  const int jj = img(j, d.Ny), kk = img(k, d.Nz);
+  const int ii = d.box ? img(i, d.Nx) : i;
+  double v;
+  if (OP != ST_COPY && ii >= xlo && ii < xhi) {
+    v = stencil_at<OP>(d, a, b, c, ii, jj, kk);
+  } else {
+    if (ii == i && jj == j && kk == k) return;
+    v = out[at(d, ii, jj, kk)];
+  }
+  out[at(d, i, j, k)] = v;
+}
+
+static dim3 grid3(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 4), d.Nz); }
+
+template <int OP>
+static int launch_stencil(const fr_geom* g, double* out, const double* a,
+                          const double* b, const double* c, int xlo, int xhi,
+                          void* s) {
+  if (!geom_ok(g) || !out) return FR_EINVAL;
+  Dev d = make_dev(g);
+  hipLaunchKernelGGL(k_stencil<OP>, grid3(d), dim3(64, 4), 0, S(s), d, out, a, b, c,
+                     xlo, xhi);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+// Ghost shell only: k faces (all i,j), j faces (interior k, all i), and for
+// the box the i faces (interior j,k).
+__global__ void k_shell(Dev d, double* __restrict__ f) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nk = 2 * d.NxNy;
+  const long nj = 2L * d.Nx * (d.Nz - 2);
+  const long ni = d.box ? 2L * (d.Ny - 2) * (d.Nz - 2) : 0;
+  int i, j, k;
+  if (t < nk) {
+    k = (t < d.NxNy) ? 0 : d.Nz - 1;
+    long r = t % d.NxNy;
+    j = (int)(r / d.Nx);
+    i = (int)(r % d.Nx);
+  } else if ((t -= nk) < nj) {
+    long half = (long)d.Nx * (d.Nz - 2);
+    j = (t < half) ? 0 : d.Ny - 1;
+    long r = t % half;
+    k = 1 + (int)(r / d.Nx);
+    i = (int)(r % d.Nx);
+  } else if ((t -= nj) < ni) {
+    long half = (long)(d.Ny - 2) * (d.Nz - 2);
+    i = (t < half) ? 0 : d.Nx - 1;
+    long r = t % half;
+    k = 1 + (int)(r / (d.Ny - 2));
+    j = 1 + (int)(r % (d.Ny - 2));
+  } else {
+    return;
+  }
+  const int ii = d.box ? img(i, d.Nx) : i;
+  f[at(d, i, j, k)] = f[at(d, ii, img(j, d.Ny), img(k, d.Nz))];
+}
+
+int ghost_fill(const Dev& d, double* f, hipStream_t s) {
+  long n = 2 * d.NxNy + 2L * d.Nx * (d.Nz - 2) + (d.box ? 2L * (d.Ny - 2) * (d.Nz - 2) : 0);
+  hipLaunchKernelGGL(k_shell, dim3(nblocks(n, 256)), dim3(256), 0, s, d, f);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+// Cumulative halo on one rank (communicator.cpp:422-511 box, 520-626 duct):
+// z phase (all i,j), y phase (all i, interior k), x phase (box; interior
+// j,k).  Each thread reads both ghosts of its line before adding, as the
+// reference packs both send buffers before unpacking.
+__global__ void k_cum_z(Dev d, double* __restrict__ f) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.NxNy) return;
+  const long top = (long)(d.Nz - 1) * d.NxNy + t, bot = t;
+  const double s0 = f[top], s1 = f[bot];
+  f[(long)(d.Nz - 2) * d.NxNy + t] += s1;
+  f[d.NxNy + t] += s0;
+}
+__global__ void k_cum_y(Dev d, double* __restrict__ f) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long n = (long)d.Nx * (d.Nz - 2);
+  if (t >= n) return;
+  const int i = (int)(t % d.Nx), k = 1 + (int)(t / d.Nx);
+  const double s0 = f[at(d, i, d.Ny - 1, k)], s1 = f[at(d, i, 0, k)];
+  f[at(d, i, d.Ny - 2, k)] += s1;
+  f[at(d, i, 1, k)] += s0;
+}
+__global__ void k_cum_x(Dev d, double* __restrict__ f) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long n = (long)(d.Ny - 2) * (d.Nz - 2);
+  if (t >= n) return;
+  const int j = 1 + (int)(t % (d.Ny - 2)), k = 1 + (int)(t / (d.Ny - 2));
+  const double s0 = f[at(d, d.Nx - 1, j, k)], s1 = f[at(d, 0, j, k)];
+  f[at(d, d.Nx - 2, j, k)] += s1;
+  f[at(d, 1, j, k)] += s0;
+}
+
+int cum_fold(const Dev& d, double* f, hipStream_t s) {
+  hipLaunchKernelGGL(k_cum_z, dim3(nblocks(d.NxNy, 256)), dim3(256), 0, s, d, f);
+  hipLaunchKernelGGL(k_cum_y, dim3(nblocks((long)d.Nx * (d.Nz - 2), 256)), dim3(256), 0,
+                     s, d, f);
+  if (d.box)
+    hipLaunchKernelGGL(k_cum_x, dim3(nblocks((long)(d.Ny - 2) * (d.Nz - 2), 256)),
+                       dim3(256), 0, s, d, f);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+// Element-wise forms of tensor0.cpp:93-241 (see fr_t0_elementwise).
+__global__ void k_elem(long n, int op, double* __restrict__ out, double a,
+                       const double* __restrict__ A, double b,
+                       const double* __restrict__ B, double c,
+                       const double* __restrict__ C) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double v;
+  switch (op) {
+    case 0:
+      v = a * A[t];
+      if (B) v = v + b * B[t];
+      if (C) v = v + c * C[t];
+      out[t] = v;
+      break;
+    case 1: out[t] += a * A[t]; break;
+    case 2: out[t] = A[t] * B[t] + b * C[t]; break;
+    case 3: out[t] = a / A[t]; break;
+    case 4: out[t] = A[t] / B[t]; break;
+    case 5: out[t] = a; break;
+    case 6: out[t] += a; break;
+    case 7: out[t] *= a; break;
+    case 8: out[t] = A[t]; break;
+    case 9: out[t] *= A[t]; break;
+    case 10: out[t] -= A[t]; break;
+    case 11: out[t] += A[t]; break;
+  }
+}
+
+// ---- deterministic reductions ------------------------------------------------
+constexpr int RB = 256;   // threads per reduction block
+constexpr int RG = 1024;  // fixed number of partial blocks
+
+template <int OP>
+__device__ __forceinline__ double rcomb(double x, double y) {
+  if (OP == 0 || OP == 5) return x + y;
+  if (OP == 2 || OP == 4) return y > x ? y : x;
+  return y < x ? y : x;  // min
+}
+
+template <int OP>
+__device__ __forceinline__ double rmap(double v) {
+  if (OP == 4) return v >= 0 ? v : -v;
+  if (OP == 5) return v * v;
+  return v;
+}
+
+template <int OP>
+__device__ double block_reduce(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = RB / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = rcomb<OP>(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+template <int OP>
+__global__ void __launch_bounds__(RB) k_reduce_part(Dev d, const double* __restrict__ f,
+                                                    double init, double* __restrict__ part) {
+  __shared__ double sh[RB];
+  const int ni = d.Nx - 2, nj = d.Ny - 2, nk = d.Nz - 2;
+  const long n = (long)ni * nj * nk;
+  double acc = init;
+  for (long t = (long)blockIdx.x * RB + threadIdx.x; t < n; t += (long)RG * RB) {
+    const int i = 1 + (int)(t % ni);
+    const long r = t / ni;
+    const int j = 1 + (int)(r % nj), k = 1 + (int)(r / nj);
+    acc = rcomb<OP>(acc, rmap<OP>(f[at(d, i, j, k)]));
+  }
+  double r = block_reduce<OP>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+template <int OP>
+__global__ void __launch_bounds__(RB) k_reduce_final(const double* __restrict__ part,
+                                                     double init, double* __restrict__ out) {
+  __shared__ double sh[RB];
+  double acc = init;
+  for (int t = threadIdx.x; t < RG; t += RB) acc = rcomb<OP>(acc, part[t]);
+  double r = block_reduce<OP>(acc, sh);
+  if (threadIdx.x == 0) *out = r;
+}
+
+// Reduction workspace (partials + result), one per process.
+static double* g_red_ws = nullptr;
+static double* g_red_host = nullptr;
+
+static int red_ws() {
+  if (!g_red_ws) {
+    FR_CHECK(hipMalloc(&g_red_ws, sizeof(double) * (RG + 8)));
+    FR_CHECK(hipHostMalloc(&g_red_host, sizeof(double) * 8, hipHostMallocDefault));
+  }
+  return FR_OK;
+}
+
+// Launch the reduction and leave the result in device memory at *dres.
+int reduce_async(const Dev& d, const double* f, int op, double* dres, hipStream_t s) {
+  int rc = red_ws();
+  if (rc) return rc;
+  const double first = 0.0;
+  switch (op) {
+    case 0:
+    case 1:
+      hipLaunchKernelGGL(k_reduce_part<0>, dim3(RG), dim3(RB), 0, s, d, f, first, g_red_ws);
+      hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(RB), 0, s, g_red_ws, first, dres);
+      break;
+    case 5:
+      hipLaunchKernelGGL(k_reduce_part<5>, dim3(RG), dim3(RB), 0, s, d, f, first, g_red_ws);
+      hipLaunchKernelGGL(k_reduce_final<5>, dim3(1), dim3(RB), 0, s, g_red_ws, first, dres);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_reduce_part<2>, dim3(RG), dim3(RB), 0, s, d, f, -HUGE_VAL, g_red_ws);
+      hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), dim3(RB), 0, s, g_red_ws, -HUGE_VAL, dres);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_reduce_part<3>, dim3(RG), dim3(RB), 0, s, d, f, HUGE_VAL, g_red_ws);
+      hipLaunchKernelGGL(k_reduce_final<3>, dim3(1), dim3(RB), 0, s, g_red_ws, HUGE_VAL, dres);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_reduce_part<4>, dim3(RG), dim3(RB), 0, s, d, f, 0.0, g_red_ws);
+      hipLaunchKernelGGL(k_reduce_final<4>, dim3(1), dim3(RB), 0, s, g_red_ws, 0.0, dres);
+      break;
+    default:
+      return FR_EINVAL;
+  }
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+int reduce_sync(const Dev& d, const double* f, int op, double* result, hipStream_t s) {
+  int rc = red_ws();
+  if (rc) return rc;
+  rc = reduce_async(d, f, op, g_red_ws + RG, s);
+  if (rc) return rc;
+  FR_CHECK(hipMemcpyAsync(g_red_host, g_red_ws + RG, sizeof(double), hipMemcpyDeviceToHost, s));
+  FR_CHECK(hipStreamSynchronize(s));
+  double v = g_red_host[0];
+  if (op == 1) v = v / ((double)(d.Nx - 2) * (d.Ny - 2) * (d.Nz - 2));
+  *result = v;
+  return FR_OK;
+}
+
+// communicator.cpp:635-646: one block per x index i.
+__global__ void __launch_bounds__(RB) k_reducex(Dev d, const double* __restrict__ f,
+                                                double denom, double* __restrict__ out) {
+  __shared__ double sh[RB];
+  const int i = blockIdx.x;
+  const int nj = d.Ny - 2, nk = d.Nz - 2;
+  double acc = 0;
+  for (long t = threadIdx.x; t < (long)nj * nk; t += RB) {
+    const int j = 1 + (int)(t % nj), k = 1 + (int)(t / nj);
+    acc += f[at(d, i, j, k)];
+  }
+  double r = block_reduce<0>(acc, sh);
+  if (threadIdx.x == 0) out[i] = r / denom;
+}
+
+int reducex(const Dev& d, const double* f, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reducex, dim3(d.Nx), dim3(RB), 0, s, d, f,
+                     (double)((d.Ny - 2) * (d.Nz - 2)), out);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+int elementwise(long n, int op, double* out, double a, const double* A, double b,
+                const double* B, double c, const double* C, hipStream_t s) {
+  hipLaunchKernelGGL(k_elem, dim3(nblocks(n, 256)), dim3(256), 0, s, n, op, out, a, A, b,
+                     B, c, C);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+}  // namespace fr
+
+using namespace fr;
+
+extern "C" {
+
+const char* fr_version(void) { return "fr_amd 0.1 gfx950"; }
+
+int fr_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int fr_t0_div_f2c(const fr_geom* g, double* out, const double* tx, const double* ty,
+                  const double* tz, void* s) {
+  if (!tx || !ty || !tz) return FR_EINVAL;
+  return launch_stencil<ST_DIV>(g, out, tx, ty, tz, 0, g->Nx - 1, s);
+}
+
+int fr_t0_del2(const fr_geom* g, double* out, const double* in, void* s) {
+  if (!in) return FR_EINVAL;
+  return launch_stencil<ST_DEL2>(g, out, in, nullptr, nullptr, 1, g->Nx - 1, s);
+}
+
+int fr_t1_grad_c2f(const fr_geom* g, double* ox, double* oy, double* oz,
+                   const double* in, void* s) {
+  if (!in || !ox || !oy || !oz) return FR_EINVAL;
+  int rc = launch_stencil<ST_GX>(g, ox, in, nullptr, nullptr, 1, g->Nx, s);
+  if (!rc) rc = launch_stencil<ST_GY>(g, oy, in, nullptr, nullptr, 0, g->Nx, s);
+  if (!rc) rc = launch_stencil<ST_GZ>(g, oz, in, nullptr, nullptr, 0, g->Nx, s);
+  return rc;
+}
+
+int fr_t0_interp(const fr_geom* g, double* out, const double* in, int axis, int dir,
+                 void* s) {
+  if (!in || axis < 0 || axis > 2 || dir < 0 || dir > 1) return FR_EINVAL;
+  const int Nx = g->Nx;
+  if (dir == 0) {
+    if (axis == 0) return launch_stencil<ST_IX_C2F>(g, out, in, nullptr, nullptr, 1, Nx, s);
+    if (axis == 1) return launch_stencil<ST_IY_C2F>(g, out, in, nullptr, nullptr, 0, Nx, s);
+    return launch_stencil<ST_IZ_C2F>(g, out, in, nullptr, nullptr, 0, Nx, s);
+  }
+  if (axis == 0) return launch_stencil<ST_IX_F2C>(g, out, in, nullptr, nullptr, 0, Nx - 1, s);
+  if (axis == 1) return launch_stencil<ST_IY_F2C>(g, out, in, nullptr, nullptr, 0, Nx, s);
+  return launch_stencil<ST_IZ_F2C>(g, out, in, nullptr, nullptr, 0, Nx, s);
+}
+
+int fr_t0_i_f2c(const fr_geom* g, double* out, const double* tx, const double* ty,
+                const double* tz, void* s) {
+  if (!tx || !ty || !tz) return FR_EINVAL;
+  return launch_stencil<ST_I_F2C>(g, out, tx, ty, tz, 0, g->Nx - 1, s);
+}
+
+int fr_t0_update_ghosts(const fr_geom* g, double* f, int cumulative, void* s) {
+  if (!geom_ok(g) || !f) return FR_EINVAL;
+  Dev d = make_dev(g);
+  if (cumulative) return cum_fold(d, f, S(s));
+  return ghost_fill(d, f, S(s));
+}
+
+int fr_t0_elementwise(const fr_geom* g, int op, double* out, double a, const double* A,
+                      double b, const double* B, double c, const double* Cf, void* s) {
+  if (!geom_ok(g) || !out || op < 0 || op > 11) return FR_EINVAL;
+  if ((op == 0 || op == 1 || op == 3 || op == 8 || op == 9 || op == 10 || op == 11) && !A)
+    return FR_EINVAL;
+  if ((op == 2 || op == 4) && (!A || !B)) return FR_EINVAL;
+  if (op == 2 && !Cf) return FR_EINVAL;
+  Dev d = make_dev(g);
+  return elementwise(d.size, op, out, a, A, b, B, c, Cf, S(s));
+}
+
+int fr_t0_reduce(const fr_geom* g, const double* f, int op, double* result, void* s) {
+  if (!geom_ok(g) || !f || !result || op < 0 || op > 5) return FR_EINVAL;
+  Dev d = make_dev(g);
+  return reduce_sync(d, f, op, result, S(s));
+}
+
+int fr_t0_reducex(const fr_geom* g, const double* f, double* out, void* s) {
+  if (!geom_ok(g) || !f || !out) return FR_EINVAL;
+  Dev d = make_dev(g);
+  return reducex(d, f, out, S(s));
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+// Internal interfaces between the kernel files and the host classes of
+// libfr_amd.so (not part of the C ABI).
+#pragma once
+#include "fr_common.hpp"
+
+namespace fr {
+
+// ---- fields (fields.hip) -------------------------------------------------------
+int ghost_fill(const Dev& d, double* f, hipStream_t s);
+int cum_fold(const Dev& d, double* f, hipStream_t s);
+int reduce_async(const Dev& d, const double* f, int op, double* dres, hipStream_t s);
+int reduce_sync(const Dev& d, const double* f, int op, double* result, hipStream_t s);
+int reducex(const Dev& d, const double* f, double* out, hipStream_t s);
+int elementwise(long n, int op, double* out, double a, const double* A, double b,
+                const double* B, double c, const double* C, hipStream_t s);
+
+// ---- fused box substep kernels (box.hip) ------------------------------------------
+struct BoxCoef {
+  double cpost_neg;  // -(dt*post)
+  double cpre_neg;   // -dt*pre
+  double cpost;      // dt*post
+  double cpre;       // dt*pre
+  int last;          // RK4_count == 3
+};
+
+struct WopCoef {
+  double a_s;      // -mp/Vcell
+  double b_g;      // Mu0/3.
+  double c_l;      // Mu0
+  double A;        // forcing A
+  double sx_forc;  // A*Rho0*U0 (x component only)
+  double rho0;
+  double gx, gy, gz;
+  int gravity;
+};
+
+struct DivCoef {
+  double kc;     // k/Cp
+  double bq;     // -R/(Cp*P0_new*Vcell)
+  double qmean;  // mean of RHS_Part_Temp (subtracted when cooling)
+  int cooling;
+  double s;      // 1./(dt*pre)
+};
+
+int box_update_rho(const Dev& d, const BoxCoef& cf, const double* const ru_int[3],
+                   const double* rho, double* rho_np1, double* rho_new,
+                   const double* const rf[3], double* const u[3], hipStream_t s);
+int box_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w,
+                      const double* const u[3], const double* const ru_int[3],
+                      const double* const S[3], const double* const ru[3],
+                      double* const ru_np1[3], double* const ru_new[3],
+                      const double* const rf[3], hipStream_t s);
+int box_div_rhs(const Dev& d, const DivCoef& q, const double* rho_new, const double* Q,
+                const double* const ru_new[3], double* const rf[3], double* rhs,
+                hipStream_t s);
+int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P,
+                     double* const ru_np1[3], double* const ru_new[3], hipStream_t s);
+
+// ---- diagnostics (diag.hip) ----------------------------------------------------------
+int diag_tke(const Dev& d, const double* u, const double* rho, const double* red,
+             double* out, hipStream_t s);
+int diag_diss(const Dev& d, const double* const u[3], const double* rho, double mu,
+              double* out, hipStream_t s);
+int vsum(const double* v, int n, double* part257, double* host, hipStream_t s);
+
+// ---- particles (particles.hip) -------------------------------------------------------
+struct PartPtrs {
+  double* s[7][4];  // x,y,z,u,v,w,T  x  base,_int,_new,_np1
+  int* flag;
+  double *rhs_u, *rhs_v, *rhs_w, *rhs_T;
+  int np;
+};
+
+struct AdvCoef {
+  double dt_post, dt_pre;      // dt*post, dt*pre
+  double iTp;                  // 1/Tp
+  double gx, gy, gz;           // ParticleGravity * g
+  double dtpost_iS, dtpre_iS;  // dt*post*iS, dt*pre*iS
+  double alpha;
+  int last;
+};
+
+// Device particle set (SoA, the 29 migrated values of particle.cpp:252-262
+// plus the RHS arrays), its cell buckets and migration workspace.
+class PartDev {
+ public:
+  int np = 0, cap = 0;
+  double* s[7][4] = {};
+  int* flag = nullptr;
+  double *rhs_u = nullptr, *rhs_v = nullptr, *rhs_w = nullptr, *rhs_T = nullptr;
+  long ncell = 0;
+  int *cnt = nullptr, *start = nullptr, *list = nullptr, *key = nullptr, *slot = nullptr;
+  int* scan_tmp = nullptr;
+  int *flg = nullptr, *nl = nullptr, *pNL = nullptr, *holepos = nullptr, *tailnl = nullptr;
+  int *evt = nullptr, *islo = nullptr, *plo = nullptr, *dst = nullptr, *stgf = nullptr;
+  double* stg = nullptr;
+  int* hbuf = nullptr;
+  bool cells_stage_valid = false;
+  long moved_last = 0;
+
+  int alloc(int capacity, const Dev& d);
+  void release();
+  PartPtrs ptrs() const;
+  int advance(const Dev& d, const AdvCoef& c, const double* const u[3], hipStream_t st);
+  int temp(const Dev& d, int stage, double c0, const double* rho, double iTth, hipStream_t st);
+  int build_cells(const Dev& d, int stage, hipStream_t st);
+  int deposit_cic(const Dev& d, int stage, const double* w, double* out, bool halo,
+                  hipStream_t st);
+  int deposit_vel(const Dev& d, double* const out[3], hipStream_t st);
+  int migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st);
+  void swap_stages(int a, int b);
+  int copy_stage(int dst_stage, int src_stage, hipStream_t st);
+};
+
+}  // namespace fr

@@ -1,0 +1,638 @@
+// Lagrangian particle kernels on gfx950 (particle.cpp:58-658, 745-865).
+//
+//  * advance : 1-D linear gather of the face velocity at x_int (truncating
+//              cell index, particle.cpp:340-360), then the position, drag and
+//              heat ODE updates of the RK4 stage (particle.cpp:314-399,480-496)
+//  * temp    : trilinear gather of T = (P0/R)/rho on cell centres (rounded
+//              index, particle.cpp:402-456) and the heat RHS (460-478)
+//  * cells   : particles bucketed by rounded cell index, each bucket in
+//              ascending particle order (count + scan + scatter + sort)
+//  * deposit : per target cell, the contributions of the particles in the
+//              neighbouring buckets summed in ascending particle order --
+//              the order of the reference's serial scatter loop, so the
+//              deposits are deterministic and equal the reference's bit for
+//              bit; no floating-point atomics (particle.cpp:498-658)
+//  * migrate : the one-rank swap-with-last removal + append order of
+//              SortValid{Z,Y,X}/FromBuf (particle.cpp:150-311, 853-858),
+//              computed in parallel from closed-form event ranks
+#include <climits>
+
+#include "fr_internal.hpp"
+
+namespace fr {
+
+__device__ __forceinline__ long clampi(long q, long n) { return q < 0 ? 0 : (q >= n ? n - 1 : q); }
+
+// particle.cpp:314-399, 480-496 (one RK4 stage) with the gather 340-360.
+__global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const double* __restrict__ ux,
+                               const double* __restrict__ uy, const double* __restrict__ uz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.np) return;
+  const double xi = p.s[0][1][i], yi = p.s[1][1][i], zi = p.s[2][1][i];
+  const int ti = (int)(xi * d.idx), tj = (int)(yi * d.idy), tk = (int)(zi * d.idz);
+  const double x_l = (xi * d.idx - ((int)(xi * d.idx)));
+  const double y_l = (yi * d.idy - ((int)(yi * d.idy)));
+  const double z_l = (zi * d.idz - ((int)(zi * d.idz)));
+  const int ii = ti + 1, jj = tj + 1, kk = tk + 1;
+  const long q = (long)kk * d.NxNy + (long)jj * d.Nx + ii;
+  const double ug = x_l * ux[clampi(q + 1, d.size)] + (1 - x_l) * ux[clampi(q, d.size)];
+  const double vg = y_l * uy[clampi(q + d.Nx, d.size)] + (1 - y_l) * uy[clampi(q, d.size)];
+  const double wg = z_l * uz[clampi(q + d.NxNy, d.size)] + (1 - z_l) * uz[clampi(q, d.size)];
+  // positions
+  const double ui = p.s[3][1][i], vi = p.s[4][1][i], wi = p.s[5][1][i];
+  {
+    const double X[3] = {ui, vi, wi};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const double np1 = p.s[a][3][i] + c.dt_post * X[a];
+      p.s[a][3][i] = np1;
+      p.s[a][2][i] = c.last ? np1 : p.s[a][0][i] + c.dt_pre * X[a];
+    }
+  }
+  // velocities
+  const double ru = (ug - ui) * c.iTp, rv = (vg - vi) * c.iTp, rw = (wg - wi) * c.iTp;
+  p.rhs_u[i] = ru;
+  p.rhs_v[i] = rv;
+  p.rhs_w[i] = rw;
+  {
+    const double R[3] = {ru, rv, rw};
+    const double G[3] = {c.gx, c.gy, c.gz};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const double np1 = p.s[3 + a][3][i] + c.dt_post * (R[a] + G[a]);
+      p.s[3 + a][3][i] = np1;
+      p.s[3 + a][2][i] = c.last ? np1 : p.s[3 + a][0][i] + c.dt_pre * (R[a] + G[a]);
+    }
+  }
+  // temperature (RHS_T from the previous stage)
+  const double rt = p.rhs_T[i];
+  const double tn = p.s[6][3][i] + c.dtpost_iS * (rt + c.alpha);
+  p.s[6][3][i] = tn;
+  p.s[6][2][i] = c.last ? tn : p.s[6][0][i] + c.dtpre_iS * (rt + c.alpha);
+}
+
+// particle.cpp:402-478: Tg at the stage `st` position (1 = _int, 2 = _new),
+// T = c0/rho (grid.cpp:676, c0 = P0/R), RHS_T = (Tg - T_stage)*iTth.
+__global__ void k_part_temp(Dev d, PartPtrs p, int st, double c0, const double* __restrict__ rho,
+                            double iTth) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.np) return;
+  const double X = p.s[0][st][i], Y = p.s[1][st][i], Z = p.s[2][st][i];
+  int ii = (int)(round(X * d.idx));
+  int jj = (int)(round(Y * d.idy));
+  int kk = (int)(round(Z * d.idz));
+  const double x_l = X * d.idx - ii + 0.5;
+  const double y_l = Y * d.idy - jj + 0.5;
+  const double z_l = Z * d.idz - kk + 0.5;
+  ii += 1;
+  jj += 1;
+  kk += 1;
+  auto T = [&](int a, int b, int e) {
+    return c0 / rho[clampi((long)e * d.NxNy + (long)b * d.Nx + a, d.size)];
+  };
+  const double Tg = x_l * y_l * z_l * T(ii, jj, kk) +
+                    (1 - x_l) * y_l * z_l * T(ii - 1, jj, kk) +
+                    x_l * (1 - y_l) * z_l * T(ii, jj - 1, kk) +
+                    x_l * y_l * (1 - z_l) * T(ii, jj, kk - 1) +
+                    (1 - x_l) * (1 - y_l) * z_l * T(ii - 1, jj - 1, kk) +
+                    (1 - x_l) * y_l * (1 - z_l) * T(ii - 1, jj, kk - 1) +
+                    x_l * (y_l - 1) * (z_l - 1) * T(ii, jj - 1, kk - 1) +
+                    (1 - x_l) * (1 - y_l) * (1 - z_l) * T(ii - 1, jj - 1, kk - 1);
+  p.rhs_T[i] = (Tg - p.s[6][st][i]) * iTth;
+}
+
+// ---- device-wide exclusive scan of int32 ------------------------------------
+constexpr int SB = 256, SPT = 4, STILE = SB * SPT;
+
+__global__ void k_scan_tiles(const int* __restrict__ in, long n, int* __restrict__ tsum) {
+  __shared__ int sh[SB];
+  const long base = (long)blockIdx.x * STILE;
+  int s = 0;
+  for (int e = 0; e < SPT; e++) {
+    long t = base + threadIdx.x * SPT + e;
+    if (t < n) s += in[t];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = SB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tsum[blockIdx.x] = sh[0];
+}
+
+// exclusive scan of tile sums in one block; total stored at out_total
+__global__ void k_scan_sums(int* __restrict__ tsum, int ntiles, int* __restrict__ out_total) {
+  __shared__ int sh[SB];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < ntiles; b0 += SB) {
+    const int t = b0 + threadIdx.x;
+    const int v = t < ntiles ? tsum[t] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < SB; o <<= 1) {
+      int a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += a;
+      __syncthreads();
+    }
+    if (t < ntiles) tsum[t] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == SB - 1) carry += sh[SB - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ void k_scan_apply(const int* __restrict__ in, long n, const int* __restrict__ tsum,
+                             int* __restrict__ out) {
+  __shared__ int sh[SB];
+  const long base = (long)blockIdx.x * STILE + threadIdx.x * SPT;
+  int v[SPT];
+  int s = 0;
+  for (int e = 0; e < SPT; e++) {
+    v[e] = (base + e < n) ? in[base + e] : 0;
+    s += v[e];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < SB; o <<= 1) {
+    int a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += a;
+    __syncthreads();
+  }
+  int run = tsum[blockIdx.x] + sh[threadIdx.x] - s;
+  for (int e = 0; e < SPT; e++) {
+    if (base + e < n) out[base + e] = run;
+    run += v[e];
+  }
+}
+
+// out[0..n) = exclusive scan of in, out[n] = total.  tmp holds n/STILE+1 ints.
+int scan_ex(const int* in, long n, int* out, int* tmp, hipStream_t s) {
+  const int nt = (int)((n + STILE - 1) / STILE);
+  if (nt == 0) {
+    hipMemsetAsync(out, 0, sizeof(int), s);
+    return FR_OK;
+  }
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SB), 0, s, in, n, tmp);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SB), 0, s, tmp, nt, out + n);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(SB), 0, s, in, n, tmp, out);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+// ---- cell buckets (rounded index of the upper cell centre) -------------------
+__device__ __forceinline__ long round_cell(const Dev& d, double x, double y, double z) {
+  const int ii = (int)(round(x * d.idx)) + 1;
+  const int jj = (int)(round(y * d.idy)) + 1;
+  const int kk = (int)(round(z * d.idz)) + 1;
+  return clampi((long)kk * d.NxNy + (long)jj * d.Nx + ii, d.size);
+}
+
+__global__ void k_bucket_count(Dev d, PartPtrs p, int st, int* __restrict__ cnt,
+                               int* __restrict__ key, int* __restrict__ slot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.np) return;
+  const long c = round_cell(d, p.s[0][st][i], p.s[1][st][i], p.s[2][st][i]);
+  key[i] = (int)c;
+  slot[i] = atomicAdd(&cnt[c], 1);
+}
+
+__global__ void k_bucket_scatter(int np, const int* __restrict__ key, const int* __restrict__ slot,
+                                 const int* __restrict__ start, int* __restrict__ list) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  list[start[key[i]] + slot[i]] = i;
+}
+
+// Sort each bucket ascending (slots from atomics come in any order).
+__global__ void k_bucket_sort(long ncell, const int* __restrict__ start, int* __restrict__ list) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int b = start[c], e = start[c + 1];
+  if (e - b < 2) return;
+  for (int u = b + 1; u < e; u++) {
+    const int v = list[u];
+    int w = u - 1;
+    while (w >= b && list[w] > v) {
+      list[w + 1] = list[w];
+      w--;
+    }
+    list[w + 1] = v;
+  }
+}
+
+// ---- deposits -----------------------------------------------------------------
+// K-way merge over bucket ranges in ascending particle order.
+template <int NB>
+struct Merge {
+  int cur[NB], end[NB];
+  __device__ __forceinline__ int next(const int* __restrict__ list, int& which) {
+    int best = INT_MAX, wb = -1;
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+      if (cur[q] < end[q]) {
+        const int v = list[cur[q]];
+        if (v < best) {
+          best = v;
+          wb = q;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+      if (q == wb) cur[q]++;
+    which = wb;
+    return best;
+  }
+};
+
+// Trilinear (cloud-in-cell) deposit of particle.cpp:526-559 (w == nullptr,
+// concentration) or 586-658 (weight RHS_T) at the stage `st` positions.
+// Writes the raw per-cell sums for every ghosted cell (before the
+// cumulative halo).
+__global__ void __launch_bounds__(256) k_dep_cic(Dev d, PartPtrs p, int st,
+                                                 const double* __restrict__ wgt,
+                                                 const int* __restrict__ start,
+                                                 const int* __restrict__ list,
+                                                 double* __restrict__ out) {
+  const int a = blockIdx.x * 64 + threadIdx.x;
+  const int b = blockIdx.y * 4 + threadIdx.y;
+  const int c = blockIdx.z;
+  if (a >= d.Nx || b >= d.Ny) return;
+  Merge<8> m;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int A = a + (q & 1), B = b + ((q >> 1) & 1), C = c + (q >> 2);
+    if (A < d.Nx && B < d.Ny && C < d.Nz) {
+      const long cell = at(d, A, B, C);
+      m.cur[q] = start[cell];
+      m.end[q] = start[cell + 1];
+    } else {
+      m.cur[q] = m.end[q] = 0;
+    }
+  }
+  double acc = 0.0;
+  for (;;) {
+    int q;
+    const int i = m.next(list, q);
+    if (q < 0) break;
+    const double X = p.s[0][st][i], Y = p.s[1][st][i], Z = p.s[2][st][i];
+    const int ii = (int)(round(X * d.idx));
+    const int jj = (int)(round(Y * d.idy));
+    const int kk = (int)(round(Z * d.idz));
+    const double x_l = X * d.idx - ii + 0.5;
+    const double y_l = Y * d.idy - jj + 0.5;
+    const double z_l = Z * d.idz - kk + 0.5;
+    const double wx = (q & 1) ? (1 - x_l) : x_l;
+    const double wy = ((q >> 1) & 1) ? (1 - y_l) : y_l;
+    const double wz = (q >> 2) ? (1 - z_l) : z_l;
+    double t = wx * wy * wz;
+    if (wgt) t = t * wgt[i];
+    acc = acc + t;
+  }
+  out[at(d, a, b, c)] = acc;
+}
+
+// 1-D linear momentum deposit (particle.cpp:498-524) of component M at the
+// _int positions: target (a,b,c) collects x_l*R from particles whose
+// truncating base is one cell below along M and (1-x_l)*R from particles
+// whose base is the cell itself.  Their rounded buckets lie in a 3x2x2 block.
+template <int M>
+__device__ __forceinline__ double dep_vel_comp(const Dev& d, const PartPtrs& p,
+                                               const double* __restrict__ R,
+                                               const int* __restrict__ start,
+                                               const int* __restrict__ list, int a, int b,
+                                               int c) {
+  Merge<12> m;
+#pragma unroll
+  for (int q = 0; q < 12; q++) {
+    // offsets: axis M in {-1,0,1}, the other two in {0,1}
+    const int o3 = q % 3 - 1, o2a = (q / 3) & 1, o2b = (q / 6) & 1;
+    const int A = a + (M == 0 ? o3 : o2a);
+    const int B = b + (M == 1 ? o3 : (M == 0 ? o2a : o2b));
+    const int C = c + (M == 2 ? o3 : o2b);
+    if (A >= 0 && B >= 0 && C >= 0 && A < d.Nx && B < d.Ny && C < d.Nz) {
+      const long cell = at(d, A, B, C);
+      m.cur[q] = start[cell];
+      m.end[q] = start[cell + 1];
+    } else {
+      m.cur[q] = m.end[q] = 0;
+    }
+  }
+  double acc = 0.0;
+  for (;;) {
+    int q;
+    const int i = m.next(list, q);
+    if (q < 0) break;
+    const double X = p.s[0][1][i], Y = p.s[1][1][i], Z = p.s[2][1][i];
+    const int ti = (int)(X * d.idx) + 1, tj = (int)(Y * d.idy) + 1, tk = (int)(Z * d.idz) + 1;
+    const double pos = M == 0 ? X : (M == 1 ? Y : Z);
+    const double id = M == 0 ? d.idx : (M == 1 ? d.idy : d.idz);
+    const double x_l = (pos * id - ((int)(pos * id)));
+    const int tm = M == 0 ? ti : (M == 1 ? tj : tk);
+    const int t1 = M == 0 ? tj : ti, t2 = M == 2 ? tj : tk;
+    const int c1 = M == 0 ? b : a, c2 = M == 2 ? b : c;
+    const int cm = M == 0 ? a : (M == 1 ? b : c);
+    if (t1 != c1 || t2 != c2) continue;
+    if (tm == cm)
+      acc = acc + (1 - x_l) * R[i];
+    else if (tm == cm - 1)
+      acc = acc + x_l * R[i];
+  }
+  return acc;
+}
+
+__global__ void __launch_bounds__(256) k_dep_vel(Dev d, PartPtrs p,
+                                                 const int* __restrict__ start,
+                                                 const int* __restrict__ list,
+                                                 double* __restrict__ ox, double* __restrict__ oy,
+                                                 double* __restrict__ oz) {
+  const int a = blockIdx.x * 64 + threadIdx.x;
+  const int b = blockIdx.y * 4 + threadIdx.y;
+  const int c = blockIdx.z;
+  if (a >= d.Nx || b >= d.Ny) return;
+  const long n = at(d, a, b, c);
+  ox[n] = dep_vel_comp<0>(d, p, p.rhs_u, start, list, a, b, c);
+  oy[n] = dep_vel_comp<1>(d, p, p.rhs_v, start, list, a, b, c);
+  oz[n] = dep_vel_comp<2>(d, p, p.rhs_w, start, list, a, b, c);
+}
+
+// ---- migration (one rank) -------------------------------------------------------
+// For axis a with positions P = a_new: leaver flag 2 if P > hi, 1 if P < lo.
+__global__ void k_mig_flag(int np, const double* __restrict__ P, double lo, double hi,
+                           int* __restrict__ flg, int* __restrict__ nl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double v = P[i];
+  const int f = v > hi ? 2 : (v < lo ? 1 : 0);
+  flg[i] = f;
+  nl[i] = f == 0;
+}
+
+// Front holes (leavers at i < K) get rank m = i - pNL[i]; tail non-leavers
+// (i >= K) get rank from the end m = K - pNL[i] - 1.
+__global__ void k_mig_rank(int np, const int* __restrict__ flg, const int* __restrict__ pNL,
+                           int* __restrict__ holepos, int* __restrict__ tailnl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int K = pNL[np];
+  if (i < K && flg[i] != 0) holepos[i - pNL[i]] = i;
+  if (i >= K && flg[i] == 0) tailnl[K - pNL[i] - 1] = i;
+}
+
+// Event order of the serial scan: E = np - x with x = q_m for the front hole
+// of rank m (q_0 = np, q_m = position of the tail non-leaver of rank m-1),
+// x = q_h for a leaver sitting at K (h = number of front holes), x = i for
+// every other tail leaver.
+__global__ void k_mig_event(int np, const int* __restrict__ flg, const int* __restrict__ pNL,
+                            const int* __restrict__ tailnl, int* __restrict__ evt,
+                            int* __restrict__ islo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np || flg[i] == 0) return;
+  const int K = pNL[np];
+  const int h = K - pNL[K < np ? K : np];
+  int x;
+  if (i < K) {
+    const int m = i - pNL[i];
+    x = m == 0 ? np : tailnl[m - 1];
+  } else if (i == K) {
+    x = h == 0 ? np : tailnl[h - 1];
+  } else {
+    x = i;
+  }
+  const int e = np - x;
+  evt[e] = i;
+  islo[e] = flg[i] == 1;
+}
+
+struct MigAxis {
+  int axis;
+  double L;
+};
+
+// Stage the moved particles (tail non-leavers and leavers, the latter
+// wrapped by MODE_L*), then write them to their destinations.
+__global__ void k_mig_gather(int np, int nleave, PartPtrs p, MigAxis ax,
+                             const int* __restrict__ pNL, const int* __restrict__ holepos,
+                             const int* __restrict__ tailnl, const int* __restrict__ evt,
+                             const int* __restrict__ plo, double* __restrict__ stg,
+                             int* __restrict__ stgf, int* __restrict__ dst, long cap) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int K = pNL[np];
+  const int h = K - pNL[K < np ? K : np];
+  if (s >= h + nleave) return;
+  int src, to;
+  bool leaver;
+  if (s < h) {
+    src = tailnl[s];
+    to = holepos[s];
+    leaver = false;
+  } else {
+    const int e = s - h;
+    src = evt[e];
+    const int nlo = plo[nleave];
+    const int r = plo[e];
+    const bool lo = (plo[e + 1] - r) != 0;
+    to = K + (lo ? r : nlo + (e - r));
+    leaver = true;
+  }
+  dst[s] = to;
+  for (int q = 0; q < 7; q++)
+    for (int t = 0; t < 4; t++) stg[(long)(q * 4 + t) * cap + s] = p.s[q][t][src];
+  stgf[s] = p.flag[src];
+  if (leaver) {
+    // MODE_L* (particle.cpp:58-104): wrap _new, base and _np1 of this axis
+    const int a = ax.axis;
+    double vn = stg[(long)(a * 4 + 2) * cap + s];
+    double v0 = stg[(long)(a * 4 + 0) * cap + s];
+    double v1 = stg[(long)(a * 4 + 3) * cap + s];
+    if (vn > ax.L) {
+      vn -= ax.L;
+      v0 -= ax.L;
+      v1 -= ax.L;
+    } else if (vn < 0) {
+      vn += ax.L;
+      v0 += ax.L;
+      v1 += ax.L;
+    }
+    stg[(long)(a * 4 + 2) * cap + s] = vn;
+    stg[(long)(a * 4 + 0) * cap + s] = v0;
+    stg[(long)(a * 4 + 3) * cap + s] = v1;
+  }
+}
+
+__global__ void k_mig_scatter(int np, int nleave, PartPtrs p, const int* __restrict__ pNL,
+                              const double* __restrict__ stg, const int* __restrict__ stgf,
+                              const int* __restrict__ dst, long cap) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int K = pNL[np];
+  const int h = K - pNL[K < np ? K : np];
+  if (s >= h + nleave) return;
+  const int to = dst[s];
+  for (int q = 0; q < 7; q++)
+    for (int t = 0; t < 4; t++) p.s[q][t][to] = stg[(long)(q * 4 + t) * cap + s];
+  p.flag[to] = stgf[s];
+}
+
+}  // namespace fr
+
+// ---- host side of the particle set ---------------------------------------------
+namespace fr {
+
+int PartDev::alloc(int capacity, const Dev& d) {
+  cap = capacity < 1 ? 1 : capacity;
+  ncell = d.size;
+  for (int q = 0; q < 7; q++)
+    for (int t = 0; t < 4; t++) FR_CHECK(hipMalloc(&s[q][t], sizeof(double) * cap));
+  FR_CHECK(hipMalloc(&flag, sizeof(int) * cap));
+  double** r[4] = {&rhs_u, &rhs_v, &rhs_w, &rhs_T};
+  for (auto pp : r) {
+    FR_CHECK(hipMalloc(pp, sizeof(double) * cap));
+    FR_CHECK(hipMemset(*pp, 0, sizeof(double) * cap));
+  }
+  FR_CHECK(hipMalloc(&cnt, sizeof(int) * (ncell + 1)));
+  FR_CHECK(hipMalloc(&start, sizeof(int) * (ncell + 1)));
+  FR_CHECK(hipMalloc(&list, sizeof(int) * cap));
+  FR_CHECK(hipMalloc(&key, sizeof(int) * cap));
+  FR_CHECK(hipMalloc(&slot, sizeof(int) * cap));
+  const long big = ncell > cap ? ncell : cap;
+  FR_CHECK(hipMalloc(&scan_tmp, sizeof(int) * (big / STILE + 2)));
+  int** m[9] = {&flg, &nl, &pNL, &holepos, &tailnl, &evt, &islo, &plo, &dst};
+  for (auto pp : m) FR_CHECK(hipMalloc(pp, sizeof(int) * (cap + 1)));
+  FR_CHECK(hipMalloc(&stgf, sizeof(int) * cap));
+  FR_CHECK(hipMalloc(&stg, sizeof(double) * 28 * (size_t)cap));
+  FR_CHECK(hipHostMalloc(&hbuf, sizeof(int) * 4, hipHostMallocDefault));
+  return FR_OK;
+}
+
+void PartDev::release() {
+  for (int q = 0; q < 7; q++)
+    for (int t = 0; t < 4; t++) hipFree(s[q][t]);
+  void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, cnt, start, list, key, slot, scan_tmp,
+                 flg, nl, pNL, holepos, tailnl, evt, islo, plo, dst, stgf, stg};
+  for (void* p : all) hipFree(p);
+  hipHostFree(hbuf);
+}
+
+PartPtrs PartDev::ptrs() const {
+  PartPtrs p;
+  for (int q = 0; q < 7; q++)
+    for (int t = 0; t < 4; t++) p.s[q][t] = s[q][t];
+  p.flag = flag;
+  p.rhs_u = rhs_u;
+  p.rhs_v = rhs_v;
+  p.rhs_w = rhs_w;
+  p.rhs_T = rhs_T;
+  p.np = np;
+  return p;
+}
+
+int PartDev::advance(const Dev& d, const AdvCoef& c, const double* const u[3], hipStream_t st) {
+  if (np == 0) return FR_OK;
+  hipLaunchKernelGGL(k_part_advance, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), c,
+                     u[0], u[1], u[2]);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+int PartDev::temp(const Dev& d, int stage, double c0, const double* rho, double iTth,
+                  hipStream_t st) {
+  if (np == 0) return FR_OK;
+  hipLaunchKernelGGL(k_part_temp, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), stage,
+                     c0, rho, iTth);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+int PartDev::build_cells(const Dev& d, int stage, hipStream_t st) {
+  FR_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * ncell, st));
+  if (np > 0) {
+    hipLaunchKernelGGL(k_bucket_count, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), stage,
+                       cnt, key, slot);
+  }
+  int rc = scan_ex(cnt, ncell, start, scan_tmp, st);
+  if (rc) return rc;
+  if (np > 0) {
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, key, slot,
+                       start, list);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(nblocks(ncell, 256)), dim3(256), 0, st, ncell, start,
+                       list);
+  }
+  FR_LAUNCH_CHECK();
+  cells_stage_valid = true;
+  return FR_OK;
+}
+
+static dim3 grid3p(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 4), d.Nz); }
+
+int PartDev::deposit_cic(const Dev& d, int stage, const double* w, double* out, bool halo,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_dep_cic, grid3p(d), dim3(64, 4), 0, st, d, ptrs(), stage, w, start, list,
+                     out);
+  FR_LAUNCH_CHECK();
+  int rc = cum_fold(d, out, st);
+  if (!rc && halo) rc = ghost_fill(d, out, st);
+  return rc;
+}
+
+int PartDev::deposit_vel(const Dev& d, double* const out[3], hipStream_t st) {
+  hipLaunchKernelGGL(k_dep_vel, grid3p(d), dim3(64, 4), 0, st, d, ptrs(), start, list, out[0],
+                     out[1], out[2]);
+  FR_LAUNCH_CHECK();
+  for (int a = 0; a < 3; a++) {
+    int rc = cum_fold(d, out[a], st);
+    if (!rc) rc = ghost_fill(d, out[a], st);
+    if (rc) return rc;
+  }
+  return FR_OK;
+}
+
+// particle::Send_Recv on one rank: axes z, y, x (particle.cpp:853-858).
+int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st) {
+  if (np == 0) return FR_OK;
+  const unsigned nb = nblocks(np, 256);
+  for (int ax = 2; ax >= 0; ax--) {
+    hipLaunchKernelGGL(k_mig_flag, dim3(nb), dim3(256), 0, st, np, (const double*)s[ax][2], lo[ax],
+                       hi[ax], flg, nl);
+    int rc = scan_ex(nl, np, pNL, scan_tmp, st);
+    if (rc) return rc;
+    FR_CHECK(hipMemcpyAsync(hbuf, pNL + np, sizeof(int), hipMemcpyDeviceToHost, st));
+    FR_CHECK(hipStreamSynchronize(st));
+    const int K = hbuf[0];
+    const int nleave = np - K;
+    moved_last += nleave;
+    if (nleave == 0) continue;
+    hipLaunchKernelGGL(k_mig_rank, dim3(nb), dim3(256), 0, st, np, flg, pNL, holepos, tailnl);
+    hipLaunchKernelGGL(k_mig_event, dim3(nb), dim3(256), 0, st, np, flg, pNL, tailnl, evt, islo);
+    rc = scan_ex(islo, nleave, plo, scan_tmp, st);
+    if (rc) return rc;
+    MigAxis m{ax, L[ax]};
+    const unsigned ng = nblocks(2L * nleave, 256);
+    hipLaunchKernelGGL(k_mig_gather, dim3(ng), dim3(256), 0, st, np, nleave, ptrs(), m, pNL,
+                       holepos, tailnl, evt, plo, stg, stgf, dst, (long)cap);
+    hipLaunchKernelGGL(k_mig_scatter, dim3(ng), dim3(256), 0, st, np, nleave, ptrs(), pNL, stg,
+                       stgf, dst, (long)cap);
+    FR_LAUNCH_CHECK();
+  }
+  cells_stage_valid = false;
+  return FR_OK;
+}
+
+void PartDev::swap_stages(int a, int b) {
+  for (int q = 0; q < 7; q++) std::swap(s[q][a], s[q][b]);
+}
+
+int PartDev::copy_stage(int dst_stage, int src_stage, hipStream_t st) {
+  if (np == 0) return FR_OK;
+  for (int q = 0; q < 7; q++)
+    FR_CHECK(hipMemcpyAsync(s[q][dst_stage], s[q][src_stage], sizeof(double) * np,
+                            hipMemcpyDeviceToDevice, st));
+  return FR_OK;
+}
+
+}  // namespace fr

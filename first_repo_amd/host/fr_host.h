@@ -1,0 +1,266 @@
+// Host classes of the MI355X solver library, mirroring the reference's
+// plug-in surface: params (params.h:8-130), gridsize (gridsize.h:5-59),
+// tensor0 (tensor0.h:6-132), tensor1 (tensor1.h:6-90), particle
+// (particle.h:12-155), poisson (poisson.h:19-81) and grid (grid.h:16-115).
+// Field storage is device memory with the reference's ghosted layout; the
+// member functions launch the gfx950 kernels of csrc/.  One process drives
+// one device; MPI is replaced by single-rank semantics (every neighbour is
+// self), so halos are periodic wraps and reductions are device trees.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../csrc/fr_internal.hpp"
+
+const double PI(3.141592653589793);  // params.h:5
+
+// params.cpp:6-77: fixed-order '#'-commented parameter file.
+class params {
+ public:
+  int IsBox_ = 1, Nx_tot_ = 0, Ny_tot_ = 0, Nz_tot_ = 0, bs_ = 1, Np_track_ = 0;
+  double Lx_ = 0, Ly_ = 0, Lz_ = 0, T_final_ = 0, dt_ = 0, threshold_ = 0;
+  int Statistics_ = 0, Stat_print_ = 0, Data_freq_fast_ = 1, Data_freq_slow_ = 1;
+  std::string Data_dir_, Stat_dir_;
+  int Initial_ = 0, PreCond_ = 0, Solver_ = 0, cooling_ = 0;
+  int Iteration_ = 30, Iteration1_ = 30, Iteration2_ = 30;
+  double epsilon_ = 1e-7;
+  int TWC_ = 1, ParticleGravity_ = 0;
+  double A_ = 0, A1_ = 0, A2_ = 0, Rho0_ = 1, U0_ = 0, T0_ = 1, Mu0_ = 0, k_ = 0;
+  double gx_ = 0, gy_ = 0, gz_ = 0, gx1_ = 0, gy1_ = 0, gz1_ = 0, gx2_ = 0, gy2_ = 0, gz2_ = 0;
+  double Cp_ = 0, Cv_ = 0, R_ = 0, np0_ = 0, Dp_ = 0, Rhop_ = 0, Cvp_ = 0, epsilonp_ = 0;
+  double Nu_ = 0, I0_ = 0, I01_ = 0, I02_ = 0, Tp_ = 0, mp_ = 0, P0_ = 0, W_ = 0;
+  int N0_ = 0;
+  bool ok = false;
+
+  params() {}
+  explicit params(const char* path);
+  void update(double time);
+  double Lx() const { return Lx_; }
+  double Ly() const { return Ly_; }
+  double Lz() const { return Lz_; }
+  double Tp() const { return Tp_; }
+  double R() const { return R_; }
+  double P0() const { return P0_; }
+  double mp() const { return mp_; }
+  double Nu() const { return Nu_; }
+  double I0() const { return I0_; }
+  double epsilonp() const { return epsilonp_; }
+  double Cvp() const { return Cvp_; }
+  double Dp() const { return Dp_; }
+  double k() const { return k_; }
+  double Cv() const { return Cv_; }
+  double Cp() const { return Cp_; }
+  double Mu0() const { return Mu0_; }
+  double T0() const { return T0_; }
+  double U0() const { return U0_; }
+  double Rho0() const { return Rho0_; }
+  double A() const { return A_; }
+  double gx() const { return gx_; }
+  double gy() const { return gy_; }
+  double gz() const { return gz_; }
+  double Vol() const { return Lx_ * Ly_ * Lz_; }
+  bool IsBox() const { return IsBox_ != 0; }
+  int N0() const { return N0_; }
+  int Nx_tot() const { return Nx_tot_; }
+  int Ny_tot() const { return Ny_tot_; }
+  int Nz_tot() const { return Nz_tot_; }
+  int bs() const { return bs_; }
+  double T_final() const { return T_final_; }
+  double dt() const { return dt_; }
+  int Initial() const { return Initial_; }
+  int Iteration() const { return Iteration_; }
+  double epsilon() const { return epsilon_; }
+  bool Is_Cooling() const { return cooling_ != 0; }
+  bool TWC() const { return TWC_ != 0; }
+  bool ParticleGravity() const { return ParticleGravity_ != 0; }
+  int Solver() const { return Solver_; }
+};
+
+// gridsize.cpp:4-88 on a single rank.
+class gridsize {
+ public:
+  fr_geom g;
+  int Nx_tot_, Ny_tot_, Nz_tot_;
+  double xl_, xh_, yl_, yh_, zl_, zh_;
+  gridsize() {}
+  explicit gridsize(const params& p);
+  int Nx() const { return g.Nx; }
+  int Ny() const { return g.Ny; }
+  int Nz() const { return g.Nz; }
+  int bs() const { return g.bs; }
+  double dx() const { return g.dx; }
+  double dy() const { return g.dy; }
+  double dz() const { return g.dz; }
+  double Lx() const { return g.Lx; }
+  double Ly() const { return g.Ly; }
+  double Lz() const { return g.Lz; }
+  double Vcell() const { return g.dx * g.dy * g.dz; }
+  int size() const { return g.Nx * g.Ny * g.Nz; }
+  long size_tot() const { return (long)Nx_tot_ * Ny_tot_ * Nz_tot_; }
+  bool IsBox() const { return g.is_box != 0; }
+  fr::Dev dev() const { return fr::make_dev(&g); }
+};
+
+// tensor0: a ghosted scalar field in device memory (tensor0.h:6-132).
+class tensor0 {
+  gridsize* p_ = nullptr;
+  double* ptr = nullptr;
+  hipStream_t st_ = 0;
+
+ public:
+  tensor0() {}
+  tensor0(gridsize* p, hipStream_t st);
+  ~tensor0();
+  tensor0(const tensor0&) = delete;
+  tensor0& operator=(const tensor0&) = delete;
+  void init(gridsize* p, hipStream_t st);
+  double* pointer() const { return ptr; }
+  int size() const { return p_->size(); }
+  void swap(tensor0& o) { std::swap(ptr, o.ptr); }
+  void Update_Ghosts();
+  void Update_Ghosts_CUM();
+  void assign(double a);        // operator=(double)
+  void add(double a);           // operator+=(double)
+  void copy(const tensor0& a);  // operator=(tensor0)
+  void Equal_Divide(double a, const tensor0& b);
+  void Equal_Divide(const tensor0& a, const tensor0& b);
+  double sum();
+  double mean();
+  void make_mean_zero();
+  double max();
+  double min();
+  double max_abs();
+  void download(double* host) const;
+  void upload(const double* host);
+};
+
+// tensor1: three staggered components (tensor1.h:6-90).
+class tensor1 {
+ public:
+  tensor0 x, y, z;
+  tensor1() {}
+  void init(gridsize* p, hipStream_t st) {
+    x.init(p, st);
+    y.init(p, st);
+    z.init(p, st);
+  }
+  tensor0& c(int m) { return m == 0 ? x : (m == 1 ? y : z); }
+  void swap(tensor1& o) {
+    x.swap(o.x);
+    y.swap(o.y);
+    z.swap(o.z);
+  }
+  void copy(const tensor1& a) {
+    x.copy(a.x);
+    y.copy(a.y);
+    z.copy(a.z);
+  }
+  void assign(double a) {
+    x.assign(a);
+    y.assign(a);
+    z.assign(a);
+  }
+  void Equal_Divide(double a, const tensor1& b) {
+    x.Equal_Divide(a, b.x);
+    y.Equal_Divide(a, b.y);
+    z.Equal_Divide(a, b.z);
+  }
+  void make_mean_zero() {
+    x.make_mean_zero();
+    y.make_mean_zero();
+    z.make_mean_zero();
+  }
+  double* const* ptrs3(double* out[3]) {
+    out[0] = x.pointer();
+    out[1] = y.pointer();
+    out[2] = z.pointer();
+    return out;
+  }
+};
+
+// particle (particle.h:12-155): device SoA set on one rank.
+class particle {
+ public:
+  params* param_ = nullptr;
+  gridsize* size_ = nullptr;
+  hipStream_t st_ = 0;
+  fr::PartDev dev;
+  int Np = 0;
+  particle() {}
+  int init(params* p, gridsize* s, hipStream_t st, int capacity);
+  ~particle() { dev.release(); }
+  int load_random();  // particle.cpp:919-958
+  // particle.cpp:340-399 + 314-338 + 480-496 (gather and the three ODE updates)
+  int advance(int k, double pre, double post, tensor1& U);
+  int part2gas_concentration(tensor0& A);       // particle.cpp:526-559
+  int part2gas_velocity(tensor1& A);            // particle.cpp:498-524
+  int Send_Recv();                              // particle.cpp:853-858
+  // gas2part_Temp_{new,int} + Compute_RHS_Temp_{new,int} (402-478)
+  int temp(int stage, double c0, const tensor0& Rho);
+  int part2gas_Temp(int stage, tensor0& A);     // particle.cpp:586-658
+  int rebuild_cells(int stage);
+};
+
+// poisson (poisson.h:19-81), FFT solver branch.
+class poisson {
+ public:
+  fr_box_poisson* s = nullptr;
+  params* par = nullptr;
+  int num_iteration_ = 0;
+  hipStream_t st_ = 0;
+  poisson() {}
+  int init(params* p, gridsize* g, hipStream_t st);
+  ~poisson();
+  int Solve(tensor1& C, tensor0& P, tensor0& RHS);  // poisson.cpp:115-117
+  int num_iteration() const { return num_iteration_; }
+};
+
+// grid (grid.h:16-115): the box time-step methods.
+class grid {
+ public:
+  params param;
+  gridsize size;
+  hipStream_t st = 0;
+  double RK4_preCoeff[4], RK4_postCoeff[4];
+  double mean_energy_transferred = 0, C_BC = 0;
+  particle part;
+  tensor1 RU, RU_int, RU_new, RU_np1, RHS_RU, U, Rho_face, dummy;
+  tensor0 P, RHS_Pois, C, Rho, Rho_int, Rho_new, Rho_np1, RHS_Part_Temp, scratch;
+  double P0 = 0, P0_int = 0, P0_new = 0, P0_np1 = 0, dP0_dt = 0, T_cur = 0;
+  int num_timestep = 0, RK4_count = 0;
+  poisson PS_;
+  std::string ic_dir;
+  // mean of RHS_Part_Temp after its last deposit (tensor0::mean)
+  double qmean = 0;
+  bool qmean_valid = false;
+  int err = 0;  // first nonzero status of a method
+  // device timers
+  hipEvent_t ev[4];
+  hipEvent_t evk[8];  // momentum kernel start/stop per substep
+  double t_poisson = 0, t_particles = 0, t_step = 0, t_momentum = 0;
+  int n_momentum = 0;
+  int iters_step = 0;
+  double* dred = nullptr;  // device scratch for diagnostics (Nx doubles x 4)
+
+  grid() {}
+  ~grid();
+  int setup(const char* params_path, const char* ic_dir_);
+  int Initialize();
+  int Update_Rho();
+  int Update_P0();
+  int Update_Particle();
+  int Update_RU_WOP();
+  int Compute_Div_U_new();
+  int Compute_RHS_Pois();
+  int Solve_Poisson();
+  int Update_RU_WP();
+  int TimeAdvance_RK4();
+  int Correct_mean_RU();
+  int TimeAdvance();
+  int step();  // simulation.cpp:36-70, box calls
+  int diagnostics(double* out);
+  tensor0* field0(const std::string& name);
+  tensor1* field1(const std::string& name);
+};

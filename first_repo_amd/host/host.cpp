@@ -1,0 +1,850 @@
+// Host side of the box solver: the reference classes (params, gridsize,
+// tensor0, tensor1, particle, poisson, grid) driving the gfx950 kernels, and
+// the fr_sim_* C entry points.  Compiled by hipcc with the kernels into
+// libfr_amd.so.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "fr_host.h"
+
+using namespace fr;
+
+// ---------------------------------------------------------------- params
+// params.cpp:6-64: lines whose first character is NUL or '#' are dropped,
+// the rest are read as whitespace-separated tokens in a fixed order.
+params::params(const char* path) {
+  std::ifstream in(path);
+  if (!in) return;
+  std::ostringstream trimmed;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    trimmed << line << "\n";
+  }
+  std::istringstream s(trimmed.str());
+  s >> IsBox_;
+  s >> Nx_tot_ >> Ny_tot_ >> Nz_tot_;
+  s >> Lx_ >> Ly_ >> Lz_;
+  s >> bs_;
+  s >> Np_track_;
+  s >> T_final_ >> dt_ >> threshold_;
+  s >> Statistics_ >> Stat_print_;
+  s >> Data_freq_fast_ >> Data_freq_slow_;
+  s >> Data_dir_ >> Stat_dir_;
+  s >> Initial_;
+  s >> PreCond_ >> Solver_;
+  s >> cooling_;
+  s >> Iteration1_ >> Iteration2_ >> epsilon_;
+  s >> TWC_ >> ParticleGravity_;
+  s >> A1_ >> A2_;
+  s >> Rho0_;
+  s >> U0_;
+  s >> T0_;
+  s >> Mu0_ >> k_;
+  s >> gx1_ >> gy1_ >> gz1_ >> gx2_ >> gy2_ >> gz2_;
+  s >> Cp_ >> Cv_;
+  s >> np0_;
+  s >> Dp_ >> Rhop_;
+  s >> Cvp_;
+  s >> epsilonp_;
+  s >> Nu_;
+  s >> I01_ >> I02_;
+  if (s.fail()) return;
+  // params.cpp:50-63
+  R_ = Cp_ - Cv_;
+  P0_ = Rho0_ * T0_ * R_;
+  Tp_ = Rhop_ * Dp_ * Dp_ / (18 * Mu0_);
+  mp_ = Rhop_ * PI * Dp_ * Dp_ * Dp_ / 6.;
+  W_ = Ly_;
+  N0_ = (int)(np0_ * Vol());
+  Iteration_ = Iteration1_;
+  A_ = A1_;
+  gx_ = gx1_;
+  gy_ = gy1_;
+  gz_ = gz1_;
+  I0_ = I01_;
+  ok = true;
+}
+
+void params::update(double time) {  // params.cpp:66-77
+  if (time > threshold_) {
+    Iteration_ = Iteration2_;
+    A_ = A2_;
+    gx_ = gx2_;
+    gy_ = gy2_;
+    gz_ = gz2_;
+    I0_ = I02_;
+  }
+}
+
+// gridsize.cpp:22-28, 83-88 on one rank.
+gridsize::gridsize(const params& p) {
+  Nx_tot_ = p.Nx_tot();
+  Ny_tot_ = p.Ny_tot();
+  Nz_tot_ = p.Nz_tot();
+  g.bs = p.bs();
+  g.Nx = Nx_tot_ + 2 * g.bs;
+  g.Ny = Ny_tot_ + 2 * g.bs;
+  g.Nz = Nz_tot_ + 2 * g.bs;
+  g.Lx = p.Lx();
+  g.Ly = p.Ly();
+  g.Lz = p.Lz();
+  g.dx = g.Lx / Nx_tot_;
+  g.dy = g.Ly / Ny_tot_;
+  g.dz = g.Lz / Nz_tot_;
+  g.is_box = p.IsBox() ? 1 : 0;
+  xl_ = 0 * g.dx;
+  xh_ = (Nx_tot_ - 1 + 1) * g.dx;
+  yl_ = 0 * g.dy;
+  yh_ = (Ny_tot_ - 1 + 1) * g.dy;
+  zl_ = 0 * g.dz;
+  zh_ = (Nz_tot_ - 1 + 1) * g.dz;
+}
+
+// ---------------------------------------------------------------- tensor0
+tensor0::tensor0(gridsize* p, hipStream_t st) { init(p, st); }
+
+void tensor0::init(gridsize* p, hipStream_t st) {
+  p_ = p;
+  st_ = st;
+  if (hipMalloc(&ptr, sizeof(double) * (size_t)p_->size()) != hipSuccess) {
+    fprintf(stderr, "fr_amd: tensor0 allocation of %d doubles failed\n", p_->size());
+    ptr = nullptr;
+    return;
+  }
+  hipMemsetAsync(ptr, 0, sizeof(double) * (size_t)p_->size(), st_);
+}
+
+tensor0::~tensor0() {
+  if (ptr) hipFree(ptr);
+}
+
+void tensor0::Update_Ghosts() { fr_t0_update_ghosts(&p_->g, ptr, 0, st_); }
+void tensor0::Update_Ghosts_CUM() { fr_t0_update_ghosts(&p_->g, ptr, 1, st_); }
+void tensor0::assign(double a) {
+  fr_t0_elementwise(&p_->g, 5, ptr, a, nullptr, 0, nullptr, 0, nullptr, st_);
+}
+void tensor0::add(double a) {
+  fr_t0_elementwise(&p_->g, 6, ptr, a, nullptr, 0, nullptr, 0, nullptr, st_);
+}
+void tensor0::copy(const tensor0& a) {
+  hipMemcpyAsync(ptr, a.ptr, sizeof(double) * (size_t)p_->size(), hipMemcpyDeviceToDevice, st_);
+}
+void tensor0::Equal_Divide(double a, const tensor0& b) {
+  fr_t0_elementwise(&p_->g, 3, ptr, a, b.ptr, 0, nullptr, 0, nullptr, st_);
+}
+void tensor0::Equal_Divide(const tensor0& a, const tensor0& b) {
+  fr_t0_elementwise(&p_->g, 4, ptr, 0, a.ptr, 0, b.ptr, 0, nullptr, st_);
+}
+double tensor0::sum() {
+  double r = 0;
+  fr_t0_reduce(&p_->g, ptr, 0, &r, st_);
+  return r;
+}
+double tensor0::mean() {
+  double r = 0;
+  fr_t0_reduce(&p_->g, ptr, 1, &r, st_);
+  return r;
+}
+void tensor0::make_mean_zero() {  // tensor0.cpp:437-441: (*this) -= mean
+  double m = mean();
+  add(-m);
+}
+double tensor0::max() {
+  double r = 0;
+  fr_t0_reduce(&p_->g, ptr, 2, &r, st_);
+  return r;
+}
+double tensor0::min() {
+  double r = 0;
+  fr_t0_reduce(&p_->g, ptr, 3, &r, st_);
+  return r;
+}
+double tensor0::max_abs() {
+  double r = 0;
+  fr_t0_reduce(&p_->g, ptr, 4, &r, st_);
+  return r;
+}
+void tensor0::download(double* host) const {
+  hipMemcpyAsync(host, ptr, sizeof(double) * (size_t)p_->size(), hipMemcpyDeviceToHost, st_);
+  hipStreamSynchronize(st_);
+}
+void tensor0::upload(const double* host) {
+  hipMemcpyAsync(ptr, host, sizeof(double) * (size_t)p_->size(), hipMemcpyHostToDevice, st_);
+  hipStreamSynchronize(st_);
+}
+
+// ---------------------------------------------------------------- particle
+// glibc random_r TYPE_3 (default seed 1): r[i] = 16807 r[i-1] mod (2^31-1)
+// for i < 31, r[i] = r[i-31] for i < 34, then r[i] = r[i-31] + r[i-3] mod
+// 2^32 with the first 310 outputs discarded; rand() = r >> 1.  Restated from
+// its published description so that placement does not depend on any other
+// rand() caller in the process.
+namespace {
+struct GlibcRand {
+  uint32_t ring[34];
+  int head = 0;
+  explicit GlibcRand(unsigned seed) {
+    int32_t r[34];
+    r[0] = (int32_t)seed;
+    for (int i = 1; i < 31; i++) {
+      long hi = r[i - 1] / 127773, lo = r[i - 1] % 127773;
+      long word = 16807 * lo - 2836 * hi;
+      if (word < 0) word += 2147483647;
+      r[i] = (int32_t)word;
+    }
+    for (int i = 31; i < 34; i++) r[i] = r[i - 31];
+    for (int i = 0; i < 34; i++) ring[i] = (uint32_t)r[i];
+    for (int i = 34; i < 344; i++) next();
+  }
+  int next() {
+    uint32_t v = ring[(head + 3) % 34] + ring[(head + 31) % 34];
+    ring[head] = v;
+    head = (head + 1) % 34;
+    return (int)(v >> 1);
+  }
+};
+}  // namespace
+
+int particle::init(params* p, gridsize* s, hipStream_t st, int capacity) {
+  param_ = p;
+  size_ = s;
+  st_ = st;
+  Np = 0;
+  return dev.alloc(capacity, s->dev());
+}
+
+// particle.cpp:919-958 on one rank: x,y,z uniform from rand(), u = U0,
+// v = w = 0, T = T0, flag = running index; all other stages zero.
+int particle::load_random() {
+  const int n = param_->N0();
+  if (n > dev.cap) return FR_EOVERFLOW;
+  Np = n;
+  dev.np = n;
+  GlibcRand rng(1);
+  const double RM = 2147483647.0;
+  std::vector<double> X(n), Y(n), Z(n), Uu(n, param_->U0()), zero(n, 0.0), T(n, param_->T0());
+  std::vector<int> fl(n);
+  const double xl = size_->xl_, xh = size_->xh_, yl = size_->yl_, yh = size_->yh_,
+               zl = size_->zl_, zh = size_->zh_;
+  for (int i = 0; i < n; i++) {
+    X[i] = ((double)rng.next() / RM) * (xh - xl) + xl;
+    Y[i] = ((double)rng.next() / RM) * (yh - yl) + yl;
+    Z[i] = ((double)rng.next() / RM) * (zh - zl) + zl;
+    fl[i] = i;
+  }
+  const std::vector<double>* base[7] = {&X, &Y, &Z, &Uu, &zero, &zero, &T};
+  for (int q = 0; q < 7; q++)
+    for (int t = 0; t < 4; t++)
+      FR_CHECK(hipMemcpyAsync(dev.s[q][t], t == 0 ? base[q]->data() : zero.data(),
+                              sizeof(double) * n, hipMemcpyHostToDevice, st_));
+  FR_CHECK(hipMemcpyAsync(dev.flag, fl.data(), sizeof(int) * n, hipMemcpyHostToDevice, st_));
+  FR_CHECK(hipStreamSynchronize(st_));
+  return FR_OK;
+}
+
+int particle::advance(int k, double pre, double post, tensor1& U) {
+  const params& p = *param_;
+  AdvCoef c;
+  const double dt = p.dt();
+  c.dt_post = dt * post;
+  c.dt_pre = dt * pre;
+  c.iTp = (p.Tp() != 0) ? 1. / p.Tp() : 0.0;
+  c.gx = p.ParticleGravity() * p.gx();
+  c.gy = p.ParticleGravity() * p.gy();
+  c.gz = p.ParticleGravity() * p.gz();
+  const double iS = 1. / (p.mp() * p.Cvp());
+  c.dtpost_iS = dt * post * iS;
+  c.dtpre_iS = dt * pre * iS;
+  c.alpha = 0.25 * p.epsilonp() * PI * p.Dp() * p.Dp() * p.I0();
+  c.last = (k == 3);
+  const double* u[3] = {U.x.pointer(), U.y.pointer(), U.z.pointer()};
+  return dev.advance(size_->dev(), c, u, st_);
+}
+
+int particle::part2gas_concentration(tensor0& A) {
+  return dev.deposit_cic(size_->dev(), 1, nullptr, A.pointer(), false, st_);
+}
+
+int particle::part2gas_velocity(tensor1& A) {
+  double* o[3] = {A.x.pointer(), A.y.pointer(), A.z.pointer()};
+  return dev.deposit_vel(size_->dev(), o, st_);
+}
+
+int particle::Send_Recv() {
+  const double lo[3] = {size_->xl_, size_->yl_, size_->zl_};
+  const double hi[3] = {size_->xh_, size_->yh_, size_->zh_};
+  const double L[3] = {size_->Lx(), size_->Ly(), size_->Lz()};
+  return dev.migrate(lo, hi, L, st_);
+}
+
+int particle::temp(int stage, double c0, const tensor0& Rho) {
+  const double iTth = param_->Nu() * PI * param_->Dp() * param_->k();
+  return dev.temp(size_->dev(), stage, c0, Rho.pointer(), iTth, st_);
+}
+
+int particle::part2gas_Temp(int stage, tensor0& A) {
+  return dev.deposit_cic(size_->dev(), stage, dev.rhs_T, A.pointer(), true, st_);
+}
+
+int particle::rebuild_cells(int stage) { return dev.build_cells(size_->dev(), stage, st_); }
+
+// ---------------------------------------------------------------- poisson
+int poisson::init(params* p, gridsize* g, hipStream_t st) {
+  par = p;
+  st_ = st;
+  return fr_box_poisson_create(&g->g, &s);
+}
+poisson::~poisson() { fr_box_poisson_destroy(s); }
+int poisson::Solve(tensor1& C, tensor0& P, tensor0& RHS) {
+  return fr_box_poisson_solve(s, C.x.pointer(), C.y.pointer(), C.z.pointer(), P.pointer(),
+                              RHS.pointer(), par->Iteration(), par->epsilon(),
+                              &num_iteration_, st_);
+}
+
+// ---------------------------------------------------------------- grid
+#define FR_TRY(x)                \
+  do {                           \
+    int rc_ = (x);               \
+    if (rc_) {                   \
+      if (!err) err = rc_;       \
+      return rc_;                \
+    }                            \
+  } while (0)
+
+grid::~grid() {
+  if (dred) hipFree(dred);
+  for (auto& e : ev) hipEventDestroy(e);
+  for (auto& e : evk) hipEventDestroy(e);
+  if (st) hipStreamDestroy(st);
+}
+
+int grid::setup(const char* params_path, const char* ic) {
+  param = params(params_path);
+  if (!param.ok) return FR_EINVAL;
+  if (!param.IsBox()) return FR_EINVAL;  // the duct path is not built here
+  if (param.bs() != 1) return FR_EINVAL;
+  size = gridsize(param);
+  ic_dir = ic ? ic : ".";
+  FR_CHECK(hipStreamCreate(&st));
+  for (auto& e : ev) FR_CHECK(hipEventCreate(&e));
+  for (auto& e : evk) FR_CHECK(hipEventCreate(&e));
+  // grid.cpp:16-23
+  RK4_preCoeff[0] = 0.5;
+  RK4_preCoeff[1] = 0.5;
+  RK4_preCoeff[2] = 1.;
+  RK4_preCoeff[3] = 1. / 6.;
+  RK4_postCoeff[0] = 1. / 6.;
+  RK4_postCoeff[1] = 1. / 3.;
+  RK4_postCoeff[2] = 1. / 3.;
+  RK4_postCoeff[3] = 1. / 6.;
+  for (tensor1* f : {&RU, &RU_int, &RU_new, &RU_np1, &RHS_RU, &U, &Rho_face, &dummy})
+    f->init(&size, st);
+  for (tensor0* f : {&P, &RHS_Pois, &C, &Rho, &Rho_int, &Rho_new, &Rho_np1, &RHS_Part_Temp,
+                     &scratch})
+    f->init(&size, st);
+  if (!scratch.pointer()) return FR_ENOMEM;
+  FR_TRY(part.init(&param, &size, st, param.N0() > 0 ? param.N0() : 1));
+  FR_TRY(PS_.init(&param, &size, st));
+  FR_CHECK(hipMalloc(&dred, sizeof(double) * (4 * (size_t)size.Nx() + 512)));
+  return FR_OK;
+}
+
+static int read_block(const std::string& path, const gridsize& s, std::vector<double>& ghosted) {
+  const int nx = s.Nx_tot_, ny = s.Ny_tot_, nz = s.Nz_tot_;
+  std::vector<double> blk((size_t)nx * ny * nz);
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return FR_EINVAL;
+  size_t got = fread(blk.data(), sizeof(double), blk.size(), f);
+  fclose(f);
+  if (got != blk.size()) return FR_EINVAL;
+  ghosted.assign((size_t)s.size(), 0.0);
+  size_t n = 0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        ghosted[((size_t)(k + 1) * s.Ny() + (j + 1)) * s.Nx() + (i + 1)] = blk[n++];
+  return FR_OK;
+}
+
+// grid.cpp:110-231 (Initial = 0 and 2 for the box).
+int grid::Initialize() {
+  if (param.Initial() == 0) {
+    const char* names[3] = {"U.bin", "V.bin", "W.bin"};
+    std::vector<double> h;
+    for (int m = 0; m < 3; m++) {
+      FR_TRY(read_block(ic_dir + "/" + names[m], size, h));
+      RU.c(m).upload(h.data());
+      RU.c(m).Update_Ghosts();  // communicator::read -> Update_Ghosts
+    }
+    RU.x.add(param.Rho0() * param.U0());
+  } else if (param.Initial() == 2) {
+    RU.assign(0);
+    RU.x.assign(param.Rho0() * param.U0());
+  } else {
+    return FR_EINVAL;  // restart files: not built here
+  }
+  Rho.assign(param.Rho0());
+  P.assign(0);
+  FR_TRY(part.load_random());
+  P0 = param.P0();
+  T_cur = 0;
+  num_timestep = 0;
+  RU_int.copy(RU);
+  RU_np1.copy(RU);
+  Rho_int.copy(Rho);
+  Rho_np1.copy(Rho);
+  P0_int = P0;
+  P0_np1 = P0;
+  for (int m = 0; m < 3; m++)
+    FR_TRY(fr_t0_interp(&size.g, Rho_face.c(m).pointer(), Rho.pointer(), m, 0, st));
+  FR_TRY(part.dev.copy_stage(1, 0, st));
+  FR_TRY(part.dev.copy_stage(3, 0, st));
+  // T = (P0_int/R)/Rho_int; gas2part_Temp_int; Compute_RHS_Temp_int;
+  // part2gas_Temp_int (grid.cpp:223-226)
+  FR_TRY(part.temp(1, P0_int / param.R(), Rho_int));
+  FR_TRY(part.rebuild_cells(1));
+  FR_TRY(part.part2gas_Temp(1, RHS_Part_Temp));
+  qmean_valid = false;
+  U.x.Equal_Divide(RU.x, Rho_face.x);
+  U.y.Equal_Divide(RU.y, Rho_face.y);
+  U.z.Equal_Divide(RU.z, Rho_face.z);
+  C_BC = U.x.max();
+  FR_CHECK(hipStreamSynchronize(st));
+  return FR_OK;
+}
+
+static BoxCoef box_coef(const grid& g) {
+  BoxCoef c;
+  const double dt = g.param.dt(), pre = g.RK4_preCoeff[g.RK4_count],
+               post = g.RK4_postCoeff[g.RK4_count];
+  c.cpost_neg = -(dt * post);
+  c.cpre_neg = -dt * pre;
+  c.cpost = dt * post;
+  c.cpre = dt * pre;
+  c.last = g.RK4_count == 3;
+  return c;
+}
+
+// grid.cpp:327-333; the same kernel also forms U = RU_int/Rho_face, the first
+// statement of grid::Update_Particle (grid.cpp:662), from unchanged inputs.
+int grid::Update_Rho() {
+  const double* ri[3] = {RU_int.x.pointer(), RU_int.y.pointer(), RU_int.z.pointer()};
+  const double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
+  double* u[3] = {U.x.pointer(), U.y.pointer(), U.z.pointer()};
+  FR_TRY(box_update_rho(size.dev(), box_coef(*this), ri, Rho.pointer(), Rho_np1.pointer(),
+                        Rho_new.pointer(), rf, u, st));
+  return FR_OK;
+}
+
+int grid::Update_P0() {  // grid.cpp:539-554
+  if (!qmean_valid) {
+    qmean = RHS_Part_Temp.mean();
+    qmean_valid = true;
+  }
+  mean_energy_transferred = qmean;
+  if (param.IsBox()) {
+    if (param.Is_Cooling())
+      dP0_dt = 0;
+    else
+      dP0_dt = -param.R() / param.Cv() * mean_energy_transferred;
+    P0_np1 += param.dt() * RK4_postCoeff[RK4_count] * dP0_dt;
+    if (RK4_count != 3)
+      P0_new = P0 + param.dt() * RK4_preCoeff[RK4_count] * dP0_dt;
+    else
+      P0_new = P0_np1;
+  } else {
+    P0_new = P0;
+    P0_np1 = P0;
+  }
+  return FR_OK;
+}
+
+int grid::Update_Particle() {  // grid.cpp:659-680
+  hipEventRecord(ev[2], st);
+  FR_TRY(part.advance(RK4_count, RK4_preCoeff[RK4_count], RK4_postCoeff[RK4_count], U));
+  FR_TRY(part.part2gas_concentration(C));
+  if (param.TWC())
+    FR_TRY(part.part2gas_velocity(RHS_RU));
+  else
+    RHS_RU.assign(0);
+  FR_TRY(part.Send_Recv());
+  FR_TRY(part.temp(2, P0_new / param.R(), Rho_new));
+  FR_TRY(part.rebuild_cells(2));
+  FR_TRY(part.part2gas_Temp(2, RHS_Part_Temp));
+  hipEventRecord(ev[3], st);
+  qmean = RHS_Part_Temp.mean();
+  qmean_valid = true;
+  float ms = 0;
+  hipEventElapsedTime(&ms, ev[2], ev[3]);
+  t_particles += ms;
+  return FR_OK;
+}
+
+int grid::Update_RU_WOP() {  // grid.cpp:387-425
+  WopCoef w;
+  w.a_s = -param.mp() / size.Vcell();
+  w.b_g = param.Mu0() / 3.;
+  w.c_l = param.Mu0();
+  w.A = param.A();
+  w.sx_forc = param.A() * param.Rho0() * param.U0();
+  w.rho0 = param.Rho0();
+  w.gx = param.gx();
+  w.gy = param.gy();
+  w.gz = param.gz();
+  w.gravity = (w.gx != 0 || w.gy != 0 || w.gz != 0);
+  const double* u[3] = {U.x.pointer(), U.y.pointer(), U.z.pointer()};
+  const double* ri[3] = {RU_int.x.pointer(), RU_int.y.pointer(), RU_int.z.pointer()};
+  const double* S[3] = {RHS_RU.x.pointer(), RHS_RU.y.pointer(), RHS_RU.z.pointer()};
+  const double* r0[3] = {RU.x.pointer(), RU.y.pointer(), RU.z.pointer()};
+  double* n1[3] = {RU_np1.x.pointer(), RU_np1.y.pointer(), RU_np1.z.pointer()};
+  double* nw[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
+  const double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
+  hipEventRecord(evk[2 * (RK4_count & 3)], st);
+  FR_TRY(box_update_ru_wop(size.dev(), box_coef(*this), w, u, ri, S, r0, n1, nw, rf, st));
+  hipEventRecord(evk[2 * (RK4_count & 3) + 1], st);
+  return FR_OK;
+}
+
+// grid.cpp:556-563 is evaluated inside the Compute_RHS_Pois kernel, where its
+// only consumer is; this call has nothing to launch.
+int grid::Compute_Div_U_new() { return FR_OK; }
+
+int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
+  DivCoef q;
+  q.kc = param.k() / param.Cp();
+  q.bq = -param.R() / (param.Cp() * P0_new * size.Vcell());
+  q.cooling = param.Is_Cooling();
+  if (q.cooling && !qmean_valid) {
+    qmean = RHS_Part_Temp.mean();
+    qmean_valid = true;
+  }
+  q.qmean = qmean;
+  q.s = (1. / (param.dt() * RK4_preCoeff[RK4_count]));
+  const double* rn[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
+  double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
+  FR_TRY(box_div_rhs(size.dev(), q, Rho_new.pointer(), RHS_Part_Temp.pointer(), rn, rf,
+                     RHS_Pois.pointer(), st));
+  RHS_Pois.make_mean_zero();
+  return FR_OK;
+}
+
+int grid::Solve_Poisson() {  // grid.cpp:639-643
+  dummy.Equal_Divide(1., Rho_face);
+  hipEventRecord(ev[0], st);
+  FR_TRY(PS_.Solve(dummy, P, RHS_Pois));
+  hipEventRecord(ev[1], st);
+  hipEventSynchronize(ev[1]);
+  float ms = 0;
+  hipEventElapsedTime(&ms, ev[0], ev[1]);
+  t_poisson += ms;
+  iters_step += PS_.num_iteration();
+  return FR_OK;
+}
+
+int grid::Update_RU_WP() {  // grid.cpp:651-657
+  double* n1[3] = {RU_np1.x.pointer(), RU_np1.y.pointer(), RU_np1.z.pointer()};
+  double* nw[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
+  FR_TRY(box_update_ru_wp(size.dev(), box_coef(*this), P.pointer(), n1, nw, st));
+  return FR_OK;
+}
+
+// grid.cpp:705-712.  Rho_new and RU_new are rewritten whole before their
+// next read, so exchanging buffers equals the copies.
+int grid::TimeAdvance_RK4() {
+  Rho_int.swap(Rho_new);
+  RU_int.swap(RU_new);
+  P0_int = P0_new;
+  part.dev.swap_stages(1, 2);
+  return FR_OK;
+}
+
+int grid::Correct_mean_RU() {  // grid.cpp:565-577
+  RU_np1.make_mean_zero();
+  RU_np1.x.add(param.Rho0() * param.U0());
+  return FR_OK;
+}
+
+int grid::TimeAdvance() {  // grid.cpp:315-325
+  Rho.copy(Rho_np1);
+  RU.copy(RU_np1);
+  P0 = P0_np1;
+  FR_TRY(part.dev.copy_stage(0, 3, st));
+  T_cur += param.dt();
+  num_timestep++;
+  return FR_OK;
+}
+
+int grid::step() {  // simulation.cpp:36-70 (box calls)
+  param.update(T_cur);
+  t_poisson = t_particles = 0;
+  iters_step = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  for (RK4_count = 0; RK4_count < 4; RK4_count++) {
+    FR_TRY(Update_Rho());
+    FR_TRY(Update_P0());
+    FR_TRY(Update_Particle());
+    FR_TRY(Update_RU_WOP());
+    FR_TRY(Compute_Div_U_new());
+    FR_TRY(Compute_RHS_Pois());
+    FR_TRY(Solve_Poisson());
+    FR_TRY(Update_RU_WP());
+    FR_TRY(TimeAdvance_RK4());
+  }
+  FR_TRY(Correct_mean_RU());
+  FR_TRY(TimeAdvance());
+  FR_CHECK(hipStreamSynchronize(st));
+  t_step = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  t_momentum = 0;
+  for (int k = 0; k < 4; k++) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, evk[2 * k], evk[2 * k + 1]);
+    t_momentum += ms;
+  }
+  n_momentum = 4;
+  return FR_OK;
+}
+
+// grid.cpp:714-811 subset; see fr_sim_diagnostics.
+int grid::diagnostics(double* out) {
+  const Dev d = size.dev();
+  out[1] = Rho.mean();
+  out[2] = Rho.max();
+  out[3] = Rho.min();
+  out[4] = C.max();
+  out[5] = C.min();
+  out[6] = C.mean();
+  double* rred = dred;
+  double* ured = dred + size.Nx();
+  double* prof = dred + 2 * size.Nx();
+  double* psum = dred + 4 * size.Nx();
+  FR_TRY(reducex(d, Rho.pointer(), rred, st));
+  double tke = 0;
+  for (int m = 0; m < 3; m++) {
+    scratch.Equal_Divide(RU.c(m), Rho);  // dummy2 = RU/Rho (grid.cpp:738)
+    FR_TRY(reducex(d, scratch.pointer(), ured, st));
+    FR_TRY(diag_tke(d, scratch.pointer(), Rho.pointer(), ured, scratch.pointer(), st));
+    tke = tke + scratch.mean();
+  }
+  out[0] = tke;
+  // U = RU_np1/Rho_face (grid.cpp:723); dissipation profile (794-811)
+  for (int m = 0; m < 3; m++) dummy.c(m).Equal_Divide(RU_np1.c(m), Rho_face.c(m));
+  const double* uf[3] = {dummy.x.pointer(), dummy.y.pointer(), dummy.z.pointer()};
+  FR_TRY(diag_diss(d, uf, Rho.pointer(), param.Mu0(), scratch.pointer(), st));
+  FR_TRY(reducex(d, scratch.pointer(), prof, st));
+  std::vector<double> h(size.Nx());
+  FR_CHECK(hipMemcpyAsync(h.data(), prof, sizeof(double) * size.Nx(), hipMemcpyDeviceToHost, st));
+  FR_CHECK(hipStreamSynchronize(st));
+  double s = 0;
+  for (int i = 1; i < size.Nx() - 1; i++) s += h[i];
+  out[7] = s / size.Nx_tot_;
+  double ts = 0;
+  if (part.dev.np > 0) FR_TRY(vsum(part.dev.s[6][0], part.dev.np, psum, &ts, st));
+  out[8] = part.dev.np > 0 ? ts / part.dev.np : 0.0;
+  out[9] = PS_.num_iteration();
+  return FR_OK;
+}
+
+tensor0* grid::field0(const std::string& n) {
+  if (n == "P") return &P;
+  if (n == "RHS_Pois") return &RHS_Pois;
+  if (n == "C") return &C;
+  if (n == "Rho") return &Rho;
+  if (n == "Rho_int") return &Rho_int;
+  if (n == "Rho_new") return &Rho_new;
+  if (n == "Rho_np1") return &Rho_np1;
+  if (n == "RHS_Part_Temp") return &RHS_Part_Temp;
+  return nullptr;
+}
+
+tensor1* grid::field1(const std::string& n) {
+  if (n == "RU") return &RU;
+  if (n == "RU_int") return &RU_int;
+  if (n == "RU_new") return &RU_new;
+  if (n == "RU_np1") return &RU_np1;
+  if (n == "RHS_RU") return &RHS_RU;
+  if (n == "U") return &U;
+  if (n == "Rho_face") return &Rho_face;
+  return nullptr;
+}
+
+// ---------------------------------------------------------------- C ABI
+struct fr_sim {
+  grid g;
+};
+
+extern "C" {
+
+int fr_sim_create(const char* params_path, const char* ic_dir, fr_sim** out) {
+  if (!params_path || !out) return FR_EINVAL;
+  fr_sim* s = new fr_sim();
+  int rc = s->g.setup(params_path, ic_dir);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return FR_OK;
+}
+
+int fr_sim_destroy(fr_sim* s) {
+  delete s;
+  return FR_OK;
+}
+
+int fr_sim_initialize(fr_sim* s) { return s ? s->g.Initialize() : FR_EINVAL; }
+
+int fr_sim_step(fr_sim* s) { return s ? s->g.step() : FR_EINVAL; }
+
+int fr_sim_call(fr_sim* s, const char* m, int k) {
+  if (!s || !m) return FR_EINVAL;
+  grid& g = s->g;
+  std::string n(m);
+  g.RK4_count = k;
+  int rc;
+  if (n == "update") {
+    g.param.update(g.T_cur);
+    rc = 0;
+  } else if (n == "Update_Rho") rc = g.Update_Rho();
+  else if (n == "Update_P0") rc = g.Update_P0();
+  else if (n == "Update_Particle") rc = g.Update_Particle();
+  else if (n == "Update_RU_WOP") rc = g.Update_RU_WOP();
+  else if (n == "Compute_Div_U_new") rc = g.Compute_Div_U_new();
+  else if (n == "Compute_RHS_Pois") rc = g.Compute_RHS_Pois();
+  else if (n == "Solve_Poisson") rc = g.Solve_Poisson();
+  else if (n == "Update_RU_WP") rc = g.Update_RU_WP();
+  else if (n == "TimeAdvance_RK4") rc = g.TimeAdvance_RK4();
+  else if (n == "Correct_mean_RU") rc = g.Correct_mean_RU();
+  else if (n == "TimeAdvance") rc = g.TimeAdvance();
+  else if (n == "Send_Recv") rc = g.part.Send_Recv();
+  else return FR_EINVAL;
+  if (!rc) rc = (int)hipStreamSynchronize(g.st);
+  return rc;
+}
+
+static tensor0* pick(fr_sim* s, const char* name, int comp) {
+  if (comp < 0) return s->g.field0(name);
+  tensor1* t = s->g.field1(name);
+  return (t && comp < 3) ? &t->c(comp) : nullptr;
+}
+
+int fr_sim_get_field(fr_sim* s, const char* name, int comp, double* host) {
+  if (!s || !name || !host) return FR_EINVAL;
+  tensor0* t = pick(s, name, comp);
+  if (!t) return FR_EINVAL;
+  t->download(host);
+  return (int)hipGetLastError();
+}
+
+int fr_sim_set_field(fr_sim* s, const char* name, int comp, const double* host) {
+  if (!s || !name || !host) return FR_EINVAL;
+  tensor0* t = pick(s, name, comp);
+  if (!t) return FR_EINVAL;
+  t->upload(host);
+  if (std::string(name) == "RHS_Part_Temp") s->g.qmean_valid = false;
+  return (int)hipGetLastError();
+}
+
+static double* scal(fr_sim* s, const char* name) {
+  grid& g = s->g;
+  std::string n(name);
+  if (n == "P0") return &g.P0;
+  if (n == "P0_int") return &g.P0_int;
+  if (n == "P0_new") return &g.P0_new;
+  if (n == "P0_np1") return &g.P0_np1;
+  if (n == "T_cur") return &g.T_cur;
+  if (n == "C_BC") return &g.C_BC;
+  return nullptr;
+}
+
+int fr_sim_get_scalar(fr_sim* s, const char* name, double* v) {
+  if (!s || !name || !v) return FR_EINVAL;
+  double* p = scal(s, name);
+  if (!p) return FR_EINVAL;
+  *v = *p;
+  return FR_OK;
+}
+
+int fr_sim_set_scalar(fr_sim* s, const char* name, double v) {
+  if (!s || !name) return FR_EINVAL;
+  double* p = scal(s, name);
+  if (!p) return FR_EINVAL;
+  *p = v;
+  return FR_OK;
+}
+
+int fr_sim_geom(fr_sim* s, fr_geom* out) {
+  if (!s || !out) return FR_EINVAL;
+  *out = s->g.size.g;
+  return FR_OK;
+}
+
+int fr_sim_np(fr_sim* s, int* np) {
+  if (!s || !np) return FR_EINVAL;
+  *np = s->g.part.dev.np;
+  return FR_OK;
+}
+
+int fr_sim_get_particles(fr_sim* s, int q, int st, double* host) {
+  if (!s || !host || q < 0 || q > 7 || st < 0 || st > 3) return FR_EINVAL;
+  PartDev& p = s->g.part.dev;
+  const double* src = q == 7 ? p.rhs_T : p.s[q][st];
+  FR_CHECK(hipMemcpy(host, src, sizeof(double) * p.np, hipMemcpyDeviceToHost));
+  return FR_OK;
+}
+
+int fr_sim_set_particles(fr_sim* s, int np, int q, int st, const double* host) {
+  if (!s || !host || q < 0 || q > 7 || st < 0 || st > 3) return FR_EINVAL;
+  PartDev& p = s->g.part.dev;
+  if (np > p.cap || np < 0) return FR_EOVERFLOW;
+  p.np = np;
+  s->g.part.Np = np;
+  double* dst = q == 7 ? p.rhs_T : p.s[q][st];
+  FR_CHECK(hipMemcpy(dst, host, sizeof(double) * np, hipMemcpyHostToDevice));
+  return FR_OK;
+}
+
+int fr_sim_get_flags(fr_sim* s, int* host) {
+  if (!s || !host) return FR_EINVAL;
+  PartDev& p = s->g.part.dev;
+  FR_CHECK(hipMemcpy(host, p.flag, sizeof(int) * p.np, hipMemcpyDeviceToHost));
+  return FR_OK;
+}
+
+int fr_sim_set_flags(fr_sim* s, int np, const int* host) {
+  if (!s || !host) return FR_EINVAL;
+  PartDev& p = s->g.part.dev;
+  if (np > p.cap || np < 0) return FR_EOVERFLOW;
+  p.np = np;
+  s->g.part.Np = np;
+  FR_CHECK(hipMemcpy(p.flag, host, sizeof(int) * np, hipMemcpyHostToDevice));
+  return FR_OK;
+}
+
+int fr_sim_particles_changed(fr_sim* s) {
+  if (!s) return FR_EINVAL;
+  int rc = s->g.part.rebuild_cells(1);
+  if (!rc) rc = (int)hipStreamSynchronize(s->g.st);
+  return rc;
+}
+
+int fr_sim_diagnostics(fr_sim* s, double* out) {
+  if (!s || !out) return FR_EINVAL;
+  return s->g.diagnostics(out);
+}
+
+int fr_sim_timing(fr_sim* s, double* out) {
+  if (!s || !out) return FR_EINVAL;
+  out[0] = s->g.t_step;
+  out[1] = s->g.t_poisson;
+  out[2] = s->g.t_particles;
+  out[3] = s->g.iters_step;
+  out[4] = s->g.t_momentum;
+  out[5] = s->g.n_momentum;
+  return FR_OK;
+}
+
+}  // extern "C"

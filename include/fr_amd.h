@@ -1,0 +1,168 @@
+/*
+ * fr_amd.h - C ABI of the MI355X (gfx950) time-step library libfr_amd.so.
+ *
+ * Plain pointers and sizes only.  Every field pointer is a DEVICE pointer to
+ * a ghosted double array laid out like the reference's tensor0
+ * (tensor0.cpp:72-80): value(i,j,k) = ptr[(k*Ny + j)*Nx + i], i fastest,
+ * Nx = Nx_tot + 2*bs.  A tensor1 is passed as its three components (x, y, z)
+ * on the left faces of each cell.  Streams are hipStream_t passed as void*
+ * (NULL = default stream).  Every entry point returns 0 on success or a
+ * nonzero HIP error code / FR_E* code; nothing is allocated implicitly except
+ * where an entry point says it owns a workspace.
+ *
+ * Each entry point names the reference interface it replaces (file:line).
+ * The reference has no C ABI: its plug-in surface is the C++ classes tensor0,
+ * tensor1, particle, poisson, poisson_IO and grid; the host library keeps
+ * those classes (first_repo_amd/host/) and they call the functions below.
+ */
+#ifndef FR_AMD_H
+#define FR_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FR_OK 0
+#define FR_EINVAL 1001   /* bad argument (size, pointer, unsupported shape) */
+#define FR_ENOMEM 1002   /* device allocation failed */
+#define FR_EOVERFLOW 1003 /* particle capacity exceeded */
+
+/* Grid geometry of one rank's ghosted arrays (gridsize.h:9-58). */
+typedef struct fr_geom {
+  int Nx, Ny, Nz;  /* ghosted sizes, N_tot + 2*bs */
+  int bs;          /* ghost width; kernels support bs == 1 (grid.cpp:366) */
+  double dx, dy, dz;
+  double Lx, Ly, Lz;
+  int is_box;      /* 1: triply periodic box, 0: inflow/outflow duct */
+} fr_geom;
+
+/* ---------------------------------------------------------------- library */
+/* Library version string and device check. */
+const char* fr_version(void);
+/* Number of visible HIP devices (0 when none: every compute entry point then
+ * fails with the HIP error of its first launch). */
+int fr_device_count(void);
+
+/* -------------------------------------------------- tensor0/tensor1 stencils
+ * Each applies the reference write range and then the reference halo
+ * (Update_Ghosts: periodic wrap in the box; y,z wrap only in the duct). */
+/* tensor0::Equal_Div_F2C (tensor0.cpp:303-310) */
+int fr_t0_div_f2c(const fr_geom* g, double* out, const double* tx,
+                  const double* ty, const double* tz, void* stream);
+/* tensor0::Equal_Del2 (tensor0.cpp:312-319) */
+int fr_t0_del2(const fr_geom* g, double* out, const double* in, void* stream);
+/* tensor1::Equal_Grad_C2F (tensor1.cpp:242-260) */
+int fr_t1_grad_c2f(const fr_geom* g, double* ox, double* oy, double* oz,
+                   const double* in, void* stream);
+/* tensor0::Equal_I{x,y,z}_{C2F,F2C} (tensor0.cpp:353-406):
+ * axis 0/1/2, dir 0 = C2F, 1 = F2C */
+int fr_t0_interp(const fr_geom* g, double* out, const double* in, int axis,
+                 int dir, void* stream);
+/* tensor0::Equal_I_F2C (tensor0.cpp:408-416) */
+int fr_t0_i_f2c(const fr_geom* g, double* out, const double* tx,
+                const double* ty, const double* tz, void* stream);
+/* tensor0::Update_Ghosts / Update_Ghosts_CUM (tensor0.cpp:83-91 ->
+ * communicator.cpp:177-396 / 422-626), single rank */
+int fr_t0_update_ghosts(const fr_geom* g, double* f, int cumulative,
+                        void* stream);
+/* Element-wise arithmetic over the whole ghosted array (tensor0.cpp:93-241):
+ * out = a*A + b*B + c*C with NULL fields treated as absent;
+ * op selects the form:
+ *   0: out = a*A + b*B + c*C      (Equal_LinComb, Equal_Mult(double,..))
+ *   1: out += a*A                 (PlusEqual_Mult(double, tensor0))
+ *   2: out = A*B + b*C            (Equal_LinComb(tensor0,tensor0,double,..))
+ *   3: out = a / A                (Equal_Divide(double, tensor0))
+ *   4: out = A / B                (Equal_Divide(tensor0, tensor0))
+ *   5: out = a                    (operator=(double))
+ *   6: out += a                   (operator+=(double))
+ *   7: out *= a                   (operator*=(double))
+ *   8: out = A                    (operator=(tensor0))
+ *   9: out *= A                   (operator*=(tensor0))
+ *  10: out -= A                   (operator-=(tensor0))
+ *  11: out += A                   (operator+=(tensor0)) */
+int fr_t0_elementwise(const fr_geom* g, int op, double* out, double a,
+                      const double* A, double b, const double* B, double c,
+                      const double* Cf, void* stream);
+/* Reductions over the interior (tensor0.cpp:418-516):
+ * op 0 sum, 1 mean, 2 max, 3 min, 4 max_abs, 5 sum_squares.
+ * The result is written to *result (host memory); synchronises the stream.
+ * Sums are deterministic tree sums (order differs from the reference's serial
+ * i->j->k loop at the level of rounding). */
+int fr_t0_reduce(const fr_geom* g, const double* f, int op, double* result,
+                 void* stream);
+/* communicator::REDUCEX (communicator.cpp:635-646): out[i] = sum over
+ * interior j,k of f(i,j,k) / (Ny_tot*Nz_tot), for every i; out is a device
+ * array of Nx doubles. */
+int fr_t0_reducex(const fr_geom* g, const double* f, double* out,
+                  void* stream);
+
+/* ------------------------------------------------------------ pressure
+ * Box pressure solve: the defect-correction loop of poisson::FFT
+ * (poisson.cpp:391-411) around the constant-coefficient 3-D FFT solve
+ * poisson::CCP_FFT (poisson.cpp:413-462), on one device.
+ * Solves div(C grad P) = RHS with P as the initial guess (poisson.h:79); C
+ * is the face coefficient (1/rho_face in grid.cpp:641).  Stops after
+ * `iterations` or when max|dp| <= epsilon; *num_iter receives the iteration
+ * count (poisson.h:80).  Interior sizes must be powers of two in [2, 1024].
+ * P's ghosts must hold its periodic images on entry; they do on exit.
+ * The solver object holds its FFT workspace. */
+typedef struct fr_box_poisson fr_box_poisson;
+int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out);
+int fr_box_poisson_destroy(fr_box_poisson* s);
+int fr_box_poisson_solve(fr_box_poisson* s, const double* cx,
+                         const double* cy, const double* cz, double* P,
+                         const double* RHS, int iterations, double epsilon,
+                         int* num_iter, void* stream);
+
+/* ------------------------------------------------------------ whole solver
+ * The box solver (grid.h:16-115 with its particle set, box poisson solver
+ * and the simulation.cpp:34-71 step loop) held on one device.  Created from a
+ * params.in file (params.cpp:6-64); Initial=0 reads U.bin/V.bin/W.bin from
+ * ic_dir (grid.cpp:184-196).  Host buffers passed to the get/set calls use
+ * the ghosted layout above; particle arrays are Np doubles (int32 for flags). */
+typedef struct fr_sim fr_sim;
+int fr_sim_create(const char* params_path, const char* ic_dir, fr_sim** out);
+int fr_sim_destroy(fr_sim* s);
+/* grid::Initialize (grid.cpp:110-231) */
+int fr_sim_initialize(fr_sim* s);
+/* One full time step: 4 RK4 substeps, Correct_mean_RU, TimeAdvance
+ * (simulation.cpp:36-70, box calls). */
+int fr_sim_step(fr_sim* s);
+/* One named grid method (the names of grid.h:93-114, plus "update" for
+ * params::update and "Send_Recv" for particle::Send_Recv) at RK4 stage k. */
+int fr_sim_call(fr_sim* s, const char* method, int rk4_count);
+/* Field transfer: comp -1 for a tensor0, 0..2 for a tensor1 component.
+ * Names: RU RU_int RU_new RU_np1 RHS_RU U Rho_face (tensor1); P RHS_Pois C
+ * Rho Rho_int Rho_new Rho_np1 RHS_Part_Temp (tensor0). */
+int fr_sim_get_field(fr_sim* s, const char* name, int comp, double* host);
+int fr_sim_set_field(fr_sim* s, const char* name, int comp,
+                     const double* host);
+/* Scalars: P0 P0_int P0_new P0_np1 T_cur C_BC */
+int fr_sim_get_scalar(fr_sim* s, const char* name, double* value);
+int fr_sim_set_scalar(fr_sim* s, const char* name, double value);
+int fr_sim_geom(fr_sim* s, fr_geom* out);
+int fr_sim_np(fr_sim* s, int* np);
+/* Particle stage q (0..6 = x,y,z,u,v,w,T), s (0..3 = base,_int,_new,_np1);
+ * q == 7 is RHS_T (s ignored). */
+int fr_sim_get_particles(fr_sim* s, int q, int st, double* host);
+int fr_sim_set_particles(fr_sim* s, int np, int q, int st,
+                         const double* host);
+int fr_sim_get_flags(fr_sim* s, int* host);
+int fr_sim_set_flags(fr_sim* s, int np, const int* host);
+/* Rebuild the particle cell lists after particle positions were set. */
+int fr_sim_particles_changed(fr_sim* s);
+/* Step diagnostics (grid.cpp:714-811 subset): out[0..9] = TKE, Rho_Mean,
+ * Rho_Max, Rho_Min, C_Max, C_Min, C_Mean, mean dissipation, mean particle
+ * temperature, Poisson iterations of the last solve. */
+int fr_sim_diagnostics(fr_sim* s, double* out);
+/* Timing of the last fr_sim_step: out[0] whole step (ms, host clock around
+ * the synchronised step), [1] Poisson solves (ms, HIP events), [2] particle
+ * phase (ms, HIP events), [3] Poisson iterations in the step, [4] summed
+ * duration of the fused momentum kernel (ms, HIP events on its stream),
+ * [5] its launch count. */
+int fr_sim_timing(fr_sim* s, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FR_AMD_H */

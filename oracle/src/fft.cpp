@@ -83,22 +83,40 @@ void fft1d(double* re, double* im, int n, long stride, int sign) {
   }
 }
 
-// 3-D transform of an [nz][ny][nx] array (x fastest).
+// 3-D transform of an [nz][ny][nx] array (x fastest).  The forward transform
+// (sign -1) runs the x, y, z passes; the inverse (sign +1) runs z, y, x.  The
+// pass order does not change the transform; this one is chosen so the device
+// solver, which fuses the forward and inverse z passes, matches bit for bit.
 void fft3d(std::vector<double>& re, std::vector<double>& im, int nx, int ny,
            int nz, int sign) {
   const long sx = 1, sy = nx, sz = (long)nx * ny;
+  auto px = [&]() {
 #pragma omp parallel for schedule(static)
-  for (int k = 0; k < nz; k++)
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        fft1d(&re[k * sz + j * sy], &im[k * sz + j * sy], nx, sx, sign);
+  };
+  auto py = [&]() {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; k++)
+      for (int i = 0; i < nx; i++)
+        fft1d(&re[k * sz + i], &im[k * sz + i], ny, sy, sign);
+  };
+  auto pz = [&]() {
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; j++)
-      fft1d(&re[k * sz + j * sy], &im[k * sz + j * sy], nx, sx, sign);
-#pragma omp parallel for schedule(static)
-  for (int k = 0; k < nz; k++)
-    for (int i = 0; i < nx; i++)
-      fft1d(&re[k * sz + i], &im[k * sz + i], ny, sy, sign);
-#pragma omp parallel for schedule(static)
-  for (int j = 0; j < ny; j++)
-    for (int i = 0; i < nx; i++)
-      fft1d(&re[j * sy + i], &im[j * sy + i], nz, sz, sign);
+      for (int i = 0; i < nx; i++)
+        fft1d(&re[j * sy + i], &im[j * sy + i], nz, sz, sign);
+  };
+  if (sign < 0) {
+    px();
+    py();
+    pz();
+  } else {
+    pz();
+    py();
+    px();
+  }
 }
 
 // 2-D transform of an [nz][ny] plane (y fastest).

@@ -1,0 +1,96 @@
+"""Helpers for the GPU parity tests: device buffers (torch is only used for
+device memory), the oracle/device state sync and comparisons."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+import first_repo_amd as fr
+from first_repo_amd.configs import BoxWorkload
+import oracle_lib as OL
+
+# live state between steps (the reference's RHS_RU and Rho_new hold dead
+# intermediates there, and Update_RU_WOP leaves Rho_face shifted by Rho0 until
+# Compute_RHS_Pois rewrites it; the fused kernels do not store those)
+FIELDS1 = ["RU", "RU_int", "RU_np1", "Rho_face"]
+FIELDS0 = ["P", "RHS_Pois", "C", "Rho", "Rho_int", "Rho_np1", "RHS_Part_Temp"]
+SYNC1 = FIELDS1 + ["RU_new", "U", "RHS_RU"]
+SYNC0 = FIELDS0 + ["Rho_new"]
+
+
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU test needs a HIP device"
+    return torch
+
+
+def dbuf(a):
+    """Device copy of a numpy float64 array."""
+    torch = torch_dev()
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda")
+
+
+def dptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def rel(a, b):
+    """Normwise relative difference max|a-b| / max|b|."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.size == 0:
+        return 0.0
+    den = np.max(np.abs(b))
+    num = np.max(np.abs(a - b))
+    return float(num / den) if den > 0 else float(num)
+
+
+def make_box_case(tmp, n=16, n_particles=2000, seed=7, **kw):
+    w = BoxWorkload(n=n, n_particles=n_particles, steps=5, seed=seed, **kw)
+    d = os.path.join(str(tmp), f"box{n}_{n_particles}_{seed}")
+    w.build(d)
+    return os.path.join(d, "params.in"), d
+
+
+def sync_oracle_to_device(og: OL.OracleGrid, sim: fr.Sim):
+    """Copy the complete oracle grid state into the device solver."""
+    for f in SYNC1:
+        for c in range(3):
+            sim.set_field(f, og.field(f, c), c)
+    for f in SYNC0:
+        sim.set_field(f, og.field(f))
+    for s in ("P0", "P0_int", "P0_new", "P0_np1", "T_cur", "C_BC"):
+        sim.set_scalar(s, og.scalar(s)[0])
+    for q in range(7):
+        for s in range(4):
+            sim.set_part(q, s, og.part(q, s))
+    n = og.np_
+    sim.set_part(7, 0, np.ctypeslib.as_array(OL.orc().orc_grid_part(og.h, 7, 0), shape=(n,)))
+    sim.set_flags(og.flag())
+    sim.particles_changed()
+
+
+def compare_state(og, sim, tol=1e-12, fields1=FIELDS1, fields0=FIELDS0, particles=True):
+    """Largest normwise relative difference per quantity; asserts flags equal."""
+    worst = {}
+    for f in fields1:
+        for c in range(3):
+            worst[f"{f}.{c}"] = rel(sim.field(f, c), og.field(f, c))
+    for f in fields0:
+        worst[f] = rel(sim.field(f), og.field(f))
+    if particles:
+        assert sim.np_ == og.np_
+        assert np.array_equal(sim.flags(), og.flag()), "particle order differs"
+        for q in range(7):
+            for s in range(4):
+                worst[f"part{q}.{s}"] = rel(sim.part(q, s), og.part(q, s))
+    bad = {k: v for k, v in worst.items() if not v <= tol}
+    return worst, bad
