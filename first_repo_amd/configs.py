@@ -142,7 +142,7 @@ def solenoidal_velocity(n: tuple[int, int, int], L: tuple[float, float, float],
     A = []
     for _ in range(3):
         a = rng.standard_normal((nx, ny, nz))
-        A.append(np.fft.irfftn(np.fft.rfftn(a) * filt, s=(nx, ny, nz)))
+        A.append(np.fft.irfftn(np.fft.rfftn(a) * filt, s=(nx, ny, nz), axes=(0, 1, 2)))
     Ax, Ay, Az = A
     dx, dy, dz = L[0] / nx, L[1] / ny, L[2] / nz
 
