@@ -1,4 +1,4 @@
-"""Helpers for the GPU parity tests: device buffers (torch is only used for
+"""Helpers for the GPU parity tests: device buffers through the HIP runtime
 device memory), the oracle/device state sync and comparisons."""
 from __future__ import annotations
 
@@ -20,26 +20,60 @@ SYNC1 = FIELDS1 + ["RU_new", "U", "RHS_RU"]
 SYNC0 = FIELDS0 + ["Rho_new"]
 
 
-def torch_dev():
-    import torch
-    assert torch.cuda.is_available(), "GPU test needs a HIP device"
-    return torch
+_hip = None
+
+
+def hip():
+    """The HIP runtime libfr_amd.so itself links (/opt/rocm).  torch is not
+    imported in GPU tests: it bundles a second HIP/HSA runtime, and two in one
+    process cannot share the device."""
+    global _hip
+    if _hip is None:
+        fr.lib()  # loads libamdhip64.so.7 as a dependency
+        _hip = C.CDLL("libamdhip64.so.7")
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
+        n = fr.lib().fr_device_count()
+        assert n > 0, "GPU test needs a HIP device"
+    return _hip
+
+
+class DBuf:
+    """A device copy of a numpy float64 array."""
+
+    def __init__(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.shape = a.shape
+        self.nbytes = a.nbytes
+        self.p = C.c_void_p()
+        assert hip().hipMalloc(C.byref(self.p), max(self.nbytes, 8)) == 0
+        assert hip().hipMemcpy(self.p, a.ctypes.data, self.nbytes, 1) == 0  # H2D
+
+    def get(self):
+        out = np.empty(self.shape)
+        assert hip().hipDeviceSynchronize() == 0
+        assert hip().hipMemcpy(out.ctypes.data, self.p, self.nbytes, 2) == 0  # D2H
+        return out
+
+    def __del__(self):
+        try:
+            hip().hipFree(self.p)
+        except Exception:
+            pass
 
 
 def dbuf(a):
-    """Device copy of a numpy float64 array."""
-    torch = torch_dev()
-    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda")
+    return DBuf(a)
 
 
 def dptr(t):
-    return C.c_void_p(t.data_ptr())
+    return t.p
 
 
 def host(t):
-    import torch
-    torch.cuda.synchronize()
-    return t.cpu().numpy()
+    return t.get()
 
 
 def rel(a, b):
