@@ -13,6 +13,7 @@
 // (the same algorithm the CPU oracle uses, so the two agree bit for bit).
 // A workgroup holds W pencils of length L in LDS, padded by one complex per
 // pencil to spread the LDS banks, within a 64 KB budget (pencils + twiddles).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -21,37 +22,68 @@
 namespace fr {
 
 constexpr int FT = 256;        // threads per FFT workgroup
-constexpr int FLDS = 4096;     // complex elements (64 KB) of LDS per workgroup
-
-// Pencils per workgroup for length L: W*(L+1) + 2*(L/2+1) <= FLDS, W <= cap.
-__host__ __device__ inline int pencils(int L, int cap) {
-  int w = (FLDS - 2 * (L / 2 + 1)) / (L + 1);
+// Pencils per workgroup for length L within `budget` complex elements:
+// W*(L+1) + 2*(L/2+1) <= budget, W a power of two, W <= cap.
+inline int pencils(int L, int cap, int budget) {
+  int w = (budget - 2 * (L / 2 + 1)) / (L + 1);
   if (w < 1) w = 1;
-  return w < cap ? w : cap;
+  int p2 = 1;
+  while (p2 * 2 <= w) p2 *= 2;
+  return p2 < cap ? p2 : cap;
 }
 
 __device__ __forceinline__ int bitrev(int x, int logn) { return __brev(x) >> (32 - logn); }
 
+__device__ __forceinline__ void bfly(double2& u, double2& t, const double2 w) {
+  const double vr = t.x * w.x - t.y * w.y;
+  const double vi = t.x * w.y + t.y * w.x;
+  const double2 a = u;
+  u = make_double2(a.x + vr, a.y + vi);
+  t = make_double2(a.x - vr, a.y - vi);
+}
+
 // In-place radix-2 DIT over W pencils (stride ps complex) already in
 // bit-reversed order.  tw[k] = (cos(2 pi k/L), sign*sin(2 pi k/L)), k <= L/2.
-__device__ void fft_lds(double2* buf, int L, int W, int ps, const double2* tw) {
-  const int nb = W * (L >> 1);
-  for (int half = 1; half < L; half <<= 1) {
+// Stages are applied in pairs from registers (each thread owns the four
+// elements two consecutive stages combine), with exactly the butterflies and
+// twiddles of the stage-by-stage algorithm, so the result is bit-identical.
+__device__ void fft_lds(double2* buf, int L, int lg, int W, int ps, const double2* tw) {
+  int half = 1, s = 0;
+  const int q4 = L >> 2;
+  for (; s + 1 < lg; s += 2, half <<= 2) {
     __syncthreads();
-    const int step = L / (2 * half);
-    for (int b = threadIdx.x; b < nb; b += FT) {
-      const int p = b / (L >> 1);
-      const int bb = b - p * (L >> 1);
+    const int stepA = L / (2 * half), stepB = L / (4 * half);
+    for (int b = threadIdx.x; b < W * q4; b += FT) {
+      const int p = b / q4;
+      const int bb = b - p * q4;
+      const int grp = bb / half;
+      const int pos = bb - grp * half;
+      const int i0 = p * ps + grp * 4 * half + pos;
+      double2 x0 = buf[i0], x1 = buf[i0 + half], x2 = buf[i0 + 2 * half], x3 = buf[i0 + 3 * half];
+      const double2 wa = tw[pos * stepA];
+      bfly(x0, x1, wa);
+      bfly(x2, x3, wa);
+      bfly(x0, x2, tw[pos * stepB]);
+      bfly(x1, x3, tw[(pos + half) * stepB]);
+      buf[i0] = x0;
+      buf[i0 + half] = x1;
+      buf[i0 + 2 * half] = x2;
+      buf[i0 + 3 * half] = x3;
+    }
+  }
+  if (s < lg) {  // odd number of stages: the last one alone
+    __syncthreads();
+    const int step = L / (2 * half), h2 = L >> 1;
+    for (int b = threadIdx.x; b < W * h2; b += FT) {
+      const int p = b / h2;
+      const int bb = b - p * h2;
       const int grp = bb / half;
       const int pos = bb - grp * half;
       const int i0 = p * ps + grp * 2 * half + pos;
-      const int i1 = i0 + half;
-      const double2 w = tw[pos * step];
-      const double2 u = buf[i0], t = buf[i1];
-      const double vr = t.x * w.x - t.y * w.y;
-      const double vi = t.x * w.y + t.y * w.x;
-      buf[i0] = make_double2(u.x + vr, u.y + vi);
-      buf[i1] = make_double2(u.x - vr, u.y - vi);
+      double2 u = buf[i0], t = buf[i0 + half];
+      bfly(u, t, tw[pos * step]);
+      buf[i0] = u;
+      buf[i0 + half] = t;
     }
   }
   __syncthreads();
@@ -107,7 +139,7 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
     }
     buf[p * ps + bitrev(e, q.lx)] = make_double2(r, 0.0);
   }
-  fft_lds(buf, L, W, ps, tws);
+  fft_lds(buf, L, q.lx, W, ps, tws);
   for (int t = threadIdx.x; t < W * L; t += FT) {
     const int p = t / L, e = t - p * L;
     const long row = row0 + p;
@@ -144,7 +176,7 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
     const int e = t / W, p = t - e * W;
     if (i0 + p < q.nx) buf[p * ps + bitrev(e, lg)] = data[base + e * stride + i0 + p];
   }
-  fft_lds(buf, L, W, ps, tws);
+  fft_lds(buf, L, lg, W, ps, tws);
   if (divide) {
     // lambda = (ax + ay) + az, host tables from the reference formula
     for (int t = threadIdx.x; t < W * L; t += FT) {
@@ -169,7 +201,7 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
         buf[p * ps + r] = a;
       }
     }
-    fft_lds(buf, L, W, ps, tws2);
+    fft_lds(buf, L, lg, W, ps, tws2);
   }
   for (int t = threadIdx.x; t < W * L; t += FT) {
     const int e = t / W, p = t - e * W;
@@ -197,7 +229,7 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q, const double2* __r
     const long row = row0 + p;
     buf[p * ps + bitrev(e, q.lx)] = row < rows ? in[row * L + e] : make_double2(0.0, 0.0);
   }
-  fft_lds(buf, L, W, ps, tws);
+  fft_lds(buf, L, q.lx, W, ps, tws);
   double m = 0.0;
   for (int t = threadIdx.x; t < W * L; t += FT) {
     const int p = t / L, e = t - p * L;
@@ -247,6 +279,7 @@ struct fr_box_poisson {
   double* dres = nullptr;
   double* hres = nullptr;
   int nbx = 0;
+  int lds_max = 0;
 };
 
 static int ilog2(int n) {
@@ -304,9 +337,30 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
     FR_CHECK(hipMalloc(&s->lam[a], sizeof(double) * n[a]));
     FR_CHECK(hipMemcpy(s->lam[a], lam.data(), sizeof(double) * n[a], hipMemcpyHostToDevice));
   }
-  s->q.wx = pencils(n[0], n[1] * n[2]);
-  s->q.wy = pencils(n[1], n[0]);
-  s->q.wz = pencils(n[2], n[0]);
+  // LDS budget: gfx950 has 160 KB per CU; use what the device grants a
+  // workgroup, at most 160 KB.
+  int dev = 0, maxsh = 65536;
+  if (hipGetDevice(&dev) == hipSuccess)
+    if (hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      maxsh = 65536;
+  if (maxsh > 163840) maxsh = 163840;
+  if (maxsh < 65536) maxsh = 65536;
+  const int budget = maxsh / (int)sizeof(double2) - 160;  // room for the static reduction array
+  s->q.wx = pencils(n[0], n[1] * n[2], budget);
+  s->q.wy = pencils(n[1], n[0], budget);
+  s->q.wz = pencils(n[2], n[0], budget);
+  const size_t need = sizeof(double2) * (size_t)std::max(
+      {s->q.wx * (n[0] + 1) + n[0] / 2 + 1, s->q.wy * (n[1] + 1) + 2 * (n[1] / 2 + 1),
+       s->q.wz * (n[2] + 1) + 2 * (n[2] / 2 + 1)});
+  if (need > 65536) {
+    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_strided,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_inv,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  }
+  s->lds_max = (int)need;
   const int W = s->q.wx;
   s->nbx = (int)(((long)n[1] * n[2] + W - 1) / W);
   FR_CHECK(hipMalloc(&s->bmax, sizeof(double) * (s->nbx + 1)));

@@ -80,6 +80,15 @@ struct AdvCoef {
   int last;
 };
 
+// Cell buckets of the particle set: particles grouped by cell index (rounded
+// or truncating), each bucket in ascending particle order, with the deposit
+// operands of each particle gathered into bucket order.
+struct Buckets {
+  int *cnt = nullptr, *start = nullptr, *list = nullptr, *key = nullptr, *slot = nullptr;
+  double* s[6] = {};
+  bool valid = false;
+};
+
 // Device particle set (SoA, the 29 migrated values of particle.cpp:252-262
 // plus the RHS arrays), its cell buckets and migration workspace.
 class PartDev {
@@ -89,13 +98,13 @@ class PartDev {
   int* flag = nullptr;
   double *rhs_u = nullptr, *rhs_v = nullptr, *rhs_w = nullptr, *rhs_T = nullptr;
   long ncell = 0;
-  int *cnt = nullptr, *start = nullptr, *list = nullptr, *key = nullptr, *slot = nullptr;
+  Buckets R;  // rounded index, operands x_l,y_l,z_l,RHS_T
+  Buckets T;  // truncating index, operands x_l,y_l,z_l,RHS_u,RHS_v,RHS_w
   int* scan_tmp = nullptr;
   int *flg = nullptr, *nl = nullptr, *pNL = nullptr, *holepos = nullptr, *tailnl = nullptr;
   int *evt = nullptr, *islo = nullptr, *plo = nullptr, *dst = nullptr, *stgf = nullptr;
   double* stg = nullptr;
   int* hbuf = nullptr;
-  bool cells_stage_valid = false;
   long moved_last = 0;
 
   int alloc(int capacity, const Dev& d);
@@ -103,9 +112,11 @@ class PartDev {
   PartPtrs ptrs() const;
   int advance(const Dev& d, const AdvCoef& c, const double* const u[3], hipStream_t st);
   int temp(const Dev& d, int stage, double c0, const double* rho, double iTth, hipStream_t st);
-  int build_cells(const Dev& d, int stage, hipStream_t st);
-  int deposit_cic(const Dev& d, int stage, const double* w, double* out, bool halo,
-                  hipStream_t st);
+  // buckets of the `stage` positions; mode 0 rounded (R), 1 truncating (T)
+  int build(const Dev& d, int stage, int mode, hipStream_t st);
+  // trilinear deposit from R: weighted by RHS_T (heat) or not (concentration)
+  int deposit_cic(const Dev& d, bool weighted, double* out, bool halo, hipStream_t st);
+  // momentum deposit from T
   int deposit_vel(const Dev& d, double* const out[3], hipStream_t st);
   int migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st);
   void swap_stages(int a, int b);

@@ -185,19 +185,30 @@ int scan_ex(const int* in, long n, int* out, int* tmp, hipStream_t s) {
   return FR_OK;
 }
 
-// ---- cell buckets (rounded index of the upper cell centre) -------------------
-__device__ __forceinline__ long round_cell(const Dev& d, double x, double y, double z) {
-  const int ii = (int)(round(x * d.idx)) + 1;
-  const int jj = (int)(round(y * d.idy)) + 1;
-  const int kk = (int)(round(z * d.idz)) + 1;
+// ---- cell buckets ---------------------------------------------------------------
+// mode 0: rounded index of the upper cell centre (x2i_round, particle.cpp:
+// 121-134), used by the trilinear deposits; mode 1: truncating index (x2i,
+// particle.cpp:106-119), used by the 1-D momentum deposit.
+__device__ __forceinline__ long bucket_cell(const Dev& d, int mode, double x, double y,
+                                            double z) {
+  int ii, jj, kk;
+  if (mode == 0) {
+    ii = (int)(round(x * d.idx)) + 1;
+    jj = (int)(round(y * d.idy)) + 1;
+    kk = (int)(round(z * d.idz)) + 1;
+  } else {
+    ii = (int)(x * d.idx) + 1;
+    jj = (int)(y * d.idy) + 1;
+    kk = (int)(z * d.idz) + 1;
+  }
   return clampi((long)kk * d.NxNy + (long)jj * d.Nx + ii, d.size);
 }
 
-__global__ void k_bucket_count(Dev d, PartPtrs p, int st, int* __restrict__ cnt,
+__global__ void k_bucket_count(Dev d, PartPtrs p, int st, int mode, int* __restrict__ cnt,
                                int* __restrict__ key, int* __restrict__ slot) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.np) return;
-  const long c = round_cell(d, p.s[0][st][i], p.s[1][st][i], p.s[2][st][i]);
+  const long c = bucket_cell(d, mode, p.s[0][st][i], p.s[1][st][i], p.s[2][st][i]);
   key[i] = (int)c;
   slot[i] = atomicAdd(&cnt[c], 1);
 }
@@ -226,13 +237,44 @@ __global__ void k_bucket_sort(long ncell, const int* __restrict__ start, int* __
   }
 }
 
+// Per-particle deposit operands gathered into bucket order, so the deposit
+// kernels read them contiguously.
+// mode 0 (particle.cpp:536-542): x_l = x*idx - round(x*idx) + 0.5, w = RHS_T.
+// mode 1 (particle.cpp:505-511): x_l = x*idx - trunc(x*idx), w = RHS_u,v,w.
+__global__ void k_bucket_gather(int np, int mode, PartPtrs p, int st,
+                                const int* __restrict__ list, double idx, double idy,
+                                double idz, double* __restrict__ s0, double* __restrict__ s1,
+                                double* __restrict__ s2, double* __restrict__ s3,
+                                double* __restrict__ s4, double* __restrict__ s5) {
+  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= np) return;
+  const int i = list[pos];
+  const double X = p.s[0][st][i], Y = p.s[1][st][i], Z = p.s[2][st][i];
+  if (mode == 0) {
+    const int ii = (int)(round(X * idx));
+    const int jj = (int)(round(Y * idy));
+    const int kk = (int)(round(Z * idz));
+    s0[pos] = X * idx - ii + 0.5;
+    s1[pos] = Y * idy - jj + 0.5;
+    s2[pos] = Z * idz - kk + 0.5;
+    s3[pos] = p.rhs_T[i];
+  } else {
+    s0[pos] = (X * idx - ((int)(X * idx)));
+    s1[pos] = (Y * idy - ((int)(Y * idy)));
+    s2[pos] = (Z * idz - ((int)(Z * idz)));
+    s3[pos] = p.rhs_u[i];
+    s4[pos] = p.rhs_v[i];
+    s5[pos] = p.rhs_w[i];
+  }
+}
+
 // ---- deposits -----------------------------------------------------------------
 // K-way merge over bucket ranges in ascending particle order.
 template <int NB>
 struct Merge {
   int cur[NB], end[NB];
-  __device__ __forceinline__ int next(const int* __restrict__ list, int& which) {
-    int best = INT_MAX, wb = -1;
+  __device__ __forceinline__ int next(const int* __restrict__ list, int& which, int& pos) {
+    int best = INT_MAX, wb = -1, bp = 0;
 #pragma unroll
     for (int q = 0; q < NB; q++) {
       if (cur[q] < end[q]) {
@@ -240,6 +282,7 @@ struct Merge {
         if (v < best) {
           best = v;
           wb = q;
+          bp = cur[q];
         }
       }
     }
@@ -247,18 +290,21 @@ struct Merge {
     for (int q = 0; q < NB; q++)
       if (q == wb) cur[q]++;
     which = wb;
+    pos = bp;
     return best;
   }
 };
 
-// Trilinear (cloud-in-cell) deposit of particle.cpp:526-559 (w == nullptr,
-// concentration) or 586-658 (weight RHS_T) at the stage `st` positions.
-// Writes the raw per-cell sums for every ghosted cell (before the
-// cumulative halo).
-__global__ void __launch_bounds__(256) k_dep_cic(Dev d, PartPtrs p, int st,
-                                                 const double* __restrict__ wgt,
-                                                 const int* __restrict__ start,
+// Trilinear deposit of particle.cpp:526-559 (sw == nullptr, concentration)
+// or 586-658 (weight RHS_T): raw per-cell sums for every ghosted cell before
+// the cumulative halo, in ascending particle order (the reference's scatter
+// order), from the rounded-index buckets.
+__global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ start,
                                                  const int* __restrict__ list,
+                                                 const double* __restrict__ sxl,
+                                                 const double* __restrict__ syl,
+                                                 const double* __restrict__ szl,
+                                                 const double* __restrict__ sw,
                                                  double* __restrict__ out) {
   const int a = blockIdx.x * 64 + threadIdx.x;
   const int b = blockIdx.y * 4 + threadIdx.y;
@@ -278,78 +324,57 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, PartPtrs p, int st,
   }
   double acc = 0.0;
   for (;;) {
-    int q;
-    const int i = m.next(list, q);
+    int q, pos;
+    m.next(list, q, pos);
     if (q < 0) break;
-    const double X = p.s[0][st][i], Y = p.s[1][st][i], Z = p.s[2][st][i];
-    const int ii = (int)(round(X * d.idx));
-    const int jj = (int)(round(Y * d.idy));
-    const int kk = (int)(round(Z * d.idz));
-    const double x_l = X * d.idx - ii + 0.5;
-    const double y_l = Y * d.idy - jj + 0.5;
-    const double z_l = Z * d.idz - kk + 0.5;
+    const double x_l = sxl[pos], y_l = syl[pos], z_l = szl[pos];
     const double wx = (q & 1) ? (1 - x_l) : x_l;
     const double wy = ((q >> 1) & 1) ? (1 - y_l) : y_l;
     const double wz = (q >> 2) ? (1 - z_l) : z_l;
     double t = wx * wy * wz;
-    if (wgt) t = t * wgt[i];
+    if (sw) t = t * sw[pos];
     acc = acc + t;
   }
   out[at(d, a, b, c)] = acc;
 }
 
-// 1-D linear momentum deposit (particle.cpp:498-524) of component M at the
-// _int positions: target (a,b,c) collects x_l*R from particles whose
-// truncating base is one cell below along M and (1-x_l)*R from particles
-// whose base is the cell itself.  Their rounded buckets lie in a 3x2x2 block.
+// 1-D linear momentum deposit (particle.cpp:498-524) from the truncating-
+// index buckets: component M of target t collects (1-x_l)*R from bucket t and
+// x_l*R from bucket t - e_M, in ascending particle order.
 template <int M>
-__device__ __forceinline__ double dep_vel_comp(const Dev& d, const PartPtrs& p,
-                                               const double* __restrict__ R,
-                                               const int* __restrict__ start,
-                                               const int* __restrict__ list, int a, int b,
-                                               int c) {
-  Merge<12> m;
-#pragma unroll
-  for (int q = 0; q < 12; q++) {
-    // offsets: axis M in {-1,0,1}, the other two in {0,1}
-    const int o3 = q % 3 - 1, o2a = (q / 3) & 1, o2b = (q / 6) & 1;
-    const int A = a + (M == 0 ? o3 : o2a);
-    const int B = b + (M == 1 ? o3 : (M == 0 ? o2a : o2b));
-    const int C = c + (M == 2 ? o3 : o2b);
-    if (A >= 0 && B >= 0 && C >= 0 && A < d.Nx && B < d.Ny && C < d.Nz) {
-      const long cell = at(d, A, B, C);
-      m.cur[q] = start[cell];
-      m.end[q] = start[cell + 1];
-    } else {
-      m.cur[q] = m.end[q] = 0;
-    }
+__device__ __forceinline__ double dep_vel_comp(const Dev& d, const int* __restrict__ start,
+                                               const int* __restrict__ list,
+                                               const double* __restrict__ l,
+                                               const double* __restrict__ R, long t,
+                                               int cm) {
+  const long e = M == 0 ? 1 : (M == 1 ? d.Nx : d.NxNy);
+  Merge<2> m;
+  m.cur[0] = start[t];
+  m.end[0] = start[t + 1];
+  if (cm > 0) {
+    m.cur[1] = start[t - e];
+    m.end[1] = start[t - e + 1];
+  } else {
+    m.cur[1] = m.end[1] = 0;
   }
   double acc = 0.0;
   for (;;) {
-    int q;
-    const int i = m.next(list, q);
+    int q, pos;
+    m.next(list, q, pos);
     if (q < 0) break;
-    const double X = p.s[0][1][i], Y = p.s[1][1][i], Z = p.s[2][1][i];
-    const int ti = (int)(X * d.idx) + 1, tj = (int)(Y * d.idy) + 1, tk = (int)(Z * d.idz) + 1;
-    const double pos = M == 0 ? X : (M == 1 ? Y : Z);
-    const double id = M == 0 ? d.idx : (M == 1 ? d.idy : d.idz);
-    const double x_l = (pos * id - ((int)(pos * id)));
-    const int tm = M == 0 ? ti : (M == 1 ? tj : tk);
-    const int t1 = M == 0 ? tj : ti, t2 = M == 2 ? tj : tk;
-    const int c1 = M == 0 ? b : a, c2 = M == 2 ? b : c;
-    const int cm = M == 0 ? a : (M == 1 ? b : c);
-    if (t1 != c1 || t2 != c2) continue;
-    if (tm == cm)
-      acc = acc + (1 - x_l) * R[i];
-    else if (tm == cm - 1)
-      acc = acc + x_l * R[i];
+    acc = acc + (q == 0 ? (1 - l[pos]) * R[pos] : l[pos] * R[pos]);
   }
   return acc;
 }
 
-__global__ void __launch_bounds__(256) k_dep_vel(Dev d, PartPtrs p,
-                                                 const int* __restrict__ start,
+__global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ start,
                                                  const int* __restrict__ list,
+                                                 const double* __restrict__ sxl,
+                                                 const double* __restrict__ syl,
+                                                 const double* __restrict__ szl,
+                                                 const double* __restrict__ sru,
+                                                 const double* __restrict__ srv,
+                                                 const double* __restrict__ srw,
                                                  double* __restrict__ ox, double* __restrict__ oy,
                                                  double* __restrict__ oz) {
   const int a = blockIdx.x * 64 + threadIdx.x;
@@ -357,9 +382,9 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, PartPtrs p,
   const int c = blockIdx.z;
   if (a >= d.Nx || b >= d.Ny) return;
   const long n = at(d, a, b, c);
-  ox[n] = dep_vel_comp<0>(d, p, p.rhs_u, start, list, a, b, c);
-  oy[n] = dep_vel_comp<1>(d, p, p.rhs_v, start, list, a, b, c);
-  oz[n] = dep_vel_comp<2>(d, p, p.rhs_w, start, list, a, b, c);
+  ox[n] = dep_vel_comp<0>(d, start, list, sxl, sru, n, a);
+  oy[n] = dep_vel_comp<1>(d, start, list, syl, srv, n, b);
+  oz[n] = dep_vel_comp<2>(d, start, list, szl, srw, n, c);
 }
 
 // ---- migration (one rank) -------------------------------------------------------
@@ -484,6 +509,23 @@ __global__ void k_mig_scatter(int np, int nleave, PartPtrs p, const int* __restr
 // ---- host side of the particle set ---------------------------------------------
 namespace fr {
 
+static int alloc_buckets(Buckets& B, long ncell, int cap, int nops) {
+  FR_CHECK(hipMalloc(&B.cnt, sizeof(int) * (ncell + 1)));
+  FR_CHECK(hipMalloc(&B.start, sizeof(int) * (ncell + 1)));
+  FR_CHECK(hipMalloc(&B.list, sizeof(int) * cap));
+  FR_CHECK(hipMalloc(&B.key, sizeof(int) * cap));
+  FR_CHECK(hipMalloc(&B.slot, sizeof(int) * cap));
+  for (int q = 0; q < nops; q++) FR_CHECK(hipMalloc(&B.s[q], sizeof(double) * cap));
+  return FR_OK;
+}
+
+static void free_buckets(Buckets& B) {
+  void* all[] = {B.cnt, B.start, B.list, B.key, B.slot};
+  for (void* p : all) (void)hipFree(p);
+  for (double* p : B.s)
+    if (p) (void)hipFree(p);
+}
+
 int PartDev::alloc(int capacity, const Dev& d) {
   cap = capacity < 1 ? 1 : capacity;
   ncell = d.size;
@@ -495,11 +537,9 @@ int PartDev::alloc(int capacity, const Dev& d) {
     FR_CHECK(hipMalloc(pp, sizeof(double) * cap));
     FR_CHECK(hipMemset(*pp, 0, sizeof(double) * cap));
   }
-  FR_CHECK(hipMalloc(&cnt, sizeof(int) * (ncell + 1)));
-  FR_CHECK(hipMalloc(&start, sizeof(int) * (ncell + 1)));
-  FR_CHECK(hipMalloc(&list, sizeof(int) * cap));
-  FR_CHECK(hipMalloc(&key, sizeof(int) * cap));
-  FR_CHECK(hipMalloc(&slot, sizeof(int) * cap));
+  int rc = alloc_buckets(R, ncell, cap, 4);
+  if (!rc) rc = alloc_buckets(T, ncell, cap, 6);
+  if (rc) return rc;
   const long big = ncell > cap ? ncell : cap;
   FR_CHECK(hipMalloc(&scan_tmp, sizeof(int) * (big / STILE + 2)));
   int** m[9] = {&flg, &nl, &pNL, &holepos, &tailnl, &evt, &islo, &plo, &dst};
@@ -512,11 +552,13 @@ int PartDev::alloc(int capacity, const Dev& d) {
 
 void PartDev::release() {
   for (int q = 0; q < 7; q++)
-    for (int t = 0; t < 4; t++) hipFree(s[q][t]);
-  void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, cnt, start, list, key, slot, scan_tmp,
-                 flg, nl, pNL, holepos, tailnl, evt, islo, plo, dst, stgf, stg};
-  for (void* p : all) hipFree(p);
-  hipHostFree(hbuf);
+    for (int t = 0; t < 4; t++) (void)hipFree(s[q][t]);
+  void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, scan_tmp, flg, nl, pNL, holepos, tailnl,
+                 evt, islo, plo, dst, stgf, stg};
+  for (void* p : all) (void)hipFree(p);
+  free_buckets(R);
+  free_buckets(T);
+  (void)hipHostFree(hbuf);
 }
 
 PartPtrs PartDev::ptrs() const {
@@ -549,31 +591,34 @@ int PartDev::temp(const Dev& d, int stage, double c0, const double* rho, double 
   return FR_OK;
 }
 
-int PartDev::build_cells(const Dev& d, int stage, hipStream_t st) {
-  FR_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * ncell, st));
-  if (np > 0) {
+int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st) {
+  Buckets& B = mode == 0 ? R : T;
+  FR_CHECK(hipMemsetAsync(B.cnt, 0, sizeof(int) * ncell, st));
+  if (np > 0)
     hipLaunchKernelGGL(k_bucket_count, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), stage,
-                       cnt, key, slot);
-  }
-  int rc = scan_ex(cnt, ncell, start, scan_tmp, st);
+                       mode, B.cnt, B.key, B.slot);
+  int rc = scan_ex(B.cnt, ncell, B.start, scan_tmp, st);
   if (rc) return rc;
   if (np > 0) {
-    hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, key, slot,
-                       start, list);
-    hipLaunchKernelGGL(k_bucket_sort, dim3(nblocks(ncell, 256)), dim3(256), 0, st, ncell, start,
-                       list);
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
+                       B.slot, B.start, B.list);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(nblocks(ncell, 256)), dim3(256), 0, st, ncell,
+                       B.start, B.list);
+    hipLaunchKernelGGL(k_bucket_gather, dim3(nblocks(np, 256)), dim3(256), 0, st, np, mode,
+                       ptrs(), stage, B.list, d.idx, d.idy, d.idz, B.s[0], B.s[1], B.s[2],
+                       B.s[3], B.s[4], B.s[5]);
   }
   FR_LAUNCH_CHECK();
-  cells_stage_valid = true;
+  B.valid = true;
   return FR_OK;
 }
 
 static dim3 grid3p(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 4), d.Nz); }
 
-int PartDev::deposit_cic(const Dev& d, int stage, const double* w, double* out, bool halo,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(k_dep_cic, grid3p(d), dim3(64, 4), 0, st, d, ptrs(), stage, w, start, list,
-                     out);
+int PartDev::deposit_cic(const Dev& d, bool weighted, double* out, bool halo, hipStream_t st) {
+  if (!R.valid) return FR_EINVAL;
+  hipLaunchKernelGGL(k_dep_cic, grid3p(d), dim3(64, 4), 0, st, d, R.start, R.list, R.s[0],
+                     R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out);
   FR_LAUNCH_CHECK();
   int rc = cum_fold(d, out, st);
   if (!rc && halo) rc = ghost_fill(d, out, st);
@@ -581,8 +626,9 @@ int PartDev::deposit_cic(const Dev& d, int stage, const double* w, double* out, 
 }
 
 int PartDev::deposit_vel(const Dev& d, double* const out[3], hipStream_t st) {
-  hipLaunchKernelGGL(k_dep_vel, grid3p(d), dim3(64, 4), 0, st, d, ptrs(), start, list, out[0],
-                     out[1], out[2]);
+  if (!T.valid) return FR_EINVAL;
+  hipLaunchKernelGGL(k_dep_vel, grid3p(d), dim3(64, 4), 0, st, d, T.start, T.list, T.s[0],
+                     T.s[1], T.s[2], T.s[3], T.s[4], T.s[5], out[0], out[1], out[2]);
   FR_LAUNCH_CHECK();
   for (int a = 0; a < 3; a++) {
     int rc = cum_fold(d, out[a], st);
@@ -619,7 +665,8 @@ int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], 
                        stgf, dst, (long)cap);
     FR_LAUNCH_CHECK();
   }
-  cells_stage_valid = false;
+  R.valid = false;
+  T.valid = false;
   return FR_OK;
 }
 

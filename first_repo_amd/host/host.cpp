@@ -265,11 +265,16 @@ int particle::advance(int k, double pre, double post, tensor1& U) {
   return dev.advance(size_->dev(), c, u, st_);
 }
 
+// The rounded-index buckets were built on the _new positions at the end of
+// the previous substep (or at initialisation), which are this substep's _int
+// positions: particle order and positions are unchanged since.
 int particle::part2gas_concentration(tensor0& A) {
-  return dev.deposit_cic(size_->dev(), 1, nullptr, A.pointer(), false, st_);
+  return dev.deposit_cic(size_->dev(), false, A.pointer(), false, st_);
 }
 
 int particle::part2gas_velocity(tensor1& A) {
+  int rc = dev.build(size_->dev(), 1, 1, st_);
+  if (rc) return rc;
   double* o[3] = {A.x.pointer(), A.y.pointer(), A.z.pointer()};
   return dev.deposit_vel(size_->dev(), o, st_);
 }
@@ -286,11 +291,13 @@ int particle::temp(int stage, double c0, const tensor0& Rho) {
   return dev.temp(size_->dev(), stage, c0, Rho.pointer(), iTth, st_);
 }
 
+// Requires rebuild_cells(stage) after the RHS_T of that stage was formed.
 int particle::part2gas_Temp(int stage, tensor0& A) {
-  return dev.deposit_cic(size_->dev(), stage, dev.rhs_T, A.pointer(), true, st_);
+  (void)stage;
+  return dev.deposit_cic(size_->dev(), true, A.pointer(), true, st_);
 }
 
-int particle::rebuild_cells(int stage) { return dev.build_cells(size_->dev(), stage, st_); }
+int particle::rebuild_cells(int stage) { return dev.build(size_->dev(), stage, 0, st_); }
 
 // ---------------------------------------------------------------- poisson
 int poisson::init(params* p, gridsize* g, hipStream_t st) {
