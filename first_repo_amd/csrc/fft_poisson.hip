@@ -15,6 +15,7 @@
 // pencil to spread the LDS banks, within a 64 KB budget (pencils + twiddles).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "fr_internal.hpp"
@@ -345,7 +346,13 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
       maxsh = 65536;
   if (maxsh > 163840) maxsh = 163840;
   if (maxsh < 65536) maxsh = 65536;
-  const int budget = maxsh / (int)sizeof(double2) - 160;  // room for the static reduction array
+  // Default budget 32 KB per workgroup (several workgroups per CU);
+  // FR_FFT_LDS_BYTES overrides it for tuning, capped by the device limit.
+  int want = 32768;
+  if (const char* e = getenv("FR_FFT_LDS_BYTES")) want = atoi(e);
+  if (want < 8192) want = 8192;
+  if (want > maxsh) want = maxsh;
+  const int budget = want / (int)sizeof(double2) - 160;  // room for the static reduction array
   s->q.wx = pencils(n[0], n[1] * n[2], budget);
   s->q.wy = pencils(n[1], n[0], budget);
   s->q.wz = pencils(n[2], n[0], budget);
