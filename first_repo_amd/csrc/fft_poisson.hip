@@ -2,17 +2,22 @@
 // (poisson.cpp:391-411) around the constant-coefficient spectral solve
 // poisson::CCP_FFT (poisson.cpp:413-462).
 //
-// The 3-D complex transform is done as LDS-resident batched 1-D transforms:
-//   pass X : residual (computed inline) -> FFT along x          (rows)
-//   pass Y : FFT along y                                          (strided)
+// The residual is real, so the x transform is real-to-complex and only the
+// nx/2+1 non-negative x frequencies are carried through the y and z passes:
+//   pass X : residual (computed inline) -> real FFT along x         (rows)
+//   pass Y : FFT along y                                             (strided)
 //   pass Z : FFT along z, divide by lambda(k), inverse FFT along z
 //   pass Y': inverse FFT along y
-//   pass X': inverse FFT along x, dp = -Re/N, P += dp, max|dp|
-// Each 1-D transform is an in-place radix-2 decimation-in-time FFT on
-// bit-reversed input with host-computed twiddles cos(2 pi k/n), sign*sin(..)
-// (the same algorithm the CPU oracle uses, so the two agree bit for bit).
-// A workgroup holds W pencils of length L in LDS, padded by one complex per
-// pencil to spread the LDS banks, within a 64 KB budget (pencils + twiddles).
+//   pass X': inverse real FFT along x, dp = -Re/N, P += dp, max|dp|
+// Division by the real, even lambda(k) keeps the spectrum Hermitian, so the
+// result equals the full complex transform of the reference's solve.
+//
+// 1-D transforms: in-place radix-2 decimation in time on bit-reversed input
+// with host twiddles (cos(2 pi k/n), sign*sin(2 pi k/n)); the real transform
+// of length N packs x_2m + i x_2m+1 into an N/2-point complex transform and
+// unpacks with the length-N twiddles.  The CPU oracle uses the same
+// algorithm and operation order, so the two agree bit for bit.  A workgroup
+// holds W pencils of length L in LDS, padded by one complex per pencil.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -22,18 +27,22 @@
 
 namespace fr {
 
-constexpr int FT = 256;        // threads per FFT workgroup
-// Pencils per workgroup for length L within `budget` complex elements:
-// W*(L+1) + 2*(L/2+1) <= budget, W a power of two, W <= cap.
-inline int pencils(int L, int cap, int budget) {
-  int w = (budget - 2 * (L / 2 + 1)) / (L + 1);
+constexpr int FT = 256;  // threads per FFT workgroup
+
+// Pencils per workgroup for length L within `budget` complex elements after
+// `extra` complex of twiddles: W*(L+1) + extra <= budget, W a power of two,
+// W <= cap.
+inline int pencils(int L, int cap, int budget, int extra) {
+  int w = (budget - extra) / (L + 1);
   if (w < 1) w = 1;
   int p2 = 1;
   while (p2 * 2 <= w) p2 *= 2;
   return p2 < cap ? p2 : cap;
 }
 
-__device__ __forceinline__ int bitrev(int x, int logn) { return __brev(x) >> (32 - logn); }
+__device__ __forceinline__ int bitrev(int x, int logn) {
+  return logn == 0 ? 0 : (int)(__brev((unsigned)x) >> (32 - logn));
+}
 
 __device__ __forceinline__ void bfly(double2& u, double2& t, const double2 w) {
   const double vr = t.x * w.x - t.y * w.y;
@@ -96,61 +105,86 @@ __device__ void load_tw(double2* tws, const double2* tw, int L) {
 
 struct PoisDev {
   Dev d;
-  int nx, ny, nz, lx, ly, lz;  // interior sizes and log2
-  int wx, wy, wz;              // pencils per workgroup for each pass
-  double tot;                  // nx*ny*nz
+  int nx, ny, nz;      // interior sizes
+  int nxh;             // nx/2 + 1 retained x frequencies
+  int lxm, ly, lz;     // log2(nx/2), log2(ny), log2(nz)
+  int wx, wy, wz;      // pencils per workgroup for each pass
+  double tot;          // nx*ny*nz
 };
 
-// Pass X: per row (j,k) the residual r = (div(C grad P) - RHS) * cc with
-// cc = -1/I_F2C(C) (poisson.cpp:393-403), forward FFT in x.
+// Residual r = (div(C grad P) - RHS) * cc, cc = -1/I_F2C(C)
+// (poisson.cpp:393-403), at interior cell c.
+__device__ __forceinline__ double residual(const Dev& d, const double* __restrict__ P,
+                                           const double* __restrict__ cx,
+                                           const double* __restrict__ cy,
+                                           const double* __restrict__ cz,
+                                           const double* __restrict__ RHS, long c) {
+  const long sx = 1, sy = d.Nx, sz = d.NxNy;
+  const double Cx0 = cx[c], Cx1 = cx[c + sx];
+  const double Cy0 = cy[c], Cy1 = cy[c + sy];
+  const double Cz0 = cz[c], Cz1 = cz[c + sz];
+  const double OneThird = 1. / 3.;
+  const double ccf = OneThird * ((Cx0 + Cx1) * 0.5 + (Cy0 + Cy1) * 0.5 + (Cz0 + Cz1) * 0.5);
+  const double cc = -1. / ccf;
+  const double pc = P[c];
+  const double D = (((P[c + sx] - pc) * d.idx) * Cx1 - ((pc - P[c - sx]) * d.idx) * Cx0) * d.idx +
+                   (((P[c + sy] - pc) * d.idy) * Cy1 - ((pc - P[c - sy]) * d.idy) * Cy0) * d.idy +
+                   (((P[c + sz] - pc) * d.idz) * Cz1 - ((pc - P[c - sz]) * d.idz) * Cz0) * d.idz;
+  return (D - RHS[c]) * cc;
+}
+
+// Pass X: per row (j,k) the residual, packed as z_m = r_2m + i r_2m+1, an
+// nx/2-point FFT, then the real-transform unpacking to X_k, k = 0..nx/2:
+//   E = (Z_k + conj Z_{M-k})/2, O = (Z_k - conj Z_{M-k})/(2i), X = E + W^k O.
 __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __restrict__ P,
                                                    const double* __restrict__ cx,
                                                    const double* __restrict__ cy,
                                                    const double* __restrict__ cz,
                                                    const double* __restrict__ RHS,
-                                                   const double2* __restrict__ tw,
+                                                   const double2* __restrict__ twM,
+                                                   const double2* __restrict__ twN,
                                                    double2* __restrict__ out) {
   extern __shared__ double2 sm[];
   const Dev& d = q.d;
-  const int L = q.nx, W = q.wx, ps = L + 1;
+  const int M = q.nx >> 1, W = q.wx, ps = M + 1;
   double2* buf = sm;
-  double2* tws = sm + W * ps;
-  load_tw(tws, tw, L);
+  double2* twm = sm + W * ps;
+  double2* twn = twm + (M / 2 + 1);
+  load_tw(twm, twM, M);
+  load_tw(twn, twN, q.nx);
   const long rows = (long)q.ny * q.nz;
   const long row0 = (long)blockIdx.x * W;
-  const long sx = 1, sy = d.Nx, sz = d.NxNy;
-  for (int t = threadIdx.x; t < W * L; t += FT) {
-    const int p = t / L, e = t - p * L;
+  for (int t = threadIdx.x; t < W * M; t += FT) {
+    const int p = t / M, m = t - p * M;
     const long row = row0 + p;
-    double r = 0.0;
+    double r0 = 0.0, r1 = 0.0;
     if (row < rows) {
       const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
-      const long c = at(d, e + 1, jj + 1, kk + 1);
-      const double Cx0 = cx[c], Cx1 = cx[c + sx];
-      const double Cy0 = cy[c], Cy1 = cy[c + sy];
-      const double Cz0 = cz[c], Cz1 = cz[c + sz];
-      const double OneThird = 1. / 3.;
-      const double ccf = OneThird * ((Cx0 + Cx1) * 0.5 + (Cy0 + Cy1) * 0.5 + (Cz0 + Cz1) * 0.5);
-      const double cc = -1. / ccf;
-      const double pc = P[c];
-      const double D = (((P[c + sx] - pc) * d.idx) * Cx1 - ((pc - P[c - sx]) * d.idx) * Cx0) * d.idx +
-                       (((P[c + sy] - pc) * d.idy) * Cy1 - ((pc - P[c - sy]) * d.idy) * Cy0) * d.idy +
-                       (((P[c + sz] - pc) * d.idz) * Cz1 - ((pc - P[c - sz]) * d.idz) * Cz0) * d.idz;
-      r = (D - RHS[c]) * cc;
+      const long c = at(d, 2 * m + 1, jj + 1, kk + 1);
+      r0 = residual(d, P, cx, cy, cz, RHS, c);
+      r1 = residual(d, P, cx, cy, cz, RHS, c + 1);
     }
-    buf[p * ps + bitrev(e, q.lx)] = make_double2(r, 0.0);
+    buf[p * ps + bitrev(m, q.lxm)] = make_double2(r0, r1);
   }
-  fft_lds(buf, L, q.lx, W, ps, tws);
-  for (int t = threadIdx.x; t < W * L; t += FT) {
-    const int p = t / L, e = t - p * L;
+  fft_lds(buf, M, q.lxm, W, ps, twm);
+  for (int t = threadIdx.x; t < W * (M + 1); t += FT) {
+    const int p = t / (M + 1), k = t - p * (M + 1);
     const long row = row0 + p;
-    if (row < rows) out[row * L + e] = buf[p * ps + e];
+    if (row >= rows) continue;
+    const double2 A = buf[p * ps + (k == M ? 0 : k)];
+    const double2 B = buf[p * ps + (k == 0 ? 0 : M - k)];
+    const double Er = 0.5 * (A.x + B.x), Ei = 0.5 * (A.y - B.y);
+    const double Dr = A.x - B.x, Di = A.y + B.y;
+    const double Or = 0.5 * Di, Oi = -0.5 * Dr;
+    const double2 w = twn[k];
+    const double Tr = Or * w.x - Oi * w.y, Ti = Or * w.y + Oi * w.x;
+    out[row * (M + 1) + k] = make_double2(Er + Tr, Ei + Ti);
   }
 }
 
 // Strided pass along y (axis 1) or z (axis 2) over a tile of W consecutive x
-// indices.  For axis 2 with `divide` set: forward FFT, divide by lambda except
-// the zero mode (poisson.cpp:438-451), then inverse FFT (tw2) in LDS.
+// frequencies.  For axis 2 with `divide` set: forward FFT, divide by lambda
+// except the zero mode (poisson.cpp:438-451), then inverse FFT (tw2) in LDS.
 __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int divide,
                                                      const double2* __restrict__ tw,
                                                      const double2* __restrict__ tw2,
@@ -163,19 +197,20 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
   const int lg = axis == 1 ? q.ly : q.lz;
   const int W = axis == 1 ? q.wy : q.wz;
   const int ps = L + 1;
+  const int nxh = q.nxh;
   double2* buf = sm;
   double2* tws = sm + W * ps;
   double2* tws2 = tws + (L / 2 + 1);
   load_tw(tws, tw, L);
   if (divide) load_tw(tws2, tw2, L);
-  const int tiles_x = (q.nx + W - 1) / W;
+  const int tiles_x = (nxh + W - 1) / W;
   const int i0 = (blockIdx.x % tiles_x) * W;
   const int other = blockIdx.x / tiles_x;  // k for axis 1, j for axis 2
-  const long stride = axis == 1 ? (long)q.nx : (long)q.nx * q.ny;
-  const long base = axis == 1 ? (long)other * q.nx * q.ny : (long)other * q.nx;
+  const long stride = axis == 1 ? (long)nxh : (long)nxh * q.ny;
+  const long base = axis == 1 ? (long)other * nxh * q.ny : (long)other * nxh;
   for (int t = threadIdx.x; t < W * L; t += FT) {
     const int e = t / W, p = t - e * W;
-    if (i0 + p < q.nx) buf[p * ps + bitrev(e, lg)] = data[base + e * stride + i0 + p];
+    if (i0 + p < nxh) buf[p * ps + bitrev(e, lg)] = data[base + e * stride + i0 + p];
   }
   fft_lds(buf, L, lg, W, ps, tws);
   if (divide) {
@@ -183,7 +218,7 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
     for (int t = threadIdx.x; t < W * L; t += FT) {
       const int e = t / W, p = t - e * W;
       const int ix = i0 + p, jy = other, kz = e;
-      if (ix < q.nx && ix + jy + kz > 0) {
+      if (ix < nxh && ix + jy + kz > 0) {
         const double lam = lamx[ix] + lamy[jy] + lamz[kz];
         double2 v = buf[p * ps + e];
         v.y /= lam;
@@ -206,42 +241,61 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
   }
   for (int t = threadIdx.x; t < W * L; t += FT) {
     const int e = t / W, p = t - e * W;
-    if (i0 + p < q.nx) data[base + e * stride + i0 + p] = buf[p * ps + e];
+    if (i0 + p < nxh) data[base + e * stride + i0 + p] = buf[p * ps + e];
   }
 }
 
-// Pass X': inverse FFT along x; dp = (-1)*Re/N (poisson.cpp:455-461);
-// P += dp at the interior; per-block max|dp| (tensor0::max_abs).
+// Pass X': inverse real FFT along x: Z_k = (X_k + conj X_{M-k}) +
+// i w^k (X_k - conj X_{M-k}), k < M, an nx/2-point inverse FFT, x_2m = Re z_m,
+// x_2m+1 = Im z_m; dp = (-1)*x/N (poisson.cpp:455-461); P += dp at the
+// interior; per-block max|dp| (tensor0::max_abs).
 __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q, const double2* __restrict__ in,
-                                                   const double2* __restrict__ tw,
+                                                   const double2* __restrict__ twM,
+                                                   const double2* __restrict__ twN,
                                                    double* __restrict__ P,
                                                    double* __restrict__ bmax) {
   extern __shared__ double2 sm[];
   __shared__ double red[FT];
   const Dev& d = q.d;
-  const int L = q.nx, W = q.wx, ps = L + 1;
+  const int M = q.nx >> 1, W = q.wx, ps = M + 1;
   double2* buf = sm;
-  double2* tws = sm + W * ps;
-  load_tw(tws, tw, L);
+  double2* twm = sm + W * ps;
+  double2* twn = twm + (M / 2 + 1);
+  load_tw(twm, twM, M);
+  load_tw(twn, twN, q.nx);
+  __syncthreads();
   const long rows = (long)q.ny * q.nz;
   const long row0 = (long)blockIdx.x * W;
-  for (int t = threadIdx.x; t < W * L; t += FT) {
-    const int p = t / L, e = t - p * L;
+  for (int t = threadIdx.x; t < W * M; t += FT) {
+    const int p = t / M, k = t - p * M;
     const long row = row0 + p;
-    buf[p * ps + bitrev(e, q.lx)] = row < rows ? in[row * L + e] : make_double2(0.0, 0.0);
+    double2 z = make_double2(0.0, 0.0);
+    if (row < rows) {
+      const double2 A = in[row * (M + 1) + k], B = in[row * (M + 1) + (M - k)];
+      const double Sr = A.x + B.x, Si = A.y - B.y;
+      const double Dr = A.x - B.x, Di = A.y + B.y;
+      const double2 w = twn[k];
+      const double Tr = Dr * w.x - Di * w.y, Ti = Dr * w.y + Di * w.x;
+      z = make_double2(Sr - Ti, Si + Tr);
+    }
+    buf[p * ps + bitrev(k, q.lxm)] = z;
   }
-  fft_lds(buf, L, q.lx, W, ps, tws);
+  fft_lds(buf, M, q.lxm, W, ps, twm);
   double m = 0.0;
-  for (int t = threadIdx.x; t < W * L; t += FT) {
-    const int p = t / L, e = t - p * L;
+  for (int t = threadIdx.x; t < W * M; t += FT) {
+    const int p = t / M, e = t - p * M;
     const long row = row0 + p;
     if (row < rows) {
       const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
-      const double dp = (-1) * buf[p * ps + e].x / q.tot;
-      const long c = at(d, e + 1, jj + 1, kk + 1);
-      P[c] = P[c] + dp;
-      const double a = dp >= 0 ? dp : -dp;
-      m = a > m ? a : m;
+      const double2 z = buf[p * ps + e];
+      const double dp0 = (-1) * z.x / q.tot;
+      const double dp1 = (-1) * z.y / q.tot;
+      const long c = at(d, 2 * e + 1, jj + 1, kk + 1);
+      P[c] = P[c] + dp0;
+      P[c + 1] = P[c + 1] + dp1;
+      const double a0 = dp0 >= 0 ? dp0 : -dp0, a1 = dp1 >= 0 ? dp1 : -dp1;
+      m = a0 > m ? a0 : m;
+      m = a1 > m ? a1 : m;
     }
   }
   red[threadIdx.x] = m;
@@ -273,20 +327,35 @@ using namespace fr;
 struct fr_box_poisson {
   fr_geom g;
   PoisDev q;
-  double2* data = nullptr;  // nx*ny*nz complex
-  double2 *twf[3] = {}, *twi[3] = {};
-  double *lam[3] = {};
+  double2* data = nullptr;  // (nx/2+1)*ny*nz complex
+  double2 *twf[3] = {}, *twi[3] = {};  // length nx, ny, nz
+  double2 *twfm = nullptr, *twim = nullptr;  // length nx/2
+  double* lam[3] = {};
   double* bmax = nullptr;
   double* dres = nullptr;
   double* hres = nullptr;
   int nbx = 0;
-  int lds_max = 0;
+  size_t lds[3] = {};  // dynamic LDS bytes of the x, y and z passes
 };
 
 static int ilog2(int n) {
   int l = 0;
   while ((1 << l) < n) l++;
   return (1 << l) == n ? l : -1;
+}
+
+// Twiddles exactly as the oracle forms them: a = 2.0*PI*k/n,
+// (cos a, sign*sin a) for k = 0..n/2.
+static int upload_tw(int n, int sign, double2** out) {
+  const double PI_ = 3.141592653589793;
+  std::vector<double2> t(n / 2 + 1);
+  for (int k = 0; k <= n / 2; k++) {
+    double ang = 2.0 * PI_ * (double)k / (double)n;
+    t[k] = make_double2(cos(ang), sign * sin(ang));
+  }
+  FR_CHECK(hipMalloc(out, sizeof(double2) * t.size()));
+  FR_CHECK(hipMemcpy(*out, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice));
+  return FR_OK;
 }
 
 extern "C" {
@@ -299,32 +368,26 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
     lg[a] = ilog2(n[a]);
     if (lg[a] < 1 || n[a] > 1024) return FR_EINVAL;
   }
+  if (n[0] < 4) return FR_EINVAL;
   fr_box_poisson* s = new fr_box_poisson();
   s->g = *g;
-  s->q.d = make_dev(g);
-  s->q.nx = n[0];
-  s->q.ny = n[1];
-  s->q.nz = n[2];
-  s->q.lx = lg[0];
-  s->q.ly = lg[1];
-  s->q.lz = lg[2];
-  s->q.tot = (double)((long)n[0] * n[1] * n[2]);
-  const double PI_ = 3.141592653589793;
+  PoisDev& q = s->q;
+  q.d = make_dev(g);
+  q.nx = n[0];
+  q.ny = n[1];
+  q.nz = n[2];
+  q.nxh = n[0] / 2 + 1;
+  q.lxm = lg[0] - 1;
+  q.ly = lg[1];
+  q.lz = lg[2];
+  q.tot = (double)((long)n[0] * n[1] * n[2]);
   const double Ls[3] = {g->Lx, g->Ly, g->Lz};
   const double ds[3] = {g->dx, g->dy, g->dz};
-  FR_CHECK(hipMalloc(&s->data, sizeof(double2) * (size_t)n[0] * n[1] * n[2]));
+  FR_CHECK(hipMalloc(&s->data, sizeof(double2) * (size_t)q.nxh * n[1] * n[2]));
   for (int a = 0; a < 3; a++) {
-    // twiddles exactly as the oracle: a = 2.0*PI*k/n, (cos a, sign*sin a)
-    std::vector<double2> f(n[a] / 2 + 1), iv(n[a] / 2 + 1);
-    for (int k = 0; k <= n[a] / 2; k++) {
-      double ang = 2.0 * PI_ * (double)k / (double)n[a];
-      f[k] = make_double2(cos(ang), -1 * sin(ang));
-      iv[k] = make_double2(cos(ang), 1 * sin(ang));
-    }
-    FR_CHECK(hipMalloc(&s->twf[a], sizeof(double2) * f.size()));
-    FR_CHECK(hipMalloc(&s->twi[a], sizeof(double2) * iv.size()));
-    FR_CHECK(hipMemcpy(s->twf[a], f.data(), sizeof(double2) * f.size(), hipMemcpyHostToDevice));
-    FR_CHECK(hipMemcpy(s->twi[a], iv.data(), sizeof(double2) * iv.size(), hipMemcpyHostToDevice));
+    int rc = upload_tw(n[a], -1, &s->twf[a]);
+    if (!rc) rc = upload_tw(n[a], 1, &s->twi[a]);
+    if (rc) return rc;
     // modified wavenumber term per axis (poisson.cpp:415-447):
     // (2/d^2)*(1-cos(kk*d)), kk = ((i<n/2)?i:i-n)*2pi/L
     double TWO_PI(2 * 3.141592653589793);
@@ -338,27 +401,32 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
     FR_CHECK(hipMalloc(&s->lam[a], sizeof(double) * n[a]));
     FR_CHECK(hipMemcpy(s->lam[a], lam.data(), sizeof(double) * n[a], hipMemcpyHostToDevice));
   }
-  // LDS budget: gfx950 has 160 KB per CU; use what the device grants a
-  // workgroup, at most 160 KB.
+  {
+    int rc = upload_tw(n[0] / 2, -1, &s->twfm);
+    if (!rc) rc = upload_tw(n[0] / 2, 1, &s->twim);
+    if (rc) return rc;
+  }
+  // LDS budget per workgroup: 32 KB by default (several workgroups per CU);
+  // FR_FFT_LDS_BYTES overrides it for tuning, capped by the device limit.
   int dev = 0, maxsh = 65536;
   if (hipGetDevice(&dev) == hipSuccess)
     if (hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       maxsh = 65536;
   if (maxsh > 163840) maxsh = 163840;
   if (maxsh < 65536) maxsh = 65536;
-  // Default budget 32 KB per workgroup (several workgroups per CU);
-  // FR_FFT_LDS_BYTES overrides it for tuning, capped by the device limit.
   int want = 32768;
   if (const char* e = getenv("FR_FFT_LDS_BYTES")) want = atoi(e);
   if (want < 8192) want = 8192;
   if (want > maxsh) want = maxsh;
   const int budget = want / (int)sizeof(double2) - 160;  // room for the static reduction array
-  s->q.wx = pencils(n[0], n[1] * n[2], budget);
-  s->q.wy = pencils(n[1], n[0], budget);
-  s->q.wz = pencils(n[2], n[0], budget);
-  const size_t need = sizeof(double2) * (size_t)std::max(
-      {s->q.wx * (n[0] + 1) + n[0] / 2 + 1, s->q.wy * (n[1] + 1) + 2 * (n[1] / 2 + 1),
-       s->q.wz * (n[2] + 1) + 2 * (n[2] / 2 + 1)});
+  const int M = n[0] / 2;
+  q.wx = pencils(M, n[1] * n[2], budget, (M / 2 + 1) + (n[0] / 2 + 1));
+  q.wy = pencils(n[1], q.nxh, budget, 2 * (n[1] / 2 + 1));
+  q.wz = pencils(n[2], q.nxh, budget, 2 * (n[2] / 2 + 1));
+  s->lds[0] = sizeof(double2) * (q.wx * (M + 1) + (M / 2 + 1) + (n[0] / 2 + 1));
+  s->lds[1] = sizeof(double2) * (q.wy * (n[1] + 1) + 2 * (n[1] / 2 + 1));
+  s->lds[2] = sizeof(double2) * (q.wz * (n[2] + 1) + 2 * (n[2] / 2 + 1));
+  const size_t need = std::max({s->lds[0], s->lds[1], s->lds[2]});
   if (need > 65536) {
     FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
@@ -367,9 +435,7 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
     FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_inv,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
   }
-  s->lds_max = (int)need;
-  const int W = s->q.wx;
-  s->nbx = (int)(((long)n[1] * n[2] + W - 1) / W);
+  s->nbx = (int)(((long)n[1] * n[2] + q.wx - 1) / q.wx);
   FR_CHECK(hipMalloc(&s->bmax, sizeof(double) * (s->nbx + 1)));
   FR_CHECK(hipMalloc(&s->dres, sizeof(double) * 2));
   FR_CHECK(hipHostMalloc(&s->hres, sizeof(double) * 2, hipHostMallocDefault));
@@ -379,50 +445,36 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
 
 int fr_box_poisson_destroy(fr_box_poisson* s) {
   if (!s) return FR_OK;
-  hipFree(s->data);
+  (void)hipFree(s->data);
   for (int a = 0; a < 3; a++) {
-    hipFree(s->twf[a]);
-    hipFree(s->twi[a]);
-    hipFree(s->lam[a]);
+    (void)hipFree(s->twf[a]);
+    (void)hipFree(s->twi[a]);
+    (void)hipFree(s->lam[a]);
   }
-  hipFree(s->bmax);
-  hipFree(s->dres);
-  hipHostFree(s->hres);
+  (void)hipFree(s->twfm);
+  (void)hipFree(s->twim);
+  (void)hipFree(s->bmax);
+  (void)hipFree(s->dres);
+  (void)hipHostFree(s->hres);
   delete s;
   return FR_OK;
 }
 
 // One defect-correction iteration; leaves max|dp| in s->dres[0].
 static int box_poisson_iter(fr_box_poisson* s, const double* cx, const double* cy,
-                            const double* cz, double* P, const double* RHS,
-                            hipStream_t st) {
+                            const double* cz, double* P, const double* RHS, hipStream_t st) {
   PoisDev& q = s->q;
-  const size_t lds_x = sizeof(double2) * (q.wx * (q.nx + 1) + q.nx / 2 + 1);
-  hipLaunchKernelGGL(k_pass_x_fwd, dim3(s->nbx), dim3(FT), lds_x, st, q, P, cx, cy, cz,
-                     RHS, s->twf[0], s->data);
-  {
-    const int L = q.ny, W = q.wy;
-    const size_t lds = sizeof(double2) * (W * (L + 1) + L / 2 + 1);
-    const int tiles = (q.nx + W - 1) / W;
-    hipLaunchKernelGGL(k_pass_strided, dim3(tiles * q.nz), dim3(FT), lds, st, q, 1, 0,
-                       s->twf[1], s->twf[1], s->lam[0], s->lam[1], s->lam[2], s->data);
-  }
-  {
-    const int L = q.nz, W = q.wz;
-    const size_t lds = sizeof(double2) * (W * (L + 1) + 2 * (L / 2 + 1));
-    const int tiles = (q.nx + W - 1) / W;
-    hipLaunchKernelGGL(k_pass_strided, dim3(tiles * q.ny), dim3(FT), lds, st, q, 2, 1,
-                       s->twf[2], s->twi[2], s->lam[0], s->lam[1], s->lam[2], s->data);
-  }
-  {
-    const int L = q.ny, W = q.wy;
-    const size_t lds = sizeof(double2) * (W * (L + 1) + L / 2 + 1);
-    const int tiles = (q.nx + W - 1) / W;
-    hipLaunchKernelGGL(k_pass_strided, dim3(tiles * q.nz), dim3(FT), lds, st, q, 1, 0,
-                       s->twi[1], s->twi[1], s->lam[0], s->lam[1], s->lam[2], s->data);
-  }
-  hipLaunchKernelGGL(k_pass_x_inv, dim3(s->nbx), dim3(FT), lds_x, st, q, s->data, s->twi[0],
-                     P, s->bmax);
+  hipLaunchKernelGGL(k_pass_x_fwd, dim3(s->nbx), dim3(FT), s->lds[0], st, q, P, cx, cy, cz, RHS,
+                     s->twfm, s->twf[0], s->data);
+  const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.nxh + q.wz - 1) / q.wz;
+  hipLaunchKernelGGL(k_pass_strided, dim3(ty * q.nz), dim3(FT), s->lds[1], st, q, 1, 0,
+                     s->twf[1], s->twf[1], s->lam[0], s->lam[1], s->lam[2], s->data);
+  hipLaunchKernelGGL(k_pass_strided, dim3(tz * q.ny), dim3(FT), s->lds[2], st, q, 2, 1,
+                     s->twf[2], s->twi[2], s->lam[0], s->lam[1], s->lam[2], s->data);
+  hipLaunchKernelGGL(k_pass_strided, dim3(ty * q.nz), dim3(FT), s->lds[1], st, q, 1, 0,
+                     s->twi[1], s->twi[1], s->lam[0], s->lam[1], s->lam[2], s->data);
+  hipLaunchKernelGGL(k_pass_x_inv, dim3(s->nbx), dim3(FT), s->lds[0], st, q, s->data, s->twim,
+                     s->twi[0], P, s->bmax);
   hipLaunchKernelGGL(k_max_final, dim3(1), dim3(FT), 0, st, s->bmax, s->nbx, s->dres);
   FR_LAUNCH_CHECK();
   return ghost_fill(q.d, P, st);

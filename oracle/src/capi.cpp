@@ -148,6 +148,10 @@ void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign) {
 void orc_fft2d(int ny, int nz, double* re, double* im, int sign) {
   fft2d(re, im, ny, nz, sign);
 }
+void orc_rfft1d(const double* x, int n, double* Xre, double* Xim) { rfft1d(x, n, Xre, Xim); }
+void orc_irfft1d(const double* Xre, const double* Xim, int n, double* x) {
+  irfft1d(Xre, Xim, n, x);
+}
 // Box (is_box=1, poisson.cpp:391-462) or duct (is_box=0, poisson_IO.cpp:
 // 81-452) defect-correction solve of div(C grad P) = RHS.  Returns the
 // iteration count.

@@ -119,6 +119,53 @@ void fft3d(std::vector<double>& re, std::vector<double>& im, int nx, int ny,
   }
 }
 
+// Real-input transform by the packing method (e.g. Press et al., Numerical
+// Recipes 12.3): with z_m = x_2m + i x_2m+1 and Z its M-point DFT (M = n/2),
+//   E_k = (Z_k + conj Z_{M-k})/2,  O_k = (Z_k - conj Z_{M-k})/(2i),
+//   X_k = E_k + W^k O_k,  W = exp(-2 pi i/n),  k = 0..M  (Z_M = Z_0).
+void rfft1d(const double* x, int n, double* Xre, double* Xim) {
+  const int M = n / 2;
+  std::vector<double> zr(M), zi(M), c, s;
+  for (int m = 0; m < M; m++) {
+    zr[m] = x[2 * m];
+    zi[m] = x[2 * m + 1];
+  }
+  fft1d(zr.data(), zi.data(), M, 1, -1);
+  twiddles(n, -1, c, s);
+  for (int k = 0; k <= M; k++) {
+    const int a = (k == M) ? 0 : k, b = (k == 0) ? 0 : M - k;
+    const double Ar = zr[a], Ai = zi[a], Br = zr[b], Bi = zi[b];
+    const double Er = 0.5 * (Ar + Br), Ei = 0.5 * (Ai - Bi);
+    const double Dr = Ar - Br, Di = Ai + Bi;
+    const double Or = 0.5 * Di, Oi = -0.5 * Dr;
+    const double Tr = Or * c[k] - Oi * s[k], Ti = Or * s[k] + Oi * c[k];
+    Xre[k] = Er + Tr;
+    Xim[k] = Ei + Ti;
+  }
+}
+
+// Inverse: Z_k = (X_k + conj X_{M-k}) + i V^k (X_k - conj X_{M-k}),
+// V = exp(+2 pi i/n), k < M; z = unnormalised M-point inverse DFT of Z;
+// x_2m = Re z_m, x_2m+1 = Im z_m.  Returns n times the input of rfft1d.
+void irfft1d(const double* Xre, const double* Xim, int n, double* x) {
+  const int M = n / 2;
+  std::vector<double> zr(M), zi(M), c, s;
+  twiddles(n, +1, c, s);
+  for (int k = 0; k < M; k++) {
+    const double Ar = Xre[k], Ai = Xim[k], Br = Xre[M - k], Bi = Xim[M - k];
+    const double Sr = Ar + Br, Si = Ai - Bi;
+    const double Dr = Ar - Br, Di = Ai + Bi;
+    const double Tr = Dr * c[k] - Di * s[k], Ti = Dr * s[k] + Di * c[k];
+    zr[k] = Sr - Ti;
+    zi[k] = Si + Tr;
+  }
+  fft1d(zr.data(), zi.data(), M, 1, +1);
+  for (int m = 0; m < M; m++) {
+    x[2 * m] = zr[m];
+    x[2 * m + 1] = zi[m];
+  }
+}
+
 // 2-D transform of an [nz][ny] plane (y fastest).
 void fft2d(double* re, double* im, int ny, int nz, int sign) {
   for (int k = 0; k < nz; k++) fft1d(re + (long)k * ny, im + (long)k * ny, ny, 1, sign);

@@ -272,6 +272,11 @@ void fft1d(double* re, double* im, int n, long stride, int sign);
 void fft3d(std::vector<double>& re, std::vector<double>& im, int nx, int ny,
            int nz, int sign);
 void fft2d(double* re, double* im, int ny, int nz, int sign);
+// Real transform of n (even, >= 4) contiguous reals: X_k, k = 0..n/2, of the
+// sign -1 DFT, via an n/2-point complex transform of z_m = x_2m + i x_2m+1.
+void rfft1d(const double* x, int n, double* Xre, double* Xim);
+// Inverse of rfft1d (sign +1, unnormalised: returns n * x) from X_0..X_n/2.
+void irfft1d(const double* Xre, const double* Xim, int n, double* x);
 
 // ---------------------------------------------------------------- particles
 struct Particles {  // particle.h:12-155 (one rank)
@@ -328,6 +333,7 @@ struct BoxPoisson {  // poisson.h:19-81, FFT branch (Solver=0)
   void init(const Params* p, const Geom* gg);
   void Solve(const F1& C, F0& P, const F0& RHS);  // poisson.cpp:115-117,391-411
   void CCP_FFT(const F0& RHS, F0& P);             // poisson.cpp:413-462
+  void CCP_FFT_complex(const F0& RHS, F0& P);     // full complex, odd Nx
 };
 
 struct DuctPoisson {  // poisson_IO.h:20-88

@@ -34,8 +34,74 @@ void BoxPoisson::Solve(const F1& C, F0& P, const F0& RHS) {
   num_iteration = i;
 }
 
-// poisson.cpp:413-462
+// poisson.cpp:413-462.  The residual is real and the divisor is real and
+// even in every wavenumber, so the solve runs on the nx/2+1 non-negative x
+// frequencies: real transform along x, complex along y and z, divide, complex
+// inverse along z and y, real inverse along x.  This is the transform the
+// reference's full complex FFT computes (its imaginary part is zero to
+// round-off and is discarded at poisson.cpp:461); odd Nx falls back to the
+// full complex transform.
 void BoxPoisson::CCP_FFT(const F0& RHS, F0& P) {
+  const int nx = g->Nx_tot, ny = g->Ny_tot, nz = g->Nz_tot, bs = g->bs;
+  if (nx % 2 != 0 || nx < 4) {
+    CCP_FFT_complex(RHS, P);
+    return;
+  }
+  const int tot = g->size_tot();
+  const int nh = nx / 2 + 1;
+  const long sy = nh, sz = (long)nh * ny;
+  double TWO_PI(2 * 3.141592653589793);
+  double Two_PI_Over_Lx = TWO_PI / g->Lx;
+  double Two_PI_Over_Ly = TWO_PI / g->Ly;
+  double Two_PI_Over_Lz = TWO_PI / g->Lz;
+  re.assign((size_t)nh * ny * nz, 0.0);
+  im.assign((size_t)nh * ny * nz, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; k++) {
+    std::vector<double> row(nx);
+    for (int j = 0; j < ny; j++) {
+      for (int i = 0; i < nx; i++) row[i] = RHS(i + bs, j + bs, k + bs);
+      rfft1d(row.data(), nx, &re[k * sz + j * sy], &im[k * sz + j * sy]);
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int i = 0; i < nh; i++) fft1d(&re[k * sz + i], &im[k * sz + i], ny, sy, -1);
+  double dx = g->dx, dy = g->dy, dz = g->dz;
+  double dx2 = 2. / (dx * dx), dy2 = 2. / (dy * dy), dz2 = 2. / (dz * dz);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nh; i++) {
+      double* r = &re[j * sy + i];
+      double* m = &im[j * sy + i];
+      fft1d(r, m, nz, sz, -1);
+      for (int k = 0; k < nz; k++)
+        if (i + j + k > 0) {
+          double ii = ((i < nx / 2) ? i : i - nx) * Two_PI_Over_Lx;
+          double jj = ((j < ny / 2) ? j : j - ny) * Two_PI_Over_Ly;
+          double kk = ((k < nz / 2) ? k : k - nz) * Two_PI_Over_Lz;
+          m[k * sz] /= ((dx2) * (1 - cos(ii * dx)) + (dy2) * (1 - cos(jj * dy)) +
+                        (dz2) * (1 - cos(kk * dz)));
+          r[k * sz] /= ((dx2) * (1 - cos(ii * dx)) + (dy2) * (1 - cos(jj * dy)) +
+                        (dz2) * (1 - cos(kk * dz)));
+        }
+      fft1d(r, m, nz, sz, +1);
+    }
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int i = 0; i < nh; i++) fft1d(&re[k * sz + i], &im[k * sz + i], ny, sy, +1);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; k++) {
+    std::vector<double> row(nx);
+    for (int j = 0; j < ny; j++) {
+      irfft1d(&re[k * sz + j * sy], &im[k * sz + j * sy], nx, row.data());
+      for (int i = 0; i < nx; i++) P(i + bs, j + bs, k + bs) = (-1) * row[i] / (tot);
+    }
+  }
+}
+
+// poisson.cpp:413-462 with the full complex 3-D transform.
+void BoxPoisson::CCP_FFT_complex(const F0& RHS, F0& P) {
   const int nx = g->Nx_tot, ny = g->Ny_tot, nz = g->Nz_tot, bs = g->bs;
   const int tot = g->size_tot();
   double TWO_PI(2 * 3.141592653589793);

@@ -64,6 +64,8 @@ def orc():
         L.orc_reducex.argtypes = [G, dp, dp]
         L.orc_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, C.c_int]
         L.orc_fft2d.argtypes = [C.c_int, C.c_int, dp, dp, C.c_int]
+        L.orc_rfft1d.argtypes = [dp, C.c_int, dp, dp]
+        L.orc_irfft1d.argtypes = [dp, dp, C.c_int, dp]
         L.orc_poisson.argtypes = [G, dp, dp, dp, dp, dp, C.c_int, C.c_double]
         L.orc_poisson.restype = C.c_int
         L.orc_glibc_rand.argtypes = [C.c_int, ip]

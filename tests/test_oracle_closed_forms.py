@@ -37,6 +37,28 @@ def test_fft_matches_brute_force_dft():
         assert np.abs(back - x).max() <= 1e-13 * np.abs(x).max()
 
 
+@pytest.mark.parametrize("n", [4, 8, 64, 256])
+def test_real_fft_matches_numpy(n):
+    """rfft1d equals numpy.fft.rfft; irfft1d(rfft1d(x)) = n x."""
+    O = OL.orc()
+    x = np.random.default_rng(n).standard_normal(n)
+    Xr, Xi = np.zeros(n // 2 + 1), np.zeros(n // 2 + 1)
+    O.orc_rfft1d(x, n, Xr, Xi)
+    ref = np.fft.rfft(x)
+    assert np.abs(Xr + 1j * Xi - ref).max() <= 1e-13 * np.abs(ref).max()
+    # Hermitian ends are real
+    assert Xi[0] == 0.0 and abs(Xi[-1]) <= 1e-14 * np.abs(ref).max()
+    back = np.zeros(n)
+    O.orc_irfft1d(Xr, Xi, n, back)
+    assert np.abs(back / n - x).max() <= 1e-14 * np.abs(x).max() * np.log2(n)
+    # the inverse of an arbitrary half spectrum is numpy's irfft
+    Yr, Yi = np.random.default_rng(n + 1).standard_normal((2, n // 2 + 1))
+    Yi[0] = Yi[-1] = 0.0
+    O.orc_irfft1d(Yr, Yi, n, back)
+    ref = np.fft.irfft(Yr + 1j * Yi, n) * n
+    assert np.abs(back - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
 def _geom(n, box, L=(2 * np.pi,) * 3):
     return OL.make_geom(*n, L=L, box=box)
 
@@ -87,11 +109,13 @@ def test_box_poisson_manufactured():
     assert np.abs(d - d.mean()).max() <= 1e-10
 
 
-def test_box_poisson_constant_coefficient_exact_in_one_iteration():
+@pytest.mark.parametrize("n", [(16, 8, 32), (15, 6, 10)])
+def test_box_poisson_constant_coefficient_exact_in_one_iteration(n):
     """With C = 1 the spectral solve is the exact inverse of the discrete
-    Laplacian; the zero mode passes through as -mean(r) (poisson.cpp:442)."""
+    Laplacian; the zero mode passes through as -mean(r) (poisson.cpp:442).
+    Even Nx runs the real-to-complex solve, odd Nx the full complex one."""
     O = OL.orc()
-    g = _geom((16, 8, 32), True, L=(1.0, 2.0, 3.0))
+    g = _geom(n, True, L=(1.0, 2.0, 3.0))
     rng = np.random.default_rng(4)
     rhs = rng.standard_normal(OL.shape(g))
     inner = (slice(1, -1),) * 3
