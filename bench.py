@@ -24,6 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = ("Mcell\u00b7steps/s full RK4 step w/ particles; "
+          "Poisson iters+ms to 1e-10 vs plain")
 
 
 def cpu_baseline(params, d, threads):
@@ -85,6 +87,8 @@ def main():
     ms_per_step = wall / args.steps * 1e3
     value = ncell * args.steps / wall / 1e6
     diag = sim.diagnostics()
+    n_it = iters / args.steps / 4
+    step_bytes = ncell * (4 * 416 + 4 * 96 * n_it) + args.particles * 4 * 288
     # dominant stencil kernel: the fused momentum update (grid::Update_RU_WOP);
     # algorithmic bytes per launch = 21 doubles per ghosted cell (reads U, RU_int,
     # S, RU, RU_np1: 15; writes RU_np1, RU_new: 6)
@@ -94,9 +98,9 @@ def main():
     per_launch_ms = kern_ms / max(kern_n, 1)
     achieved = alg_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
     out = {
-        "metric": "Mcell*steps/s full RK4 step w/ particles",
+        "metric": METRIC,
         "value": round(value, 3),
-        "unit": "Mcell*steps/s",
+        "unit": "Mcell\u00b7steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -113,6 +117,10 @@ def main():
                    "poisson_iters_per_substep": round(iters / args.steps / 4, 3),
                    "poisson_ms_per_step": round(t_poisson / args.steps, 3),
                    "particle_ms_per_step": round(t_part / args.steps, 3),
+                   # whole-step algorithmic bytes: 416 B/cell/substep of stencils,
+                   # 96 B/cell per Poisson iteration, 288 B/particle/substep
+                   "step_alg_GB": round(step_bytes / 1e9, 2),
+                   "step_hbm_frac": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
                    "TKE": float(diag[0]), "Rho_Mean": float(diag[1]),
                    "mean_particle_T": float(diag[8])},
         "roofline": {"kernel": "k_update_ru_wop", "bound": "hbm",
@@ -126,7 +134,7 @@ def main():
         threads = min(os.cpu_count() or 1, 16)
         print(f"bench: CPU oracle sample, 1 step, {threads} threads", file=sys.stderr, flush=True)
         cw = cpu_baseline(params, tmp, threads)
-        out["cpu_baseline"] = {"value": round(ncell / cw / 1e6, 4), "unit": "Mcell*steps/s",
+        out["cpu_baseline"] = {"value": round(ncell / cw / 1e6, 4), "unit": "Mcell\u00b7steps/s",
                                "cores": threads, "kind": "port",
                                "sample": f"oracle, 1 RK4 step of the same {args.n}^3 / "
                                          f"{args.particles}-particle workload after "

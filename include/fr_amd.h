@@ -103,7 +103,8 @@ int fr_t0_reducex(const fr_geom* g, const double* f, double* out,
  * Solves div(C grad P) = RHS with P as the initial guess (poisson.h:79); C
  * is the face coefficient (1/rho_face in grid.cpp:641).  Stops after
  * `iterations` or when max|dp| <= epsilon; *num_iter receives the iteration
- * count (poisson.h:80).  Interior sizes must be powers of two in [2, 1024].
+ * count (poisson.h:80).  Interior sizes must be powers of two in [2, 1024],
+ * with Nx_tot >= 4 (the x transform is real-to-complex).
  * P's ghosts must hold its periodic images on entry; they do on exit.
  * The solver object holds its FFT workspace. */
 typedef struct fr_box_poisson fr_box_poisson;

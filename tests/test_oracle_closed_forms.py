@@ -245,6 +245,61 @@ def test_rk4_drag_amplification(small_grid):
         assert np.abs(got - (base - g0) * R).max() <= 1e-13
 
 
+def _shear_case(d, n=(8, 16, 4), amp=0.8, A=0.3, mu=0.05, dt=0.01):
+    """u = amp*sin(y) on x-faces, v = w = 0, rho = 1, no particles."""
+    from first_repo_amd.configs import Params
+    nx, ny, nz = n
+    L = 2 * math.pi
+    y = (np.arange(ny) + 0.5) * (L / ny)
+    U = np.broadcast_to((amp * np.sin(y))[None, :, None], (nz, ny, nx)).copy()
+    Z = np.zeros((nz, ny, nx))
+    for name, a in (("U.bin", U), ("V.bin", Z), ("W.bin", Z)):
+        a.astype("<f8").tofile(f"{d}/{name}")
+    p = Params(Nx_tot=nx, Ny_tot=ny, Nz_tot=nz, dt=dt, A1=A, A2=A, Mu0=mu, np0=0.0,
+               TWC=1, Cooling=0, epsilon=1e-10)
+    p.write(f"{d}/params.in")
+    return U, L / ny
+
+
+def test_shear_wave_rk4_decay(tmp_path):
+    """Single Fourier mode u_x = sin(y): convection and pressure vanish
+    exactly, so one full step (grid.cpp:327-712, all four substeps and the
+    projection) multiplies RU_x by the RK4 polynomial R(z), z = dt*(A +
+    Mu0*lambda_y), lambda_y = -(2/dy^2)(1 - cos dy): pins the viscous and
+    forcing terms, their signs, the RK4 weights and Correct_mean_RU."""
+    d = str(tmp_path)
+    A, mu, dt = 0.3, 0.05, 0.01
+    U, dy = _shear_case(d, A=A, mu=mu, dt=dt)
+    og = OL.OracleGrid(d + "/params.in", d)
+    og.initialize()
+    assert og.np_ == 0
+    inner = (slice(1, -1),) * 3
+    ru0 = og.field("RU", 0)[inner].copy()
+    assert np.abs(ru0 - U).max() <= 1e-15
+    og.step()
+    lam = -(2 / dy ** 2) * (1 - math.cos(dy))
+    z = dt * (A + mu * lam)
+    R = 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+    ru1 = og.field("RU", 0)[inner]
+    assert np.abs(ru1 - R * U).max() <= 1e-14
+    assert np.abs(og.field("RU", 1)[inner]).max() == 0.0
+    assert np.abs(og.field("Rho")[inner] - 1.0).max() == 0.0
+    assert np.abs(og.field("P")[inner]).max() == 0.0
+
+
+def test_mass_conserved_over_step(small_grid):
+    """Periodic flux divergence: the interior sum of Rho is unchanged by a
+    full two-way coupled step (Update_Rho, grid.cpp:327-333)."""
+    og = OL.OracleGrid(small_grid + "/params.in", small_grid)
+    og.initialize()
+    inner = (slice(1, -1),) * 3
+    m0 = og.field("Rho")[inner].sum()
+    og.step()
+    og.step()
+    m1 = og.field("Rho")[inner].sum()
+    assert abs(m1 - m0) <= 1e-13 * abs(m0)
+
+
 def test_deposit_conservation(small_grid):
     """CIC weights of every particle sum to one: the concentration field sums
     to Np and the heat deposit sums to sum(RHS_T) (after the cumulative halo)."""
