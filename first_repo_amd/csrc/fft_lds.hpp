@@ -1,0 +1,147 @@
+// LDS radix-2 FFT building blocks shared by the box and duct pressure
+// solves (gfx950, FP64).  A workgroup of FT threads holds W pencils of
+// length L in LDS (stride L+1 complex) and transforms them in place.
+#pragma once
+#include <cmath>
+#include <vector>
+
+#include "fr_internal.hpp"
+
+namespace fr {
+
+constexpr int FT = 256;  // threads per FFT workgroup
+
+// Pencils per workgroup for length L within `budget` complex elements after
+// `extra` complex of twiddles: W*(L+1) + extra <= budget, W a power of two,
+// W <= cap.
+inline int pencils(int L, int cap, int budget, int extra) {
+  int w = (budget - extra) / (L + 1);
+  if (w < 1) w = 1;
+  int p2 = 1;
+  while (p2 * 2 <= w) p2 *= 2;
+  return p2 < cap ? p2 : cap;
+}
+
+__device__ __forceinline__ int bitrev(int x, int logn) {
+  return logn == 0 ? 0 : (int)(__brev((unsigned)x) >> (32 - logn));
+}
+
+__device__ __forceinline__ void bfly(double2& u, double2& t, const double2 w) {
+  const double vr = t.x * w.x - t.y * w.y;
+  const double vi = t.x * w.y + t.y * w.x;
+  const double2 a = u;
+  u = make_double2(a.x + vr, a.y + vi);
+  t = make_double2(a.x - vr, a.y - vi);
+}
+
+// In-place radix-2 DIT over W pencils (stride ps complex) already in
+// bit-reversed order.  tw[k] = (cos(2 pi k/L), sign*sin(2 pi k/L)), k <= L/2.
+// Stages are applied in pairs from registers (each thread owns the four
+// elements two consecutive stages combine), with exactly the butterflies and
+// twiddles of the stage-by-stage algorithm, so the result is bit-identical.
+__device__ inline void fft_lds(double2* buf, int L, int lg, int W, int ps, const double2* tw) {
+  int half = 1, s = 0;
+  const int q4 = L >> 2;
+  for (; s + 1 < lg; s += 2, half <<= 2) {
+    __syncthreads();
+    const int stepA = L / (2 * half), stepB = L / (4 * half);
+    for (int b = threadIdx.x; b < W * q4; b += FT) {
+      const int p = b / q4;
+      const int bb = b - p * q4;
+      const int grp = bb / half;
+      const int pos = bb - grp * half;
+      const int i0 = p * ps + grp * 4 * half + pos;
+      double2 x0 = buf[i0], x1 = buf[i0 + half], x2 = buf[i0 + 2 * half], x3 = buf[i0 + 3 * half];
+      const double2 wa = tw[pos * stepA];
+      bfly(x0, x1, wa);
+      bfly(x2, x3, wa);
+      bfly(x0, x2, tw[pos * stepB]);
+      bfly(x1, x3, tw[(pos + half) * stepB]);
+      buf[i0] = x0;
+      buf[i0 + half] = x1;
+      buf[i0 + 2 * half] = x2;
+      buf[i0 + 3 * half] = x3;
+    }
+  }
+  if (s < lg) {  // odd number of stages: the last one alone
+    __syncthreads();
+    const int step = L / (2 * half), h2 = L >> 1;
+    for (int b = threadIdx.x; b < W * h2; b += FT) {
+      const int p = b / h2;
+      const int bb = b - p * h2;
+      const int grp = bb / half;
+      const int pos = bb - grp * half;
+      const int i0 = p * ps + grp * 2 * half + pos;
+      double2 u = buf[i0], t = buf[i0 + half];
+      bfly(u, t, tw[pos * step]);
+      buf[i0] = u;
+      buf[i0 + half] = t;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ inline void load_tw(double2* tws, const double2* tw, int L) {
+  for (int t = threadIdx.x; t <= L / 2; t += FT) tws[t] = tw[t];
+}
+
+// Residual r = (div(C grad P) - RHS) * cc, cc = -1/I_F2C(C)
+// (poisson.cpp:393-403), at interior cell c.
+__device__ __forceinline__ double residual(const Dev& d, const double* __restrict__ P,
+                                           const double* __restrict__ cx,
+                                           const double* __restrict__ cy,
+                                           const double* __restrict__ cz,
+                                           const double* __restrict__ RHS, long c) {
+  const long sx = 1, sy = d.Nx, sz = d.NxNy;
+  const double Cx0 = cx[c], Cx1 = cx[c + sx];
+  const double Cy0 = cy[c], Cy1 = cy[c + sy];
+  const double Cz0 = cz[c], Cz1 = cz[c + sz];
+  const double OneThird = 1. / 3.;
+  const double ccf = OneThird * ((Cx0 + Cx1) * 0.5 + (Cy0 + Cy1) * 0.5 + (Cz0 + Cz1) * 0.5);
+  const double cc = -1. / ccf;
+  const double pc = P[c];
+  const double D = (((P[c + sx] - pc) * d.idx) * Cx1 - ((pc - P[c - sx]) * d.idx) * Cx0) * d.idx +
+                   (((P[c + sy] - pc) * d.idy) * Cy1 - ((pc - P[c - sy]) * d.idy) * Cy0) * d.idy +
+                   (((P[c + sz] - pc) * d.idz) * Cz1 - ((pc - P[c - sz]) * d.idz) * Cz0) * d.idz;
+  return (D - RHS[c]) * cc;
+}
+
+// Block-wide max of one value per thread (FT threads); thread 0 stores it.
+__device__ inline void block_max_store(double m, double* red, double* out) {
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = FT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+}  // namespace fr
+
+namespace fr {
+
+// Max of n per-block maxima into out[0] (one workgroup), on stream st.
+int max_final(const double* b, int n, double* out, hipStream_t st);
+
+inline int ilog2(int n) {
+  int l = 0;
+  while ((1 << l) < n) l++;
+  return (1 << l) == n ? l : -1;
+}
+
+// Twiddles exactly as the oracle forms them: a = 2.0*PI*k/n,
+// (cos a, sign*sin a) for k = 0..n/2.
+inline int upload_tw(int n, int sign, double2** out) {
+  const double PI_ = 3.141592653589793;
+  std::vector<double2> t(n / 2 + 1);
+  for (int k = 0; k <= n / 2; k++) {
+    double ang = 2.0 * PI_ * (double)k / (double)n;
+    t[k] = make_double2(cos(ang), sign * sin(ang));
+  }
+  FR_CHECK(hipMalloc(out, sizeof(double2) * t.size()));
+  FR_CHECK(hipMemcpy(*out, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice));
+  return FR_OK;
+}
+
+}  // namespace fr

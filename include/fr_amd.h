@@ -115,6 +115,24 @@ int fr_box_poisson_solve(fr_box_poisson* s, const double* cx,
                          const double* RHS, int iterations, double epsilon,
                          int* num_iter, void* stream);
 
+/* Duct pressure solve: the defect-correction loop of poisson_IO::Solve
+ * (poisson_IO.cpp:81-114) around poisson_IO::CCP_FFT (poisson_IO.cpp:116-206):
+ * 2-D FFT over the periodic y,z directions, one Thomas solve along x per
+ * (y,z) mode with Neumann ends and the (0,0) mode pinned
+ * (poisson_IO.cpp:208-452), inverse FFT, P += Re/(Ny_tot*Nz_tot).  g must be
+ * a duct (is_box = 0).  Same arguments and stopping rule as the box solve,
+ * except that the error is max|dp| of the iteration (taken before P += dp,
+ * poisson_IO.cpp:100-101).  On exit P's y,z ghosts hold periodic images and
+ * its x ghosts Neumann copies (poisson_IO.cpp:102-107).  Ny_tot and Nz_tot
+ * must be powers of two (Ny_tot >= 4, both <= 1024), Nx_tot >= 2. */
+typedef struct fr_duct_poisson fr_duct_poisson;
+int fr_duct_poisson_create(const fr_geom* g, fr_duct_poisson** out);
+int fr_duct_poisson_destroy(fr_duct_poisson* s);
+int fr_duct_poisson_solve(fr_duct_poisson* s, const double* cx,
+                          const double* cy, const double* cz, double* P,
+                          const double* RHS, int iterations, double epsilon,
+                          int* num_iter, void* stream);
+
 /* ------------------------------------------------------------ whole solver
  * The box solver (grid.h:16-115 with its particle set, box poisson solver
  * and the simulation.cpp:34-71 step loop) held on one device.  Created from a

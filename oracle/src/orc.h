@@ -345,8 +345,10 @@ struct DuctPoisson {  // poisson_IO.h:20-88
   double num_iteration2 = 0;
   void init(const Params* p, const Geom* gg);
   void Solve(const F1& C, F0& P, const F0& RHS);  // poisson_IO.cpp:81-114
-  void CCP_FFT(const F0& RHS, F0& P);             // poisson_IO.cpp:116-152
+  void CCP_FFT(const F0& RHS, F0& P);             // poisson_IO.cpp:116-206
+  void CCP_FFT_complex(const F0& RHS, F0& P);     // full complex, odd Ny
   void thomas(F0& val, F1& tc, F1& td, int comp, int j, int k);  // 208-452
+  void thomas_line(double* v, long stride, long off, int j, int k) const;
 };
 
 // ---------------------------------------------------------------- grid

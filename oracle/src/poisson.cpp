@@ -183,9 +183,99 @@ void DuctPoisson::Solve(const F1& C, F0& P, const F0& RHS) {
   num_iteration2 /= i;
 }
 
-// poisson_IO.cpp:116-206: per interior x-slice a 2-D FFT over (y,z) with y
-// fastest; Thomas along x per mode; inverse 2-D FFT; P = Re/(Ny*Nz).
+// poisson_IO.cpp:116-206: per interior x-slice a 2-D FFT over (y,z), a
+// Thomas solve along x per (y,z) mode, the inverse 2-D FFT, P = Re/(Ny*Nz).
+// The slice data and the Thomas coefficients are real, so conjugate modes
+// stay conjugate: the solve runs on the Ny/2+1 non-negative y frequencies
+// (real transform along y, complex along z; inverse: complex along z, real
+// along y), which is the reference's full complex solve with its zero
+// imaginary part dropped.  Odd Ny falls back to the full complex transform.
 void DuctPoisson::CCP_FFT(const F0& RHS, F0& P) {
+  const int Nx = g->Nx, bs = g->bs, ny = g->Ny_tot, nz = g->Nz_tot;
+  if (ny % 2 != 0 || ny < 4) {
+    CCP_FFT_complex(RHS, P);
+    return;
+  }
+  const int nxi = Nx - 2 * bs, nh = ny / 2 + 1;
+  const long nmode = (long)nh * nz;
+  // spectrum [i][jh][k], k fastest
+  std::vector<double> sr((size_t)nxi * nmode), si((size_t)nxi * nmode);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nxi; i++) {
+    std::vector<double> row(ny);
+    double* r = &sr[(size_t)i * nmode];
+    double* m = &si[(size_t)i * nmode];
+    std::vector<double> xr(nh), xi(nh);
+    for (int k = 0; k < nz; k++) {
+      for (int j = 0; j < ny; j++) row[j] = RHS(i + bs, j + bs, k + bs);
+      rfft1d(row.data(), ny, xr.data(), xi.data());
+      for (int jh = 0; jh < nh; jh++) {
+        r[(long)jh * nz + k] = xr[jh];
+        m[(long)jh * nz + k] = xi[jh];
+      }
+    }
+    for (int jh = 0; jh < nh; jh++) fft1d(r + (long)jh * nz, m + (long)jh * nz, nz, 1, -1);
+  }
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int jh = 0; jh < nh; jh++) {
+      thomas_line(sr.data(), nmode, (long)jh * nz + k, jh, k);
+      thomas_line(si.data(), nmode, (long)jh * nz + k, jh, k);
+    }
+  const int tot = ny * nz;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nxi; i++) {
+    std::vector<double> row(ny), xr(nh), xi(nh);
+    double* r = &sr[(size_t)i * nmode];
+    double* m = &si[(size_t)i * nmode];
+    for (int jh = 0; jh < nh; jh++) fft1d(r + (long)jh * nz, m + (long)jh * nz, nz, 1, +1);
+    for (int k = 0; k < nz; k++) {
+      for (int jh = 0; jh < nh; jh++) {
+        xr[jh] = r[(long)jh * nz + k];
+        xi[jh] = m[(long)jh * nz + k];
+      }
+      irfft1d(xr.data(), xi.data(), ny, row.data());
+      for (int j = 0; j < ny; j++) P(i + bs, j + bs, k + bs) = row[j] / (tot);
+    }
+  }
+}
+
+// One Thomas line (poisson_IO.cpp:208-329) over the nxi interior x-points of
+// mode (j, k), values v[i*stride + off]: first and last rows Neumann (b0),
+// mode (0,0) pinned by a zero last d'.
+void DuctPoisson::thomas_line(double* v, long stride, long off, int j, int k) const {
+  const int bs = g->bs, Nyt = g->Ny_tot, Nzt = g->Nz_tot;
+  const int n = g->Nx - 2 * bs;
+  double TWO_PI(2 * 3.141592653589793);
+  double Two_PI_Over_Ly = TWO_PI / g->Ly;
+  double Two_PI_Over_Lz = TWO_PI / g->Lz;
+  double jj = ((j < Nyt / 2) ? j : j - Nyt) * Two_PI_Over_Ly;
+  double kk = ((k < Nzt / 2) ? k : k - Nzt) * Two_PI_Over_Lz;
+  double inv_dx2 = 1. / (g->dx * g->dx);
+  double dy = g->dy, dz = g->dz;
+  double dy2 = 2. / (dy * dy), dz2 = 2. / (dz * dz);
+  double a = inv_dx2;
+  double b0 = -((dy2) * (1 - cos(jj * dy)) + (dz2) * (1 - cos(kk * dz))) - inv_dx2;
+  double b = -((dy2) * (1 - cos(jj * dy)) + (dz2) * (1 - cos(kk * dz))) - 2 * inv_dx2;
+  double c = inv_dx2;
+  std::vector<double> cp(n), dp(n);
+  auto V = [&](int i) -> double& { return v[(long)i * stride + off]; };
+  cp[0] = c / b0;
+  dp[0] = V(0) / b0;
+  for (int i = 1; i < n - 1; i++) {
+    cp[i] = c / (b - cp[i - 1] * a);
+    dp[i] = (V(i) - dp[i - 1] * a) / (b - cp[i - 1] * a);
+  }
+  if (j + k == 0)
+    dp[n - 1] = 0;
+  else
+    dp[n - 1] = (V(n - 1) - dp[n - 2] * a) / (b0 - cp[n - 2] * a);
+  V(n - 1) = dp[n - 1];
+  for (int i = n - 2; i >= 0; i--) V(i) = dp[i] - cp[i] * V(i + 1);
+}
+
+// poisson_IO.cpp:116-206 with the full complex 2-D transform (odd Ny).
+void DuctPoisson::CCP_FFT_complex(const F0& RHS, F0& P) {
   const int Nx = g->Nx, bs = g->bs, ny = g->Ny_tot, nz = g->Nz_tot;
   std::vector<double> re((size_t)ny * nz), im((size_t)ny * nz);
   for (int i = bs; i < Nx - bs; i++) {

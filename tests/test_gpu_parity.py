@@ -126,6 +126,43 @@ def test_box_poisson_matches_oracle(n):
     assert np.array_equal(host(dP), ref), rel(host(dP), ref)
 
 
+@pytest.mark.parametrize("n", [(12, 8, 4), (10, 16, 8), (48, 32, 16)], ids=str)
+def test_duct_poisson_matches_oracle(n):
+    """fr_duct_poisson_solve vs the oracle's poisson_IO::Solve restatement
+    (2-D FFT + Thomas defect correction, Neumann x ghosts) on a
+    variable-coefficient problem: bit-exact P and the same iteration count."""
+    import ctypes as C
+    L = fr.lib()
+    g = fr.make_geom(*n, L=(3.0, 1.0, 2 * np.pi), box=False)
+    og = OL.Geom.from_buffer_copy(g)
+    shp = (g.Nz, g.Ny, g.Nx)
+    rng = np.random.default_rng(7)
+    O = OL.orc()
+    rho = 1.0 + 2.0 * rng.random(shp)
+    O.orc_update_ghosts(og, rho, 0)
+    C3 = []
+    for axis in range(3):
+        rf = np.ones(shp)  # the inlet x-face ghost keeps 1 (unused by the solve)
+        O.orc_interp(og, rf, rho, axis, 0)
+        C3.append(1.0 / rf)
+    rhs = rng.standard_normal(shp)
+    rhs -= rhs[1:-1, 1:-1, 1:-1].mean()  # solvable with Neumann x ends
+    P0 = np.zeros(shp)
+    ref = P0.copy()
+    it_ref = O.orc_poisson(og, C3[0], C3[1], C3[2], ref, rhs, 30, 1e-10)
+    assert it_ref < 30
+    h = C.c_void_p()
+    fr.check(L.fr_duct_poisson_create(g, C.byref(h)))
+    dC = [dbuf(c) for c in C3]
+    dP, dR = dbuf(P0), dbuf(rhs)
+    it = C.c_int()
+    fr.check(L.fr_duct_poisson_solve(h, dptr(dC[0]), dptr(dC[1]), dptr(dC[2]), dptr(dP),
+                                     dptr(dR), 30, 1e-10, C.byref(it), None))
+    L.fr_duct_poisson_destroy(h)
+    assert it.value == it_ref
+    assert np.array_equal(host(dP), ref), rel(host(dP), ref)
+
+
 @pytest.fixture(scope="module")
 def case(tmp_path_factory):
     return make_box_case(tmp_path_factory.mktemp("box"), n=16, n_particles=3000)
