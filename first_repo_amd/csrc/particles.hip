@@ -220,12 +220,14 @@ __global__ void k_bucket_scatter(int np, const int* __restrict__ key, const int*
   list[start[key[i]] + slot[i]] = i;
 }
 
-// Sort each bucket ascending (slots from atomics come in any order).
-__global__ void k_bucket_sort(long ncell, const int* __restrict__ start, int* __restrict__ list) {
+// Sort each bucket ascending (slots from atomics come in any order), then
+// record every particle's final position in `pos` (the slot array reused).
+__global__ void k_bucket_sort(long ncell, const int* __restrict__ start, int* __restrict__ list,
+                              int* __restrict__ pos) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const int b = start[c], e = start[c + 1];
-  if (e - b < 2) return;
+  if (e == b) return;
   for (int u = b + 1; u < e; u++) {
     const int v = list[u];
     int w = u - 1;
@@ -235,20 +237,22 @@ __global__ void k_bucket_sort(long ncell, const int* __restrict__ start, int* __
     }
     list[w + 1] = v;
   }
+  for (int u = b; u < e; u++) pos[list[u]] = u;
 }
 
-// Per-particle deposit operands gathered into bucket order, so the deposit
-// kernels read them contiguously.
+// Per-particle deposit operands written to the particle's bucket position, so
+// the deposit kernels read them contiguously (reads here are in particle
+// order; the scattered side is the write).
 // mode 0 (particle.cpp:536-542): x_l = x*idx - round(x*idx) + 0.5, w = RHS_T.
 // mode 1 (particle.cpp:505-511): x_l = x*idx - trunc(x*idx), w = RHS_u,v,w.
 __global__ void k_bucket_gather(int np, int mode, PartPtrs p, int st,
-                                const int* __restrict__ list, double idx, double idy,
+                                const int* __restrict__ posv, double idx, double idy,
                                 double idz, double* __restrict__ s0, double* __restrict__ s1,
                                 double* __restrict__ s2, double* __restrict__ s3,
                                 double* __restrict__ s4, double* __restrict__ s5) {
-  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos >= np) return;
-  const int i = list[pos];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int pos = posv[i];
   const double X = p.s[0][st][i], Y = p.s[1][st][i], Z = p.s[2][st][i];
   if (mode == 0) {
     const int ii = (int)(round(X * idx));
@@ -269,36 +273,62 @@ __global__ void k_bucket_gather(int np, int mode, PartPtrs p, int st,
 }
 
 // ---- deposits -----------------------------------------------------------------
-// K-way merge over bucket ranges in ascending particle order.
+// Target tiles of DTX x DTY x 1 cells.  The bucket starts of the source cells
+// a tile reads (DTX+1 cells and their end, DTY+1 rows, 2 planes) are staged in
+// LDS once per workgroup.
+constexpr int DTX = 64, DTY = 4, DSR = DTX + 2, DROWS = (DTY + 1) * 2;
+
+// Stage start[] of rows (b0 + oy + r%(DTY+1), c0 + r/(DTY+1)), columns a0 + ox
+// + j, j < DSR; entries outside the grid are never read as a bucket.
+__device__ __forceinline__ void stage_starts(const Dev& d, const int* __restrict__ start,
+                                             int a0, int b0, int c0, int (*ss)[DSR]) {
+  const int tid = threadIdx.y * DTX + threadIdx.x;
+  for (int t = tid; t < DROWS * DSR; t += DTX * DTY) {
+    const int r = t / DSR, j = t - r * DSR;
+    const int B = b0 + r % (DTY + 1), Cz = c0 + r / (DTY + 1), A = a0 + j;
+    const long cell = (long)Cz * d.NxNy + (long)B * d.Nx + A;
+    int v = 0;
+    if (B >= 0 && B < d.Ny && Cz >= 0 && Cz < d.Nz && cell >= 0)
+      v = start[cell < d.size ? cell : d.size];
+    ss[r][j] = v;
+  }
+  __syncthreads();
+}
+
+// K-way merge over bucket ranges in ascending particle order; the head
+// particle index of every range is kept in a register.
 template <int NB>
 struct Merge {
-  int cur[NB], end[NB];
-  __device__ __forceinline__ int next(const int* __restrict__ list, int& which, int& pos) {
-    int best = INT_MAX, wb = -1, bp = 0;
+  int cur[NB], end[NB], head[NB];
+  __device__ __forceinline__ void init(const int* __restrict__ list) {
 #pragma unroll
-    for (int q = 0; q < NB; q++) {
-      if (cur[q] < end[q]) {
-        const int v = list[cur[q]];
-        if (v < best) {
-          best = v;
-          wb = q;
-          bp = cur[q];
-        }
-      }
-    }
+    for (int q = 0; q < NB; q++) head[q] = cur[q] < end[q] ? list[cur[q]] : INT_MAX;
+  }
+  // returns the range of the next particle (-1 when all are done), its position
+  __device__ __forceinline__ int next(const int* __restrict__ list, int& pos) {
+    int best = INT_MAX, wb = -1;
 #pragma unroll
     for (int q = 0; q < NB; q++)
-      if (q == wb) cur[q]++;
-    which = wb;
-    pos = bp;
-    return best;
+      if (head[q] < best) {
+        best = head[q];
+        wb = q;
+      }
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+      if (q == wb) {
+        pos = cur[q];
+        cur[q]++;
+        head[q] = cur[q] < end[q] ? list[cur[q]] : INT_MAX;
+      }
+    return wb;
   }
 };
 
 // Trilinear deposit of particle.cpp:526-559 (sw == nullptr, concentration)
 // or 586-658 (weight RHS_T): raw per-cell sums for every ghosted cell before
 // the cumulative halo, in ascending particle order (the reference's scatter
-// order), from the rounded-index buckets.
+// order), from the rounded-index buckets.  Target (a,b,c) collects from the
+// bucket of cell (a+ox, b+oy, c+oz) the weight with (1-x_l) where ox = 1.
 __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ start,
                                                  const int* __restrict__ list,
                                                  const double* __restrict__ sxl,
@@ -306,26 +336,28 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
                                                  const double* __restrict__ szl,
                                                  const double* __restrict__ sw,
                                                  double* __restrict__ out) {
-  const int a = blockIdx.x * 64 + threadIdx.x;
-  const int b = blockIdx.y * 4 + threadIdx.y;
-  const int c = blockIdx.z;
+  __shared__ int ss[DROWS][DSR];
+  const int a0 = blockIdx.x * DTX, b0 = blockIdx.y * DTY, c = blockIdx.z;
+  stage_starts(d, start, a0, b0, c, ss);
+  const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
   if (a >= d.Nx || b >= d.Ny) return;
   Merge<8> m;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    const int A = a + (q & 1), B = b + ((q >> 1) & 1), C = c + (q >> 2);
-    if (A < d.Nx && B < d.Ny && C < d.Nz) {
-      const long cell = at(d, A, B, C);
-      m.cur[q] = start[cell];
-      m.end[q] = start[cell + 1];
+    const int ox = q & 1, oy = (q >> 1) & 1, oz = q >> 2;
+    if (a + ox < d.Nx && b + oy < d.Ny && c + oz < d.Nz) {
+      const int r = threadIdx.y + oy + (DTY + 1) * oz, j = threadIdx.x + ox;
+      m.cur[q] = ss[r][j];
+      m.end[q] = ss[r][j + 1];
     } else {
       m.cur[q] = m.end[q] = 0;
     }
   }
+  m.init(list);
   double acc = 0.0;
   for (;;) {
-    int q, pos;
-    m.next(list, q, pos);
+    int pos;
+    const int q = m.next(list, pos);
     if (q < 0) break;
     const double x_l = sxl[pos], y_l = syl[pos], z_l = szl[pos];
     const double wx = (q & 1) ? (1 - x_l) : x_l;
@@ -340,27 +372,29 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
 
 // 1-D linear momentum deposit (particle.cpp:498-524) from the truncating-
 // index buckets: component M of target t collects (1-x_l)*R from bucket t and
-// x_l*R from bucket t - e_M, in ascending particle order.
+// x_l*R from bucket t - e_M, in ascending particle order.  LDS rows hold
+// y = b0-1.., z = c-1..c, x = a0-1.. (the lower neighbours).
 template <int M>
-__device__ __forceinline__ double dep_vel_comp(const Dev& d, const int* __restrict__ start,
-                                               const int* __restrict__ list,
+__device__ __forceinline__ double dep_vel_comp(const int (*ss)[DSR], const int* __restrict__ list,
                                                const double* __restrict__ l,
-                                               const double* __restrict__ R, long t,
-                                               int cm) {
-  const long e = M == 0 ? 1 : (M == 1 ? d.Nx : d.NxNy);
+                                               const double* __restrict__ R, int cm) {
+  const int r0 = threadIdx.y + 1 + (DTY + 1), j0 = threadIdx.x + 1;
+  const int r1 = M == 1 ? r0 - 1 : (M == 2 ? r0 - (DTY + 1) : r0);
+  const int j1 = M == 0 ? j0 - 1 : j0;
   Merge<2> m;
-  m.cur[0] = start[t];
-  m.end[0] = start[t + 1];
+  m.cur[0] = ss[r0][j0];
+  m.end[0] = ss[r0][j0 + 1];
   if (cm > 0) {
-    m.cur[1] = start[t - e];
-    m.end[1] = start[t - e + 1];
+    m.cur[1] = ss[r1][j1];
+    m.end[1] = ss[r1][j1 + 1];
   } else {
     m.cur[1] = m.end[1] = 0;
   }
+  m.init(list);
   double acc = 0.0;
   for (;;) {
-    int q, pos;
-    m.next(list, q, pos);
+    int pos;
+    const int q = m.next(list, pos);
     if (q < 0) break;
     acc = acc + (q == 0 ? (1 - l[pos]) * R[pos] : l[pos] * R[pos]);
   }
@@ -377,14 +411,15 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ 
                                                  const double* __restrict__ srw,
                                                  double* __restrict__ ox, double* __restrict__ oy,
                                                  double* __restrict__ oz) {
-  const int a = blockIdx.x * 64 + threadIdx.x;
-  const int b = blockIdx.y * 4 + threadIdx.y;
-  const int c = blockIdx.z;
+  __shared__ int ss[DROWS][DSR];
+  const int a0 = blockIdx.x * DTX, b0 = blockIdx.y * DTY, c = blockIdx.z;
+  stage_starts(d, start, a0 - 1, b0 - 1, c - 1, ss);
+  const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
   if (a >= d.Nx || b >= d.Ny) return;
   const long n = at(d, a, b, c);
-  ox[n] = dep_vel_comp<0>(d, start, list, sxl, sru, n, a);
-  oy[n] = dep_vel_comp<1>(d, start, list, syl, srv, n, b);
-  oz[n] = dep_vel_comp<2>(d, start, list, szl, srw, n, c);
+  ox[n] = dep_vel_comp<0>(ss, list, sxl, sru, a);
+  oy[n] = dep_vel_comp<1>(ss, list, syl, srv, b);
+  oz[n] = dep_vel_comp<2>(ss, list, szl, srw, c);
 }
 
 // ---- migration (one rank) -------------------------------------------------------
@@ -603,9 +638,9 @@ int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st) {
     hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
                        B.slot, B.start, B.list);
     hipLaunchKernelGGL(k_bucket_sort, dim3(nblocks(ncell, 256)), dim3(256), 0, st, ncell,
-                       B.start, B.list);
+                       B.start, B.list, B.slot);
     hipLaunchKernelGGL(k_bucket_gather, dim3(nblocks(np, 256)), dim3(256), 0, st, np, mode,
-                       ptrs(), stage, B.list, d.idx, d.idy, d.idz, B.s[0], B.s[1], B.s[2],
+                       ptrs(), stage, B.slot, d.idx, d.idy, d.idz, B.s[0], B.s[1], B.s[2],
                        B.s[3], B.s[4], B.s[5]);
   }
   FR_LAUNCH_CHECK();
