@@ -90,20 +90,24 @@ struct Buckets {
 };
 
 // Device particle set (SoA, the 29 migrated values of particle.cpp:252-262
-// plus the RHS arrays), its cell buckets and migration workspace.
+// plus the RHS arrays), its cell buckets and migration workspace.  Storage
+// slot i holds the particle of reference index rank[i] (inv is the inverse);
+// `alt` is a second storage set for re-sorting and host staging.
 class PartDev {
  public:
   int np = 0, cap = 0;
   double* s[7][4] = {};
   int* flag = nullptr;
   double *rhs_u = nullptr, *rhs_v = nullptr, *rhs_w = nullptr, *rhs_T = nullptr;
+  int *rank = nullptr, *inv = nullptr, *rank_alt = nullptr;
+  PartPtrs alt = {};
   long ncell = 0;
   Buckets R;  // rounded index, operands x_l,y_l,z_l,RHS_T
   Buckets T;  // truncating index, operands x_l,y_l,z_l,RHS_u,RHS_v,RHS_w
   int* scan_tmp = nullptr;
   int *flg = nullptr, *nl = nullptr, *pNL = nullptr, *holepos = nullptr, *tailnl = nullptr;
-  int *evt = nullptr, *islo = nullptr, *plo = nullptr, *dst = nullptr, *stgf = nullptr;
-  double* stg = nullptr;
+  int *evt = nullptr, *islo = nullptr, *plo = nullptr, *dst = nullptr, *mslot = nullptr,
+      *mlv = nullptr;
   int* hbuf = nullptr;
   long moved_last = 0;
 
@@ -113,7 +117,7 @@ class PartDev {
   int advance(const Dev& d, const AdvCoef& c, const double* const u[3], hipStream_t st);
   int temp(const Dev& d, int stage, double c0, const double* rho, double iTth, hipStream_t st);
   // buckets of the `stage` positions; mode 0 rounded (R), 1 truncating (T)
-  int build(const Dev& d, int stage, int mode, hipStream_t st);
+  int build(const Dev& d, int stage, int mode, hipStream_t st, bool operands = true);
   // trilinear deposit from R: weighted by RHS_T (heat) or not (concentration)
   int deposit_cic(const Dev& d, bool weighted, double* out, bool halo, hipStream_t st);
   // momentum deposit from T
@@ -121,6 +125,14 @@ class PartDev {
   int migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st);
   void swap_stages(int a, int b);
   int copy_stage(int dst_stage, int src_stage, hipStream_t st);
+  // storage order: identity, or re-sorted by cell (values unchanged)
+  int reset_order(hipStream_t st);
+  int resort(const Dev& d, hipStream_t st);
+  // host transfers of one array in reference order
+  int get(double* host, const double* src, hipStream_t st);
+  int set(const double* host, double* dst, hipStream_t st);
+  int get_flags(int* host, hipStream_t st);
+  int set_flags(const int* host, hipStream_t st);
 };
 
 }  // namespace fr

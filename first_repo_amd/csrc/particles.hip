@@ -15,6 +15,13 @@
 //  * migrate : the one-rank swap-with-last removal + append order of
 //              SortValid{Z,Y,X}/FromBuf (particle.cpp:150-311, 853-858),
 //              computed in parallel from closed-form event ranks
+//
+// Storage order is not the reference order.  Every storage slot carries the
+// particle's reference index (`rank`, with its inverse `inv`); the reference
+// order only decides the order of deposit sums and the outcome of migration,
+// which therefore rewrites ranks instead of moving particle data.  Once per
+// time step the slots are re-sorted by cell so gathers and bucket builds touch
+// memory in cell order.
 #include <climits>
 
 #include "fr_internal.hpp"
@@ -213,17 +220,20 @@ __global__ void k_bucket_count(Dev d, PartPtrs p, int st, int mode, int* __restr
   slot[i] = atomicAdd(&cnt[c], 1);
 }
 
+// Buckets hold reference indices (ranks).
 __global__ void k_bucket_scatter(int np, const int* __restrict__ key, const int* __restrict__ slot,
-                                 const int* __restrict__ start, int* __restrict__ list) {
+                                 const int* __restrict__ start, const int* __restrict__ rank,
+                                 int* __restrict__ list) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  list[start[key[i]] + slot[i]] = i;
+  list[start[key[i]] + slot[i]] = rank[i];
 }
 
-// Sort each bucket ascending (slots from atomics come in any order), then
-// record every particle's final position in `pos` (the slot array reused).
+// Sort each bucket by reference index (slots from atomics come in any order),
+// then record every storage slot's bucket position in `pos` (the atomic-slot
+// array reused).
 __global__ void k_bucket_sort(long ncell, const int* __restrict__ start, int* __restrict__ list,
-                              int* __restrict__ pos) {
+                              const int* __restrict__ inv, int* __restrict__ pos) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const int b = start[c], e = start[c + 1];
@@ -237,7 +247,7 @@ __global__ void k_bucket_sort(long ncell, const int* __restrict__ start, int* __
     }
     list[w + 1] = v;
   }
-  for (int u = b; u < e; u++) pos[list[u]] = u;
+  for (int u = b; u < e; u++) pos[inv[list[u]]] = u;
 }
 
 // Per-particle deposit operands written to the particle's bucket position, so
@@ -423,15 +433,20 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ 
 }
 
 // ---- migration (one rank) -------------------------------------------------------
-// For axis a with positions P = a_new: leaver flag 2 if P > hi, 1 if P < lo.
-__global__ void k_mig_flag(int np, const double* __restrict__ P, double lo, double hi,
-                           int* __restrict__ flg, int* __restrict__ nl) {
+// Flags live in reference order: flg[rank] = 2 if P > hi, 1 if P < lo, for the
+// leavers only (flg is cleared first).
+__global__ void k_mig_flag(int np, const double* __restrict__ P, const int* __restrict__ rank,
+                           double lo, double hi, int* __restrict__ flg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const double v = P[i];
   const int f = v > hi ? 2 : (v < lo ? 1 : 0);
-  flg[i] = f;
-  nl[i] = f == 0;
+  if (f) flg[rank[i]] = f;
+}
+
+__global__ void k_mig_nl(int np, const int* __restrict__ flg, int* __restrict__ nl) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < np) nl[r] = flg[r] == 0;
 }
 
 // Front holes (leavers at i < K) get rank m = i - pNL[i]; tail non-leavers
@@ -475,23 +490,25 @@ struct MigAxis {
   double L;
 };
 
-// Stage the moved particles (tail non-leavers and leavers, the latter
-// wrapped by MODE_L*), then write them to their destinations.
-__global__ void k_mig_gather(int np, int nleave, PartPtrs p, MigAxis ax,
-                             const int* __restrict__ pNL, const int* __restrict__ holepos,
-                             const int* __restrict__ tailnl, const int* __restrict__ evt,
-                             const int* __restrict__ plo, double* __restrict__ stg,
-                             int* __restrict__ stgf, int* __restrict__ dst, long cap) {
+// The moved particles: tail non-leaver s < h goes to the front hole s; leaver
+// of event e goes to K + (its rank among the lo leavers, or nlo + its rank
+// among the hi leavers) -- the append order lo then hi.  Records the storage
+// slot (read from inv before any rank changes), the new reference index and
+// whether the particle is a leaver.
+__global__ void k_mig_moves(int np, int nleave, const int* __restrict__ pNL,
+                            const int* __restrict__ holepos, const int* __restrict__ tailnl,
+                            const int* __restrict__ evt, const int* __restrict__ plo,
+                            const int* __restrict__ inv, int* __restrict__ mslot,
+                            int* __restrict__ mto, int* __restrict__ mlv) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int K = pNL[np];
   const int h = K - pNL[K < np ? K : np];
   if (s >= h + nleave) return;
-  int src, to;
-  bool leaver;
+  int src, to, lv;
   if (s < h) {
     src = tailnl[s];
     to = holepos[s];
-    leaver = false;
+    lv = 0;
   } else {
     const int e = s - h;
     src = evt[e];
@@ -499,18 +516,30 @@ __global__ void k_mig_gather(int np, int nleave, PartPtrs p, MigAxis ax,
     const int r = plo[e];
     const bool lo = (plo[e + 1] - r) != 0;
     to = K + (lo ? r : nlo + (e - r));
-    leaver = true;
+    lv = 1;
   }
-  dst[s] = to;
-  for (int q = 0; q < 7; q++)
-    for (int t = 0; t < 4; t++) stg[(long)(q * 4 + t) * cap + s] = p.s[q][t][src];
-  stgf[s] = p.flag[src];
-  if (leaver) {
-    // MODE_L* (particle.cpp:58-104): wrap _new, base and _np1 of this axis
+  mslot[s] = inv[src];
+  mto[s] = to;
+  mlv[s] = lv;
+}
+
+// New reference indices of the moved slots; leavers are wrapped by MODE_L*
+// (particle.cpp:58-104): _new, base and _np1 of this axis shift by -+L when
+// _new > L or < 0.
+__global__ void k_mig_apply(int np, int nleave, PartPtrs p, MigAxis ax,
+                            const int* __restrict__ pNL, const int* __restrict__ mslot,
+                            const int* __restrict__ mto, const int* __restrict__ mlv,
+                            int* __restrict__ rank, int* __restrict__ inv) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int K = pNL[np];
+  const int h = K - pNL[K < np ? K : np];
+  if (s >= h + nleave) return;
+  const int slot = mslot[s], to = mto[s];
+  rank[slot] = to;
+  inv[to] = slot;
+  if (mlv[s]) {
     const int a = ax.axis;
-    double vn = stg[(long)(a * 4 + 2) * cap + s];
-    double v0 = stg[(long)(a * 4 + 0) * cap + s];
-    double v1 = stg[(long)(a * 4 + 3) * cap + s];
+    double vn = p.s[a][2][slot], v0 = p.s[a][0][slot], v1 = p.s[a][3][slot];
     if (vn > ax.L) {
       vn -= ax.L;
       v0 -= ax.L;
@@ -520,23 +549,55 @@ __global__ void k_mig_gather(int np, int nleave, PartPtrs p, MigAxis ax,
       v0 += ax.L;
       v1 += ax.L;
     }
-    stg[(long)(a * 4 + 2) * cap + s] = vn;
-    stg[(long)(a * 4 + 0) * cap + s] = v0;
-    stg[(long)(a * 4 + 3) * cap + s] = v1;
+    p.s[a][2][slot] = vn;
+    p.s[a][0][slot] = v0;
+    p.s[a][3][slot] = v1;
   }
 }
 
-__global__ void k_mig_scatter(int np, int nleave, PartPtrs p, const int* __restrict__ pNL,
-                              const double* __restrict__ stg, const int* __restrict__ stgf,
-                              const int* __restrict__ dst, long cap) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int K = pNL[np];
-  const int h = K - pNL[K < np ? K : np];
-  if (s >= h + nleave) return;
-  const int to = dst[s];
+// ---- storage order ------------------------------------------------------------------
+__global__ void k_iota2(int n, int* __restrict__ a, int* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = b[i] = i;
+}
+
+__global__ void k_inv(int n, const int* __restrict__ rank, int* __restrict__ inv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) inv[rank[i]] = i;
+}
+
+// Reference order <-> storage order of one array.
+template <typename T>
+__global__ void k_to_ref(int n, const int* __restrict__ rank, const T* __restrict__ src,
+                         T* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[rank[i]] = src[i];
+}
+
+template <typename T>
+__global__ void k_from_ref(int n, const int* __restrict__ rank, const T* __restrict__ src,
+                           T* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[rank[i]];
+}
+
+// New slot u takes the particle of reference index list[u] (cell order).
+__global__ void k_resort(int np, const int* __restrict__ list, const int* __restrict__ inv,
+                         PartPtrs a, const int* __restrict__ ra, PartPtrs b,
+                         int* __restrict__ rb) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= np) return;
+  const int o = inv[list[u]];
+#pragma unroll
   for (int q = 0; q < 7; q++)
-    for (int t = 0; t < 4; t++) p.s[q][t][to] = stg[(long)(q * 4 + t) * cap + s];
-  p.flag[to] = stgf[s];
+#pragma unroll
+    for (int t = 0; t < 4; t++) b.s[q][t][u] = a.s[q][t][o];
+  b.rhs_u[u] = a.rhs_u[o];
+  b.rhs_v[u] = a.rhs_v[o];
+  b.rhs_w[u] = a.rhs_w[o];
+  b.rhs_T[u] = a.rhs_T[o];
+  b.flag[u] = a.flag[o];
+  rb[u] = ra[o];
 }
 
 }  // namespace fr
@@ -565,31 +626,40 @@ int PartDev::alloc(int capacity, const Dev& d) {
   cap = capacity < 1 ? 1 : capacity;
   ncell = d.size;
   for (int q = 0; q < 7; q++)
-    for (int t = 0; t < 4; t++) FR_CHECK(hipMalloc(&s[q][t], sizeof(double) * cap));
+    for (int t = 0; t < 4; t++) {
+      FR_CHECK(hipMalloc(&s[q][t], sizeof(double) * cap));
+      FR_CHECK(hipMalloc(&alt.s[q][t], sizeof(double) * cap));
+    }
   FR_CHECK(hipMalloc(&flag, sizeof(int) * cap));
-  double** r[4] = {&rhs_u, &rhs_v, &rhs_w, &rhs_T};
+  FR_CHECK(hipMalloc(&alt.flag, sizeof(int) * cap));
+  double** r[8] = {&rhs_u, &rhs_v, &rhs_w, &rhs_T, &alt.rhs_u, &alt.rhs_v, &alt.rhs_w, &alt.rhs_T};
   for (auto pp : r) {
     FR_CHECK(hipMalloc(pp, sizeof(double) * cap));
     FR_CHECK(hipMemset(*pp, 0, sizeof(double) * cap));
   }
+  FR_CHECK(hipMalloc(&rank, sizeof(int) * cap));
+  FR_CHECK(hipMalloc(&rank_alt, sizeof(int) * cap));
+  FR_CHECK(hipMalloc(&inv, sizeof(int) * cap));
   int rc = alloc_buckets(R, ncell, cap, 4);
   if (!rc) rc = alloc_buckets(T, ncell, cap, 6);
   if (rc) return rc;
   const long big = ncell > cap ? ncell : cap;
   FR_CHECK(hipMalloc(&scan_tmp, sizeof(int) * (big / STILE + 2)));
-  int** m[9] = {&flg, &nl, &pNL, &holepos, &tailnl, &evt, &islo, &plo, &dst};
+  int** m[11] = {&flg, &nl, &pNL, &holepos, &tailnl, &evt, &islo, &plo, &dst, &mslot, &mlv};
   for (auto pp : m) FR_CHECK(hipMalloc(pp, sizeof(int) * (cap + 1)));
-  FR_CHECK(hipMalloc(&stgf, sizeof(int) * cap));
-  FR_CHECK(hipMalloc(&stg, sizeof(double) * 28 * (size_t)cap));
   FR_CHECK(hipHostMalloc(&hbuf, sizeof(int) * 4, hipHostMallocDefault));
   return FR_OK;
 }
 
 void PartDev::release() {
   for (int q = 0; q < 7; q++)
-    for (int t = 0; t < 4; t++) (void)hipFree(s[q][t]);
-  void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, scan_tmp, flg, nl, pNL, holepos, tailnl,
-                 evt, islo, plo, dst, stgf, stg};
+    for (int t = 0; t < 4; t++) {
+      (void)hipFree(s[q][t]);
+      (void)hipFree(alt.s[q][t]);
+    }
+  void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, alt.flag, alt.rhs_u, alt.rhs_v, alt.rhs_w,
+                 alt.rhs_T, rank, rank_alt, inv, scan_tmp, flg, nl, pNL, holepos, tailnl,
+                 evt, islo, plo, dst, mslot, mlv};
   for (void* p : all) (void)hipFree(p);
   free_buckets(R);
   free_buckets(T);
@@ -626,7 +696,7 @@ int PartDev::temp(const Dev& d, int stage, double c0, const double* rho, double 
   return FR_OK;
 }
 
-int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st) {
+int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool operands) {
   Buckets& B = mode == 0 ? R : T;
   FR_CHECK(hipMemsetAsync(B.cnt, 0, sizeof(int) * ncell, st));
   if (np > 0)
@@ -636,12 +706,13 @@ int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st) {
   if (rc) return rc;
   if (np > 0) {
     hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
-                       B.slot, B.start, B.list);
+                       B.slot, B.start, rank, B.list);
     hipLaunchKernelGGL(k_bucket_sort, dim3(nblocks(ncell, 256)), dim3(256), 0, st, ncell,
-                       B.start, B.list, B.slot);
-    hipLaunchKernelGGL(k_bucket_gather, dim3(nblocks(np, 256)), dim3(256), 0, st, np, mode,
-                       ptrs(), stage, B.slot, d.idx, d.idy, d.idz, B.s[0], B.s[1], B.s[2],
-                       B.s[3], B.s[4], B.s[5]);
+                       B.start, B.list, inv, B.slot);
+    if (operands)
+      hipLaunchKernelGGL(k_bucket_gather, dim3(nblocks(np, 256)), dim3(256), 0, st, np, mode,
+                         ptrs(), stage, B.slot, d.idx, d.idy, d.idz, B.s[0], B.s[1], B.s[2],
+                         B.s[3], B.s[4], B.s[5]);
   }
   FR_LAUNCH_CHECK();
   B.valid = true;
@@ -673,13 +744,17 @@ int PartDev::deposit_vel(const Dev& d, double* const out[3], hipStream_t st) {
   return FR_OK;
 }
 
-// particle::Send_Recv on one rank: axes z, y, x (particle.cpp:853-858).
+// particle::Send_Recv on one rank: axes z, y, x (particle.cpp:853-858).  The
+// event order is computed on reference indices; only the reference indices of
+// the moved particles change, and the leavers' coordinates are wrapped.
 int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st) {
   if (np == 0) return FR_OK;
   const unsigned nb = nblocks(np, 256);
   for (int ax = 2; ax >= 0; ax--) {
-    hipLaunchKernelGGL(k_mig_flag, dim3(nb), dim3(256), 0, st, np, (const double*)s[ax][2], lo[ax],
-                       hi[ax], flg, nl);
+    FR_CHECK(hipMemsetAsync(flg, 0, sizeof(int) * np, st));
+    hipLaunchKernelGGL(k_mig_flag, dim3(nb), dim3(256), 0, st, np, (const double*)s[ax][2], rank,
+                       lo[ax], hi[ax], flg);
+    hipLaunchKernelGGL(k_mig_nl, dim3(nb), dim3(256), 0, st, np, flg, nl);
     int rc = scan_ex(nl, np, pNL, scan_tmp, st);
     if (rc) return rc;
     FR_CHECK(hipMemcpyAsync(hbuf, pNL + np, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -693,11 +768,12 @@ int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], 
     rc = scan_ex(islo, nleave, plo, scan_tmp, st);
     if (rc) return rc;
     MigAxis m{ax, L[ax]};
+    // at most 2*nleave particles move (h <= nleave front holes + nleave leavers)
     const unsigned ng = nblocks(2L * nleave, 256);
-    hipLaunchKernelGGL(k_mig_gather, dim3(ng), dim3(256), 0, st, np, nleave, ptrs(), m, pNL,
-                       holepos, tailnl, evt, plo, stg, stgf, dst, (long)cap);
-    hipLaunchKernelGGL(k_mig_scatter, dim3(ng), dim3(256), 0, st, np, nleave, ptrs(), pNL, stg,
-                       stgf, dst, (long)cap);
+    hipLaunchKernelGGL(k_mig_moves, dim3(ng), dim3(256), 0, st, np, nleave, pNL, holepos, tailnl,
+                       evt, plo, inv, mslot, dst, mlv);
+    hipLaunchKernelGGL(k_mig_apply, dim3(ng), dim3(256), 0, st, np, nleave, ptrs(), m, pNL,
+                       mslot, dst, mlv, rank, inv);
     FR_LAUNCH_CHECK();
   }
   R.valid = false;
@@ -714,6 +790,82 @@ int PartDev::copy_stage(int dst_stage, int src_stage, hipStream_t st) {
   for (int q = 0; q < 7; q++)
     FR_CHECK(hipMemcpyAsync(s[q][dst_stage], s[q][src_stage], sizeof(double) * np,
                             hipMemcpyDeviceToDevice, st));
+  return FR_OK;
+}
+
+int PartDev::reset_order(hipStream_t st) {
+  if (np > 0)
+    hipLaunchKernelGGL(k_iota2, dim3(nblocks(np, 256)), dim3(256), 0, st, np, rank, inv);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+// Storage slots re-sorted by truncating cell index of the base positions,
+// ties by reference index (the truncating buckets of stage 0).  Particle
+// values, reference indices and buckets of the rounded index are unchanged.
+int PartDev::resort(const Dev& d, hipStream_t st) {
+  if (np < 2) return FR_OK;
+  int rc = build(d, 0, 1, st, false);
+  if (rc) return rc;
+  PartPtrs a = ptrs(), b = alt;
+  b.np = np;
+  hipLaunchKernelGGL(k_resort, dim3(nblocks(np, 256)), dim3(256), 0, st, np, T.list, inv, a, rank,
+                     b, rank_alt);
+  FR_LAUNCH_CHECK();
+  for (int q = 0; q < 7; q++)
+    for (int t = 0; t < 4; t++) std::swap(s[q][t], alt.s[q][t]);
+  std::swap(flag, alt.flag);
+  std::swap(rhs_u, alt.rhs_u);
+  std::swap(rhs_v, alt.rhs_v);
+  std::swap(rhs_w, alt.rhs_w);
+  std::swap(rhs_T, alt.rhs_T);
+  std::swap(rank, rank_alt);
+  hipLaunchKernelGGL(k_inv, dim3(nblocks(np, 256)), dim3(256), 0, st, np, rank, inv);
+  FR_LAUNCH_CHECK();
+  T.valid = false;
+  return FR_OK;
+}
+
+// Host transfer in reference order; `alt` storage is the staging buffer.
+int PartDev::get(double* host, const double* src, hipStream_t st) {
+  if (np == 0) return FR_OK;
+  double* tmp = alt.s[0][0];
+  hipLaunchKernelGGL(k_to_ref<double>, dim3(nblocks(np, 256)), dim3(256), 0, st, np, rank, src,
+                     tmp);
+  FR_LAUNCH_CHECK();
+  FR_CHECK(hipMemcpyAsync(host, tmp, sizeof(double) * np, hipMemcpyDeviceToHost, st));
+  FR_CHECK(hipStreamSynchronize(st));
+  return FR_OK;
+}
+
+int PartDev::set(const double* host, double* dst, hipStream_t st) {
+  if (np == 0) return FR_OK;
+  double* tmp = alt.s[0][0];
+  FR_CHECK(hipMemcpyAsync(tmp, host, sizeof(double) * np, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_from_ref<double>, dim3(nblocks(np, 256)), dim3(256), 0, st, np, rank, tmp,
+                     dst);
+  FR_LAUNCH_CHECK();
+  FR_CHECK(hipStreamSynchronize(st));
+  return FR_OK;
+}
+
+int PartDev::get_flags(int* host, hipStream_t st) {
+  if (np == 0) return FR_OK;
+  hipLaunchKernelGGL(k_to_ref<int>, dim3(nblocks(np, 256)), dim3(256), 0, st, np, rank,
+                     (const int*)flag, alt.flag);
+  FR_LAUNCH_CHECK();
+  FR_CHECK(hipMemcpyAsync(host, alt.flag, sizeof(int) * np, hipMemcpyDeviceToHost, st));
+  FR_CHECK(hipStreamSynchronize(st));
+  return FR_OK;
+}
+
+int PartDev::set_flags(const int* host, hipStream_t st) {
+  if (np == 0) return FR_OK;
+  FR_CHECK(hipMemcpyAsync(alt.flag, host, sizeof(int) * np, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_from_ref<int>, dim3(nblocks(np, 256)), dim3(256), 0, st, np, rank,
+                     (const int*)alt.flag, flag);
+  FR_LAUNCH_CHECK();
+  FR_CHECK(hipStreamSynchronize(st));
   return FR_OK;
 }
 

@@ -242,6 +242,8 @@ int particle::load_random() {
       FR_CHECK(hipMemcpyAsync(dev.s[q][t], t == 0 ? base[q]->data() : zero.data(),
                               sizeof(double) * n, hipMemcpyHostToDevice, st_));
   FR_CHECK(hipMemcpyAsync(dev.flag, fl.data(), sizeof(int) * n, hipMemcpyHostToDevice, st_));
+  int rc = dev.reset_order(st_);
+  if (rc) return rc;
   FR_CHECK(hipStreamSynchronize(st_));
   return FR_OK;
 }
@@ -472,6 +474,9 @@ int grid::Update_P0() {  // grid.cpp:539-554
 
 int grid::Update_Particle() {  // grid.cpp:659-680
   hipEventRecord(ev[2], st);
+  // storage back in cell order once per step (reference order is kept in the
+  // particles' ranks, so no result depends on it)
+  if (RK4_count == 0) FR_TRY(part.dev.resort(size.dev(), st));
   FR_TRY(part.advance(RK4_count, RK4_preCoeff[RK4_count], RK4_postCoeff[RK4_count], U));
   FR_TRY(part.part2gas_concentration(C));
   if (param.TWC())
@@ -799,36 +804,39 @@ int fr_sim_get_particles(fr_sim* s, int q, int st, double* host) {
   if (!s || !host || q < 0 || q > 7 || st < 0 || st > 3) return FR_EINVAL;
   PartDev& p = s->g.part.dev;
   const double* src = q == 7 ? p.rhs_T : p.s[q][st];
-  FR_CHECK(hipMemcpy(host, src, sizeof(double) * p.np, hipMemcpyDeviceToHost));
-  return FR_OK;
+  return p.get(host, src, s->g.st);
 }
 
 int fr_sim_set_particles(fr_sim* s, int np, int q, int st, const double* host) {
   if (!s || !host || q < 0 || q > 7 || st < 0 || st > 3) return FR_EINVAL;
   PartDev& p = s->g.part.dev;
   if (np > p.cap || np < 0) return FR_EOVERFLOW;
-  p.np = np;
-  s->g.part.Np = np;
+  if (np != p.np) {  // a new particle set: storage in reference order
+    p.np = np;
+    s->g.part.Np = np;
+    int rc = p.reset_order(s->g.st);
+    if (rc) return rc;
+  }
   double* dst = q == 7 ? p.rhs_T : p.s[q][st];
-  FR_CHECK(hipMemcpy(dst, host, sizeof(double) * np, hipMemcpyHostToDevice));
-  return FR_OK;
+  return p.set(host, dst, s->g.st);
 }
 
 int fr_sim_get_flags(fr_sim* s, int* host) {
   if (!s || !host) return FR_EINVAL;
-  PartDev& p = s->g.part.dev;
-  FR_CHECK(hipMemcpy(host, p.flag, sizeof(int) * p.np, hipMemcpyDeviceToHost));
-  return FR_OK;
+  return s->g.part.dev.get_flags(host, s->g.st);
 }
 
 int fr_sim_set_flags(fr_sim* s, int np, const int* host) {
   if (!s || !host) return FR_EINVAL;
   PartDev& p = s->g.part.dev;
   if (np > p.cap || np < 0) return FR_EOVERFLOW;
-  p.np = np;
-  s->g.part.Np = np;
-  FR_CHECK(hipMemcpy(p.flag, host, sizeof(int) * np, hipMemcpyHostToDevice));
-  return FR_OK;
+  if (np != p.np) {
+    p.np = np;
+    s->g.part.Np = np;
+    int rc = p.reset_order(s->g.st);
+    if (rc) return rc;
+  }
+  return p.set_flags(host, s->g.st);
 }
 
 int fr_sim_particles_changed(fr_sim* s) {
