@@ -310,12 +310,12 @@ __device__ __forceinline__ void stage_starts(const Dev& d, const int* __restrict
 template <int NB>
 struct Merge {
   int cur[NB], end[NB], head[NB];
-  __device__ __forceinline__ void init(const int* list) {
+  __device__ __forceinline__ void init(const int* __restrict__ list) {
 #pragma unroll
     for (int q = 0; q < NB; q++) head[q] = cur[q] < end[q] ? list[cur[q]] : INT_MAX;
   }
   // returns the range of the next particle (-1 when all are done), its position
-  __device__ __forceinline__ int next(const int* list, int& pos) {
+  __device__ __forceinline__ int next(const int* __restrict__ list, int& pos) {
     int best = INT_MAX, wb = -1;
 #pragma unroll
     for (int q = 0; q < NB; q++)
@@ -334,63 +334,6 @@ struct Merge {
   }
 };
 
-// The particles of a tile's source rows (each row's buckets are one
-// contiguous range of the bucket arrays) are copied into LDS when they fit,
-// so the merges run on LDS; a tile with more falls back to global memory.
-// roff[r] is row r's offset in the LDS copy.
-template <int NOPS, int CAP>
-__device__ __forceinline__ bool stage_particles(const int (*ss)[DSR], int* roff,
-                                                const int* __restrict__ list,
-                                                const double* const* ops, int* lrank,
-                                                double (*lop)[CAP]) {
-  const int tid = threadIdx.y * DTX + threadIdx.x;
-  if (tid == 0) {
-    roff[0] = 0;
-    for (int r = 0; r < DROWS; r++) roff[r + 1] = roff[r] + (ss[r][DSR - 1] - ss[r][0]);
-  }
-  __syncthreads();
-  const int tot = roff[DROWS];
-  const bool fit = tot <= CAP;
-  if (fit) {
-    for (int t = tid; t < tot; t += DTX * DTY) {
-      int r = 0;
-      while (t >= roff[r + 1]) r++;
-      const int src = ss[r][0] + (t - roff[r]);
-      lrank[t] = list[src];
-#pragma unroll
-      for (int k = 0; k < NOPS; k++) lop[k][t] = ops[k] ? ops[k][src] : 0.0;
-    }
-  }
-  __syncthreads();
-  return fit;
-}
-
-// Sum of one target cell's trilinear weights over its 8 source buckets, in
-// ascending reference index.  cur/end of source q are positions in `list`
-// and in the operand arrays (LDS or global).
-template <bool W>
-__device__ __forceinline__ double cic_sum(Merge<8>& m, const int* list, const double* xl,
-                                          const double* yl, const double* zl,
-                                          const double* w) {
-  m.init(list);
-  double acc = 0.0;
-  for (;;) {
-    int pos;
-    const int q = m.next(list, pos);
-    if (q < 0) break;
-    const double x_l = xl[pos], y_l = yl[pos], z_l = zl[pos];
-    const double wx = (q & 1) ? (1 - x_l) : x_l;
-    const double wy = ((q >> 1) & 1) ? (1 - y_l) : y_l;
-    const double wz = (q >> 2) ? (1 - z_l) : z_l;
-    double t = wx * wy * wz;
-    if (W) t = t * w[pos];
-    acc = acc + t;
-  }
-  return acc;
-}
-
-constexpr int CIC_CAP = 768, VEL_CAP = 512;
-
 // Trilinear deposit of particle.cpp:526-559 (sw == nullptr, concentration)
 // or 586-658 (weight RHS_T): raw per-cell sums for every ghosted cell before
 // the cumulative halo, in ascending particle order (the reference's scatter
@@ -404,13 +347,8 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
                                                  const double* __restrict__ sw,
                                                  double* __restrict__ out) {
   __shared__ int ss[DROWS][DSR];
-  __shared__ int roff[DROWS + 1];
-  __shared__ int lrank[CIC_CAP];
-  __shared__ double lop[4][CIC_CAP];
   const int a0 = blockIdx.x * DTX, b0 = blockIdx.y * DTY, c = blockIdx.z;
   stage_starts(d, start, a0, b0, c, ss);
-  const double* ops[4] = {sxl, syl, szl, sw};
-  const bool fit = stage_particles<4, CIC_CAP>(ss, roff, list, ops, lrank, lop);
   const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
   if (a >= d.Nx || b >= d.Ny) return;
   Merge<8> m;
@@ -419,20 +357,26 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
     const int ox = q & 1, oy = (q >> 1) & 1, oz = q >> 2;
     if (a + ox < d.Nx && b + oy < d.Ny && c + oz < d.Nz) {
       const int r = threadIdx.y + oy + (DTY + 1) * oz, j = threadIdx.x + ox;
-      const int base = fit ? roff[r] - ss[r][0] : 0;
-      m.cur[q] = ss[r][j] + base;
-      m.end[q] = ss[r][j + 1] + base;
+      m.cur[q] = ss[r][j];
+      m.end[q] = ss[r][j + 1];
     } else {
       m.cur[q] = m.end[q] = 0;
     }
   }
-  double acc;
-  if (fit)
-    acc = sw ? cic_sum<true>(m, lrank, lop[0], lop[1], lop[2], lop[3])
-             : cic_sum<false>(m, lrank, lop[0], lop[1], lop[2], lop[3]);
-  else
-    acc = sw ? cic_sum<true>(m, list, sxl, syl, szl, sw)
-             : cic_sum<false>(m, list, sxl, syl, szl, sw);
+  m.init(list);
+  double acc = 0.0;
+  for (;;) {
+    int pos;
+    const int q = m.next(list, pos);
+    if (q < 0) break;
+    const double x_l = sxl[pos], y_l = syl[pos], z_l = szl[pos];
+    const double wx = (q & 1) ? (1 - x_l) : x_l;
+    const double wy = ((q >> 1) & 1) ? (1 - y_l) : y_l;
+    const double wz = (q >> 2) ? (1 - z_l) : z_l;
+    double t = wx * wy * wz;
+    if (sw) t = t * sw[pos];
+    acc = acc + t;
+  }
   out[at(d, a, b, c)] = acc;
 }
 
@@ -441,20 +385,18 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
 // x_l*R from bucket t - e_M, in ascending particle order.  LDS rows hold
 // y = b0-1.., z = c-1..c, x = a0-1.. (the lower neighbours).
 template <int M>
-__device__ __forceinline__ double dep_vel_comp(const int (*ss)[DSR], const int* roff, bool fit,
-                                               const int* list, const double* l,
-                                               const double* R, int cm) {
+__device__ __forceinline__ double dep_vel_comp(const int (*ss)[DSR], const int* __restrict__ list,
+                                               const double* __restrict__ l,
+                                               const double* __restrict__ R, int cm) {
   const int r0 = threadIdx.y + 1 + (DTY + 1), j0 = threadIdx.x + 1;
   const int r1 = M == 1 ? r0 - 1 : (M == 2 ? r0 - (DTY + 1) : r0);
   const int j1 = M == 0 ? j0 - 1 : j0;
   Merge<2> m;
-  const int base0 = fit ? roff[r0] - ss[r0][0] : 0;
-  m.cur[0] = ss[r0][j0] + base0;
-  m.end[0] = ss[r0][j0 + 1] + base0;
+  m.cur[0] = ss[r0][j0];
+  m.end[0] = ss[r0][j0 + 1];
   if (cm > 0) {
-    const int base1 = fit ? roff[r1] - ss[r1][0] : 0;
-    m.cur[1] = ss[r1][j1] + base1;
-    m.end[1] = ss[r1][j1 + 1] + base1;
+    m.cur[1] = ss[r1][j1];
+    m.end[1] = ss[r1][j1 + 1];
   } else {
     m.cur[1] = m.end[1] = 0;
   }
@@ -480,25 +422,14 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ 
                                                  double* __restrict__ ox, double* __restrict__ oy,
                                                  double* __restrict__ oz) {
   __shared__ int ss[DROWS][DSR];
-  __shared__ int roff[DROWS + 1];
-  __shared__ int lrank[VEL_CAP];
-  __shared__ double lop[6][VEL_CAP];
   const int a0 = blockIdx.x * DTX, b0 = blockIdx.y * DTY, c = blockIdx.z;
   stage_starts(d, start, a0 - 1, b0 - 1, c - 1, ss);
-  const double* ops[6] = {sxl, syl, szl, sru, srv, srw};
-  const bool fit = stage_particles<6, VEL_CAP>(ss, roff, list, ops, lrank, lop);
   const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
   if (a >= d.Nx || b >= d.Ny) return;
   const long n = at(d, a, b, c);
-  if (fit) {
-    ox[n] = dep_vel_comp<0>(ss, roff, true, lrank, lop[0], lop[3], a);
-    oy[n] = dep_vel_comp<1>(ss, roff, true, lrank, lop[1], lop[4], b);
-    oz[n] = dep_vel_comp<2>(ss, roff, true, lrank, lop[2], lop[5], c);
-  } else {
-    ox[n] = dep_vel_comp<0>(ss, roff, false, list, sxl, sru, a);
-    oy[n] = dep_vel_comp<1>(ss, roff, false, list, syl, srv, b);
-    oz[n] = dep_vel_comp<2>(ss, roff, false, list, szl, srw, c);
-  }
+  ox[n] = dep_vel_comp<0>(ss, list, sxl, sru, a);
+  oy[n] = dep_vel_comp<1>(ss, list, syl, srv, b);
+  oz[n] = dep_vel_comp<2>(ss, list, szl, srw, c);
 }
 
 // ---- migration (one rank) -------------------------------------------------------
