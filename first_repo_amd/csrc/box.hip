@@ -11,9 +11,10 @@
 namespace fr {
 
 #define FR_CELL_PROLOGUE                                          \
-  const int i = blockIdx.x * 64 + threadIdx.x;                    \
-  const int j = blockIdx.y * 4 + threadIdx.y;                     \
-  const int k = blockIdx.z;                                       \
+  const uint3 bk = xcd_block();                                   \
+  const int i = bk.x * 64 + threadIdx.x;                          \
+  const int j = bk.y * 4 + threadIdx.y;                           \
+  const int k = bk.z;                                             \
   if (i >= d.Nx || j >= d.Ny) return;                            \
   const long n = at(d, i, j, k);                                  \
   const long c = at(d, img(i, d.Nx), img(j, d.Ny), img(k, d.Nz)); \

@@ -56,7 +56,8 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
   load_tw(twm, twM, M);
   load_tw(twn, twN, q.nx);
   const long rows = (long)q.ny * q.nz;
-  const long row0 = (long)blockIdx.x * W;
+  const unsigned bid = blockIdx.x;
+  const long row0 = (long)bid * W;
   for (int t = threadIdx.x; t < W * M; t += FT) {
     const int p = t / M, m = t - p * M;
     const long row = row0 + p;
@@ -107,8 +108,9 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
   load_tw(tws, tw, L);
   if (divide) load_tw(tws2, tw2, L);
   const int tiles_x = (nxh + W - 1) / W;
-  const int i0 = (blockIdx.x % tiles_x) * W;
-  const int other = blockIdx.x / tiles_x;  // k for axis 1, j for axis 2
+  const unsigned bid = xcd_block().x;
+  const int i0 = (bid % tiles_x) * W;
+  const int other = bid / tiles_x;  // k for axis 1, j for axis 2
   const long stride = axis == 1 ? (long)nxh : (long)nxh * q.ny;
   const long base = axis == 1 ? (long)other * nxh * q.ny : (long)other * nxh;
   for (int t = threadIdx.x; t < W * L; t += FT) {
@@ -168,7 +170,8 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q, const double2* __r
   load_tw(twn, twN, q.nx);
   __syncthreads();
   const long rows = (long)q.ny * q.nz;
-  const long row0 = (long)blockIdx.x * W;
+  const unsigned bid = blockIdx.x;
+  const long row0 = (long)bid * W;
   for (int t = threadIdx.x; t < W * M; t += FT) {
     const int p = t / M, k = t - p * M;
     const long row = row0 + p;
@@ -201,7 +204,7 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q, const double2* __r
       m = a1 > m ? a1 : m;
     }
   }
-  block_max_store(m, red, bmax + blockIdx.x);
+  block_max_store(m, red, bmax + bid);
 }
 
 __global__ void k_max_final(const double* __restrict__ b, int n, double* __restrict__ out) {

@@ -76,6 +76,20 @@ __device__ __forceinline__ long at(const Dev& d, int i, int j, int k) {
   return (long)k * d.NxNy + (long)j * d.Nx + i;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  xcd_block() renumbers the linear workgroup id so that each XCD gets
+// one contiguous run of the grid (neighbouring tiles and planes share an L2);
+// a tail of fewer than 8 groups keeps its id.  Returns (x, y, z) block
+// coordinates of the renumbered id.
+__device__ __forceinline__ uint3 xcd_block() {
+  const unsigned nbx = gridDim.x, nby = gridDim.y;
+  const unsigned nb = nbx * nby * gridDim.z;
+  unsigned lin = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+  const unsigned per = nb >> 3;
+  if (lin < (per << 3)) lin = (lin & 7u) * per + (lin >> 3);
+  return make_uint3(lin % nbx, (lin / nbx) % nby, lin / (nbx * nby));
+}
+
 inline hipStream_t S(void* s) { return (hipStream_t)s; }
 
 inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
