@@ -86,16 +86,20 @@ __device__ inline void load_tw(double2* tws, const double2* tw, int L) {
 }
 
 // Residual r = (div(C grad P) - RHS) * cc, cc = -1/I_F2C(C)
-// (poisson.cpp:393-403), at interior cell c.
-__device__ __forceinline__ double residual(const Dev& d, const double* __restrict__ P,
-                                           const double* __restrict__ cx,
-                                           const double* __restrict__ cy,
-                                           const double* __restrict__ cz,
-                                           const double* __restrict__ RHS, long c) {
+// (poisson.cpp:393-403, poisson_IO.cpp:90-97), at interior cell c.  With INV
+// the arrays hold Rho_face and C = 1./Rho_face is formed inline, the value
+// grid::Solve_Poisson passes (grid.cpp:641).
+template <bool INV>
+__device__ __forceinline__ double residual_t(const Dev& d, const double* __restrict__ P,
+                                             const double* __restrict__ cx,
+                                             const double* __restrict__ cy,
+                                             const double* __restrict__ cz,
+                                             const double* __restrict__ RHS, long c) {
   const long sx = 1, sy = d.Nx, sz = d.NxNy;
-  const double Cx0 = cx[c], Cx1 = cx[c + sx];
-  const double Cy0 = cy[c], Cy1 = cy[c + sy];
-  const double Cz0 = cz[c], Cz1 = cz[c + sz];
+  auto ld = [](const double* a, long q) { return INV ? 1. / a[q] : a[q]; };
+  const double Cx0 = ld(cx, c), Cx1 = ld(cx, c + sx);
+  const double Cy0 = ld(cy, c), Cy1 = ld(cy, c + sy);
+  const double Cz0 = ld(cz, c), Cz1 = ld(cz, c + sz);
   const double OneThird = 1. / 3.;
   const double ccf = OneThird * ((Cx0 + Cx1) * 0.5 + (Cy0 + Cy1) * 0.5 + (Cz0 + Cz1) * 0.5);
   const double cc = -1. / ccf;
@@ -104,6 +108,14 @@ __device__ __forceinline__ double residual(const Dev& d, const double* __restric
                    (((P[c + sy] - pc) * d.idy) * Cy1 - ((pc - P[c - sy]) * d.idy) * Cy0) * d.idy +
                    (((P[c + sz] - pc) * d.idz) * Cz1 - ((pc - P[c - sz]) * d.idz) * Cz0) * d.idz;
   return (D - RHS[c]) * cc;
+}
+
+__device__ __forceinline__ double residual(const Dev& d, const double* __restrict__ P,
+                                           const double* __restrict__ cx,
+                                           const double* __restrict__ cy,
+                                           const double* __restrict__ cz,
+                                           const double* __restrict__ RHS, long c) {
+  return residual_t<false>(d, P, cx, cy, cz, RHS, c);
 }
 
 // Block-wide max of one value per thread (FT threads); thread 0 stores it.

@@ -39,6 +39,7 @@ struct PoisDev {
 // Pass X: per row (j,k) the residual, packed as z_m = r_2m + i r_2m+1, an
 // nx/2-point FFT, then the real-transform unpacking to X_k, k = 0..nx/2:
 //   E = (Z_k + conj Z_{M-k})/2, O = (Z_k - conj Z_{M-k})/(2i), X = E + W^k O.
+template <bool INV>
 __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __restrict__ P,
                                                    const double* __restrict__ cx,
                                                    const double* __restrict__ cy,
@@ -65,8 +66,8 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
     if (row < rows) {
       const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
       const long c = at(d, 2 * m + 1, jj + 1, kk + 1);
-      r0 = residual(d, P, cx, cy, cz, RHS, c);
-      r1 = residual(d, P, cx, cy, cz, RHS, c + 1);
+      r0 = residual_t<INV>(d, P, cx, cy, cz, RHS, c);
+      r1 = residual_t<INV>(d, P, cx, cy, cz, RHS, c + 1);
     }
     buf[p * ps + bitrev(m, q.lxm)] = make_double2(r0, r1);
   }
@@ -238,6 +239,32 @@ struct fr_box_poisson {
   size_t lds[3] = {};  // dynamic LDS bytes of the x, y and z passes
 };
 
+static int box_poisson_iter(fr_box_poisson* s, const double* cx, const double* cy,
+                            const double* cz, bool inv, double* P, const double* RHS,
+                            hipStream_t st);
+
+namespace fr {
+
+int box_poisson_solve(fr_box_poisson* s, const double* cx, const double* cy, const double* cz,
+                      bool inv, double* P, const double* RHS, int iterations, double epsilon,
+                      int* num_iter, hipStream_t st) {
+  if (!s || !cx || !cy || !cz || !P || !RHS) return FR_EINVAL;
+  int i = 0;
+  double error = 100;
+  while ((i < iterations) && (error > epsilon)) {
+    int rc = box_poisson_iter(s, cx, cy, cz, inv, P, RHS, st);
+    if (rc) return rc;
+    FR_CHECK(hipMemcpyAsync(s->hres, s->dres, sizeof(double), hipMemcpyDeviceToHost, st));
+    FR_CHECK(hipStreamSynchronize(st));
+    error = s->hres[0];
+    i++;
+  }
+  if (num_iter) *num_iter = i;
+  return FR_OK;
+}
+
+}  // namespace fr
+
 extern "C" {
 
 int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
@@ -308,7 +335,9 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
   s->lds[2] = sizeof(double2) * (q.wz * (n[2] + 1) + 2 * (n[2] / 2 + 1));
   const size_t need = std::max({s->lds[0], s->lds[1], s->lds[2]});
   if (need > 65536) {
-    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd,
+    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
     FR_CHECK(hipFuncSetAttribute((const void*)k_pass_strided,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
@@ -340,12 +369,18 @@ int fr_box_poisson_destroy(fr_box_poisson* s) {
   return FR_OK;
 }
 
-// One defect-correction iteration; leaves max|dp| in s->dres[0].
+// One defect-correction iteration; leaves max|dp| in s->dres[0].  With inv
+// the coefficient arrays hold Rho_face and C = 1./Rho_face.
 static int box_poisson_iter(fr_box_poisson* s, const double* cx, const double* cy,
-                            const double* cz, double* P, const double* RHS, hipStream_t st) {
+                            const double* cz, bool inv, double* P, const double* RHS,
+                            hipStream_t st) {
   PoisDev& q = s->q;
-  hipLaunchKernelGGL(k_pass_x_fwd, dim3(s->nbx), dim3(FT), s->lds[0], st, q, P, cx, cy, cz, RHS,
-                     s->twfm, s->twf[0], s->data);
+  if (inv)
+    hipLaunchKernelGGL(k_pass_x_fwd<true>, dim3(s->nbx), dim3(FT), s->lds[0], st, q, P, cx, cy,
+                       cz, RHS, s->twfm, s->twf[0], s->data);
+  else
+    hipLaunchKernelGGL(k_pass_x_fwd<false>, dim3(s->nbx), dim3(FT), s->lds[0], st, q, P, cx, cy,
+                       cz, RHS, s->twfm, s->twf[0], s->data);
   const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.nxh + q.wz - 1) / q.wz;
   hipLaunchKernelGGL(k_pass_strided, dim3(ty * q.nz), dim3(FT), s->lds[1], st, q, 1, 0,
                      s->twf[1], s->twf[1], s->lam[0], s->lam[1], s->lam[2], s->data);
@@ -363,20 +398,8 @@ static int box_poisson_iter(fr_box_poisson* s, const double* cx, const double* c
 int fr_box_poisson_solve(fr_box_poisson* s, const double* cx, const double* cy,
                          const double* cz, double* P, const double* RHS, int iterations,
                          double epsilon, int* num_iter, void* stream) {
-  if (!s || !cx || !cy || !cz || !P || !RHS) return FR_EINVAL;
-  hipStream_t st = S(stream);
-  int i = 0;
-  double error = 100;
-  while ((i < iterations) && (error > epsilon)) {
-    int rc = box_poisson_iter(s, cx, cy, cz, P, RHS, st);
-    if (rc) return rc;
-    FR_CHECK(hipMemcpyAsync(s->hres, s->dres, sizeof(double), hipMemcpyDeviceToHost, st));
-    FR_CHECK(hipStreamSynchronize(st));
-    error = s->hres[0];
-    i++;
-  }
-  if (num_iter) *num_iter = i;
-  return FR_OK;
+  return fr::box_poisson_solve(s, cx, cy, cz, false, P, RHS, iterations, epsilon, num_iter,
+                               S(stream));
 }
 
 }  // extern "C"

@@ -56,6 +56,13 @@ int box_div_rhs(const Dev& d, const DivCoef& q, const double* rho_new, const dou
 int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P,
                      double* const ru_np1[3], double* const ru_new[3], hipStream_t s);
 
+// ---- box pressure solve (fft_poisson.hip) -------------------------------------
+// fr_box_poisson_solve with, when inv is set, Rho_face passed in place of C
+// and C = 1./Rho_face formed inside the residual (grid.cpp:641).
+int box_poisson_solve(fr_box_poisson* s, const double* cx, const double* cy, const double* cz,
+                      bool inv, double* P, const double* RHS, int iterations, double epsilon,
+                      int* num_iter, hipStream_t st);
+
 // ---- diagnostics (diag.hip) ----------------------------------------------------------
 int diag_tke(const Dev& d, const double* u, const double* rho, const double* red,
              double* out, hipStream_t s);

@@ -347,7 +347,8 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
                                                  const double* __restrict__ sw,
                                                  double* __restrict__ out) {
   __shared__ int ss[DROWS][DSR];
-  const int a0 = blockIdx.x * DTX, b0 = blockIdx.y * DTY, c = blockIdx.z;
+  const uint3 bk = xcd_block();
+  const int a0 = bk.x * DTX, b0 = bk.y * DTY, c = bk.z;
   stage_starts(d, start, a0, b0, c, ss);
   const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
   if (a >= d.Nx || b >= d.Ny) return;
@@ -422,7 +423,8 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ 
                                                  double* __restrict__ ox, double* __restrict__ oy,
                                                  double* __restrict__ oz) {
   __shared__ int ss[DROWS][DSR];
-  const int a0 = blockIdx.x * DTX, b0 = blockIdx.y * DTY, c = blockIdx.z;
+  const uint3 bk = xcd_block();
+  const int a0 = bk.x * DTX, b0 = bk.y * DTY, c = bk.z;
   stage_starts(d, start, a0 - 1, b0 - 1, c - 1, ss);
   const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
   if (a >= d.Nx || b >= d.Ny) return;

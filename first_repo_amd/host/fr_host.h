@@ -214,6 +214,8 @@ class poisson {
   int init(params* p, gridsize* g, hipStream_t st);
   ~poisson();
   int Solve(tensor1& C, tensor0& P, tensor0& RHS);  // poisson.cpp:115-117
+  // Solve with C = 1./Rho_face (grid.cpp:641) without storing C
+  int Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS);
   int num_iteration() const { return num_iteration_; }
 };
 

@@ -313,6 +313,11 @@ int poisson::Solve(tensor1& C, tensor0& P, tensor0& RHS) {
                               RHS.pointer(), par->Iteration(), par->epsilon(),
                               &num_iteration_, st_);
 }
+int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS) {
+  return box_poisson_solve(s, Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer(),
+                           true, P.pointer(), RHS.pointer(), par->Iteration(), par->epsilon(),
+                           &num_iteration_, st_);
+}
 
 // ---------------------------------------------------------------- grid
 #define FR_TRY(x)                \
@@ -545,9 +550,9 @@ int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
 }
 
 int grid::Solve_Poisson() {  // grid.cpp:639-643
-  dummy.Equal_Divide(1., Rho_face);
+  // C = 1./Rho_face is formed inside the solver's residual
   hipEventRecord(ev[0], st);
-  FR_TRY(PS_.Solve(dummy, P, RHS_Pois));
+  FR_TRY(PS_.Solve_face_rho(Rho_face, P, RHS_Pois));
   hipEventRecord(ev[1], st);
   hipEventSynchronize(ev[1]);
   float ms = 0;
