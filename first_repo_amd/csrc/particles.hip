@@ -128,29 +128,33 @@ __global__ void k_scan_tiles(const int* __restrict__ in, long n, int* __restrict
   if (threadIdx.x == 0) tsum[blockIdx.x] = sh[0];
 }
 
-// exclusive scan of tile sums in one block; total stored at out_total
-__global__ void k_scan_sums(int* __restrict__ tsum, int ntiles, int* __restrict__ out_total) {
-  __shared__ int sh[SB];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
+// Exclusive scan of the tile sums in one workgroup of SS threads: each thread
+// scans a contiguous chunk, the chunk totals are scanned in LDS, then added
+// back.  The grand total is stored at out_total.
+constexpr int SS = 1024;
+__global__ void __launch_bounds__(SS) k_scan_sums(int* __restrict__ tsum, int ntiles,
+                                                  int* __restrict__ out_total) {
+  __shared__ int sh[SS];
+  const int chunk = (ntiles + SS - 1) / SS;
+  const int b = threadIdx.x * chunk;
+  const int e = b + chunk < ntiles ? b + chunk : ntiles;
+  int run = 0;
+  for (int t = b; t < e; t++) {
+    const int v = tsum[t];
+    tsum[t] = run;
+    run += v;
+  }
+  sh[threadIdx.x] = run;
   __syncthreads();
-  for (int b0 = 0; b0 < ntiles; b0 += SB) {
-    const int t = b0 + threadIdx.x;
-    const int v = t < ntiles ? tsum[t] : 0;
-    sh[threadIdx.x] = v;
+  for (int o = 1; o < SS; o <<= 1) {
+    const int a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
     __syncthreads();
-    for (int o = 1; o < SB; o <<= 1) {
-      int a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += a;
-      __syncthreads();
-    }
-    if (t < ntiles) tsum[t] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == SB - 1) carry += sh[SB - 1];
+    sh[threadIdx.x] += a;
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out_total = carry;
+  const int off = sh[threadIdx.x] - run;
+  for (int t = b; t < e; t++) tsum[t] += off;
+  if (threadIdx.x == SS - 1) *out_total = sh[SS - 1];
 }
 
 __global__ void k_scan_apply(const int* __restrict__ in, long n, const int* __restrict__ tsum,
@@ -186,7 +190,7 @@ int scan_ex(const int* in, long n, int* out, int* tmp, hipStream_t s) {
     return FR_OK;
   }
   hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SB), 0, s, in, n, tmp);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SB), 0, s, tmp, nt, out + n);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SS), 0, s, tmp, nt, out + n);
   hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(SB), 0, s, in, n, tmp, out);
   FR_LAUNCH_CHECK();
   return FR_OK;
