@@ -84,6 +84,8 @@ def lib():
             "fr_sim_get_flags": ([vp, vp], i),
             "fr_sim_set_flags": ([vp, i, vp], i),
             "fr_sim_particles_changed": ([vp], i),
+            "fr_sim_set_inflow": ([vp, vp, i, vp], i),
+            "fr_sim_reserve_particles": ([vp, i], i),
             "fr_sim_diagnostics": ([vp, vp], i),
             "fr_sim_timing": ([vp, vp], i),
         }
@@ -169,6 +171,17 @@ class Sim:
     def set_flags(self, a):
         a = np.ascontiguousarray(a, dtype=np.int32)
         check(self.L.fr_sim_set_flags(self.h, len(a), _p(a)))
+
+    def set_inflow(self, planes, inlet=None):
+        """Duct inflow: planes (8, Nz, Ny); inlet particles (n, 29)."""
+        pl = None if planes is None else np.ascontiguousarray(planes, dtype=np.float64)
+        n = 0 if inlet is None else len(inlet)
+        ib = None if inlet is None else np.ascontiguousarray(inlet, dtype=np.float64)
+        check(self.L.fr_sim_set_inflow(self.h, None if pl is None else _p(pl), n,
+                                       None if ib is None else _p(ib)), "set_inflow")
+
+    def reserve_particles(self, capacity):
+        check(self.L.fr_sim_reserve_particles(self.h, int(capacity)), "reserve_particles")
 
     def particles_changed(self):
         check(self.L.fr_sim_particles_changed(self.h))

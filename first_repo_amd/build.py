@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FR_OFFLOAD_ARCH", "gfx950")
 
-KERNELS = ["fields.hip", "box.hip", "fft_poisson.hip", "duct_poisson.hip", "particles.hip",
+KERNELS = ["fields.hip", "box.hip", "duct.hip", "fft_poisson.hip", "duct_poisson.hip", "particles.hip",
            "diag.hip"]
 HOST = ["host.cpp"]
 

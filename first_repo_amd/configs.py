@@ -236,3 +236,117 @@ WORKLOADS = {
     "box256": BoxWorkload(n=256, n_particles=4_000_000, steps=200, mass_loading=0.5),
     "box512": BoxWorkload(n=512, n_particles=16_000_000, steps=200, mass_loading=0.5),
 }
+
+
+@dataclass
+class DuctWorkload:
+    """Inflow/outflow duct (IsBox=0, Initial=0), periodic in y and z.
+
+    The reference feeds the duct from a box grid (simulation.cpp:52-66); here
+    the inflow is synthetic: the box planes are a uniform stream U = u_in with
+    `noise` relative random perturbations (a new seeded draw every step), and
+    `inject_per_step` particles enter through the inlet each step.  The initial
+    state is the same stream with a streamwise density profile falling from 1 to
+    1/rho_ratio (gas heated along the duct), so the pressure solve has variable
+    coefficients from the first step.
+    """
+    n: tuple = (512, 128, 128)
+    L: tuple = (8 * math.pi, 2 * math.pi, 2 * math.pi)
+    steps: int = 200
+    seed: int = 4321
+    u_in: float = 1.0
+    noise: float = 0.05
+    rho_ratio: float = 3.0
+    inject_per_step: int = 20_000
+    capacity: int = 8_000_000
+    mu: float = 2e-3
+    stokes_time: float = 0.5
+    heating_rate: float = 0.5
+    iterations: int = 30
+    epsilon: float = 1e-10
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def dx(self):
+        return self.L[0] / self.n[0]
+
+    def params(self) -> Params:
+        nx, ny, nz = self.n
+        dt = 0.5 * self.dx / (self.u_in * (1 + 4 * self.noise))
+        mu, cp, cv = self.mu, 3.5, 2.5
+        Tp = self.stokes_time
+        # particles of the box workload's kind: mass from Dp, Rhop at Stokes time Tp
+        Dp = 0.02 * self.dx
+        Rhop = 18 * mu * Tp / (Dp * Dp)
+        mp = Rhop * math.pi * Dp ** 3 / 6.0
+        cvp = 0.6 * cv
+        eps_p = 0.4
+        I0 = self.heating_rate * mp * cvp / (0.25 * eps_p * math.pi * Dp * Dp)
+        p = Params(
+            IsBox=0, Nx_tot=nx, Ny_tot=ny, Nz_tot=nz, Lx=self.L[0], Ly=self.L[1], Lz=self.L[2],
+            dt=dt, Initial=0, Cooling=0, Iteration1=self.iterations,
+            Iteration2=self.iterations, epsilon=self.epsilon, TWC=1, A1=0.0, A2=0.0,
+            Rho0=1.0, U0=self.u_in, T0=1.0, Mu0=mu, k=mu * cp / 0.7, Cp=cp, Cv=cv, np0=0.0,
+            Dp=Dp, Rhop=Rhop, Cvp=cvp, epsilon_p=eps_p, Nu=2.0, I0_1=I0, I0_2=I0,
+            T_final=dt * self.steps * 1.5)
+        for k, v in self.extra.items():
+            setattr(p, k, v)
+        return p
+
+    def build(self, outdir: str) -> dict:
+        """params.in and U/V/W.bin (the perturbation; U0 = u_in is added by
+        grid::Initialize, grid.cpp:189)."""
+        os.makedirs(outdir, exist_ok=True)
+        nx, ny, nz = self.n
+        rng = np.random.default_rng(self.seed)
+        for name in ("U.bin", "V.bin", "W.bin"):
+            a = self.noise * self.u_in * rng.standard_normal((nz, ny, nx))
+            a.astype("<f8").tofile(os.path.join(outdir, name))
+        p = self.params()
+        p.write(os.path.join(outdir, "params.in"))
+        return {"params": p, "dt": p.dt}
+
+    def density(self):
+        """Ghosted initial density [Nz+2][Ny+2][Nx+2]: 1 at the inlet falling
+        linearly in 1/rho to 1/rho_ratio at the outlet."""
+        nx, ny, nz = self.n
+        x = (np.arange(nx + 2) - 0.5) / nx
+        inv = 1 + (self.rho_ratio - 1) * np.clip(x, 0, 1)
+        return np.broadcast_to((1.0 / inv)[None, None, :], (nz + 2, ny + 2, nx + 2)).copy()
+
+    def inflow(self, step: int) -> np.ndarray:
+        """The 8 box planes (rho_new, rho_np1, RU_new xyz, RU_np1 xyz),
+        ghosted [Nz+2][Ny+2] with periodic ghosts."""
+        _, ny, nz = self.n
+        rng = np.random.default_rng((self.seed, step))
+        pert = self.noise * self.u_in * rng.standard_normal((3, nz, ny))
+        pl = np.zeros((8, nz + 2, ny + 2))
+        pl[0] = pl[1] = 1.0
+        for m in range(3):
+            inner = pert[m] + (self.u_in if m == 0 else 0.0)
+            pl[2 + m, 1:-1, 1:-1] = inner
+            pl[5 + m, 1:-1, 1:-1] = inner
+        for q in range(2, 8):  # periodic ghosts in y and z
+            pl[q, 0, :] = pl[q, -2, :]
+            pl[q, -1, :] = pl[q, 1, :]
+            pl[q, :, 0] = pl[q, :, -2]
+            pl[q, :, -1] = pl[q, :, 1]
+        return pl
+
+    def inlet_particles(self, step: int, first_flag: int, n: int | None = None) -> np.ndarray:
+        """n particles entering this step (ToBuf layout, 29 values): x in
+        [0, dx/2), y and z uniform, velocity u_in along x, T = T0, all four
+        stages equal, flags first_flag, first_flag+1, ..."""
+        n = self.inject_per_step if n is None else n
+        rng = np.random.default_rng((self.seed, step, 7))
+        b = np.zeros((n, 29))
+        pos = [rng.random(n) * 0.5 * self.dx, rng.random(n) * self.L[1], rng.random(n) * self.L[2]]
+        vals = pos + [np.full(n, self.u_in), np.zeros(n), np.zeros(n), np.ones(n)]
+        for q, v in enumerate(vals):
+            for s in range(4):
+                b[:, q * 4 + s] = v
+        b[:, 28] = first_flag + np.arange(n)
+        return b
+
+
+WORKLOADS["duct512"] = DuctWorkload()

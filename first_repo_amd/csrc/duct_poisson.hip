@@ -38,6 +38,7 @@ struct DuctDev {
 
 // Pass Y: block (i-tile, k).  Residual rows r(i, j, k), j = 0..ny-1, packed as
 // z_m = r_2m + i r_2m+1, an ny/2-point FFT, unpacked to X_jh, jh = 0..ny/2.
+template <bool INV>
 __global__ void __launch_bounds__(FT) k_duct_y_fwd(DuctDev q, const double* __restrict__ P,
                                                    const double* __restrict__ cx,
                                                    const double* __restrict__ cy,
@@ -62,8 +63,8 @@ __global__ void __launch_bounds__(FT) k_duct_y_fwd(DuctDev q, const double* __re
     double r0 = 0.0, r1 = 0.0;
     if (i0 + p < q.nxi) {
       const long c = at(d, i0 + p + 1, 2 * m + 1, k + 1);
-      r0 = residual(d, P, cx, cy, cz, RHS, c);
-      r1 = residual(d, P, cx, cy, cz, RHS, c + d.Nx);
+      r0 = residual_t<INV>(d, P, cx, cy, cz, RHS, c);
+      r1 = residual_t<INV>(d, P, cx, cy, cz, RHS, c + d.Nx);
     }
     buf[p * ps + bitrev(m, q.lym)] = make_double2(r0, r1);
   }
@@ -273,6 +274,33 @@ struct fr_duct_poisson {
   size_t lds_y = 0, lds_z = 0;
 };
 
+static int duct_poisson_iter(fr_duct_poisson* s, const double* cx, const double* cy,
+                             const double* cz, bool inv, double* P, const double* RHS,
+                             hipStream_t st);
+
+namespace fr {
+
+// poisson_IO::Solve (poisson_IO.cpp:81-114): error = max|dp| of the iteration.
+int duct_poisson_solve(fr_duct_poisson* s, const double* cx, const double* cy, const double* cz,
+                       bool inv, double* P, const double* RHS, int iterations, double epsilon,
+                       int* num_iter, hipStream_t st) {
+  if (!s || !cx || !cy || !cz || !P || !RHS) return FR_EINVAL;
+  int i = 0;
+  double error = 100;
+  while ((i < iterations) && (error > epsilon)) {
+    int rc = duct_poisson_iter(s, cx, cy, cz, inv, P, RHS, st);
+    if (rc) return rc;
+    FR_CHECK(hipMemcpyAsync(s->hres, s->dres, sizeof(double), hipMemcpyDeviceToHost, st));
+    FR_CHECK(hipStreamSynchronize(st));
+    error = s->hres[0];
+    i++;
+  }
+  if (num_iter) *num_iter = i;
+  return FR_OK;
+}
+
+}  // namespace fr
+
 extern "C" {
 
 int fr_duct_poisson_create(const fr_geom* g, fr_duct_poisson** out) {
@@ -369,12 +397,18 @@ int fr_duct_poisson_destroy(fr_duct_poisson* s) {
   return FR_OK;
 }
 
-// One defect-correction iteration; leaves max|dp| in s->dres[0].
+// One defect-correction iteration; leaves max|dp| in s->dres[0].  With inv
+// the coefficient arrays hold Rho_face and C = 1./Rho_face.
 static int duct_poisson_iter(fr_duct_poisson* s, const double* cx, const double* cy,
-                             const double* cz, double* P, const double* RHS, hipStream_t st) {
+                             const double* cz, bool inv, double* P, const double* RHS,
+                             hipStream_t st) {
   DuctDev& q = s->q;
-  hipLaunchKernelGGL(k_duct_y_fwd, dim3(s->nby), dim3(FT), s->lds_y, st, q, P, cx, cy, cz, RHS,
-                     s->twfm, s->twfy, s->A);
+  if (inv)
+    hipLaunchKernelGGL(k_duct_y_fwd<true>, dim3(s->nby), dim3(FT), s->lds_y, st, q, P, cx, cy,
+                       cz, RHS, s->twfm, s->twfy, s->A);
+  else
+    hipLaunchKernelGGL(k_duct_y_fwd<false>, dim3(s->nby), dim3(FT), s->lds_y, st, q, P, cx, cy,
+                       cz, RHS, s->twfm, s->twfy, s->A);
   hipLaunchKernelGGL(k_duct_z<true>, dim3(s->nbz), dim3(FT), s->lds_z, st, q, s->twfz, s->A,
                      s->T);
   hipLaunchKernelGGL(k_duct_thomas, dim3(nblocks(q.nm, 256)), dim3(256), 0, st, q, s->lamy,
@@ -396,20 +430,8 @@ static int duct_poisson_iter(fr_duct_poisson* s, const double* cx, const double*
 int fr_duct_poisson_solve(fr_duct_poisson* s, const double* cx, const double* cy,
                           const double* cz, double* P, const double* RHS, int iterations,
                           double epsilon, int* num_iter, void* stream) {
-  if (!s || !cx || !cy || !cz || !P || !RHS) return FR_EINVAL;
-  hipStream_t st = S(stream);
-  int i = 0;
-  double error = 100;
-  while ((i < iterations) && (error > epsilon)) {
-    int rc = duct_poisson_iter(s, cx, cy, cz, P, RHS, st);
-    if (rc) return rc;
-    FR_CHECK(hipMemcpyAsync(s->hres, s->dres, sizeof(double), hipMemcpyDeviceToHost, st));
-    FR_CHECK(hipStreamSynchronize(st));
-    error = s->hres[0];
-    i++;
-  }
-  if (num_iter) *num_iter = i;
-  return FR_OK;
+  return fr::duct_poisson_solve(s, cx, cy, cz, false, P, RHS, iterations, epsilon, num_iter,
+                                S(stream));
 }
 
 }  // extern "C"

@@ -56,12 +56,38 @@ int box_div_rhs(const Dev& d, const DivCoef& q, const double* rho_new, const dou
 int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P,
                      double* const ru_np1[3], double* const ru_new[3], hipStream_t s);
 
+// ---- duct substep kernels (duct.hip) ----------------------------------------------
+int duct_update_rho(const Dev& d, const BoxCoef& cf, double cbc, const double* const ru_int[3],
+                    const double* rho, const double* rho_int, double* rho_np1, double* rho_new,
+                    const double* const rf[3], double* const u[3], const double* in_new,
+                    const double* in_np1, hipStream_t s);
+int duct_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w, double cbc,
+                       const double* const u[3], const double* const ru_int[3],
+                       const double* const S[3], const double* const ru[3],
+                       double* const ru_np1[3], double* const ru_new[3], const double* in_new,
+                       const double* in_np1, hipStream_t s);
+int duct_div(const Dev& d, const DivCoef& q, const double* rho_new, const double* Q,
+             double* const rf[3], double* dvg, hipStream_t s);
+int duct_planes(const Dev& d, const double* ru_new_x, const double* rfx, double* out2,
+                hipStream_t s);
+int duct_scale(const Dev& d, double ratio, int last, double* ru_new_x, double* ru_np1_x,
+               hipStream_t s);
+int duct_rhs(const Dev& d, double sc, const double* const ru_new[3], const double* const rf[3],
+             const double* dvg, double* rhs, hipStream_t s);
+int duct_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, double* const ru_np1[3],
+                      double* const ru_new[3], hipStream_t s);
+
 // ---- box pressure solve (fft_poisson.hip) -------------------------------------
 // fr_box_poisson_solve with, when inv is set, Rho_face passed in place of C
 // and C = 1./Rho_face formed inside the residual (grid.cpp:641).
 int box_poisson_solve(fr_box_poisson* s, const double* cx, const double* cy, const double* cz,
                       bool inv, double* P, const double* RHS, int iterations, double epsilon,
                       int* num_iter, hipStream_t st);
+
+// fr_duct_poisson_solve with the same Rho_face option (duct_poisson.hip).
+int duct_poisson_solve(fr_duct_poisson* s, const double* cx, const double* cy, const double* cz,
+                       bool inv, double* P, const double* RHS, int iterations, double epsilon,
+                       int* num_iter, hipStream_t st);
 
 // ---- diagnostics (diag.hip) ----------------------------------------------------------
 int diag_tke(const Dev& d, const double* u, const double* rho, const double* red,
@@ -116,6 +142,8 @@ class PartDev {
   int *evt = nullptr, *islo = nullptr, *plo = nullptr, *dst = nullptr, *mslot = nullptr,
       *mlv = nullptr;
   int* hbuf = nullptr;
+  double* inj_buf = nullptr;
+  int inj_cap = 0;
   long moved_last = 0;
 
   int alloc(int capacity, const Dev& d);
@@ -129,8 +157,12 @@ class PartDev {
   int deposit_cic(const Dev& d, bool weighted, double* out, bool halo, hipStream_t st);
   // momentum deposit from T
   int deposit_vel(const Dev& d, double* const out[3], hipStream_t st);
-  int migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st);
+  // periodic migration (box); with duct set, x leavers are dropped and the
+  // n_in inlet particles of `inj` (29 values each, host memory) appended
+  int migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st,
+              bool duct = false, const double* inj = nullptr, int n_in = 0);
   void swap_stages(int a, int b);
+  void swap_storage();  // exchange the storage set with `alt`
   int copy_stage(int dst_stage, int src_stage, hipStream_t st);
   // storage order: identity, or re-sorted by cell (values unchanged)
   int reset_order(hipStream_t st);

@@ -561,6 +561,53 @@ __global__ void k_mig_apply(int np, int nleave, PartPtrs p, MigAxis ax,
   }
 }
 
+// ---- duct inlet/outlet ----------------------------------------------------------------
+__global__ void k_live(int np, const int* __restrict__ rank, int K, int* __restrict__ live) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) live[i] = rank[i] < K;
+}
+
+// Slots with live[i] move to slot pos[i] of the second storage set.
+__global__ void k_compact(int np, const int* __restrict__ live, const int* __restrict__ pos,
+                          PartPtrs a, const int* __restrict__ ra, PartPtrs b,
+                          int* __restrict__ rb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np || !live[i]) return;
+  const int u = pos[i];
+#pragma unroll
+  for (int q = 0; q < 7; q++)
+#pragma unroll
+    for (int t = 0; t < 4; t++) b.s[q][t][u] = a.s[q][t][i];
+  b.rhs_u[u] = a.rhs_u[i];
+  b.rhs_v[u] = a.rhs_v[i];
+  b.rhs_w[u] = a.rhs_w[i];
+  b.rhs_T[u] = a.rhs_T[i];
+  b.flag[u] = a.flag[i];
+  rb[u] = ra[i];
+}
+
+// Inlet particle t (ToBuf order: 4 stages of x,y,z,u,v,w,T, then the flag as
+// a double) becomes slot and reference index np0 + t; its RHS values start
+// at zero, as the reference's resized vectors do.
+__global__ void k_inject(int n, int np0, const double* __restrict__ buf, PartPtrs p,
+                         int* __restrict__ rank, int* __restrict__ inv) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int u = np0 + t;
+  const double* b = buf + (long)t * 29;
+#pragma unroll
+  for (int q = 0; q < 7; q++)
+#pragma unroll
+    for (int st = 0; st < 4; st++) p.s[q][st][u] = b[q * 4 + st];
+  p.flag[u] = (int)b[28];
+  p.rhs_u[u] = 0.0;
+  p.rhs_v[u] = 0.0;
+  p.rhs_w[u] = 0.0;
+  p.rhs_T[u] = 0.0;
+  rank[u] = u;
+  inv[u] = u;
+}
+
 // ---- storage order ------------------------------------------------------------------
 __global__ void k_iota2(int n, int* __restrict__ a, int* __restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -665,7 +712,7 @@ void PartDev::release() {
     }
   void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, alt.flag, alt.rhs_u, alt.rhs_v, alt.rhs_w,
                  alt.rhs_T, rank, rank_alt, inv, scan_tmp, flg, nl, pNL, holepos, tailnl,
-                 evt, islo, plo, dst, mslot, mlv};
+                 evt, islo, plo, dst, mslot, mlv, inj_buf};
   for (void* p : all) (void)hipFree(p);
   free_buckets(R);
   free_buckets(T);
@@ -753,10 +800,10 @@ int PartDev::deposit_vel(const Dev& d, double* const out[3], hipStream_t st) {
 // particle::Send_Recv on one rank: axes z, y, x (particle.cpp:853-858).  The
 // event order is computed on reference indices; only the reference indices of
 // the moved particles change, and the leavers' coordinates are wrapped.
-int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st) {
-  if (np == 0) return FR_OK;
-  const unsigned nb = nblocks(np, 256);
-  for (int ax = 2; ax >= 0; ax--) {
+int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st,
+                     bool duct, const double* inj, int n_in) {
+  for (int ax = 2; ax >= 0 && np > 0; ax--) {
+    const unsigned nb = nblocks(np, 256);
     FR_CHECK(hipMemsetAsync(flg, 0, sizeof(int) * np, st));
     hipLaunchKernelGGL(k_mig_flag, dim3(nb), dim3(256), 0, st, np, (const double*)s[ax][2], rank,
                        lo[ax], hi[ax], flg);
@@ -781,10 +828,53 @@ int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], 
     hipLaunchKernelGGL(k_mig_apply, dim3(ng), dim3(256), 0, st, np, nleave, ptrs(), m, pNL,
                        mslot, dst, mlv, rank, inv);
     FR_LAUNCH_CHECK();
+    if (duct && ax == 0) {
+      // the leavers now hold reference indices K..np-1 and are not received
+      // back: keep the slots with rank < K, in storage order
+      hipLaunchKernelGGL(k_live, dim3(nb), dim3(256), 0, st, np, rank, K, nl);
+      rc = scan_ex(nl, np, pNL, scan_tmp, st);
+      if (rc) return rc;
+      PartPtrs a = ptrs(), b = alt;
+      hipLaunchKernelGGL(k_compact, dim3(nb), dim3(256), 0, st, np, nl, pNL, a, rank, b,
+                         rank_alt);
+      FR_LAUNCH_CHECK();
+      swap_storage();
+      np = K;
+      if (np > 0)
+        hipLaunchKernelGGL(k_inv, dim3(nblocks(np, 256)), dim3(256), 0, st, np, rank, inv);
+      FR_LAUNCH_CHECK();
+    }
+  }
+  if (duct && n_in > 0) {  // FromBuf of the inlet particles (particle.cpp:286-301)
+    if (np + n_in > cap) return FR_EOVERFLOW;
+    if (n_in > inj_cap) {
+      (void)hipFree(inj_buf);
+      inj_buf = nullptr;
+      FR_CHECK(hipMalloc(&inj_buf, sizeof(double) * 29 * (size_t)n_in));
+      inj_cap = n_in;
+    }
+    FR_CHECK(hipMemcpyAsync(inj_buf, inj, sizeof(double) * 29 * (size_t)n_in,
+                            hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_inject, dim3(nblocks(n_in, 256)), dim3(256), 0, st, n_in, np,
+                       (const double*)inj_buf, ptrs(), rank, inv);
+    FR_LAUNCH_CHECK();
+    np += n_in;
+    FR_CHECK(hipStreamSynchronize(st));  // the host buffer may be reused
   }
   R.valid = false;
   T.valid = false;
   return FR_OK;
+}
+
+void PartDev::swap_storage() {
+  for (int q = 0; q < 7; q++)
+    for (int t = 0; t < 4; t++) std::swap(s[q][t], alt.s[q][t]);
+  std::swap(flag, alt.flag);
+  std::swap(rhs_u, alt.rhs_u);
+  std::swap(rhs_v, alt.rhs_v);
+  std::swap(rhs_w, alt.rhs_w);
+  std::swap(rhs_T, alt.rhs_T);
+  std::swap(rank, rank_alt);
 }
 
 void PartDev::swap_stages(int a, int b) {
@@ -818,14 +908,7 @@ int PartDev::resort(const Dev& d, hipStream_t st) {
   hipLaunchKernelGGL(k_resort, dim3(nblocks(np, 256)), dim3(256), 0, st, np, T.list, inv, a, rank,
                      b, rank_alt);
   FR_LAUNCH_CHECK();
-  for (int q = 0; q < 7; q++)
-    for (int t = 0; t < 4; t++) std::swap(s[q][t], alt.s[q][t]);
-  std::swap(flag, alt.flag);
-  std::swap(rhs_u, alt.rhs_u);
-  std::swap(rhs_v, alt.rhs_v);
-  std::swap(rhs_w, alt.rhs_w);
-  std::swap(rhs_T, alt.rhs_T);
-  std::swap(rank, rank_alt);
+  swap_storage();
   hipLaunchKernelGGL(k_inv, dim3(nblocks(np, 256)), dim3(256), 0, st, np, rank, inv);
   FR_LAUNCH_CHECK();
   T.valid = false;

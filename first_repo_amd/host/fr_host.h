@@ -197,16 +197,22 @@ class particle {
   int part2gas_concentration(tensor0& A);       // particle.cpp:526-559
   int part2gas_velocity(tensor1& A);            // particle.cpp:498-524
   int Send_Recv();                              // particle.cpp:853-858
+  // duct rank that is inlet and outlet: Send_Recv(particle& pbox)
+  // (particle.cpp:860-865, 814-851) with the box's Rbuf1 given as n inlet
+  // particles of 29 values each
+  int Send_Recv_duct(const double* inlet, int n);
   // gas2part_Temp_{new,int} + Compute_RHS_Temp_{new,int} (402-478)
   int temp(int stage, double c0, const tensor0& Rho);
   int part2gas_Temp(int stage, tensor0& A);     // particle.cpp:586-658
   int rebuild_cells(int stage);
 };
 
-// poisson (poisson.h:19-81), FFT solver branch.
+// poisson (poisson.h:19-81), FFT solver branch, and poisson_IO
+// (poisson_IO.h:20-88) for the duct.
 class poisson {
  public:
   fr_box_poisson* s = nullptr;
+  fr_duct_poisson* sd = nullptr;
   params* par = nullptr;
   int num_iteration_ = 0;
   hipStream_t st_ = 0;
@@ -219,7 +225,9 @@ class poisson {
   int num_iteration() const { return num_iteration_; }
 };
 
-// grid (grid.h:16-115): the box time-step methods.
+// grid (grid.h:16-115): the box and duct time-step methods.  For the duct
+// the box grid g of the reference's grid::*(grid& g) methods is replaced by
+// the inflow planes and inlet particles set with set_inflow.
 class grid {
  public:
   params param;
@@ -229,7 +237,10 @@ class grid {
   double mean_energy_transferred = 0, C_BC = 0;
   particle part;
   tensor1 RU, RU_int, RU_new, RU_np1, RHS_RU, U, Rho_face, dummy;
-  tensor0 P, RHS_Pois, C, Rho, Rho_int, Rho_new, Rho_np1, RHS_Part_Temp, scratch;
+  tensor0 P, RHS_Pois, C, Rho, Rho_int, Rho_new, Rho_np1, RHS_Part_Temp, scratch, divergence;
+  double U_avg_in = 0, U_avg_out = 0;
+  double* inflow = nullptr;    // duct: 8 ghosted [Nz][Ny] planes (device)
+  std::vector<double> inlet;   // duct: inlet particles not yet received
   double P0 = 0, P0_int = 0, P0_new = 0, P0_np1 = 0, dP0_dt = 0, T_cur = 0;
   int num_timestep = 0, RK4_count = 0;
   poisson PS_;
@@ -249,7 +260,13 @@ class grid {
   grid() {}
   ~grid();
   int setup(const char* params_path, const char* ic_dir_);
+  int reserve(int capacity);  // particle capacity (before Initialize)
   int Initialize();
+  int Finish_Initialize();  // grid.cpp:197-229
+  // duct inflow: rho_new, rho_np1, RU_new x,y,z, RU_np1 x,y,z planes of the
+  // box (grid.cpp:351-364, 482-504) and n inlet particles (29 values each)
+  int set_inflow(const double* planes, int n, const double* particles);
+  int Correct_OutFlow();  // grid.cpp:579-627
   int Update_Rho();
   int Update_P0();
   int Update_Particle();
@@ -261,7 +278,7 @@ class grid {
   int TimeAdvance_RK4();
   int Correct_mean_RU();
   int TimeAdvance();
-  int step();  // simulation.cpp:36-70, box calls
+  int step();  // simulation.cpp:36-70, box or duct (GRID_IO) calls
   int diagnostics(double* out);
   tensor0* field0(const std::string& name);
   tensor1* field1(const std::string& name);

@@ -288,6 +288,15 @@ int particle::Send_Recv() {
   return dev.migrate(lo, hi, L, st_);
 }
 
+int particle::Send_Recv_duct(const double* inlet, int n) {
+  const double lo[3] = {size_->xl_, size_->yl_, size_->zl_};
+  const double hi[3] = {size_->xh_, size_->yh_, size_->zh_};
+  const double L[3] = {size_->Lx(), size_->Ly(), size_->Lz()};
+  int rc = dev.migrate(lo, hi, L, st_, true, inlet, n);
+  Np = dev.np;
+  return rc;
+}
+
 int particle::temp(int stage, double c0, const tensor0& Rho) {
   const double iTth = param_->Nu() * PI * param_->Dp() * param_->k();
   return dev.temp(size_->dev(), stage, c0, Rho.pointer(), iTth, st_);
@@ -305,15 +314,26 @@ int particle::rebuild_cells(int stage) { return dev.build(size_->dev(), stage, 0
 int poisson::init(params* p, gridsize* g, hipStream_t st) {
   par = p;
   st_ = st;
-  return fr_box_poisson_create(&g->g, &s);
+  return g->IsBox() ? fr_box_poisson_create(&g->g, &s) : fr_duct_poisson_create(&g->g, &sd);
 }
-poisson::~poisson() { fr_box_poisson_destroy(s); }
+poisson::~poisson() {
+  fr_box_poisson_destroy(s);
+  fr_duct_poisson_destroy(sd);
+}
 int poisson::Solve(tensor1& C, tensor0& P, tensor0& RHS) {
+  if (sd)
+    return fr_duct_poisson_solve(sd, C.x.pointer(), C.y.pointer(), C.z.pointer(), P.pointer(),
+                                 RHS.pointer(), par->Iteration(), par->epsilon(),
+                                 &num_iteration_, st_);
   return fr_box_poisson_solve(s, C.x.pointer(), C.y.pointer(), C.z.pointer(), P.pointer(),
                               RHS.pointer(), par->Iteration(), par->epsilon(),
                               &num_iteration_, st_);
 }
 int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS) {
+  if (sd)
+    return duct_poisson_solve(sd, Rho_face.x.pointer(), Rho_face.y.pointer(),
+                              Rho_face.z.pointer(), true, P.pointer(), RHS.pointer(),
+                              par->Iteration(), par->epsilon(), &num_iteration_, st_);
   return box_poisson_solve(s, Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer(),
                            true, P.pointer(), RHS.pointer(), par->Iteration(), par->epsilon(),
                            &num_iteration_, st_);
@@ -331,6 +351,7 @@ int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS) {
 
 grid::~grid() {
   if (dred) hipFree(dred);
+  if (inflow) hipFree(inflow);
   for (auto& e : ev) hipEventDestroy(e);
   for (auto& e : evk) hipEventDestroy(e);
   if (st) hipStreamDestroy(st);
@@ -339,7 +360,6 @@ grid::~grid() {
 int grid::setup(const char* params_path, const char* ic) {
   param = params(params_path);
   if (!param.ok) return FR_EINVAL;
-  if (!param.IsBox()) return FR_EINVAL;  // the duct path is not built here
   if (param.bs() != 1) return FR_EINVAL;
   size = gridsize(param);
   ic_dir = ic ? ic : ".";
@@ -358,9 +378,14 @@ int grid::setup(const char* params_path, const char* ic) {
   for (tensor1* f : {&RU, &RU_int, &RU_new, &RU_np1, &RHS_RU, &U, &Rho_face, &dummy})
     f->init(&size, st);
   for (tensor0* f : {&P, &RHS_Pois, &C, &Rho, &Rho_int, &Rho_new, &Rho_np1, &RHS_Part_Temp,
-                     &scratch})
+                     &scratch, &divergence})
     f->init(&size, st);
-  if (!scratch.pointer()) return FR_ENOMEM;
+  if (!divergence.pointer()) return FR_ENOMEM;
+  if (!param.IsBox()) {
+    const size_t m = (size_t)size.Ny() * size.Nz();
+    FR_CHECK(hipMalloc(&inflow, sizeof(double) * 8 * m));
+    FR_CHECK(hipMemset(inflow, 0, sizeof(double) * 8 * m));
+  }
   FR_TRY(part.init(&param, &size, st, param.N0() > 0 ? param.N0() : 1));
   FR_TRY(PS_.init(&param, &size, st));
   FR_CHECK(hipMalloc(&dred, sizeof(double) * (4 * (size_t)size.Nx() + 512)));
@@ -384,7 +409,14 @@ static int read_block(const std::string& path, const gridsize& s, std::vector<do
   return FR_OK;
 }
 
-// grid.cpp:110-231 (Initial = 0 and 2 for the box).
+int grid::reserve(int capacity) {
+  part.dev.release();
+  part.dev = PartDev();
+  part.Np = 0;
+  return part.dev.alloc(capacity, size.dev());
+}
+
+// grid.cpp:110-231 (Initial = 0 and 2); particles are loaded in the box only.
 int grid::Initialize() {
   if (param.Initial() == 0) {
     const char* names[3] = {"U.bin", "V.bin", "W.bin"};
@@ -403,10 +435,20 @@ int grid::Initialize() {
   }
   Rho.assign(param.Rho0());
   P.assign(0);
-  FR_TRY(part.load_random());
+  if (param.IsBox()) {
+    FR_TRY(part.load_random());
+  } else {
+    part.Np = part.dev.np = 0;
+  }
   P0 = param.P0();
   T_cur = 0;
   num_timestep = 0;
+  return Finish_Initialize();
+}
+
+// grid.cpp:197-229: the time-integration stages, Rho_face, the particle
+// stages and the first particle heat source, and C_BC.
+int grid::Finish_Initialize() {
   RU_int.copy(RU);
   RU_np1.copy(RU);
   Rho_int.copy(Rho);
@@ -431,6 +473,15 @@ int grid::Initialize() {
   return FR_OK;
 }
 
+int grid::set_inflow(const double* planes, int n, const double* particles) {
+  if (param.IsBox() || n < 0) return FR_EINVAL;
+  if (planes)
+    FR_CHECK(hipMemcpy(inflow, planes, sizeof(double) * 8 * (size_t)size.Ny() * size.Nz(),
+                       hipMemcpyHostToDevice));
+  if (n > 0 && particles) inlet.insert(inlet.end(), particles, particles + (size_t)n * 29);
+  return FR_OK;
+}
+
 static BoxCoef box_coef(const grid& g) {
   BoxCoef c;
   const double dt = g.param.dt(), pre = g.RK4_preCoeff[g.RK4_count],
@@ -449,6 +500,13 @@ int grid::Update_Rho() {
   const double* ri[3] = {RU_int.x.pointer(), RU_int.y.pointer(), RU_int.z.pointer()};
   const double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
   double* u[3] = {U.x.pointer(), U.y.pointer(), U.z.pointer()};
+  if (!param.IsBox()) {  // grid.cpp:335-385
+    const size_t m = (size_t)size.Ny() * size.Nz();
+    FR_TRY(duct_update_rho(size.dev(), box_coef(*this), C_BC, ri, Rho.pointer(),
+                           Rho_int.pointer(), Rho_np1.pointer(), Rho_new.pointer(), rf, u,
+                           inflow, inflow + m, st));
+    return FR_OK;
+  }
   FR_TRY(box_update_rho(size.dev(), box_coef(*this), ri, Rho.pointer(), Rho_np1.pointer(),
                         Rho_new.pointer(), rf, u, st));
   return FR_OK;
@@ -488,7 +546,12 @@ int grid::Update_Particle() {  // grid.cpp:659-680
     FR_TRY(part.part2gas_velocity(RHS_RU));
   else
     RHS_RU.assign(0);
-  FR_TRY(part.Send_Recv());
+  if (param.IsBox()) {
+    FR_TRY(part.Send_Recv());
+  } else {
+    FR_TRY(part.Send_Recv_duct(inlet.data(), (int)(inlet.size() / 29)));
+    inlet.clear();
+  }
   FR_TRY(part.temp(2, P0_new / param.R(), Rho_new));
   FR_TRY(part.rebuild_cells(2));
   FR_TRY(part.part2gas_Temp(2, RHS_Part_Temp));
@@ -521,30 +584,77 @@ int grid::Update_RU_WOP() {  // grid.cpp:387-425
   double* nw[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
   const double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
   hipEventRecord(evk[2 * (RK4_count & 3)], st);
-  FR_TRY(box_update_ru_wop(size.dev(), box_coef(*this), w, u, ri, S, r0, n1, nw, rf, st));
+  if (param.IsBox()) {
+    FR_TRY(box_update_ru_wop(size.dev(), box_coef(*this), w, u, ri, S, r0, n1, nw, rf, st));
+  } else {  // grid.cpp:427-537
+    const size_t m = (size_t)size.Ny() * size.Nz();
+    FR_TRY(duct_update_ru_wop(size.dev(), box_coef(*this), w, C_BC, u, ri, S, r0, n1, nw,
+                              inflow + 2 * m, inflow + 5 * m, st));
+  }
   hipEventRecord(evk[2 * (RK4_count & 3) + 1], st);
   return FR_OK;
 }
 
 // grid.cpp:556-563 is evaluated inside the Compute_RHS_Pois kernel, where its
 // only consumer is; this call has nothing to launch.
-int grid::Compute_Div_U_new() { return FR_OK; }
+static DivCoef div_coef(grid& g) {
+  DivCoef q;
+  q.kc = g.param.k() / g.param.Cp();
+  q.bq = -g.param.R() / (g.param.Cp() * g.P0_new * g.size.Vcell());
+  q.cooling = g.param.Is_Cooling();
+  if (q.cooling && !g.qmean_valid) {
+    g.qmean = g.RHS_Part_Temp.mean();
+    g.qmean_valid = true;
+  }
+  q.qmean = g.qmean;
+  q.s = (1. / (g.param.dt() * g.RK4_preCoeff[g.RK4_count]));
+  return q;
+}
+
+// Box: grid.cpp:556-563 is evaluated inside the Compute_RHS_Pois kernel,
+// where its only consumer is.  Duct: the divergence field is stored (its sum
+// enters Correct_OutFlow) and Rho_face = I_C2F(Rho_new) of grid.cpp:582 is
+// formed in the same pass (nothing reads Rho_face in between).
+int grid::Compute_Div_U_new() {
+  if (param.IsBox()) return FR_OK;
+  double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
+  FR_TRY(duct_div(size.dev(), div_coef(*this), Rho_new.pointer(), RHS_Part_Temp.pointer(), rf,
+                  divergence.pointer(), st));
+  return FR_OK;
+}
+
+// grid.cpp:579-627 for a device that holds inlet and outlet: mass balance of
+// the interior divergence and the inlet/outlet x-face fluxes, outlet RU_new.x
+// rescaled, C_BC = mean outlet velocity.
+int grid::Correct_OutFlow() {
+  if (param.IsBox()) return FR_OK;
+  const Dev d = size.dev();
+  FR_TRY(reduce_async(d, divergence.pointer(), 0, dred, st));
+  FR_TRY(duct_planes(d, RU_new.x.pointer(), Rho_face.x.pointer(), dred + 1, st));
+  double h[3];
+  FR_CHECK(hipMemcpyAsync(h, dred, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+  FR_CHECK(hipStreamSynchronize(st));
+  const double sum_div = h[0] * size.dx();
+  const double sum_out = h[1], sum_in = h[2];
+  const double ratio = (sum_div + sum_in) / sum_out;
+  U_avg_in = sum_in / (size.Ny_tot_ * size.Nz_tot_);
+  U_avg_out = sum_out / (size.Ny_tot_ * size.Nz_tot_);
+  C_BC = U_avg_out;
+  FR_TRY(duct_scale(d, ratio, RK4_count == 3, RU_new.x.pointer(), RU_np1.x.pointer(), st));
+  return FR_OK;
+}
 
 int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
-  DivCoef q;
-  q.kc = param.k() / param.Cp();
-  q.bq = -param.R() / (param.Cp() * P0_new * size.Vcell());
-  q.cooling = param.Is_Cooling();
-  if (q.cooling && !qmean_valid) {
-    qmean = RHS_Part_Temp.mean();
-    qmean_valid = true;
-  }
-  q.qmean = qmean;
-  q.s = (1. / (param.dt() * RK4_preCoeff[RK4_count]));
+  const DivCoef q = div_coef(*this);
   const double* rn[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
   double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
-  FR_TRY(box_div_rhs(size.dev(), q, Rho_new.pointer(), RHS_Part_Temp.pointer(), rn, rf,
-                     RHS_Pois.pointer(), st));
+  if (param.IsBox()) {
+    FR_TRY(box_div_rhs(size.dev(), q, Rho_new.pointer(), RHS_Part_Temp.pointer(), rn, rf,
+                       RHS_Pois.pointer(), st));
+  } else {
+    const double* rfc[3] = {rf[0], rf[1], rf[2]};
+    FR_TRY(duct_rhs(size.dev(), q.s, rn, rfc, divergence.pointer(), RHS_Pois.pointer(), st));
+  }
   RHS_Pois.make_mean_zero();
   return FR_OK;
 }
@@ -565,7 +675,10 @@ int grid::Solve_Poisson() {  // grid.cpp:639-643
 int grid::Update_RU_WP() {  // grid.cpp:651-657
   double* n1[3] = {RU_np1.x.pointer(), RU_np1.y.pointer(), RU_np1.z.pointer()};
   double* nw[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
-  FR_TRY(box_update_ru_wp(size.dev(), box_coef(*this), P.pointer(), n1, nw, st));
+  if (param.IsBox())
+    FR_TRY(box_update_ru_wp(size.dev(), box_coef(*this), P.pointer(), n1, nw, st));
+  else
+    FR_TRY(duct_update_ru_wp(size.dev(), box_coef(*this), P.pointer(), n1, nw, st));
   return FR_OK;
 }
 
@@ -580,8 +693,13 @@ int grid::TimeAdvance_RK4() {
 }
 
 int grid::Correct_mean_RU() {  // grid.cpp:565-577
-  RU_np1.make_mean_zero();
-  RU_np1.x.add(param.Rho0() * param.U0());
+  if (param.IsBox()) {
+    RU_np1.make_mean_zero();
+    RU_np1.x.add(param.Rho0() * param.U0());
+  } else {
+    RU_np1.y.make_mean_zero();
+    RU_np1.z.make_mean_zero();
+  }
   return FR_OK;
 }
 
@@ -606,6 +724,7 @@ int grid::step() {  // simulation.cpp:36-70 (box calls)
     FR_TRY(Update_Particle());
     FR_TRY(Update_RU_WOP());
     FR_TRY(Compute_Div_U_new());
+    FR_TRY(Correct_OutFlow());  // duct only
     FR_TRY(Compute_RHS_Pois());
     FR_TRY(Solve_Poisson());
     FR_TRY(Update_RU_WP());
@@ -674,6 +793,7 @@ tensor0* grid::field0(const std::string& n) {
   if (n == "Rho_new") return &Rho_new;
   if (n == "Rho_np1") return &Rho_np1;
   if (n == "RHS_Part_Temp") return &RHS_Part_Temp;
+  if (n == "divergence") return &divergence;
   return nullptr;
 }
 
@@ -731,12 +851,21 @@ int fr_sim_call(fr_sim* s, const char* m, int k) {
   else if (n == "Update_RU_WOP") rc = g.Update_RU_WOP();
   else if (n == "Compute_Div_U_new") rc = g.Compute_Div_U_new();
   else if (n == "Compute_RHS_Pois") rc = g.Compute_RHS_Pois();
-  else if (n == "Solve_Poisson") rc = g.Solve_Poisson();
+  else if (n == "Solve_Poisson" || n == "Solve_Poisson_IO") rc = g.Solve_Poisson();
+  else if (n == "Correct_OutFlow") rc = g.Correct_OutFlow();
+  else if (n == "Finish_Initialize") rc = g.Finish_Initialize();
   else if (n == "Update_RU_WP") rc = g.Update_RU_WP();
   else if (n == "TimeAdvance_RK4") rc = g.TimeAdvance_RK4();
   else if (n == "Correct_mean_RU") rc = g.Correct_mean_RU();
   else if (n == "TimeAdvance") rc = g.TimeAdvance();
-  else if (n == "Send_Recv") rc = g.part.Send_Recv();
+  else if (n == "Send_Recv") {
+    if (g.param.IsBox()) {
+      rc = g.part.Send_Recv();
+    } else {
+      rc = g.part.Send_Recv_duct(g.inlet.data(), (int)(g.inlet.size() / 29));
+      g.inlet.clear();
+    }
+  }
   else return FR_EINVAL;
   if (!rc) rc = (int)hipStreamSynchronize(g.st);
   return rc;
@@ -774,6 +903,8 @@ static double* scal(fr_sim* s, const char* name) {
   if (n == "P0_np1") return &g.P0_np1;
   if (n == "T_cur") return &g.T_cur;
   if (n == "C_BC") return &g.C_BC;
+  if (n == "U_avg_in") return &g.U_avg_in;
+  if (n == "U_avg_out") return &g.U_avg_out;
   return nullptr;
 }
 
@@ -854,6 +985,16 @@ int fr_sim_particles_changed(fr_sim* s) {
 int fr_sim_diagnostics(fr_sim* s, double* out) {
   if (!s || !out) return FR_EINVAL;
   return s->g.diagnostics(out);
+}
+
+int fr_sim_set_inflow(fr_sim* s, const double* planes, int n_inlet, const double* inlet) {
+  if (!s) return FR_EINVAL;
+  return s->g.set_inflow(planes, n_inlet, inlet);
+}
+
+int fr_sim_reserve_particles(fr_sim* s, int capacity) {
+  if (!s || capacity < 1) return FR_EINVAL;
+  return s->g.reserve(capacity);
 }
 
 int fr_sim_timing(fr_sim* s, double* out) {

@@ -152,11 +152,11 @@ int fr_sim_step(fr_sim* s);
 int fr_sim_call(fr_sim* s, const char* method, int rk4_count);
 /* Field transfer: comp -1 for a tensor0, 0..2 for a tensor1 component.
  * Names: RU RU_int RU_new RU_np1 RHS_RU U Rho_face (tensor1); P RHS_Pois C
- * Rho Rho_int Rho_new Rho_np1 RHS_Part_Temp (tensor0). */
+ * Rho Rho_int Rho_new Rho_np1 RHS_Part_Temp divergence (tensor0). */
 int fr_sim_get_field(fr_sim* s, const char* name, int comp, double* host);
 int fr_sim_set_field(fr_sim* s, const char* name, int comp,
                      const double* host);
-/* Scalars: P0 P0_int P0_new P0_np1 T_cur C_BC */
+/* Scalars: P0 P0_int P0_new P0_np1 T_cur C_BC U_avg_in U_avg_out */
 int fr_sim_get_scalar(fr_sim* s, const char* name, double* value);
 int fr_sim_set_scalar(fr_sim* s, const char* name, double value);
 int fr_sim_geom(fr_sim* s, fr_geom* out);
@@ -168,6 +168,20 @@ int fr_sim_set_particles(fr_sim* s, int np, int q, int st,
                          const double* host);
 int fr_sim_get_flags(fr_sim* s, int* host);
 int fr_sim_set_flags(fr_sim* s, int np, const int* host);
+/* Duct (is_box = 0) inflow, the box grid's part in grid::Update_Rho(grid&),
+ * Update_RU_WOP(grid&) and Update_Particle(grid&) (grid.cpp:351-364,
+ * 482-504; particle.cpp:814-851): `planes` holds 8 ghosted planes of
+ * Ny*Nz doubles (j fastest) -- g.Rho_new(1,j,k), g.Rho_np1(1,j,k),
+ * g.RU_new.x(2,j,k), g.RU_new.y(1,j,k), g.RU_new.z(1,j,k), then the same
+ * three RU_np1 planes -- read by the following substeps (NULL keeps the
+ * current planes); `inlet` holds n_inlet particles entering at the inlet
+ * (the box's Rbuf1), 29 doubles each in particle::ToBuf order
+ * (particle.cpp:252-262), appended at the next duct migration. */
+int fr_sim_set_inflow(fr_sim* s, const double* planes, int n_inlet,
+                      const double* inlet);
+/* Particle capacity of the solver (default: the params.in particle count);
+ * call before fr_sim_initialize.  Exceeding it fails with FR_EOVERFLOW. */
+int fr_sim_reserve_particles(fr_sim* s, int capacity);
 /* Rebuild the particle cell lists after particle positions were set. */
 int fr_sim_particles_changed(fr_sim* s);
 /* Step diagnostics (grid.cpp:714-811 subset): out[0..9] = TKE, Rho_Mean,

@@ -207,17 +207,40 @@ void orc_grid_initialize(void* h, const double* U, const double* V,
   ((Grid*)h)->Initialize(U, V, W);
 }
 void orc_grid_finish_initialize(void* h) { ((Grid*)h)->Finish_Initialize(); }
-void orc_grid_step(void* h) { ((Grid*)h)->box_step(); }
+void orc_grid_step(void* h) {
+  Grid* g = (Grid*)h;
+  if (g->par.IsBox)
+    g->box_step();
+  else
+    g->duct_step();
+}
+// Duct inflow: planes = 8 ghosted [Nz][Ny] planes in the order rho_new,
+// rho_np1, ru_new x,y,z, ru_np1 x,y,z; pbuf = n_in particles x 29 values
+// (appended to any not yet consumed).
+void orc_grid_set_inflow(void* h, const double* planes, int n_in, const double* pbuf) {
+  Grid* g = (Grid*)h;
+  const size_t m = (size_t)g->g.Ny * g->g.Nz;
+  std::vector<double>* dst[8] = {&g->in.rho_new,   &g->in.rho_np1,   &g->in.ru_new[0],
+                                 &g->in.ru_new[1], &g->in.ru_new[2], &g->in.ru_np1[0],
+                                 &g->in.ru_np1[1], &g->in.ru_np1[2]};
+  if (planes)
+    for (int q = 0; q < 8; q++) dst[q]->assign(planes + q * m, planes + (q + 1) * m);
+  if (pbuf && n_in > 0) g->in.pbuf.insert(g->in.pbuf.end(), pbuf, pbuf + (size_t)n_in * 29);
+}
 // Run one named grid method (the names of grid.h:93-114).
 int orc_grid_call(void* h, const char* name, int rk4_count) {
   Grid* g = (Grid*)h;
   std::string n(name);
   g->RK4_count = rk4_count;
+  const bool duct = !g->par.IsBox;
   if (n == "update") g->par.update(g->T_cur);
-  else if (n == "Update_Rho") g->Update_Rho();
+  else if (n == "Update_Rho") duct ? g->Update_Rho_duct() : g->Update_Rho();
   else if (n == "Update_P0") g->Update_P0();
-  else if (n == "Update_Particle") g->Update_Particle();
-  else if (n == "Update_RU_WOP") g->Update_RU_WOP();
+  else if (n == "Update_Particle") duct ? g->Update_Particle_duct() : g->Update_Particle();
+  else if (n == "Update_RU_WOP") duct ? g->Update_RU_WOP_duct() : g->Update_RU_WOP();
+  else if (n == "Correct_OutFlow" && duct) g->Correct_OutFlow();
+  else if (n == "Finish_Initialize") g->Finish_Initialize();
+  else if (n == "Solve_Poisson_IO" && duct) g->Solve_Poisson_IO();
   else if (n == "Compute_Div_U_new") g->Compute_Div_U_new();
   else if (n == "Compute_RHS_Pois") g->Compute_RHS_Pois();
   else if (n == "Solve_Poisson") g->Solve_Poisson();
@@ -225,7 +248,7 @@ int orc_grid_call(void* h, const char* name, int rk4_count) {
   else if (n == "TimeAdvance_RK4") g->TimeAdvance_RK4();
   else if (n == "Correct_mean_RU") g->Correct_mean_RU();
   else if (n == "TimeAdvance") g->TimeAdvance();
-  else if (n == "Send_Recv") g->part.Send_Recv();
+  else if (n == "Send_Recv") duct ? g->part.Send_Recv_duct(g->in.pbuf) : g->part.Send_Recv();
   else return -1;
   return 0;
 }

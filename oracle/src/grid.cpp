@@ -243,6 +243,172 @@ void Grid::box_step() {
   TimeAdvance();
 }
 
+// ---- duct (IsBox = 0): the box grid g of the reference is the Inflow `in`
+
+// grid.cpp:335-385 for a rank that is inlet and outlet.
+void Grid::Update_Rho_duct() {
+  const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+  RHS_Rho.Equal_Div_F2C(RU_int);
+  for (int j = 0; j < Ny; j++)
+    for (int k = 0; k < Nz; k++) RHS_Rho(Nx - 1, j, k) = 0;
+  Rho_np1.PlusEqual_Mult(-(par.dt * RK4_post[RK4_count]), RHS_Rho);
+  if (RK4_count != 3)
+    Rho_new.Equal_LinComb(1, Rho, -par.dt * RK4_pre[RK4_count], RHS_Rho);
+  else
+    Rho_new.copy(Rho_np1);
+  for (int j = 0; j < Ny; j++)
+    for (int k = 0; k < Nz; k++) Rho_new(0, j, k) = in.rho_new[(size_t)k * Ny + j];
+  if (RK4_count == 3)
+    for (int j = 0; j < Ny; j++)
+      for (int k = 0; k < Nz; k++) Rho_np1(0, j, k) = in.rho_np1[(size_t)k * Ny + j];
+  const double idx = 1. / g.dx;
+  const int i = Nx - 1;
+  for (int j = 0; j < Ny; j++)
+    for (int k = 0; k < Nz; k++)
+      Rho_np1(i, j, k) -=
+          (par.dt * RK4_post[RK4_count]) * C_BC * (Rho_int(i, j, k) - Rho_int(i - 1, j, k)) * idx;
+  if (RK4_count != 3)
+    for (int j = 0; j < Ny; j++)
+      for (int k = 0; k < Nz; k++)
+        Rho_new(i, j, k) = Rho(i, j, k) - par.dt * RK4_pre[RK4_count] * C_BC *
+                                              (Rho_int(i, j, k) - Rho_int(i - 1, j, k)) * idx;
+  else
+    for (int j = 0; j < Ny; j++)
+      for (int k = 0; k < Nz; k++) Rho_new(i, j, k) = Rho_np1(i, j, k);
+}
+
+// grid.cpp:427-537: no forcing or gravity; RHS zero on the outlet slice;
+// inflow copies; convective outflow.
+void Grid::Update_RU_WOP_duct() {
+  const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+  divergence.Equal_Div_F2C(U);
+  dummy2.Equal_Grad_C2F(divergence);
+  dummy.Equal_Del2(U);
+  RHS_RU.Equal_LinComb(-par.mp / g.Vcell(), RHS_RU, par.Mu0 / 3., dummy2, par.Mu0, dummy);
+  dummy.Equal_I_C2F(U.x);
+  dummy2.Equal_Ix_C2F(RU_int);
+  dummy.mul(dummy2);
+  dummy2.x.Equal_Div_F2C(dummy);
+  RHS_RU.x.sub(dummy2.x);
+  dummy.Equal_I_C2F(U.y);
+  dummy2.Equal_Iy_C2F(RU_int);
+  dummy.mul(dummy2);
+  dummy2.y.Equal_Div_F2C(dummy);
+  RHS_RU.y.sub(dummy2.y);
+  dummy.Equal_I_C2F(U.z);
+  dummy2.Equal_Iz_C2F(RU_int);
+  dummy.mul(dummy2);
+  dummy2.z.Equal_Div_F2C(dummy);
+  RHS_RU.z.sub(dummy2.z);
+  for (int j = 0; j < Ny; j++)
+    for (int k = 0; k < Nz; k++) {
+      RHS_RU.x(Nx - 1, j, k) = 0;
+      RHS_RU.y(Nx - 1, j, k) = 0;
+      RHS_RU.z(Nx - 1, j, k) = 0;
+    }
+  RU_np1.PlusEqual_Mult(par.dt * RK4_post[RK4_count], RHS_RU);
+  if (RK4_count != 3)
+    RU_new.Equal_LinComb(1, RU, par.dt * RK4_pre[RK4_count], RHS_RU);
+  else
+    RU_new.copy(RU_np1);
+  for (int j = 0; j < Ny; j++)
+    for (int k = 0; k < Nz; k++) {
+      const size_t q = (size_t)k * Ny + j;
+      RU_new.x(1, j, k) = in.ru_new[0][q];
+      RU_new.y(0, j, k) = in.ru_new[1][q];
+      RU_new.z(0, j, k) = in.ru_new[2][q];
+    }
+  if (RK4_count == 3)
+    for (int j = 0; j < Ny; j++)
+      for (int k = 0; k < Nz; k++) {
+        const size_t q = (size_t)k * Ny + j;
+        RU_np1.x(1, j, k) = in.ru_np1[0][q];
+        RU_np1.y(0, j, k) = in.ru_np1[1][q];
+        RU_np1.z(0, j, k) = in.ru_np1[2][q];
+      }
+  const double idx = 1. / g.dx;
+  const int i = Nx - 1;
+  F0* n1[3] = {&RU_np1.x, &RU_np1.y, &RU_np1.z};
+  F0* nw[3] = {&RU_new.x, &RU_new.y, &RU_new.z};
+  const F0* ri[3] = {&RU_int.x, &RU_int.y, &RU_int.z};
+  const F0* r0[3] = {&RU.x, &RU.y, &RU.z};
+  for (int j = 0; j < Ny; j++)
+    for (int k = 0; k < Nz; k++)
+      for (int c = 0; c < 3; c++)
+        (*n1[c])(i, j, k) -= (par.dt * RK4_post[RK4_count]) * C_BC *
+                             ((*ri[c])(i, j, k) - (*ri[c])(i - 1, j, k)) * idx;
+  for (int j = 0; j < Ny; j++)
+    for (int k = 0; k < Nz; k++)
+      for (int c = 0; c < 3; c++)
+        (*nw[c])(i, j, k) = RK4_count != 3
+                                ? (*r0[c])(i, j, k) - par.dt * RK4_pre[RK4_count] * C_BC *
+                                                          ((*ri[c])(i, j, k) - (*ri[c])(i - 1, j, k)) * idx
+                                : (*n1[c])(i, j, k);
+}
+
+// grid.cpp:682-703
+void Grid::Update_Particle_duct() {
+  U.Equal_Divide(RU_int, Rho_face);
+  part.gas2part_velocity(U);
+  part.update_position(RK4_count, RK4_pre[RK4_count], RK4_post[RK4_count]);
+  part.update_velocity(RK4_count, RK4_pre[RK4_count], RK4_post[RK4_count]);
+  part.update_Temp(RK4_count, RK4_pre[RK4_count], RK4_post[RK4_count]);
+  part.part2gas_concentration(dummy.x);
+  C.copy(dummy.x);
+  if (par.TWC)
+    part.part2gas_velocity(RHS_RU);
+  else
+    RHS_RU.set(0);
+  part.Send_Recv_duct(in.pbuf);
+  T.Equal_Divide(P0_new / par.R, Rho_new);
+  part.gas2part_Temp(T, true);
+  part.Compute_RHS_Temp(true);
+  part.part2gas_Temp(RHS_Part_Temp, true);
+}
+
+// grid.cpp:579-627 for a rank that is inlet and outlet (the Reduce/Bcast
+// pairs are identities).
+void Grid::Correct_OutFlow() {
+  const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+  double sum_div = divergence.sum() * g.dx;
+  Rho_face.Equal_I_C2F(Rho_new);
+  U.Equal_Divide(RU_new, Rho_face);
+  double sum_out = 0;
+  for (int j = 1; j < Ny - 1; j++)
+    for (int k = 1; k < Nz - 1; k++) sum_out += U.x(Nx - 1, j, k);
+  double sum_in = 0;
+  for (int j = 1; j < Ny - 1; j++)
+    for (int k = 1; k < Nz - 1; k++) sum_in += U.x(1, j, k);
+  double ratio = (sum_div + sum_in) / sum_out;
+  U_avg_in = sum_in / (g.Ny_tot * g.Nz_tot);
+  U_avg_out = sum_out / (g.Ny_tot * g.Nz_tot);
+  C_BC = U_avg_out;
+  for (int j = 0; j < Ny; j++)
+    for (int k = 0; k < Nz; k++) RU_new.x(Nx - 1, j, k) *= ratio;
+  if (RK4_count == 3)
+    for (int j = 0; j < Ny; j++)
+      for (int k = 0; k < Nz; k++) RU_np1.x(Nx - 1, j, k) *= ratio;
+}
+
+// simulation.cpp:36-70, the GRID_IO calls, with the box replaced by `in`.
+void Grid::duct_step() {
+  par.update(T_cur);
+  for (RK4_count = 0; RK4_count < 4; RK4_count++) {
+    Update_Rho_duct();
+    Update_P0();
+    Update_Particle_duct();
+    Update_RU_WOP_duct();
+    Compute_Div_U_new();
+    Correct_OutFlow();
+    Compute_RHS_Pois();
+    Solve_Poisson_IO();
+    Update_RU_WP();
+    TimeAdvance_RK4();
+  }
+  Correct_mean_RU();
+  TimeAdvance();
+}
+
 // grid.cpp:714-811 subset: out = {TKE, Rho_Mean, Rho_Max, Rho_Min, C_Max,
 // C_Min, C_Mean, mean dissipation (x-average of the DISSIPATION profile),
 // mean particle temperature (extension: sum of T in vector order / Np),

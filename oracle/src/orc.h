@@ -306,7 +306,10 @@ struct Particles {  // particle.h:12-155 (one rank)
   void part2gas_concentration(F0& A);                    // particle.cpp:526-559
   void part2gas_Temp(F0& A, bool use_new);               // particle.cpp:586-658
   void Send_Recv();                                      // particle.cpp:853-858
-  void SortValid(int axis);                              // particle.cpp:150-250
+  // duct rank that is inlet and outlet: y,z periodic, x leavers dropped, the
+  // inlet particles appended (particle.cpp:814-851, 860-865); consumes inbuf
+  void Send_Recv_duct(std::vector<double>& inbuf);
+  void SortValid(int axis, bool keep = true);            // particle.cpp:150-250
   void load_random();                                    // particle.cpp:919-958
   void copy_stage(int dst, int src);  // whole-vector stage copies (grid.cpp)
   std::vector<double>* stage(int q, int s);  // q in 0..6 (x..T), s in 0..3
@@ -351,6 +354,18 @@ struct DuctPoisson {  // poisson_IO.h:20-88
   void thomas_line(double* v, long stride, long off, int j, int k) const;
 };
 
+// ---------------------------------------------------------------- duct inflow
+// What the duct copies from the box grid g each substep (grid.cpp:351-364,
+// 482-504), as ghosted [Nz][Ny] planes (j fastest): rho_new = g.Rho_new(1,j,k),
+// rho_np1 = g.Rho_np1(1,j,k), ru_new[0] = g.RU_new.x(2,j,k), ru_new[1..2] =
+// g.RU_new.{y,z}(1,j,k), ru_np1 likewise; and the particles entering through
+// the inlet (the box's Rbuf1, particle.cpp:814-851), 29 values each in ToBuf
+// order (particle.cpp:252-262), consumed by the next duct migration.
+struct Inflow {
+  std::vector<double> rho_new, rho_np1, ru_new[3], ru_np1[3];
+  std::vector<double> pbuf;
+};
+
 // ---------------------------------------------------------------- grid
 struct Grid {  // grid.h:16-115
   Params par;
@@ -365,6 +380,7 @@ struct Grid {  // grid.h:16-115
   int num_timestep = 0, RK4_count = 0;
   BoxPoisson PS;
   DuctPoisson PS_IO;
+  Inflow in;  // duct only
   int last_iterations = 0;
   void setup(const Params& p);
   // Initialize (grid.cpp:110-231), Initial = 0/2 forms; the velocity field of
@@ -384,6 +400,12 @@ struct Grid {  // grid.h:16-115
   void Correct_mean_RU();     // grid.cpp:565-577
   void TimeAdvance();         // grid.cpp:315-325
   void box_step();            // simulation.cpp:34-71, box calls only
+  // duct (IsBox = 0) forms, with the box grid replaced by `in`
+  void Update_Rho_duct();       // grid.cpp:335-385
+  void Update_RU_WOP_duct();    // grid.cpp:427-537
+  void Update_Particle_duct();  // grid.cpp:682-703
+  void Correct_OutFlow();       // grid.cpp:579-627
+  void duct_step();             // simulation.cpp:36-70, duct calls
   // step diagnostics (grid.cpp:714-866 subset) + mean particle T (extension)
   void diagnostics(double* out);
   F0* field0(const std::string& name);

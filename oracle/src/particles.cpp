@@ -261,7 +261,7 @@ void Particles::part2gas_Temp(F0& A, bool use_new) {
 // FromBuf, SWAP) and 745-812 (Send_Recv_*) for one rank: leavers are packed
 // while scanning upward with swap-with-last removal; the '<lo' list
 // (received as Rbuf0) then the '>hi' list (Rbuf1) are appended.
-void Particles::SortValid(int axis) {
+void Particles::SortValid(int axis, bool keep) {
   std::vector<double>* pos_new = axis == 0 ? &x_new : axis == 1 ? &y_new : &z_new;
   double lo = axis == 0 ? g->xl : axis == 1 ? g->yl : g->zl;
   double hi = axis == 0 ? g->xh : axis == 1 ? g->yh : g->zh;
@@ -310,6 +310,7 @@ void Particles::SortValid(int axis) {
     i++;
   }
   RESIZE();
+  if (!keep) return;  // duct x: nothing is received back
   // one rank: Rbuf0 <- Sbuf1 (from the FRONT/TOP/RIGHT neighbour's '<lo'
   // list), Rbuf1 <- Sbuf0; FromBuf appends Rbuf0 then Rbuf1.
   const std::vector<double>* R[2] = {&Sbuf1, &Sbuf0};
@@ -330,6 +331,23 @@ void Particles::Send_Recv() {  // particle.cpp:853-858: Z, then Y, then X
   SortValid(2);
   SortValid(1);
   SortValid(0);
+}
+
+// particle.cpp:860-865 with Send_Recv_X(particle& pbox), 814-851, on a rank
+// that is both inlet and outlet: Rnum0 = 0, Rbuf1 = the box's Rbuf1.
+void Particles::Send_Recv_duct(std::vector<double>& inbuf) {
+  SortValid(2);
+  SortValid(1);
+  SortValid(0, false);
+  for (size_t o = 0; o + 29 <= inbuf.size(); o += 29) {
+    size_t c = o;
+    for (int q = 0; q < 7; q++)
+      for (int s = 0; s < 4; s++) stage(q, s)->push_back(inbuf[c++]);
+    flag.push_back((int)inbuf[c++]);
+    Np++;
+  }
+  RESIZE();
+  inbuf.clear();
 }
 
 // particle.cpp:919-958 on one rank (flag = running index).

@@ -75,6 +75,7 @@ def orc():
         L.orc_grid_initialize.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_grid_finish_initialize.argtypes = [C.c_void_p]
         L.orc_grid_step.argtypes = [C.c_void_p]
+        L.orc_grid_set_inflow.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_grid_call.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.orc_grid_call.restype = C.c_int
         L.orc_grid_field.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
@@ -162,6 +163,13 @@ class OracleGrid:
 
     def step(self):
         self.L.orc_grid_step(self.h)
+
+    def set_inflow(self, planes, pbuf=None):
+        """Duct inflow: planes (8, Nz, Ny) float64; pbuf (n, 29) particles."""
+        planes = np.ascontiguousarray(planes, dtype=np.float64)
+        n = 0 if pbuf is None else len(pbuf)
+        pb = None if pbuf is None else np.ascontiguousarray(pbuf, dtype=np.float64)
+        self.L.orc_grid_set_inflow(self.h, ptr(planes), n, ptr(pb))
 
     def call(self, name, k=0):
         r = self.L.orc_grid_call(self.h, name.encode(), k)
