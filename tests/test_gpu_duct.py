@@ -60,8 +60,10 @@ def _compare(og, sim, tol=1e-12):
             bad.append((name, r))
     assert sim.np_ == og.np_, (sim.np_, og.np_)
     assert np.array_equal(sim.flags(), og.flag()), "particle order"
+    # stage _new is dead between substeps (TimeAdvance_RK4 exchanges the
+    # device's _int and _new buffers where the reference copies)
     for q in range(7):
-        for s in range(4):
+        for s in (0, 1, 3):
             r = rel(sim.part(q, s), og.part(q, s))
             if r > tol:
                 bad.append(("part", q, s, r))
