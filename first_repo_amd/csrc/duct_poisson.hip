@@ -153,10 +153,13 @@ __global__ void k_duct_cprime(DuctDev q, const double* __restrict__ lamy,
 // Forward sweep and back substitution of every mode (poisson_IO.cpp:231-329
 // for the real part, 353-452 for the imaginary part with the same
 // coefficients), in place on T[i][m].  Mode (0,0) is pinned by a zero last d'.
-__global__ void __launch_bounds__(256) k_duct_thomas(DuctDev q, const double* __restrict__ lamy,
-                                                     const double* __restrict__ lamz,
-                                                     const double* __restrict__ cp,
-                                                     double2* __restrict__ T) {
+// The sweeps are serial in i; the values and c' of TB consecutive rows are
+// loaded together ahead of the dependent arithmetic.
+constexpr int TB = 16;
+__global__ void __launch_bounds__(64) k_duct_thomas(DuctDev q, const double* __restrict__ lamy,
+                                                    const double* __restrict__ lamz,
+                                                    const double* __restrict__ cp,
+                                                    double2* __restrict__ T) {
   const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= q.nm) return;
   double b0, b;
@@ -164,29 +167,53 @@ __global__ void __launch_bounds__(256) k_duct_thomas(DuctDev q, const double* __
   const double a = q.a;
   const long nm = q.nm;
   const int n = q.nxi;
-  double2 v = T[m];
-  double2 dd = make_double2(v.x / b0, v.y / b0);
-  T[m] = dd;
-  for (int i = 1; i < n - 1; i++) {
-    const double den = b - cp[(long)(i - 1) * nm + m] * a;
-    v = T[(long)i * nm + m];
-    dd = make_double2((v.x - dd.x * a) / den, (v.y - dd.y * a) / den);
-    T[(long)i * nm + m] = dd;
+  double2 dd = make_double2(0.0, 0.0);
+  for (int i0 = 0; i0 < n; i0 += TB) {
+    double2 vv[TB];
+    double cc[TB];
+#pragma unroll
+    for (int u = 0; u < TB; u++) {
+      const int i = i0 + u;
+      vv[u] = i < n ? T[(long)i * nm + m] : make_double2(0.0, 0.0);
+      cc[u] = (i >= 1 && i < n) ? cp[(long)(i - 1) * nm + m] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < TB; u++) {
+      const int i = i0 + u;
+      if (i >= n) break;
+      const double2 v = vv[u];
+      if (i == 0) {
+        dd = make_double2(v.x / b0, v.y / b0);
+      } else if (i < n - 1) {
+        const double den = b - cc[u] * a;
+        dd = make_double2((v.x - dd.x * a) / den, (v.y - dd.y * a) / den);
+      } else if (m == 0) {
+        dd = make_double2(0.0, 0.0);
+      } else {
+        const double den = b0 - cc[u] * a;
+        dd = make_double2((v.x - dd.x * a) / den, (v.y - dd.y * a) / den);
+      }
+      T[(long)i * nm + m] = dd;
+    }
   }
-  double2 x;
-  if (m == 0) {
-    x = make_double2(0.0, 0.0);
-  } else {
-    const double den = b0 - cp[(long)(n - 2) * nm + m] * a;
-    v = T[(long)(n - 1) * nm + m];
-    x = make_double2((v.x - dd.x * a) / den, (v.y - dd.y * a) / den);
-  }
-  T[(long)(n - 1) * nm + m] = x;
-  for (int i = n - 2; i >= 0; i--) {
-    const double c = cp[(long)i * nm + m];
-    const double2 di = T[(long)i * nm + m];
-    x = make_double2(di.x - c * x.x, di.y - c * x.y);
-    T[(long)i * nm + m] = x;
+  // back substitution x_i = d'_i - c'_i x_{i+1}, from x_{n-1} = d'_{n-1}
+  double2 x = dd;
+  for (int i1 = n - 2; i1 >= 0; i1 -= TB) {
+    double2 vv[TB];
+    double cc[TB];
+#pragma unroll
+    for (int u = 0; u < TB; u++) {
+      const int i = i1 - u;
+      vv[u] = i >= 0 ? T[(long)i * nm + m] : make_double2(0.0, 0.0);
+      cc[u] = i >= 0 ? cp[(long)i * nm + m] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < TB; u++) {
+      const int i = i1 - u;
+      if (i < 0) break;
+      x = make_double2(vv[u].x - cc[u] * x.x, vv[u].y - cc[u] * x.y);
+      T[(long)i * nm + m] = x;
+    }
   }
 }
 
@@ -411,7 +438,7 @@ static int duct_poisson_iter(fr_duct_poisson* s, const double* cx, const double*
                        cz, RHS, s->twfm, s->twfy, s->A);
   hipLaunchKernelGGL(k_duct_z<true>, dim3(s->nbz), dim3(FT), s->lds_z, st, q, s->twfz, s->A,
                      s->T);
-  hipLaunchKernelGGL(k_duct_thomas, dim3(nblocks(q.nm, 256)), dim3(256), 0, st, q, s->lamy,
+  hipLaunchKernelGGL(k_duct_thomas, dim3(nblocks(q.nm, 64)), dim3(64), 0, st, q, s->lamy,
                      s->lamz, s->cp, s->T);
   hipLaunchKernelGGL(k_duct_z<false>, dim3(s->nbz), dim3(FT), s->lds_z, st, q, s->twiz, s->T,
                      s->A);
