@@ -180,20 +180,21 @@ __global__ void __launch_bounds__(64) k_duct_thomas(DuctDev q, const double* __r
 #pragma unroll
     for (int u = 0; u < TB; u++) {
       const int i = i0 + u;
-      if (i >= n) break;
-      const double2 v = vv[u];
-      if (i == 0) {
-        dd = make_double2(v.x / b0, v.y / b0);
-      } else if (i < n - 1) {
-        const double den = b - cc[u] * a;
-        dd = make_double2((v.x - dd.x * a) / den, (v.y - dd.y * a) / den);
-      } else if (m == 0) {
-        dd = make_double2(0.0, 0.0);
-      } else {
-        const double den = b0 - cc[u] * a;
-        dd = make_double2((v.x - dd.x * a) / den, (v.y - dd.y * a) / den);
+      if (i < n) {
+        const double2 v = vv[u];
+        if (i == 0) {
+          dd = make_double2(v.x / b0, v.y / b0);
+        } else if (i < n - 1) {
+          const double den = b - cc[u] * a;
+          dd = make_double2((v.x - dd.x * a) / den, (v.y - dd.y * a) / den);
+        } else if (m == 0) {
+          dd = make_double2(0.0, 0.0);
+        } else {
+          const double den = b0 - cc[u] * a;
+          dd = make_double2((v.x - dd.x * a) / den, (v.y - dd.y * a) / den);
+        }
+        T[(long)i * nm + m] = dd;
       }
-      T[(long)i * nm + m] = dd;
     }
   }
   // back substitution x_i = d'_i - c'_i x_{i+1}, from x_{n-1} = d'_{n-1}
@@ -210,9 +211,10 @@ __global__ void __launch_bounds__(64) k_duct_thomas(DuctDev q, const double* __r
 #pragma unroll
     for (int u = 0; u < TB; u++) {
       const int i = i1 - u;
-      if (i < 0) break;
-      x = make_double2(vv[u].x - cc[u] * x.x, vv[u].y - cc[u] * x.y);
-      T[(long)i * nm + m] = x;
+      if (i >= 0) {
+        x = make_double2(vv[u].x - cc[u] * x.x, vv[u].y - cc[u] * x.y);
+        T[(long)i * nm + m] = x;
+      }
     }
   }
 }
