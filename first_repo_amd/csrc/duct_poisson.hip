@@ -156,64 +156,67 @@ __global__ void k_duct_cprime(DuctDev q, const double* __restrict__ lamy,
 // The sweeps are serial in i; the values and c' of TB consecutive rows are
 // loaded together ahead of the dependent arithmetic.
 constexpr int TB = 16;
+// One thread per (mode, real or imaginary part): the two parts are
+// independent systems with identical coefficients.
 __global__ void __launch_bounds__(64) k_duct_thomas(DuctDev q, const double* __restrict__ lamy,
                                                     const double* __restrict__ lamz,
                                                     const double* __restrict__ cp,
-                                                    double2* __restrict__ T) {
-  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= q.nm) return;
+                                                    double* __restrict__ T) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * q.nm) return;
+  const long m = t >> 1;
   double b0, b;
   thomas_coef(q, lamy, lamz, m, b0, b);
   const double a = q.a;
   const long nm = q.nm;
   const int n = q.nxi;
-  double2 dd = make_double2(0.0, 0.0);
+  // T as doubles: row i, mode m, part t&1 at (i*nm + m)*2 + (t&1) = i*2nm + t
+  const long rs = 2 * nm;
+  double dd = 0.0;
   for (int i0 = 0; i0 < n; i0 += TB) {
-    double2 vv[TB];
-    double cc[TB];
+    double vv[TB], cc[TB];
 #pragma unroll
     for (int u = 0; u < TB; u++) {
       const int i = i0 + u;
-      vv[u] = i < n ? T[(long)i * nm + m] : make_double2(0.0, 0.0);
+      vv[u] = i < n ? T[(long)i * rs + t] : 0.0;
       cc[u] = (i >= 1 && i < n) ? cp[(long)(i - 1) * nm + m] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < TB; u++) {
       const int i = i0 + u;
       if (i < n) {
-        const double2 v = vv[u];
+        const double v = vv[u];
         if (i == 0) {
-          dd = make_double2(v.x / b0, v.y / b0);
+          dd = v / b0;
         } else if (i < n - 1) {
           const double den = b - cc[u] * a;
-          dd = make_double2((v.x - dd.x * a) / den, (v.y - dd.y * a) / den);
+          dd = (v - dd * a) / den;
         } else if (m == 0) {
-          dd = make_double2(0.0, 0.0);
+          dd = 0.0;
         } else {
           const double den = b0 - cc[u] * a;
-          dd = make_double2((v.x - dd.x * a) / den, (v.y - dd.y * a) / den);
+          dd = (v - dd * a) / den;
         }
-        T[(long)i * nm + m] = dd;
+        T[(long)i * rs + t] = dd;
       }
     }
   }
   // back substitution x_i = d'_i - c'_i x_{i+1}, from x_{n-1} = d'_{n-1}
-  double2 x = dd;
+  double x = dd;
   for (int i1 = n - 2; i1 >= 0; i1 -= TB) {
-    double2 vv[TB];
-    double cc[TB];
+    double vv[TB], cc[TB];
 #pragma unroll
     for (int u = 0; u < TB; u++) {
       const int i = i1 - u;
-      vv[u] = i >= 0 ? T[(long)i * nm + m] : make_double2(0.0, 0.0);
+      vv[u] = i >= 0 ? T[(long)i * rs + t] : 0.0;
       cc[u] = i >= 0 ? cp[(long)i * nm + m] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < TB; u++) {
       const int i = i1 - u;
       if (i >= 0) {
-        x = make_double2(vv[u].x - cc[u] * x.x, vv[u].y - cc[u] * x.y);
-        T[(long)i * nm + m] = x;
+        x = vv[u] - cc[u] * x;
+        T[(long)i * rs + t] = x;
       }
     }
   }
@@ -440,8 +443,8 @@ static int duct_poisson_iter(fr_duct_poisson* s, const double* cx, const double*
                        cz, RHS, s->twfm, s->twfy, s->A);
   hipLaunchKernelGGL(k_duct_z<true>, dim3(s->nbz), dim3(FT), s->lds_z, st, q, s->twfz, s->A,
                      s->T);
-  hipLaunchKernelGGL(k_duct_thomas, dim3(nblocks(q.nm, 64)), dim3(64), 0, st, q, s->lamy,
-                     s->lamz, s->cp, s->T);
+  hipLaunchKernelGGL(k_duct_thomas, dim3(nblocks(2 * q.nm, 64)), dim3(64), 0, st, q, s->lamy,
+                     s->lamz, s->cp, (double*)s->T);
   hipLaunchKernelGGL(k_duct_z<false>, dim3(s->nbz), dim3(FT), s->lds_z, st, q, s->twiz, s->T,
                      s->A);
   hipLaunchKernelGGL(k_duct_y_inv, dim3(s->nby), dim3(FT), s->lds_y, st, q, s->A, s->twim,
