@@ -68,6 +68,9 @@ def lib():
             "fr_duct_poisson_create": ([G, C.POINTER(vp)], i),
             "fr_duct_poisson_destroy": ([vp], i),
             "fr_duct_poisson_solve": ([vp, vp, vp, vp, vp, vp, i, d, C.POINTER(i), vp], i),
+            "fr_pcg_create": ([G, C.POINTER(vp)], i),
+            "fr_pcg_destroy": ([vp], i),
+            "fr_pcg_solve": ([vp, vp, vp, vp, vp, vp, i, d, C.POINTER(i), C.POINTER(d), vp], i),
             "fr_sim_create": ([C.c_char_p, C.c_char_p, C.POINTER(vp)], i),
             "fr_sim_destroy": ([vp], i),
             "fr_sim_initialize": ([vp], i),

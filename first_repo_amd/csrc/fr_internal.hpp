@@ -89,6 +89,11 @@ int duct_poisson_solve(fr_duct_poisson* s, const double* cx, const double* cy, c
                        bool inv, double* P, const double* RHS, int iterations, double epsilon,
                        int* num_iter, hipStream_t st);
 
+// PCG + FP32 multigrid (mg_pcg.hip) on Rho_face (C = 1./Rho_face).
+int pcg_solve(fr_pcg* s, const double* rfx, const double* rfy, const double* rfz, double* P,
+              const double* RHS, int maxit, double tol, int* iters, double* relres,
+              hipStream_t st);
+
 // ---- diagnostics (diag.hip) ----------------------------------------------------------
 int diag_tke(const Dev& d, const double* u, const double* rho, const double* red,
              double* out, hipStream_t s);

@@ -1,0 +1,483 @@
+// Preconditioned CG pressure solve with an FP32 geometric-multigrid V-cycle
+// as preconditioner (gfx950).  Counterpart of the reference's Hypre branch
+// poisson::Solve with Solver = 2 (PCG, poisson.cpp:115-172, 219-248): the same
+// 7-point operator div(C grad P) with C = 1/Rho_face, the same two-norm
+// relative-residual test at epsilon, at most Iteration iterations; BoomerAMG
+// is replaced by the V-cycle, the singular row pin of poisson.cpp:138-145 by
+// zero-mean preconditioned residuals.
+//
+// Precision: the outer iteration (solution, residual, search direction,
+// operator, dot products) is FP64; the V-cycle (levels, coefficients,
+// red-black Gauss-Seidel, restriction, prolongation) is FP32.  Every
+// reduction uses one fixed blocked order, so results do not depend on the
+// launch and the CPU oracle reproduces them bit for bit.
+//
+// Levels hold interior cells only (no ghosts): periodic neighbours by index
+// wrap; on the duct the x end faces carry a zero coefficient.
+#include <cmath>
+#include <vector>
+
+#include "fr_internal.hpp"
+
+namespace fr {
+
+constexpr int DNB = 1024, DNT = 256;  // reduction: chunks, threads per chunk
+
+struct MGL {
+  int nx, ny, nz;
+  float *cx, *cy, *cz, *e, *r, *t;
+  __device__ __forceinline__ long at(int i, int j, int k) const {
+    return ((long)k * ny + j) * nx + i;
+  }
+  long n() const { return (long)nx * ny * nz; }
+};
+
+__global__ void k_mg_coef0(Dev d, int duct, const double* __restrict__ rfx,
+                           const double* __restrict__ rfy, const double* __restrict__ rfz, MGL v) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)v.nx * v.ny * v.nz) return;
+  const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny), k = (int)(t / ((long)v.nx * v.ny));
+  const long c = at(d, i + 1, j + 1, k + 1);
+  v.cx[t] = (duct && i == 0) ? 0.f : (float)((1. / rfx[c]) * d.idx2);
+  v.cy[t] = (float)((1. / rfy[c]) * d.idy2);
+  v.cz[t] = (float)((1. / rfz[c]) * d.idz2);
+}
+
+// coarse face = sum of the four fine faces it covers / 16
+__global__ void k_mg_coarsen(MGL a, MGL c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)c.nx * c.ny * c.nz) return;
+  const int i = (int)(t % c.nx), j = (int)((t / c.nx) % c.ny), k = (int)(t / ((long)c.nx * c.ny));
+  const int fi = 2 * i, fj = 2 * j, fk = 2 * k;
+  c.cx[t] = (((a.cx[a.at(fi, fj, fk)] + a.cx[a.at(fi, fj + 1, fk)]) + a.cx[a.at(fi, fj, fk + 1)]) +
+             a.cx[a.at(fi, fj + 1, fk + 1)]) * 0.0625f;
+  c.cy[t] = (((a.cy[a.at(fi, fj, fk)] + a.cy[a.at(fi + 1, fj, fk)]) + a.cy[a.at(fi, fj, fk + 1)]) +
+             a.cy[a.at(fi + 1, fj, fk + 1)]) * 0.0625f;
+  c.cz[t] = (((a.cz[a.at(fi, fj, fk)] + a.cz[a.at(fi + 1, fj, fk)]) + a.cz[a.at(fi, fj + 1, fk)]) +
+             a.cz[a.at(fi + 1, fj + 1, fk)]) * 0.0625f;
+}
+
+// Face coefficients (-x,+x,-y,+y,-z,+z) and neighbour indices of cell (i,j,k).
+__device__ __forceinline__ void mg_nbrs(const MGL& v, int duct, int i, int j, int k, float c[6],
+                                        long q[6]) {
+  const int ip = i + 1 == v.nx ? 0 : i + 1, im = i == 0 ? v.nx - 1 : i - 1;
+  const int jp = j + 1 == v.ny ? 0 : j + 1, jm = j == 0 ? v.ny - 1 : j - 1;
+  const int kp = k + 1 == v.nz ? 0 : k + 1, km = k == 0 ? v.nz - 1 : k - 1;
+  q[0] = v.at(im, j, k);
+  q[1] = v.at(ip, j, k);
+  q[2] = v.at(i, jm, k);
+  q[3] = v.at(i, jp, k);
+  q[4] = v.at(i, j, km);
+  q[5] = v.at(i, j, kp);
+  const long s = v.at(i, j, k);
+  c[0] = v.cx[s];
+  c[1] = (duct && i + 1 == v.nx) ? 0.f : v.cx[q[1]];
+  c[2] = v.cy[s];
+  c[3] = v.cy[q[3]];
+  c[4] = v.cz[s];
+  c[5] = v.cz[q[5]];
+}
+
+// red-black half sweep of S e = r: e_c = (r_c + sum C_f e_f) / sum C_f
+__global__ void k_mg_smooth(MGL v, int duct, int color) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)v.nx * v.ny * v.nz) return;
+  const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny), k = (int)(t / ((long)v.nx * v.ny));
+  if (((i + j + k) & 1) != color) return;
+  float c[6];
+  long q[6];
+  mg_nbrs(v, duct, i, j, k, c, q);
+  float num = v.r[t];
+  float den = 0.f;
+#pragma unroll
+  for (int f = 0; f < 6; f++) {
+    num = num + c[f] * v.e[q[f]];
+    den = den + c[f];
+  }
+  v.e[t] = num / den;
+}
+
+// t = r - S e
+__global__ void k_mg_residual(MGL v, int duct) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)v.nx * v.ny * v.nz) return;
+  const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny), k = (int)(t / ((long)v.nx * v.ny));
+  float c[6];
+  long q[6];
+  mg_nbrs(v, duct, i, j, k, c, q);
+  const float ec = v.e[t];
+  float se = 0.f;
+#pragma unroll
+  for (int f = 0; f < 6; f++) se = se + c[f] * (ec - v.e[q[f]]);
+  v.t[t] = v.r[t] - se;
+}
+
+// coarse r = mean of the 8 fine residuals (order dk, dj, di); coarse e = 0
+__global__ void k_mg_restrict(MGL f, MGL c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)c.nx * c.ny * c.nz) return;
+  const int i = (int)(t % c.nx), j = (int)((t / c.nx) % c.ny), k = (int)(t / ((long)c.nx * c.ny));
+  float a = 0.f;
+#pragma unroll
+  for (int dk = 0; dk < 2; dk++)
+#pragma unroll
+    for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+      for (int di = 0; di < 2; di++) a = a + f.t[f.at(2 * i + di, 2 * j + dj, 2 * k + dk)];
+  c.r[t] = a * 0.125f;
+  c.e[t] = 0.f;
+}
+
+// fine e += coarse e of the parent cell (piecewise constant)
+__global__ void k_mg_prolong(MGL f, MGL c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)f.nx * f.ny * f.nz) return;
+  const int i = (int)(t % f.nx), j = (int)((t / f.nx) % f.ny), k = (int)(t / ((long)f.nx * f.ny));
+  f.e[t] = f.e[t] + c.e[c.at(i / 2, j / 2, k / 2)];
+}
+
+__global__ void k_fill_f(long n, float* __restrict__ a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0.f;
+}
+
+// ---- fine level, double, ghosted fields --------------------------------------
+__device__ __forceinline__ void tijk(long t, int nx, int ny, int& i, int& j, int& k) {
+  i = (int)(t % nx) + 1;
+  j = (int)((t / nx) % ny) + 1;
+  k = (int)(t / ((long)nx * ny)) + 1;
+}
+
+// S p = -div(C grad p), C = 1./Rho_face, with the operation order of the
+// defect-correction residual (poisson.cpp:393-397).
+__device__ __forceinline__ double sfine(const Dev& d, const double* __restrict__ p,
+                                        const double* __restrict__ rfx,
+                                        const double* __restrict__ rfy,
+                                        const double* __restrict__ rfz, long c) {
+  const long sx = 1, sy = d.Nx, sz = d.NxNy;
+  const double Cx0 = 1. / rfx[c], Cx1 = 1. / rfx[c + sx];
+  const double Cy0 = 1. / rfy[c], Cy1 = 1. / rfy[c + sy];
+  const double Cz0 = 1. / rfz[c], Cz1 = 1. / rfz[c + sz];
+  const double pc = p[c];
+  const double D = (((p[c + sx] - pc) * d.idx) * Cx1 - ((pc - p[c - sx]) * d.idx) * Cx0) * d.idx +
+                   (((p[c + sy] - pc) * d.idy) * Cy1 - ((pc - p[c - sy]) * d.idy) * Cy0) * d.idy +
+                   (((p[c + sz] - pc) * d.idz) * Cz1 - ((pc - p[c - sz]) * d.idz) * Cz0) * d.idz;
+  return -D;
+}
+
+struct Fine {
+  Dev d;
+  int nx, ny, nz;
+  long n;
+};
+
+// op 0: r = -RHS - S x; op 1: q = S p
+__global__ void k_pcg_apply(Fine F, int op, const double* __restrict__ x,
+                            const double* __restrict__ rhs, const double* __restrict__ rfx,
+                            const double* __restrict__ rfy, const double* __restrict__ rfz,
+                            double* __restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.n) return;
+  int i, j, k;
+  tijk(t, F.nx, F.ny, i, j, k);
+  const long c = at(F.d, i, j, k);
+  const double s = sfine(F.d, x, rfx, rfy, rfz, c);
+  out[c] = op == 0 ? -rhs[c] - s : s;
+}
+
+// Blocked sums over the interior.  mode 0: a.b (ghosted doubles); mode 1:
+// sum of the float level-0 array e.
+template <int MODE>
+__global__ void __launch_bounds__(DNT) k_dsum(Fine F, const double* __restrict__ a,
+                                              const double* __restrict__ b,
+                                              const float* __restrict__ e,
+                                              double* __restrict__ part) {
+  __shared__ double sh[DNT];
+  const long chunk = (F.n + DNB - 1) / DNB;
+  const long lo = (long)blockIdx.x * chunk;
+  const long hi = lo + chunk < F.n ? lo + chunk : F.n;
+  double acc = 0.0;
+  for (long t = lo + threadIdx.x; t < hi; t += DNT) {
+    if (MODE == 0) {
+      int i, j, k;
+      tijk(t, F.nx, F.ny, i, j, k);
+      const long c = at(F.d, i, j, k);
+      acc += a[c] * b[c];
+    } else {
+      acc += (double)e[t];
+    }
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = DNT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_dsum_final(const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int b = 0; b < DNB; b++) s += part[b];
+  *out = s;
+}
+
+// level-0 r = (float) residual
+__global__ void k_pcg_tofloat(Fine F, const double* __restrict__ r, float* __restrict__ f) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.n) return;
+  int i, j, k;
+  tijk(t, F.nx, F.ny, i, j, k);
+  f[t] = (float)r[at(F.d, i, j, k)];
+}
+
+// z = (double) e - mean, mean = *msum / n
+__global__ void k_pcg_todouble(Fine F, const float* __restrict__ e, const double* __restrict__ msum,
+                               double* __restrict__ z) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.n) return;
+  int i, j, k;
+  tijk(t, F.nx, F.ny, i, j, k);
+  const double m = *msum / (double)F.n;
+  z[at(F.d, i, j, k)] = (double)e[t] - m;
+}
+
+// op 0: p = z; op 1: x += alpha p, r -= alpha q; op 2: p = z + beta p
+__global__ void k_pcg_axpy(Fine F, int op, double s, double* __restrict__ x,
+                           double* __restrict__ r, double* __restrict__ p,
+                           const double* __restrict__ q) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.n) return;
+  int i, j, k;
+  tijk(t, F.nx, F.ny, i, j, k);
+  const long c = at(F.d, i, j, k);
+  if (op == 0) {
+    p[c] = q[c];
+  } else if (op == 1) {
+    x[c] = x[c] + s * p[c];
+    r[c] = r[c] - s * q[c];
+  } else {
+    p[c] = q[c] + s * p[c];
+  }
+}
+
+__global__ void k_neumann_x2(Dev d, double* __restrict__ P) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)d.Ny * d.Nz) return;
+  const long base = t * d.Nx;
+  P[base] = P[base + 1];
+  P[base + d.Nx - 1] = P[base + d.Nx - 2];
+}
+
+}  // namespace fr
+
+using namespace fr;
+
+struct fr_pcg {
+  fr_geom g;
+  Fine F;
+  int duct = 0;
+  std::vector<MGL> L;
+  double *r = nullptr, *z = nullptr, *p = nullptr;  // ghosted fine fields
+  double* part = nullptr;
+  double* dres = nullptr;   // [0] scalar results
+  double* hres = nullptr;   // pinned host copy
+  int pre = 2, post = 2, coarse_sweeps = 20;
+};
+
+static int mg_launch_n(long n) { return (int)((n + 255) / 256); }
+
+static int fill_fine_ghosts(fr_pcg* s, double* f, hipStream_t st) {
+  int rc = ghost_fill(s->F.d, f, st);
+  if (rc) return rc;
+  if (s->duct) {
+    hipLaunchKernelGGL(k_neumann_x2, dim3(nblocks((long)s->F.d.Ny * s->F.d.Nz, 256)), dim3(256),
+                       0, st, s->F.d, f);
+    FR_LAUNCH_CHECK();
+  }
+  return FR_OK;
+}
+
+static void smooth_pair(fr_pcg* s, const MGL& v, int first, hipStream_t st) {
+  hipLaunchKernelGGL(k_mg_smooth, dim3(mg_launch_n(v.n())), dim3(256), 0, st, v, s->duct, first);
+  hipLaunchKernelGGL(k_mg_smooth, dim3(mg_launch_n(v.n())), dim3(256), 0, st, v, s->duct,
+                     1 - first);
+}
+
+static void vcycle(fr_pcg* s, size_t l, hipStream_t st) {
+  const MGL& v = s->L[l];
+  if (l + 1 == s->L.size()) {
+    for (int k = 0; k < s->coarse_sweeps; k++) smooth_pair(s, v, 0, st);
+    for (int k = 0; k < s->coarse_sweeps; k++) smooth_pair(s, v, 1, st);
+    return;
+  }
+  for (int k = 0; k < s->pre; k++) smooth_pair(s, v, 0, st);
+  hipLaunchKernelGGL(k_mg_residual, dim3(mg_launch_n(v.n())), dim3(256), 0, st, v, s->duct);
+  const MGL& c = s->L[l + 1];
+  hipLaunchKernelGGL(k_mg_restrict, dim3(mg_launch_n(c.n())), dim3(256), 0, st, v, c);
+  vcycle(s, l + 1, st);
+  hipLaunchKernelGGL(k_mg_prolong, dim3(mg_launch_n(v.n())), dim3(256), 0, st, v, c);
+  for (int k = 0; k < s->post; k++) smooth_pair(s, v, 1, st);
+}
+
+// blocked sum into s->dres[slot]
+static void dsum(fr_pcg* s, int mode, const double* a, const double* b, const float* e, int slot,
+                 hipStream_t st) {
+  if (mode == 0)
+    hipLaunchKernelGGL(k_dsum<0>, dim3(DNB), dim3(DNT), 0, st, s->F, a, b, e, s->part);
+  else
+    hipLaunchKernelGGL(k_dsum<1>, dim3(DNB), dim3(DNT), 0, st, s->F, a, b, e, s->part);
+  hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(64), 0, st, s->part, s->dres + slot);
+}
+
+static int read_scalar(fr_pcg* s, int slot, double* v, hipStream_t st) {
+  FR_LAUNCH_CHECK();
+  FR_CHECK(hipMemcpyAsync(s->hres, s->dres + slot, sizeof(double), hipMemcpyDeviceToHost, st));
+  FR_CHECK(hipStreamSynchronize(st));
+  *v = s->hres[0];
+  return FR_OK;
+}
+
+// z = M r: r to float, one V-cycle, zero-mean z in double
+static int precond(fr_pcg* s, hipStream_t st) {
+  const MGL& f = s->L[0];
+  const unsigned nb = mg_launch_n(s->F.n);
+  hipLaunchKernelGGL(k_pcg_tofloat, dim3(nb), dim3(256), 0, st, s->F, (const double*)s->r, f.r);
+  hipLaunchKernelGGL(k_fill_f, dim3(nb), dim3(256), 0, st, f.n(), f.e);
+  vcycle(s, 0, st);
+  dsum(s, 1, nullptr, nullptr, f.e, 1, st);
+  hipLaunchKernelGGL(k_pcg_todouble, dim3(nb), dim3(256), 0, st, s->F, (const float*)f.e,
+                     (const double*)(s->dres + 1), s->z);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+namespace fr {
+
+int pcg_solve(fr_pcg* s, const double* rfx, const double* rfy, const double* rfz, double* P,
+              const double* RHS, int maxit, double tol, int* iters, double* relres,
+              hipStream_t st) {
+  if (!s || !rfx || !rfy || !rfz || !P || !RHS) return FR_EINVAL;
+  const Fine& F = s->F;
+  const unsigned nb = mg_launch_n(F.n);
+  // level coefficients
+  hipLaunchKernelGGL(k_mg_coef0, dim3(nb), dim3(256), 0, st, F.d, s->duct, rfx, rfy, rfz, s->L[0]);
+  for (size_t l = 1; l < s->L.size(); l++)
+    hipLaunchKernelGGL(k_mg_coarsen, dim3(mg_launch_n(s->L[l].n())), dim3(256), 0, st, s->L[l - 1],
+                       s->L[l]);
+  int rc = fill_fine_ghosts(s, P, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pcg_apply, dim3(nb), dim3(256), 0, st, F, 0, (const double*)P, RHS, rfx, rfy,
+                     rfz, s->r);
+  double bn2, rr;
+  dsum(s, 0, RHS, RHS, nullptr, 0, st);
+  if ((rc = read_scalar(s, 0, &bn2, st))) return rc;
+  const double bn = std::sqrt(bn2);
+  dsum(s, 0, s->r, s->r, nullptr, 0, st);
+  if ((rc = read_scalar(s, 0, &rr, st))) return rc;
+  int it = 0;
+  if (std::sqrt(rr) > tol * bn) {
+    if ((rc = precond(s, st))) return rc;
+    hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(256), 0, st, F, 0, 0.0, P, s->r, s->p,
+                       (const double*)s->z);
+    double rz;
+    dsum(s, 0, s->r, s->z, nullptr, 0, st);
+    if ((rc = read_scalar(s, 0, &rz, st))) return rc;
+    while (it < maxit) {
+      if ((rc = fill_fine_ghosts(s, s->p, st))) return rc;
+      // q = S p in z's storage
+      hipLaunchKernelGGL(k_pcg_apply, dim3(nb), dim3(256), 0, st, F, 1, (const double*)s->p, RHS,
+                         rfx, rfy, rfz, s->z);
+      double pq;
+      dsum(s, 0, s->p, s->z, nullptr, 0, st);
+      if ((rc = read_scalar(s, 0, &pq, st))) return rc;
+      const double alpha = rz / pq;
+      hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(256), 0, st, F, 1, alpha, P, s->r, s->p,
+                         (const double*)s->z);
+      it++;
+      dsum(s, 0, s->r, s->r, nullptr, 0, st);
+      if ((rc = read_scalar(s, 0, &rr, st))) return rc;
+      if (std::sqrt(rr) <= tol * bn) break;
+      if ((rc = precond(s, st))) return rc;
+      double rz2;
+      dsum(s, 0, s->r, s->z, nullptr, 0, st);
+      if ((rc = read_scalar(s, 0, &rz2, st))) return rc;
+      const double beta = rz2 / rz;
+      rz = rz2;
+      hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(256), 0, st, F, 2, beta, P, s->r, s->p,
+                         (const double*)s->z);
+    }
+  }
+  if ((rc = fill_fine_ghosts(s, P, st))) return rc;
+  FR_LAUNCH_CHECK();
+  if (iters) *iters = it;
+  if (relres) *relres = bn > 0 ? std::sqrt(rr) / bn : 0.0;
+  return FR_OK;
+}
+
+}  // namespace fr
+
+extern "C" {
+
+int fr_pcg_create(const fr_geom* g, fr_pcg** out) {
+  if (!geom_ok(g) || !out) return FR_EINVAL;
+  fr_pcg* s = new fr_pcg();
+  s->g = *g;
+  s->duct = g->is_box ? 0 : 1;
+  s->F.d = make_dev(g);
+  s->F.nx = g->Nx - 2;
+  s->F.ny = g->Ny - 2;
+  s->F.nz = g->Nz - 2;
+  s->F.n = (long)s->F.nx * s->F.ny * s->F.nz;
+  int nx = s->F.nx, ny = s->F.ny, nz = s->F.nz;
+  while (true) {
+    MGL v;
+    v.nx = nx;
+    v.ny = ny;
+    v.nz = nz;
+    const size_t n = (size_t)nx * ny * nz;
+    float** a[6] = {&v.cx, &v.cy, &v.cz, &v.e, &v.r, &v.t};
+    for (auto pp : a) FR_CHECK(hipMalloc(pp, sizeof(float) * n));
+    s->L.push_back(v);
+    if (nx % 2 || ny % 2 || nz % 2 || nx < 8 || ny < 8 || nz < 8) break;
+    nx /= 2;
+    ny /= 2;
+    nz /= 2;
+  }
+  const size_t gs = (size_t)g->Nx * g->Ny * g->Nz;
+  for (double** pp : {&s->r, &s->z, &s->p}) {
+    FR_CHECK(hipMalloc(pp, sizeof(double) * gs));
+    FR_CHECK(hipMemset(*pp, 0, sizeof(double) * gs));
+  }
+  FR_CHECK(hipMalloc(&s->part, sizeof(double) * DNB));
+  FR_CHECK(hipMalloc(&s->dres, sizeof(double) * 4));
+  FR_CHECK(hipHostMalloc(&s->hres, sizeof(double) * 2, hipHostMallocDefault));
+  *out = s;
+  return FR_OK;
+}
+
+int fr_pcg_destroy(fr_pcg* s) {
+  if (!s) return FR_OK;
+  for (MGL& v : s->L) {
+    float* a[6] = {v.cx, v.cy, v.cz, v.e, v.r, v.t};
+    for (float* p : a) (void)hipFree(p);
+  }
+  (void)hipFree(s->r);
+  (void)hipFree(s->z);
+  (void)hipFree(s->p);
+  (void)hipFree(s->part);
+  (void)hipFree(s->dres);
+  (void)hipHostFree(s->hres);
+  delete s;
+  return FR_OK;
+}
+
+int fr_pcg_solve(fr_pcg* s, const double* rfx, const double* rfy, const double* rfz, double* P,
+                 const double* RHS, int iterations, double epsilon, int* num_iter,
+                 double* relres, void* stream) {
+  return fr::pcg_solve(s, rfx, rfy, rfz, P, RHS, iterations, epsilon, num_iter, relres,
+                       S(stream));
+}
+
+}  // extern "C"

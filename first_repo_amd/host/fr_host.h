@@ -213,6 +213,8 @@ class poisson {
  public:
   fr_box_poisson* s = nullptr;
   fr_duct_poisson* sd = nullptr;
+  fr_pcg* sp = nullptr;  // Solver = 2: PCG with the FP32 multigrid preconditioner
+  double relres_ = 0;
   params* par = nullptr;
   int num_iteration_ = 0;
   hipStream_t st_ = 0;

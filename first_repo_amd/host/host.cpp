@@ -314,13 +314,16 @@ int particle::rebuild_cells(int stage) { return dev.build(size_->dev(), stage, 0
 int poisson::init(params* p, gridsize* g, hipStream_t st) {
   par = p;
   st_ = st;
+  if (p->Solver() == 2) return fr_pcg_create(&g->g, &sp);
   return g->IsBox() ? fr_box_poisson_create(&g->g, &s) : fr_duct_poisson_create(&g->g, &sd);
 }
 poisson::~poisson() {
   fr_box_poisson_destroy(s);
   fr_duct_poisson_destroy(sd);
+  fr_pcg_destroy(sp);
 }
 int poisson::Solve(tensor1& C, tensor0& P, tensor0& RHS) {
+  if (sp) return FR_EINVAL;  // the PCG solver takes Rho_face (Solve_face_rho)
   if (sd)
     return fr_duct_poisson_solve(sd, C.x.pointer(), C.y.pointer(), C.z.pointer(), P.pointer(),
                                  RHS.pointer(), par->Iteration(), par->epsilon(),
@@ -330,6 +333,10 @@ int poisson::Solve(tensor1& C, tensor0& P, tensor0& RHS) {
                               &num_iteration_, st_);
 }
 int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS) {
+  if (sp)
+    return pcg_solve(sp, Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer(),
+                     P.pointer(), RHS.pointer(), par->Iteration(), par->epsilon(),
+                     &num_iteration_, &relres_, st_);
   if (sd)
     return duct_poisson_solve(sd, Rho_face.x.pointer(), Rho_face.y.pointer(),
                               Rho_face.z.pointer(), true, P.pointer(), RHS.pointer(),

@@ -133,6 +133,25 @@ int fr_duct_poisson_solve(fr_duct_poisson* s, const double* cx,
                           const double* RHS, int iterations, double epsilon,
                           int* num_iter, void* stream);
 
+/* Variable-coefficient pressure solve by preconditioned conjugate gradients,
+ * the counterpart of the reference's Hypre branch of poisson::Solve with
+ * Solver = 2 (poisson.cpp:115-172, 219-248): the 7-point operator
+ * div(C grad P) with C = 1/Rho_face, stopping when ||r||_2 <= epsilon *
+ * ||RHS||_2 or after `iterations` iterations.  The preconditioner (one
+ * geometric-multigrid V-cycle, red-black Gauss-Seidel, in FP32) replaces
+ * BoomerAMG; the outer iteration and every residual are FP64.  Takes the face
+ * densities rfx, rfy, rfz (Rho_face), works on box and duct geometries (duct:
+ * closed x ends, Neumann x ghosts of P on exit), needs even interior sizes.
+ * RHS must have zero mean over the interior; P is the initial guess.
+ * *relres receives ||r||/||RHS|| at exit. */
+typedef struct fr_pcg fr_pcg;
+int fr_pcg_create(const fr_geom* g, fr_pcg** out);
+int fr_pcg_destroy(fr_pcg* s);
+int fr_pcg_solve(fr_pcg* s, const double* rfx, const double* rfy,
+                 const double* rfz, double* P, const double* RHS,
+                 int iterations, double epsilon, int* num_iter, double* relres,
+                 void* stream);
+
 /* ------------------------------------------------------------ whole solver
  * The box solver (grid.h:16-115 with its particle set, box poisson solver
  * and the simulation.cpp:34-71 step loop) held on one device.  Created from a

@@ -148,6 +148,27 @@ void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign) {
 void orc_fft2d(int ny, int nz, double* re, double* im, int sign) {
   fft2d(re, im, ny, nz, sign);
 }
+// PCG with the FP32 multigrid preconditioner on div(C grad P) = RHS with
+// C = 1/Rho_face (face densities rfx, rfy, rfz).  Returns the iteration count;
+// *relres = ||r||/||RHS|| at exit.
+int orc_pcg(const orc_geom* gg, const double* rfx, const double* rfy, const double* rfz,
+            double* P, const double* RHS, int maxit, double tol, double* relres) {
+  Geom g = mk(gg);
+  F1 rf;
+  rf.init(&g);
+  load(rf.x, rfx);
+  load(rf.y, rfy);
+  load(rf.z, rfz);
+  F0 p, r;
+  p.init(&g);
+  r.init(&g);
+  load(p, P);
+  load(r, RHS);
+  MGPCG s;
+  int it = s.solve(rf, p, r, maxit, tol, relres);
+  std::memcpy(P, p.v.data(), p.v.size() * 8);
+  return it;
+}
 void orc_rfft1d(const double* x, int n, double* Xre, double* Xim) { rfft1d(x, n, Xre, Xim); }
 void orc_irfft1d(const double* Xre, const double* Xim, int n, double* x) {
   irfft1d(Xre, Xim, n, x);

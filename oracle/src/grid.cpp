@@ -179,12 +179,20 @@ void Grid::Compute_RHS_Pois() {  // grid.cpp:629-637
 }
 
 void Grid::Solve_Poisson() {  // grid.cpp:639-643
+  if (par.Solver == 2) {
+    last_iterations = mg.solve(Rho_face, P, RHS_Pois, par.Iteration, par.epsilon, &last_relres);
+    return;
+  }
   dummy.Equal_Divide(1., Rho_face);
   PS.Solve(dummy, P, RHS_Pois);
   last_iterations = PS.num_iteration;
 }
 
 void Grid::Solve_Poisson_IO() {  // grid.cpp:645-649
+  if (par.Solver == 2) {
+    last_iterations = mg.solve(Rho_face, P, RHS_Pois, par.Iteration, par.epsilon, &last_relres);
+    return;
+  }
   dummy.Equal_Divide(1., Rho_face);
   PS_IO.Solve(dummy, P, RHS_Pois);
   last_iterations = PS_IO.num_iteration1;

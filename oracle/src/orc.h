@@ -354,6 +354,33 @@ struct DuctPoisson {  // poisson_IO.h:20-88
   void thomas_line(double* v, long stride, long off, int j, int k) const;
 };
 
+// ---------------------------------------------------------------- PCG + GMG
+// Preconditioned CG on S = -div(C grad), C = 1/Rho_face, with one FP32
+// geometric-multigrid V-cycle as preconditioner (stands in for the Hypre
+// PCG/BoomerAMG branch, poisson.cpp:115-248).  Levels hold interior cells
+// only, face coefficients C*inv_d2 (left face of each cell), periodic wrap,
+// and closed x ends on the duct.
+struct MGLevel {
+  int nx = 0, ny = 0, nz = 0;
+  std::vector<float> cx, cy, cz, e, r, t;
+  size_t at(int i, int j, int k) const { return ((size_t)k * ny + j) * nx + i; }
+};
+
+struct MGPCG {
+  const Geom* g = nullptr;
+  bool duct = false;
+  std::vector<MGLevel> L;
+  int pre = 2, post = 2, coarse_sweeps = 20;
+  void build(const Geom* gg);
+  void coefficients(const F1& Rho_face);
+  void smooth(MGLevel& v, int color);
+  void residual(MGLevel& v);
+  void vcycle(size_t level);
+  // P: initial guess in, solution out (ghosts filled); RHS zero-mean.
+  // Stops when ||r||_2 <= tol ||RHS||_2 or after maxit iterations.
+  int solve(const F1& Rho_face, F0& P, const F0& RHS, int maxit, double tol, double* relres);
+};
+
 // ---------------------------------------------------------------- duct inflow
 // What the duct copies from the box grid g each substep (grid.cpp:351-364,
 // 482-504), as ghosted [Nz][Ny] planes (j fastest): rho_new = g.Rho_new(1,j,k),
@@ -381,6 +408,8 @@ struct Grid {  // grid.h:16-115
   BoxPoisson PS;
   DuctPoisson PS_IO;
   Inflow in;  // duct only
+  MGPCG mg;   // Solver = 2
+  double last_relres = 0;
   int last_iterations = 0;
   void setup(const Params& p);
   // Initialize (grid.cpp:110-231), Initial = 0/2 forms; the velocity field of

@@ -65,6 +65,9 @@ def orc():
         L.orc_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, C.c_int]
         L.orc_fft2d.argtypes = [C.c_int, C.c_int, dp, dp, C.c_int]
         L.orc_rfft1d.argtypes = [dp, C.c_int, dp, dp]
+        L.orc_pcg.argtypes = [G, dp, dp, dp, dp, dp, C.c_int, C.c_double,
+                              C.POINTER(C.c_double)]
+        L.orc_pcg.restype = C.c_int
         L.orc_irfft1d.argtypes = [dp, dp, C.c_int, dp]
         L.orc_poisson.argtypes = [G, dp, dp, dp, dp, dp, C.c_int, C.c_double]
         L.orc_poisson.restype = C.c_int
