@@ -136,6 +136,99 @@ __global__ void k_mg_prolong(MGL f, MGL c) {
   f.e[t] = f.e[t] + c.e[c.at(i / 2, j / 2, k / 2)];
 }
 
+// The tail of the V-cycle from level l0 down, for small levels, in one
+// workgroup: the same per-cell operations as the per-level kernels, with a
+// workgroup barrier where those have a launch boundary.
+constexpr int MGMAX = 16, SMALL = 1024;
+struct MGSet {
+  MGL v[MGMAX];
+  int nl;
+};
+
+__device__ __forceinline__ void sm_smooth(const MGL& v, int duct, int color) {
+  const long n = (long)v.nx * v.ny * v.nz;
+  for (long t = threadIdx.x; t < n; t += SMALL) {
+    const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny),
+              k = (int)(t / ((long)v.nx * v.ny));
+    if (((i + j + k) & 1) != color) continue;
+    float c[6];
+    long q[6];
+    mg_nbrs(v, duct, i, j, k, c, q);
+    float num = v.r[t];
+    float den = 0.f;
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+      num = num + c[f] * v.e[q[f]];
+      den = den + c[f];
+    }
+    v.e[t] = num / den;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(SMALL) k_mg_small(MGSet S, int l0, int duct, int pre, int post,
+                                                    int csweeps) {
+  const int last = S.nl - 1;
+  for (int l = l0; l < last; l++) {
+    const MGL& v = S.v[l];
+    for (int k = 0; k < pre; k++) {
+      sm_smooth(v, duct, 0);
+      sm_smooth(v, duct, 1);
+    }
+    const long n = (long)v.nx * v.ny * v.nz;
+    for (long t = threadIdx.x; t < n; t += SMALL) {
+      const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny),
+                k = (int)(t / ((long)v.nx * v.ny));
+      float c[6];
+      long q[6];
+      mg_nbrs(v, duct, i, j, k, c, q);
+      const float ec = v.e[t];
+      float se = 0.f;
+#pragma unroll
+      for (int f = 0; f < 6; f++) se = se + c[f] * (ec - v.e[q[f]]);
+      v.t[t] = v.r[t] - se;
+    }
+    __syncthreads();
+    const MGL& c = S.v[l + 1];
+    const long nc = (long)c.nx * c.ny * c.nz;
+    for (long t = threadIdx.x; t < nc; t += SMALL) {
+      const int i = (int)(t % c.nx), j = (int)((t / c.nx) % c.ny),
+                k = (int)(t / ((long)c.nx * c.ny));
+      float a = 0.f;
+      for (int dk = 0; dk < 2; dk++)
+        for (int dj = 0; dj < 2; dj++)
+          for (int di = 0; di < 2; di++) a = a + v.t[v.at(2 * i + di, 2 * j + dj, 2 * k + dk)];
+      c.r[t] = a * 0.125f;
+      c.e[t] = 0.f;
+    }
+    __syncthreads();
+  }
+  const MGL& cv = S.v[last];
+  for (int k = 0; k < csweeps; k++) {
+    sm_smooth(cv, duct, 0);
+    sm_smooth(cv, duct, 1);
+  }
+  for (int k = 0; k < csweeps; k++) {
+    sm_smooth(cv, duct, 1);
+    sm_smooth(cv, duct, 0);
+  }
+  for (int l = last - 1; l >= l0; l--) {
+    const MGL& v = S.v[l];
+    const MGL& c = S.v[l + 1];
+    const long n = (long)v.nx * v.ny * v.nz;
+    for (long t = threadIdx.x; t < n; t += SMALL) {
+      const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny),
+                k = (int)(t / ((long)v.nx * v.ny));
+      v.e[t] = v.e[t] + c.e[c.at(i / 2, j / 2, k / 2)];
+    }
+    __syncthreads();
+    for (int k = 0; k < post; k++) {
+      sm_smooth(v, duct, 1);
+      sm_smooth(v, duct, 0);
+    }
+  }
+}
+
 __global__ void k_fill_f(long n, float* __restrict__ a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) a[t] = 0.f;
@@ -216,11 +309,18 @@ __global__ void __launch_bounds__(DNT) k_dsum(Fine F, const double* __restrict__
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ void k_dsum_final(const double* __restrict__ part, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
-  double s = 0.0;
-  for (int b = 0; b < DNB; b++) s += part[b];
-  *out = s;
+// chunk sums combined by a pairwise tree (DNB/2 threads)
+__global__ void __launch_bounds__(DNB / 2) k_dsum_final(const double* __restrict__ part,
+                                                        double* __restrict__ out) {
+  __shared__ double sh[DNB];
+  sh[threadIdx.x] = part[threadIdx.x];
+  sh[threadIdx.x + DNB / 2] = part[threadIdx.x + DNB / 2];
+  __syncthreads();
+  for (int s = DNB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
 }
 
 // level-0 r = (float) residual
@@ -284,6 +384,7 @@ struct fr_pcg {
   double* dres = nullptr;   // [0] scalar results
   double* hres = nullptr;   // pinned host copy
   int pre = 2, post = 2, coarse_sweeps = 20;
+  int l_small = -1;  // first level handled by the one-workgroup kernel
 };
 
 static int mg_launch_n(long n) { return (int)((n + 255) / 256); }
@@ -307,6 +408,14 @@ static void smooth_pair(fr_pcg* s, const MGL& v, int first, hipStream_t st) {
 
 static void vcycle(fr_pcg* s, size_t l, hipStream_t st) {
   const MGL& v = s->L[l];
+  if ((int)l == s->l_small) {
+    MGSet S;
+    S.nl = (int)s->L.size();
+    for (int q = 0; q < S.nl; q++) S.v[q] = s->L[q];
+    hipLaunchKernelGGL(k_mg_small, dim3(1), dim3(SMALL), 0, st, S, (int)l, s->duct, s->pre,
+                       s->post, s->coarse_sweeps);
+    return;
+  }
   if (l + 1 == s->L.size()) {
     for (int k = 0; k < s->coarse_sweeps; k++) smooth_pair(s, v, 0, st);
     for (int k = 0; k < s->coarse_sweeps; k++) smooth_pair(s, v, 1, st);
@@ -328,7 +437,7 @@ static void dsum(fr_pcg* s, int mode, const double* a, const double* b, const fl
     hipLaunchKernelGGL(k_dsum<0>, dim3(DNB), dim3(DNT), 0, st, s->F, a, b, e, s->part);
   else
     hipLaunchKernelGGL(k_dsum<1>, dim3(DNB), dim3(DNT), 0, st, s->F, a, b, e, s->part);
-  hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(64), 0, st, s->part, s->dres + slot);
+  hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(DNB / 2), 0, st, s->part, s->dres + slot);
 }
 
 static int read_scalar(fr_pcg* s, int slot, double* v, hipStream_t st) {
@@ -445,6 +554,12 @@ int fr_pcg_create(const fr_geom* g, fr_pcg** out) {
     ny /= 2;
     nz /= 2;
   }
+  if ((int)s->L.size() > MGMAX) return FR_EINVAL;
+  for (size_t l = 0; l < s->L.size(); l++)
+    if (s->L[l].n() <= 32768) {
+      s->l_small = (int)l;
+      break;
+    }
   const size_t gs = (size_t)g->Nx * g->Ny * g->Nz;
   for (double** pp : {&s->r, &s->z, &s->p}) {
     FR_CHECK(hipMalloc(pp, sizeof(double) * gs));

@@ -12,8 +12,8 @@
 // Arithmetic is fixed so the device solver reproduces it bit for bit: the
 // V-cycle runs in float with the expression orders written here, the outer
 // iteration in double, and every sum uses the blocked order of dsum() (1024
-// chunks, 256 strided partial sums per chunk, pairwise tree, chunk sums in
-// sequence).
+// chunks, 256 strided partial sums per chunk, pairwise tree, then the chunk
+// sums by the same pairwise tree).
 #include <cmath>
 #include <cstring>
 
@@ -29,7 +29,7 @@ constexpr int NB = 1024, NT = 256;
 template <typename F>
 static double dsum(long n, F f) {
   const long chunk = (n + NB - 1) / NB;
-  double total = 0.0;
+  double bsum[NB];
   for (int b = 0; b < NB; b++) {
     const long lo = (long)b * chunk;
     double part[NT];
@@ -40,9 +40,11 @@ static double dsum(long n, F f) {
     }
     for (int s = NT / 2; s > 0; s >>= 1)
       for (int th = 0; th < s; th++) part[th] += part[th + s];
-    total += part[0];
+    bsum[b] = part[0];
   }
-  return total;
+  for (int s = NB / 2; s > 0; s >>= 1)
+    for (int b = 0; b < s; b++) bsum[b] += bsum[b + s];
+  return bsum[0];
 }
 
 void MGPCG::build(const Geom* gg) {
