@@ -556,7 +556,7 @@ int fr_pcg_create(const fr_geom* g, fr_pcg** out) {
   }
   if ((int)s->L.size() > MGMAX) return FR_EINVAL;
   for (size_t l = 0; l < s->L.size(); l++)
-    if (s->L[l].n() <= 32768) {
+    if (s->L[l].n() <= 4096) {
       s->l_small = (int)l;
       break;
     }
