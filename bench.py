@@ -54,7 +54,11 @@ def main():
                     help="fft: the plain FP64 defect-correction solve (Solver=0); pcg: "
                          "PCG with the FP32 multigrid preconditioner to ||r||/||b|| <= 1e-10 "
                          "(Solver=2)")
-    ap.add_argument("--workload", default="box256", choices=["box256", "duct512"],
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
+                    help="fp32: the precision form (FP32 fields and stencils; FP64 pressure, "
+                         "Poisson residual, sums and particles); its error against the FP64 "
+                         "path over the same steps is measured after the timed region")
+    ap.add_argument("--workload", default="box256", choices=["box256", "box512", "duct512"],
                     help="box256: the headline 256^3 box; duct512: the 512x128x128 "
                          "inflow/outflow duct (density ratio 3, 2e4 inlet particles/step)")
     args = ap.parse_args()
@@ -69,6 +73,9 @@ def main():
 
     if args.workload == "duct512":
         return bench_duct(args)
+    if args.workload == "box512":
+        args.n, args.particles = 512, 16_000_000
+    fp32 = args.precision == "fp32"
 
     tmp = tempfile.mkdtemp(prefix="frbench_")
     w = BoxWorkload(n=args.n, n_particles=args.particles, steps=200, mass_loading=0.5,
@@ -77,7 +84,7 @@ def main():
           flush=True)
     w.build(tmp)
     params = os.path.join(tmp, "params.in")
-    sim = fr.Sim(params, tmp)
+    sim = fr.Sim(params, tmp, fp32=fp32)
     sim.initialize()
     print("bench: initialised", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
@@ -100,12 +107,14 @@ def main():
     diag = sim.diagnostics()
     n_it = iters / args.steps / 4
     step_bytes = ncell * (4 * 416 + 4 * 96 * n_it) + args.particles * 4 * 288
+    if fp32:  # FP32 fields: the stencil streams halve, P and the particles do not
+        step_bytes = ncell * (4 * 208 + 4 * 80 * n_it) + args.particles * 4 * 264
     # dominant stencil kernel: the fused momentum update (grid::Update_RU_WOP);
-    # algorithmic bytes per launch = 21 doubles per ghosted cell (reads U, RU_int,
+    # algorithmic bytes per launch = 21 values per ghosted cell (reads U, RU_int,
     # S, RU, RU_np1: 15; writes RU_np1, RU_new: 6)
     g = sim.g
     ghosted = g.Nx * g.Ny * g.Nz
-    alg_bytes = 21 * 8 * ghosted
+    alg_bytes = 21 * (4 if fp32 else 8) * ghosted
     per_launch_ms = kern_ms / max(kern_n, 1)
     achieved = alg_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
     out = {
@@ -119,9 +128,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if fp32 else "f64",
         "data": "synthetic",
-        "config": {"workload": f"box {args.n}^3, {args.particles} particles, plain FP64, " +
+        "config": {"workload": f"box {args.n}^3, {args.particles} particles, " +
+                               ("precision form (FP32 fields/stencils, FP64 pressure and "
+                                "particles), " if fp32 else "plain FP64, ") +
                                ("FFT pressure solve to max|dp|<=1e-10" if args.solver == "fft"
                                 else "PCG + FP32 multigrid pressure solve to ||r||/||b||<=1e-10"),
                    "cells": ncell, "particles": args.particles,
@@ -142,6 +153,21 @@ def main():
                      "avg_launch_ms": round(per_launch_ms, 4)},
     }
     sim.close()
+    if fp32:
+        # error of the precision form against the FP64 path after the same steps
+        print("bench: FP64 reference run for the error of the precision form", file=sys.stderr,
+              flush=True)
+        ref = fr.Sim(params, tmp)
+        ref.initialize()
+        for _ in range(args.warmup + args.steps):
+            ref.step()
+        dref = ref.diagnostics()
+        ref.close()
+        out["config"]["rel_error_vs_fp64"] = {
+            name: float(abs(diag[k] - dref[k]) / abs(dref[k]))
+            for k, name in ((0, "TKE"), (7, "dissipation"), (1, "Rho_Mean"),
+                            (8, "mean_particle_T"))}
+        out["config"]["error_steps"] = args.warmup + args.steps
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         print(f"bench: CPU oracle sample, 1 step, {threads} threads", file=sys.stderr, flush=True)

@@ -72,6 +72,7 @@ def lib():
             "fr_pcg_destroy": ([vp], i),
             "fr_pcg_solve": ([vp, vp, vp, vp, vp, vp, i, d, C.POINTER(i), C.POINTER(d), vp], i),
             "fr_sim_create": ([C.c_char_p, C.c_char_p, C.POINTER(vp)], i),
+            "fr_sim_create_ex": ([C.c_char_p, C.c_char_p, i, C.POINTER(vp)], i),
             "fr_sim_destroy": ([vp], i),
             "fr_sim_initialize": ([vp], i),
             "fr_sim_step": ([vp], i),
@@ -109,14 +110,20 @@ def _p(a):
     return C.c_void_p(a.ctypes.data)
 
 
+FR_SIM_FP32 = 1
+
+
 class Sim:
     """The box solver held on the device (fr_sim_* of include/fr_amd.h)."""
 
-    def __init__(self, params_path: str, ic_dir: str | None = None):
+    def __init__(self, params_path: str, ic_dir: str | None = None, fp32: bool = False):
+        """fp32 selects the precision form (FR_SIM_FP32): FP32 fields and
+        stencils, FP64 pressure, residual, sums and particles."""
         self.L = lib()
         h = C.c_void_p()
-        check(self.L.fr_sim_create(params_path.encode(), (ic_dir or ".").encode(), C.byref(h)),
-              "fr_sim_create")
+        check(self.L.fr_sim_create_ex(params_path.encode(), (ic_dir or ".").encode(),
+                                      FR_SIM_FP32 if fp32 else 0, C.byref(h)),
+              "fr_sim_create_ex")
         self.h = h
         self.g = Geom()
         check(self.L.fr_sim_geom(self.h, C.byref(self.g)), "fr_sim_geom")

@@ -89,14 +89,16 @@ __device__ inline void load_tw(double2* tws, const double2* tw, int L) {
 // (poisson.cpp:393-403, poisson_IO.cpp:90-97), at interior cell c.  With INV
 // the arrays hold Rho_face and C = 1./Rho_face is formed inline, the value
 // grid::Solve_Poisson passes (grid.cpp:641).
-template <bool INV>
+// TC is the type of the coefficient arrays (float in the FP32 precision form,
+// promoted to double here); P and the right-hand side are always double.
+template <bool INV, typename TC = double>
 __device__ __forceinline__ double residual_t(const Dev& d, const double* __restrict__ P,
-                                             const double* __restrict__ cx,
-                                             const double* __restrict__ cy,
-                                             const double* __restrict__ cz,
+                                             const TC* __restrict__ cx,
+                                             const TC* __restrict__ cy,
+                                             const TC* __restrict__ cz,
                                              const double* __restrict__ RHS, long c) {
   const long sx = 1, sy = d.Nx, sz = d.NxNy;
-  auto ld = [](const double* a, long q) { return INV ? 1. / a[q] : a[q]; };
+  auto ld = [](const TC* a, long q) { return INV ? 1. / (double)a[q] : (double)a[q]; };
   const double Cx0 = ld(cx, c), Cx1 = ld(cx, c + sx);
   const double Cy0 = ld(cy, c), Cy1 = ld(cy, c + sy);
   const double Cz0 = ld(cz, c), Cz1 = ld(cz, c + sz);

@@ -39,11 +39,11 @@ struct PoisDev {
 // Pass X: per row (j,k) the residual, packed as z_m = r_2m + i r_2m+1, an
 // nx/2-point FFT, then the real-transform unpacking to X_k, k = 0..nx/2:
 //   E = (Z_k + conj Z_{M-k})/2, O = (Z_k - conj Z_{M-k})/(2i), X = E + W^k O.
-template <bool INV>
+template <bool INV, typename TC>
 __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __restrict__ P,
-                                                   const double* __restrict__ cx,
-                                                   const double* __restrict__ cy,
-                                                   const double* __restrict__ cz,
+                                                   const TC* __restrict__ cx,
+                                                   const TC* __restrict__ cy,
+                                                   const TC* __restrict__ cz,
                                                    const double* __restrict__ RHS,
                                                    const double2* __restrict__ twM,
                                                    const double2* __restrict__ twN,
@@ -66,8 +66,8 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
     if (row < rows) {
       const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
       const long c = at(d, 2 * m + 1, jj + 1, kk + 1);
-      r0 = residual_t<INV>(d, P, cx, cy, cz, RHS, c);
-      r1 = residual_t<INV>(d, P, cx, cy, cz, RHS, c + 1);
+      r0 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c);
+      r1 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c + 1);
     }
     buf[p * ps + bitrev(m, q.lxm)] = make_double2(r0, r1);
   }
@@ -239,15 +239,38 @@ struct fr_box_poisson {
   size_t lds[3] = {};  // dynamic LDS bytes of the x, y and z passes
 };
 
-static int box_poisson_iter(fr_box_poisson* s, const double* cx, const double* cy,
-                            const double* cz, bool inv, double* P, const double* RHS,
-                            hipStream_t st);
+// One defect-correction iteration; leaves max|dp| in s->dres[0].  With inv
+// the coefficient arrays hold Rho_face and C = 1./Rho_face.
+template <typename TC>
+static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz,
+                            bool inv, double* P, const double* RHS, hipStream_t st) {
+  PoisDev& q = s->q;
+  if (inv)
+    hipLaunchKernelGGL((k_pass_x_fwd<true, TC>), dim3(s->nbx), dim3(FT), s->lds[0], st, q, P, cx,
+                       cy, cz, RHS, s->twfm, s->twf[0], s->data);
+  else
+    hipLaunchKernelGGL((k_pass_x_fwd<false, TC>), dim3(s->nbx), dim3(FT), s->lds[0], st, q, P,
+                       cx, cy, cz, RHS, s->twfm, s->twf[0], s->data);
+  const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.nxh + q.wz - 1) / q.wz;
+  hipLaunchKernelGGL(k_pass_strided, dim3(ty * q.nz), dim3(FT), s->lds[1], st, q, 1, 0,
+                     s->twf[1], s->twf[1], s->lam[0], s->lam[1], s->lam[2], s->data);
+  hipLaunchKernelGGL(k_pass_strided, dim3(tz * q.ny), dim3(FT), s->lds[2], st, q, 2, 1,
+                     s->twf[2], s->twi[2], s->lam[0], s->lam[1], s->lam[2], s->data);
+  hipLaunchKernelGGL(k_pass_strided, dim3(ty * q.nz), dim3(FT), s->lds[1], st, q, 1, 0,
+                     s->twi[1], s->twi[1], s->lam[0], s->lam[1], s->lam[2], s->data);
+  hipLaunchKernelGGL(k_pass_x_inv, dim3(s->nbx), dim3(FT), s->lds[0], st, q, s->data, s->twim,
+                     s->twi[0], P, s->bmax);
+  FR_LAUNCH_CHECK();
+  int rc = max_final(s->bmax, s->nbx, s->dres, st);
+  return rc ? rc : ghost_fill(q.d, P, st);
+}
 
 namespace fr {
 
-int box_poisson_solve(fr_box_poisson* s, const double* cx, const double* cy, const double* cz,
-                      bool inv, double* P, const double* RHS, int iterations, double epsilon,
-                      int* num_iter, hipStream_t st) {
+template <typename TC>
+int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz, bool inv,
+                      double* P, const double* RHS, int iterations, double epsilon, int* num_iter,
+                      hipStream_t st) {
   if (!s || !cx || !cy || !cz || !P || !RHS) return FR_EINVAL;
   int i = 0;
   double error = 100;
@@ -262,6 +285,12 @@ int box_poisson_solve(fr_box_poisson* s, const double* cx, const double* cy, con
   if (num_iter) *num_iter = i;
   return FR_OK;
 }
+template int box_poisson_solve<double>(fr_box_poisson*, const double*, const double*,
+                                       const double*, bool, double*, const double*, int, double,
+                                       int*, hipStream_t);
+template int box_poisson_solve<float>(fr_box_poisson*, const float*, const float*, const float*,
+                                      bool, double*, const double*, int, double, int*,
+                                      hipStream_t);
 
 }  // namespace fr
 
@@ -335,9 +364,11 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
   s->lds[2] = sizeof(double2) * (q.wz * (n[2] + 1) + 2 * (n[2] / 2 + 1));
   const size_t need = std::max({s->lds[0], s->lds[1], s->lds[2]});
   if (need > 65536) {
-    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<false>,
+    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<false, double>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<true>,
+    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<true, double>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<true, float>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
     FR_CHECK(hipFuncSetAttribute((const void*)k_pass_strided,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
@@ -367,32 +398,6 @@ int fr_box_poisson_destroy(fr_box_poisson* s) {
   (void)hipHostFree(s->hres);
   delete s;
   return FR_OK;
-}
-
-// One defect-correction iteration; leaves max|dp| in s->dres[0].  With inv
-// the coefficient arrays hold Rho_face and C = 1./Rho_face.
-static int box_poisson_iter(fr_box_poisson* s, const double* cx, const double* cy,
-                            const double* cz, bool inv, double* P, const double* RHS,
-                            hipStream_t st) {
-  PoisDev& q = s->q;
-  if (inv)
-    hipLaunchKernelGGL(k_pass_x_fwd<true>, dim3(s->nbx), dim3(FT), s->lds[0], st, q, P, cx, cy,
-                       cz, RHS, s->twfm, s->twf[0], s->data);
-  else
-    hipLaunchKernelGGL(k_pass_x_fwd<false>, dim3(s->nbx), dim3(FT), s->lds[0], st, q, P, cx, cy,
-                       cz, RHS, s->twfm, s->twf[0], s->data);
-  const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.nxh + q.wz - 1) / q.wz;
-  hipLaunchKernelGGL(k_pass_strided, dim3(ty * q.nz), dim3(FT), s->lds[1], st, q, 1, 0,
-                     s->twf[1], s->twf[1], s->lam[0], s->lam[1], s->lam[2], s->data);
-  hipLaunchKernelGGL(k_pass_strided, dim3(tz * q.ny), dim3(FT), s->lds[2], st, q, 2, 1,
-                     s->twf[2], s->twi[2], s->lam[0], s->lam[1], s->lam[2], s->data);
-  hipLaunchKernelGGL(k_pass_strided, dim3(ty * q.nz), dim3(FT), s->lds[1], st, q, 1, 0,
-                     s->twi[1], s->twi[1], s->lam[0], s->lam[1], s->lam[2], s->data);
-  hipLaunchKernelGGL(k_pass_x_inv, dim3(s->nbx), dim3(FT), s->lds[0], st, q, s->data, s->twim,
-                     s->twi[0], P, s->bmax);
-  FR_LAUNCH_CHECK();
-  int rc = max_final(s->bmax, s->nbx, s->dres, st);
-  return rc ? rc : ghost_fill(q.d, P, st);
 }
 
 int fr_box_poisson_solve(fr_box_poisson* s, const double* cx, const double* cy,

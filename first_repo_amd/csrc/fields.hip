@@ -30,52 +30,50 @@ enum StencilOp {
   ST_COPY,      // Update_Ghosts alone
 };
 
-template <int OP>
-__device__ __forceinline__ double stencil_at(const Dev& d, const double* __restrict__ a,
-                                             const double* __restrict__ b,
-                                             const double* __restrict__ c, int i,
-                                             int j, int k) {
+template <int OP, typename T>
+__device__ __forceinline__ T stencil_at(const Dev& d, const T* __restrict__ a,
+                                        const T* __restrict__ b, const T* __restrict__ c, int i,
+                                        int j, int k) {
+  const T idx = (T)d.idx, idy = (T)d.idy, idz = (T)d.idz;
+  const T idx2 = (T)d.idx2, idy2 = (T)d.idy2, idz2 = (T)d.idz2;
+  const T h = (T)0.5;
   const long n = at(d, i, j, k);
   const long sx = 1, sy = d.Nx, sz = d.NxNy;
   if (OP == ST_DIV)
-    return (a[n + sx] - a[n]) * d.idx + (b[n + sy] - b[n]) * d.idy +
-           (c[n + sz] - c[n]) * d.idz;
+    return (a[n + sx] - a[n]) * idx + (b[n + sy] - b[n]) * idy + (c[n + sz] - c[n]) * idz;
   if (OP == ST_DEL2)
-    return (a[n + sx] - 2 * a[n] + a[n - sx]) * d.idx2 +
-           (a[n + sy] - 2 * a[n] + a[n - sy]) * d.idy2 +
-           (a[n + sz] - 2 * a[n] + a[n - sz]) * d.idz2;
-  if (OP == ST_IX_C2F) return (a[n] + a[n - sx]) * 0.5;
-  if (OP == ST_IY_C2F) return (a[n] + a[n - sy]) * 0.5;
-  if (OP == ST_IZ_C2F) return (a[n] + a[n - sz]) * 0.5;
-  if (OP == ST_IX_F2C) return (a[n] + a[n + sx]) * 0.5;
-  if (OP == ST_IY_F2C) return (a[n] + a[n + sy]) * 0.5;
-  if (OP == ST_IZ_F2C) return (a[n] + a[n + sz]) * 0.5;
+    return (a[n + sx] - 2 * a[n] + a[n - sx]) * idx2 + (a[n + sy] - 2 * a[n] + a[n - sy]) * idy2 +
+           (a[n + sz] - 2 * a[n] + a[n - sz]) * idz2;
+  if (OP == ST_IX_C2F) return (a[n] + a[n - sx]) * h;
+  if (OP == ST_IY_C2F) return (a[n] + a[n - sy]) * h;
+  if (OP == ST_IZ_C2F) return (a[n] + a[n - sz]) * h;
+  if (OP == ST_IX_F2C) return (a[n] + a[n + sx]) * h;
+  if (OP == ST_IY_F2C) return (a[n] + a[n + sy]) * h;
+  if (OP == ST_IZ_F2C) return (a[n] + a[n + sz]) * h;
   if (OP == ST_I_F2C) {
-    const double OneThird = 1. / 3.;
-    return OneThird * ((a[n] + a[n + sx]) * 0.5 + (b[n] + b[n + sy]) * 0.5 +
-                       (c[n] + c[n + sz]) * 0.5);
+    const T OneThird = (T)(1. / 3.);
+    return OneThird * ((a[n] + a[n + sx]) * h + (b[n] + b[n + sy]) * h + (c[n] + c[n + sz]) * h);
   }
-  if (OP == ST_GX) return (a[n] - a[n - sx]) * d.idx;
-  if (OP == ST_GY) return (a[n] - a[n - sy]) * d.idy;
-  if (OP == ST_GZ) return (a[n] - a[n - sz]) * d.idz;
-  return 0.0;
+  if (OP == ST_GX) return (a[n] - a[n - sx]) * idx;
+  if (OP == ST_GY) return (a[n] - a[n - sy]) * idy;
+  if (OP == ST_GZ) return (a[n] - a[n - sz]) * idz;
+  return (T)0;
 }
 
-template <int OP>
-__global__ void __launch_bounds__(256) k_stencil(Dev d, double* __restrict__ out,
-                                                 const double* __restrict__ a,
-                                                 const double* __restrict__ b,
-                                                 const double* __restrict__ c,
-                                                 int xlo, int xhi) {
+template <int OP, typename T>
+__global__ void __launch_bounds__(256) k_stencil(Dev d, T* __restrict__ out,
+                                                 const T* __restrict__ a,
+                                                 const T* __restrict__ b,
+                                                 const T* __restrict__ c, int xlo, int xhi) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y * 4 + threadIdx.y;
   const int k = blockIdx.z;
   if (i >= d.Nx || j >= d.Ny) return;
   const int jj = img(j, d.Ny), kk = img(k, d.Nz);
   const int ii = d.box ? img(i, d.Nx) : i;
-  double v;
+  T v;
   if (OP != ST_COPY && ii >= xlo && ii < xhi) {
-    v = stencil_at<OP>(d, a, b, c, ii, jj, kk);
+    v = stencil_at<OP, T>(d, a, b, c, ii, jj, kk);
   } else {
     if (ii == i && jj == j && kk == k) return;
     v = out[at(d, ii, jj, kk)];
@@ -85,21 +83,27 @@ __global__ void __launch_bounds__(256) k_stencil(Dev d, double* __restrict__ out
 
 static dim3 grid3(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 4), d.Nz); }
 
-template <int OP>
-static int launch_stencil(const fr_geom* g, double* out, const double* a,
-                          const double* b, const double* c, int xlo, int xhi,
-                          void* s) {
+template <typename T>
+struct nodeduce {
+  using type = T;
+};
+
+template <int OP, typename T>
+static int launch_stencil(const fr_geom* g, T* out, const typename nodeduce<T>::type* a,
+                          const typename nodeduce<T>::type* b,
+                          const typename nodeduce<T>::type* c, int xlo, int xhi, void* s) {
   if (!geom_ok(g) || !out) return FR_EINVAL;
   Dev d = make_dev(g);
-  hipLaunchKernelGGL(k_stencil<OP>, grid3(d), dim3(64, 4), 0, S(s), d, out, a, b, c,
-                     xlo, xhi);
+  hipLaunchKernelGGL((k_stencil<OP, T>), grid3(d), dim3(64, 4), 0, S(s), d, out, a, b, c, xlo,
+                     xhi);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
 
 // Ghost shell only: k faces (all i,j), j faces (interior k, all i), and for
 // the box the i faces (interior j,k).
-__global__ void k_shell(Dev d, double* __restrict__ f) {
+template <typename T>
+__global__ void k_shell(Dev d, T* __restrict__ f) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long nk = 2 * d.NxNy;
   const long nj = 2L * d.Nx * (d.Nz - 2);
@@ -129,63 +133,72 @@ __global__ void k_shell(Dev d, double* __restrict__ f) {
   f[at(d, i, j, k)] = f[at(d, ii, img(j, d.Ny), img(k, d.Nz))];
 }
 
-int ghost_fill(const Dev& d, double* f, hipStream_t s) {
+template <typename T>
+int ghost_fill(const Dev& d, T* f, hipStream_t s) {
   long n = 2 * d.NxNy + 2L * d.Nx * (d.Nz - 2) + (d.box ? 2L * (d.Ny - 2) * (d.Nz - 2) : 0);
-  hipLaunchKernelGGL(k_shell, dim3(nblocks(n, 256)), dim3(256), 0, s, d, f);
+  hipLaunchKernelGGL(k_shell<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, d, f);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
+template int ghost_fill<double>(const Dev&, double*, hipStream_t);
+template int ghost_fill<float>(const Dev&, float*, hipStream_t);
 
 // Cumulative halo on one rank (communicator.cpp:422-511 box, 520-626 duct):
 // z phase (all i,j), y phase (all i, interior k), x phase (box; interior
 // j,k).  Each thread reads both ghosts of its line before adding, as the
 // reference packs both send buffers before unpacking.
-__global__ void k_cum_z(Dev d, double* __restrict__ f) {
+template <typename T>
+__global__ void k_cum_z(Dev d, T* __restrict__ f) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= d.NxNy) return;
   const long top = (long)(d.Nz - 1) * d.NxNy + t, bot = t;
-  const double s0 = f[top], s1 = f[bot];
+  const T s0 = f[top], s1 = f[bot];
   f[(long)(d.Nz - 2) * d.NxNy + t] += s1;
   f[d.NxNy + t] += s0;
 }
-__global__ void k_cum_y(Dev d, double* __restrict__ f) {
+template <typename T>
+__global__ void k_cum_y(Dev d, T* __restrict__ f) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long n = (long)d.Nx * (d.Nz - 2);
   if (t >= n) return;
   const int i = (int)(t % d.Nx), k = 1 + (int)(t / d.Nx);
-  const double s0 = f[at(d, i, d.Ny - 1, k)], s1 = f[at(d, i, 0, k)];
+  const T s0 = f[at(d, i, d.Ny - 1, k)], s1 = f[at(d, i, 0, k)];
   f[at(d, i, d.Ny - 2, k)] += s1;
   f[at(d, i, 1, k)] += s0;
 }
-__global__ void k_cum_x(Dev d, double* __restrict__ f) {
+template <typename T>
+__global__ void k_cum_x(Dev d, T* __restrict__ f) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long n = (long)(d.Ny - 2) * (d.Nz - 2);
   if (t >= n) return;
   const int j = 1 + (int)(t % (d.Ny - 2)), k = 1 + (int)(t / (d.Ny - 2));
-  const double s0 = f[at(d, d.Nx - 1, j, k)], s1 = f[at(d, 0, j, k)];
+  const T s0 = f[at(d, d.Nx - 1, j, k)], s1 = f[at(d, 0, j, k)];
   f[at(d, d.Nx - 2, j, k)] += s1;
   f[at(d, 1, j, k)] += s0;
 }
 
-int cum_fold(const Dev& d, double* f, hipStream_t s) {
-  hipLaunchKernelGGL(k_cum_z, dim3(nblocks(d.NxNy, 256)), dim3(256), 0, s, d, f);
-  hipLaunchKernelGGL(k_cum_y, dim3(nblocks((long)d.Nx * (d.Nz - 2), 256)), dim3(256), 0,
-                     s, d, f);
+template <typename T>
+int cum_fold(const Dev& d, T* f, hipStream_t s) {
+  hipLaunchKernelGGL(k_cum_z<T>, dim3(nblocks(d.NxNy, 256)), dim3(256), 0, s, d, f);
+  hipLaunchKernelGGL(k_cum_y<T>, dim3(nblocks((long)d.Nx * (d.Nz - 2), 256)), dim3(256), 0, s,
+                     d, f);
   if (d.box)
-    hipLaunchKernelGGL(k_cum_x, dim3(nblocks((long)(d.Ny - 2) * (d.Nz - 2), 256)),
+    hipLaunchKernelGGL(k_cum_x<T>, dim3(nblocks((long)(d.Ny - 2) * (d.Nz - 2), 256)),
                        dim3(256), 0, s, d, f);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
+template int cum_fold<double>(const Dev&, double*, hipStream_t);
+template int cum_fold<float>(const Dev&, float*, hipStream_t);
 
 // Element-wise forms of tensor0.cpp:93-241 (see fr_t0_elementwise).
-__global__ void k_elem(long n, int op, double* __restrict__ out, double a,
-                       const double* __restrict__ A, double b,
-                       const double* __restrict__ B, double c,
-                       const double* __restrict__ C) {
+template <typename T>
+__global__ void k_elem(long n, int op, T* __restrict__ out, double a_, const T* __restrict__ A,
+                       double b_, const T* __restrict__ B, double c_, const T* __restrict__ C) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  double v;
+  const T a = (T)a_, b = (T)b_, c = (T)c_;
+  T v;
   switch (op) {
     case 0:
       v = a * A[t];
@@ -237,9 +250,9 @@ __device__ double block_reduce(double v, double* sh) {
   return sh[0];
 }
 
-template <int OP>
-__global__ void __launch_bounds__(RB) k_reduce_part(Dev d, const double* __restrict__ f,
-                                                    double init, double* __restrict__ part) {
+template <int OP, typename T>
+__global__ void __launch_bounds__(RB) k_reduce_part(Dev d, const T* __restrict__ f, double init,
+                                                    double* __restrict__ part) {
   __shared__ double sh[RB];
   const int ni = d.Nx - 2, nj = d.Ny - 2, nk = d.Nz - 2;
   const long n = (long)ni * nj * nk;
@@ -248,7 +261,7 @@ __global__ void __launch_bounds__(RB) k_reduce_part(Dev d, const double* __restr
     const int i = 1 + (int)(t % ni);
     const long r = t / ni;
     const int j = 1 + (int)(r % nj), k = 1 + (int)(r / nj);
-    acc = rcomb<OP>(acc, rmap<OP>(f[at(d, i, j, k)]));
+    acc = rcomb<OP>(acc, rmap<OP>((double)f[at(d, i, j, k)]));
   }
   double r = block_reduce<OP>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
@@ -276,31 +289,35 @@ static int red_ws() {
   return FR_OK;
 }
 
-// Launch the reduction and leave the result in device memory at *dres.
-int reduce_async(const Dev& d, const double* f, int op, double* dres, hipStream_t s) {
+// Launch the reduction and leave the result in device memory at *dres.  Sums
+// and extrema are formed in double whatever the field type.
+template <typename T>
+int reduce_async(const Dev& d, const T* f, int op, double* dres, hipStream_t s) {
   int rc = red_ws();
   if (rc) return rc;
   const double first = 0.0;
   switch (op) {
     case 0:
     case 1:
-      hipLaunchKernelGGL(k_reduce_part<0>, dim3(RG), dim3(RB), 0, s, d, f, first, g_red_ws);
+      hipLaunchKernelGGL((k_reduce_part<0, T>), dim3(RG), dim3(RB), 0, s, d, f, first, g_red_ws);
       hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(RB), 0, s, g_red_ws, first, dres);
       break;
     case 5:
-      hipLaunchKernelGGL(k_reduce_part<5>, dim3(RG), dim3(RB), 0, s, d, f, first, g_red_ws);
+      hipLaunchKernelGGL((k_reduce_part<5, T>), dim3(RG), dim3(RB), 0, s, d, f, first, g_red_ws);
       hipLaunchKernelGGL(k_reduce_final<5>, dim3(1), dim3(RB), 0, s, g_red_ws, first, dres);
       break;
     case 2:
-      hipLaunchKernelGGL(k_reduce_part<2>, dim3(RG), dim3(RB), 0, s, d, f, -HUGE_VAL, g_red_ws);
+      hipLaunchKernelGGL((k_reduce_part<2, T>), dim3(RG), dim3(RB), 0, s, d, f, -HUGE_VAL,
+                         g_red_ws);
       hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), dim3(RB), 0, s, g_red_ws, -HUGE_VAL, dres);
       break;
     case 3:
-      hipLaunchKernelGGL(k_reduce_part<3>, dim3(RG), dim3(RB), 0, s, d, f, HUGE_VAL, g_red_ws);
+      hipLaunchKernelGGL((k_reduce_part<3, T>), dim3(RG), dim3(RB), 0, s, d, f, HUGE_VAL,
+                         g_red_ws);
       hipLaunchKernelGGL(k_reduce_final<3>, dim3(1), dim3(RB), 0, s, g_red_ws, HUGE_VAL, dres);
       break;
     case 4:
-      hipLaunchKernelGGL(k_reduce_part<4>, dim3(RG), dim3(RB), 0, s, d, f, 0.0, g_red_ws);
+      hipLaunchKernelGGL((k_reduce_part<4, T>), dim3(RG), dim3(RB), 0, s, d, f, 0.0, g_red_ws);
       hipLaunchKernelGGL(k_reduce_final<4>, dim3(1), dim3(RB), 0, s, g_red_ws, 0.0, dres);
       break;
     default:
@@ -309,8 +326,11 @@ int reduce_async(const Dev& d, const double* f, int op, double* dres, hipStream_
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
+template int reduce_async<double>(const Dev&, const double*, int, double*, hipStream_t);
+template int reduce_async<float>(const Dev&, const float*, int, double*, hipStream_t);
 
-int reduce_sync(const Dev& d, const double* f, int op, double* result, hipStream_t s) {
+template <typename T>
+int reduce_sync(const Dev& d, const T* f, int op, double* result, hipStream_t s) {
   int rc = red_ws();
   if (rc) return rc;
   rc = reduce_async(d, f, op, g_red_ws + RG, s);
@@ -322,6 +342,8 @@ int reduce_sync(const Dev& d, const double* f, int op, double* result, hipStream
   *result = v;
   return FR_OK;
 }
+template int reduce_sync<double>(const Dev&, const double*, int, double*, hipStream_t);
+template int reduce_sync<float>(const Dev&, const float*, int, double*, hipStream_t);
 
 // communicator.cpp:635-646: one block per x index i.
 __global__ void __launch_bounds__(RB) k_reducex(Dev d, const double* __restrict__ f,
@@ -345,13 +367,40 @@ int reducex(const Dev& d, const double* f, double* out, hipStream_t s) {
   return FR_OK;
 }
 
-int elementwise(long n, int op, double* out, double a, const double* A, double b,
-                const double* B, double c, const double* C, hipStream_t s) {
-  hipLaunchKernelGGL(k_elem, dim3(nblocks(n, 256)), dim3(256), 0, s, n, op, out, a, A, b,
-                     B, c, C);
+template <typename T>
+int elementwise(long n, int op, T* out, double a, const T* A, double b, const T* B, double c,
+                const T* C, hipStream_t s) {
+  hipLaunchKernelGGL(k_elem<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, n, op, out, a, A, b, B,
+                     c, C);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
+template int elementwise<double>(long, int, double*, double, const double*, double,
+                                 const double*, double, const double*, hipStream_t);
+template int elementwise<float>(long, int, float*, double, const float*, double, const float*,
+                                double, const float*, hipStream_t);
+
+__global__ void k_f2d(long n, const float* __restrict__ a, double* __restrict__ b) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (double)a[t];
+}
+
+int to_double(const float* src, double* dst, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_f2d, dim3(nblocks(n, 256)), dim3(256), 0, s, n, src, dst);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+// Equal_I{x,y,z}_C2F (tensor0.cpp:353-378) into `out`, any field type.
+template <typename T>
+int interp_c2f(const fr_geom* g, T* out, const T* in, int axis, hipStream_t s) {
+  const int Nx = g->Nx;
+  if (axis == 0) return launch_stencil<ST_IX_C2F, T>(g, out, in, nullptr, nullptr, 1, Nx, s);
+  if (axis == 1) return launch_stencil<ST_IY_C2F, T>(g, out, in, nullptr, nullptr, 0, Nx, s);
+  return launch_stencil<ST_IZ_C2F, T>(g, out, in, nullptr, nullptr, 0, Nx, s);
+}
+template int interp_c2f<double>(const fr_geom*, double*, const double*, int, hipStream_t);
+template int interp_c2f<float>(const fr_geom*, float*, const float*, int, hipStream_t);
 
 }  // namespace fr
 

@@ -6,13 +6,23 @@
 namespace fr {
 
 // ---- fields (fields.hip) -------------------------------------------------------
-int ghost_fill(const Dev& d, double* f, hipStream_t s);
-int cum_fold(const Dev& d, double* f, hipStream_t s);
-int reduce_async(const Dev& d, const double* f, int op, double* dres, hipStream_t s);
-int reduce_sync(const Dev& d, const double* f, int op, double* result, hipStream_t s);
+// Templates instantiated for double (the FP64 path) and float (the FP32
+// precision form); reductions accumulate in double for both.
+template <typename T>
+int ghost_fill(const Dev& d, T* f, hipStream_t s);
+template <typename T>
+int cum_fold(const Dev& d, T* f, hipStream_t s);
+template <typename T>
+int reduce_async(const Dev& d, const T* f, int op, double* dres, hipStream_t s);
+template <typename T>
+int reduce_sync(const Dev& d, const T* f, int op, double* result, hipStream_t s);
 int reducex(const Dev& d, const double* f, double* out, hipStream_t s);
-int elementwise(long n, int op, double* out, double a, const double* A, double b,
-                const double* B, double c, const double* C, hipStream_t s);
+template <typename T>
+int elementwise(long n, int op, T* out, double a, const T* A, double b, const T* B, double c,
+                const T* C, hipStream_t s);
+template <typename T>
+int interp_c2f(const fr_geom* g, T* out, const T* in, int axis, hipStream_t s);
+int to_double(const float* src, double* dst, long n, hipStream_t s);
 
 // ---- fused box substep kernels (box.hip) ------------------------------------------
 struct BoxCoef {
@@ -42,19 +52,20 @@ struct DivCoef {
   double s;      // 1./(dt*pre)
 };
 
-int box_update_rho(const Dev& d, const BoxCoef& cf, const double* const ru_int[3],
-                   const double* rho, double* rho_np1, double* rho_new,
-                   const double* const rf[3], double* const u[3], hipStream_t s);
-int box_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w,
-                      const double* const u[3], const double* const ru_int[3],
-                      const double* const S[3], const double* const ru[3],
-                      double* const ru_np1[3], double* const ru_new[3],
-                      const double* const rf[3], hipStream_t s);
-int box_div_rhs(const Dev& d, const DivCoef& q, const double* rho_new, const double* Q,
-                const double* const ru_new[3], double* const rf[3], double* rhs,
-                hipStream_t s);
-int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P,
-                     double* const ru_np1[3], double* const ru_new[3], hipStream_t s);
+template <typename T>
+int box_update_rho(const Dev& d, const BoxCoef& cf, const T* const ru_int[3], const T* rho,
+                   T* rho_np1, T* rho_new, const T* const rf[3], T* const u[3], hipStream_t s);
+template <typename T>
+int box_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w, const T* const u[3],
+                      const T* const ru_int[3], const T* const S[3], const T* const ru[3],
+                      T* const ru_np1[3], T* const ru_new[3], const T* const rf[3],
+                      hipStream_t s);
+template <typename T>
+int box_div_rhs(const Dev& d, const DivCoef& q, const T* rho_new, const T* Q,
+                const T* const ru_new[3], T* const rf[3], double* rhs, hipStream_t s);
+template <typename T>
+int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, T* const ru_np1[3],
+                     T* const ru_new[3], hipStream_t s);
 
 // ---- duct substep kernels (duct.hip) ----------------------------------------------
 int duct_update_rho(const Dev& d, const BoxCoef& cf, double cbc, const double* const ru_int[3],
@@ -80,9 +91,11 @@ int duct_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, double* 
 // ---- box pressure solve (fft_poisson.hip) -------------------------------------
 // fr_box_poisson_solve with, when inv is set, Rho_face passed in place of C
 // and C = 1./Rho_face formed inside the residual (grid.cpp:641).
-int box_poisson_solve(fr_box_poisson* s, const double* cx, const double* cy, const double* cz,
-                      bool inv, double* P, const double* RHS, int iterations, double epsilon,
-                      int* num_iter, hipStream_t st);
+// TC: double, or float for the FP32 precision form (P and RHS stay double).
+template <typename TC>
+int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz, bool inv,
+                      double* P, const double* RHS, int iterations, double epsilon, int* num_iter,
+                      hipStream_t st);
 
 // fr_duct_poisson_solve with the same Rho_face option (duct_poisson.hip).
 int duct_poisson_solve(fr_duct_poisson* s, const double* cx, const double* cy, const double* cz,
@@ -154,14 +167,18 @@ class PartDev {
   int alloc(int capacity, const Dev& d);
   void release();
   PartPtrs ptrs() const;
-  int advance(const Dev& d, const AdvCoef& c, const double* const u[3], hipStream_t st);
-  int temp(const Dev& d, int stage, double c0, const double* rho, double iTth, hipStream_t st);
+  template <typename F>
+  int advance(const Dev& d, const AdvCoef& c, const F* const u[3], hipStream_t st);
+  template <typename F>
+  int temp(const Dev& d, int stage, double c0, const F* rho, double iTth, hipStream_t st);
   // buckets of the `stage` positions; mode 0 rounded (R), 1 truncating (T)
   int build(const Dev& d, int stage, int mode, hipStream_t st, bool operands = true);
   // trilinear deposit from R: weighted by RHS_T (heat) or not (concentration)
-  int deposit_cic(const Dev& d, bool weighted, double* out, bool halo, hipStream_t st);
+  template <typename F>
+  int deposit_cic(const Dev& d, bool weighted, F* out, bool halo, hipStream_t st);
   // momentum deposit from T
-  int deposit_vel(const Dev& d, double* const out[3], hipStream_t st);
+  template <typename F>
+  int deposit_vel(const Dev& d, F* const out[3], hipStream_t st);
   // periodic migration (box); with duct set, x leavers are dropped and the
   // n_in inlet particles of `inj` (29 values each, host memory) appended
   int migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st,

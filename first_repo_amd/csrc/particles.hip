@@ -31,8 +31,9 @@ namespace fr {
 __device__ __forceinline__ long clampi(long q, long n) { return q < 0 ? 0 : (q >= n ? n - 1 : q); }
 
 // particle.cpp:314-399, 480-496 (one RK4 stage) with the gather 340-360.
-__global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const double* __restrict__ ux,
-                               const double* __restrict__ uy, const double* __restrict__ uz) {
+template <typename T>
+__global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict__ ux,
+                               const T* __restrict__ uy, const T* __restrict__ uz) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.np) return;
   const double xi = p.s[0][1][i], yi = p.s[1][1][i], zi = p.s[2][1][i];
@@ -42,9 +43,12 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const double* __res
   const double z_l = (zi * d.idz - ((int)(zi * d.idz)));
   const int ii = ti + 1, jj = tj + 1, kk = tk + 1;
   const long q = (long)kk * d.NxNy + (long)jj * d.Nx + ii;
-  const double ug = x_l * ux[clampi(q + 1, d.size)] + (1 - x_l) * ux[clampi(q, d.size)];
-  const double vg = y_l * uy[clampi(q + d.Nx, d.size)] + (1 - y_l) * uy[clampi(q, d.size)];
-  const double wg = z_l * uz[clampi(q + d.NxNy, d.size)] + (1 - z_l) * uz[clampi(q, d.size)];
+  const double ug = x_l * (double)ux[clampi(q + 1, d.size)] +
+                    (1 - x_l) * (double)ux[clampi(q, d.size)];
+  const double vg = y_l * (double)uy[clampi(q + d.Nx, d.size)] +
+                    (1 - y_l) * (double)uy[clampi(q, d.size)];
+  const double wg = z_l * (double)uz[clampi(q + d.NxNy, d.size)] +
+                    (1 - z_l) * (double)uz[clampi(q, d.size)];
   // positions
   const double ui = p.s[3][1][i], vi = p.s[4][1][i], wi = p.s[5][1][i];
   {
@@ -80,7 +84,8 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const double* __res
 
 // particle.cpp:402-478: Tg at the stage `st` position (1 = _int, 2 = _new),
 // T = c0/rho (grid.cpp:676, c0 = P0/R), RHS_T = (Tg - T_stage)*iTth.
-__global__ void k_part_temp(Dev d, PartPtrs p, int st, double c0, const double* __restrict__ rho,
+template <typename F>
+__global__ void k_part_temp(Dev d, PartPtrs p, int st, double c0, const F* __restrict__ rho,
                             double iTth) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.np) return;
@@ -95,7 +100,7 @@ __global__ void k_part_temp(Dev d, PartPtrs p, int st, double c0, const double* 
   jj += 1;
   kk += 1;
   auto T = [&](int a, int b, int e) {
-    return c0 / rho[clampi((long)e * d.NxNy + (long)b * d.Nx + a, d.size)];
+    return c0 / (double)rho[clampi((long)e * d.NxNy + (long)b * d.Nx + a, d.size)];
   };
   const double Tg = x_l * y_l * z_l * T(ii, jj, kk) +
                     (1 - x_l) * y_l * z_l * T(ii - 1, jj, kk) +
@@ -343,13 +348,14 @@ struct Merge {
 // the cumulative halo, in ascending particle order (the reference's scatter
 // order), from the rounded-index buckets.  Target (a,b,c) collects from the
 // bucket of cell (a+ox, b+oy, c+oz) the weight with (1-x_l) where ox = 1.
+template <typename T>
 __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ start,
                                                  const int* __restrict__ list,
                                                  const double* __restrict__ sxl,
                                                  const double* __restrict__ syl,
                                                  const double* __restrict__ szl,
                                                  const double* __restrict__ sw,
-                                                 double* __restrict__ out) {
+                                                 T* __restrict__ out) {
   __shared__ int ss[DROWS][DSR];
   const uint3 bk = xcd_block();
   const int a0 = bk.x * DTX, b0 = bk.y * DTY, c = bk.z;
@@ -382,7 +388,7 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
     if (sw) t = t * sw[pos];
     acc = acc + t;
   }
-  out[at(d, a, b, c)] = acc;
+  out[at(d, a, b, c)] = (T)acc;
 }
 
 // 1-D linear momentum deposit (particle.cpp:498-524) from the truncating-
@@ -416,6 +422,7 @@ __device__ __forceinline__ double dep_vel_comp(const int (*ss)[DSR], const int* 
   return acc;
 }
 
+template <typename T>
 __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ start,
                                                  const int* __restrict__ list,
                                                  const double* __restrict__ sxl,
@@ -424,8 +431,8 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ 
                                                  const double* __restrict__ sru,
                                                  const double* __restrict__ srv,
                                                  const double* __restrict__ srw,
-                                                 double* __restrict__ ox, double* __restrict__ oy,
-                                                 double* __restrict__ oz) {
+                                                 T* __restrict__ ox, T* __restrict__ oy,
+                                                 T* __restrict__ oz) {
   __shared__ int ss[DROWS][DSR];
   const uint3 bk = xcd_block();
   const int a0 = bk.x * DTX, b0 = bk.y * DTY, c = bk.z;
@@ -433,9 +440,9 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ 
   const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
   if (a >= d.Nx || b >= d.Ny) return;
   const long n = at(d, a, b, c);
-  ox[n] = dep_vel_comp<0>(ss, list, sxl, sru, a);
-  oy[n] = dep_vel_comp<1>(ss, list, syl, srv, b);
-  oz[n] = dep_vel_comp<2>(ss, list, szl, srw, c);
+  ox[n] = (T)dep_vel_comp<0>(ss, list, sxl, sru, a);
+  oy[n] = (T)dep_vel_comp<1>(ss, list, syl, srv, b);
+  oz[n] = (T)dep_vel_comp<2>(ss, list, szl, srw, c);
 }
 
 // ---- migration (one rank) -------------------------------------------------------
@@ -732,18 +739,20 @@ PartPtrs PartDev::ptrs() const {
   return p;
 }
 
-int PartDev::advance(const Dev& d, const AdvCoef& c, const double* const u[3], hipStream_t st) {
+template <typename T>
+int PartDev::advance(const Dev& d, const AdvCoef& c, const T* const u[3], hipStream_t st) {
   if (np == 0) return FR_OK;
-  hipLaunchKernelGGL(k_part_advance, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), c,
+  hipLaunchKernelGGL(k_part_advance<T>, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), c,
                      u[0], u[1], u[2]);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
 
-int PartDev::temp(const Dev& d, int stage, double c0, const double* rho, double iTth,
+template <typename T>
+int PartDev::temp(const Dev& d, int stage, double c0, const T* rho, double iTth,
                   hipStream_t st) {
   if (np == 0) return FR_OK;
-  hipLaunchKernelGGL(k_part_temp, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), stage,
+  hipLaunchKernelGGL(k_part_temp<T>, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), stage,
                      c0, rho, iTth);
   FR_LAUNCH_CHECK();
   return FR_OK;
@@ -774,9 +783,10 @@ int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool opera
 
 static dim3 grid3p(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 4), d.Nz); }
 
-int PartDev::deposit_cic(const Dev& d, bool weighted, double* out, bool halo, hipStream_t st) {
+template <typename T>
+int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStream_t st) {
   if (!R.valid) return FR_EINVAL;
-  hipLaunchKernelGGL(k_dep_cic, grid3p(d), dim3(64, 4), 0, st, d, R.start, R.list, R.s[0],
+  hipLaunchKernelGGL(k_dep_cic<T>, grid3p(d), dim3(64, 4), 0, st, d, R.start, R.list, R.s[0],
                      R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out);
   FR_LAUNCH_CHECK();
   int rc = cum_fold(d, out, st);
@@ -784,9 +794,10 @@ int PartDev::deposit_cic(const Dev& d, bool weighted, double* out, bool halo, hi
   return rc;
 }
 
-int PartDev::deposit_vel(const Dev& d, double* const out[3], hipStream_t st) {
+template <typename F>
+int PartDev::deposit_vel(const Dev& d, F* const out[3], hipStream_t st) {
   if (!T.valid) return FR_EINVAL;
-  hipLaunchKernelGGL(k_dep_vel, grid3p(d), dim3(64, 4), 0, st, d, T.start, T.list, T.s[0],
+  hipLaunchKernelGGL(k_dep_vel<F>, grid3p(d), dim3(64, 4), 0, st, d, T.start, T.list, T.s[0],
                      T.s[1], T.s[2], T.s[3], T.s[4], T.s[5], out[0], out[1], out[2]);
   FR_LAUNCH_CHECK();
   for (int a = 0; a < 3; a++) {
@@ -957,5 +968,16 @@ int PartDev::set_flags(const int* host, hipStream_t st) {
   FR_CHECK(hipStreamSynchronize(st));
   return FR_OK;
 }
+
+template int PartDev::advance<double>(const Dev&, const AdvCoef&, const double* const[3],
+                                      hipStream_t);
+template int PartDev::advance<float>(const Dev&, const AdvCoef&, const float* const[3],
+                                     hipStream_t);
+template int PartDev::temp<double>(const Dev&, int, double, const double*, double, hipStream_t);
+template int PartDev::temp<float>(const Dev&, int, double, const float*, double, hipStream_t);
+template int PartDev::deposit_cic<double>(const Dev&, bool, double*, bool, hipStream_t);
+template int PartDev::deposit_cic<float>(const Dev&, bool, float*, bool, hipStream_t);
+template int PartDev::deposit_vel<double>(const Dev&, double* const[3], hipStream_t);
+template int PartDev::deposit_vel<float>(const Dev&, float* const[3], hipStream_t);
 
 }  // namespace fr

@@ -103,11 +103,15 @@ class gridsize {
   fr::Dev dev() const { return fr::make_dev(&g); }
 };
 
-// tensor0: a ghosted scalar field in device memory (tensor0.h:6-132).
+// tensor0: a ghosted scalar field in device memory (tensor0.h:6-132), FP64
+// or, in the FP32 precision form, FP32 storage.
 class tensor0 {
   gridsize* p_ = nullptr;
-  double* ptr = nullptr;
+  void* ptr = nullptr;
   hipStream_t st_ = 0;
+  bool f32 = false;
+  size_t es() const { return f32 ? sizeof(float) : sizeof(double); }
+  double reduce(int op);
 
  public:
   tensor0() {}
@@ -115,10 +119,18 @@ class tensor0 {
   ~tensor0();
   tensor0(const tensor0&) = delete;
   tensor0& operator=(const tensor0&) = delete;
-  void init(gridsize* p, hipStream_t st);
-  double* pointer() const { return ptr; }
+  void init(gridsize* p, hipStream_t st, bool fp32 = false);
+  double* pointer() const { return (double*)ptr; }  // FP64 storage
+  template <typename T>
+  T* p() const {
+    return (T*)ptr;
+  }
+  bool is_f32() const { return f32; }
   int size() const { return p_->size(); }
-  void swap(tensor0& o) { std::swap(ptr, o.ptr); }
+  void swap(tensor0& o) {
+    std::swap(ptr, o.ptr);
+    std::swap(f32, o.f32);
+  }
   void Update_Ghosts();
   void Update_Ghosts_CUM();
   void assign(double a);        // operator=(double)
@@ -132,7 +144,7 @@ class tensor0 {
   double max();
   double min();
   double max_abs();
-  void download(double* host) const;
+  void download(double* host) const;  // converted to double
   void upload(const double* host);
 };
 
@@ -141,10 +153,10 @@ class tensor1 {
  public:
   tensor0 x, y, z;
   tensor1() {}
-  void init(gridsize* p, hipStream_t st) {
-    x.init(p, st);
-    y.init(p, st);
-    z.init(p, st);
+  void init(gridsize* p, hipStream_t st, bool fp32 = false) {
+    x.init(p, st, fp32);
+    y.init(p, st, fp32);
+    z.init(p, st, fp32);
   }
   tensor0& c(int m) { return m == 0 ? x : (m == 1 ? y : z); }
   void swap(tensor1& o) {
@@ -223,7 +235,7 @@ class poisson {
   ~poisson();
   int Solve(tensor1& C, tensor0& P, tensor0& RHS);  // poisson.cpp:115-117
   // Solve with C = 1./Rho_face (grid.cpp:641) without storing C
-  int Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS);
+  int Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS);  // P: FP64 storage
   int num_iteration() const { return num_iteration_; }
 };
 
@@ -242,6 +254,12 @@ class grid {
   tensor0 P, RHS_Pois, C, Rho, Rho_int, Rho_new, Rho_np1, RHS_Part_Temp, scratch, divergence;
   double U_avg_in = 0, U_avg_out = 0;
   double* inflow = nullptr;    // duct: 8 ghosted [Nz][Ny] planes (device)
+  // FP32 precision form (box): every field but P and RHS_Pois in FP32 storage
+  // and FP32 stencil arithmetic; P, the Poisson right-hand side and residual
+  // and all sums in FP64
+  bool f32 = false;
+  tensor1 d64a, d64b;  // FP64 copies for the diagnostics of the FP32 form
+  tensor0 d64r;
   std::vector<double> inlet;   // duct: inlet particles not yet received
   double P0 = 0, P0_int = 0, P0_new = 0, P0_np1 = 0, dP0_dt = 0, T_cur = 0;
   int num_timestep = 0, RK4_count = 0;
@@ -252,8 +270,8 @@ class grid {
   bool qmean_valid = false;
   int err = 0;  // first nonzero status of a method
   // device timers
-  hipEvent_t ev[4];
-  hipEvent_t evk[8];  // momentum kernel start/stop per substep
+  hipEvent_t ev[4] = {};
+  hipEvent_t evk[8] = {};  // momentum kernel start/stop per substep
   double t_poisson = 0, t_particles = 0, t_step = 0, t_momentum = 0;
   int n_momentum = 0;
   int iters_step = 0;

@@ -2,6 +2,7 @@
 // tensor0, tensor1, particle, poisson, grid) driving the gfx950 kernels, and
 // the fr_sim_* C entry points.  Compiled by hipcc with the kernels into
 // libfr_amd.so.
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -107,73 +108,116 @@ gridsize::gridsize(const params& p) {
 // ---------------------------------------------------------------- tensor0
 tensor0::tensor0(gridsize* p, hipStream_t st) { init(p, st); }
 
-void tensor0::init(gridsize* p, hipStream_t st) {
+void tensor0::init(gridsize* p, hipStream_t st, bool fp32) {
   p_ = p;
   st_ = st;
-  if (hipMalloc(&ptr, sizeof(double) * (size_t)p_->size()) != hipSuccess) {
-    fprintf(stderr, "fr_amd: tensor0 allocation of %d doubles failed\n", p_->size());
+  f32 = fp32;
+  const size_t bytes = es() * (size_t)p_->size();
+  if (hipMalloc(&ptr, bytes) != hipSuccess) {
+    fprintf(stderr, "fr_amd: tensor0 allocation of %d values failed\n", p_->size());
     ptr = nullptr;
     return;
   }
-  hipMemsetAsync(ptr, 0, sizeof(double) * (size_t)p_->size(), st_);
+  hipMemsetAsync(ptr, 0, bytes, st_);
 }
 
 tensor0::~tensor0() {
   if (ptr) hipFree(ptr);
 }
 
-void tensor0::Update_Ghosts() { fr_t0_update_ghosts(&p_->g, ptr, 0, st_); }
-void tensor0::Update_Ghosts_CUM() { fr_t0_update_ghosts(&p_->g, ptr, 1, st_); }
+// The FP32 storage of the precision form and the FP64 storage share one code
+// path through these helpers; reductions accumulate in double either way.
+#define FR_T0_DISPATCH(expr_f, expr_d) \
+  do {                                 \
+    if (f32) {                         \
+      expr_f;                          \
+    } else {                           \
+      expr_d;                          \
+    }                                  \
+  } while (0)
+
+void tensor0::Update_Ghosts() {
+  const Dev d = p_->dev();
+  FR_T0_DISPATCH(ghost_fill(d, p<float>(), st_), ghost_fill(d, p<double>(), st_));
+}
+void tensor0::Update_Ghosts_CUM() {
+  const Dev d = p_->dev();
+  FR_T0_DISPATCH(cum_fold(d, p<float>(), st_), cum_fold(d, p<double>(), st_));
+}
 void tensor0::assign(double a) {
-  fr_t0_elementwise(&p_->g, 5, ptr, a, nullptr, 0, nullptr, 0, nullptr, st_);
+  const long n = p_->size();
+  FR_T0_DISPATCH(elementwise<float>(n, 5, p<float>(), a, nullptr, 0, nullptr, 0, nullptr, st_),
+                 elementwise<double>(n, 5, p<double>(), a, nullptr, 0, nullptr, 0, nullptr, st_));
 }
 void tensor0::add(double a) {
-  fr_t0_elementwise(&p_->g, 6, ptr, a, nullptr, 0, nullptr, 0, nullptr, st_);
+  const long n = p_->size();
+  FR_T0_DISPATCH(elementwise<float>(n, 6, p<float>(), a, nullptr, 0, nullptr, 0, nullptr, st_),
+                 elementwise<double>(n, 6, p<double>(), a, nullptr, 0, nullptr, 0, nullptr, st_));
+}
+// Operands of one operation share one storage type; a mixed call would read
+// past the end of the narrower buffer, so it is refused.
+static bool same_type(bool a, bool b, const char* what) {
+  if (a == b) return true;
+  fprintf(stderr, "fr_amd: %s on fields of different precision refused\n", what);
+  return false;
 }
 void tensor0::copy(const tensor0& a) {
-  hipMemcpyAsync(ptr, a.ptr, sizeof(double) * (size_t)p_->size(), hipMemcpyDeviceToDevice, st_);
+  if (!same_type(f32, a.f32, "copy")) return;
+  hipMemcpyAsync(ptr, a.ptr, es() * (size_t)p_->size(), hipMemcpyDeviceToDevice, st_);
 }
 void tensor0::Equal_Divide(double a, const tensor0& b) {
-  fr_t0_elementwise(&p_->g, 3, ptr, a, b.ptr, 0, nullptr, 0, nullptr, st_);
+  const long n = p_->size();
+  if (!same_type(f32, b.f32, "Equal_Divide")) return;
+  FR_T0_DISPATCH(
+      elementwise<float>(n, 3, p<float>(), a, b.p<float>(), 0, nullptr, 0, nullptr, st_),
+      elementwise<double>(n, 3, p<double>(), a, b.p<double>(), 0, nullptr, 0, nullptr, st_));
 }
 void tensor0::Equal_Divide(const tensor0& a, const tensor0& b) {
-  fr_t0_elementwise(&p_->g, 4, ptr, 0, a.ptr, 0, b.ptr, 0, nullptr, st_);
+  const long n = p_->size();
+  if (!same_type(f32, a.f32, "Equal_Divide") || !same_type(f32, b.f32, "Equal_Divide")) return;
+  FR_T0_DISPATCH(
+      elementwise<float>(n, 4, p<float>(), 0, a.p<float>(), 0, b.p<float>(), 0, nullptr, st_),
+      elementwise<double>(n, 4, p<double>(), 0, a.p<double>(), 0, b.p<double>(), 0, nullptr,
+                          st_));
 }
-double tensor0::sum() {
+double tensor0::reduce(int op) {
   double r = 0;
-  fr_t0_reduce(&p_->g, ptr, 0, &r, st_);
+  const Dev d = p_->dev();
+  FR_T0_DISPATCH(reduce_sync(d, (const float*)ptr, op, &r, st_),
+                 reduce_sync(d, (const double*)ptr, op, &r, st_));
   return r;
 }
-double tensor0::mean() {
-  double r = 0;
-  fr_t0_reduce(&p_->g, ptr, 1, &r, st_);
-  return r;
-}
+double tensor0::sum() { return reduce(0); }
+double tensor0::mean() { return reduce(1); }
 void tensor0::make_mean_zero() {  // tensor0.cpp:437-441: (*this) -= mean
   double m = mean();
   add(-m);
 }
-double tensor0::max() {
-  double r = 0;
-  fr_t0_reduce(&p_->g, ptr, 2, &r, st_);
-  return r;
-}
-double tensor0::min() {
-  double r = 0;
-  fr_t0_reduce(&p_->g, ptr, 3, &r, st_);
-  return r;
-}
-double tensor0::max_abs() {
-  double r = 0;
-  fr_t0_reduce(&p_->g, ptr, 4, &r, st_);
-  return r;
-}
+double tensor0::max() { return reduce(2); }
+double tensor0::min() { return reduce(3); }
+double tensor0::max_abs() { return reduce(4); }
 void tensor0::download(double* host) const {
-  hipMemcpyAsync(host, ptr, sizeof(double) * (size_t)p_->size(), hipMemcpyDeviceToHost, st_);
+  const size_t n = (size_t)p_->size();
+  if (f32) {
+    std::vector<float> h(n);
+    hipMemcpyAsync(h.data(), ptr, sizeof(float) * n, hipMemcpyDeviceToHost, st_);
+    hipStreamSynchronize(st_);
+    for (size_t i = 0; i < n; i++) host[i] = (double)h[i];
+    return;
+  }
+  hipMemcpyAsync(host, ptr, sizeof(double) * n, hipMemcpyDeviceToHost, st_);
   hipStreamSynchronize(st_);
 }
 void tensor0::upload(const double* host) {
-  hipMemcpyAsync(ptr, host, sizeof(double) * (size_t)p_->size(), hipMemcpyHostToDevice, st_);
+  const size_t n = (size_t)p_->size();
+  if (f32) {
+    std::vector<float> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = (float)host[i];
+    hipMemcpyAsync(ptr, h.data(), sizeof(float) * n, hipMemcpyHostToDevice, st_);
+    hipStreamSynchronize(st_);
+    return;
+  }
+  hipMemcpyAsync(ptr, host, sizeof(double) * n, hipMemcpyHostToDevice, st_);
   hipStreamSynchronize(st_);
 }
 
@@ -263,6 +307,10 @@ int particle::advance(int k, double pre, double post, tensor1& U) {
   c.dtpre_iS = dt * pre * iS;
   c.alpha = 0.25 * p.epsilonp() * PI * p.Dp() * p.Dp() * p.I0();
   c.last = (k == 3);
+  if (U.x.is_f32()) {
+    const float* u[3] = {U.x.p<float>(), U.y.p<float>(), U.z.p<float>()};
+    return dev.advance(size_->dev(), c, u, st_);
+  }
   const double* u[3] = {U.x.pointer(), U.y.pointer(), U.z.pointer()};
   return dev.advance(size_->dev(), c, u, st_);
 }
@@ -271,12 +319,17 @@ int particle::advance(int k, double pre, double post, tensor1& U) {
 // the previous substep (or at initialisation), which are this substep's _int
 // positions: particle order and positions are unchanged since.
 int particle::part2gas_concentration(tensor0& A) {
+  if (A.is_f32()) return dev.deposit_cic(size_->dev(), false, A.p<float>(), false, st_);
   return dev.deposit_cic(size_->dev(), false, A.pointer(), false, st_);
 }
 
 int particle::part2gas_velocity(tensor1& A) {
   int rc = dev.build(size_->dev(), 1, 1, st_);
   if (rc) return rc;
+  if (A.x.is_f32()) {
+    float* o[3] = {A.x.p<float>(), A.y.p<float>(), A.z.p<float>()};
+    return dev.deposit_vel(size_->dev(), o, st_);
+  }
   double* o[3] = {A.x.pointer(), A.y.pointer(), A.z.pointer()};
   return dev.deposit_vel(size_->dev(), o, st_);
 }
@@ -299,12 +352,14 @@ int particle::Send_Recv_duct(const double* inlet, int n) {
 
 int particle::temp(int stage, double c0, const tensor0& Rho) {
   const double iTth = param_->Nu() * PI * param_->Dp() * param_->k();
+  if (Rho.is_f32()) return dev.temp(size_->dev(), stage, c0, Rho.p<float>(), iTth, st_);
   return dev.temp(size_->dev(), stage, c0, Rho.pointer(), iTth, st_);
 }
 
 // Requires rebuild_cells(stage) after the RHS_T of that stage was formed.
 int particle::part2gas_Temp(int stage, tensor0& A) {
   (void)stage;
+  if (A.is_f32()) return dev.deposit_cic(size_->dev(), true, A.p<float>(), true, st_);
   return dev.deposit_cic(size_->dev(), true, A.pointer(), true, st_);
 }
 
@@ -337,13 +392,17 @@ int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS) {
     return pcg_solve(sp, Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer(),
                      P.pointer(), RHS.pointer(), par->Iteration(), par->epsilon(),
                      &num_iteration_, &relres_, st_);
+  if (s && Rho_face.x.is_f32())
+    return box_poisson_solve<float>(s, Rho_face.x.p<float>(), Rho_face.y.p<float>(),
+                                    Rho_face.z.p<float>(), true, P.pointer(), RHS.pointer(),
+                                    par->Iteration(), par->epsilon(), &num_iteration_, st_);
   if (sd)
     return duct_poisson_solve(sd, Rho_face.x.pointer(), Rho_face.y.pointer(),
                               Rho_face.z.pointer(), true, P.pointer(), RHS.pointer(),
                               par->Iteration(), par->epsilon(), &num_iteration_, st_);
-  return box_poisson_solve(s, Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer(),
-                           true, P.pointer(), RHS.pointer(), par->Iteration(), par->epsilon(),
-                           &num_iteration_, st_);
+  return box_poisson_solve<double>(s, Rho_face.x.pointer(), Rho_face.y.pointer(),
+                                   Rho_face.z.pointer(), true, P.pointer(), RHS.pointer(),
+                                   par->Iteration(), par->epsilon(), &num_iteration_, st_);
 }
 
 // ---------------------------------------------------------------- grid
@@ -359,8 +418,12 @@ int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS) {
 grid::~grid() {
   if (dred) hipFree(dred);
   if (inflow) hipFree(inflow);
-  for (auto& e : ev) hipEventDestroy(e);
-  for (auto& e : evk) hipEventDestroy(e);
+  // a grid whose setup stopped early holds null handles; destroying those
+  // would leave an error behind for the next launch check
+  for (auto& e : ev)
+    if (e) hipEventDestroy(e);
+  for (auto& e : evk)
+    if (e) hipEventDestroy(e);
   if (st) hipStreamDestroy(st);
 }
 
@@ -368,6 +431,7 @@ int grid::setup(const char* params_path, const char* ic) {
   param = params(params_path);
   if (!param.ok) return FR_EINVAL;
   if (param.bs() != 1) return FR_EINVAL;
+  if (f32 && (!param.IsBox() || param.Solver() != 0)) return FR_EINVAL;
   size = gridsize(param);
   ic_dir = ic ? ic : ".";
   FR_CHECK(hipStreamCreate(&st));
@@ -382,11 +446,17 @@ int grid::setup(const char* params_path, const char* ic) {
   RK4_postCoeff[1] = 1. / 3.;
   RK4_postCoeff[2] = 1. / 3.;
   RK4_postCoeff[3] = 1. / 6.;
-  for (tensor1* f : {&RU, &RU_int, &RU_new, &RU_np1, &RHS_RU, &U, &Rho_face, &dummy})
-    f->init(&size, st);
-  for (tensor0* f : {&P, &RHS_Pois, &C, &Rho, &Rho_int, &Rho_new, &Rho_np1, &RHS_Part_Temp,
-                     &scratch, &divergence})
-    f->init(&size, st);
+  for (tensor1* f : {&RU, &RU_int, &RU_new, &RU_np1, &RHS_RU, &U, &Rho_face})
+    f->init(&size, st, f32);
+  dummy.init(&size, st);
+  if (f32) {
+    d64a.init(&size, st);
+    d64b.init(&size, st);
+    d64r.init(&size, st);
+  }
+  for (tensor0* f : {&C, &Rho, &Rho_int, &Rho_new, &Rho_np1, &RHS_Part_Temp})
+    f->init(&size, st, f32);
+  for (tensor0* f : {&P, &RHS_Pois, &scratch, &divergence}) f->init(&size, st);
   if (!divergence.pointer()) return FR_ENOMEM;
   if (!param.IsBox()) {
     const size_t m = (size_t)size.Ny() * size.Nz();
@@ -462,8 +532,12 @@ int grid::Finish_Initialize() {
   Rho_np1.copy(Rho);
   P0_int = P0;
   P0_np1 = P0;
-  for (int m = 0; m < 3; m++)
-    FR_TRY(fr_t0_interp(&size.g, Rho_face.c(m).pointer(), Rho.pointer(), m, 0, st));
+  for (int m = 0; m < 3; m++) {
+    if (f32)
+      FR_TRY(interp_c2f(&size.g, Rho_face.c(m).p<float>(), Rho.p<float>(), m, st));
+    else
+      FR_TRY(interp_c2f(&size.g, Rho_face.c(m).pointer(), Rho.pointer(), m, st));
+  }
   FR_TRY(part.dev.copy_stage(1, 0, st));
   FR_TRY(part.dev.copy_stage(3, 0, st));
   // T = (P0_int/R)/Rho_int; gas2part_Temp_int; Compute_RHS_Temp_int;
@@ -514,8 +588,16 @@ int grid::Update_Rho() {
                            inflow, inflow + m, st));
     return FR_OK;
   }
-  FR_TRY(box_update_rho(size.dev(), box_coef(*this), ri, Rho.pointer(), Rho_np1.pointer(),
-                        Rho_new.pointer(), rf, u, st));
+  if (f32) {
+    const float* ri_[3] = {RU_int.x.p<float>(), RU_int.y.p<float>(), RU_int.z.p<float>()};
+    const float* rf_[3] = {Rho_face.x.p<float>(), Rho_face.y.p<float>(), Rho_face.z.p<float>()};
+    float* u_[3] = {U.x.p<float>(), U.y.p<float>(), U.z.p<float>()};
+    FR_TRY(box_update_rho<float>(size.dev(), box_coef(*this), ri_, Rho.p<float>(),
+                                 Rho_np1.p<float>(), Rho_new.p<float>(), rf_, u_, st));
+    return FR_OK;
+  }
+  FR_TRY(box_update_rho<double>(size.dev(), box_coef(*this), ri, Rho.pointer(),
+                                Rho_np1.pointer(), Rho_new.pointer(), rf, u, st));
   return FR_OK;
 }
 
@@ -591,8 +673,22 @@ int grid::Update_RU_WOP() {  // grid.cpp:387-425
   double* nw[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
   const double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
   hipEventRecord(evk[2 * (RK4_count & 3)], st);
-  if (param.IsBox()) {
-    FR_TRY(box_update_ru_wop(size.dev(), box_coef(*this), w, u, ri, S, r0, n1, nw, rf, st));
+  if (param.IsBox() && f32) {
+    auto P3 = [](tensor1& t) { return std::array<float*, 3>{t.x.p<float>(), t.y.p<float>(), t.z.p<float>()}; };
+    auto a = P3(U), b = P3(RU_int), c = P3(RHS_RU), e = P3(RU), f = P3(RU_np1), g = P3(RU_new),
+         h = P3(Rho_face);
+    const float* u_[3] = {a[0], a[1], a[2]};
+    const float* ri_[3] = {b[0], b[1], b[2]};
+    const float* S_[3] = {c[0], c[1], c[2]};
+    const float* r0_[3] = {e[0], e[1], e[2]};
+    float* n1_[3] = {f[0], f[1], f[2]};
+    float* nw_[3] = {g[0], g[1], g[2]};
+    const float* rf_[3] = {h[0], h[1], h[2]};
+    FR_TRY(box_update_ru_wop<float>(size.dev(), box_coef(*this), w, u_, ri_, S_, r0_, n1_, nw_,
+                                    rf_, st));
+  } else if (param.IsBox()) {
+    FR_TRY(box_update_ru_wop<double>(size.dev(), box_coef(*this), w, u, ri, S, r0, n1, nw, rf,
+                                     st));
   } else {  // grid.cpp:427-537
     const size_t m = (size_t)size.Ny() * size.Nz();
     FR_TRY(duct_update_ru_wop(size.dev(), box_coef(*this), w, C_BC, u, ri, S, r0, n1, nw,
@@ -655,9 +751,14 @@ int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
   const DivCoef q = div_coef(*this);
   const double* rn[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
   double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
-  if (param.IsBox()) {
-    FR_TRY(box_div_rhs(size.dev(), q, Rho_new.pointer(), RHS_Part_Temp.pointer(), rn, rf,
-                       RHS_Pois.pointer(), st));
+  if (param.IsBox() && f32) {
+    const float* rn_[3] = {RU_new.x.p<float>(), RU_new.y.p<float>(), RU_new.z.p<float>()};
+    float* rf_[3] = {Rho_face.x.p<float>(), Rho_face.y.p<float>(), Rho_face.z.p<float>()};
+    FR_TRY(box_div_rhs<float>(size.dev(), q, Rho_new.p<float>(), RHS_Part_Temp.p<float>(), rn_,
+                              rf_, RHS_Pois.pointer(), st));
+  } else if (param.IsBox()) {
+    FR_TRY(box_div_rhs<double>(size.dev(), q, Rho_new.pointer(), RHS_Part_Temp.pointer(), rn,
+                               rf, RHS_Pois.pointer(), st));
   } else {
     const double* rfc[3] = {rf[0], rf[1], rf[2]};
     FR_TRY(duct_rhs(size.dev(), q.s, rn, rfc, divergence.pointer(), RHS_Pois.pointer(), st));
@@ -682,8 +783,12 @@ int grid::Solve_Poisson() {  // grid.cpp:639-643
 int grid::Update_RU_WP() {  // grid.cpp:651-657
   double* n1[3] = {RU_np1.x.pointer(), RU_np1.y.pointer(), RU_np1.z.pointer()};
   double* nw[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
-  if (param.IsBox())
-    FR_TRY(box_update_ru_wp(size.dev(), box_coef(*this), P.pointer(), n1, nw, st));
+  if (param.IsBox() && f32) {
+    float* n1_[3] = {RU_np1.x.p<float>(), RU_np1.y.p<float>(), RU_np1.z.p<float>()};
+    float* nw_[3] = {RU_new.x.p<float>(), RU_new.y.p<float>(), RU_new.z.p<float>()};
+    FR_TRY(box_update_ru_wp<float>(size.dev(), box_coef(*this), P.pointer(), n1_, nw_, st));
+  } else if (param.IsBox())
+    FR_TRY(box_update_ru_wp<double>(size.dev(), box_coef(*this), P.pointer(), n1, nw, st));
   else
     FR_TRY(duct_update_ru_wp(size.dev(), box_coef(*this), P.pointer(), n1, nw, st));
   return FR_OK;
@@ -764,19 +869,35 @@ int grid::diagnostics(double* out) {
   double* ured = dred + size.Nx();
   double* prof = dred + 2 * size.Nx();
   double* psum = dred + 4 * size.Nx();
-  FR_TRY(reducex(d, Rho.pointer(), rred, st));
+  // The FP32 form widens its fields into FP64 copies; the sums are the same.
+  const long n = size.size();
+  auto wide0 = [&](tensor0& f, tensor0& tmp) -> tensor0* {
+    if (!f32) return &f;
+    to_double(f.p<float>(), tmp.pointer(), n, st);
+    return &tmp;
+  };
+  auto wide1 = [&](tensor1& f, tensor1& tmp) -> tensor1* {
+    if (!f32) return &f;
+    for (int m = 0; m < 3; m++) to_double(f.c(m).p<float>(), tmp.c(m).pointer(), n, st);
+    return &tmp;
+  };
+  tensor0* rho = wide0(Rho, d64r);
+  tensor1* ru = wide1(RU, d64a);
+  FR_TRY(reducex(d, rho->pointer(), rred, st));
   double tke = 0;
   for (int m = 0; m < 3; m++) {
-    scratch.Equal_Divide(RU.c(m), Rho);  // dummy2 = RU/Rho (grid.cpp:738)
+    scratch.Equal_Divide(ru->c(m), *rho);  // dummy2 = RU/Rho (grid.cpp:738)
     FR_TRY(reducex(d, scratch.pointer(), ured, st));
-    FR_TRY(diag_tke(d, scratch.pointer(), Rho.pointer(), ured, scratch.pointer(), st));
+    FR_TRY(diag_tke(d, scratch.pointer(), rho->pointer(), ured, scratch.pointer(), st));
     tke = tke + scratch.mean();
   }
   out[0] = tke;
   // U = RU_np1/Rho_face (grid.cpp:723); dissipation profile (794-811)
-  for (int m = 0; m < 3; m++) dummy.c(m).Equal_Divide(RU_np1.c(m), Rho_face.c(m));
+  tensor1* rn = wide1(RU_np1, d64a);
+  tensor1* rf = wide1(Rho_face, d64b);
+  for (int m = 0; m < 3; m++) dummy.c(m).Equal_Divide(rn->c(m), rf->c(m));
   const double* uf[3] = {dummy.x.pointer(), dummy.y.pointer(), dummy.z.pointer()};
-  FR_TRY(diag_diss(d, uf, Rho.pointer(), param.Mu0(), scratch.pointer(), st));
+  FR_TRY(diag_diss(d, uf, rho->pointer(), param.Mu0(), scratch.pointer(), st));
   FR_TRY(reducex(d, scratch.pointer(), prof, st));
   std::vector<double> h(size.Nx());
   FR_CHECK(hipMemcpyAsync(h.data(), prof, sizeof(double) * size.Nx(), hipMemcpyDeviceToHost, st));
@@ -823,8 +944,13 @@ struct fr_sim {
 extern "C" {
 
 int fr_sim_create(const char* params_path, const char* ic_dir, fr_sim** out) {
-  if (!params_path || !out) return FR_EINVAL;
+  return fr_sim_create_ex(params_path, ic_dir, 0, out);
+}
+
+int fr_sim_create_ex(const char* params_path, const char* ic_dir, int flags, fr_sim** out) {
+  if (!params_path || !out || (flags & ~FR_SIM_FP32)) return FR_EINVAL;
   fr_sim* s = new fr_sim();
+  s->g.f32 = (flags & FR_SIM_FP32) != 0;
   int rc = s->g.setup(params_path, ic_dir);
   if (rc) {
     delete s;

@@ -160,6 +160,14 @@ int fr_pcg_solve(fr_pcg* s, const double* rfx, const double* rfy,
  * the ghosted layout above; particle arrays are Np doubles (int32 for flags). */
 typedef struct fr_sim fr_sim;
 int fr_sim_create(const char* params_path, const char* ic_dir, fr_sim** out);
+/* fr_sim_create with options.  FR_SIM_FP32 selects the precision form: every
+ * field but P held and advanced in FP32 (stencils, Rho_face, the Poisson
+ * right-hand side, the particle deposits), while P, the Poisson residual and
+ * every sum stay FP64; particles stay FP64.  Box geometry with the FFT
+ * solver (Solver = 0) only, else FR_EINVAL.  No counterpart in the
+ * reference, which runs FP64 throughout (grid.h:16-115). */
+#define FR_SIM_FP32 1
+int fr_sim_create_ex(const char* params_path, const char* ic_dir, int flags, fr_sim** out);
 int fr_sim_destroy(fr_sim* s);
 /* grid::Initialize (grid.cpp:110-231) */
 int fr_sim_initialize(fr_sim* s);
