@@ -58,6 +58,9 @@ def main():
                     help="fp32: the precision form (FP32 fields and stencils; FP64 pressure, "
                          "Poisson residual, sums and particles); its error against the FP64 "
                          "path over the same steps is measured after the timed region")
+    ap.add_argument("--no-error", action="store_true",
+                    help="with --precision fp32: skip the FP64 run that measures the error "
+                         "(it runs after the timed region)")
     ap.add_argument("--workload", default="box256", choices=["box256", "box512", "duct512"],
                     help="box256: the headline 256^3 box; duct512: the 512x128x128 "
                          "inflow/outflow duct (density ratio 3, 2e4 inlet particles/step)")
@@ -153,7 +156,7 @@ def main():
                      "avg_launch_ms": round(per_launch_ms, 4)},
     }
     sim.close()
-    if fp32:
+    if fp32 and not args.no_error:
         # error of the precision form against the FP64 path after the same steps
         print("bench: FP64 reference run for the error of the precision form", file=sys.stderr,
               flush=True)
