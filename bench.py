@@ -24,6 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_update_ru_wop launch from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# (each scaled by a plain copy of known size: factors 2.0 and 1.0), measured on
+# the named workloads; profiles/hbm_traffic_box256_fp64_box512_fp32.md
+RU_WOP_TRAFFIC = {(256, "fp64"): 5.020e9, (512, "fp32"): 16.696e9}
 METRIC = ("Mcell\u00b7steps/s full RK4 step w/ particles; "
           "Poisson iters+ms to 1e-10 vs plain")
 
@@ -40,6 +44,26 @@ def cpu_baseline(params, d, threads):
     og.step()
     wall = time.perf_counter() - t
     return wall
+
+
+def calibrate_copy():
+    """A plain device-to-device copy of 256 MiB, three times, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters to bytes (run it under rocprofv3 --pmc)."""
+    import ctypes as C
+    import first_repo_amd as fr
+    fr.lib()  # loads the HIP runtime the library links
+    h = C.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    n = 1 << 28
+    a, b = C.c_void_p(), C.c_void_p()
+    assert h.hipMalloc(C.byref(a), n) == 0 and h.hipMalloc(C.byref(b), n) == 0
+    assert h.hipMemset(a, 1, n) == 0
+    for _ in range(3):
+        assert h.hipMemcpy(b, a, n, 3) == 0  # device to device
+    assert h.hipDeviceSynchronize() == 0
+    print(f"bench: copied {n} bytes 3 times", flush=True)
 
 
 def main():
@@ -64,7 +88,11 @@ def main():
     ap.add_argument("--workload", default="box256", choices=["box256", "box512", "duct512"],
                     help="box256: the headline 256^3 box; duct512: the 512x128x128 "
                          "inflow/outflow duct (density ratio 3, 2e4 inlet particles/step)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)
@@ -151,7 +179,9 @@ def main():
                    "mean_particle_T": float(diag[8])},
         "roofline": {"kernel": "k_update_ru_wop", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": RU_WOP_TRAFFIC.get((args.n, args.precision))
+                     if args.particles == {256: 4_000_000, 512: 16_000_000}.get(args.n) else None,
                      "alg_bytes_per_launch": alg_bytes,
                      "avg_launch_ms": round(per_launch_ms, 4)},
     }
