@@ -71,6 +71,27 @@ def lib():
             "fr_pcg_create": ([G, C.POINTER(vp)], i),
             "fr_pcg_destroy": ([vp], i),
             "fr_pcg_solve": ([vp, vp, vp, vp, vp, vp, i, d, C.POINTER(i), C.POINTER(d), vp], i),
+            "fr_unet_create": ([i, C.POINTER(vp)], i),
+            "fr_unet_destroy": ([vp], i),
+            "fr_unet_num_params": ([vp, C.POINTER(C.c_longlong)], i),
+            "fr_unet_get_params": ([vp, vp], i),
+            "fr_unet_set_params": ([vp, vp], i),
+            "fr_unet_get_grads": ([vp, vp], i),
+            "fr_unet_zero_grads": ([vp, vp], i),
+            "fr_unet_forward": ([vp, i, i, i, vp, vp, vp], i),
+            "fr_unet_backward": ([vp, i, i, i, vp, vp], i),
+            "fr_unet_adam_step": ([vp, C.c_float, vp], i),
+            "fr_unet_data_create": ([i, i, i, i, C.POINTER(vp)], i),
+            "fr_unet_data_destroy": ([vp], i),
+            "fr_unet_data_count": ([vp, C.POINTER(i)], i),
+            "fr_unet_train": ([vp, vp, i, i, i, i, i, C.c_float, C.c_uint, C.POINTER(d),
+                               C.POINTER(d), vp], i),
+            "fr_pcg_set_learned": ([vp, vp, i], i),
+            "fr_pcg_set_collect": ([vp, vp], i),
+            "fr_pcg_learned_stats": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(d)], i),
+            "fr_sim_learned_stats": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(d)], i),
+            "fr_sim_set_learned": ([vp, vp, i], i),
+            "fr_sim_set_collect": ([vp, vp], i),
             "fr_sim_create": ([C.c_char_p, C.c_char_p, C.POINTER(vp)], i),
             "fr_sim_create_ex": ([C.c_char_p, C.c_char_p, i, C.POINTER(vp)], i),
             "fr_sim_destroy": ([vp], i),

@@ -12,7 +12,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FR_OFFLOAD_ARCH", "gfx950")
 
 KERNELS = ["fields.hip", "box.hip", "duct.hip", "fft_poisson.hip", "duct_poisson.hip",
-           "mg_pcg.hip", "particles.hip", "diag.hip"]
+           "mg_pcg.hip", "unet.hip", "particles.hip", "diag.hip"]
 HOST = ["host.cpp"]
 
 # -ffp-contract=off: the FP64 path reproduces the reference's unfused

@@ -27,6 +27,13 @@
     }                                                                      \
   } while (0)
 
+// Propagates a nonzero status.
+#define FR_RET(expr)       \
+  do {                     \
+    int rc_ = (expr);      \
+    if (rc_) return rc_;   \
+  } while (0)
+
 namespace fr {
 
 // Grid constants as the reference caches them (tensor0.cpp:18-26):

@@ -196,4 +196,17 @@ class PartDev {
   int set_flags(const int* host, hipStream_t st);
 };
 
+// ---- learned preconditioner (unet.hip) --------------------------------------------
+// Network output y (interior order, FP32) for the correction equation
+// S dP = sign * r, and the scale that turns y into dP.
+int unet_infer(fr_unet* u, const Dev& d, const double* r, double sign, double rrms,
+               const double* rfx, const double* rfy, const double* rfz, const double* Pprev,
+               const float** y, double* scale, hipStream_t st);
+int unet_data_begin(fr_unet_data* D, const Dev& d, const double* r, double sign, double rrms,
+                    const double* rfx, const double* rfy, const double* rfz, const double* Pprev,
+                    double* scale, hipStream_t st);
+bool unet_data_room(const fr_unet_data* D);
+int unet_data_end(fr_unet_data* D, const Dev& d, const double* Pstart, const double* Pfinal,
+                  double scale, hipStream_t st);
+
 }  // namespace fr

@@ -14,6 +14,16 @@
 //
 // Levels hold interior cells only (no ghosts): periodic neighbours by index
 // wrap; on the duct the x end faces carry a zero coefficient.
+//
+// Learned forms (unet.hip; no reference counterpart): mode 1 moves the
+// initial iterate along the network's pressure increment d by the step
+// (r, d) / (d, S d) that minimises the S-norm error along d (so the error
+// never grows); mode 2 also preconditions with z = a N(r) + V(r - a S N(r)),
+// a the same optimal step (network, then one V-cycle on what it leaves),
+// inside flexible PCG (Polak-Ribiere beta), and falls back to the plain
+// V-cycle for the rest of the solve when an iteration reduces the residual
+// norm by less than 3x.
+// With a pair store attached, every solve records (input, increment) pairs.
 #include <cmath>
 #include <vector>
 
@@ -343,6 +353,29 @@ __global__ void k_pcg_todouble(Fine F, const float* __restrict__ e, const double
   z[at(F.d, i, j, k)] = (double)e[t] - m;
 }
 
+// z (+)= scale * (y - mean), y in interior order, mean = *msum / n
+__global__ void k_pcg_netz(Fine F, const float* __restrict__ y, const double* __restrict__ msum,
+                           double scale, int add, double* __restrict__ z) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.n) return;
+  int i, j, k;
+  tijk(t, F.nx, F.ny, i, j, k);
+  const long c = at(F.d, i, j, k);
+  const double v = scale * ((double)y[t] - *msum / (double)F.n);
+  z[c] = add ? z[c] + v : v;
+}
+
+// op 0: a = b - s a; op 1: a = a + s b (interior)
+__global__ void k_pcg_vec(Fine F, int op, double s, double* __restrict__ a,
+                          const double* __restrict__ b) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.n) return;
+  int i, j, k;
+  tijk(t, F.nx, F.ny, i, j, k);
+  const long c = at(F.d, i, j, k);
+  a[c] = op == 0 ? b[c] - s * a[c] : a[c] + s * b[c];
+}
+
 // op 0: p = z; op 1: x += alpha p, r -= alpha q; op 2: p = z + beta p
 __global__ void k_pcg_axpy(Fine F, int op, double s, double* __restrict__ x,
                            double* __restrict__ r, double* __restrict__ p,
@@ -385,6 +418,13 @@ struct fr_pcg {
   double* hres = nullptr;   // pinned host copy
   int pre = 2, post = 2, coarse_sweeps = 20;
   int l_small = -1;  // first level handled by the one-workgroup kernel
+  // learned forms
+  fr_unet* net = nullptr;
+  int net_mode = 0;
+  fr_unet_data* collect = nullptr;
+  double *p0 = nullptr, *ta = nullptr, *tb = nullptr, *rold = nullptr;  // ghosted
+  int fallbacks = 0, guesses_kept = 0;
+  double guess_gain = 0.0;  // sum over kept guesses of log10(rr before / rr after)
 };
 
 static int mg_launch_n(long n) { return (int)((n + 255) / 256); }
@@ -464,6 +504,50 @@ static int precond(fr_pcg* s, hipStream_t st) {
 
 namespace fr {
 
+static int ensure_buf(fr_pcg* s, double** p) {
+  if (*p) return FR_OK;
+  const size_t gs = (size_t)s->g.Nx * s->g.Ny * s->g.Nz;
+  FR_CHECK(hipMalloc(p, sizeof(double) * gs));
+  FR_CHECK(hipMemset(*p, 0, sizeof(double) * gs));
+  return FR_OK;
+}
+
+// Optimal step along d with q = S d: (r, d) / (d, q), 0 if not a descent.
+static int opt_step(fr_pcg* s, const double* d, const double* q, double* a, hipStream_t st) {
+  double rd, dq;
+  dsum(s, 0, s->r, d, nullptr, 0, st);
+  FR_RET(read_scalar(s, 0, &rd, st));
+  dsum(s, 0, d, q, nullptr, 0, st);
+  FR_RET(read_scalar(s, 0, &dq, st));
+  *a = (dq > 0 && rd > 0) ? rd / dq : 0.0;
+  return FR_OK;
+}
+
+// z = a N(r) + V(r - a S N(r)), N(r) = scale * (net(-r) - mean).
+static int precond_net(fr_pcg* s, const double* rfx, const double* rfy, const double* rfz,
+                       double rrms, hipStream_t st) {
+  const Fine& F = s->F;
+  const unsigned nb = mg_launch_n(F.n);
+  const float* y;
+  double sc, a;
+  FR_RET(unet_infer(s->net, F.d, s->r, -1.0, rrms, rfx, rfy, rfz, nullptr, &y, &sc, st));
+  dsum(s, 1, nullptr, nullptr, y, 1, st);
+  hipLaunchKernelGGL(k_pcg_netz, dim3(nb), dim3(256), 0, st, F, y, (const double*)(s->dres + 1),
+                     sc, 0, s->tb);
+  FR_RET(fill_fine_ghosts(s, s->tb, st));
+  hipLaunchKernelGGL(k_pcg_apply, dim3(nb), dim3(256), 0, st, F, 1, (const double*)s->tb,
+                     (const double*)nullptr, rfx, rfy, rfz, s->ta);
+  FR_RET(opt_step(s, s->tb, s->ta, &a, st));
+  hipLaunchKernelGGL(k_pcg_vec, dim3(nb), dim3(256), 0, st, F, 0, a, s->ta, (const double*)s->r);
+  std::swap(s->r, s->ta);
+  const int rc = precond(s, st);
+  std::swap(s->r, s->ta);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pcg_vec, dim3(nb), dim3(256), 0, st, F, 1, a, s->z, (const double*)s->tb);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
 int pcg_solve(fr_pcg* s, const double* rfx, const double* rfy, const double* rfz, double* P,
               const double* RHS, int maxit, double tol, int* iters, double* relres,
               hipStream_t st) {
@@ -486,8 +570,51 @@ int pcg_solve(fr_pcg* s, const double* rfx, const double* rfy, const double* rfz
   dsum(s, 0, s->r, s->r, nullptr, 0, st);
   if ((rc = read_scalar(s, 0, &rr, st))) return rc;
   int it = 0;
+  const size_t gbytes = sizeof(double) * (size_t)s->g.Nx * s->g.Ny * s->g.Nz;
+  const bool todo = std::sqrt(rr) > tol * bn;
+  // pair collection: the input at entry, the increment at exit
+  double cscale = 0;
+  const bool collecting = todo && s->collect && unet_data_room(s->collect);
+  if (collecting) {
+    if ((rc = ensure_buf(s, &s->p0))) return rc;
+    FR_CHECK(hipMemcpyAsync(s->p0, P, gbytes, hipMemcpyDeviceToDevice, st));
+    if ((rc = unet_data_begin(s->collect, F.d, s->r, -1.0, std::sqrt(rr / F.n), rfx, rfy, rfz, P,
+                              &cscale, st)))
+      return rc;
+  }
+  bool flex = false;
+  if (todo && s->net && s->net_mode >= 1) {
+    // learned initial iterate: P += a d along the network's increment d
+    const float* y;
+    double sc, a;
+    if ((rc = unet_infer(s->net, F.d, s->r, -1.0, std::sqrt(rr / F.n), rfx, rfy, rfz, P, &y, &sc,
+                         st)))
+      return rc;
+    dsum(s, 1, nullptr, nullptr, y, 1, st);
+    hipLaunchKernelGGL(k_pcg_netz, dim3(nb), dim3(256), 0, st, F, y, (const double*)(s->dres + 1),
+                       sc, 0, s->p);
+    if ((rc = fill_fine_ghosts(s, s->p, st))) return rc;
+    hipLaunchKernelGGL(k_pcg_apply, dim3(nb), dim3(256), 0, st, F, 1, (const double*)s->p, RHS,
+                       rfx, rfy, rfz, s->z);
+    if ((rc = opt_step(s, s->p, s->z, &a, st))) return rc;
+    if (a > 0) {
+      hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(256), 0, st, F, 1, a, P, s->r, s->p,
+                         (const double*)s->z);
+      const double rr0 = rr;
+      dsum(s, 0, s->r, s->r, nullptr, 0, st);
+      if ((rc = read_scalar(s, 0, &rr, st))) return rc;
+      s->guesses_kept++;
+      if (rr > 0) s->guess_gain += std::log10(rr0 / rr);
+    }
+    if (s->net_mode >= 2) {
+      for (double** b : {&s->ta, &s->tb, &s->rold})
+        if ((rc = ensure_buf(s, b))) return rc;
+      flex = true;
+    }
+  }
   if (std::sqrt(rr) > tol * bn) {
-    if ((rc = precond(s, st))) return rc;
+    if ((rc = flex ? precond_net(s, rfx, rfy, rfz, std::sqrt(rr / F.n), st) : precond(s, st)))
+      return rc;
     hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(256), 0, st, F, 0, 0.0, P, s->r, s->p,
                        (const double*)s->z);
     double rz;
@@ -502,21 +629,41 @@ int pcg_solve(fr_pcg* s, const double* rfx, const double* rfy, const double* rfz
       dsum(s, 0, s->p, s->z, nullptr, 0, st);
       if ((rc = read_scalar(s, 0, &pq, st))) return rc;
       const double alpha = rz / pq;
+      if (flex) FR_CHECK(hipMemcpyAsync(s->rold, s->r, gbytes, hipMemcpyDeviceToDevice, st));
       hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(256), 0, st, F, 1, alpha, P, s->r, s->p,
                          (const double*)s->z);
       it++;
+      const double rr_prev = rr;
       dsum(s, 0, s->r, s->r, nullptr, 0, st);
       if ((rc = read_scalar(s, 0, &rr, st))) return rc;
       if (std::sqrt(rr) <= tol * bn) break;
-      if ((rc = precond(s, st))) return rc;
+      bool restart = false;
+      if (flex && rr > rr_prev / 9.0) {  // the network stopped paying: plain V-cycles
+        flex = false;
+        restart = true;
+        s->fallbacks++;
+      }
+      if ((rc = flex ? precond_net(s, rfx, rfy, rfz, std::sqrt(rr / F.n), st) : precond(s, st)))
+        return rc;
       double rz2;
       dsum(s, 0, s->r, s->z, nullptr, 0, st);
       if ((rc = read_scalar(s, 0, &rz2, st))) return rc;
-      const double beta = rz2 / rz;
+      double beta = rz2 / rz;
+      if (flex) {  // Polak-Ribiere: (z, r_new - r_old) / rz
+        double zro;
+        dsum(s, 0, s->rold, s->z, nullptr, 0, st);
+        if ((rc = read_scalar(s, 0, &zro, st))) return rc;
+        beta = (rz2 - zro) / rz;
+      }
+      if (restart) beta = 0.0;
       rz = rz2;
       hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(256), 0, st, F, 2, beta, P, s->r, s->p,
                          (const double*)s->z);
     }
+  }
+  if (collecting) {
+    if ((rc = fill_fine_ghosts(s, P, st))) return rc;
+    if ((rc = unet_data_end(s->collect, F.d, s->p0, P, cscale, st))) return rc;
   }
   if ((rc = fill_fine_ghosts(s, P, st))) return rc;
   FR_LAUNCH_CHECK();
@@ -578,13 +725,32 @@ int fr_pcg_destroy(fr_pcg* s) {
     float* a[6] = {v.cx, v.cy, v.cz, v.e, v.r, v.t};
     for (float* p : a) (void)hipFree(p);
   }
-  (void)hipFree(s->r);
-  (void)hipFree(s->z);
-  (void)hipFree(s->p);
+  for (double* p : {s->r, s->z, s->p, s->p0, s->ta, s->tb, s->rold}) (void)hipFree(p);
   (void)hipFree(s->part);
   (void)hipFree(s->dres);
   (void)hipHostFree(s->hres);
   delete s;
+  return FR_OK;
+}
+
+int fr_pcg_set_learned(fr_pcg* s, fr_unet* net, int mode) {
+  if (!s || mode < 0 || mode > 2 || (mode > 0 && !net)) return FR_EINVAL;
+  s->net = mode ? net : nullptr;
+  s->net_mode = mode;
+  return FR_OK;
+}
+
+int fr_pcg_set_collect(fr_pcg* s, fr_unet_data* d) {
+  if (!s) return FR_EINVAL;
+  s->collect = d;
+  return FR_OK;
+}
+
+int fr_pcg_learned_stats(fr_pcg* s, int* guesses_kept, int* fallbacks, double* log10_gain) {
+  if (!s) return FR_EINVAL;
+  if (guesses_kept) *guesses_kept = s->guesses_kept;
+  if (fallbacks) *fallbacks = s->fallbacks;
+  if (log10_gain) *log10_gain = s->guess_gain;
   return FR_OK;
 }
 

@@ -1130,6 +1130,21 @@ int fr_sim_reserve_particles(fr_sim* s, int capacity) {
   return s->g.reserve(capacity);
 }
 
+int fr_sim_set_learned(fr_sim* s, fr_unet* net, int mode) {
+  if (!s || !s->g.PS_.sp) return FR_EINVAL;
+  return fr_pcg_set_learned(s->g.PS_.sp, net, mode);
+}
+
+int fr_sim_set_collect(fr_sim* s, fr_unet_data* d) {
+  if (!s || !s->g.PS_.sp) return FR_EINVAL;
+  return fr_pcg_set_collect(s->g.PS_.sp, d);
+}
+
+int fr_sim_learned_stats(fr_sim* s, int* guesses_kept, int* fallbacks, double* log10_gain) {
+  if (!s || !s->g.PS_.sp) return FR_EINVAL;
+  return fr_pcg_learned_stats(s->g.PS_.sp, guesses_kept, fallbacks, log10_gain);
+}
+
 int fr_sim_timing(fr_sim* s, double* out) {
   if (!s || !out) return FR_EINVAL;
   out[0] = s->g.t_step;

@@ -152,6 +152,60 @@ int fr_pcg_solve(fr_pcg* s, const double* rfx, const double* rfy,
                  int iterations, double epsilon, int* num_iter, double* relres,
                  void* stream);
 
+/* ------------------------------------------------------------ learned preconditioner
+ * No counterpart in the reference, which uses Hypre only for this solve
+ * (poisson.cpp:115-388): a 3-level 3-D U-Net (16/32/64 channels, 3x3x3
+ * convolutions, 2x2x2 average pooling, 2x2x2 stride-2 up-convolutions, a 1x1
+ * output) with BF16 weights and activations and FP32 accumulation on the
+ * matrix cores.  Input channels: the correction equation's right-hand side
+ * (RHS - S P at entry), 1/rho, the previous P; output: the pressure
+ * increment.  Interior sizes must be multiples of 4.
+ *
+ * fr_unet_forward/backward take device arrays over an nx x ny x nz interior
+ * grid, x-fastest: x is [n][3] FP32 inputs, y and gy are [n] FP32.
+ * fr_unet_backward follows fr_unet_forward at the same size and adds the
+ * parameter gradients to the ones held (fr_unet_zero_grads clears them).
+ * Parameters are one FP32 vector (fr_unet_num_params values): per 3x3x3
+ * convolution (c1a 16->16 [input channels 3..15 unused], c1b 16->16,
+ * c2a 16->32, c2b 32->32, c3a 32->64, c3b 64->64) W[27][ci][co] then b[co],
+ * tap = (dz+1)*9 + (dy+1)*3 + (dx+1); then up2 W[8][64][32] b[32]; d2a
+ * 64->32; d2b 32->32; up1 W[8][32][16] b[16]; d1a 32->16; d1b 16->16; the
+ * output W[16] b[1].  (Up-convolution offset o = 4 oz + 2 oy + ox.) */
+typedef struct fr_unet fr_unet;
+typedef struct fr_unet_data fr_unet_data;
+int fr_unet_create(int seed, fr_unet** out);
+int fr_unet_destroy(fr_unet* u);
+int fr_unet_num_params(fr_unet* u, long long* n);
+int fr_unet_get_params(fr_unet* u, float* host);
+int fr_unet_set_params(fr_unet* u, const float* host);
+int fr_unet_get_grads(fr_unet* u, float* host);
+int fr_unet_zero_grads(fr_unet* u, void* stream);
+int fr_unet_forward(fr_unet* u, int nx, int ny, int nz, const float* x, float* y,
+                    void* stream);
+int fr_unet_backward(fr_unet* u, int nx, int ny, int nz, const float* gy, void* stream);
+int fr_unet_adam_step(fr_unet* u, float lr, void* stream);
+/* Training pairs produced by the solver (fr_pcg_set_collect / fr_sim_set_collect):
+ * up to `capacity` pairs of an nx x ny x nz interior, stored in BF16. */
+int fr_unet_data_create(int nx, int ny, int nz, int capacity, fr_unet_data** out);
+int fr_unet_data_destroy(fr_unet_data* d);
+int fr_unet_data_count(fr_unet_data* d, int* n);
+/* `iters` Adam steps (mean-squared error, batches of random cx x cy x cz
+ * crops of stored pairs, the previous-P channel dropped from half of them,
+ * cosine-decayed learning rate); the mean batch losses of the first and the
+ * last 20 steps are returned. */
+int fr_unet_train(fr_unet* u, fr_unet_data* d, int iters, int batch, int cx, int cy, int cz,
+                  float lr, unsigned seed, double* loss_first, double* loss_last,
+                  void* stream);
+/* PCG with the network: mode 0 plain; 1 learned initial iterate; 2 also the
+ * flexible preconditioner z = N(r) + V(r - S N(r)), falling back to the plain
+ * V-cycle when the network stops paying.  Same stopping test. */
+int fr_pcg_set_learned(fr_pcg* s, fr_unet* net, int mode);
+int fr_pcg_set_collect(fr_pcg* s, fr_unet_data* d);
+/* Counters since creation: learned initial iterates applied, fallbacks to
+ * the plain V-cycle, and the summed log10 of the squared-residual reduction
+ * the applied initial iterates achieved. */
+int fr_pcg_learned_stats(fr_pcg* s, int* guesses_kept, int* fallbacks, double* log10_gain);
+
 /* ------------------------------------------------------------ whole solver
  * The box solver (grid.h:16-115 with its particle set, box poisson solver
  * and the simulation.cpp:34-71 step loop) held on one device.  Created from a
@@ -209,6 +263,11 @@ int fr_sim_set_inflow(fr_sim* s, const double* planes, int n_inlet,
 /* Particle capacity of the solver (default: the params.in particle count);
  * call before fr_sim_initialize.  Exceeding it fails with FR_EOVERFLOW. */
 int fr_sim_reserve_particles(fr_sim* s, int capacity);
+/* Solver = 2 only: the learned forms and pair collection of fr_pcg_* on the
+ * grid's own pressure solve (no counterpart in the reference). */
+int fr_sim_set_learned(fr_sim* s, fr_unet* net, int mode);
+int fr_sim_set_collect(fr_sim* s, fr_unet_data* d);
+int fr_sim_learned_stats(fr_sim* s, int* guesses_kept, int* fallbacks, double* log10_gain);
 /* Rebuild the particle cell lists after particle positions were set. */
 int fr_sim_particles_changed(fr_sim* s);
 /* Step diagnostics (grid.cpp:714-811 subset): out[0..9] = TKE, Rho_Mean,

@@ -72,6 +72,12 @@ def orc():
         L.orc_poisson.argtypes = [G, dp, dp, dp, dp, dp, C.c_int, C.c_double]
         L.orc_poisson.restype = C.c_int
         L.orc_glibc_rand.argtypes = [C.c_int, ip]
+        fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        L.orc_unet_num_params.argtypes = []
+        L.orc_unet_num_params.restype = C.c_long
+        L.orc_unet_forward.argtypes = [C.c_int, C.c_int, C.c_int, fp, C.c_int, dp, dp]
+        L.orc_unet_grad.argtypes = [C.c_int, C.c_int, C.c_int, fp, C.c_int, dp, dp, dp, dp]
+        L.orc_conv3.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]
         L.orc_grid_create.argtypes = [C.c_char_p]
         L.orc_grid_create.restype = C.c_void_p
         L.orc_grid_destroy.argtypes = [C.c_void_p]
