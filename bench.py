@@ -85,7 +85,7 @@ def main():
     ap.add_argument("--no-error", action="store_true",
                     help="with --precision fp32: skip the FP64 run that measures the error "
                          "(it runs after the timed region)")
-    ap.add_argument("--workload", default="box256", choices=["box256", "box512", "duct512"],
+    ap.add_argument("--workload", default="box256", choices=["box256", "box512", "duct512", "unet512"],
                     help="box256: the headline 256^3 box; duct512: the 512x128x128 "
                          "inflow/outflow duct (density ratio 3, 2e4 inlet particles/step)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -104,6 +104,8 @@ def main():
 
     if args.workload == "duct512":
         return bench_duct(args)
+    if args.workload == "unet512":
+        return bench_unet(args)
     if args.workload == "box512":
         args.n, args.particles = 512, 16_000_000
     fp32 = args.precision == "fp32"
@@ -211,6 +213,88 @@ def main():
                                          f"{args.particles}-particle workload after "
                                          f"initialisation ({cw:.1f} s), OpenMP field loops"}
     print(json.dumps(out), flush=True)
+
+
+UNET_CONV = [(16, 16, 1), (16, 16, 1), (16, 32, 2), (32, 32, 2), (32, 64, 4), (64, 64, 4),
+             (64, 32, 2), (32, 32, 2), (32, 16, 1), (16, 16, 1)]  # (ci, co, level)
+
+
+def unet_flops(nx, ny, nz):
+    """Multiply-adds x 2 of one forward pass of the U-Net (3x3x3 convolutions,
+    up-convolutions, 1x1 output; the first convolution's padded input
+    channels counted as the 3 real ones)."""
+    n1 = nx * ny * nz
+    f = 0
+    for l, (ci, co, lv) in enumerate(UNET_CONV):
+        n = n1 // (lv ** 3)
+        f += 2 * 27 * (3 if l == 0 else ci) * co * n
+    f += 2 * 64 * 32 * (n1 // 8) + 2 * 32 * 16 * n1  # up-convolutions (fine pixels)
+    f += 2 * 16 * n1
+    return f
+
+
+def bench_unet(args):
+    """Throughput of the learned preconditioner's network at the duct's full
+    512x128x128 grid: forward passes and training steps (forward + backward)
+    on the BF16 matrix cores; roofline = achieved dense BF16 TFLOP/s against
+    the 2.5 PFLOP/s peak."""
+    import ctypes as C
+    import numpy as np
+    import first_repo_amd as fr
+    L = fr.lib()
+    h = C.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    nx, ny, nz = 512, 128, 128
+    n = nx * ny * nz
+    net = C.c_void_p()
+    fr.check(L.fr_unet_create(1, C.byref(net)))
+    x = np.random.default_rng(0).standard_normal((n, 3)).astype(np.float32)
+    gy = (np.random.default_rng(1).standard_normal(n) / n).astype(np.float32)
+    dx, dy, dg = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for p, nb in ((dx, x.nbytes), (dy, 4 * n), (dg, gy.nbytes)):
+        assert h.hipMalloc(C.byref(p), nb) == 0
+    assert h.hipMemcpy(dx, x.ctypes.data, x.nbytes, 1) == 0
+    assert h.hipMemcpy(dg, gy.ctypes.data, gy.nbytes, 1) == 0
+    flops = unet_flops(nx, ny, nz)
+
+    def fwd():
+        fr.check(L.fr_unet_forward(net, nx, ny, nz, dx, dy, None))
+
+    def train():
+        fwd()
+        fr.check(L.fr_unet_zero_grads(net, None))
+        fr.check(L.fr_unet_backward(net, nx, ny, nz, dg, None))
+
+    out = {}
+    for name, fn, mult in (("forward", fwd, 1), ("train_step", train, 3)):
+        for _ in range(args.warmup):
+            fn()
+        assert h.hipDeviceSynchronize() == 0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        assert h.hipDeviceSynchronize() == 0
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        out[name] = (ms, mult * flops / (ms * 1e-3) / 1e12)
+    L.fr_unet_destroy(net)
+    fms, ftf = out["forward"]
+    line = {
+        "metric": "U-Net forward TFLOP/s at 512x128x128 (learned preconditioner)",
+        "value": round(ftf, 2), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(fms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": "3-level U-Net 16/32/64 channels, 3x3x3 convolutions, "
+                               "512x128x128 grid, random weights and inputs",
+                   "forward_flop": flops,
+                   "train_step_ms": round(out["train_step"][0], 3),
+                   "train_step_TFLOPs": round(out["train_step"][1], 2)},
+        "roofline": {"kernel": "U-Net forward (k_conv3 on v_mfma_f32_16x16x32_bf16)",
+                     "bound": "mfma", "achieved": round(ftf, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(ftf / 2500.0, 4), "traffic": None},
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_duct(args):
