@@ -343,11 +343,42 @@ struct Merge {
   }
 };
 
+// Capacity of a tile's (particle, target) contribution list in LDS; a tile
+// with more contributions uses the per-target merge instead.
+constexpr int PCAP = 1024;
+
+// Exclusive scan of one int per thread over a 256-thread (4-wave) block;
+// returns the thread's offset, *total gets the sum.  wtot: 4 ints of LDS.
+__device__ __forceinline__ int block_excl_scan(int v, int* wtot, int* total) {
+  const int lane = threadIdx.x & 63, w = (threadIdx.y * blockDim.x + threadIdx.x) >> 6;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wtot[w] = inc;
+  __syncthreads();
+  int base = 0;
+  for (int k = 0; k < w; k++) base += wtot[k];
+  *total = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+  return base + inc - v;
+}
+
 // Trilinear deposit of particle.cpp:526-559 (sw == nullptr, concentration)
 // or 586-658 (weight RHS_T): raw per-cell sums for every ghosted cell before
 // the cumulative halo, in ascending particle order (the reference's scatter
 // order), from the rounded-index buckets.  Target (a,b,c) collects from the
 // bucket of cell (a+ox, b+oy, c+oz) the weight with (1-x_l) where ox = 1.
+//
+// A 64x4 target tile reads the particles of 10 source rows; each row's
+// buckets are one contiguous range of the bucket arrays.  Particle-parallel:
+// every source particle writes its (reference index, weight) for each of its
+// in-tile targets into that target's segment of an LDS list (segment sizes
+// from the bucket starts, offsets by a block scan); each contribution's
+// place in its segment is the number of smaller reference indices there;
+// each target then adds its segment in that order.  The same sum as the
+// reference's loop, with no per-target merge.
 template <typename T>
 __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ start,
                                                  const int* __restrict__ list,
@@ -357,22 +388,109 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
                                                  const double* __restrict__ sw,
                                                  T* __restrict__ out) {
   __shared__ int ss[DROWS][DSR];
+  __shared__ int seg[DTX * DTY + 1];
+  __shared__ int cur[DTX * DTY];
+  __shared__ int wtot[4];
+  __shared__ int rcnt[DROWS];
+  __shared__ int rk[PCAP];
+  __shared__ unsigned char tg[PCAP];
+  __shared__ double wv[PCAP];
+  __shared__ unsigned short ord[PCAP];
   const uint3 bk = xcd_block();
   const int a0 = bk.x * DTX, b0 = bk.y * DTY, c = bk.z;
   stage_starts(d, start, a0, b0, c, ss);
+  const int tid = threadIdx.y * DTX + threadIdx.x;
   const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
-  if (a >= d.Nx || b >= d.Ny) return;
-  Merge<8> m;
+  const bool live = a < d.Nx && b < d.Ny;
+  int cb[8], ce[8], nt = 0;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int ox = q & 1, oy = (q >> 1) & 1, oz = q >> 2;
-    if (a + ox < d.Nx && b + oy < d.Ny && c + oz < d.Nz) {
+    if (live && a + ox < d.Nx && b + oy < d.Ny && c + oz < d.Nz) {
       const int r = threadIdx.y + oy + (DTY + 1) * oz, j = threadIdx.x + ox;
-      m.cur[q] = ss[r][j];
-      m.end[q] = ss[r][j + 1];
+      cb[q] = ss[r][j];
+      ce[q] = ss[r][j + 1];
     } else {
-      m.cur[q] = m.end[q] = 0;
+      cb[q] = ce[q] = 0;
     }
+    nt += ce[q] - cb[q];
+  }
+  const int jmax = min(DTX + 1, d.Nx - a0);  // cells of a source row inside the grid
+  if (tid < DROWS) rcnt[tid] = ss[tid][jmax] - ss[tid][0];
+  int total;
+  const int off = block_excl_scan(nt, wtot, &total);  // its barrier also publishes rcnt
+  if (total <= PCAP) {
+    seg[tid] = off;
+    cur[tid] = off;
+    if (tid == 0) seg[DTX * DTY] = total;
+    int rpre[DROWS + 1];
+    rpre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < DROWS; r++) rpre[r + 1] = rpre[r] + rcnt[r];
+    __syncthreads();
+    const int npart = rpre[DROWS];
+    for (int f = tid; f < npart; f += DTX * DTY) {
+      int r = 0, base = 0;  // source row of contribution f (unrolled: rpre stays in registers)
+#pragma unroll
+      for (int k = 1; k < DROWS; k++) r += f >= rpre[k];
+#pragma unroll
+      for (int k = 0; k < DROWS; k++)
+        if (k == r) base = rpre[k];
+      const int u = ss[r][0] + (f - base);
+      int lo = 0, hi = jmax - 1;  // source cell: last bucket starting at or before u
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ss[r][mid] <= u) lo = mid;
+        else hi = mid - 1;
+      }
+      const int sa = a0 + lo, sb = b0 + r % (DTY + 1), oz = r / (DTY + 1);
+      const double x_l = sxl[u], y_l = syl[u], z_l = szl[u];
+      const int rank = list[u];
+#pragma unroll
+      for (int oy = 0; oy < 2; oy++) {
+        const int tb = sb - oy - b0;
+        if (tb < 0 || tb >= DTY) continue;
+#pragma unroll
+        for (int ox = 0; ox < 2; ox++) {
+          const int ta = sa - ox - a0;
+          if (ta < 0 || ta >= DTX) continue;
+          const double wx = ox ? (1 - x_l) : x_l;
+          const double wy = oy ? (1 - y_l) : y_l;
+          const double wz = oz ? (1 - z_l) : z_l;
+          double t = wx * wy * wz;
+          if (sw) t = t * sw[u];
+          const int tl = tb * DTX + ta;
+          const int k = atomicAdd(&cur[tl], 1);
+          if (k < seg[tl + 1]) {
+            rk[k] = rank;
+            tg[k] = (unsigned char)tl;
+            wv[k] = t;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int j = tid; j < total; j += DTX * DTY) {
+      const int tl = tg[j], r = rk[j];
+      const int s0 = seg[tl], s1 = seg[tl + 1];
+      int n = 0;
+      for (int k = s0; k < s1; k++) n += rk[k] < r;
+      ord[s0 + n] = (unsigned short)j;
+    }
+    __syncthreads();
+    if (live) {
+      double acc = 0.0;
+      for (int k = off; k < off + nt; k++) acc = acc + wv[ord[k]];
+      out[at(d, a, b, c)] = (T)acc;
+    }
+    return;
+  }
+  if (!live) return;
+  Merge<8> m;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    m.cur[q] = cb[q];
+    m.end[q] = ce[q];
   }
   m.init(list);
   double acc = 0.0;
@@ -433,12 +551,113 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ 
                                                  const double* __restrict__ srw,
                                                  T* __restrict__ ox, T* __restrict__ oy,
                                                  T* __restrict__ oz) {
+  // Particle-parallel form as in k_dep_cic: 3 segments per target (one per
+  // component); a source particle in cell s adds (1-l_M) R_M to target s and
+  // l_M R_M to target s + e_M.
+  constexpr int VCAP = 896, NT = DTX * DTY;
   __shared__ int ss[DROWS][DSR];
+  __shared__ int seg[3 * NT + 1];
+  __shared__ int cur[3 * NT];
+  __shared__ int wtot[4];
+  __shared__ int rcnt[DROWS];
+  __shared__ int rk[VCAP];
+  __shared__ unsigned short tg[VCAP];
+  __shared__ double wv[VCAP];
+  __shared__ unsigned short ord[VCAP];
   const uint3 bk = xcd_block();
   const int a0 = bk.x * DTX, b0 = bk.y * DTY, c = bk.z;
   stage_starts(d, start, a0 - 1, b0 - 1, c - 1, ss);
+  const int tid = threadIdx.y * DTX + threadIdx.x;
   const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
-  if (a >= d.Nx || b >= d.Ny) return;
+  const bool live = a < d.Nx && b < d.Ny;
+  // per-component contribution counts of this target (buckets t and t - e_M)
+  int nm[3] = {0, 0, 0};
+  if (live) {
+    const int r0 = threadIdx.y + 1 + (DTY + 1), j0 = threadIdx.x + 1;
+    const int own = ss[r0][j0 + 1] - ss[r0][j0];
+    nm[0] = own + (a > 0 ? ss[r0][j0] - ss[r0][j0 - 1] : 0);
+    nm[1] = own + (b > 0 ? ss[r0 - 1][j0 + 1] - ss[r0 - 1][j0] : 0);
+    nm[2] = own + (c > 0 ? ss[r0 - (DTY + 1)][j0 + 1] - ss[r0 - (DTY + 1)][j0] : 0);
+  }
+  // source cells a0-1 .. a0+63 of each row inside the grid: j in [jlo, jhi)
+  const int jlo = a0 == 0 ? 1 : 0, jhi = min(DTX + 1, d.Nx - a0 + 1);
+  if (tid < DROWS) rcnt[tid] = ss[tid][jhi] - ss[tid][jlo];
+  int total;
+  const int off = block_excl_scan(nm[0] + nm[1] + nm[2], wtot, &total);
+  if (total <= VCAP) {
+    seg[3 * tid] = off;
+    seg[3 * tid + 1] = off + nm[0];
+    seg[3 * tid + 2] = off + nm[0] + nm[1];
+    cur[3 * tid] = off;
+    cur[3 * tid + 1] = off + nm[0];
+    cur[3 * tid + 2] = off + nm[0] + nm[1];
+    if (tid == 0) seg[3 * NT] = total;
+    int rpre[DROWS + 1];
+    rpre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < DROWS; r++) rpre[r + 1] = rpre[r] + rcnt[r];
+    __syncthreads();
+    const int npart = rpre[DROWS];
+    const double* L[3] = {sxl, syl, szl};
+    const double* R[3] = {sru, srv, srw};
+    for (int f = tid; f < npart; f += NT) {
+      int r = 0, base = 0;
+#pragma unroll
+      for (int k = 1; k < DROWS; k++) r += f >= rpre[k];
+#pragma unroll
+      for (int k = 0; k < DROWS; k++)
+        if (k == r) base = rpre[k];
+      const int u = ss[r][jlo] + (f - base);
+      int lo = jlo, hi = jhi - 1;  // source cell: last bucket starting at or before u
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ss[r][mid] <= u) lo = mid;
+        else hi = mid - 1;
+      }
+      const int sa = lo - 1, sb = r % (DTY + 1) - 1, sc = r / (DTY + 1) - 1;  // tile-relative
+      const int rank = list[u];
+#pragma unroll
+      for (int M = 0; M < 3; M++) {
+        const double l = L[M][u], Rv = R[M][u];
+#pragma unroll
+        for (int up = 0; up < 2; up++) {  // up = 0: target s, weight (1-l) R; 1: s + e_M, l R
+          const int ta = sa + (M == 0 ? up : 0), tb = sb + (M == 1 ? up : 0),
+                    tc = sc + (M == 2 ? up : 0);
+          if (tc != 0 || ta < 0 || ta >= DTX || tb < 0 || tb >= DTY) continue;
+          if (a0 + ta >= d.Nx || b0 + tb >= d.Ny) continue;
+          const int sgi = 3 * (tb * DTX + ta) + M;
+          const int k = atomicAdd(&cur[sgi], 1);
+          if (k < seg[sgi + 1]) {
+            rk[k] = rank;
+            tg[k] = (unsigned short)sgi;
+            wv[k] = up == 0 ? (1 - l) * Rv : l * Rv;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int j = tid; j < total; j += NT) {
+      const int sgi = tg[j], rr = rk[j];
+      const int s0 = seg[sgi], s1 = seg[sgi + 1];
+      int n = 0;
+      for (int k = s0; k < s1; k++) n += rk[k] < rr;
+      ord[s0 + n] = (unsigned short)j;
+    }
+    __syncthreads();
+    if (live) {
+      const long n = at(d, a, b, c);
+      T* o[3] = {ox, oy, oz};
+      int k = off;
+#pragma unroll
+      for (int M = 0; M < 3; M++) {
+        double acc = 0.0;
+        for (int e = 0; e < nm[M]; e++, k++) acc = acc + wv[ord[k]];
+        o[M][n] = (T)acc;
+      }
+    }
+    return;
+  }
+  if (!live) return;
   const long n = at(d, a, b, c);
   ox[n] = (T)dep_vel_comp<0>(ss, list, sxl, sru, a);
   oy[n] = (T)dep_vel_comp<1>(ss, list, syl, srv, b);
