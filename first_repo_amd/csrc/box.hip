@@ -109,6 +109,139 @@ __global__ void __launch_bounds__(256) k_update_ru_wop(
   }
 }
 
+// The same update, z-marching: a workgroup owns a 64 x 4 column of interior
+// cells over KZ planes and keeps planes k-1, k, k+1 of the six stencil
+// arrays (U and RU_int, with a one-cell x/y halo) in an LDS ring; plane k+2
+// is fetched into registers while plane k is computed, so every U/RU_int
+// value is read from HBM about once.  The expressions are those of
+// k_update_ru_wop with each u[c + offset] read from the ring.  Only interior
+// cells are written; the caller restores the periodic ghosts of the six
+// outputs afterwards (a ghost's value is its image's, as k_update_ru_wop
+// computes it).
+constexpr int WX = 64, WY = 4, WKZ = 16, WHX = WX + 2, WHY = WY + 2;
+constexpr int WPL = 6 * WHY * WHX;               // values per plane of the ring
+constexpr int WPT = (WPL + WX * WY - 1) / (WX * WY);  // per thread
+
+template <int GRAV, typename T>
+__global__ void __launch_bounds__(256) k_update_ru_wop_z(
+    Dev d, BoxCoef cf, WopCoef w, const T* __restrict__ ux, const T* __restrict__ uy,
+    const T* __restrict__ uz, const T* __restrict__ rux, const T* __restrict__ ruy,
+    const T* __restrict__ ruz, const T* __restrict__ sxp, const T* __restrict__ syp,
+    const T* __restrict__ szp, const T* __restrict__ ru0x, const T* __restrict__ ru0y,
+    const T* __restrict__ ru0z, T* __restrict__ np1x, T* __restrict__ np1y, T* __restrict__ np1z,
+    T* __restrict__ newx, T* __restrict__ newy, T* __restrict__ newz, const T* __restrict__ rfx,
+    const T* __restrict__ rfy, const T* __restrict__ rfz) {
+  __shared__ T ring[3][6][WHY][WHX];
+  const T* src[6] = {ux, uy, uz, rux, ruy, ruz};
+  const uint3 bk = xcd_block();
+  const int i0 = 1 + bk.x * WX, j0 = 1 + bk.y * WY, k0 = 1 + bk.z * WKZ;
+  const int k1 = min(d.Nz - 1, k0 + WKZ);
+  const int tid = threadIdx.y * WX + threadIdx.x;
+  // this thread's share of a plane: value t = tid + 256 q -> (array, y, x)
+  auto fetch = [&](int kk, T* reg) {
+#pragma unroll
+    for (int q = 0; q < WPT; q++) {
+      const int t = tid + q * WX * WY;
+      T v = (T)0;
+      if (t < WPL) {
+        const int a = t / (WHY * WHX), r = t % (WHY * WHX);
+        const int yy = j0 - 1 + r / WHX, xx = i0 - 1 + r % WHX;
+        if (xx < d.Nx && yy < d.Ny) v = src[a][at(d, xx, yy, kk)];
+      }
+      reg[q] = v;
+    }
+  };
+  auto store = [&](int slot, const T* reg) {
+#pragma unroll
+    for (int q = 0; q < WPT; q++) {
+      const int t = tid + q * WX * WY;
+      if (t < WPL) (&ring[slot][0][0][0])[t] = reg[q];
+    }
+  };
+  T reg[WPT];
+  fetch(k0 - 1, reg);
+  store((k0 - 1) % 3, reg);
+  fetch(k0, reg);
+  store(k0 % 3, reg);
+  if (k0 < k1) fetch(k0 + 1, reg);
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const bool live = i < d.Nx - 1 && j < d.Ny - 1;
+  const T* Sd[3] = {sxp, syp, szp};
+  const T* R0[3] = {ru0x, ru0y, ru0z};
+  const T* RI[3] = {rux, ruy, ruz};
+  T* N1[3] = {np1x, np1y, np1z};
+  T* NW[3] = {newx, newy, newz};
+  const T* RF[3] = {rfx, rfy, rfz};
+  const T gv[3] = {(T)w.gx, (T)w.gy, (T)w.gz};
+  const int tx = threadIdx.x + 1, ty = threadIdx.y + 1;
+  // per-cell operands of the next plane, fetched one iteration ahead:
+  // S, RU_int, RU, RU_np1 (x, y, z each) and Rho_face under gravity
+  constexpr int NOP = GRAV ? 15 : 12;
+  T op[NOP], opn[NOP];
+  auto fetch_ops = [&](int kk, T* o) {
+    const long c = at(d, i, j, kk);
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      o[m] = Sd[m][c];
+      o[3 + m] = RI[m][c];
+      o[6 + m] = R0[m][c];
+      o[9 + m] = N1[m][c];
+      if constexpr (GRAV != 0) o[12 + m] = RF[m][c];
+    }
+  };
+  if (live) fetch_ops(k0, op);
+  for (int k = k0; k < k1; k++) {
+    __syncthreads();  // plane k-2's slot is free
+    store((k + 1) % 3, reg);
+    __syncthreads();
+    if (k + 1 < k1) {
+      fetch(k + 2, reg);
+      if (live) fetch_ops(k + 1, opn);
+    }
+    if (!live) continue;
+    const int sl[3] = {(k + 2) % 3, k % 3, (k + 1) % 3};  // slots of k-1, k, k+1
+    auto A = [&](int a, int ox, int oy, int oz) -> T {
+      return ring[sl[oz + 1]][a][ty + oy][tx + ox];
+    };
+    auto dv_at = [&](int ox, int oy, int oz) -> T {
+      return (A(0, ox + 1, oy, oz) - A(0, ox, oy, oz)) * (T)d.idx +
+             (A(1, ox, oy + 1, oz) - A(1, ox, oy, oz)) * (T)d.idy +
+             (A(2, ox, oy, oz + 1) - A(2, ox, oy, oz)) * (T)d.idz;
+    };
+    const long c = at(d, i, j, k);
+    const T dv = dv_at(0, 0, 0);
+    const T g[3] = {(dv - dv_at(-1, 0, 0)) * (T)d.idx, (dv - dv_at(0, -1, 0)) * (T)d.idy,
+                    (dv - dv_at(0, 0, -1)) * (T)d.idz};
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      const T lap = (A(m, 1, 0, 0) - 2 * A(m, 0, 0, 0) + A(m, -1, 0, 0)) * (T)d.idx2 +
+                    (A(m, 0, 1, 0) - 2 * A(m, 0, 0, 0) + A(m, 0, -1, 0)) * (T)d.idy2 +
+                    (A(m, 0, 0, 1) - 2 * A(m, 0, 0, 0) + A(m, 0, 0, -1)) * (T)d.idz2;
+      T r = (T)w.a_s * op[m] + (T)w.b_g * g[m] + (T)w.c_l * lap;
+      // convection: F_e(o) = ((U_m(o) + U_m(o - e)) * 0.5) * ((RU_e(o) + RU_e(o - e_m)) * 0.5)
+      const int mx = m == 0, my = m == 1, mz = m == 2;
+      const T h = (T)0.5;
+      auto F = [&](int e, int ox, int oy, int oz) -> T {
+        const int ex = e == 0, ey = e == 1, ez = e == 2;
+        return ((A(m, ox, oy, oz) + A(m, ox - ex, oy - ey, oz - ez)) * h) *
+               ((A(3 + e, ox, oy, oz) + A(3 + e, ox - mx, oy - my, oz - mz)) * h);
+      };
+      const T cv = (F(0, 1, 0, 0) - F(0, 0, 0, 0)) * (T)d.idx +
+                   (F(1, 0, 1, 0) - F(1, 0, 0, 0)) * (T)d.idy +
+                   (F(2, 0, 0, 1) - F(2, 0, 0, 0)) * (T)d.idz;
+      r = r - cv;
+      r = r + (T)w.A * op[3 + m];
+      if (m == 0) r = r - (T)w.sx_forc;
+      if constexpr (GRAV != 0) r = r + (op[12 + m] - (T)w.rho0) * gv[m];
+      const T np1 = op[9 + m] + (T)cf.cpost * r;
+      N1[m][c] = np1;
+      NW[m][c] = cf.last ? np1 : 1 * op[6 + m] + (T)cf.cpre * r;
+    }
+#pragma unroll
+    for (int q = 0; q < NOP; q++) op[q] = opn[q];
+  }
+}
+
 // grid::Compute_Div_U_new (grid.cpp:556-563) + grid::Compute_RHS_Pois
 // (grid.cpp:629-637) up to the zero-mean shift: writes Rho_face = I_C2F(
 // Rho_new) and RHS_Pois = (Div_F2C(RU_new/Rho_face) - divergence)/(dt*pre).
@@ -191,17 +324,24 @@ int box_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w, const T
                       const T* const ru_int[3], const T* const S[3], const T* const ru[3],
                       T* const ru_np1[3], T* const ru_new[3], const T* const rf[3],
                       hipStream_t s) {
+  // z-marching kernel over the interior, then the periodic ghosts of the outputs
+  const dim3 g(nblocks(d.Nx - 2, WX), nblocks(d.Ny - 2, WY), nblocks(d.Nz - 2, WKZ));
   if (w.gravity)
-    hipLaunchKernelGGL((k_update_ru_wop<1, T>), grid3(d), dim3(64, 4), 0, s, d, cf, w, u[0], u[1],
+    hipLaunchKernelGGL((k_update_ru_wop_z<1, T>), g, dim3(WX, WY), 0, s, d, cf, w, u[0], u[1],
                        u[2], ru_int[0], ru_int[1], ru_int[2], S[0], S[1], S[2], ru[0], ru[1],
                        ru[2], ru_np1[0], ru_np1[1], ru_np1[2], ru_new[0], ru_new[1], ru_new[2],
                        rf[0], rf[1], rf[2]);
   else
-    hipLaunchKernelGGL((k_update_ru_wop<0, T>), grid3(d), dim3(64, 4), 0, s, d, cf, w, u[0], u[1],
+    hipLaunchKernelGGL((k_update_ru_wop_z<0, T>), g, dim3(WX, WY), 0, s, d, cf, w, u[0], u[1],
                        u[2], ru_int[0], ru_int[1], ru_int[2], S[0], S[1], S[2], ru[0], ru[1],
                        ru[2], ru_np1[0], ru_np1[1], ru_np1[2], ru_new[0], ru_new[1], ru_new[2],
                        rf[0], rf[1], rf[2]);
   FR_LAUNCH_CHECK();
+  for (int m = 0; m < 3; m++) {
+    int rc = ghost_fill(d, ru_np1[m], s);
+    if (!rc) rc = ghost_fill(d, ru_new[m], s);
+    if (rc) return rc;
+  }
   return FR_OK;
 }
 
