@@ -1,6 +1,7 @@
 // LDS radix-2 FFT building blocks shared by the box and duct pressure
-// solves (gfx950, FP64).  A workgroup of FT threads holds W pencils of
-// length L in LDS (stride L+1 complex) and transforms them in place.
+// solves (gfx950).  A workgroup of FT threads holds W pencils of length L in
+// LDS (stride L+1 complex) and transforms them in place.  C2 is double2 (the
+// FP64 solves) or float2 (the inner transform of the FP32 precision form).
 #pragma once
 #include <cmath>
 #include <vector>
@@ -26,12 +27,15 @@ __device__ __forceinline__ int bitrev(int x, int logn) {
   return logn == 0 ? 0 : (int)(__brev((unsigned)x) >> (32 - logn));
 }
 
-__device__ __forceinline__ void bfly(double2& u, double2& t, const double2 w) {
-  const double vr = t.x * w.x - t.y * w.y;
-  const double vi = t.x * w.y + t.y * w.x;
-  const double2 a = u;
-  u = make_double2(a.x + vr, a.y + vi);
-  t = make_double2(a.x - vr, a.y - vi);
+template <typename C2>
+__device__ __forceinline__ void bfly(C2& u, C2& t, const C2 w) {
+  const auto vr = t.x * w.x - t.y * w.y;
+  const auto vi = t.x * w.y + t.y * w.x;
+  const C2 a = u;
+  u.x = a.x + vr;
+  u.y = a.y + vi;
+  t.x = a.x - vr;
+  t.y = a.y - vi;
 }
 
 // In-place radix-2 DIT over W pencils (stride ps complex) already in
@@ -39,7 +43,8 @@ __device__ __forceinline__ void bfly(double2& u, double2& t, const double2 w) {
 // Stages are applied in pairs from registers (each thread owns the four
 // elements two consecutive stages combine), with exactly the butterflies and
 // twiddles of the stage-by-stage algorithm, so the result is bit-identical.
-__device__ inline void fft_lds(double2* buf, int L, int lg, int W, int ps, const double2* tw) {
+template <typename C2>
+__device__ inline void fft_lds(C2* buf, int L, int lg, int W, int ps, const C2* tw) {
   int half = 1, s = 0;
   const int q4 = L >> 2;
   for (; s + 1 < lg; s += 2, half <<= 2) {
@@ -51,8 +56,8 @@ __device__ inline void fft_lds(double2* buf, int L, int lg, int W, int ps, const
       const int grp = bb / half;
       const int pos = bb - grp * half;
       const int i0 = p * ps + grp * 4 * half + pos;
-      double2 x0 = buf[i0], x1 = buf[i0 + half], x2 = buf[i0 + 2 * half], x3 = buf[i0 + 3 * half];
-      const double2 wa = tw[pos * stepA];
+      C2 x0 = buf[i0], x1 = buf[i0 + half], x2 = buf[i0 + 2 * half], x3 = buf[i0 + 3 * half];
+      const C2 wa = tw[pos * stepA];
       bfly(x0, x1, wa);
       bfly(x2, x3, wa);
       bfly(x0, x2, tw[pos * stepB]);
@@ -72,7 +77,7 @@ __device__ inline void fft_lds(double2* buf, int L, int lg, int W, int ps, const
       const int grp = bb / half;
       const int pos = bb - grp * half;
       const int i0 = p * ps + grp * 2 * half + pos;
-      double2 u = buf[i0], t = buf[i0 + half];
+      C2 u = buf[i0], t = buf[i0 + half];
       bfly(u, t, tw[pos * step]);
       buf[i0] = u;
       buf[i0 + half] = t;
@@ -81,7 +86,8 @@ __device__ inline void fft_lds(double2* buf, int L, int lg, int W, int ps, const
   __syncthreads();
 }
 
-__device__ inline void load_tw(double2* tws, const double2* tw, int L) {
+template <typename C2>
+__device__ inline void load_tw(C2* tws, const C2* tw, int L) {
   for (int t = threadIdx.x; t <= L / 2; t += FT) tws[t] = tw[t];
 }
 
@@ -155,6 +161,19 @@ inline int upload_tw(int n, int sign, double2** out) {
   }
   FR_CHECK(hipMalloc(out, sizeof(double2) * t.size()));
   FR_CHECK(hipMemcpy(*out, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice));
+  return FR_OK;
+}
+
+// The same twiddles rounded to float.
+inline int upload_tw(int n, int sign, float2** out) {
+  const double PI_ = 3.141592653589793;
+  std::vector<float2> t(n / 2 + 1);
+  for (int k = 0; k <= n / 2; k++) {
+    double ang = 2.0 * PI_ * (double)k / (double)n;
+    t[k] = make_float2((float)cos(ang), (float)(sign * sin(ang)));
+  }
+  FR_CHECK(hipMalloc(out, sizeof(float2) * t.size()));
+  FR_CHECK(hipMemcpy(*out, t.data(), sizeof(float2) * t.size(), hipMemcpyHostToDevice));
   return FR_OK;
 }
 

@@ -39,21 +39,36 @@ struct PoisDev {
 // Pass X: per row (j,k) the residual, packed as z_m = r_2m + i r_2m+1, an
 // nx/2-point FFT, then the real-transform unpacking to X_k, k = 0..nx/2:
 //   E = (Z_k + conj Z_{M-k})/2, O = (Z_k - conj Z_{M-k})/(2i), X = E + W^k O.
-template <bool INV, typename TC>
+// R: real type of the spectrum (double; float for the precision form's inner
+// transform, where the residual is still formed in double and rounded once).
+template <typename R>
+struct Cx;
+template <>
+struct Cx<double> {
+  using T = double2;
+};
+template <>
+struct Cx<float> {
+  using T = float2;
+};
+
+template <bool INV, typename TC, typename R>
 __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __restrict__ P,
                                                    const TC* __restrict__ cx,
                                                    const TC* __restrict__ cy,
                                                    const TC* __restrict__ cz,
                                                    const double* __restrict__ RHS,
-                                                   const double2* __restrict__ twM,
-                                                   const double2* __restrict__ twN,
-                                                   double2* __restrict__ out) {
-  extern __shared__ double2 sm[];
+                                                   const typename Cx<R>::T* __restrict__ twM,
+                                                   const typename Cx<R>::T* __restrict__ twN,
+                                                   typename Cx<R>::T* __restrict__ out) {
+  using C2 = typename Cx<R>::T;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  C2* sm = reinterpret_cast<C2*>(smraw);
   const Dev& d = q.d;
   const int M = q.nx >> 1, W = q.wx, ps = M + 1;
-  double2* buf = sm;
-  double2* twm = sm + W * ps;
-  double2* twn = twm + (M / 2 + 1);
+  C2* buf = sm;
+  C2* twm = sm + W * ps;
+  C2* twn = twm + (M / 2 + 1);
   load_tw(twm, twM, M);
   load_tw(twn, twN, q.nx);
   const long rows = (long)q.ny * q.nz;
@@ -69,43 +84,52 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
       r0 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c);
       r1 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c + 1);
     }
-    buf[p * ps + bitrev(m, q.lxm)] = make_double2(r0, r1);
+    C2 v;
+    v.x = (R)r0;
+    v.y = (R)r1;
+    buf[p * ps + bitrev(m, q.lxm)] = v;
   }
   fft_lds(buf, M, q.lxm, W, ps, twm);
   for (int t = threadIdx.x; t < W * (M + 1); t += FT) {
     const int p = t / (M + 1), k = t - p * (M + 1);
     const long row = row0 + p;
     if (row >= rows) continue;
-    const double2 A = buf[p * ps + (k == M ? 0 : k)];
-    const double2 B = buf[p * ps + (k == 0 ? 0 : M - k)];
-    const double Er = 0.5 * (A.x + B.x), Ei = 0.5 * (A.y - B.y);
-    const double Dr = A.x - B.x, Di = A.y + B.y;
-    const double Or = 0.5 * Di, Oi = -0.5 * Dr;
-    const double2 w = twn[k];
-    const double Tr = Or * w.x - Oi * w.y, Ti = Or * w.y + Oi * w.x;
-    out[row * (M + 1) + k] = make_double2(Er + Tr, Ei + Ti);
+    const C2 A = buf[p * ps + (k == M ? 0 : k)];
+    const C2 B = buf[p * ps + (k == 0 ? 0 : M - k)];
+    const R Er = (R)0.5 * (A.x + B.x), Ei = (R)0.5 * (A.y - B.y);
+    const R Dr = A.x - B.x, Di = A.y + B.y;
+    const R Or = (R)0.5 * Di, Oi = (R)-0.5 * Dr;
+    const C2 w = twn[k];
+    const R Tr = Or * w.x - Oi * w.y, Ti = Or * w.y + Oi * w.x;
+    C2 o;
+    o.x = Er + Tr;
+    o.y = Ei + Ti;
+    out[row * (M + 1) + k] = o;
   }
 }
 
 // Strided pass along y (axis 1) or z (axis 2) over a tile of W consecutive x
 // frequencies.  For axis 2 with `divide` set: forward FFT, divide by lambda
 // except the zero mode (poisson.cpp:438-451), then inverse FFT (tw2) in LDS.
+template <typename R>
 __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int divide,
-                                                     const double2* __restrict__ tw,
-                                                     const double2* __restrict__ tw2,
+                                                     const typename Cx<R>::T* __restrict__ tw,
+                                                     const typename Cx<R>::T* __restrict__ tw2,
                                                      const double* __restrict__ lamx,
                                                      const double* __restrict__ lamy,
                                                      const double* __restrict__ lamz,
-                                                     double2* __restrict__ data) {
-  extern __shared__ double2 sm[];
+                                                     typename Cx<R>::T* __restrict__ data) {
+  using C2 = typename Cx<R>::T;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  C2* sm = reinterpret_cast<C2*>(smraw);
   const int L = axis == 1 ? q.ny : q.nz;
   const int lg = axis == 1 ? q.ly : q.lz;
   const int W = axis == 1 ? q.wy : q.wz;
   const int ps = L + 1;
   const int nxh = q.nxh;
-  double2* buf = sm;
-  double2* tws = sm + W * ps;
-  double2* tws2 = tws + (L / 2 + 1);
+  C2* buf = sm;
+  C2* tws = sm + W * ps;
+  C2* tws2 = tws + (L / 2 + 1);
   load_tw(tws, tw, L);
   if (divide) load_tw(tws2, tw2, L);
   const int tiles_x = (nxh + W - 1) / W;
@@ -125,8 +149,8 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
       const int e = t / W, p = t - e * W;
       const int ix = i0 + p, jy = other, kz = e;
       if (ix < nxh && ix + jy + kz > 0) {
-        const double lam = lamx[ix] + lamy[jy] + lamz[kz];
-        double2 v = buf[p * ps + e];
+        const R lam = (R)(lamx[ix] + lamy[jy] + lamz[kz]);
+        C2 v = buf[p * ps + e];
         v.y /= lam;
         v.x /= lam;
         buf[p * ps + e] = v;
@@ -138,7 +162,7 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
       const int p = t / L, e = t - p * L;
       const int r = bitrev(e, lg);
       if (e < r) {
-        double2 a = buf[p * ps + e];
+        C2 a = buf[p * ps + e];
         buf[p * ps + e] = buf[p * ps + r];
         buf[p * ps + r] = a;
       }
@@ -155,18 +179,22 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
 // i w^k (X_k - conj X_{M-k}), k < M, an nx/2-point inverse FFT, x_2m = Re z_m,
 // x_2m+1 = Im z_m; dp = (-1)*x/N (poisson.cpp:455-461); P += dp at the
 // interior; per-block max|dp| (tensor0::max_abs).
-__global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q, const double2* __restrict__ in,
-                                                   const double2* __restrict__ twM,
-                                                   const double2* __restrict__ twN,
+template <typename R>
+__global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q,
+                                                   const typename Cx<R>::T* __restrict__ in,
+                                                   const typename Cx<R>::T* __restrict__ twM,
+                                                   const typename Cx<R>::T* __restrict__ twN,
                                                    double* __restrict__ P,
                                                    double* __restrict__ bmax) {
-  extern __shared__ double2 sm[];
+  using C2 = typename Cx<R>::T;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  C2* sm = reinterpret_cast<C2*>(smraw);
   __shared__ double red[FT];
   const Dev& d = q.d;
   const int M = q.nx >> 1, W = q.wx, ps = M + 1;
-  double2* buf = sm;
-  double2* twm = sm + W * ps;
-  double2* twn = twm + (M / 2 + 1);
+  C2* buf = sm;
+  C2* twm = sm + W * ps;
+  C2* twn = twm + (M / 2 + 1);
   load_tw(twm, twM, M);
   load_tw(twn, twN, q.nx);
   __syncthreads();
@@ -176,14 +204,17 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q, const double2* __r
   for (int t = threadIdx.x; t < W * M; t += FT) {
     const int p = t / M, k = t - p * M;
     const long row = row0 + p;
-    double2 z = make_double2(0.0, 0.0);
+    C2 z;
+    z.x = (R)0;
+    z.y = (R)0;
     if (row < rows) {
-      const double2 A = in[row * (M + 1) + k], B = in[row * (M + 1) + (M - k)];
-      const double Sr = A.x + B.x, Si = A.y - B.y;
-      const double Dr = A.x - B.x, Di = A.y + B.y;
-      const double2 w = twn[k];
-      const double Tr = Dr * w.x - Di * w.y, Ti = Dr * w.y + Di * w.x;
-      z = make_double2(Sr - Ti, Si + Tr);
+      const C2 A = in[row * (M + 1) + k], B = in[row * (M + 1) + (M - k)];
+      const R Sr = A.x + B.x, Si = A.y - B.y;
+      const R Dr = A.x - B.x, Di = A.y + B.y;
+      const C2 w = twn[k];
+      const R Tr = Dr * w.x - Di * w.y, Ti = Dr * w.y + Di * w.x;
+      z.x = Sr - Ti;
+      z.y = Si + Tr;
     }
     buf[p * ps + bitrev(k, q.lxm)] = z;
   }
@@ -194,9 +225,9 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q, const double2* __r
     const long row = row0 + p;
     if (row < rows) {
       const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
-      const double2 z = buf[p * ps + e];
-      const double dp0 = (-1) * z.x / q.tot;
-      const double dp1 = (-1) * z.y / q.tot;
+      const C2 z = buf[p * ps + e];
+      const double dp0 = (-1) * (double)z.x / q.tot;
+      const double dp1 = (-1) * (double)z.y / q.tot;
       const long c = at(d, 2 * e + 1, jj + 1, kk + 1);
       P[c] = P[c] + dp0;
       P[c + 1] = P[c + 1] + dp1;
@@ -228,6 +259,13 @@ using namespace fr;
 struct fr_box_poisson {
   fr_geom g;
   PoisDev q;
+  // float spectrum, twiddles and pencil widths of the precision form's inner
+  // transform (allocated at its first use)
+  PoisDev q32;
+  float2* data32 = nullptr;
+  float2 *twf32[3] = {}, *twi32[3] = {}, *twfm32 = nullptr, *twim32 = nullptr;
+  size_t lds32[3] = {};
+  int nbx32 = 0;
   double2* data = nullptr;  // (nx/2+1)*ny*nz complex
   double2 *twf[3] = {}, *twi[3] = {};  // length nx, ny, nz
   double2 *twfm = nullptr, *twim = nullptr;  // length nx/2
@@ -240,29 +278,104 @@ struct fr_box_poisson {
 };
 
 // One defect-correction iteration; leaves max|dp| in s->dres[0].  With inv
-// the coefficient arrays hold Rho_face and C = 1./Rho_face.
-template <typename TC>
+// the coefficient arrays hold Rho_face and C = 1./Rho_face.  R is the real
+// type of the spectral solve (the residual and P stay double).
+template <typename TC, typename R>
 static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz,
                             bool inv, double* P, const double* RHS, hipStream_t st) {
-  PoisDev& q = s->q;
+  using C2 = typename Cx<R>::T;
+  const bool f = sizeof(R) == 4;
+  PoisDev& q = f ? s->q32 : s->q;
+  C2* data = f ? (C2*)s->data32 : (C2*)s->data;
+  C2* const* twf = f ? (C2* const*)s->twf32 : (C2* const*)s->twf;
+  C2* const* twi = f ? (C2* const*)s->twi32 : (C2* const*)s->twi;
+  const C2* twfm = f ? (const C2*)s->twfm32 : (const C2*)s->twfm;
+  const C2* twim = f ? (const C2*)s->twim32 : (const C2*)s->twim;
+  const size_t* lds = f ? s->lds32 : s->lds;
+  const int nbx = f ? s->nbx32 : s->nbx;
   if (inv)
-    hipLaunchKernelGGL((k_pass_x_fwd<true, TC>), dim3(s->nbx), dim3(FT), s->lds[0], st, q, P, cx,
-                       cy, cz, RHS, s->twfm, s->twf[0], s->data);
+    hipLaunchKernelGGL((k_pass_x_fwd<true, TC, R>), dim3(nbx), dim3(FT), lds[0], st, q, P, cx, cy,
+                       cz, RHS, twfm, twf[0], data);
   else
-    hipLaunchKernelGGL((k_pass_x_fwd<false, TC>), dim3(s->nbx), dim3(FT), s->lds[0], st, q, P,
-                       cx, cy, cz, RHS, s->twfm, s->twf[0], s->data);
+    hipLaunchKernelGGL((k_pass_x_fwd<false, TC, R>), dim3(nbx), dim3(FT), lds[0], st, q, P, cx,
+                       cy, cz, RHS, twfm, twf[0], data);
   const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.nxh + q.wz - 1) / q.wz;
-  hipLaunchKernelGGL(k_pass_strided, dim3(ty * q.nz), dim3(FT), s->lds[1], st, q, 1, 0,
-                     s->twf[1], s->twf[1], s->lam[0], s->lam[1], s->lam[2], s->data);
-  hipLaunchKernelGGL(k_pass_strided, dim3(tz * q.ny), dim3(FT), s->lds[2], st, q, 2, 1,
-                     s->twf[2], s->twi[2], s->lam[0], s->lam[1], s->lam[2], s->data);
-  hipLaunchKernelGGL(k_pass_strided, dim3(ty * q.nz), dim3(FT), s->lds[1], st, q, 1, 0,
-                     s->twi[1], s->twi[1], s->lam[0], s->lam[1], s->lam[2], s->data);
-  hipLaunchKernelGGL(k_pass_x_inv, dim3(s->nbx), dim3(FT), s->lds[0], st, q, s->data, s->twim,
-                     s->twi[0], P, s->bmax);
+  hipLaunchKernelGGL(k_pass_strided<R>, dim3(ty * q.nz), dim3(FT), lds[1], st, q, 1, 0, twf[1],
+                     twf[1], s->lam[0], s->lam[1], s->lam[2], data);
+  hipLaunchKernelGGL(k_pass_strided<R>, dim3(tz * q.ny), dim3(FT), lds[2], st, q, 2, 1, twf[2],
+                     twi[2], s->lam[0], s->lam[1], s->lam[2], data);
+  hipLaunchKernelGGL(k_pass_strided<R>, dim3(ty * q.nz), dim3(FT), lds[1], st, q, 1, 0, twi[1],
+                     twi[1], s->lam[0], s->lam[1], s->lam[2], data);
+  hipLaunchKernelGGL(k_pass_x_inv<R>, dim3(nbx), dim3(FT), lds[0], st, q, data, twim, twi[0], P,
+                     s->bmax);
   FR_LAUNCH_CHECK();
-  int rc = max_final(s->bmax, s->nbx, s->dres, st);
+  int rc = max_final(s->bmax, nbx, s->dres, st);
   return rc ? rc : ghost_fill(q.d, P, st);
+}
+
+// Pencil widths and LDS bytes of the three passes for complex elements of
+// `csize` bytes within `want` bytes of LDS.
+static void plan_passes(PoisDev& q, int csize, int want, size_t lds[3], int* nbx) {
+  const int nx = q.nx, ny = q.ny, nz = q.nz, M = nx / 2;
+  const int budget = want / csize - 160 * 16 / csize;  // room for the static reduction array
+  q.wx = pencils(M, ny * nz, budget, (M / 2 + 1) + (nx / 2 + 1));
+  q.wy = pencils(ny, q.nxh, budget, 2 * (ny / 2 + 1));
+  q.wz = pencils(nz, q.nxh, budget, 2 * (nz / 2 + 1));
+  lds[0] = (size_t)csize * (q.wx * (M + 1) + (M / 2 + 1) + (nx / 2 + 1));
+  lds[1] = (size_t)csize * (q.wy * (ny + 1) + 2 * (ny / 2 + 1));
+  lds[2] = (size_t)csize * (q.wz * (nz + 1) + 2 * (nz / 2 + 1));
+  *nbx = (int)(((long)ny * nz + q.wx - 1) / q.wx);
+}
+
+// LDS budget per workgroup: 32 KB by default (several workgroups per CU);
+// FR_FFT_LDS_BYTES overrides it for tuning, capped by the device limit.
+static int lds_budget() {
+  int dev = 0, maxsh = 65536;
+  if (hipGetDevice(&dev) == hipSuccess)
+    if (hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      maxsh = 65536;
+  if (maxsh > 163840) maxsh = 163840;
+  if (maxsh < 65536) maxsh = 65536;
+  int want = 32768;
+  if (const char* e = getenv("FR_FFT_LDS_BYTES")) want = atoi(e);
+  if (want < 8192) want = 8192;
+  if (want > maxsh) want = maxsh;
+  return want;
+}
+
+template <typename R>
+static int allow_lds(size_t need) {
+  if (need <= 65536) return FR_OK;
+  FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<false, double, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<true, double, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<true, float, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  FR_CHECK(hipFuncSetAttribute((const void*)k_pass_strided<R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_inv<R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  return FR_OK;
+}
+
+// Float resources of the precision form, at its first solve.
+static int ensure_f32(fr_box_poisson* s) {
+  if (s->data32) return FR_OK;
+  PoisDev& q = s->q32;
+  q = s->q;
+  plan_passes(q, (int)sizeof(float2), lds_budget(), s->lds32, &s->nbx32);
+  FR_RET(allow_lds<float>(std::max({s->lds32[0], s->lds32[1], s->lds32[2]})));
+  const int n[3] = {q.nx, q.ny, q.nz};
+  for (int a = 0; a < 3; a++) {
+    FR_RET(upload_tw(n[a], -1, &s->twf32[a]));
+    FR_RET(upload_tw(n[a], 1, &s->twi32[a]));
+  }
+  FR_RET(upload_tw(n[0] / 2, -1, &s->twfm32));
+  FR_RET(upload_tw(n[0] / 2, 1, &s->twim32));
+  if (s->nbx32 > s->nbx) return FR_EINVAL;  // bmax holds nbx entries
+  FR_CHECK(hipMalloc(&s->data32, sizeof(float2) * (size_t)q.nxh * n[1] * n[2]));
+  return FR_OK;
 }
 
 namespace fr {
@@ -272,10 +385,15 @@ int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* c
                       double* P, const double* RHS, int iterations, double epsilon, int* num_iter,
                       hipStream_t st) {
   if (!s || !cx || !cy || !cz || !P || !RHS) return FR_EINVAL;
+  // FP32 coefficients (the precision form): FP32 inner spectral solve, with
+  // the FP64 residual and P of the defect correction around it
+  constexpr bool f32 = sizeof(TC) == 4;
+  if (f32) FR_RET(ensure_f32(s));
   int i = 0;
   double error = 100;
   while ((i < iterations) && (error > epsilon)) {
-    int rc = box_poisson_iter(s, cx, cy, cz, inv, P, RHS, st);
+    int rc = f32 ? box_poisson_iter<TC, float>(s, cx, cy, cz, inv, P, RHS, st)
+                 : box_poisson_iter<TC, double>(s, cx, cy, cz, inv, P, RHS, st);
     if (rc) return rc;
     FR_CHECK(hipMemcpyAsync(s->hres, s->dres, sizeof(double), hipMemcpyDeviceToHost, st));
     FR_CHECK(hipStreamSynchronize(st));
@@ -342,40 +460,11 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
     if (!rc) rc = upload_tw(n[0] / 2, 1, &s->twim);
     if (rc) return rc;
   }
-  // LDS budget per workgroup: 32 KB by default (several workgroups per CU);
-  // FR_FFT_LDS_BYTES overrides it for tuning, capped by the device limit.
-  int dev = 0, maxsh = 65536;
-  if (hipGetDevice(&dev) == hipSuccess)
-    if (hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-      maxsh = 65536;
-  if (maxsh > 163840) maxsh = 163840;
-  if (maxsh < 65536) maxsh = 65536;
-  int want = 32768;
-  if (const char* e = getenv("FR_FFT_LDS_BYTES")) want = atoi(e);
-  if (want < 8192) want = 8192;
-  if (want > maxsh) want = maxsh;
-  const int budget = want / (int)sizeof(double2) - 160;  // room for the static reduction array
-  const int M = n[0] / 2;
-  q.wx = pencils(M, n[1] * n[2], budget, (M / 2 + 1) + (n[0] / 2 + 1));
-  q.wy = pencils(n[1], q.nxh, budget, 2 * (n[1] / 2 + 1));
-  q.wz = pencils(n[2], q.nxh, budget, 2 * (n[2] / 2 + 1));
-  s->lds[0] = sizeof(double2) * (q.wx * (M + 1) + (M / 2 + 1) + (n[0] / 2 + 1));
-  s->lds[1] = sizeof(double2) * (q.wy * (n[1] + 1) + 2 * (n[1] / 2 + 1));
-  s->lds[2] = sizeof(double2) * (q.wz * (n[2] + 1) + 2 * (n[2] / 2 + 1));
-  const size_t need = std::max({s->lds[0], s->lds[1], s->lds[2]});
-  if (need > 65536) {
-    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<false, double>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<true, double>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<true, float>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_strided,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-    FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_inv,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  plan_passes(q, (int)sizeof(double2), lds_budget(), s->lds, &s->nbx);
+  {
+    const int rc = allow_lds<double>(std::max({s->lds[0], s->lds[1], s->lds[2]}));
+    if (rc) return rc;
   }
-  s->nbx = (int)(((long)n[1] * n[2] + q.wx - 1) / q.wx);
   FR_CHECK(hipMalloc(&s->bmax, sizeof(double) * (s->nbx + 1)));
   FR_CHECK(hipMalloc(&s->dres, sizeof(double) * 2));
   FR_CHECK(hipHostMalloc(&s->hres, sizeof(double) * 2, hipHostMallocDefault));
@@ -386,11 +475,16 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
 int fr_box_poisson_destroy(fr_box_poisson* s) {
   if (!s) return FR_OK;
   (void)hipFree(s->data);
+  (void)hipFree(s->data32);
   for (int a = 0; a < 3; a++) {
     (void)hipFree(s->twf[a]);
     (void)hipFree(s->twi[a]);
     (void)hipFree(s->lam[a]);
+    (void)hipFree(s->twf32[a]);
+    (void)hipFree(s->twi32[a]);
   }
+  (void)hipFree(s->twfm32);
+  (void)hipFree(s->twim32);
   (void)hipFree(s->twfm);
   (void)hipFree(s->twim);
   (void)hipFree(s->bmax);
