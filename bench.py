@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_update_ru_wop launch from rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (each scaled by a plain copy of known size: factors 2.0 and 1.0), measured on
 # the named workloads; profiles/hbm_traffic_box256_fp64_box512_fp32.md
-RU_WOP_TRAFFIC = {(256, "fp64"): 5.020e9, (512, "fp32"): 16.696e9}
+RU_WOP_TRAFFIC = {(256, "fp64"): 3.191e9, (512, "fp32"): 12.495e9}
 METRIC = ("Mcell\u00b7steps/s full RK4 step w/ particles; "
           "Poisson iters+ms to 1e-10 vs plain")
 
@@ -179,7 +179,8 @@ def main():
                    "step_hbm_frac": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
                    "TKE": float(diag[0]), "Rho_Mean": float(diag[1]),
                    "mean_particle_T": float(diag[8])},
-        "roofline": {"kernel": "k_update_ru_wop", "bound": "hbm",
+        "roofline": {"kernel": "k_update_ru_wop_z (+ periodic ghost shells of its 6 outputs)",
+                     "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": RU_WOP_TRAFFIC.get((args.n, args.precision))
