@@ -136,6 +136,7 @@ struct AdvCoef {
 // operands of each particle gathered into bucket order.
 struct Buckets {
   int *cnt = nullptr, *start = nullptr, *list = nullptr, *key = nullptr, *slot = nullptr;
+  int* list2 = nullptr;  // unsorted bucket lists while a build places them
   double* s[6] = {};
   bool valid = false;
 };

@@ -238,25 +238,22 @@ __global__ void k_bucket_scatter(int np, const int* __restrict__ key, const int*
   list[start[key[i]] + slot[i]] = rank[i];
 }
 
-// Sort each bucket by reference index (slots from atomics come in any order),
-// then record every storage slot's bucket position in `pos` (the atomic-slot
-// array reused).
-__global__ void k_bucket_sort(long ncell, const int* __restrict__ start, int* __restrict__ list,
-                              const int* __restrict__ inv, int* __restrict__ pos) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
+// Place every particle in its bucket by reference index (slots from atomics
+// come in any order): its position is the bucket start plus the number of
+// smaller reference indices in the bucket's unsorted list; recorded per
+// storage slot in `pos` (the atomic-slot array reused).  One thread per
+// particle, so the cost follows the particles, not the cells.
+__global__ void k_bucket_place(int np, const int* __restrict__ key, const int* __restrict__ start,
+                               const int* __restrict__ rank, const int* __restrict__ unsorted,
+                               int* __restrict__ list, int* __restrict__ pos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int c = key[i], r = rank[i];
   const int b = start[c], e = start[c + 1];
-  if (e == b) return;
-  for (int u = b + 1; u < e; u++) {
-    const int v = list[u];
-    int w = u - 1;
-    while (w >= b && list[w] > v) {
-      list[w + 1] = list[w];
-      w--;
-    }
-    list[w + 1] = v;
-  }
-  for (int u = b; u < e; u++) pos[inv[list[u]]] = u;
+  int n = 0;
+  for (int u = b; u < e; u++) n += unsorted[u] < r;
+  list[b + n] = r;
+  pos[i] = b + n;
 }
 
 // Per-particle deposit operands written to the particle's bucket position, so
@@ -888,6 +885,7 @@ static int alloc_buckets(Buckets& B, long ncell, int cap, int nops) {
   FR_CHECK(hipMalloc(&B.cnt, sizeof(int) * (ncell + 1)));
   FR_CHECK(hipMalloc(&B.start, sizeof(int) * (ncell + 1)));
   FR_CHECK(hipMalloc(&B.list, sizeof(int) * cap));
+  FR_CHECK(hipMalloc(&B.list2, sizeof(int) * cap));
   FR_CHECK(hipMalloc(&B.key, sizeof(int) * cap));
   FR_CHECK(hipMalloc(&B.slot, sizeof(int) * cap));
   for (int q = 0; q < nops; q++) FR_CHECK(hipMalloc(&B.s[q], sizeof(double) * cap));
@@ -895,7 +893,7 @@ static int alloc_buckets(Buckets& B, long ncell, int cap, int nops) {
 }
 
 static void free_buckets(Buckets& B) {
-  void* all[] = {B.cnt, B.start, B.list, B.key, B.slot};
+  void* all[] = {B.cnt, B.start, B.list, B.list2, B.key, B.slot};
   for (void* p : all) (void)hipFree(p);
   for (double* p : B.s)
     if (p) (void)hipFree(p);
@@ -987,9 +985,9 @@ int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool opera
   if (rc) return rc;
   if (np > 0) {
     hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
-                       B.slot, B.start, rank, B.list);
-    hipLaunchKernelGGL(k_bucket_sort, dim3(nblocks(ncell, 256)), dim3(256), 0, st, ncell,
-                       B.start, B.list, inv, B.slot);
+                       B.slot, B.start, rank, B.list2);
+    hipLaunchKernelGGL(k_bucket_place, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
+                       B.start, rank, B.list2, B.list, B.slot);
     if (operands)
       hipLaunchKernelGGL(k_bucket_gather, dim3(nblocks(np, 256)), dim3(256), 0, st, np, mode,
                          ptrs(), stage, B.slot, d.idx, d.idy, d.idz, B.s[0], B.s[1], B.s[2],
