@@ -46,6 +46,34 @@ def cpu_baseline(params, d, threads):
     return wall
 
 
+def stream_copy_gbs(nbytes=1 << 30, reps=5):
+    """Achievable HBM bandwidth on this device: a plain device-to-device copy
+    of 1 GiB, (read + write bytes) / time, best of `reps` (after the timed
+    region; the library's own HIP runtime, no other work queued)."""
+    import ctypes as C
+    import first_repo_amd as fr
+    fr.lib()
+    h = C.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipFree.argtypes = [C.c_void_p]
+    a, b = C.c_void_p(), C.c_void_p()
+    if h.hipMalloc(C.byref(a), nbytes) or h.hipMalloc(C.byref(b), nbytes):
+        return None
+    best = None
+    h.hipMemcpy(b, a, nbytes, 3)
+    for _ in range(reps):
+        assert h.hipDeviceSynchronize() == 0
+        t = time.perf_counter()
+        assert h.hipMemcpy(b, a, nbytes, 3) == 0
+        assert h.hipDeviceSynchronize() == 0
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    h.hipFree(a)
+    h.hipFree(b)
+    return 2 * nbytes / best / 1e9
+
+
 def calibrate_copy():
     """A plain device-to-device copy of 256 MiB, three times, for scaling the
     FETCH_SIZE / WRITE_SIZE counters to bytes (run it under rocprofv3 --pmc)."""
@@ -189,6 +217,11 @@ def main():
                      "avg_launch_ms": round(per_launch_ms, 4)},
     }
     sim.close()
+    sgbs = stream_copy_gbs()
+    if sgbs:
+        # the same kernel against the bandwidth a plain copy reaches here
+        out["roofline"]["stream_copy_GBs"] = round(sgbs, 1)
+        out["roofline"]["frac_of_stream"] = round(achieved / sgbs, 4)
     if fp32 and not args.no_error:
         # error of the precision form against the FP64 path after the same steps
         print("bench: FP64 reference run for the error of the precision form", file=sys.stderr,

@@ -51,8 +51,12 @@ __device__ inline void fft_lds(C2* buf, int L, int lg, int W, int ps, const C2* 
     __syncthreads();
     const int stepA = L / (2 * half), stepB = L / (4 * half);
     for (int b = threadIdx.x; b < W * q4; b += FT) {
-      const int p = b / q4;
-      const int bb = b - p * q4;
+      // first pair of stages (groups of 4 adjacent elements): pencil index
+      // fastest across lanes, so a wave's 64-byte groups from W pencils
+      // (stride L+1) fall on distinct LDS banks; later stages: position
+      // fastest (consecutive elements)
+      const int p = half == 1 ? b % W : b / q4;
+      const int bb = half == 1 ? b / W : b - p * q4;
       const int grp = bb / half;
       const int pos = bb - grp * half;
       const int i0 = p * ps + grp * 4 * half + pos;
