@@ -255,13 +255,13 @@ __global__ void __launch_bounds__(RB) k_reduce_part(Dev d, const T* __restrict__
                                                     double* __restrict__ part) {
   __shared__ double sh[RB];
   const int ni = d.Nx - 2, nj = d.Ny - 2, nk = d.Nz - 2;
-  const long n = (long)ni * nj * nk;
+  const int rows = nj * nk;
   double acc = init;
-  for (long t = (long)blockIdx.x * RB + threadIdx.x; t < n; t += (long)RG * RB) {
-    const int i = 1 + (int)(t % ni);
-    const long r = t / ni;
-    const int j = 1 + (int)(r % nj), k = 1 + (int)(r / nj);
-    acc = rcomb<OP>(acc, rmap<OP>((double)f[at(d, i, j, k)]));
+  // block b takes interior rows b, b + RG, ...; its threads stride along x
+  for (int row = blockIdx.x; row < rows; row += RG) {
+    const int j = 1 + row % nj, k = 1 + row / nj;
+    const T* fr = f + at(d, 0, j, k);
+    for (int i = 1 + threadIdx.x; i <= ni; i += RB) acc = rcomb<OP>(acc, rmap<OP>((double)fr[i]));
   }
   double r = block_reduce<OP>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;

@@ -33,6 +33,15 @@ namespace fr {
 
 constexpr int DNB = 1024, DNT = 256;  // reduction: chunks, threads per chunk
 
+// (i, j, k) of linear index t = (k*ny + j)*nx + i; 32-bit arithmetic (every
+// grid here has fewer than 2^31 cells), much cheaper than 64-bit division.
+__device__ __forceinline__ void dec3(long t, int nx, int ny, int& i, int& j, int& k) {
+  const unsigned u = (unsigned)t, q = u / (unsigned)nx;
+  i = (int)(u - q * (unsigned)nx);
+  j = (int)(q % (unsigned)ny);
+  k = (int)(q / (unsigned)ny);
+}
+
 struct MGL {
   int nx, ny, nz;
   float *cx, *cy, *cz, *e, *r, *t;
@@ -46,7 +55,8 @@ __global__ void k_mg_coef0(Dev d, int duct, const double* __restrict__ rfx,
                            const double* __restrict__ rfy, const double* __restrict__ rfz, MGL v) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)v.nx * v.ny * v.nz) return;
-  const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny), k = (int)(t / ((long)v.nx * v.ny));
+  int i, j, k;
+  dec3(t, v.nx, v.ny, i, j, k);
   const long c = at(d, i + 1, j + 1, k + 1);
   v.cx[t] = (duct && i == 0) ? 0.f : (float)((1. / rfx[c]) * d.idx2);
   v.cy[t] = (float)((1. / rfy[c]) * d.idy2);
@@ -57,7 +67,8 @@ __global__ void k_mg_coef0(Dev d, int duct, const double* __restrict__ rfx,
 __global__ void k_mg_coarsen(MGL a, MGL c) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)c.nx * c.ny * c.nz) return;
-  const int i = (int)(t % c.nx), j = (int)((t / c.nx) % c.ny), k = (int)(t / ((long)c.nx * c.ny));
+  int i, j, k;
+  dec3(t, c.nx, c.ny, i, j, k);
   const int fi = 2 * i, fj = 2 * j, fk = 2 * k;
   c.cx[t] = (((a.cx[a.at(fi, fj, fk)] + a.cx[a.at(fi, fj + 1, fk)]) + a.cx[a.at(fi, fj, fk + 1)]) +
              a.cx[a.at(fi, fj + 1, fk + 1)]) * 0.0625f;
@@ -92,7 +103,8 @@ __device__ __forceinline__ void mg_nbrs(const MGL& v, int duct, int i, int j, in
 __global__ void k_mg_smooth(MGL v, int duct, int color) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)v.nx * v.ny * v.nz) return;
-  const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny), k = (int)(t / ((long)v.nx * v.ny));
+  int i, j, k;
+  dec3(t, v.nx, v.ny, i, j, k);
   if (((i + j + k) & 1) != color) return;
   float c[6];
   long q[6];
@@ -111,7 +123,8 @@ __global__ void k_mg_smooth(MGL v, int duct, int color) {
 __global__ void k_mg_residual(MGL v, int duct) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)v.nx * v.ny * v.nz) return;
-  const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny), k = (int)(t / ((long)v.nx * v.ny));
+  int i, j, k;
+  dec3(t, v.nx, v.ny, i, j, k);
   float c[6];
   long q[6];
   mg_nbrs(v, duct, i, j, k, c, q);
@@ -126,7 +139,8 @@ __global__ void k_mg_residual(MGL v, int duct) {
 __global__ void k_mg_restrict(MGL f, MGL c) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)c.nx * c.ny * c.nz) return;
-  const int i = (int)(t % c.nx), j = (int)((t / c.nx) % c.ny), k = (int)(t / ((long)c.nx * c.ny));
+  int i, j, k;
+  dec3(t, c.nx, c.ny, i, j, k);
   float a = 0.f;
 #pragma unroll
   for (int dk = 0; dk < 2; dk++)
@@ -142,7 +156,8 @@ __global__ void k_mg_restrict(MGL f, MGL c) {
 __global__ void k_mg_prolong(MGL f, MGL c) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)f.nx * f.ny * f.nz) return;
-  const int i = (int)(t % f.nx), j = (int)((t / f.nx) % f.ny), k = (int)(t / ((long)f.nx * f.ny));
+  int i, j, k;
+  dec3(t, f.nx, f.ny, i, j, k);
   f.e[t] = f.e[t] + c.e[c.at(i / 2, j / 2, k / 2)];
 }
 
@@ -158,8 +173,8 @@ struct MGSet {
 __device__ __forceinline__ void sm_smooth(const MGL& v, int duct, int color) {
   const long n = (long)v.nx * v.ny * v.nz;
   for (long t = threadIdx.x; t < n; t += SMALL) {
-    const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny),
-              k = (int)(t / ((long)v.nx * v.ny));
+    int i, j, k;
+  dec3(t, v.nx, v.ny, i, j, k);
     if (((i + j + k) & 1) != color) continue;
     float c[6];
     long q[6];
@@ -187,8 +202,8 @@ __global__ void __launch_bounds__(SMALL) k_mg_small(MGSet S, int l0, int duct, i
     }
     const long n = (long)v.nx * v.ny * v.nz;
     for (long t = threadIdx.x; t < n; t += SMALL) {
-      const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny),
-                k = (int)(t / ((long)v.nx * v.ny));
+      int i, j, k;
+  dec3(t, v.nx, v.ny, i, j, k);
       float c[6];
       long q[6];
       mg_nbrs(v, duct, i, j, k, c, q);
@@ -202,8 +217,8 @@ __global__ void __launch_bounds__(SMALL) k_mg_small(MGSet S, int l0, int duct, i
     const MGL& c = S.v[l + 1];
     const long nc = (long)c.nx * c.ny * c.nz;
     for (long t = threadIdx.x; t < nc; t += SMALL) {
-      const int i = (int)(t % c.nx), j = (int)((t / c.nx) % c.ny),
-                k = (int)(t / ((long)c.nx * c.ny));
+      int i, j, k;
+  dec3(t, c.nx, c.ny, i, j, k);
       float a = 0.f;
       for (int dk = 0; dk < 2; dk++)
         for (int dj = 0; dj < 2; dj++)
@@ -227,8 +242,8 @@ __global__ void __launch_bounds__(SMALL) k_mg_small(MGSet S, int l0, int duct, i
     const MGL& c = S.v[l + 1];
     const long n = (long)v.nx * v.ny * v.nz;
     for (long t = threadIdx.x; t < n; t += SMALL) {
-      const int i = (int)(t % v.nx), j = (int)((t / v.nx) % v.ny),
-                k = (int)(t / ((long)v.nx * v.ny));
+      int i, j, k;
+  dec3(t, v.nx, v.ny, i, j, k);
       v.e[t] = v.e[t] + c.e[c.at(i / 2, j / 2, k / 2)];
     }
     __syncthreads();
@@ -246,9 +261,10 @@ __global__ void k_fill_f(long n, float* __restrict__ a) {
 
 // ---- fine level, double, ghosted fields --------------------------------------
 __device__ __forceinline__ void tijk(long t, int nx, int ny, int& i, int& j, int& k) {
-  i = (int)(t % nx) + 1;
-  j = (int)((t / nx) % ny) + 1;
-  k = (int)(t / ((long)nx * ny)) + 1;
+  dec3(t, nx, ny, i, j, k);
+  i++;
+  j++;
+  k++;
 }
 
 // S p = -div(C grad p), C = 1./Rho_face, with the operation order of the

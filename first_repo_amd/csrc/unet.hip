@@ -49,6 +49,15 @@ struct U3 {
 
 static U3 make_u3(int nx, int ny, int nz) { return U3{nx, ny, nz, (long)nx * ny * nz}; }
 
+// (x, y, z) of pixel t = (z*ny + y)*nx + x in 32-bit arithmetic (grids here
+// have fewer than 2^31 pixels).
+__device__ __forceinline__ void xyz(long t, int nx, int ny, int& x, int& y, int& z) {
+  const unsigned u = (unsigned)t, q = u / (unsigned)nx;
+  x = (int)(u - q * (unsigned)nx);
+  y = (int)(q % (unsigned)ny);
+  z = (int)(q / (unsigned)ny);
+}
+
 // FP32 -> BF16, round to nearest even (NaN kept quiet).
 __host__ __device__ __forceinline__ u16 f2bf(float f) {
   uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -342,7 +351,8 @@ __global__ void k_pool(U3 dc, int C, const u16* __restrict__ in, int is, int io,
   if (t >= dc.n * C) return;
   const long q = t / C;
   const int c = (int)(t - q * C);
-  const int x = (int)(q % dc.nx), y = (int)((q / dc.nx) % dc.ny), z = (int)(q / ((long)dc.nx * dc.ny));
+  int x, y, z;
+  xyz(q, dc.nx, dc.ny, x, y, z);
   const int fx = 2 * dc.nx, fy = 2 * dc.ny;
   float s = 0.f;
 #pragma unroll
@@ -362,7 +372,8 @@ __global__ void k_pool_bwd(U3 dc, int C, const float* __restrict__ gout, float* 
   const long p = t / C;
   const int c = (int)(t - p * C);
   const int fx = 2 * dc.nx, fy = 2 * dc.ny;
-  const int x = (int)(p % fx), y = (int)((p / fx) % fy), z = (int)(p / ((long)fx * fy));
+  int x, y, z;
+  xyz(p, fx, fy, x, y, z);
   const long q = ((long)(z >> 1) * dc.ny + (y >> 1)) * dc.nx + (x >> 1);
   gin[p * gs + go + c] += gout[q * C + c] * 0.125f;
 }
@@ -378,7 +389,8 @@ __global__ void k_upconv(U3 dc, int CIN, int COUT, const u16* __restrict__ in,
   const long p = t / COUT;
   const int co = (int)(t - p * COUT);
   const int fx = 2 * dc.nx, fy = 2 * dc.ny;
-  const int x = (int)(p % fx), y = (int)((p / fx) % fy), z = (int)(p / ((long)fx * fy));
+  int x, y, z;
+  xyz(p, fx, fy, x, y, z);
   const long q = ((long)(z >> 1) * dc.ny + (y >> 1)) * dc.nx + (x >> 1);
   const int o = ((z & 1) << 2) | ((y & 1) << 1) | (x & 1);
   float s = 0.f;
@@ -394,7 +406,8 @@ __global__ void k_upconv_bwd(U3 dc, int CIN, int COUT, const float* __restrict__
   if (t >= dc.n * CIN) return;
   const long q = t / CIN;
   const int ci = (int)(t - q * CIN);
-  const int x = (int)(q % dc.nx), y = (int)((q / dc.nx) % dc.ny), z = (int)(q / ((long)dc.nx * dc.ny));
+  int x, y, z;
+  xyz(q, dc.nx, dc.ny, x, y, z);
   const int fx = 2 * dc.nx, fy = 2 * dc.ny;
   float s = 0.f;
   for (int o = 0; o < 8; o++) {
@@ -418,8 +431,8 @@ __global__ void k_upconv_dw(U3 dc, int CIN, int COUT, const u16* __restrict__ in
   const int fx = 2 * dc.nx, fy = 2 * dc.ny;
   float s = 0.f;
   for (long q = q0; q < q1; q++) {
-    const int x = (int)(q % dc.nx), y = (int)((q / dc.nx) % dc.ny),
-              z = (int)(q / ((long)dc.nx * dc.ny));
+    int x, y, z;
+  xyz(q, dc.nx, dc.ny, x, y, z);
     const long p =
         ((long)(2 * z + (o >> 2)) * fy + (2 * y + ((o >> 1) & 1))) * fx + 2 * x + (o & 1);
     s += bf2f(in[q * CIN + ci]) * gout[p * gs + go + co];
@@ -521,7 +534,11 @@ __global__ void k_unet_input(Dev d, const double* __restrict__ rhs, const double
   const int nx = d.Nx - 2, ny = d.Ny - 2, nz = d.Nz - 2;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (long)nx * ny * nz) return;
-  const int i = (int)(p % nx) + 1, j = (int)((p / nx) % ny) + 1, k = (int)(p / ((long)nx * ny)) + 1;
+  int i, j, k;
+  xyz(p, nx, ny, i, j, k);
+  i++;
+  j++;
+  k++;
   const long c = at(d, i, j, k);
   const double ir = (1. / rfx[c] + 1. / rfx[c + 1] + 1. / rfy[c] + 1. / rfy[c + d.Nx] +
                      1. / rfz[c] + 1. / rfz[c + d.NxNy]) / 6.;
@@ -539,7 +556,11 @@ __global__ void k_unet_apply(Dev d, const float* __restrict__ y, double s, doubl
   const int nx = d.Nx - 2, ny = d.Ny - 2, nz = d.Nz - 2;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (long)nx * ny * nz) return;
-  const int i = (int)(p % nx) + 1, j = (int)((p / nx) % ny) + 1, k = (int)(p / ((long)nx * ny)) + 1;
+  int i, j, k;
+  xyz(p, nx, ny, i, j, k);
+  i++;
+  j++;
+  k++;
   const long c = at(d, i, j, k);
   const double v = s * (double)y[p];
   P[c] = accumulate ? P[c] + v : v;
@@ -551,7 +572,8 @@ __global__ void k_crop(U3 full, const u16* __restrict__ xs, const u16* __restric
                        float* __restrict__ tg) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= cr.n) return;
-  const int x = (int)(p % cr.nx), y = (int)((p / cr.nx) % cr.ny), z = (int)(p / ((long)cr.nx * cr.ny));
+  int x, y, z;
+  xyz(p, cr.nx, cr.ny, x, y, z);
   const long q = ((long)(z + oz) * full.ny + (y + oy)) * full.nx + (x + ox);
   uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = make_uint4(0u, 0u, 0u, 0u);
   v0.x = xs[q * 3] | ((uint32_t)xs[q * 3 + 1] << 16);
@@ -576,7 +598,11 @@ __global__ void k_store_target(Dev d, const double* __restrict__ Pprev,
   const int nx = d.Nx - 2, ny = d.Ny - 2, nz = d.Nz - 2;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (long)nx * ny * nz) return;
-  const int i = (int)(p % nx) + 1, j = (int)((p / nx) % ny) + 1, k = (int)(p / ((long)nx * ny)) + 1;
+  int i, j, k;
+  xyz(p, nx, ny, i, j, k);
+  i++;
+  j++;
+  k++;
   const long c = at(d, i, j, k);
   ts[p] = f2bf((float)((Pnew[c] - Pprev[c]) * is));
 }
