@@ -3,7 +3,10 @@
 // poisson::CCP_FFT (poisson.cpp:413-462).
 //
 // The residual is real, so the x transform is real-to-complex and only the
-// nx/2+1 non-negative x frequencies are carried through the y and z passes:
+// nx/2+1 non-negative x frequencies are carried through the y and z passes.
+// The spectrum is stored (kz, kx, y)-major, element ((k*nxh + kx)*ny + j):
+// the y passes read and write whole contiguous pencils, the z pass rows of W
+// consecutive y.
 //   pass X : residual (computed inline) -> real FFT along x         (rows)
 //   pass Y : FFT along y                                             (strided)
 //   pass Z : FFT along z, divide by lambda(k), inverse FFT along z
@@ -91,7 +94,7 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
   }
   fft_lds(buf, M, q.lxm, W, ps, twm);
   for (int t = threadIdx.x; t < W * (M + 1); t += FT) {
-    const int p = t / (M + 1), k = t - p * (M + 1);
+    const int k = t / W, p = t - k * W;  // pencil fastest: consecutive y in the spectrum
     const long row = row0 + p;
     if (row >= rows) continue;
     const C2 A = buf[p * ps + (k == M ? 0 : k)];
@@ -104,7 +107,8 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
     C2 o;
     o.x = Er + Tr;
     o.y = Ei + Ti;
-    out[row * (M + 1) + k] = o;
+    const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
+    out[((long)kk * (M + 1) + k) * q.ny + jj] = o;
   }
 }
 
@@ -132,23 +136,33 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
   C2* tws2 = tws + (L / 2 + 1);
   load_tw(tws, tw, L);
   if (divide) load_tw(tws2, tw2, L);
-  const int tiles_x = (nxh + W - 1) / W;
+  // axis 1: W pencils (kx = i0 + p) of plane k = other, each contiguous in y;
+  // axis 2: W pencils (y = i0 + p) of frequency kx = other, stride nxh*ny in z
+  const int span = axis == 1 ? nxh : q.ny;  // range of the tile index
+  const int tiles = (span + W - 1) / W;
   const unsigned bid = xcd_block().x;
-  const int i0 = (bid % tiles_x) * W;
-  const int other = bid / tiles_x;  // k for axis 1, j for axis 2
-  const long stride = axis == 1 ? (long)nxh : (long)nxh * q.ny;
-  const long base = axis == 1 ? (long)other * nxh * q.ny : (long)other * nxh;
-  for (int t = threadIdx.x; t < W * L; t += FT) {
-    const int e = t / W, p = t - e * W;
-    if (i0 + p < nxh) buf[p * ps + bitrev(e, lg)] = data[base + e * stride + i0 + p];
+  const int i0 = (bid % tiles) * W;
+  const int other = bid / tiles;
+  if (axis == 1) {
+    const long base = ((long)other * nxh + i0) * q.ny;
+    for (int t = threadIdx.x; t < W * L; t += FT) {
+      const int p = t / L, e = t - p * L;
+      if (i0 + p < nxh) buf[p * ps + bitrev(e, lg)] = data[base + (long)p * q.ny + e];
+    }
+  } else {
+    const long stride = (long)nxh * q.ny, base = (long)other * q.ny + i0;
+    for (int t = threadIdx.x; t < W * L; t += FT) {
+      const int e = t / W, p = t - e * W;
+      if (i0 + p < q.ny) buf[p * ps + bitrev(e, lg)] = data[base + e * stride + p];
+    }
   }
   fft_lds(buf, L, lg, W, ps, tws);
   if (divide) {
     // lambda = (ax + ay) + az, host tables from the reference formula
     for (int t = threadIdx.x; t < W * L; t += FT) {
       const int e = t / W, p = t - e * W;
-      const int ix = i0 + p, jy = other, kz = e;
-      if (ix < nxh && ix + jy + kz > 0) {
+      const int ix = other, jy = i0 + p, kz = e;  // divide runs on axis 2 only
+      if (jy < q.ny && ix + jy + kz > 0) {
         const R lam = (R)(lamx[ix] + lamy[jy] + lamz[kz]);
         C2 v = buf[p * ps + e];
         v.y /= lam;
@@ -169,9 +183,18 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
     }
     fft_lds(buf, L, lg, W, ps, tws2);
   }
-  for (int t = threadIdx.x; t < W * L; t += FT) {
-    const int e = t / W, p = t - e * W;
-    if (i0 + p < nxh) data[base + e * stride + i0 + p] = buf[p * ps + e];
+  if (axis == 1) {
+    const long base = ((long)other * nxh + i0) * q.ny;
+    for (int t = threadIdx.x; t < W * L; t += FT) {
+      const int p = t / L, e = t - p * L;
+      if (i0 + p < nxh) data[base + (long)p * q.ny + e] = buf[p * ps + e];
+    }
+  } else {
+    const long stride = (long)nxh * q.ny, base = (long)other * q.ny + i0;
+    for (int t = threadIdx.x; t < W * L; t += FT) {
+      const int e = t / W, p = t - e * W;
+      if (i0 + p < q.ny) data[base + e * stride + p] = buf[p * ps + e];
+    }
   }
 }
 
@@ -202,13 +225,15 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q,
   const unsigned bid = blockIdx.x;
   const long row0 = (long)bid * W;
   for (int t = threadIdx.x; t < W * M; t += FT) {
-    const int p = t / M, k = t - p * M;
+    const int k = t / W, p = t - k * W;  // pencil fastest: consecutive y in the spectrum
     const long row = row0 + p;
     C2 z;
     z.x = (R)0;
     z.y = (R)0;
     if (row < rows) {
-      const C2 A = in[row * (M + 1) + k], B = in[row * (M + 1) + (M - k)];
+      const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
+      const long b = (long)kk * (M + 1) * q.ny + jj;
+      const C2 A = in[b + (long)k * q.ny], B = in[b + (long)(M - k) * q.ny];
       const R Sr = A.x + B.x, Si = A.y - B.y;
       const R Dr = A.x - B.x, Di = A.y + B.y;
       const C2 w = twn[k];
@@ -299,10 +324,10 @@ static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const
   else
     hipLaunchKernelGGL((k_pass_x_fwd<false, TC, R>), dim3(nbx), dim3(FT), lds[0], st, q, P, cx,
                        cy, cz, RHS, twfm, twf[0], data);
-  const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.nxh + q.wz - 1) / q.wz;
+  const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.ny + q.wz - 1) / q.wz;
   hipLaunchKernelGGL(k_pass_strided<R>, dim3(ty * q.nz), dim3(FT), lds[1], st, q, 1, 0, twf[1],
                      twf[1], s->lam[0], s->lam[1], s->lam[2], data);
-  hipLaunchKernelGGL(k_pass_strided<R>, dim3(tz * q.ny), dim3(FT), lds[2], st, q, 2, 1, twf[2],
+  hipLaunchKernelGGL(k_pass_strided<R>, dim3(tz * q.nxh), dim3(FT), lds[2], st, q, 2, 1, twf[2],
                      twi[2], s->lam[0], s->lam[1], s->lam[2], data);
   hipLaunchKernelGGL(k_pass_strided<R>, dim3(ty * q.nz), dim3(FT), lds[1], st, q, 1, 0, twi[1],
                      twi[1], s->lam[0], s->lam[1], s->lam[2], data);
@@ -320,7 +345,7 @@ static void plan_passes(PoisDev& q, int csize, int want, size_t lds[3], int* nbx
   const int budget = want / csize - 160 * 16 / csize;  // room for the static reduction array
   q.wx = pencils(M, ny * nz, budget, (M / 2 + 1) + (nx / 2 + 1));
   q.wy = pencils(ny, q.nxh, budget, 2 * (ny / 2 + 1));
-  q.wz = pencils(nz, q.nxh, budget, 2 * (nz / 2 + 1));
+  q.wz = pencils(nz, ny, budget, 2 * (nz / 2 + 1));
   lds[0] = (size_t)csize * (q.wx * (M + 1) + (M / 2 + 1) + (nx / 2 + 1));
   lds[1] = (size_t)csize * (q.wy * (ny + 1) + 2 * (ny / 2 + 1));
   lds[2] = (size_t)csize * (q.wz * (nz + 1) + 2 * (nz / 2 + 1));
