@@ -116,6 +116,12 @@ def main():
     ap.add_argument("--workload", default="box256", choices=["box256", "box512", "duct512", "unet512"],
                     help="box256: the headline 256^3 box; duct512: the 512x128x128 "
                          "inflow/outflow duct (density ratio 3, 2e4 inlet particles/step)")
+    ap.add_argument("--learned", type=int, default=0, choices=[0, 1, 2],
+                    help="duct512 with --solver pcg: 1 = learned initial iterate, 2 = also the "
+                         "learned flexible preconditioner; the network is trained first on "
+                         "pairs from the solver's own run (--pairs, --train-iters)")
+    ap.add_argument("--pairs", type=int, default=400)
+    ap.add_argument("--train-iters", type=int, default=300)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -340,31 +346,81 @@ def bench_duct(args):
     import first_repo_amd as fr
     from first_repo_amd.configs import DuctWorkload
 
+    import ctypes as C
+    if args.learned and args.solver != "pcg":
+        print("bench.py: --learned needs --solver pcg", file=sys.stderr)
+        sys.exit(2)
     w = DuctWorkload(extra={"Solver": 2} if args.solver == "pcg" else {})
     tmp = tempfile.mkdtemp(prefix="frduct_")
     print("bench: building duct 512x128x128", file=sys.stderr, flush=True)
     w.build(tmp)
     params = os.path.join(tmp, "params.in")
-    sim = fr.Sim(params, tmp)
-    sim.reserve_particles(w.capacity)
-    sim.initialize()
-    sim.set_field("Rho", w.density())
-    sim.call("Finish_Initialize")
-    flag0 = 0
-    wall = iters = t_poisson = t_part = 0.0
-    for s in range(args.warmup + args.steps):
-        inlet = w.inlet_particles(s, flag0)
-        flag0 += len(inlet)
-        sim.set_inflow(w.inflow(s), inlet)
+    L = fr.lib()
+
+    def new_sim():
+        sm = fr.Sim(params, tmp)
+        sm.reserve_particles(w.capacity)
+        sm.initialize()
+        sm.set_field("Rho", w.density())
+        sm.call("Finish_Initialize")
+        return sm
+
+    def run(sm, nsteps, timed_from):
+        flag0 = 0
+        acc = [0.0, 0.0, 0.0, 0.0]  # wall, iters, poisson ms, particle ms
+        for s in range(nsteps):
+            inlet = w.inlet_particles(s, flag0)
+            flag0 += len(inlet)
+            sm.set_inflow(w.inflow(s), inlet)
+            t0 = time.perf_counter()
+            sm.step()
+            dt = time.perf_counter() - t0
+            if s >= timed_from:
+                tm = sm.timing()
+                acc[0] += dt
+                acc[1] += tm[3]
+                acc[2] += tm[1]
+                acc[3] += tm[2]
+        return acc
+
+    learned = None
+    if args.learned:
+        # pairs from the solver's own run, then GPU training, then the timed
+        # steps with the network; the plain run of the same steps is timed too
+        nx, ny, nz = w.n
+        data, net = C.c_void_p(), C.c_void_p()
+        fr.check(L.fr_unet_data_create(nx, ny, nz, args.pairs, C.byref(data)))
+        fr.check(L.fr_unet_create(1, C.byref(net)))
+        print(f"bench: collecting {args.pairs} pairs", file=sys.stderr, flush=True)
+        sc = new_sim()
+        fr.check(L.fr_sim_set_collect(sc.h, data))
+        run(sc, (args.pairs + 3) // 4, 10 ** 9)
+        sc.close()
+        cnt = C.c_int()
+        L.fr_unet_data_count(data, C.byref(cnt))
+        print(f"bench: training on {cnt.value} pairs", file=sys.stderr, flush=True)
+        l0, l1 = C.c_double(), C.c_double()
         t0 = time.perf_counter()
-        sim.step()
-        dt = time.perf_counter() - t0
-        if s >= args.warmup:
-            tm = sim.timing()
-            wall += dt
-            t_poisson += tm[1]
-            t_part += tm[2]
-            iters += tm[3]
+        fr.check(L.fr_unet_train(net, data, args.train_iters, 1, nx // 2, ny, nz,
+                                 C.c_float(3e-4), 7, C.byref(l0), C.byref(l1), None))
+        t_train = time.perf_counter() - t0
+        L.fr_unet_data_destroy(data)
+        print("bench: plain PCG run", file=sys.stderr, flush=True)
+        sp = new_sim()
+        plain = run(sp, args.warmup + args.steps, args.warmup)
+        sp.close()
+        learned = {"mode": args.learned, "pairs": cnt.value, "train_iters": args.train_iters,
+                   "train_crop": [nx // 2, ny, nz], "train_s": round(t_train, 2),
+                   "loss_first": l0.value, "loss_last": l1.value,
+                   "plain_poisson_iters_per_substep": round(plain[1] / args.steps / 4, 3),
+                   "plain_poisson_ms_per_step": round(plain[2] / args.steps, 3),
+                   "plain_ms_per_step": round(plain[0] / args.steps * 1e3, 3)}
+    print("bench: timed run", file=sys.stderr, flush=True)
+    sim = new_sim()
+    if args.learned:
+        fr.check(L.fr_sim_set_learned(sim.h, net, args.learned))
+    acc = run(sim, args.warmup + args.steps, args.warmup)
+    wall, iters, t_poisson, t_part = acc
     ncell = w.n[0] * w.n[1] * w.n[2]
     diag = sim.diagnostics()
     out = {
@@ -391,6 +447,12 @@ def bench_duct(args):
                    "particle_ms_per_step": round(t_part / args.steps, 3),
                    "TKE": float(diag[0]), "Rho_Mean": float(diag[1])},
     }
+    if learned:
+        out["config"]["workload"] += (", learned initial iterate" if args.learned == 1 else
+                                      ", learned initial iterate + flexible preconditioner")
+        out["config"]["learned"] = learned
+        out["dtype"] = "f64 (network bf16, fp32 accumulate)"
+        L.fr_unet_destroy(net)
     sim.close()
     print(json.dumps(out), flush=True)
 
