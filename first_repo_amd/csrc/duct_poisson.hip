@@ -172,15 +172,21 @@ __global__ void __launch_bounds__(64) k_duct_thomas(DuctDev q, const double* __r
   const int n = q.nxi;
   // T as doubles: row i, mode m, part t&1 at (i*nm + m)*2 + (t&1) = i*2nm + t
   const long rs = 2 * nm;
-  double dd = 0.0;
-  for (int i0 = 0; i0 < n; i0 += TB) {
-    double vv[TB], cc[TB];
+  // Register blocks of TB rows, the next block's loads issued before the
+  // current block's sequential recurrence (the loads do not depend on it).
+  auto load_f = [&](int i0, double* vv, double* cc) {
 #pragma unroll
     for (int u = 0; u < TB; u++) {
       const int i = i0 + u;
       vv[u] = i < n ? T[(long)i * rs + t] : 0.0;
       cc[u] = (i >= 1 && i < n) ? cp[(long)(i - 1) * nm + m] : 0.0;
     }
+  };
+  double dd = 0.0;
+  double vv[TB], cc[TB], nv[TB], nc[TB];
+  load_f(0, vv, cc);
+  for (int i0 = 0; i0 < n; i0 += TB) {
+    if (i0 + TB < n) load_f(i0 + TB, nv, nc);
 #pragma unroll
     for (int u = 0; u < TB; u++) {
       const int i = i0 + u;
@@ -200,17 +206,25 @@ __global__ void __launch_bounds__(64) k_duct_thomas(DuctDev q, const double* __r
         T[(long)i * rs + t] = dd;
       }
     }
+#pragma unroll
+    for (int u = 0; u < TB; u++) {
+      vv[u] = nv[u];
+      cc[u] = nc[u];
+    }
   }
   // back substitution x_i = d'_i - c'_i x_{i+1}, from x_{n-1} = d'_{n-1}
-  double x = dd;
-  for (int i1 = n - 2; i1 >= 0; i1 -= TB) {
-    double vv[TB], cc[TB];
+  auto load_b = [&](int i1, double* vv, double* cc) {
 #pragma unroll
     for (int u = 0; u < TB; u++) {
       const int i = i1 - u;
       vv[u] = i >= 0 ? T[(long)i * rs + t] : 0.0;
       cc[u] = i >= 0 ? cp[(long)i * nm + m] : 0.0;
     }
+  };
+  double x = dd;
+  load_b(n - 2, vv, cc);
+  for (int i1 = n - 2; i1 >= 0; i1 -= TB) {
+    if (i1 - TB >= 0) load_b(i1 - TB, nv, nc);
 #pragma unroll
     for (int u = 0; u < TB; u++) {
       const int i = i1 - u;
@@ -218,6 +232,11 @@ __global__ void __launch_bounds__(64) k_duct_thomas(DuctDev q, const double* __r
         x = vv[u] - cc[u] * x;
         T[(long)i * rs + t] = x;
       }
+    }
+#pragma unroll
+    for (int u = 0; u < TB; u++) {
+      vv[u] = nv[u];
+      cc[u] = nc[u];
     }
   }
 }

@@ -36,19 +36,21 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
                                const T* __restrict__ uy, const T* __restrict__ uz) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.np) return;
+  // T = float (precision form): the gas-side arithmetic (interpolation
+  // weights, interpolated velocity, drag right-hand side) runs in FP32; the
+  // particle state and its RK4 accumulation stay FP64.  T = double is the
+  // reference arithmetic.
+  using Cm = T;
   const double xi = p.s[0][1][i], yi = p.s[1][1][i], zi = p.s[2][1][i];
   const int ti = (int)(xi * d.idx), tj = (int)(yi * d.idy), tk = (int)(zi * d.idz);
-  const double x_l = (xi * d.idx - ((int)(xi * d.idx)));
-  const double y_l = (yi * d.idy - ((int)(yi * d.idy)));
-  const double z_l = (zi * d.idz - ((int)(zi * d.idz)));
+  const Cm x_l = (Cm)(xi * d.idx - ((int)(xi * d.idx)));
+  const Cm y_l = (Cm)(yi * d.idy - ((int)(yi * d.idy)));
+  const Cm z_l = (Cm)(zi * d.idz - ((int)(zi * d.idz)));
   const int ii = ti + 1, jj = tj + 1, kk = tk + 1;
   const long q = (long)kk * d.NxNy + (long)jj * d.Nx + ii;
-  const double ug = x_l * (double)ux[clampi(q + 1, d.size)] +
-                    (1 - x_l) * (double)ux[clampi(q, d.size)];
-  const double vg = y_l * (double)uy[clampi(q + d.Nx, d.size)] +
-                    (1 - y_l) * (double)uy[clampi(q, d.size)];
-  const double wg = z_l * (double)uz[clampi(q + d.NxNy, d.size)] +
-                    (1 - z_l) * (double)uz[clampi(q, d.size)];
+  const Cm ug = x_l * (Cm)ux[clampi(q + 1, d.size)] + (1 - x_l) * (Cm)ux[clampi(q, d.size)];
+  const Cm vg = y_l * (Cm)uy[clampi(q + d.Nx, d.size)] + (1 - y_l) * (Cm)uy[clampi(q, d.size)];
+  const Cm wg = z_l * (Cm)uz[clampi(q + d.NxNy, d.size)] + (1 - z_l) * (Cm)uz[clampi(q, d.size)];
   // positions
   const double ui = p.s[3][1][i], vi = p.s[4][1][i], wi = p.s[5][1][i];
   {
@@ -61,7 +63,9 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
     }
   }
   // velocities
-  const double ru = (ug - ui) * c.iTp, rv = (vg - vi) * c.iTp, rw = (wg - wi) * c.iTp;
+  const double ru = (double)((ug - (Cm)ui) * (Cm)c.iTp);
+  const double rv = (double)((vg - (Cm)vi) * (Cm)c.iTp);
+  const double rw = (double)((wg - (Cm)wi) * (Cm)c.iTp);
   p.rhs_u[i] = ru;
   p.rhs_v[i] = rv;
   p.rhs_w[i] = rw;
@@ -89,28 +93,32 @@ __global__ void k_part_temp(Dev d, PartPtrs p, int st, double c0, const F* __res
                             double iTth) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.np) return;
+  // F = float (precision form): gas temperature interpolation and the heat
+  // right-hand side in FP32; F = double is the reference arithmetic.
+  using Cm = F;
   const double X = p.s[0][st][i], Y = p.s[1][st][i], Z = p.s[2][st][i];
   int ii = (int)(round(X * d.idx));
   int jj = (int)(round(Y * d.idy));
   int kk = (int)(round(Z * d.idz));
-  const double x_l = X * d.idx - ii + 0.5;
-  const double y_l = Y * d.idy - jj + 0.5;
-  const double z_l = Z * d.idz - kk + 0.5;
+  const Cm x_l = (Cm)(X * d.idx - ii + 0.5);
+  const Cm y_l = (Cm)(Y * d.idy - jj + 0.5);
+  const Cm z_l = (Cm)(Z * d.idz - kk + 0.5);
   ii += 1;
   jj += 1;
   kk += 1;
+  const Cm cc0 = (Cm)c0;
   auto T = [&](int a, int b, int e) {
-    return c0 / (double)rho[clampi((long)e * d.NxNy + (long)b * d.Nx + a, d.size)];
+    return cc0 / (Cm)rho[clampi((long)e * d.NxNy + (long)b * d.Nx + a, d.size)];
   };
-  const double Tg = x_l * y_l * z_l * T(ii, jj, kk) +
-                    (1 - x_l) * y_l * z_l * T(ii - 1, jj, kk) +
-                    x_l * (1 - y_l) * z_l * T(ii, jj - 1, kk) +
-                    x_l * y_l * (1 - z_l) * T(ii, jj, kk - 1) +
-                    (1 - x_l) * (1 - y_l) * z_l * T(ii - 1, jj - 1, kk) +
-                    (1 - x_l) * y_l * (1 - z_l) * T(ii - 1, jj, kk - 1) +
-                    x_l * (y_l - 1) * (z_l - 1) * T(ii, jj - 1, kk - 1) +
-                    (1 - x_l) * (1 - y_l) * (1 - z_l) * T(ii - 1, jj - 1, kk - 1);
-  p.rhs_T[i] = (Tg - p.s[6][st][i]) * iTth;
+  const Cm Tg = x_l * y_l * z_l * T(ii, jj, kk) +
+                (1 - x_l) * y_l * z_l * T(ii - 1, jj, kk) +
+                x_l * (1 - y_l) * z_l * T(ii, jj - 1, kk) +
+                x_l * y_l * (1 - z_l) * T(ii, jj, kk - 1) +
+                (1 - x_l) * (1 - y_l) * z_l * T(ii - 1, jj - 1, kk) +
+                (1 - x_l) * y_l * (1 - z_l) * T(ii - 1, jj, kk - 1) +
+                x_l * (y_l - 1) * (z_l - 1) * T(ii, jj - 1, kk - 1) +
+                (1 - x_l) * (1 - y_l) * (1 - z_l) * T(ii - 1, jj - 1, kk - 1);
+  p.rhs_T[i] = (double)((Tg - (Cm)p.s[6][st][i]) * (Cm)iTth);
 }
 
 // ---- device-wide exclusive scan of int32 ------------------------------------
