@@ -206,7 +206,7 @@ class particle {
   int load_random();  // particle.cpp:919-958
   // particle.cpp:340-399 + 314-338 + 480-496 (gather and the three ODE updates)
   int advance(int k, double pre, double post, tensor1& U);
-  int part2gas_concentration(tensor0& A);       // particle.cpp:526-559
+  int part2gas_concentration(tensor0& A, hipStream_t on = 0);  // particle.cpp:526-559
   int part2gas_velocity(tensor1& A);            // particle.cpp:498-524
   int Send_Recv();                              // particle.cpp:853-858
   // duct rank that is inlet and outlet: Send_Recv(particle& pbox)
@@ -272,6 +272,10 @@ class grid {
   // device timers
   hipEvent_t ev[4] = {};
   hipEvent_t evk[8] = {};  // momentum kernel start/stop per substep
+  // second stream (default priority) for the concentration deposit, which is
+  // independent of the momentum deposit; fork/join events
+  hipStream_t st2 = 0;
+  hipEvent_t evf = nullptr, evj = nullptr;
   double t_poisson = 0, t_particles = 0, t_step = 0, t_momentum = 0;
   int n_momentum = 0;
   int iters_step = 0;

@@ -318,9 +318,10 @@ int particle::advance(int k, double pre, double post, tensor1& U) {
 // The rounded-index buckets were built on the _new positions at the end of
 // the previous substep (or at initialisation), which are this substep's _int
 // positions: particle order and positions are unchanged since.
-int particle::part2gas_concentration(tensor0& A) {
-  if (A.is_f32()) return dev.deposit_cic(size_->dev(), false, A.p<float>(), false, st_);
-  return dev.deposit_cic(size_->dev(), false, A.pointer(), false, st_);
+int particle::part2gas_concentration(tensor0& A, hipStream_t on) {
+  const hipStream_t s = on ? on : st_;
+  if (A.is_f32()) return dev.deposit_cic(size_->dev(), false, A.p<float>(), false, s);
+  return dev.deposit_cic(size_->dev(), false, A.pointer(), false, s);
 }
 
 int particle::part2gas_velocity(tensor1& A) {
@@ -424,6 +425,9 @@ grid::~grid() {
     if (e) hipEventDestroy(e);
   for (auto& e : evk)
     if (e) hipEventDestroy(e);
+  if (evf) hipEventDestroy(evf);
+  if (evj) hipEventDestroy(evj);
+  if (st2) hipStreamDestroy(st2);
   if (st) hipStreamDestroy(st);
 }
 
@@ -435,6 +439,9 @@ int grid::setup(const char* params_path, const char* ic) {
   size = gridsize(param);
   ic_dir = ic ? ic : ".";
   FR_CHECK(hipStreamCreate(&st));
+  FR_CHECK(hipStreamCreate(&st2));
+  FR_CHECK(hipEventCreateWithFlags(&evf, hipEventDisableTiming));
+  FR_CHECK(hipEventCreateWithFlags(&evj, hipEventDisableTiming));
   for (auto& e : ev) FR_CHECK(hipEventCreate(&e));
   for (auto& e : evk) FR_CHECK(hipEventCreate(&e));
   // grid.cpp:16-23
@@ -630,11 +637,19 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   // particles' ranks, so no result depends on it)
   if (RK4_count == 0) FR_TRY(part.dev.resort(size.dev(), st));
   FR_TRY(part.advance(RK4_count, RK4_preCoeff[RK4_count], RK4_postCoeff[RK4_count], U));
-  FR_TRY(part.part2gas_concentration(C));
+  // The concentration deposit (rounded-index buckets, output C) and the
+  // momentum deposit (truncating-index buckets, output RHS_RU) read the same
+  // particles and write disjoint data: both are latency-bound, so they run
+  // side by side on two streams and join before the migration.
+  FR_CHECK(hipEventRecord(evf, st));
+  FR_CHECK(hipStreamWaitEvent(st2, evf, 0));
+  FR_TRY(part.part2gas_concentration(C, st2));
   if (param.TWC())
     FR_TRY(part.part2gas_velocity(RHS_RU));
   else
     RHS_RU.assign(0);
+  FR_CHECK(hipEventRecord(evj, st2));
+  FR_CHECK(hipStreamWaitEvent(st, evj, 0));
   if (param.IsBox()) {
     FR_TRY(part.Send_Recv());
   } else {
