@@ -292,9 +292,23 @@ static int red_ws() {
 // Launch the reduction and leave the result in device memory at *dres.  Sums
 // and extrema are formed in double whatever the field type.
 template <typename T>
+int reduce_async_ws(const Dev& d, const T* f, int op, double* dres, double* g_red_ws,
+                    hipStream_t s);
+
+template <typename T>
 int reduce_async(const Dev& d, const T* f, int op, double* dres, hipStream_t s) {
   int rc = red_ws();
   if (rc) return rc;
+  return reduce_async_ws(d, f, op, dres, g_red_ws, s);
+}
+template int reduce_async<double>(const Dev&, const double*, int, double*, hipStream_t);
+template int reduce_async<float>(const Dev&, const float*, int, double*, hipStream_t);
+
+// The same with a caller-owned partial-sum workspace of RG doubles (for a
+// reduction on a second stream beside the default workspace's users).
+template <typename T>
+int reduce_async_ws(const Dev& d, const T* f, int op, double* dres, double* g_red_ws,
+                    hipStream_t s) {
   const double first = 0.0;
   switch (op) {
     case 0:
@@ -326,8 +340,9 @@ int reduce_async(const Dev& d, const T* f, int op, double* dres, hipStream_t s) 
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
-template int reduce_async<double>(const Dev&, const double*, int, double*, hipStream_t);
-template int reduce_async<float>(const Dev&, const float*, int, double*, hipStream_t);
+template int reduce_async_ws<double>(const Dev&, const double*, int, double*, double*,
+                                     hipStream_t);
+template int reduce_async_ws<float>(const Dev&, const float*, int, double*, double*, hipStream_t);
 
 template <typename T>
 int reduce_sync(const Dev& d, const T* f, int op, double* result, hipStream_t s) {

@@ -16,6 +16,10 @@ template <typename T>
 int reduce_async(const Dev& d, const T* f, int op, double* dres, hipStream_t s);
 template <typename T>
 int reduce_sync(const Dev& d, const T* f, int op, double* result, hipStream_t s);
+// reduce_async with a caller-owned workspace of RED_PARTS doubles
+constexpr int RED_PARTS = 1024;
+template <typename T>
+int reduce_async_ws(const Dev& d, const T* f, int op, double* dres, double* ws, hipStream_t s);
 int reducex(const Dev& d, const double* f, double* out, hipStream_t s);
 template <typename T>
 int elementwise(long n, int op, T* out, double a, const T* A, double b, const T* B, double c,

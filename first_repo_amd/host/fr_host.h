@@ -214,9 +214,9 @@ class particle {
   // particles of 29 values each
   int Send_Recv_duct(const double* inlet, int n);
   // gas2part_Temp_{new,int} + Compute_RHS_Temp_{new,int} (402-478)
-  int temp(int stage, double c0, const tensor0& Rho);
-  int part2gas_Temp(int stage, tensor0& A);     // particle.cpp:586-658
-  int rebuild_cells(int stage);
+  int temp(int stage, double c0, const tensor0& Rho, hipStream_t on = 0);
+  int part2gas_Temp(int stage, tensor0& A, hipStream_t on = 0);  // particle.cpp:586-658
+  int rebuild_cells(int stage, hipStream_t on = 0);
 };
 
 // poisson (poisson.h:19-81), FFT solver branch, and poisson_IO
@@ -275,7 +275,13 @@ class grid {
   // second stream (default priority) for the concentration deposit, which is
   // independent of the momentum deposit; fork/join events
   hipStream_t st2 = 0;
-  hipEvent_t evf = nullptr, evj = nullptr;
+  hipEvent_t evf = nullptr, evj = nullptr, evj2 = nullptr;
+  // the heat branch of Update_Particle (gas temperature at the particles,
+  // buckets, heat deposit, its mean) runs on st2 beside Update_RU_WOP and is
+  // joined by join_side() before anything reads RHS_Part_Temp or qmean
+  bool side_pending = false;
+  double* dq = nullptr;  // reduction workspace + result of that branch
+  int join_side();
   double t_poisson = 0, t_particles = 0, t_step = 0, t_momentum = 0;
   int n_momentum = 0;
   int iters_step = 0;

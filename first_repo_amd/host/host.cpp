@@ -351,20 +351,24 @@ int particle::Send_Recv_duct(const double* inlet, int n) {
   return rc;
 }
 
-int particle::temp(int stage, double c0, const tensor0& Rho) {
+int particle::temp(int stage, double c0, const tensor0& Rho, hipStream_t on) {
   const double iTth = param_->Nu() * PI * param_->Dp() * param_->k();
-  if (Rho.is_f32()) return dev.temp(size_->dev(), stage, c0, Rho.p<float>(), iTth, st_);
-  return dev.temp(size_->dev(), stage, c0, Rho.pointer(), iTth, st_);
+  const hipStream_t s = on ? on : st_;
+  if (Rho.is_f32()) return dev.temp(size_->dev(), stage, c0, Rho.p<float>(), iTth, s);
+  return dev.temp(size_->dev(), stage, c0, Rho.pointer(), iTth, s);
 }
 
 // Requires rebuild_cells(stage) after the RHS_T of that stage was formed.
-int particle::part2gas_Temp(int stage, tensor0& A) {
+int particle::part2gas_Temp(int stage, tensor0& A, hipStream_t on) {
   (void)stage;
-  if (A.is_f32()) return dev.deposit_cic(size_->dev(), true, A.p<float>(), true, st_);
-  return dev.deposit_cic(size_->dev(), true, A.pointer(), true, st_);
+  const hipStream_t s = on ? on : st_;
+  if (A.is_f32()) return dev.deposit_cic(size_->dev(), true, A.p<float>(), true, s);
+  return dev.deposit_cic(size_->dev(), true, A.pointer(), true, s);
 }
 
-int particle::rebuild_cells(int stage) { return dev.build(size_->dev(), stage, 0, st_); }
+int particle::rebuild_cells(int stage, hipStream_t on) {
+  return dev.build(size_->dev(), stage, 0, on ? on : st_);
+}
 
 // ---------------------------------------------------------------- poisson
 int poisson::init(params* p, gridsize* g, hipStream_t st) {
@@ -427,6 +431,8 @@ grid::~grid() {
     if (e) hipEventDestroy(e);
   if (evf) hipEventDestroy(evf);
   if (evj) hipEventDestroy(evj);
+  if (evj2) hipEventDestroy(evj2);
+  if (dq) hipFree(dq);
   if (st2) hipStreamDestroy(st2);
   if (st) hipStreamDestroy(st);
 }
@@ -442,6 +448,7 @@ int grid::setup(const char* params_path, const char* ic) {
   FR_CHECK(hipStreamCreate(&st2));
   FR_CHECK(hipEventCreateWithFlags(&evf, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evj, hipEventDisableTiming));
+  FR_CHECK(hipEventCreateWithFlags(&evj2, hipEventDisableTiming));
   for (auto& e : ev) FR_CHECK(hipEventCreate(&e));
   for (auto& e : evk) FR_CHECK(hipEventCreate(&e));
   // grid.cpp:16-23
@@ -473,6 +480,7 @@ int grid::setup(const char* params_path, const char* ic) {
   FR_TRY(part.init(&param, &size, st, param.N0() > 0 ? param.N0() : 1));
   FR_TRY(PS_.init(&param, &size, st));
   FR_CHECK(hipMalloc(&dred, sizeof(double) * (4 * (size_t)size.Nx() + 512)));
+  FR_CHECK(hipMalloc(&dq, sizeof(double) * (RED_PARTS + 8)));
   return FR_OK;
 }
 
@@ -609,6 +617,7 @@ int grid::Update_Rho() {
 }
 
 int grid::Update_P0() {  // grid.cpp:539-554
+  FR_TRY(join_side());
   if (!qmean_valid) {
     qmean = RHS_Part_Temp.mean();
     qmean_valid = true;
@@ -656,15 +665,37 @@ int grid::Update_Particle() {  // grid.cpp:659-680
     FR_TRY(part.Send_Recv_duct(inlet.data(), (int)(inlet.size() / 29)));
     inlet.clear();
   }
-  FR_TRY(part.temp(2, P0_new / param.R(), Rho_new));
-  FR_TRY(part.rebuild_cells(2));
-  FR_TRY(part.part2gas_Temp(2, RHS_Part_Temp));
-  hipEventRecord(ev[3], st);
-  qmean = RHS_Part_Temp.mean();
+  // Heat branch on st2, beside Update_RU_WOP (which reads none of it):
+  // gas temperature at the particles, buckets, heat deposit, its sum.
+  FR_CHECK(hipEventRecord(evf, st));
+  FR_CHECK(hipStreamWaitEvent(st2, evf, 0));
+  FR_TRY(part.temp(2, P0_new / param.R(), Rho_new, st2));
+  FR_TRY(part.rebuild_cells(2, st2));
+  FR_TRY(part.part2gas_Temp(2, RHS_Part_Temp, st2));
+  FR_CHECK(hipEventRecord(ev[3], st2));
+  if (RHS_Part_Temp.is_f32())
+    FR_TRY(reduce_async_ws(size.dev(), RHS_Part_Temp.p<float>(), 0, dq + RED_PARTS, dq, st2));
+  else
+    FR_TRY(reduce_async_ws(size.dev(), RHS_Part_Temp.pointer(), 0, dq + RED_PARTS, dq, st2));
+  FR_CHECK(hipEventRecord(evj2, st2));
+  side_pending = true;
+  qmean_valid = false;
+  return FR_OK;
+}
+
+// Joins the heat branch into the main stream and takes its mean (the
+// tensor0::mean of RHS_Part_Temp, grid.cpp:677) and the particle time.
+int grid::join_side() {
+  if (!side_pending) return FR_OK;
+  side_pending = false;
+  FR_CHECK(hipStreamWaitEvent(st, evj2, 0));
+  double h = 0;
+  FR_CHECK(hipMemcpyAsync(&h, dq + RED_PARTS, sizeof(double), hipMemcpyDeviceToHost, st));
+  FR_CHECK(hipStreamSynchronize(st));
+  qmean = h / ((double)(size.Nx() - 2) * (size.Ny() - 2) * (size.Nz() - 2));
   qmean_valid = true;
   float ms = 0;
-  hipEventElapsedTime(&ms, ev[2], ev[3]);
-  t_particles += ms;
+  if (hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) t_particles += ms;
   return FR_OK;
 }
 
@@ -734,6 +765,7 @@ static DivCoef div_coef(grid& g) {
 // enters Correct_OutFlow) and Rho_face = I_C2F(Rho_new) of grid.cpp:582 is
 // formed in the same pass (nothing reads Rho_face in between).
 int grid::Compute_Div_U_new() {
+  FR_TRY(join_side());
   if (param.IsBox()) return FR_OK;
   double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
   FR_TRY(duct_div(size.dev(), div_coef(*this), Rho_new.pointer(), RHS_Part_Temp.pointer(), rf,
@@ -763,6 +795,7 @@ int grid::Correct_OutFlow() {
 }
 
 int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
+  FR_TRY(join_side());
   const DivCoef q = div_coef(*this);
   const double* rn[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
   double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
@@ -984,6 +1017,12 @@ int fr_sim_initialize(fr_sim* s) { return s ? s->g.Initialize() : FR_EINVAL; }
 
 int fr_sim_step(fr_sim* s) { return s ? s->g.step() : FR_EINVAL; }
 
+#define FR_TRY_S(x)     \
+  do {                  \
+    int rc_ = (x);      \
+    if (rc_) return rc_; \
+  } while (0)
+
 int fr_sim_call(fr_sim* s, const char* m, int k) {
   if (!s || !m) return FR_EINVAL;
   grid& g = s->g;
@@ -1015,6 +1054,7 @@ int fr_sim_call(fr_sim* s, const char* m, int k) {
     }
   }
   else return FR_EINVAL;
+  if (!rc) rc = g.join_side();
   if (!rc) rc = (int)hipStreamSynchronize(g.st);
   return rc;
 }
@@ -1027,6 +1067,7 @@ static tensor0* pick(fr_sim* s, const char* name, int comp) {
 
 int fr_sim_get_field(fr_sim* s, const char* name, int comp, double* host) {
   if (!s || !name || !host) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   tensor0* t = pick(s, name, comp);
   if (!t) return FR_EINVAL;
   t->download(host);
@@ -1035,6 +1076,7 @@ int fr_sim_get_field(fr_sim* s, const char* name, int comp, double* host) {
 
 int fr_sim_set_field(fr_sim* s, const char* name, int comp, const double* host) {
   if (!s || !name || !host) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   tensor0* t = pick(s, name, comp);
   if (!t) return FR_EINVAL;
   t->upload(host);
@@ -1058,6 +1100,7 @@ static double* scal(fr_sim* s, const char* name) {
 
 int fr_sim_get_scalar(fr_sim* s, const char* name, double* v) {
   if (!s || !name || !v) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   double* p = scal(s, name);
   if (!p) return FR_EINVAL;
   *v = *p;
@@ -1066,6 +1109,7 @@ int fr_sim_get_scalar(fr_sim* s, const char* name, double* v) {
 
 int fr_sim_set_scalar(fr_sim* s, const char* name, double v) {
   if (!s || !name) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   double* p = scal(s, name);
   if (!p) return FR_EINVAL;
   *p = v;
@@ -1080,12 +1124,14 @@ int fr_sim_geom(fr_sim* s, fr_geom* out) {
 
 int fr_sim_np(fr_sim* s, int* np) {
   if (!s || !np) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   *np = s->g.part.dev.np;
   return FR_OK;
 }
 
 int fr_sim_get_particles(fr_sim* s, int q, int st, double* host) {
   if (!s || !host || q < 0 || q > 7 || st < 0 || st > 3) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   PartDev& p = s->g.part.dev;
   const double* src = q == 7 ? p.rhs_T : p.s[q][st];
   return p.get(host, src, s->g.st);
@@ -1093,6 +1139,7 @@ int fr_sim_get_particles(fr_sim* s, int q, int st, double* host) {
 
 int fr_sim_set_particles(fr_sim* s, int np, int q, int st, const double* host) {
   if (!s || !host || q < 0 || q > 7 || st < 0 || st > 3) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   PartDev& p = s->g.part.dev;
   if (np > p.cap || np < 0) return FR_EOVERFLOW;
   if (np != p.np) {  // a new particle set: storage in reference order
@@ -1107,11 +1154,13 @@ int fr_sim_set_particles(fr_sim* s, int np, int q, int st, const double* host) {
 
 int fr_sim_get_flags(fr_sim* s, int* host) {
   if (!s || !host) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   return s->g.part.dev.get_flags(host, s->g.st);
 }
 
 int fr_sim_set_flags(fr_sim* s, int np, const int* host) {
   if (!s || !host) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   PartDev& p = s->g.part.dev;
   if (np > p.cap || np < 0) return FR_EOVERFLOW;
   if (np != p.np) {
@@ -1125,6 +1174,7 @@ int fr_sim_set_flags(fr_sim* s, int np, const int* host) {
 
 int fr_sim_particles_changed(fr_sim* s) {
   if (!s) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   int rc = s->g.part.rebuild_cells(1);
   if (!rc) rc = (int)hipStreamSynchronize(s->g.st);
   return rc;
@@ -1132,6 +1182,7 @@ int fr_sim_particles_changed(fr_sim* s) {
 
 int fr_sim_diagnostics(fr_sim* s, double* out) {
   if (!s || !out) return FR_EINVAL;
+  FR_TRY_S(s->g.join_side());
   return s->g.diagnostics(out);
 }
 
