@@ -23,6 +23,7 @@
 // time step the slots are re-sorted by cell so gathers and bucket builds touch
 // memory in cell order.
 #include <climits>
+#include <cstdint>
 
 #include "fr_internal.hpp"
 
@@ -122,23 +123,60 @@ __global__ void k_part_temp(Dev d, PartPtrs p, int st, double c0, const F* __res
 }
 
 // ---- device-wide exclusive scan of int32 ------------------------------------
-constexpr int SB = 256, SPT = 4, STILE = SB * SPT;
+// A tile is STILE = 4096 ints: SV chunks of SB*4, each read as one int4 per
+// thread (lane-contiguous 16-byte loads), so every load instruction of a
+// wave covers 1 KB of consecutive ints.  Sums within a wave use shuffles;
+// integer sums are exact in any order.
+constexpr int SB = 256, SV = 4, STILE = SB * 4 * SV;
 
-__global__ void k_scan_tiles(const int* __restrict__ in, long n, int* __restrict__ tsum) {
-  __shared__ int sh[SB];
-  const long base = (long)blockIdx.x * STILE;
+__device__ __forceinline__ int4 load_i4(const int* __restrict__ in, long e, long n) {
+  if (e + 4 <= n) return *reinterpret_cast<const int4*>(in + e);
+  int4 v = make_int4(0, 0, 0, 0);
+  if (e < n) v.x = in[e];
+  if (e + 1 < n) v.y = in[e + 1];
+  if (e + 2 < n) v.z = in[e + 2];
+  return v;
+}
+
+__device__ __forceinline__ void store_i4(int* __restrict__ out, long e, long n, int4 v) {
+  if (e + 4 <= n) {
+    *reinterpret_cast<int4*>(out + e) = v;
+    return;
+  }
+  if (e < n) out[e] = v.x;
+  if (e + 1 < n) out[e + 1] = v.y;
+  if (e + 2 < n) out[e + 2] = v.z;
+}
+
+__device__ __forceinline__ int wave_sum(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+__global__ void __launch_bounds__(SB) k_scan_tiles(const int* __restrict__ in, long n,
+                                                   int* __restrict__ tsum) {
+  __shared__ int wt[SB / 64];
+  const long base = (long)blockIdx.x * STILE + threadIdx.x * 4;
   int s = 0;
-  for (int e = 0; e < SPT; e++) {
-    long t = base + threadIdx.x * SPT + e;
-    if (t < n) s += in[t];
+#pragma unroll
+  for (int q = 0; q < SV; q++) {
+    const int4 v = load_i4(in, base + (long)q * SB * 4, n);
+    s += v.x + v.y + v.z + v.w;
   }
-  sh[threadIdx.x] = s;
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) wt[threadIdx.x >> 6] = s;
   __syncthreads();
-  for (int o = SB / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) tsum[blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) tsum[blockIdx.x] = wt[0] + wt[1] + wt[2] + wt[3];
 }
 
 // Exclusive scan of the tile sums in one workgroup of SS threads: each thread
@@ -170,33 +208,47 @@ __global__ void __launch_bounds__(SS) k_scan_sums(int* __restrict__ tsum, int nt
   if (threadIdx.x == SS - 1) *out_total = sh[SS - 1];
 }
 
-__global__ void k_scan_apply(const int* __restrict__ in, long n, const int* __restrict__ tsum,
-                             int* __restrict__ out) {
-  __shared__ int sh[SB];
-  const long base = (long)blockIdx.x * STILE + threadIdx.x * SPT;
-  int v[SPT];
-  int s = 0;
-  for (int e = 0; e < SPT; e++) {
-    v[e] = (base + e < n) ? in[base + e] : 0;
-    s += v[e];
+// Chunk q of the tile precedes chunk q+1; within a chunk thread t's four
+// ints precede thread t+1's.  All SV wave scans are formed before the one
+// barrier that publishes the wave totals.
+__global__ void __launch_bounds__(SB) k_scan_apply(const int* __restrict__ in, long n,
+                                                   const int* __restrict__ tsum,
+                                                   int* __restrict__ out) {
+  __shared__ int wt[SV][SB / 64];
+  const long base = (long)blockIdx.x * STILE + threadIdx.x * 4;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int4 v[SV];
+  int inc[SV];
+#pragma unroll
+  for (int q = 0; q < SV; q++) {
+    v[q] = load_i4(in, base + (long)q * SB * 4, n);
+    inc[q] = wave_incl_scan(v[q].x + v[q].y + v[q].z + v[q].w, lane);
+    if (lane == 63) wt[q][w] = inc[q];
   }
-  sh[threadIdx.x] = s;
   __syncthreads();
-  for (int o = 1; o < SB; o <<= 1) {
-    int a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += a;
-    __syncthreads();
-  }
-  int run = tsum[blockIdx.x] + sh[threadIdx.x] - s;
-  for (int e = 0; e < SPT; e++) {
-    if (base + e < n) out[base + e] = run;
-    run += v[e];
+  int run = tsum[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < SV; q++) {
+    int x = run;
+    for (int u = 0; u < w; u++) x += wt[q][u];
+    x += inc[q] - (v[q].x + v[q].y + v[q].z + v[q].w);
+    int4 o;
+    o.x = x;
+    x += v[q].x;
+    o.y = x;
+    x += v[q].y;
+    o.z = x;
+    x += v[q].z;
+    o.w = x;
+    store_i4(out, base + (long)q * SB * 4, n, o);
+    run += wt[q][0] + wt[q][1] + wt[q][2] + wt[q][3];
   }
 }
 
-// out[0..n) = exclusive scan of in, out[n] = total.  tmp holds n/STILE+1 ints.
+// out[0..n) = exclusive scan of in, out[n] = total.  tmp holds n/STILE+1 ints;
+// in and out are 16-byte aligned (whole allocations).
 int scan_ex(const int* in, long n, int* out, int* tmp, hipStream_t s) {
+  if (((uintptr_t)in | (uintptr_t)out) & 15) return FR_EINVAL;
   const int nt = (int)((n + STILE - 1) / STILE);
   if (nt == 0) {
     hipMemsetAsync(out, 0, sizeof(int), s);
