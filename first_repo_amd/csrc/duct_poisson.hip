@@ -30,9 +30,11 @@ struct DuctDev {
   int nxi, ny, nz;  // interior sizes
   int M, nh;        // ny/2, ny/2+1
   int lym, lz;      // log2(ny/2), log2(nz)
-  int wy, wz;       // i-pencils per workgroup of the y and z passes
+  int wy, wz;       // i-pencils per workgroup of the y and z passes (powers of two)
+  int lwy, lwz;     // their log2
   long nm;          // modes nh*nz
   double tot;       // ny*nz
+  double itot;      // 1/tot, exact (tot is a power of two): x*itot == x/tot
   double a;         // 1/dx^2 (a = c of poisson_IO.cpp:225-229)
 };
 
@@ -55,11 +57,12 @@ __global__ void __launch_bounds__(FT) k_duct_y_fwd(DuctDev q, const double* __re
   double2* twn = twm + (M / 2 + 1);
   load_tw(twm, twM, M);
   load_tw(twn, twN, q.ny);
-  const int tiles = (q.nxi + W - 1) / W;
-  const int i0 = (blockIdx.x % tiles) * W;
+  const int lw = q.lwy;
+  const int tiles = (q.nxi + W - 1) >> lw;
   const int k = blockIdx.x / tiles;
+  const int i0 = ((int)blockIdx.x - k * tiles) << lw;
   for (int t = threadIdx.x; t < W * M; t += FT) {
-    const int p = t % W, m = t / W;
+    const int p = t & (W - 1), m = t >> lw;
     double r0 = 0.0, r1 = 0.0;
     if (i0 + p < q.nxi) {
       const long c = at(d, i0 + p + 1, 2 * m + 1, k + 1);
@@ -70,7 +73,7 @@ __global__ void __launch_bounds__(FT) k_duct_y_fwd(DuctDev q, const double* __re
   }
   fft_lds(buf, M, q.lym, W, ps, twm);
   for (int t = threadIdx.x; t < W * (M + 1); t += FT) {
-    const int p = t % W, jh = t / W;
+    const int p = t & (W - 1), jh = t >> lw;
     if (i0 + p >= q.nxi) continue;
     const double2 Av = buf[p * ps + (jh == M ? 0 : jh)];
     const double2 Bv = buf[p * ps + (jh == 0 ? 0 : M - jh)];
@@ -94,29 +97,30 @@ __global__ void __launch_bounds__(FT) k_duct_z(DuctDev q, const double2* __restr
   double2* buf = sm;
   double2* tws = sm + W * ps;
   load_tw(tws, tw, L);
-  const int tiles = (q.nxi + W - 1) / W;
-  const int i0 = (blockIdx.x % tiles) * W;
+  const int lw = q.lwz, lg = q.lz;
+  const int tiles = (q.nxi + W - 1) >> lw;
   const int jh = blockIdx.x / tiles;
+  const int i0 = ((int)blockIdx.x - jh * tiles) << lw;
   if (FWD) {
     for (int t = threadIdx.x; t < W * L; t += FT) {
-      const int p = t % W, k = t / W;
+      const int p = t & (W - 1), k = t >> lw;
       if (i0 + p < q.nxi) buf[p * ps + bitrev(k, q.lz)] = in[((long)k * q.nh + jh) * q.nxi + i0 + p];
     }
   } else {
     for (int t = threadIdx.x; t < W * L; t += FT) {
-      const int k = t % L, p = t / L;
+      const int k = t & (L - 1), p = t >> lg;
       if (i0 + p < q.nxi) buf[p * ps + bitrev(k, q.lz)] = in[((long)(i0 + p) * q.nh + jh) * L + k];
     }
   }
   fft_lds(buf, L, q.lz, W, ps, tws);
   if (FWD) {
     for (int t = threadIdx.x; t < W * L; t += FT) {
-      const int k = t % L, p = t / L;
+      const int k = t & (L - 1), p = t >> lg;
       if (i0 + p < q.nxi) out[((long)(i0 + p) * q.nh + jh) * L + k] = buf[p * ps + k];
     }
   } else {
     for (int t = threadIdx.x; t < W * L; t += FT) {
-      const int p = t % W, k = t / W;
+      const int p = t & (W - 1), k = t >> lw;
       if (i0 + p < q.nxi) out[((long)k * q.nh + jh) * q.nxi + i0 + p] = buf[p * ps + k];
     }
   }
@@ -259,12 +263,13 @@ __global__ void __launch_bounds__(FT) k_duct_y_inv(DuctDev q, const double2* __r
   load_tw(twm, twM, M);
   load_tw(twn, twN, q.ny);
   __syncthreads();
-  const int tiles = (q.nxi + W - 1) / W;
-  const int i0 = (blockIdx.x % tiles) * W;
+  const int lw = q.lwy;
+  const int tiles = (q.nxi + W - 1) >> lw;
   const int k = blockIdx.x / tiles;
+  const int i0 = ((int)blockIdx.x - k * tiles) << lw;
   const long row = (long)k * q.nh;
   for (int t = threadIdx.x; t < W * M; t += FT) {
-    const int p = t % W, jh = t / W;
+    const int p = t & (W - 1), jh = t >> lw;
     double2 z = make_double2(0.0, 0.0);
     if (i0 + p < q.nxi) {
       const double2 Av = A[(row + jh) * q.nxi + i0 + p];
@@ -280,11 +285,11 @@ __global__ void __launch_bounds__(FT) k_duct_y_inv(DuctDev q, const double2* __r
   fft_lds(buf, M, q.lym, W, ps, twm);
   double mx = 0.0;
   for (int t = threadIdx.x; t < W * M; t += FT) {
-    const int p = t % W, e = t / W;
+    const int p = t & (W - 1), e = t >> lw;
     if (i0 + p >= q.nxi) continue;
     const double2 z = buf[p * ps + e];
-    const double dp0 = z.x / q.tot;
-    const double dp1 = z.y / q.tot;
+    const double dp0 = z.x * q.itot;
+    const double dp1 = z.y * q.itot;
     const long c = at(d, i0 + p + 1, 2 * e + 1, k + 1);
     P[c] = P[c] + dp0;
     P[c + d.Nx] = P[c + d.Nx] + dp1;
@@ -372,6 +377,7 @@ int fr_duct_poisson_create(const fr_geom* g, fr_duct_poisson** out) {
   q.lz = lz;
   q.nm = (long)q.nh * nz;
   q.tot = (double)(ny * nz);
+  q.itot = 1.0 / q.tot;
   q.a = 1. / (g->dx * g->dx);
   const size_t nspec = (size_t)nxi * q.nm;
   FR_CHECK(hipMalloc(&s->A, sizeof(double2) * nspec));
@@ -413,6 +419,8 @@ int fr_duct_poisson_create(const fr_geom* g, fr_duct_poisson** out) {
   const int budget = want / (int)sizeof(double2) - 160;
   q.wy = pencils(q.M, 64, budget, (q.M / 2 + 1) + (ny / 2 + 1));
   q.wz = pencils(nz, 64, budget, nz / 2 + 1);
+  q.lwy = log2i(q.wy);
+  q.lwz = log2i(q.wz);
   s->lds_y = sizeof(double2) * (q.wy * (q.M + 1) + (q.M / 2 + 1) + (ny / 2 + 1));
   s->lds_z = sizeof(double2) * (q.wz * (nz + 1) + (nz / 2 + 1));
   s->nby = ((nxi + q.wy - 1) / q.wy) * nz;

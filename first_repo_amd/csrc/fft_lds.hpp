@@ -13,14 +13,21 @@ namespace fr {
 constexpr int FT = 256;  // threads per FFT workgroup
 
 // Pencils per workgroup for length L within `budget` complex elements after
-// `extra` complex of twiddles: W*(L+1) + extra <= budget, W a power of two,
-// W <= cap.
+// `extra` complex of twiddles: W*(L+1) + extra <= budget, W the largest power
+// of two that fits and does not exceed cap (at least 1).  Index arithmetic
+// in the passes uses shifts by log2(W).
 inline int pencils(int L, int cap, int budget, int extra) {
   int w = (budget - extra) / (L + 1);
-  if (w < 1) w = 1;
+  if (w > cap) w = cap;
   int p2 = 1;
   while (p2 * 2 <= w) p2 *= 2;
-  return p2 < cap ? p2 : cap;
+  return p2;
+}
+
+inline int log2i(int w) {
+  int l = 0;
+  while ((1 << (l + 1)) <= w) l++;
+  return l;
 }
 
 __device__ __forceinline__ int bitrev(int x, int logn) {
@@ -39,33 +46,34 @@ __device__ __forceinline__ void bfly(C2& u, C2& t, const C2 w) {
 }
 
 // In-place radix-2 DIT over W pencils (stride ps complex) already in
-// bit-reversed order.  tw[k] = (cos(2 pi k/L), sign*sin(2 pi k/L)), k <= L/2.
+// bit-reversed order; W and L = 2^lg are powers of two.
+// tw[k] = (cos(2 pi k/L), sign*sin(2 pi k/L)), k <= L/2.
 // Stages are applied in pairs from registers (each thread owns the four
 // elements two consecutive stages combine), with exactly the butterflies and
 // twiddles of the stage-by-stage algorithm, so the result is bit-identical.
+// Stage s (half = 2^s) uses twiddle index pos * 2^(lg-1-s).
 template <typename C2>
 __device__ inline void fft_lds(C2* buf, int L, int lg, int W, int ps, const C2* tw) {
-  int half = 1, s = 0;
-  const int q4 = L >> 2;
-  for (; s + 1 < lg; s += 2, half <<= 2) {
+  const int lw = __ffs(W) - 1;
+  int s = 0;
+  for (; s + 1 < lg; s += 2) {
     __syncthreads();
-    const int stepA = L / (2 * half), stepB = L / (4 * half);
-    for (int b = threadIdx.x; b < W * q4; b += FT) {
+    const int half = 1 << s, lq = lg - 2, sa = lg - 1 - s, sb = lg - 2 - s;
+    for (int b = threadIdx.x; b < (W << lq); b += FT) {
       // first pair of stages (groups of 4 adjacent elements): pencil index
       // fastest across lanes, so a wave's 64-byte groups from W pencils
       // (stride L+1) fall on distinct LDS banks; later stages: position
       // fastest (consecutive elements)
-      const int p = half == 1 ? b % W : b / q4;
-      const int bb = half == 1 ? b / W : b - p * q4;
-      const int grp = bb / half;
-      const int pos = bb - grp * half;
-      const int i0 = p * ps + grp * 4 * half + pos;
+      const int p = s == 0 ? (b & (W - 1)) : (b >> lq);
+      const int bb = s == 0 ? (b >> lw) : (b & ((1 << lq) - 1));
+      const int pos = bb & (half - 1);
+      const int i0 = p * ps + ((bb >> s) << (s + 2)) + pos;
       C2 x0 = buf[i0], x1 = buf[i0 + half], x2 = buf[i0 + 2 * half], x3 = buf[i0 + 3 * half];
-      const C2 wa = tw[pos * stepA];
+      const C2 wa = tw[pos << sa];
       bfly(x0, x1, wa);
       bfly(x2, x3, wa);
-      bfly(x0, x2, tw[pos * stepB]);
-      bfly(x1, x3, tw[(pos + half) * stepB]);
+      bfly(x0, x2, tw[pos << sb]);
+      bfly(x1, x3, tw[(pos + half) << sb]);
       buf[i0] = x0;
       buf[i0 + half] = x1;
       buf[i0 + 2 * half] = x2;
@@ -74,15 +82,14 @@ __device__ inline void fft_lds(C2* buf, int L, int lg, int W, int ps, const C2* 
   }
   if (s < lg) {  // odd number of stages: the last one alone
     __syncthreads();
-    const int step = L / (2 * half), h2 = L >> 1;
-    for (int b = threadIdx.x; b < W * h2; b += FT) {
-      const int p = b / h2;
-      const int bb = b - p * h2;
-      const int grp = bb / half;
-      const int pos = bb - grp * half;
-      const int i0 = p * ps + grp * 2 * half + pos;
+    const int half = 1 << s, lh = lg - 1, sa = lg - 1 - s;
+    for (int b = threadIdx.x; b < (W << lh); b += FT) {
+      const int p = b >> lh;
+      const int bb = b & ((1 << lh) - 1);
+      const int pos = bb & (half - 1);
+      const int i0 = p * ps + ((bb >> s) << (s + 1)) + pos;
       C2 u = buf[i0], t = buf[i0 + half];
-      bfly(u, t, tw[pos * step]);
+      bfly(u, t, tw[pos << sa]);
       buf[i0] = u;
       buf[i0 + half] = t;
     }

@@ -35,8 +35,10 @@ struct PoisDev {
   int nx, ny, nz;      // interior sizes
   int nxh;             // nx/2 + 1 retained x frequencies
   int lxm, ly, lz;     // log2(nx/2), log2(ny), log2(nz)
-  int wx, wy, wz;      // pencils per workgroup for each pass
+  int wx, wy, wz;      // pencils per workgroup for each pass (powers of two)
+  int lwx, lwy, lwz;   // their log2
   double tot;          // nx*ny*nz
+  double itot;         // 1/tot, exact (tot is a power of two): x*itot == x/tot
 };
 
 // Pass X: per row (j,k) the residual, packed as z_m = r_2m + i r_2m+1, an
@@ -74,15 +76,15 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
   C2* twn = twm + (M / 2 + 1);
   load_tw(twm, twM, M);
   load_tw(twn, twN, q.nx);
-  const long rows = (long)q.ny * q.nz;
+  const int rows = q.ny * q.nz, lm = q.lxm, lw = q.lwx;
   const unsigned bid = blockIdx.x;
-  const long row0 = (long)bid * W;
+  const int row0 = (int)bid * W;
   for (int t = threadIdx.x; t < W * M; t += FT) {
-    const int p = t / M, m = t - p * M;
-    const long row = row0 + p;
+    const int p = t >> lm, m = t & (M - 1);
+    const int row = row0 + p;
     double r0 = 0.0, r1 = 0.0;
     if (row < rows) {
-      const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
+      const int jj = row & (q.ny - 1), kk = row >> q.ly;
       const long c = at(d, 2 * m + 1, jj + 1, kk + 1);
       r0 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c);
       r1 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c + 1);
@@ -94,8 +96,8 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
   }
   fft_lds(buf, M, q.lxm, W, ps, twm);
   for (int t = threadIdx.x; t < W * (M + 1); t += FT) {
-    const int k = t / W, p = t - k * W;  // pencil fastest: consecutive y in the spectrum
-    const long row = row0 + p;
+    const int k = t >> lw, p = t & (W - 1);  // pencil fastest: consecutive y in the spectrum
+    const int row = row0 + p;
     if (row >= rows) continue;
     const C2 A = buf[p * ps + (k == M ? 0 : k)];
     const C2 B = buf[p * ps + (k == 0 ? 0 : M - k)];
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
     C2 o;
     o.x = Er + Tr;
     o.y = Ei + Ti;
-    const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
+    const int jj = row & (q.ny - 1), kk = row >> q.ly;
     out[((long)kk * (M + 1) + k) * q.ny + jj] = o;
   }
 }
@@ -129,6 +131,7 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
   const int L = axis == 1 ? q.ny : q.nz;
   const int lg = axis == 1 ? q.ly : q.lz;
   const int W = axis == 1 ? q.wy : q.wz;
+  const int lw = axis == 1 ? q.lwy : q.lwz;
   const int ps = L + 1;
   const int nxh = q.nxh;
   C2* buf = sm;
@@ -139,20 +142,20 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
   // axis 1: W pencils (kx = i0 + p) of plane k = other, each contiguous in y;
   // axis 2: W pencils (y = i0 + p) of frequency kx = other, stride nxh*ny in z
   const int span = axis == 1 ? nxh : q.ny;  // range of the tile index
-  const int tiles = (span + W - 1) / W;
+  const int tiles = (span + W - 1) >> lw;
   const unsigned bid = xcd_block().x;
-  const int i0 = (bid % tiles) * W;
-  const int other = bid / tiles;
+  const int other = (int)(bid / (unsigned)tiles);
+  const int i0 = ((int)bid - other * tiles) << lw;
   if (axis == 1) {
     const long base = ((long)other * nxh + i0) * q.ny;
     for (int t = threadIdx.x; t < W * L; t += FT) {
-      const int p = t / L, e = t - p * L;
+      const int p = t >> lg, e = t & (L - 1);
       if (i0 + p < nxh) buf[p * ps + bitrev(e, lg)] = data[base + (long)p * q.ny + e];
     }
   } else {
     const long stride = (long)nxh * q.ny, base = (long)other * q.ny + i0;
     for (int t = threadIdx.x; t < W * L; t += FT) {
-      const int e = t / W, p = t - e * W;
+      const int e = t >> lw, p = t & (W - 1);
       if (i0 + p < q.ny) buf[p * ps + bitrev(e, lg)] = data[base + e * stride + p];
     }
   }
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
   if (divide) {
     // lambda = (ax + ay) + az, host tables from the reference formula
     for (int t = threadIdx.x; t < W * L; t += FT) {
-      const int e = t / W, p = t - e * W;
+      const int e = t >> lw, p = t & (W - 1);
       const int ix = other, jy = i0 + p, kz = e;  // divide runs on axis 2 only
       if (jy < q.ny && ix + jy + kz > 0) {
         const R lam = (R)(lamx[ix] + lamy[jy] + lamz[kz]);
@@ -173,7 +176,7 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
     __syncthreads();
     // bit-reverse permutation in place for the inverse transform
     for (int t = threadIdx.x; t < W * L; t += FT) {
-      const int p = t / L, e = t - p * L;
+      const int p = t >> lg, e = t & (L - 1);
       const int r = bitrev(e, lg);
       if (e < r) {
         C2 a = buf[p * ps + e];
@@ -186,13 +189,13 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
   if (axis == 1) {
     const long base = ((long)other * nxh + i0) * q.ny;
     for (int t = threadIdx.x; t < W * L; t += FT) {
-      const int p = t / L, e = t - p * L;
+      const int p = t >> lg, e = t & (L - 1);
       if (i0 + p < nxh) data[base + (long)p * q.ny + e] = buf[p * ps + e];
     }
   } else {
     const long stride = (long)nxh * q.ny, base = (long)other * q.ny + i0;
     for (int t = threadIdx.x; t < W * L; t += FT) {
-      const int e = t / W, p = t - e * W;
+      const int e = t >> lw, p = t & (W - 1);
       if (i0 + p < q.ny) data[base + e * stride + p] = buf[p * ps + e];
     }
   }
@@ -221,17 +224,17 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q,
   load_tw(twm, twM, M);
   load_tw(twn, twN, q.nx);
   __syncthreads();
-  const long rows = (long)q.ny * q.nz;
+  const int rows = q.ny * q.nz, lm = q.lxm, lw = q.lwx;
   const unsigned bid = blockIdx.x;
-  const long row0 = (long)bid * W;
+  const int row0 = (int)bid * W;
   for (int t = threadIdx.x; t < W * M; t += FT) {
-    const int k = t / W, p = t - k * W;  // pencil fastest: consecutive y in the spectrum
-    const long row = row0 + p;
+    const int k = t >> lw, p = t & (W - 1);  // pencil fastest: consecutive y in the spectrum
+    const int row = row0 + p;
     C2 z;
     z.x = (R)0;
     z.y = (R)0;
     if (row < rows) {
-      const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
+      const int jj = row & (q.ny - 1), kk = row >> q.ly;
       const long b = (long)kk * (M + 1) * q.ny + jj;
       const C2 A = in[b + (long)k * q.ny], B = in[b + (long)(M - k) * q.ny];
       const R Sr = A.x + B.x, Si = A.y - B.y;
@@ -246,13 +249,13 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q,
   fft_lds(buf, M, q.lxm, W, ps, twm);
   double m = 0.0;
   for (int t = threadIdx.x; t < W * M; t += FT) {
-    const int p = t / M, e = t - p * M;
-    const long row = row0 + p;
+    const int p = t >> lm, e = t & (M - 1);
+    const int row = row0 + p;
     if (row < rows) {
-      const int jj = (int)(row % q.ny), kk = (int)(row / q.ny);
+      const int jj = row & (q.ny - 1), kk = row >> q.ly;
       const C2 z = buf[p * ps + e];
-      const double dp0 = (-1) * (double)z.x / q.tot;
-      const double dp1 = (-1) * (double)z.y / q.tot;
+      const double dp0 = (-1) * (double)z.x * q.itot;
+      const double dp1 = (-1) * (double)z.y * q.itot;
       const long c = at(d, 2 * e + 1, jj + 1, kk + 1);
       P[c] = P[c] + dp0;
       P[c + 1] = P[c + 1] + dp1;
@@ -346,6 +349,9 @@ static void plan_passes(PoisDev& q, int csize, int want, size_t lds[3], int* nbx
   q.wx = pencils(M, ny * nz, budget, (M / 2 + 1) + (nx / 2 + 1));
   q.wy = pencils(ny, q.nxh, budget, 2 * (ny / 2 + 1));
   q.wz = pencils(nz, ny, budget, 2 * (nz / 2 + 1));
+  q.lwx = log2i(q.wx);
+  q.lwy = log2i(q.wy);
+  q.lwz = log2i(q.wz);
   lds[0] = (size_t)csize * (q.wx * (M + 1) + (M / 2 + 1) + (nx / 2 + 1));
   lds[1] = (size_t)csize * (q.wy * (ny + 1) + 2 * (ny / 2 + 1));
   lds[2] = (size_t)csize * (q.wz * (nz + 1) + 2 * (nz / 2 + 1));
@@ -460,6 +466,7 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
   q.ly = lg[1];
   q.lz = lg[2];
   q.tot = (double)((long)n[0] * n[1] * n[2]);
+  q.itot = 1.0 / q.tot;
   const double Ls[3] = {g->Lx, g->Ly, g->Lz};
   const double ds[3] = {g->dx, g->dy, g->dz};
   FR_CHECK(hipMalloc(&s->data, sizeof(double2) * (size_t)q.nxh * n[1] * n[2]));
