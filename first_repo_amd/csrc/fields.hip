@@ -257,11 +257,26 @@ __global__ void __launch_bounds__(RB) k_reduce_part(Dev d, const T* __restrict__
   const int ni = d.Nx - 2, nj = d.Ny - 2, nk = d.Nz - 2;
   const int rows = nj * nk;
   double acc = init;
-  // block b takes interior rows b, b + RG, ...; its threads stride along x
-  for (int row = blockIdx.x; row < rows; row += RG) {
-    const int j = 1 + row % nj, k = 1 + row / nj;
-    const T* fr = f + at(d, 0, j, k);
-    for (int i = 1 + threadIdx.x; i <= ni; i += RB) acc = rcomb<OP>(acc, rmap<OP>((double)fr[i]));
+  // block b takes interior rows b, b + RG, ...; its threads stride along x.
+  // Eight rows are loaded before they are combined (in row order), so eight
+  // loads per thread are in flight instead of one.
+  constexpr int UB = 8;
+  for (int i = 1 + threadIdx.x; i <= ni; i += RB) {
+    for (int r0 = blockIdx.x; r0 < rows; r0 += UB * RG) {
+      double v[UB];
+#pragma unroll
+      for (int u = 0; u < UB; u++) {
+        const int row = r0 + u * RG;
+        v[u] = 0.0;
+        if (row < rows) {
+          const int j = 1 + row % nj, k = 1 + row / nj;
+          v[u] = (double)f[at(d, i, j, k)];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UB; u++)
+        if (r0 + u * RG < rows) acc = rcomb<OP>(acc, rmap<OP>(v[u]));
+    }
   }
   double r = block_reduce<OP>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;

@@ -167,7 +167,7 @@ class PartDev {
   int* hbuf = nullptr;
   double* inj_buf = nullptr;
   int inj_cap = 0;
-  long moved_last = 0;
+  bool flg_zero = false;  // the migration flags flg[0..cap] are all zero
 
   int alloc(int capacity, const Dev& d);
   void release();

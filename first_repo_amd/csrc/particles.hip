@@ -22,6 +22,7 @@
 // which therefore rewrites ranks instead of moving particle data.  Once per
 // time step the slots are re-sorted by cell so gathers and bucket builds touch
 // memory in cell order.
+#include <algorithm>
 #include <climits>
 #include <cstdint>
 
@@ -129,12 +130,29 @@ __global__ void k_part_temp(Dev d, PartPtrs p, int st, double c0, const F* __res
 // integer sums are exact in any order.
 constexpr int SB = 256, SV = 4, STILE = SB * 4 * SV;
 
+// EQ0: scan the indicator (in[e] == 0) instead of the values.
+template <bool EQ0>
 __device__ __forceinline__ int4 load_i4(const int* __restrict__ in, long e, long n) {
-  if (e + 4 <= n) return *reinterpret_cast<const int4*>(in + e);
   int4 v = make_int4(0, 0, 0, 0);
-  if (e < n) v.x = in[e];
-  if (e + 1 < n) v.y = in[e + 1];
-  if (e + 2 < n) v.z = in[e + 2];
+  if (e + 4 <= n) {
+    v = *reinterpret_cast<const int4*>(in + e);
+  } else {
+    if (e < n) v.x = in[e];
+    if (e + 1 < n) v.y = in[e + 1];
+    if (e + 2 < n) v.z = in[e + 2];
+    if (EQ0) {  // past the end counts nothing
+      if (e >= n) v.x = 1;
+      if (e + 1 >= n) v.y = 1;
+      if (e + 2 >= n) v.z = 1;
+      v.w = 1;
+    }
+  }
+  if (EQ0) {
+    v.x = v.x == 0;
+    v.y = v.y == 0;
+    v.z = v.z == 0;
+    v.w = v.w == 0;
+  }
   return v;
 }
 
@@ -163,6 +181,7 @@ __device__ __forceinline__ int wave_incl_scan(int x, int lane) {
   return x;
 }
 
+template <bool EQ0>
 __global__ void __launch_bounds__(SB) k_scan_tiles(const int* __restrict__ in, long n,
                                                    int* __restrict__ tsum) {
   __shared__ int wt[SB / 64];
@@ -170,7 +189,7 @@ __global__ void __launch_bounds__(SB) k_scan_tiles(const int* __restrict__ in, l
   int s = 0;
 #pragma unroll
   for (int q = 0; q < SV; q++) {
-    const int4 v = load_i4(in, base + (long)q * SB * 4, n);
+    const int4 v = load_i4<EQ0>(in, base + (long)q * SB * 4, n);
     s += v.x + v.y + v.z + v.w;
   }
   s = wave_sum(s);
@@ -211,6 +230,7 @@ __global__ void __launch_bounds__(SS) k_scan_sums(int* __restrict__ tsum, int nt
 // Chunk q of the tile precedes chunk q+1; within a chunk thread t's four
 // ints precede thread t+1's.  All SV wave scans are formed before the one
 // barrier that publishes the wave totals.
+template <bool EQ0>
 __global__ void __launch_bounds__(SB) k_scan_apply(const int* __restrict__ in, long n,
                                                    const int* __restrict__ tsum,
                                                    int* __restrict__ out) {
@@ -221,7 +241,7 @@ __global__ void __launch_bounds__(SB) k_scan_apply(const int* __restrict__ in, l
   int inc[SV];
 #pragma unroll
   for (int q = 0; q < SV; q++) {
-    v[q] = load_i4(in, base + (long)q * SB * 4, n);
+    v[q] = load_i4<EQ0>(in, base + (long)q * SB * 4, n);
     inc[q] = wave_incl_scan(v[q].x + v[q].y + v[q].z + v[q].w, lane);
     if (lane == 63) wt[q][w] = inc[q];
   }
@@ -245,20 +265,56 @@ __global__ void __launch_bounds__(SB) k_scan_apply(const int* __restrict__ in, l
   }
 }
 
-// out[0..n) = exclusive scan of in, out[n] = total.  tmp holds n/STILE+1 ints;
-// in and out are 16-byte aligned (whole allocations).
-int scan_ex(const int* in, long n, int* out, int* tmp, hipStream_t s) {
+// out[0..n) = exclusive scan of in (EQ0: of in[e] == 0), out[n] = total.
+// tmp holds n/STILE+1 ints; in and out are 16-byte aligned (whole
+// allocations).
+template <bool EQ0>
+int scan_ex_t(const int* in, long n, int* out, int* tmp, hipStream_t s) {
   if (((uintptr_t)in | (uintptr_t)out) & 15) return FR_EINVAL;
   const int nt = (int)((n + STILE - 1) / STILE);
   if (nt == 0) {
     hipMemsetAsync(out, 0, sizeof(int), s);
     return FR_OK;
   }
-  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SB), 0, s, in, n, tmp);
+  hipLaunchKernelGGL(k_scan_tiles<EQ0>, dim3(nt), dim3(SB), 0, s, in, n, tmp);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SS), 0, s, tmp, nt, out + n);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(SB), 0, s, in, n, tmp, out);
+  hipLaunchKernelGGL(k_scan_apply<EQ0>, dim3(nt), dim3(SB), 0, s, in, n, tmp, out);
   FR_LAUNCH_CHECK();
   return FR_OK;
+}
+
+int scan_ex(const int* in, long n, int* out, int* tmp, hipStream_t s) {
+  return scan_ex_t<false>(in, n, out, tmp, s);
+}
+
+// Exclusive scan of in[0..n), n = np - *pK read on the device, in one
+// workgroup (each thread one contiguous chunk); out[n] = total.  For the
+// short per-axis leaver lists, whose length the host does not wait for.
+__global__ void __launch_bounds__(SS) k_scan_small(const int* __restrict__ in, int np,
+                                                   const int* __restrict__ pK,
+                                                   int* __restrict__ out) {
+  __shared__ int sh[SS];
+  const int n = np - *pK;
+  const int chunk = (n + SS - 1) / SS;
+  const int b = threadIdx.x * chunk;
+  const int e = b + chunk < n ? b + chunk : n;
+  int run = 0;
+  for (int t = b; t < e; t++) run += in[t];
+  sh[threadIdx.x] = run;
+  __syncthreads();
+  for (int o = 1; o < SS; o <<= 1) {
+    const int a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += a;
+    __syncthreads();
+  }
+  int x = sh[threadIdx.x] - run;
+  for (int t = b; t < e; t++) {
+    const int v = in[t];
+    out[t] = x;
+    x += v;
+  }
+  if (threadIdx.x == SS - 1) out[n > 0 ? n : 0] = sh[SS - 1];
 }
 
 // ---- cell buckets ---------------------------------------------------------------
@@ -354,18 +410,21 @@ __global__ void k_bucket_gather(int np, int mode, PartPtrs p, int st,
 // LDS once per workgroup.
 constexpr int DTX = 64, DTY = 4, DSR = DTX + 2, DROWS = (DTY + 1) * 2;
 
-// Stage start[] of rows (b0 + oy + r%(DTY+1), c0 + r/(DTY+1)), columns a0 + ox
-// + j, j < DSR; entries outside the grid are never read as a bucket.
+// Stage start[] of rows (b0 + r%(DTY+1), c0 + r/(DTY+1)), columns a0 + j,
+// j < DSR.  Columns are clamped to [0, Nx] (column Nx is the row's end, the
+// start of the next row) and rows outside the grid are all zero, so every
+// staged cell outside the grid is an empty bucket: ss[r][j+1] - ss[r][j] is
+// the particle count of source cell j with no bounds tests.  Cell indices fit
+// in 32 bits (at most 1026^3 ghosted cells).
 __device__ __forceinline__ void stage_starts(const Dev& d, const int* __restrict__ start,
                                              int a0, int b0, int c0, int (*ss)[DSR]) {
   const int tid = threadIdx.y * DTX + threadIdx.x;
   for (int t = tid; t < DROWS * DSR; t += DTX * DTY) {
     const int r = t / DSR, j = t - r * DSR;
-    const int B = b0 + r % (DTY + 1), Cz = c0 + r / (DTY + 1), A = a0 + j;
-    const long cell = (long)Cz * d.NxNy + (long)B * d.Nx + A;
+    const int B = b0 + r % (DTY + 1), Cz = c0 + r / (DTY + 1);
+    const int A = min(max(a0 + j, 0), d.Nx);
     int v = 0;
-    if (B >= 0 && B < d.Ny && Cz >= 0 && Cz < d.Nz && cell >= 0)
-      v = start[cell < d.size ? cell : d.size];
+    if (B >= 0 && B < d.Ny && Cz >= 0 && Cz < d.Nz) v = start[(Cz * d.Ny + B) * d.Nx + A];
     ss[r][j] = v;
   }
   __syncthreads();
@@ -459,21 +518,16 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
   const int tid = threadIdx.y * DTX + threadIdx.x;
   const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
   const bool live = a < d.Nx && b < d.Ny;
-  int cb[8], ce[8], nt = 0;
+  // contributions to this target: buckets (a..a+1, b..b+1, c..c+1); the two
+  // x neighbours of a staged row telescope (cells outside the grid are empty,
+  // so a target outside it counts zero)
+  int nt = 0;
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int ox = q & 1, oy = (q >> 1) & 1, oz = q >> 2;
-    if (live && a + ox < d.Nx && b + oy < d.Ny && c + oz < d.Nz) {
-      const int r = threadIdx.y + oy + (DTY + 1) * oz, j = threadIdx.x + ox;
-      cb[q] = ss[r][j];
-      ce[q] = ss[r][j + 1];
-    } else {
-      cb[q] = ce[q] = 0;
-    }
-    nt += ce[q] - cb[q];
+  for (int q = 0; q < 4; q++) {
+    const int r = threadIdx.y + (q & 1) + (DTY + 1) * (q >> 1);
+    nt += ss[r][threadIdx.x + 2] - ss[r][threadIdx.x];
   }
-  const int jmax = min(DTX + 1, d.Nx - a0);  // cells of a source row inside the grid
-  if (tid < DROWS) rcnt[tid] = ss[tid][jmax] - ss[tid][0];
+  if (tid < DROWS) rcnt[tid] = ss[tid][DSR - 1] - ss[tid][0];  // source cells a0 .. a0+DTX
   int total;
   const int off = block_excl_scan(nt, wtot, &total);  // its barrier also publishes rcnt
   if (total <= PCAP) {
@@ -494,12 +548,13 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
       for (int k = 0; k < DROWS; k++)
         if (k == r) base = rpre[k];
       const int u = ss[r][0] + (f - base);
-      int lo = 0, hi = jmax - 1;  // source cell: last bucket starting at or before u
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (ss[r][mid] <= u) lo = mid;
-        else hi = mid - 1;
-      }
+      // source cell: the last of columns 0..DTX whose bucket starts at or
+      // before u (empty buckets before it share its start, later ones start
+      // after u); a fixed 7-step search over the 65 columns
+      int lo = 0;
+#pragma unroll
+      for (int w = 64; w > 0; w >>= 1)
+        if (lo + w <= DTX && ss[r][lo + w] <= u) lo += w;
       const int sa = a0 + lo, sb = b0 + r % (DTY + 1), oz = r / (DTY + 1);
       const double x_l = sxl[u], y_l = syl[u], z_l = szl[u];
       const int rank = list[u];
@@ -546,8 +601,9 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
   Merge<8> m;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    m.cur[q] = cb[q];
-    m.end[q] = ce[q];
+    const int r = threadIdx.y + ((q >> 1) & 1) + (DTY + 1) * (q >> 2), j = threadIdx.x + (q & 1);
+    m.cur[q] = ss[r][j];
+    m.end[q] = ss[r][j + 1];
   }
   m.init(list);
   double acc = 0.0;
@@ -628,17 +684,19 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ 
   const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
   const bool live = a < d.Nx && b < d.Ny;
   // per-component contribution counts of this target (buckets t and t - e_M)
+  // (staged cells outside the grid are empty buckets, so the neighbours
+  // below a = 0, b = 0, c = 0 count zero)
   int nm[3] = {0, 0, 0};
   if (live) {
     const int r0 = threadIdx.y + 1 + (DTY + 1), j0 = threadIdx.x + 1;
     const int own = ss[r0][j0 + 1] - ss[r0][j0];
-    nm[0] = own + (a > 0 ? ss[r0][j0] - ss[r0][j0 - 1] : 0);
-    nm[1] = own + (b > 0 ? ss[r0 - 1][j0 + 1] - ss[r0 - 1][j0] : 0);
-    nm[2] = own + (c > 0 ? ss[r0 - (DTY + 1)][j0 + 1] - ss[r0 - (DTY + 1)][j0] : 0);
+    nm[0] = own + (ss[r0][j0] - ss[r0][j0 - 1]);
+    nm[1] = own + (ss[r0 - 1][j0 + 1] - ss[r0 - 1][j0]);
+    nm[2] = own + (ss[r0 - (DTY + 1)][j0 + 1] - ss[r0 - (DTY + 1)][j0]);
   }
-  // source cells a0-1 .. a0+63 of each row inside the grid: j in [jlo, jhi)
-  const int jlo = a0 == 0 ? 1 : 0, jhi = min(DTX + 1, d.Nx - a0 + 1);
-  if (tid < DROWS) rcnt[tid] = ss[tid][jhi] - ss[tid][jlo];
+  // source cells a0-1 .. a0+63 of each row (j = 0 .. DTX); row 0 (plane c-1,
+  // row b0-1) reaches no target of the tile and is skipped
+  if (tid < DROWS) rcnt[tid] = tid == 0 ? 0 : ss[tid][DSR - 1] - ss[tid][0];
   int total;
   const int off = block_excl_scan(nm[0] + nm[1] + nm[2], wtot, &total);
   if (total <= VCAP) {
@@ -664,17 +722,19 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ 
 #pragma unroll
       for (int k = 0; k < DROWS; k++)
         if (k == r) base = rpre[k];
-      const int u = ss[r][jlo] + (f - base);
-      int lo = jlo, hi = jhi - 1;  // source cell: last bucket starting at or before u
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (ss[r][mid] <= u) lo = mid;
-        else hi = mid - 1;
-      }
+      const int u = ss[r][0] + (f - base);
+      int lo = 0;  // source cell: last of columns 0..DTX starting at or before u
+#pragma unroll
+      for (int w = 64; w > 0; w >>= 1)
+        if (lo + w <= DTX && ss[r][lo + w] <= u) lo += w;
       const int sa = lo - 1, sb = r % (DTY + 1) - 1, sc = r / (DTY + 1) - 1;  // tile-relative
       const int rank = list[u];
+      // components with a target in the tile: x and y need sc = 0, x and z
+      // need sb >= 0
+      const bool need[3] = {sc == 0 && sb >= 0, sc == 0, sb >= 0};
 #pragma unroll
       for (int M = 0; M < 3; M++) {
+        if (!need[M]) continue;
         const double l = L[M][u], Rv = R[M][u];
 #pragma unroll
         for (int up = 0; up < 2; up++) {  // up = 0: target s, weight (1-l) R; 1: s + e_M, l R
@@ -730,12 +790,7 @@ __global__ void k_mig_flag(int np, const double* __restrict__ P, const int* __re
   if (i >= np) return;
   const double v = P[i];
   const int f = v > hi ? 2 : (v < lo ? 1 : 0);
-  if (f) flg[rank[i]] = f;
-}
-
-__global__ void k_mig_nl(int np, const int* __restrict__ flg, int* __restrict__ nl) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < np) nl[r] = flg[r] == 0;
+  if (f) flg[rank[i]] = f;  // flg is all zero on entry (k_mig_event clears it)
 }
 
 // Front holes (leavers at i < K) get rank m = i - pNL[i]; tail non-leavers
@@ -753,7 +808,7 @@ __global__ void k_mig_rank(int np, const int* __restrict__ flg, const int* __res
 // of rank m (q_0 = np, q_m = position of the tail non-leaver of rank m-1),
 // x = q_h for a leaver sitting at K (h = number of front holes), x = i for
 // every other tail leaver.
-__global__ void k_mig_event(int np, const int* __restrict__ flg, const int* __restrict__ pNL,
+__global__ void k_mig_event(int np, int* __restrict__ flg, const int* __restrict__ pNL,
                             const int* __restrict__ tailnl, int* __restrict__ evt,
                             int* __restrict__ islo) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -772,6 +827,7 @@ __global__ void k_mig_event(int np, const int* __restrict__ flg, const int* __re
   const int e = np - x;
   evt[e] = i;
   islo[e] = flg[i] == 1;
+  flg[i] = 0;  // leaves flg all zero for the next axis
 }
 
 struct MigAxis {
@@ -784,63 +840,65 @@ struct MigAxis {
 // among the hi leavers) -- the append order lo then hi.  Records the storage
 // slot (read from inv before any rank changes), the new reference index and
 // whether the particle is a leaver.
-__global__ void k_mig_moves(int np, int nleave, const int* __restrict__ pNL,
+__global__ void k_mig_moves(int np, const int* __restrict__ pNL,
                             const int* __restrict__ holepos, const int* __restrict__ tailnl,
                             const int* __restrict__ evt, const int* __restrict__ plo,
                             const int* __restrict__ inv, int* __restrict__ mslot,
                             int* __restrict__ mto, int* __restrict__ mlv) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int K = pNL[np];
+  const int K = pNL[np], nleave = np - K;
   const int h = K - pNL[K < np ? K : np];
-  if (s >= h + nleave) return;
-  int src, to, lv;
-  if (s < h) {
-    src = tailnl[s];
-    to = holepos[s];
-    lv = 0;
-  } else {
-    const int e = s - h;
-    src = evt[e];
-    const int nlo = plo[nleave];
-    const int r = plo[e];
-    const bool lo = (plo[e + 1] - r) != 0;
-    to = K + (lo ? r : nlo + (e - r));
-    lv = 1;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < h + nleave;
+       s += gridDim.x * blockDim.x) {
+    int src, to, lv;
+    if (s < h) {
+      src = tailnl[s];
+      to = holepos[s];
+      lv = 0;
+    } else {
+      const int e = s - h;
+      src = evt[e];
+      const int nlo = plo[nleave];
+      const int r = plo[e];
+      const bool lo = (plo[e + 1] - r) != 0;
+      to = K + (lo ? r : nlo + (e - r));
+      lv = 1;
+    }
+    mslot[s] = inv[src];
+    mto[s] = to;
+    mlv[s] = lv;
   }
-  mslot[s] = inv[src];
-  mto[s] = to;
-  mlv[s] = lv;
 }
 
 // New reference indices of the moved slots; leavers are wrapped by MODE_L*
 // (particle.cpp:58-104): _new, base and _np1 of this axis shift by -+L when
 // _new > L or < 0.
-__global__ void k_mig_apply(int np, int nleave, PartPtrs p, MigAxis ax,
-                            const int* __restrict__ pNL, const int* __restrict__ mslot,
-                            const int* __restrict__ mto, const int* __restrict__ mlv,
-                            int* __restrict__ rank, int* __restrict__ inv) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int K = pNL[np];
+__global__ void k_mig_apply(int np, PartPtrs p, MigAxis ax, const int* __restrict__ pNL,
+                            const int* __restrict__ mslot, const int* __restrict__ mto,
+                            const int* __restrict__ mlv, int* __restrict__ rank,
+                            int* __restrict__ inv) {
+  const int K = pNL[np], nleave = np - K;
   const int h = K - pNL[K < np ? K : np];
-  if (s >= h + nleave) return;
-  const int slot = mslot[s], to = mto[s];
-  rank[slot] = to;
-  inv[to] = slot;
-  if (mlv[s]) {
-    const int a = ax.axis;
-    double vn = p.s[a][2][slot], v0 = p.s[a][0][slot], v1 = p.s[a][3][slot];
-    if (vn > ax.L) {
-      vn -= ax.L;
-      v0 -= ax.L;
-      v1 -= ax.L;
-    } else if (vn < 0) {
-      vn += ax.L;
-      v0 += ax.L;
-      v1 += ax.L;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < h + nleave;
+       s += gridDim.x * blockDim.x) {
+    const int slot = mslot[s], to = mto[s];
+    rank[slot] = to;
+    inv[to] = slot;
+    if (mlv[s]) {
+      const int a = ax.axis;
+      double vn = p.s[a][2][slot], v0 = p.s[a][0][slot], v1 = p.s[a][3][slot];
+      if (vn > ax.L) {
+        vn -= ax.L;
+        v0 -= ax.L;
+        v1 -= ax.L;
+      } else if (vn < 0) {
+        vn += ax.L;
+        v0 += ax.L;
+        v1 += ax.L;
+      }
+      p.s[a][2][slot] = vn;
+      p.s[a][0][slot] = v0;
+      p.s[a][3][slot] = v1;
     }
-    p.s[a][2][slot] = vn;
-    p.s[a][0][slot] = v0;
-    p.s[a][3][slot] = v1;
   }
 }
 
@@ -917,23 +975,29 @@ __global__ void k_from_ref(int n, const int* __restrict__ rank, const T* __restr
   if (i < n) dst[i] = src[rank[i]];
 }
 
-// New slot u takes the particle of reference index list[u] (cell order).
-__global__ void k_resort(int np, const int* __restrict__ list, const int* __restrict__ inv,
-                         PartPtrs a, const int* __restrict__ ra, PartPtrs b,
-                         int* __restrict__ rb) {
+// New slot u takes the particle of reference index r = list[u] (cell
+// order) from slot inv[r]; inv[r] becomes u (list is a permutation, so each
+// thread reads and writes only its own inv entry).
+// Runs only at the top of the first RK stage's particle update, right before
+// the advance.
+__global__ void k_resort(int np, const int* __restrict__ list, int* __restrict__ inv,
+                         PartPtrs a, PartPtrs b, int* __restrict__ rb) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= np) return;
-  const int o = inv[list[u]];
+  const int r = list[u];
+  const int o = inv[r];
+  // Stage slot 2 and rhs_u,v,w are written by the advance that follows
+  // before anything reads them; slots 0, 1, 3 and rhs_T carry over.
 #pragma unroll
-  for (int q = 0; q < 7; q++)
-#pragma unroll
-    for (int t = 0; t < 4; t++) b.s[q][t][u] = a.s[q][t][o];
-  b.rhs_u[u] = a.rhs_u[o];
-  b.rhs_v[u] = a.rhs_v[o];
-  b.rhs_w[u] = a.rhs_w[o];
+  for (int q = 0; q < 7; q++) {
+    b.s[q][0][u] = a.s[q][0][o];
+    b.s[q][1][u] = a.s[q][1][o];
+    b.s[q][3][u] = a.s[q][3][o];
+  }
   b.rhs_T[u] = a.rhs_T[o];
   b.flag[u] = a.flag[o];
-  rb[u] = ra[o];
+  rb[u] = r;
+  inv[r] = u;
 }
 
 }  // namespace fr
@@ -960,6 +1024,7 @@ static void free_buckets(Buckets& B) {
 }
 
 int PartDev::alloc(int capacity, const Dev& d) {
+  flg_zero = false;
   cap = capacity < 1 ? 1 : capacity;
   ncell = d.size;
   for (int q = 0; q < 7; q++)
@@ -1090,33 +1155,33 @@ int PartDev::deposit_vel(const Dev& d, F* const out[3], hipStream_t st) {
 // the moved particles change, and the leavers' coordinates are wrapped.
 int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st,
                      bool duct, const double* inj, int n_in) {
+  // The leaver count K of each axis stays on the device: the kernels read it
+  // from pNL[np] and loop over the moves, so the host does not wait for it
+  // (the duct's outflow, which shrinks the set, is the exception).
+  if (!flg_zero) FR_CHECK(hipMemsetAsync(flg, 0, sizeof(int) * (cap + 1), st));
+  flg_zero = false;  // until every axis below has cleared its flags
   for (int ax = 2; ax >= 0 && np > 0; ax--) {
     const unsigned nb = nblocks(np, 256);
-    FR_CHECK(hipMemsetAsync(flg, 0, sizeof(int) * np, st));
     hipLaunchKernelGGL(k_mig_flag, dim3(nb), dim3(256), 0, st, np, (const double*)s[ax][2], rank,
                        lo[ax], hi[ax], flg);
-    hipLaunchKernelGGL(k_mig_nl, dim3(nb), dim3(256), 0, st, np, flg, nl);
-    int rc = scan_ex(nl, np, pNL, scan_tmp, st);
+    int rc = scan_ex_t<true>(flg, np, pNL, scan_tmp, st);  // pNL: stayers before each index
     if (rc) return rc;
-    FR_CHECK(hipMemcpyAsync(hbuf, pNL + np, sizeof(int), hipMemcpyDeviceToHost, st));
-    FR_CHECK(hipStreamSynchronize(st));
-    const int K = hbuf[0];
-    const int nleave = np - K;
-    moved_last += nleave;
-    if (nleave == 0) continue;
     hipLaunchKernelGGL(k_mig_rank, dim3(nb), dim3(256), 0, st, np, flg, pNL, holepos, tailnl);
     hipLaunchKernelGGL(k_mig_event, dim3(nb), dim3(256), 0, st, np, flg, pNL, tailnl, evt, islo);
-    rc = scan_ex(islo, nleave, plo, scan_tmp, st);
-    if (rc) return rc;
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(SS), 0, st, islo, np, pNL + np, plo);
     MigAxis m{ax, L[ax]};
-    // at most 2*nleave particles move (h <= nleave front holes + nleave leavers)
-    const unsigned ng = nblocks(2L * nleave, 256);
-    hipLaunchKernelGGL(k_mig_moves, dim3(ng), dim3(256), 0, st, np, nleave, pNL, holepos, tailnl,
-                       evt, plo, inv, mslot, dst, mlv);
-    hipLaunchKernelGGL(k_mig_apply, dim3(ng), dim3(256), 0, st, np, nleave, ptrs(), m, pNL,
-                       mslot, dst, mlv, rank, inv);
+    // h + nleave <= np moves (h <= K front holes, nleave = np - K leavers)
+    const unsigned ng = std::min(nblocks(np, 256), 1024u);
+    hipLaunchKernelGGL(k_mig_moves, dim3(ng), dim3(256), 0, st, np, pNL, holepos, tailnl, evt,
+                       plo, inv, mslot, dst, mlv);
+    hipLaunchKernelGGL(k_mig_apply, dim3(ng), dim3(256), 0, st, np, ptrs(), m, pNL, mslot, dst,
+                       mlv, rank, inv);
     FR_LAUNCH_CHECK();
     if (duct && ax == 0) {
+      FR_CHECK(hipMemcpyAsync(hbuf, pNL + np, sizeof(int), hipMemcpyDeviceToHost, st));
+      FR_CHECK(hipStreamSynchronize(st));
+      const int K = hbuf[0];
+      if (K == np) continue;
       // the leavers now hold reference indices K..np-1 and are not received
       // back: keep the slots with rank < K, in storage order
       hipLaunchKernelGGL(k_live, dim3(nb), dim3(256), 0, st, np, rank, K, nl);
@@ -1133,6 +1198,7 @@ int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], 
       FR_LAUNCH_CHECK();
     }
   }
+  flg_zero = true;
   if (duct && n_in > 0) {  // FromBuf of the inlet particles (particle.cpp:286-301)
     if (np + n_in > cap) return FR_EOVERFLOW;
     if (n_in > inj_cap) {
@@ -1193,12 +1259,10 @@ int PartDev::resort(const Dev& d, hipStream_t st) {
   if (rc) return rc;
   PartPtrs a = ptrs(), b = alt;
   b.np = np;
-  hipLaunchKernelGGL(k_resort, dim3(nblocks(np, 256)), dim3(256), 0, st, np, T.list, inv, a, rank,
-                     b, rank_alt);
+  hipLaunchKernelGGL(k_resort, dim3(nblocks(np, 256)), dim3(256), 0, st, np, T.list, inv, a, b,
+                     rank_alt);
   FR_LAUNCH_CHECK();
   swap_storage();
-  hipLaunchKernelGGL(k_inv, dim3(nblocks(np, 256)), dim3(256), 0, st, np, rank, inv);
-  FR_LAUNCH_CHECK();
   T.valid = false;
   return FR_OK;
 }

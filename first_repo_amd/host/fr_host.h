@@ -281,7 +281,14 @@ class grid {
   // joined by join_side() before anything reads RHS_Part_Temp or qmean
   bool side_pending = false;
   double* dq = nullptr;  // reduction workspace + result of that branch
+  // join_side() waits for the branch on the device and copies its sum to
+  // pinned host memory (hq) behind an event; settle_side() also takes the
+  // mean on the host, waiting for that copy only if it has not landed
+  double* hq = nullptr;
+  hipEvent_t evq = nullptr;
+  bool q_inflight = false;
   int join_side();
+  int settle_side();
   double t_poisson = 0, t_particles = 0, t_step = 0, t_momentum = 0;
   int n_momentum = 0;
   int iters_step = 0;

@@ -432,7 +432,9 @@ grid::~grid() {
   if (evf) hipEventDestroy(evf);
   if (evj) hipEventDestroy(evj);
   if (evj2) hipEventDestroy(evj2);
+  if (evq) hipEventDestroy(evq);
   if (dq) hipFree(dq);
+  if (hq) hipHostFree(hq);
   if (st2) hipStreamDestroy(st2);
   if (st) hipStreamDestroy(st);
 }
@@ -449,6 +451,7 @@ int grid::setup(const char* params_path, const char* ic) {
   FR_CHECK(hipEventCreateWithFlags(&evf, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evj, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evj2, hipEventDisableTiming));
+  FR_CHECK(hipEventCreateWithFlags(&evq, hipEventDisableTiming));
   for (auto& e : ev) FR_CHECK(hipEventCreate(&e));
   for (auto& e : evk) FR_CHECK(hipEventCreate(&e));
   // grid.cpp:16-23
@@ -481,6 +484,7 @@ int grid::setup(const char* params_path, const char* ic) {
   FR_TRY(PS_.init(&param, &size, st));
   FR_CHECK(hipMalloc(&dred, sizeof(double) * (4 * (size_t)size.Nx() + 512)));
   FR_CHECK(hipMalloc(&dq, sizeof(double) * (RED_PARTS + 8)));
+  FR_CHECK(hipHostMalloc(&hq, sizeof(double) * 2, hipHostMallocDefault));
   return FR_OK;
 }
 
@@ -617,7 +621,7 @@ int grid::Update_Rho() {
 }
 
 int grid::Update_P0() {  // grid.cpp:539-554
-  FR_TRY(join_side());
+  FR_TRY(settle_side());
   if (!qmean_valid) {
     qmean = RHS_Part_Temp.mean();
     qmean_valid = true;
@@ -683,16 +687,29 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   return FR_OK;
 }
 
-// Joins the heat branch into the main stream and takes its mean (the
-// tensor0::mean of RHS_Part_Temp, grid.cpp:677) and the particle time.
+// Joins the heat branch into the main stream (a stream wait: the host does
+// not wait) and starts the copy of its sum to pinned host memory.
 int grid::join_side() {
   if (!side_pending) return FR_OK;
   side_pending = false;
   FR_CHECK(hipStreamWaitEvent(st, evj2, 0));
-  double h = 0;
-  FR_CHECK(hipMemcpyAsync(&h, dq + RED_PARTS, sizeof(double), hipMemcpyDeviceToHost, st));
-  FR_CHECK(hipStreamSynchronize(st));
-  qmean = h / ((double)(size.Nx() - 2) * (size.Ny() - 2) * (size.Nz() - 2));
+  FR_CHECK(hipMemcpyAsync(hq, dq + RED_PARTS, sizeof(double), hipMemcpyDeviceToHost, st));
+  FR_CHECK(hipEventRecord(evq, st));
+  q_inflight = true;
+  return FR_OK;
+}
+
+// join_side(), then the mean of RHS_Part_Temp on the host (the tensor0::mean
+// of grid.cpp:677) and the particle time.  Its first reader is the next
+// stage's Update_P0, after the pressure solve has synchronised the stream,
+// so the copy has normally landed.
+int grid::settle_side() {
+  int rc = join_side();
+  if (rc) return rc;
+  if (!q_inflight) return FR_OK;
+  q_inflight = false;
+  FR_CHECK(hipEventSynchronize(evq));
+  qmean = hq[0] / ((double)(size.Nx() - 2) * (size.Ny() - 2) * (size.Nz() - 2));
   qmean_valid = true;
   float ms = 0;
   if (hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) t_particles += ms;
@@ -751,6 +768,7 @@ static DivCoef div_coef(grid& g) {
   q.kc = g.param.k() / g.param.Cp();
   q.bq = -g.param.R() / (g.param.Cp() * g.P0_new * g.size.Vcell());
   q.cooling = g.param.Is_Cooling();
+  if (q.cooling) g.settle_side();
   if (q.cooling && !g.qmean_valid) {
     g.qmean = g.RHS_Part_Temp.mean();
     g.qmean_valid = true;
@@ -893,6 +911,7 @@ int grid::step() {  // simulation.cpp:36-70 (box calls)
   FR_TRY(Correct_mean_RU());
   FR_TRY(TimeAdvance());
   FR_CHECK(hipStreamSynchronize(st));
+  FR_TRY(settle_side());
   t_step = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   t_momentum = 0;
   for (int k = 0; k < 4; k++) {
@@ -1054,7 +1073,7 @@ int fr_sim_call(fr_sim* s, const char* m, int k) {
     }
   }
   else return FR_EINVAL;
-  if (!rc) rc = g.join_side();
+  if (!rc) rc = g.settle_side();
   if (!rc) rc = (int)hipStreamSynchronize(g.st);
   return rc;
 }
@@ -1067,7 +1086,7 @@ static tensor0* pick(fr_sim* s, const char* name, int comp) {
 
 int fr_sim_get_field(fr_sim* s, const char* name, int comp, double* host) {
   if (!s || !name || !host) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   tensor0* t = pick(s, name, comp);
   if (!t) return FR_EINVAL;
   t->download(host);
@@ -1076,7 +1095,7 @@ int fr_sim_get_field(fr_sim* s, const char* name, int comp, double* host) {
 
 int fr_sim_set_field(fr_sim* s, const char* name, int comp, const double* host) {
   if (!s || !name || !host) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   tensor0* t = pick(s, name, comp);
   if (!t) return FR_EINVAL;
   t->upload(host);
@@ -1100,7 +1119,7 @@ static double* scal(fr_sim* s, const char* name) {
 
 int fr_sim_get_scalar(fr_sim* s, const char* name, double* v) {
   if (!s || !name || !v) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   double* p = scal(s, name);
   if (!p) return FR_EINVAL;
   *v = *p;
@@ -1109,7 +1128,7 @@ int fr_sim_get_scalar(fr_sim* s, const char* name, double* v) {
 
 int fr_sim_set_scalar(fr_sim* s, const char* name, double v) {
   if (!s || !name) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   double* p = scal(s, name);
   if (!p) return FR_EINVAL;
   *p = v;
@@ -1124,14 +1143,14 @@ int fr_sim_geom(fr_sim* s, fr_geom* out) {
 
 int fr_sim_np(fr_sim* s, int* np) {
   if (!s || !np) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   *np = s->g.part.dev.np;
   return FR_OK;
 }
 
 int fr_sim_get_particles(fr_sim* s, int q, int st, double* host) {
   if (!s || !host || q < 0 || q > 7 || st < 0 || st > 3) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   PartDev& p = s->g.part.dev;
   const double* src = q == 7 ? p.rhs_T : p.s[q][st];
   return p.get(host, src, s->g.st);
@@ -1139,7 +1158,7 @@ int fr_sim_get_particles(fr_sim* s, int q, int st, double* host) {
 
 int fr_sim_set_particles(fr_sim* s, int np, int q, int st, const double* host) {
   if (!s || !host || q < 0 || q > 7 || st < 0 || st > 3) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   PartDev& p = s->g.part.dev;
   if (np > p.cap || np < 0) return FR_EOVERFLOW;
   if (np != p.np) {  // a new particle set: storage in reference order
@@ -1154,13 +1173,13 @@ int fr_sim_set_particles(fr_sim* s, int np, int q, int st, const double* host) {
 
 int fr_sim_get_flags(fr_sim* s, int* host) {
   if (!s || !host) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   return s->g.part.dev.get_flags(host, s->g.st);
 }
 
 int fr_sim_set_flags(fr_sim* s, int np, const int* host) {
   if (!s || !host) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   PartDev& p = s->g.part.dev;
   if (np > p.cap || np < 0) return FR_EOVERFLOW;
   if (np != p.np) {
@@ -1174,7 +1193,7 @@ int fr_sim_set_flags(fr_sim* s, int np, const int* host) {
 
 int fr_sim_particles_changed(fr_sim* s) {
   if (!s) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   int rc = s->g.part.rebuild_cells(1);
   if (!rc) rc = (int)hipStreamSynchronize(s->g.st);
   return rc;
@@ -1182,7 +1201,7 @@ int fr_sim_particles_changed(fr_sim* s) {
 
 int fr_sim_diagnostics(fr_sim* s, double* out) {
   if (!s || !out) return FR_EINVAL;
-  FR_TRY_S(s->g.join_side());
+  FR_TRY_S(s->g.settle_side());
   return s->g.diagnostics(out);
 }
 
