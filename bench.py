@@ -8,8 +8,10 @@ initial field; see first_repo_amd/configs.py).  A "step" is one full RK4 step
 projection) on device-resident state; statistics and I/O are excluded.
 
 Prints one JSON line (metric Mcell*steps/s, higher is better), with the
-roofline of the dominant stencil kernel measured with HIP events inside the
-library and the CPU oracle timed on a bounded sample on the host cores.
+roofline of the forward x pass of the pressure solve (the residual fused with
+the real FFT along x: the largest kernel of the step that runs with nothing
+beside it) measured with HIP events inside the library, and the CPU oracle
+timed on a bounded sample on the host cores.
 """
 from __future__ import annotations
 
@@ -24,10 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_update_ru_wop launch from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# HBM bytes per k_pass_x_fwd launch from rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (each scaled by a plain copy of known size: factors 2.0 and 1.0), measured on
-# the named workloads; profiles/hbm_traffic_box256_fp64_box512_fp32.md
-RU_WOP_TRAFFIC = {(256, "fp64"): 3.191e9, (512, "fp32"): 12.495e9}
+# the named workloads (profiles/hbm_traffic_*.md)
+X_PASS_TRAFFIC = {}
 METRIC = ("Mcell\u00b7steps/s full RK4 step w/ particles; "
           "Poisson iters+ms to 1e-10 vs plain")
 
@@ -157,7 +159,7 @@ def main():
     for _ in range(args.warmup):
         sim.step()
     t_poisson = t_part = iters = 0.0
-    kern_ms = kern_n = 0.0
+    kern_ms = kern_n = xp_ms = xp_n = 0.0
     t0 = time.perf_counter()
     for s in range(args.steps):
         sim.step()
@@ -167,6 +169,8 @@ def main():
         iters += tm[3]
         kern_ms += tm[4]
         kern_n += tm[5]
+        xp_ms += tm[6]
+        xp_n += tm[7]
     wall = time.perf_counter() - t0
     ncell = args.n ** 3
     ms_per_step = wall / args.steps * 1e3
@@ -184,6 +188,15 @@ def main():
     alg_bytes = 21 * (4 if fp32 else 8) * ghosted
     per_launch_ms = kern_ms / max(kern_n, 1)
     achieved = alg_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
+    # roofline kernel: the forward x pass of every pressure iteration
+    # (k_pass_x_fwd: residual of the defect correction + real FFT along x).
+    # Algorithmic bytes per launch: reads P, RHS (FP64) and the three
+    # Rho_face arrays (FP64, FP32 in the precision form) once per interior
+    # cell; writes the half spectrum, (n/2+1)*n*n complex (16 B, 8 B FP32).
+    nspec = (args.n // 2 + 1) * args.n * args.n
+    x_bytes = ncell * (28 if fp32 else 40) + nspec * (8 if fp32 else 16)
+    x_ms = xp_ms / max(xp_n, 1)
+    x_gbs = x_bytes / (x_ms * 1e-3) / 1e9 if x_ms > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -213,21 +226,28 @@ def main():
                    "step_hbm_frac": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
                    "TKE": float(diag[0]), "Rho_Mean": float(diag[1]),
                    "mean_particle_T": float(diag[8])},
-        "roofline": {"kernel": "k_update_ru_wop_z (+ periodic ghost shells of its 6 outputs)",
+        "roofline": {"kernel": "k_pass_x_fwd (pressure residual + real FFT along x)",
                      "bound": "hbm",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": RU_WOP_TRAFFIC.get((args.n, args.precision))
+                     "achieved": round(x_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(x_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": X_PASS_TRAFFIC.get((args.n, args.precision))
                      if args.particles == {256: 4_000_000, 512: 16_000_000}.get(args.n) else None,
-                     "alg_bytes_per_launch": alg_bytes,
-                     "avg_launch_ms": round(per_launch_ms, 4)},
+                     "alg_bytes_per_launch": x_bytes,
+                     "avg_launch_ms": round(x_ms, 4), "launches": int(xp_n)},
+        # the fused momentum kernel runs beside the particle heat branch on the
+        # second stream, so its launch time includes that sharing
+        "momentum_kernel": {"kernel": "k_update_ru_wop_z (+ periodic ghost shells of its 6 "
+                                      "outputs)",
+                            "alg_bytes_per_launch": alg_bytes,
+                            "avg_launch_ms": round(per_launch_ms, 4),
+                            "achieved_GBs": round(achieved, 1)},
     }
     sim.close()
     sgbs = stream_copy_gbs()
     if sgbs:
         # the same kernel against the bandwidth a plain copy reaches here
         out["roofline"]["stream_copy_GBs"] = round(sgbs, 1)
-        out["roofline"]["frac_of_stream"] = round(achieved / sgbs, 4)
+        out["roofline"]["frac_of_stream"] = round(x_gbs / sgbs, 4)
     if fp32 and not args.no_error:
         # error of the precision form against the FP64 path after the same steps
         print("bench: FP64 reference run for the error of the precision form", file=sys.stderr,

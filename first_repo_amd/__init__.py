@@ -113,6 +113,7 @@ def lib():
             "fr_sim_reserve_particles": ([vp, i], i),
             "fr_sim_diagnostics": ([vp, vp], i),
             "fr_sim_timing": ([vp, vp], i),
+            "fr_sim_timing_ex": ([vp, vp, i], i),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -223,8 +224,10 @@ class Sim:
         return out
 
     def timing(self):
-        out = np.zeros(6)
-        check(self.L.fr_sim_timing(self.h, _p(out)))
+        """[step ms, Poisson ms, particle ms, Poisson iterations, momentum
+        kernel ms, its launches, forward x-pass ms, its launches]."""
+        out = np.zeros(8)
+        check(self.L.fr_sim_timing_ex(self.h, _p(out), 8))
         return out
 
     def close(self):

@@ -26,6 +26,8 @@
 #include <cstdlib>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "fft_lds.hpp"
 
 namespace fr {
@@ -77,6 +79,9 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
   load_tw(twm, twM, M);
   load_tw(twn, twN, q.nx);
   const int rows = q.ny * q.nz, lm = q.lxm, lw = q.lwx;
+  // plain dispatch order: the neighbouring row blocks whose rows the residual
+  // stencil shares run at the same time across the XCDs (measured: an
+  // XCD-contiguous order fetches 16% more and runs 8% slower here)
   const unsigned bid = blockIdx.x;
   const int row0 = (int)bid * W;
   for (int t = threadIdx.x; t < W * M; t += FT) {
@@ -303,6 +308,11 @@ struct fr_box_poisson {
   double* hres = nullptr;
   int nbx = 0;
   size_t lds[3] = {};  // dynamic LDS bytes of the x, y and z passes
+  // HIP events around every launch of the forward x pass (residual + real
+  // FFT), read after the iteration's host synchronisation: summed ms, count
+  hipEvent_t tx0 = nullptr, tx1 = nullptr;
+  double tx_ms = 0;
+  long tx_n = 0;
 };
 
 // One defect-correction iteration; leaves max|dp| in s->dres[0].  With inv
@@ -321,12 +331,13 @@ static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const
   const C2* twim = f ? (const C2*)s->twim32 : (const C2*)s->twim;
   const size_t* lds = f ? s->lds32 : s->lds;
   const int nbx = f ? s->nbx32 : s->nbx;
+  // the events are bound to the launch (start and end of the kernel itself)
   if (inv)
-    hipLaunchKernelGGL((k_pass_x_fwd<true, TC, R>), dim3(nbx), dim3(FT), lds[0], st, q, P, cx, cy,
-                       cz, RHS, twfm, twf[0], data);
+    hipExtLaunchKernelGGL((k_pass_x_fwd<true, TC, R>), dim3(nbx), dim3(FT), lds[0], st, s->tx0,
+                          s->tx1, 0, q, P, cx, cy, cz, RHS, twfm, twf[0], data);
   else
-    hipLaunchKernelGGL((k_pass_x_fwd<false, TC, R>), dim3(nbx), dim3(FT), lds[0], st, q, P, cx,
-                       cy, cz, RHS, twfm, twf[0], data);
+    hipExtLaunchKernelGGL((k_pass_x_fwd<false, TC, R>), dim3(nbx), dim3(FT), lds[0], st, s->tx0,
+                          s->tx1, 0, q, P, cx, cy, cz, RHS, twfm, twf[0], data);
   const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.ny + q.wz - 1) / q.wz;
   hipLaunchKernelGGL(k_pass_strided<R>, dim3(ty * q.nz), dim3(FT), lds[1], st, q, 1, 0, twf[1],
                      twf[1], s->lam[0], s->lam[1], s->lam[2], data);
@@ -429,6 +440,11 @@ int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* c
     FR_CHECK(hipMemcpyAsync(s->hres, s->dres, sizeof(double), hipMemcpyDeviceToHost, st));
     FR_CHECK(hipStreamSynchronize(st));
     error = s->hres[0];
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, s->tx0, s->tx1) == hipSuccess) {
+      s->tx_ms += ms;
+      s->tx_n++;
+    }
     i++;
   }
   if (num_iter) *num_iter = i;
@@ -440,6 +456,16 @@ template int box_poisson_solve<double>(fr_box_poisson*, const double*, const dou
 template int box_poisson_solve<float>(fr_box_poisson*, const float*, const float*, const float*,
                                       bool, double*, const double*, int, double, int*,
                                       hipStream_t);
+
+void box_poisson_xtimer(fr_box_poisson* s, double* ms, long* n, bool reset) {
+  if (!s) return;
+  if (ms) *ms = s->tx_ms;
+  if (n) *n = s->tx_n;
+  if (reset) {
+    s->tx_ms = 0;
+    s->tx_n = 0;
+  }
+}
 
 }  // namespace fr
 
@@ -500,6 +526,8 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
   FR_CHECK(hipMalloc(&s->bmax, sizeof(double) * (s->nbx + 1)));
   FR_CHECK(hipMalloc(&s->dres, sizeof(double) * 2));
   FR_CHECK(hipHostMalloc(&s->hres, sizeof(double) * 2, hipHostMallocDefault));
+  FR_CHECK(hipEventCreate(&s->tx0));
+  FR_CHECK(hipEventCreate(&s->tx1));
   *out = s;
   return FR_OK;
 }
@@ -522,6 +550,8 @@ int fr_box_poisson_destroy(fr_box_poisson* s) {
   (void)hipFree(s->bmax);
   (void)hipFree(s->dres);
   (void)hipHostFree(s->hres);
+  if (s->tx0) (void)hipEventDestroy(s->tx0);
+  if (s->tx1) (void)hipEventDestroy(s->tx1);
   delete s;
   return FR_OK;
 }

@@ -100,6 +100,9 @@ template <typename TC>
 int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz, bool inv,
                       double* P, const double* RHS, int iterations, double epsilon, int* num_iter,
                       hipStream_t st);
+// Summed duration (ms, HIP events) and count of the forward x-pass launches
+// of the box solve since the last reset.
+void box_poisson_xtimer(fr_box_poisson* s, double* ms, long* n, bool reset);
 
 // fr_duct_poisson_solve with the same Rho_face option (duct_poisson.hip).
 int duct_poisson_solve(fr_duct_poisson* s, const double* cx, const double* cy, const double* cz,

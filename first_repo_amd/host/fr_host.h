@@ -291,6 +291,8 @@ class grid {
   int settle_side();
   double t_poisson = 0, t_particles = 0, t_step = 0, t_momentum = 0;
   int n_momentum = 0;
+  double t_xpass = 0;  // forward x pass of the box FFT solve: summed ms, launches
+  long n_xpass = 0;
   int iters_step = 0;
   double* dred = nullptr;  // device scratch for diagnostics (Nx doubles x 4)
 

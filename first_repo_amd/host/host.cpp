@@ -895,6 +895,7 @@ int grid::step() {  // simulation.cpp:36-70 (box calls)
   param.update(T_cur);
   t_poisson = t_particles = 0;
   iters_step = 0;
+  box_poisson_xtimer(PS_.s, nullptr, nullptr, true);
   auto t0 = std::chrono::steady_clock::now();
   for (RK4_count = 0; RK4_count < 4; RK4_count++) {
     FR_TRY(Update_Rho());
@@ -920,6 +921,7 @@ int grid::step() {  // simulation.cpp:36-70 (box calls)
     t_momentum += ms;
   }
   n_momentum = 4;
+  box_poisson_xtimer(PS_.s, &t_xpass, &n_xpass, false);
   return FR_OK;
 }
 
@@ -1230,14 +1232,14 @@ int fr_sim_learned_stats(fr_sim* s, int* guesses_kept, int* fallbacks, double* l
   return fr_pcg_learned_stats(s->g.PS_.sp, guesses_kept, fallbacks, log10_gain);
 }
 
-int fr_sim_timing(fr_sim* s, double* out) {
-  if (!s || !out) return FR_EINVAL;
-  out[0] = s->g.t_step;
-  out[1] = s->g.t_poisson;
-  out[2] = s->g.t_particles;
-  out[3] = s->g.iters_step;
-  out[4] = s->g.t_momentum;
-  out[5] = s->g.n_momentum;
+int fr_sim_timing(fr_sim* s, double* out) { return fr_sim_timing_ex(s, out, 6); }
+
+int fr_sim_timing_ex(fr_sim* s, double* out, int n) {
+  if (!s || !out || n < 0) return FR_EINVAL;
+  const double v[8] = {s->g.t_step,     s->g.t_poisson, s->g.t_particles,
+                       (double)s->g.iters_step, s->g.t_momentum, (double)s->g.n_momentum,
+                       s->g.t_xpass,    (double)s->g.n_xpass};
+  for (int i = 0; i < n && i < 8; i++) out[i] = v[i];
   return FR_OK;
 }
 

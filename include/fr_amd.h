@@ -280,6 +280,11 @@ int fr_sim_diagnostics(fr_sim* s, double* out);
  * duration of the fused momentum kernel (ms, HIP events on its stream),
  * [5] its launch count. */
 int fr_sim_timing(fr_sim* s, double* out);
+/* fr_sim_timing with out[6] the summed duration (ms, HIP events on the
+ * solver's stream) of the box FFT solve's forward x pass (residual + real
+ * FFT, k_pass_x_fwd) in the last step and out[7] its launch count; writes
+ * the first min(n, 8) values. */
+int fr_sim_timing_ex(fr_sim* s, double* out, int n);
 
 #ifdef __cplusplus
 }
