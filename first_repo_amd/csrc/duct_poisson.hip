@@ -159,7 +159,7 @@ __global__ void k_duct_cprime(DuctDev q, const double* __restrict__ lamy,
 // coefficients), in place on T[i][m].  Mode (0,0) is pinned by a zero last d'.
 // The sweeps are serial in i; the values and c' of TB consecutive rows are
 // loaded together ahead of the dependent arithmetic.
-constexpr int TB = 16;
+constexpr int TB = 32;
 // One thread per (mode, real or imaginary part): the two parts are
 // independent systems with identical coefficients.
 __global__ void __launch_bounds__(64) k_duct_thomas(DuctDev q, const double* __restrict__ lamy,
