@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_pass_x_fwd launch from rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (each scaled by a plain copy of known size: factors 2.0 and 1.0), measured on
 # the named workloads (profiles/hbm_traffic_*.md)
-X_PASS_TRAFFIC = {}
+X_PASS_TRAFFIC = {(256, "fp64"): 1.051e9, (512, "fp32"): 6.437e9}
 METRIC = ("Mcell\u00b7steps/s full RK4 step w/ particles; "
           "Poisson iters+ms to 1e-10 vs plain")
 
@@ -197,6 +197,8 @@ def main():
     x_bytes = ncell * (28 if fp32 else 40) + nspec * (8 if fp32 else 16)
     x_ms = xp_ms / max(xp_n, 1)
     x_gbs = x_bytes / (x_ms * 1e-3) / 1e9 if x_ms > 0 else 0.0
+    if xp_n == 0:  # PCG solve: no FFT pass; the fused momentum kernel instead
+        x_bytes, x_ms, x_gbs = alg_bytes, per_launch_ms, achieved
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -226,12 +228,15 @@ def main():
                    "step_hbm_frac": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
                    "TKE": float(diag[0]), "Rho_Mean": float(diag[1]),
                    "mean_particle_T": float(diag[8])},
-        "roofline": {"kernel": "k_pass_x_fwd (pressure residual + real FFT along x)",
+        "roofline": {"kernel": "k_pass_x_fwd (pressure residual + real FFT along x)" if xp_n
+                     else "k_update_ru_wop_z (+ periodic ghost shells of its 6 outputs; runs "
+                          "beside the particle heat branch)",
                      "bound": "hbm",
                      "achieved": round(x_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(x_gbs / HBM_PEAK_GBS, 4),
                      "traffic": X_PASS_TRAFFIC.get((args.n, args.precision))
-                     if args.particles == {256: 4_000_000, 512: 16_000_000}.get(args.n) else None,
+                     if xp_n and args.particles == {256: 4_000_000, 512: 16_000_000}.get(args.n)
+                     else None,
                      "alg_bytes_per_launch": x_bytes,
                      "avg_launch_ms": round(x_ms, 4), "launches": int(xp_n)},
         # the fused momentum kernel runs beside the particle heat branch on the
