@@ -164,6 +164,11 @@ class PartDev {
   Buckets R;  // rounded index, operands x_l,y_l,z_l,RHS_T
   Buckets T;  // truncating index, operands x_l,y_l,z_l,RHS_u,RHS_v,RHS_w
   int* scan_tmp = nullptr;
+  // the migration has its own scan scratch, and a copy of the reference
+  // indices (rank_dep) lets a bucket build run beside a migration that
+  // rewrites them
+  int* mig_tmp = nullptr;
+  int* rank_dep = nullptr;
   int *flg = nullptr, *nl = nullptr, *pNL = nullptr, *holepos = nullptr, *tailnl = nullptr;
   int *evt = nullptr, *islo = nullptr, *plo = nullptr, *dst = nullptr, *mslot = nullptr,
       *mlv = nullptr;
@@ -180,7 +185,9 @@ class PartDev {
   template <typename F>
   int temp(const Dev& d, int stage, double c0, const F* rho, double iTth, hipStream_t st);
   // buckets of the `stage` positions; mode 0 rounded (R), 1 truncating (T)
-  int build(const Dev& d, int stage, int mode, hipStream_t st, bool operands = true);
+  // rk: reference indices to order the buckets by (default: rank)
+  int build(const Dev& d, int stage, int mode, hipStream_t st, bool operands = true,
+            const int* rk = nullptr);
   // trilinear deposit from R: weighted by RHS_T (heat) or not (concentration)
   template <typename F>
   int deposit_cic(const Dev& d, bool weighted, F* out, bool halo, hipStream_t st);

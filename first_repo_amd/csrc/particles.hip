@@ -1047,6 +1047,8 @@ int PartDev::alloc(int capacity, const Dev& d) {
   if (rc) return rc;
   const long big = ncell > cap ? ncell : cap;
   FR_CHECK(hipMalloc(&scan_tmp, sizeof(int) * (big / STILE + 2)));
+  FR_CHECK(hipMalloc(&mig_tmp, sizeof(int) * (big / STILE + 2)));
+  FR_CHECK(hipMalloc(&rank_dep, sizeof(int) * cap));
   int** m[11] = {&flg, &nl, &pNL, &holepos, &tailnl, &evt, &islo, &plo, &dst, &mslot, &mlv};
   for (auto pp : m) FR_CHECK(hipMalloc(pp, sizeof(int) * (cap + 1)));
   FR_CHECK(hipHostMalloc(&hbuf, sizeof(int) * 4, hipHostMallocDefault));
@@ -1060,8 +1062,8 @@ void PartDev::release() {
       (void)hipFree(alt.s[q][t]);
     }
   void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, alt.flag, alt.rhs_u, alt.rhs_v, alt.rhs_w,
-                 alt.rhs_T, rank, rank_alt, inv, scan_tmp, flg, nl, pNL, holepos, tailnl,
-                 evt, islo, plo, dst, mslot, mlv, inj_buf};
+                 alt.rhs_T, rank, rank_alt, inv, scan_tmp, mig_tmp, rank_dep, flg, nl, pNL,
+                 holepos, tailnl, evt, islo, plo, dst, mslot, mlv, inj_buf};
   for (void* p : all) (void)hipFree(p);
   free_buckets(R);
   free_buckets(T);
@@ -1100,8 +1102,10 @@ int PartDev::temp(const Dev& d, int stage, double c0, const T* rho, double iTth,
   return FR_OK;
 }
 
-int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool operands) {
+int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool operands,
+                   const int* rk) {
   Buckets& B = mode == 0 ? R : T;
+  if (!rk) rk = rank;
   FR_CHECK(hipMemsetAsync(B.cnt, 0, sizeof(int) * ncell, st));
   if (np > 0)
     hipLaunchKernelGGL(k_bucket_count, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), stage,
@@ -1110,9 +1114,9 @@ int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool opera
   if (rc) return rc;
   if (np > 0) {
     hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
-                       B.slot, B.start, rank, B.list2);
+                       B.slot, B.start, rk, B.list2);
     hipLaunchKernelGGL(k_bucket_place, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
-                       B.start, rank, B.list2, B.list, B.slot);
+                       B.start, rk, B.list2, B.list, B.slot);
     if (operands)
       hipLaunchKernelGGL(k_bucket_gather, dim3(nblocks(np, 256)), dim3(256), 0, st, np, mode,
                          ptrs(), stage, B.slot, d.idx, d.idy, d.idz, B.s[0], B.s[1], B.s[2],
@@ -1164,7 +1168,7 @@ int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], 
     const unsigned nb = nblocks(np, 256);
     hipLaunchKernelGGL(k_mig_flag, dim3(nb), dim3(256), 0, st, np, (const double*)s[ax][2], rank,
                        lo[ax], hi[ax], flg);
-    int rc = scan_ex_t<true>(flg, np, pNL, scan_tmp, st);  // pNL: stayers before each index
+    int rc = scan_ex_t<true>(flg, np, pNL, mig_tmp, st);  // pNL: stayers before each index
     if (rc) return rc;
     hipLaunchKernelGGL(k_mig_rank, dim3(nb), dim3(256), 0, st, np, flg, pNL, holepos, tailnl);
     hipLaunchKernelGGL(k_mig_event, dim3(nb), dim3(256), 0, st, np, flg, pNL, tailnl, evt, islo);
@@ -1185,7 +1189,7 @@ int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], 
       // the leavers now hold reference indices K..np-1 and are not received
       // back: keep the slots with rank < K, in storage order
       hipLaunchKernelGGL(k_live, dim3(nb), dim3(256), 0, st, np, rank, K, nl);
-      rc = scan_ex(nl, np, pNL, scan_tmp, st);
+      rc = scan_ex(nl, np, pNL, mig_tmp, st);
       if (rc) return rc;
       PartPtrs a = ptrs(), b = alt;
       hipLaunchKernelGGL(k_compact, dim3(nb), dim3(256), 0, st, np, nl, pNL, a, rank, b,

@@ -207,8 +207,10 @@ class particle {
   // particle.cpp:340-399 + 314-338 + 480-496 (gather and the three ODE updates)
   int advance(int k, double pre, double post, tensor1& U);
   int part2gas_concentration(tensor0& A, hipStream_t on = 0);  // particle.cpp:526-559
-  int part2gas_velocity(tensor1& A);            // particle.cpp:498-524
-  int Send_Recv();                              // particle.cpp:853-858
+  // rk: reference indices to order the deposit by (a copy taken before a
+  // migration that runs beside it); default the current ones
+  int part2gas_velocity(tensor1& A, const int* rk = nullptr);  // particle.cpp:498-524
+  int Send_Recv(hipStream_t on = 0);                            // particle.cpp:853-858
   // duct rank that is inlet and outlet: Send_Recv(particle& pbox)
   // (particle.cpp:860-865, 814-851) with the box's Rbuf1 given as n inlet
   // particles of 29 values each
@@ -275,11 +277,16 @@ class grid {
   // second stream (default priority) for the concentration deposit, which is
   // independent of the momentum deposit; fork/join events
   hipStream_t st2 = 0;
+  // third stream (default priority): the box migration beside the deposits
+  hipStream_t st3 = 0;
+  hipEvent_t evm = nullptr, evm0 = nullptr;
   hipEvent_t evf = nullptr, evj = nullptr, evj2 = nullptr;
   // the heat branch of Update_Particle (gas temperature at the particles,
   // buckets, heat deposit, its mean) runs on st2 beside Update_RU_WOP and is
   // joined by join_side() before anything reads RHS_Part_Temp or qmean
   bool side_pending = false;
+  bool side_held = false;  // branch recorded but its launches not yet issued
+  int launch_side();
   double* dq = nullptr;  // reduction workspace + result of that branch
   // join_side() waits for the branch on the device and copies its sum to
   // pinned host memory (hq) behind an event; settle_side() also takes the

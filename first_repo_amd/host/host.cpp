@@ -324,8 +324,8 @@ int particle::part2gas_concentration(tensor0& A, hipStream_t on) {
   return dev.deposit_cic(size_->dev(), false, A.pointer(), false, s);
 }
 
-int particle::part2gas_velocity(tensor1& A) {
-  int rc = dev.build(size_->dev(), 1, 1, st_);
+int particle::part2gas_velocity(tensor1& A, const int* rk) {
+  int rc = dev.build(size_->dev(), 1, 1, st_, true, rk);
   if (rc) return rc;
   if (A.x.is_f32()) {
     float* o[3] = {A.x.p<float>(), A.y.p<float>(), A.z.p<float>()};
@@ -335,11 +335,11 @@ int particle::part2gas_velocity(tensor1& A) {
   return dev.deposit_vel(size_->dev(), o, st_);
 }
 
-int particle::Send_Recv() {
+int particle::Send_Recv(hipStream_t on) {
   const double lo[3] = {size_->xl_, size_->yl_, size_->zl_};
   const double hi[3] = {size_->xh_, size_->yh_, size_->zh_};
   const double L[3] = {size_->Lx(), size_->Ly(), size_->Lz()};
-  return dev.migrate(lo, hi, L, st_);
+  return dev.migrate(lo, hi, L, on ? on : st_);
 }
 
 int particle::Send_Recv_duct(const double* inlet, int n) {
@@ -432,6 +432,9 @@ grid::~grid() {
   if (evf) hipEventDestroy(evf);
   if (evj) hipEventDestroy(evj);
   if (evj2) hipEventDestroy(evj2);
+  if (evm) hipEventDestroy(evm);
+  if (evm0) hipEventDestroy(evm0);
+  if (st3) hipStreamDestroy(st3);
   if (evq) hipEventDestroy(evq);
   if (dq) hipFree(dq);
   if (hq) hipHostFree(hq);
@@ -448,6 +451,9 @@ int grid::setup(const char* params_path, const char* ic) {
   ic_dir = ic ? ic : ".";
   FR_CHECK(hipStreamCreate(&st));
   FR_CHECK(hipStreamCreate(&st2));
+  FR_CHECK(hipStreamCreate(&st3));
+  FR_CHECK(hipEventCreateWithFlags(&evm, hipEventDisableTiming));
+  FR_CHECK(hipEventCreateWithFlags(&evm0, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evf, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evj, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evj2, hipEventDisableTiming));
@@ -653,25 +659,57 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   // The concentration deposit (rounded-index buckets, output C) and the
   // momentum deposit (truncating-index buckets, output RHS_RU) read the same
   // particles and write disjoint data: both are latency-bound, so they run
-  // side by side on two streams and join before the migration.
+  // side by side on two streams.  In the box the migration (new-stage
+  // positions and reference indices, which neither deposit reads) runs on a
+  // third stream beside them; the momentum deposit orders its buckets by a
+  // copy of the reference indices taken before the migration starts, which
+  // is the order of the reference, whose deposits precede Send_Recv.
   FR_CHECK(hipEventRecord(evf, st));
   FR_CHECK(hipStreamWaitEvent(st2, evf, 0));
   FR_TRY(part.part2gas_concentration(C, st2));
+  FR_CHECK(hipEventRecord(evj, st2));
+  const bool box = param.IsBox();
+  const int* rk = nullptr;
+  if (box) {
+    if (param.TWC() && part.dev.np > 0) {
+      FR_CHECK(hipMemcpyAsync(part.dev.rank_dep, part.dev.rank, sizeof(int) * part.dev.np,
+                              hipMemcpyDeviceToDevice, st));
+      rk = part.dev.rank_dep;
+    }
+    FR_CHECK(hipEventRecord(evm0, st));
+  }
+  // the momentum deposit is the longer branch: its launches go first
   if (param.TWC())
-    FR_TRY(part.part2gas_velocity(RHS_RU));
+    FR_TRY(part.part2gas_velocity(RHS_RU, rk));
   else
     RHS_RU.assign(0);
-  FR_CHECK(hipEventRecord(evj, st2));
+  if (box) {
+    FR_CHECK(hipStreamWaitEvent(st3, evm0, 0));
+    FR_TRY(part.Send_Recv(st3));
+    FR_CHECK(hipEventRecord(evm, st3));
+  }
   FR_CHECK(hipStreamWaitEvent(st, evj, 0));
-  if (param.IsBox()) {
-    FR_TRY(part.Send_Recv());
+  if (box) {
+    FR_CHECK(hipStreamWaitEvent(st, evm, 0));
   } else {
     FR_TRY(part.Send_Recv_duct(inlet.data(), (int)(inlet.size() / 29)));
     inlet.clear();
   }
   // Heat branch on st2, beside Update_RU_WOP (which reads none of it):
-  // gas temperature at the particles, buckets, heat deposit, its sum.
+  // gas temperature at the particles, buckets, heat deposit, its sum.  Its
+  // launches are issued after the momentum kernel's (launch_side), so that
+  // kernel is queued first; anything that needs the branch goes through
+  // join_side(), which issues it if it is still held.
   FR_CHECK(hipEventRecord(evf, st));
+  side_held = true;
+  side_pending = true;
+  qmean_valid = false;
+  return FR_OK;
+}
+
+int grid::launch_side() {
+  if (!side_held) return FR_OK;
+  side_held = false;
   FR_CHECK(hipStreamWaitEvent(st2, evf, 0));
   FR_TRY(part.temp(2, P0_new / param.R(), Rho_new, st2));
   FR_TRY(part.rebuild_cells(2, st2));
@@ -682,8 +720,6 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   else
     FR_TRY(reduce_async_ws(size.dev(), RHS_Part_Temp.pointer(), 0, dq + RED_PARTS, dq, st2));
   FR_CHECK(hipEventRecord(evj2, st2));
-  side_pending = true;
-  qmean_valid = false;
   return FR_OK;
 }
 
@@ -691,6 +727,7 @@ int grid::Update_Particle() {  // grid.cpp:659-680
 // not wait) and starts the copy of its sum to pinned host memory.
 int grid::join_side() {
   if (!side_pending) return FR_OK;
+  FR_TRY(launch_side());
   side_pending = false;
   FR_CHECK(hipStreamWaitEvent(st, evj2, 0));
   FR_CHECK(hipMemcpyAsync(hq, dq + RED_PARTS, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -758,6 +795,8 @@ int grid::Update_RU_WOP() {  // grid.cpp:387-425
                               inflow + 2 * m, inflow + 5 * m, st));
   }
   hipEventRecord(evk[2 * (RK4_count & 3) + 1], st);
+  // the particle heat branch, held by Update_Particle, now runs beside it
+  FR_TRY(launch_side());
   return FR_OK;
 }
 
