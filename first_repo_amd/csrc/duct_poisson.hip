@@ -48,8 +48,10 @@ __global__ void __launch_bounds__(FT) k_duct_y_fwd(DuctDev q, const double* __re
                                                    const double* __restrict__ RHS,
                                                    const double2* __restrict__ twM,
                                                    const double2* __restrict__ twN,
-                                                   double2* __restrict__ A) {
+                                                   double2* __restrict__ A,
+                                                   const int* __restrict__ gate) {
   extern __shared__ double2 sm[];
+  if (*gate) return;  // the solve has converged: a queued iteration does nothing
   const Dev& d = q.d;
   const int M = q.M, W = q.wy, ps = M + 1;
   double2* buf = sm;
@@ -91,8 +93,10 @@ __global__ void __launch_bounds__(FT) k_duct_y_fwd(DuctDev q, const double* __re
 template <bool FWD>
 __global__ void __launch_bounds__(FT) k_duct_z(DuctDev q, const double2* __restrict__ tw,
                                                const double2* __restrict__ in,
-                                               double2* __restrict__ out) {
+                                               double2* __restrict__ out,
+                                               const int* __restrict__ gate) {
   extern __shared__ double2 sm[];
+  if (*gate) return;
   const int L = q.nz, W = q.wz, ps = L + 1;
   double2* buf = sm;
   double2* tws = sm + W * ps;
@@ -165,9 +169,10 @@ constexpr int TB = 32;
 __global__ void __launch_bounds__(64) k_duct_thomas(DuctDev q, const double* __restrict__ lamy,
                                                     const double* __restrict__ lamz,
                                                     const double* __restrict__ cp,
-                                                    double* __restrict__ T) {
+                                                    double* __restrict__ T,
+                                                    const int* __restrict__ gate) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * q.nm) return;
+  if (t >= 2 * q.nm || *gate) return;
   const long m = t >> 1;
   double b0, b;
   thomas_coef(q, lamy, lamz, m, b0, b);
@@ -252,8 +257,10 @@ __global__ void __launch_bounds__(FT) k_duct_y_inv(DuctDev q, const double2* __r
                                                    const double2* __restrict__ twM,
                                                    const double2* __restrict__ twN,
                                                    double* __restrict__ P,
-                                                   double* __restrict__ bmax) {
+                                                   double* __restrict__ bmax,
+                                                   const int* __restrict__ gate) {
   extern __shared__ double2 sm[];
+  if (*gate) return;
   __shared__ double red[FT];
   const Dev& d = q.d;
   const int M = q.M, W = q.wy, ps = M + 1;
@@ -328,11 +335,16 @@ struct fr_duct_poisson {
   double* hres = nullptr;
   int nby = 0, nbz = 0;
   size_t lds_y = 0, lds_z = 0;
+  // device convergence gate (done flag, iterations run), its host copy, and
+  // the previous solve's iteration count (the next batch size)
+  int* gate = nullptr;
+  int* hgate = nullptr;
+  int last_iters = 1;
 };
 
 static int duct_poisson_iter(fr_duct_poisson* s, const double* cx, const double* cy,
                              const double* cz, bool inv, double* P, const double* RHS,
-                             hipStream_t st);
+                             double eps, hipStream_t st);
 
 namespace fr {
 
@@ -341,17 +353,25 @@ int duct_poisson_solve(fr_duct_poisson* s, const double* cx, const double* cy, c
                        bool inv, double* P, const double* RHS, int iterations, double epsilon,
                        int* num_iter, hipStream_t st) {
   if (!s || !cx || !cy || !cz || !P || !RHS) return FR_EINVAL;
-  int i = 0;
-  double error = 100;
-  while ((i < iterations) && (error > epsilon)) {
-    int rc = duct_poisson_iter(s, cx, cy, cz, inv, P, RHS, st);
-    if (rc) return rc;
-    FR_CHECK(hipMemcpyAsync(s->hres, s->dres, sizeof(double), hipMemcpyDeviceToHost, st));
+  // Iterations queued in batches behind the device gate, as in the box
+  // solve: the same iterations run, one host synchronisation per batch.
+  FR_CHECK(hipMemsetAsync(s->gate, 0, sizeof(int) * 2, st));
+  int launched = 0, ran = 0, batch = s->last_iters;
+  while (launched < iterations) {
+    const int k = std::min({batch, iterations - launched, 32});
+    for (int j = 0; j < k; j++) {
+      int rc = duct_poisson_iter(s, cx, cy, cz, inv, P, RHS, epsilon, st);
+      if (rc) return rc;
+    }
+    launched += k;
+    FR_CHECK(hipMemcpyAsync(s->hgate, s->gate, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
     FR_CHECK(hipStreamSynchronize(st));
-    error = s->hres[0];
-    i++;
+    ran = s->hgate[1];
+    if (s->hgate[0]) break;
+    batch = 1;
   }
-  if (num_iter) *num_iter = i;
+  s->last_iters = ran > 0 ? ran : 1;
+  if (num_iter) *num_iter = ran;
   return FR_OK;
 }
 
@@ -428,6 +448,8 @@ int fr_duct_poisson_create(const fr_geom* g, fr_duct_poisson** out) {
   FR_CHECK(hipMalloc(&s->bmax, sizeof(double) * (s->nby + 1)));
   FR_CHECK(hipMalloc(&s->dres, sizeof(double) * 2));
   FR_CHECK(hipHostMalloc(&s->hres, sizeof(double) * 2, hipHostMallocDefault));
+  FR_CHECK(hipMalloc(&s->gate, sizeof(int) * 2));
+  FR_CHECK(hipHostMalloc(&s->hgate, sizeof(int) * 2, hipHostMallocDefault));
   hipLaunchKernelGGL(k_duct_cprime, dim3(nblocks(q.nm, 256)), dim3(256), 0, 0, q, s->lamy,
                      s->lamz, s->cp);
   FR_LAUNCH_CHECK();
@@ -452,6 +474,8 @@ int fr_duct_poisson_destroy(fr_duct_poisson* s) {
   (void)hipFree(s->bmax);
   (void)hipFree(s->dres);
   (void)hipHostFree(s->hres);
+  (void)hipFree(s->gate);
+  (void)hipHostFree(s->hgate);
   delete s;
   return FR_OK;
 }
@@ -460,24 +484,26 @@ int fr_duct_poisson_destroy(fr_duct_poisson* s) {
 // the coefficient arrays hold Rho_face and C = 1./Rho_face.
 static int duct_poisson_iter(fr_duct_poisson* s, const double* cx, const double* cy,
                              const double* cz, bool inv, double* P, const double* RHS,
-                             hipStream_t st) {
+                             double eps, hipStream_t st) {
   DuctDev& q = s->q;
+  const int* g = s->gate;
   if (inv)
     hipLaunchKernelGGL(k_duct_y_fwd<true>, dim3(s->nby), dim3(FT), s->lds_y, st, q, P, cx, cy,
-                       cz, RHS, s->twfm, s->twfy, s->A);
+                       cz, RHS, s->twfm, s->twfy, s->A, g);
   else
     hipLaunchKernelGGL(k_duct_y_fwd<false>, dim3(s->nby), dim3(FT), s->lds_y, st, q, P, cx, cy,
-                       cz, RHS, s->twfm, s->twfy, s->A);
+                       cz, RHS, s->twfm, s->twfy, s->A, g);
   hipLaunchKernelGGL(k_duct_z<true>, dim3(s->nbz), dim3(FT), s->lds_z, st, q, s->twfz, s->A,
-                     s->T);
+                     s->T, g);
   hipLaunchKernelGGL(k_duct_thomas, dim3(nblocks(2 * q.nm, 64)), dim3(64), 0, st, q, s->lamy,
-                     s->lamz, s->cp, (double*)s->T);
+                     s->lamz, s->cp, (double*)s->T, g);
   hipLaunchKernelGGL(k_duct_z<false>, dim3(s->nbz), dim3(FT), s->lds_z, st, q, s->twiz, s->T,
-                     s->A);
+                     s->A, g);
   hipLaunchKernelGGL(k_duct_y_inv, dim3(s->nby), dim3(FT), s->lds_y, st, q, s->A, s->twim,
-                     s->twiy, P, s->bmax);
+                     s->twiy, P, s->bmax, g);
   FR_LAUNCH_CHECK();
-  int rc = max_final(s->bmax, s->nby, s->dres, st);
+  // the ghost copies of P below (periodic y,z; Neumann x) are idempotent
+  int rc = max_final(s->bmax, s->nby, s->dres, st, s->gate, eps);
   if (!rc) rc = ghost_fill(q.d, P, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_neumann_x, dim3(nblocks((long)q.d.Ny * q.d.Nz, 256)), dim3(256), 0, st,

@@ -153,7 +153,12 @@ __device__ inline void block_max_store(double m, double* red, double* out) {
 namespace fr {
 
 // Max of n per-block maxima into out[0] (one workgroup), on stream st.
-int max_final(const double* b, int n, double* out, hipStream_t st);
+// With a gate (int[2], device): nothing happens once gate[0] is set;
+// otherwise gate[1] counts the iteration and gate[0] is set when the max is
+// not above eps (the stopping test of the defect-correction loops), so later
+// iterations already queued behind it do nothing.
+int max_final(const double* b, int n, double* out, hipStream_t st, int* gate = nullptr,
+              double eps = 0.0);
 
 inline int ilog2(int n) {
   int l = 0;

@@ -67,10 +67,12 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
                                                    const double* __restrict__ RHS,
                                                    const typename Cx<R>::T* __restrict__ twM,
                                                    const typename Cx<R>::T* __restrict__ twN,
-                                                   typename Cx<R>::T* __restrict__ out) {
+                                                   typename Cx<R>::T* __restrict__ out,
+                                                   const int* __restrict__ gate) {
   using C2 = typename Cx<R>::T;
   extern __shared__ __align__(16) unsigned char smraw[];
   C2* sm = reinterpret_cast<C2*>(smraw);
+  if (*gate) return;  // the solve has converged: a queued iteration does nothing
   const Dev& d = q.d;
   const int M = q.nx >> 1, W = q.wx, ps = M + 1;
   C2* buf = sm;
@@ -129,10 +131,12 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
                                                      const double* __restrict__ lamx,
                                                      const double* __restrict__ lamy,
                                                      const double* __restrict__ lamz,
-                                                     typename Cx<R>::T* __restrict__ data) {
+                                                     typename Cx<R>::T* __restrict__ data,
+                                                     const int* __restrict__ gate) {
   using C2 = typename Cx<R>::T;
   extern __shared__ __align__(16) unsigned char smraw[];
   C2* sm = reinterpret_cast<C2*>(smraw);
+  if (*gate) return;
   const int L = axis == 1 ? q.ny : q.nz;
   const int lg = axis == 1 ? q.ly : q.lz;
   const int W = axis == 1 ? q.wy : q.wz;
@@ -216,11 +220,13 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q,
                                                    const typename Cx<R>::T* __restrict__ twM,
                                                    const typename Cx<R>::T* __restrict__ twN,
                                                    double* __restrict__ P,
-                                                   double* __restrict__ bmax) {
+                                                   double* __restrict__ bmax,
+                                                   const int* __restrict__ gate) {
   using C2 = typename Cx<R>::T;
   extern __shared__ __align__(16) unsigned char smraw[];
   C2* sm = reinterpret_cast<C2*>(smraw);
   __shared__ double red[FT];
+  if (*gate) return;
   const Dev& d = q.d;
   const int M = q.nx >> 1, W = q.wx, ps = M + 1;
   C2* buf = sm;
@@ -272,15 +278,21 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q,
   block_max_store(m, red, bmax + bid);
 }
 
-__global__ void k_max_final(const double* __restrict__ b, int n, double* __restrict__ out) {
+__global__ void k_max_final(const double* __restrict__ b, int n, double* __restrict__ out,
+                            int* __restrict__ gate, double eps) {
   __shared__ double red[FT];
+  if (gate && gate[0]) return;
   double m = 0.0;
   for (int t = threadIdx.x; t < n; t += FT) m = fmax(m, b[t]);
   block_max_store(m, red, out);
+  if (gate && threadIdx.x == 0) {
+    gate[1] += 1;
+    if (!(out[0] > eps)) gate[0] = 1;
+  }
 }
 
-int max_final(const double* b, int n, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(FT), 0, st, b, n, out);
+int max_final(const double* b, int n, double* out, hipStream_t st, int* gate, double eps) {
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(FT), 0, st, b, n, out, gate, eps);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
@@ -308,11 +320,17 @@ struct fr_box_poisson {
   double* hres = nullptr;
   int nbx = 0;
   size_t lds[3] = {};  // dynamic LDS bytes of the x, y and z passes
-  // HIP events around every launch of the forward x pass (residual + real
-  // FFT), read after the iteration's host synchronisation: summed ms, count
-  hipEvent_t tx0 = nullptr, tx1 = nullptr;
+  // HIP events bound to each launch of the forward x pass (residual + real
+  // FFT) of a batch of queued iterations, read after the batch's host
+  // synchronisation for the iterations that ran: summed ms, count
+  static constexpr int QB = 16;  // most iterations queued per synchronisation
+  hipEvent_t tx0[QB] = {}, tx1[QB] = {};
   double tx_ms = 0;
   long tx_n = 0;
+  // device convergence gate (done flag, iterations run) and its host copy
+  int* gate = nullptr;
+  int* hgate = nullptr;
+  int last_iters = 1;  // iterations of the previous solve: the next batch size
 };
 
 // One defect-correction iteration; leaves max|dp| in s->dres[0].  With inv
@@ -320,7 +338,8 @@ struct fr_box_poisson {
 // type of the spectral solve (the residual and P stay double).
 template <typename TC, typename R>
 static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz,
-                            bool inv, double* P, const double* RHS, hipStream_t st) {
+                            bool inv, double* P, const double* RHS, double eps, int slot,
+                            hipStream_t st) {
   using C2 = typename Cx<R>::T;
   const bool f = sizeof(R) == 4;
   PoisDev& q = f ? s->q32 : s->q;
@@ -332,23 +351,27 @@ static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const
   const size_t* lds = f ? s->lds32 : s->lds;
   const int nbx = f ? s->nbx32 : s->nbx;
   // the events are bound to the launch (start and end of the kernel itself)
+  const int* g = s->gate;
   if (inv)
-    hipExtLaunchKernelGGL((k_pass_x_fwd<true, TC, R>), dim3(nbx), dim3(FT), lds[0], st, s->tx0,
-                          s->tx1, 0, q, P, cx, cy, cz, RHS, twfm, twf[0], data);
+    hipExtLaunchKernelGGL((k_pass_x_fwd<true, TC, R>), dim3(nbx), dim3(FT), lds[0], st,
+                          s->tx0[slot], s->tx1[slot], 0, q, P, cx, cy, cz, RHS, twfm, twf[0],
+                          data, g);
   else
-    hipExtLaunchKernelGGL((k_pass_x_fwd<false, TC, R>), dim3(nbx), dim3(FT), lds[0], st, s->tx0,
-                          s->tx1, 0, q, P, cx, cy, cz, RHS, twfm, twf[0], data);
+    hipExtLaunchKernelGGL((k_pass_x_fwd<false, TC, R>), dim3(nbx), dim3(FT), lds[0], st,
+                          s->tx0[slot], s->tx1[slot], 0, q, P, cx, cy, cz, RHS, twfm, twf[0],
+                          data, g);
   const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.ny + q.wz - 1) / q.wz;
   hipLaunchKernelGGL(k_pass_strided<R>, dim3(ty * q.nz), dim3(FT), lds[1], st, q, 1, 0, twf[1],
-                     twf[1], s->lam[0], s->lam[1], s->lam[2], data);
+                     twf[1], s->lam[0], s->lam[1], s->lam[2], data, g);
   hipLaunchKernelGGL(k_pass_strided<R>, dim3(tz * q.nxh), dim3(FT), lds[2], st, q, 2, 1, twf[2],
-                     twi[2], s->lam[0], s->lam[1], s->lam[2], data);
+                     twi[2], s->lam[0], s->lam[1], s->lam[2], data, g);
   hipLaunchKernelGGL(k_pass_strided<R>, dim3(ty * q.nz), dim3(FT), lds[1], st, q, 1, 0, twi[1],
-                     twi[1], s->lam[0], s->lam[1], s->lam[2], data);
+                     twi[1], s->lam[0], s->lam[1], s->lam[2], data, g);
   hipLaunchKernelGGL(k_pass_x_inv<R>, dim3(nbx), dim3(FT), lds[0], st, q, data, twim, twi[0], P,
-                     s->bmax);
+                     s->bmax, g);
   FR_LAUNCH_CHECK();
-  int rc = max_final(s->bmax, nbx, s->dres, st);
+  // the periodic ghost copy of P below is idempotent, so it needs no gate
+  int rc = max_final(s->bmax, nbx, s->dres, st, s->gate, eps);
   return rc ? rc : ghost_fill(q.d, P, st);
 }
 
@@ -431,23 +454,37 @@ int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* c
   // the FP64 residual and P of the defect correction around it
   constexpr bool f32 = sizeof(TC) == 4;
   if (f32) FR_RET(ensure_f32(s));
-  int i = 0;
-  double error = 100;
-  while ((i < iterations) && (error > epsilon)) {
-    int rc = f32 ? box_poisson_iter<TC, float>(s, cx, cy, cz, inv, P, RHS, st)
-                 : box_poisson_iter<TC, double>(s, cx, cy, cz, inv, P, RHS, st);
-    if (rc) return rc;
-    FR_CHECK(hipMemcpyAsync(s->hres, s->dres, sizeof(double), hipMemcpyDeviceToHost, st));
-    FR_CHECK(hipStreamSynchronize(st));
-    error = s->hres[0];
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, s->tx0, s->tx1) == hipSuccess) {
-      s->tx_ms += ms;
-      s->tx_n++;
+  // poisson::FFT's loop (poisson.cpp:391-411): while (i < iterations &&
+  // error > epsilon).  Iterations are queued in batches (the first as long
+  // as the previous solve); the device gate stops the queued ones after the
+  // one whose max|dp| passes the test, so the iterations run and their
+  // results are those of the loop, with one host synchronisation per batch.
+  FR_CHECK(hipMemsetAsync(s->gate, 0, sizeof(int) * 2, st));
+  int launched = 0, ran = 0, batch = s->last_iters;
+  while (launched < iterations) {
+    const int k = std::min({batch, iterations - launched, (int)fr_box_poisson::QB});
+    for (int j = 0; j < k; j++) {
+      int rc = f32 ? box_poisson_iter<TC, float>(s, cx, cy, cz, inv, P, RHS, epsilon, j, st)
+                   : box_poisson_iter<TC, double>(s, cx, cy, cz, inv, P, RHS, epsilon, j, st);
+      if (rc) return rc;
     }
-    i++;
+    launched += k;
+    FR_CHECK(hipMemcpyAsync(s->hgate, s->gate, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+    FR_CHECK(hipStreamSynchronize(st));
+    const int ran_now = s->hgate[1] - ran;
+    ran = s->hgate[1];
+    for (int j = 0; j < ran_now; j++) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, s->tx0[j], s->tx1[j]) == hipSuccess) {
+        s->tx_ms += ms;
+        s->tx_n++;
+      }
+    }
+    if (s->hgate[0]) break;
+    batch = 1;
   }
-  if (num_iter) *num_iter = i;
+  s->last_iters = ran > 0 ? ran : 1;
+  if (num_iter) *num_iter = ran;
   return FR_OK;
 }
 template int box_poisson_solve<double>(fr_box_poisson*, const double*, const double*,
@@ -526,8 +563,12 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
   FR_CHECK(hipMalloc(&s->bmax, sizeof(double) * (s->nbx + 1)));
   FR_CHECK(hipMalloc(&s->dres, sizeof(double) * 2));
   FR_CHECK(hipHostMalloc(&s->hres, sizeof(double) * 2, hipHostMallocDefault));
-  FR_CHECK(hipEventCreate(&s->tx0));
-  FR_CHECK(hipEventCreate(&s->tx1));
+  for (int j = 0; j < fr_box_poisson::QB; j++) {
+    FR_CHECK(hipEventCreate(&s->tx0[j]));
+    FR_CHECK(hipEventCreate(&s->tx1[j]));
+  }
+  FR_CHECK(hipMalloc(&s->gate, sizeof(int) * 2));
+  FR_CHECK(hipHostMalloc(&s->hgate, sizeof(int) * 2, hipHostMallocDefault));
   *out = s;
   return FR_OK;
 }
@@ -550,8 +591,12 @@ int fr_box_poisson_destroy(fr_box_poisson* s) {
   (void)hipFree(s->bmax);
   (void)hipFree(s->dres);
   (void)hipHostFree(s->hres);
-  if (s->tx0) (void)hipEventDestroy(s->tx0);
-  if (s->tx1) (void)hipEventDestroy(s->tx1);
+  for (int j = 0; j < fr_box_poisson::QB; j++) {
+    if (s->tx0[j]) (void)hipEventDestroy(s->tx0[j]);
+    if (s->tx1[j]) (void)hipEventDestroy(s->tx1[j]);
+  }
+  (void)hipFree(s->gate);
+  (void)hipHostFree(s->hgate);
   delete s;
   return FR_OK;
 }
