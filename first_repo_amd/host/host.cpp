@@ -667,7 +667,6 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   FR_CHECK(hipEventRecord(evf, st));
   FR_CHECK(hipStreamWaitEvent(st2, evf, 0));
   FR_TRY(part.part2gas_concentration(C, st2));
-  FR_CHECK(hipEventRecord(evj, st2));
   const bool box = param.IsBox();
   const int* rk = nullptr;
   if (box) {
@@ -683,34 +682,41 @@ int grid::Update_Particle() {  // grid.cpp:659-680
     FR_TRY(part.part2gas_velocity(RHS_RU, rk));
   else
     RHS_RU.assign(0);
+  // Heat branch on st2 (gas temperature at the particles, buckets of the new
+  // positions, heat deposit, its sum): it follows the concentration deposit
+  // on st2 and needs the migrated positions and reference indices.  Nothing
+  // on st reads it before join_side() (Update_RU_WOP reads none of it).
   if (box) {
     FR_CHECK(hipStreamWaitEvent(st3, evm0, 0));
     FR_TRY(part.Send_Recv(st3));
     FR_CHECK(hipEventRecord(evm, st3));
-  }
-  FR_CHECK(hipStreamWaitEvent(st, evj, 0));
-  if (box) {
-    FR_CHECK(hipStreamWaitEvent(st, evm, 0));
+    // box: it starts as soon as the migration is done, beside the momentum
+    // deposit's tail and the momentum update
+    FR_CHECK(hipStreamWaitEvent(st2, evm, 0));
+    side_held = true;
+    FR_TRY(launch_side());
   } else {
+    // duct: the migration (outflow drop, inlet append) reorders the storage,
+    // so it follows both deposits on st; the branch's launches are issued
+    // after the momentum kernel's (launch_side from Update_RU_WOP), so that
+    // kernel is queued first
+    FR_CHECK(hipEventRecord(evj, st2));
+    FR_CHECK(hipStreamWaitEvent(st, evj, 0));
     FR_TRY(part.Send_Recv_duct(inlet.data(), (int)(inlet.size() / 29)));
     inlet.clear();
+    FR_CHECK(hipEventRecord(evf, st));
+    FR_CHECK(hipStreamWaitEvent(st2, evf, 0));
+    side_held = true;
   }
-  // Heat branch on st2, beside Update_RU_WOP (which reads none of it):
-  // gas temperature at the particles, buckets, heat deposit, its sum.  Its
-  // launches are issued after the momentum kernel's (launch_side), so that
-  // kernel is queued first; anything that needs the branch goes through
-  // join_side(), which issues it if it is still held.
-  FR_CHECK(hipEventRecord(evf, st));
-  side_held = true;
   side_pending = true;
   qmean_valid = false;
   return FR_OK;
 }
 
+// Issues the held heat branch on st2 (its stream waits are already queued).
 int grid::launch_side() {
   if (!side_held) return FR_OK;
   side_held = false;
-  FR_CHECK(hipStreamWaitEvent(st2, evf, 0));
   FR_TRY(part.temp(2, P0_new / param.R(), Rho_new, st2));
   FR_TRY(part.rebuild_cells(2, st2));
   FR_TRY(part.part2gas_Temp(2, RHS_Part_Temp, st2));
