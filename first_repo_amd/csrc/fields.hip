@@ -359,6 +359,30 @@ template int reduce_async_ws<double>(const Dev&, const double*, int, double*, do
                                      hipStream_t);
 template int reduce_async_ws<float>(const Dev&, const float*, int, double*, double*, hipStream_t);
 
+// (*this) -= mean (tensor0.cpp:437-441) with the mean kept on the device:
+// every element gets + (T)(-(sum/denom)), the value tensor0::add(-mean)
+// adds, with no host round trip.
+template <typename T>
+__global__ void k_add_neg_mean(long n, T* __restrict__ f, const double* __restrict__ dsum,
+                               double denom) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const T a = (T)(-(dsum[0] / denom));
+  f[t] += a;
+}
+
+template <typename T>
+int make_mean_zero_async(const Dev& d, T* f, long n, double* ws, hipStream_t s) {
+  FR_RET(reduce_async_ws(d, (const T*)f, 0, ws + RED_PARTS, ws, s));
+  const double denom = (double)(d.Nx - 2) * (d.Ny - 2) * (d.Nz - 2);
+  hipLaunchKernelGGL(k_add_neg_mean<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, n, f,
+                     (const double*)(ws + RED_PARTS), denom);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+template int make_mean_zero_async<double>(const Dev&, double*, long, double*, hipStream_t);
+template int make_mean_zero_async<float>(const Dev&, float*, long, double*, hipStream_t);
+
 template <typename T>
 int reduce_sync(const Dev& d, const T* f, int op, double* result, hipStream_t s) {
   int rc = red_ws();

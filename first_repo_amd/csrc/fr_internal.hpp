@@ -20,6 +20,10 @@ int reduce_sync(const Dev& d, const T* f, int op, double* result, hipStream_t s)
 constexpr int RED_PARTS = 1024;
 template <typename T>
 int reduce_async_ws(const Dev& d, const T* f, int op, double* dres, double* ws, hipStream_t s);
+// f -= mean(f) over n elements with the mean formed on the device (ws:
+// RED_PARTS + 1 doubles); the same values as reduce_sync + elementwise add
+template <typename T>
+int make_mean_zero_async(const Dev& d, T* f, long n, double* ws, hipStream_t s);
 int reducex(const Dev& d, const double* f, double* out, hipStream_t s);
 template <typename T>
 int elementwise(long n, int op, T* out, double a, const T* A, double b, const T* B, double c,

@@ -141,6 +141,9 @@ class tensor0 {
   double sum();
   double mean();
   void make_mean_zero();
+  // make_mean_zero with the mean kept on the device (ws: RED_PARTS + 1
+  // doubles), no host synchronisation
+  void make_mean_zero_async(double* ws);
   double max();
   double min();
   double max_abs();
@@ -183,6 +186,11 @@ class tensor1 {
     x.make_mean_zero();
     y.make_mean_zero();
     z.make_mean_zero();
+  }
+  void make_mean_zero_async(double* ws) {
+    x.make_mean_zero_async(ws);
+    y.make_mean_zero_async(ws);
+    z.make_mean_zero_async(ws);
   }
   double* const* ptrs3(double* out[3]) {
     out[0] = x.pointer();
@@ -288,6 +296,7 @@ class grid {
   bool side_held = false;  // branch recorded but its launches not yet issued
   int launch_side();
   double* dq = nullptr;  // reduction workspace + result of that branch
+  double* dmz = nullptr;  // workspace of the main stream's device-side means
   // join_side() waits for the branch on the device and copies its sum to
   // pinned host memory (hq) behind an event; settle_side() also takes the
   // mean on the host, waiting for that copy only if it has not landed

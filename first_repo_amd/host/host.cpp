@@ -193,6 +193,12 @@ void tensor0::make_mean_zero() {  // tensor0.cpp:437-441: (*this) -= mean
   double m = mean();
   add(-m);
 }
+void tensor0::make_mean_zero_async(double* ws) {
+  const Dev d = p_->dev();
+  const long n = p_->size();
+  FR_T0_DISPATCH(fr::make_mean_zero_async(d, p<float>(), n, ws, st_),
+                 fr::make_mean_zero_async(d, p<double>(), n, ws, st_));
+}
 double tensor0::max() { return reduce(2); }
 double tensor0::min() { return reduce(3); }
 double tensor0::max_abs() { return reduce(4); }
@@ -437,6 +443,7 @@ grid::~grid() {
   if (st3) hipStreamDestroy(st3);
   if (evq) hipEventDestroy(evq);
   if (dq) hipFree(dq);
+  if (dmz) hipFree(dmz);
   if (hq) hipHostFree(hq);
   if (st2) hipStreamDestroy(st2);
   if (st) hipStreamDestroy(st);
@@ -490,6 +497,7 @@ int grid::setup(const char* params_path, const char* ic) {
   FR_TRY(PS_.init(&param, &size, st));
   FR_CHECK(hipMalloc(&dred, sizeof(double) * (4 * (size_t)size.Nx() + 512)));
   FR_CHECK(hipMalloc(&dq, sizeof(double) * (RED_PARTS + 8)));
+  FR_CHECK(hipMalloc(&dmz, sizeof(double) * (RED_PARTS + 8)));
   FR_CHECK(hipHostMalloc(&hq, sizeof(double) * 2, hipHostMallocDefault));
   return FR_OK;
 }
@@ -874,7 +882,7 @@ int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
     const double* rfc[3] = {rf[0], rf[1], rf[2]};
     FR_TRY(duct_rhs(size.dev(), q.s, rn, rfc, divergence.pointer(), RHS_Pois.pointer(), st));
   }
-  RHS_Pois.make_mean_zero();
+  RHS_Pois.make_mean_zero_async(dmz);
   return FR_OK;
 }
 
@@ -917,11 +925,11 @@ int grid::TimeAdvance_RK4() {
 
 int grid::Correct_mean_RU() {  // grid.cpp:565-577
   if (param.IsBox()) {
-    RU_np1.make_mean_zero();
+    RU_np1.make_mean_zero_async(dmz);
     RU_np1.x.add(param.Rho0() * param.U0());
   } else {
-    RU_np1.y.make_mean_zero();
-    RU_np1.z.make_mean_zero();
+    RU_np1.y.make_mean_zero_async(dmz);
+    RU_np1.z.make_mean_zero_async(dmz);
   }
   return FR_OK;
 }
