@@ -163,6 +163,54 @@ def test_duct_poisson_matches_oracle(n):
     assert np.array_equal(host(dP), ref), rel(host(dP), ref)
 
 
+@pytest.mark.parametrize("box", [True, False], ids=["box", "duct"])
+def test_poisson_batched_solves_match_oracle(box):
+    """One solver object, several solves: after the first, iterations are
+    queued in batches as long as the previous solve and stopped on the
+    device by the convergence test.  A looser tolerance (the batch overshoots
+    the count), an iteration cap below convergence and a repeat of the first
+    solve must each give the oracle's loop: the same count and P bit for
+    bit."""
+    import ctypes as C
+    L = fr.lib()
+    n = (16, 8, 32) if box else (48, 32, 16)
+    g = fr.make_geom(*n, L=(3.0, 1.0, 2 * np.pi), box=box)
+    og = OL.Geom.from_buffer_copy(g)
+    shp = (g.Nz, g.Ny, g.Nx)
+    rng = np.random.default_rng(11)
+    O = OL.orc()
+    rho = 1.0 + 1.5 * rng.random(shp)
+    O.orc_update_ghosts(og, rho, 0)
+    C3 = []
+    for axis in range(3):
+        rf = np.ones(shp)
+        O.orc_interp(og, rf, rho, axis, 0)
+        C3.append(1.0 / rf)
+    rhs = rng.standard_normal(shp)
+    rhs -= rhs[1:-1, 1:-1, 1:-1].mean()
+    create = L.fr_box_poisson_create if box else L.fr_duct_poisson_create
+    solve = L.fr_box_poisson_solve if box else L.fr_duct_poisson_solve
+    destroy = L.fr_box_poisson_destroy if box else L.fr_duct_poisson_destroy
+    h = C.c_void_p()
+    fr.check(create(g, C.byref(h)))
+    dC = [dbuf(c) for c in C3]
+    dR = dbuf(rhs)
+    counts = []
+    for maxit, eps in [(30, 1e-10), (30, 1e-5), (3, 1e-12), (30, 1e-10)]:
+        print(f"solve maxit={maxit} eps={eps}", flush=True)
+        ref = np.zeros(shp)
+        it_ref = O.orc_poisson(og, C3[0], C3[1], C3[2], ref, rhs, maxit, eps)
+        dP = dbuf(np.zeros(shp))
+        it = C.c_int()
+        fr.check(solve(h, dptr(dC[0]), dptr(dC[1]), dptr(dC[2]), dptr(dP), dptr(dR), maxit, eps,
+                       C.byref(it), None))
+        assert it.value == it_ref, (maxit, eps, it.value, it_ref)
+        assert np.array_equal(host(dP), ref), (maxit, eps, rel(host(dP), ref))
+        counts.append(it.value)
+    destroy(h)
+    assert counts[2] == 3 and counts[1] < counts[0] == counts[3]
+
+
 @pytest.fixture(scope="module")
 def case(tmp_path_factory):
     return make_box_case(tmp_path_factory.mktemp("box"), n=16, n_particles=3000)
