@@ -337,12 +337,8 @@ int box_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w, const T
                        ru[2], ru_np1[0], ru_np1[1], ru_np1[2], ru_new[0], ru_new[1], ru_new[2],
                        rf[0], rf[1], rf[2]);
   FR_LAUNCH_CHECK();
-  for (int m = 0; m < 3; m++) {
-    int rc = ghost_fill(d, ru_np1[m], s);
-    if (!rc) rc = ghost_fill(d, ru_new[m], s);
-    if (rc) return rc;
-  }
-  return FR_OK;
+  T* const out6[6] = {ru_np1[0], ru_new[0], ru_np1[1], ru_new[1], ru_np1[2], ru_new[2]};
+  return ghost_fill_n(d, out6, 6, s);
 }
 
 template <typename T>

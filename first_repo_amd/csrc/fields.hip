@@ -101,10 +101,9 @@ static int launch_stencil(const fr_geom* g, T* out, const typename nodeduce<T>::
 }
 
 // Ghost shell only: k faces (all i,j), j faces (interior k, all i), and for
-// the box the i faces (interior j,k).
+// the box the i faces (interior j,k).  Element t of the shell of f.
 template <typename T>
-__global__ void k_shell(Dev d, T* __restrict__ f) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void shell_one(const Dev& d, T* __restrict__ f, long t) {
   const long nk = 2 * d.NxNy;
   const long nj = 2L * d.Nx * (d.Nz - 2);
   const long ni = d.box ? 2L * (d.Ny - 2) * (d.Nz - 2) : 0;
@@ -133,12 +132,34 @@ __global__ void k_shell(Dev d, T* __restrict__ f) {
   f[at(d, i, j, k)] = f[at(d, ii, img(j, d.Ny), img(k, d.Nz))];
 }
 
+// Up to FMAX fields per launch (blockIdx.y selects the field).
+constexpr int FMAX = 6;
 template <typename T>
-int ghost_fill(const Dev& d, T* f, hipStream_t s) {
+struct FieldSet {
+  T* f[FMAX];
+};
+
+template <typename T>
+__global__ void k_shell(Dev d, FieldSet<T> fs) {
+  shell_one(d, fs.f[blockIdx.y], (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T>
+int ghost_fill_n(const Dev& d, T* const* f, int nf, hipStream_t s) {
+  if (nf < 1 || nf > FMAX) return FR_EINVAL;
+  FieldSet<T> fs{};
+  for (int q = 0; q < nf; q++) fs.f[q] = f[q];
   long n = 2 * d.NxNy + 2L * d.Nx * (d.Nz - 2) + (d.box ? 2L * (d.Ny - 2) * (d.Nz - 2) : 0);
-  hipLaunchKernelGGL(k_shell<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, d, f);
+  hipLaunchKernelGGL(k_shell<T>, dim3(nblocks(n, 256), nf), dim3(256), 0, s, d, fs);
   FR_LAUNCH_CHECK();
   return FR_OK;
+}
+template int ghost_fill_n<double>(const Dev&, double* const*, int, hipStream_t);
+template int ghost_fill_n<float>(const Dev&, float* const*, int, hipStream_t);
+
+template <typename T>
+int ghost_fill(const Dev& d, T* f, hipStream_t s) {
+  return ghost_fill_n(d, &f, 1, s);
 }
 template int ghost_fill<double>(const Dev&, double*, hipStream_t);
 template int ghost_fill<float>(const Dev&, float*, hipStream_t);
@@ -148,7 +169,8 @@ template int ghost_fill<float>(const Dev&, float*, hipStream_t);
 // j,k).  Each thread reads both ghosts of its line before adding, as the
 // reference packs both send buffers before unpacking.
 template <typename T>
-__global__ void k_cum_z(Dev d, T* __restrict__ f) {
+__global__ void k_cum_z(Dev d, FieldSet<T> fs) {
+  T* __restrict__ f = fs.f[blockIdx.y];
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= d.NxNy) return;
   const long top = (long)(d.Nz - 1) * d.NxNy + t, bot = t;
@@ -157,7 +179,8 @@ __global__ void k_cum_z(Dev d, T* __restrict__ f) {
   f[d.NxNy + t] += s0;
 }
 template <typename T>
-__global__ void k_cum_y(Dev d, T* __restrict__ f) {
+__global__ void k_cum_y(Dev d, FieldSet<T> fs) {
+  T* __restrict__ f = fs.f[blockIdx.y];
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long n = (long)d.Nx * (d.Nz - 2);
   if (t >= n) return;
@@ -167,7 +190,8 @@ __global__ void k_cum_y(Dev d, T* __restrict__ f) {
   f[at(d, i, 1, k)] += s0;
 }
 template <typename T>
-__global__ void k_cum_x(Dev d, T* __restrict__ f) {
+__global__ void k_cum_x(Dev d, FieldSet<T> fs) {
+  T* __restrict__ f = fs.f[blockIdx.y];
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long n = (long)(d.Ny - 2) * (d.Nz - 2);
   if (t >= n) return;
@@ -178,15 +202,25 @@ __global__ void k_cum_x(Dev d, T* __restrict__ f) {
 }
 
 template <typename T>
-int cum_fold(const Dev& d, T* f, hipStream_t s) {
-  hipLaunchKernelGGL(k_cum_z<T>, dim3(nblocks(d.NxNy, 256)), dim3(256), 0, s, d, f);
-  hipLaunchKernelGGL(k_cum_y<T>, dim3(nblocks((long)d.Nx * (d.Nz - 2), 256)), dim3(256), 0, s,
-                     d, f);
+int cum_fold_n(const Dev& d, T* const* f, int nf, hipStream_t s) {
+  if (nf < 1 || nf > FMAX) return FR_EINVAL;
+  FieldSet<T> fs{};
+  for (int q = 0; q < nf; q++) fs.f[q] = f[q];
+  hipLaunchKernelGGL(k_cum_z<T>, dim3(nblocks(d.NxNy, 256), nf), dim3(256), 0, s, d, fs);
+  hipLaunchKernelGGL(k_cum_y<T>, dim3(nblocks((long)d.Nx * (d.Nz - 2), 256), nf), dim3(256), 0,
+                     s, d, fs);
   if (d.box)
-    hipLaunchKernelGGL(k_cum_x<T>, dim3(nblocks((long)(d.Ny - 2) * (d.Nz - 2), 256)),
-                       dim3(256), 0, s, d, f);
+    hipLaunchKernelGGL(k_cum_x<T>, dim3(nblocks((long)(d.Ny - 2) * (d.Nz - 2), 256), nf),
+                       dim3(256), 0, s, d, fs);
   FR_LAUNCH_CHECK();
   return FR_OK;
+}
+template int cum_fold_n<double>(const Dev&, double* const*, int, hipStream_t);
+template int cum_fold_n<float>(const Dev&, float* const*, int, hipStream_t);
+
+template <typename T>
+int cum_fold(const Dev& d, T* f, hipStream_t s) {
+  return cum_fold_n(d, &f, 1, s);
 }
 template int cum_fold<double>(const Dev&, double*, hipStream_t);
 template int cum_fold<float>(const Dev&, float*, hipStream_t);

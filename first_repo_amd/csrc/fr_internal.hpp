@@ -12,6 +12,11 @@ template <typename T>
 int ghost_fill(const Dev& d, T* f, hipStream_t s);
 template <typename T>
 int cum_fold(const Dev& d, T* f, hipStream_t s);
+// the same for nf <= 6 independent fields in one launch per phase
+template <typename T>
+int ghost_fill_n(const Dev& d, T* const* f, int nf, hipStream_t s);
+template <typename T>
+int cum_fold_n(const Dev& d, T* const* f, int nf, hipStream_t s);
 template <typename T>
 int reduce_async(const Dev& d, const T* f, int op, double* dres, hipStream_t s);
 template <typename T>

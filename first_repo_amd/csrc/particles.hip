@@ -1146,12 +1146,8 @@ int PartDev::deposit_vel(const Dev& d, F* const out[3], hipStream_t st) {
   hipLaunchKernelGGL(k_dep_vel<F>, grid3p(d), dim3(64, 4), 0, st, d, T.start, T.list, T.s[0],
                      T.s[1], T.s[2], T.s[3], T.s[4], T.s[5], out[0], out[1], out[2]);
   FR_LAUNCH_CHECK();
-  for (int a = 0; a < 3; a++) {
-    int rc = cum_fold(d, out[a], st);
-    if (!rc) rc = ghost_fill(d, out[a], st);
-    if (rc) return rc;
-  }
-  return FR_OK;
+  int rc = cum_fold_n(d, out, 3, st);  // the three components, one launch per phase
+  return rc ? rc : ghost_fill_n(d, out, 3, st);
 }
 
 // particle::Send_Recv on one rank: axes z, y, x (particle.cpp:853-858).  The
