@@ -411,6 +411,27 @@ __global__ void k_pcg_axpy(Fine F, int op, double s, double* __restrict__ x,
   }
 }
 
+// k_pcg_axpy with the coefficient formed on the device from dot products in
+// device memory: s = (num - sub) / den (sub optional), or 0 when zero is set
+// (the host's alpha = rz/pq, beta = rz2/rz or (rz2 - zro)/rz, in double).
+__global__ void k_pcg_axpy_d(Fine F, int op, const double* __restrict__ num,
+                             const double* __restrict__ den, const double* __restrict__ sub,
+                             int zero, double* __restrict__ x, double* __restrict__ r,
+                             double* __restrict__ p, const double* __restrict__ q) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.n) return;
+  const double s = zero ? 0.0 : (sub ? (num[0] - sub[0]) : num[0]) / den[0];
+  int i, j, k;
+  tijk(t, F.nx, F.ny, i, j, k);
+  const long c = at(F.d, i, j, k);
+  if (op == 1) {
+    x[c] = x[c] + s * p[c];
+    r[c] = r[c] - s * q[c];
+  } else {
+    p[c] = q[c] + s * p[c];
+  }
+}
+
 __global__ void k_neumann_x2(Dev d, double* __restrict__ P) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)d.Ny * d.Nz) return;
@@ -430,7 +451,10 @@ struct fr_pcg {
   std::vector<MGL> L;
   double *r = nullptr, *z = nullptr, *p = nullptr;  // ghosted fine fields
   double* part = nullptr;
-  double* dres = nullptr;   // [0] scalar results
+  // scalar results: [0] norms read by the host, [1] the V-cycle's sum, [2]
+  // (p, S p), [3] and [4] (r, z) of the current and next iteration, [5]
+  // (r_old, z)
+  double* dres = nullptr;
   double* hres = nullptr;   // pinned host copy
   int pre = 2, post = 2, coarse_sweeps = 20;
   int l_small = -1;  // first level handled by the one-workgroup kernel
@@ -633,21 +657,21 @@ int pcg_solve(fr_pcg* s, const double* rfx, const double* rfy, const double* rfz
       return rc;
     hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(256), 0, st, F, 0, 0.0, P, s->r, s->p,
                        (const double*)s->z);
-    double rz;
-    dsum(s, 0, s->r, s->z, nullptr, 0, st);
-    if ((rc = read_scalar(s, 0, &rz, st))) return rc;
+    // (r, z) stays on the device (slots 3 and 4 alternate); alpha and beta
+    // are formed there, so the host reads only (r, r) for the stopping test
+    int rz_slot = 3;
+    dsum(s, 0, s->r, s->z, nullptr, rz_slot, st);
     while (it < maxit) {
       if ((rc = fill_fine_ghosts(s, s->p, st))) return rc;
       // q = S p in z's storage
       hipLaunchKernelGGL(k_pcg_apply, dim3(nb), dim3(256), 0, st, F, 1, (const double*)s->p, RHS,
                          rfx, rfy, rfz, s->z);
-      double pq;
-      dsum(s, 0, s->p, s->z, nullptr, 0, st);
-      if ((rc = read_scalar(s, 0, &pq, st))) return rc;
-      const double alpha = rz / pq;
+      dsum(s, 0, s->p, s->z, nullptr, 2, st);  // pq
       if (flex) FR_CHECK(hipMemcpyAsync(s->rold, s->r, gbytes, hipMemcpyDeviceToDevice, st));
-      hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(256), 0, st, F, 1, alpha, P, s->r, s->p,
-                         (const double*)s->z);
+      // alpha = rz / pq
+      hipLaunchKernelGGL(k_pcg_axpy_d, dim3(nb), dim3(256), 0, st, F, 1,
+                         (const double*)(s->dres + rz_slot), (const double*)(s->dres + 2),
+                         (const double*)nullptr, 0, P, s->r, s->p, (const double*)s->z);
       it++;
       const double rr_prev = rr;
       dsum(s, 0, s->r, s->r, nullptr, 0, st);
@@ -661,20 +685,16 @@ int pcg_solve(fr_pcg* s, const double* rfx, const double* rfy, const double* rfz
       }
       if ((rc = flex ? precond_net(s, rfx, rfy, rfz, std::sqrt(rr / F.n), st) : precond(s, st)))
         return rc;
-      double rz2;
-      dsum(s, 0, s->r, s->z, nullptr, 0, st);
-      if ((rc = read_scalar(s, 0, &rz2, st))) return rc;
-      double beta = rz2 / rz;
-      if (flex) {  // Polak-Ribiere: (z, r_new - r_old) / rz
-        double zro;
-        dsum(s, 0, s->rold, s->z, nullptr, 0, st);
-        if ((rc = read_scalar(s, 0, &zro, st))) return rc;
-        beta = (rz2 - zro) / rz;
-      }
-      if (restart) beta = 0.0;
-      rz = rz2;
-      hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(256), 0, st, F, 2, beta, P, s->r, s->p,
-                         (const double*)s->z);
+      const int rz2_slot = 7 - rz_slot;
+      dsum(s, 0, s->r, s->z, nullptr, rz2_slot, st);
+      // beta = rz2 / rz; Polak-Ribiere in the flexible form:
+      // (z, r_new - r_old) / rz = (rz2 - (r_old, z)) / rz; 0 at a restart
+      if (flex) dsum(s, 0, s->rold, s->z, nullptr, 5, st);
+      hipLaunchKernelGGL(k_pcg_axpy_d, dim3(nb), dim3(256), 0, st, F, 2,
+                         (const double*)(s->dres + rz2_slot), (const double*)(s->dres + rz_slot),
+                         flex ? (const double*)(s->dres + 5) : (const double*)nullptr,
+                         restart ? 1 : 0, P, s->r, s->p, (const double*)s->z);
+      rz_slot = rz2_slot;
     }
   }
   if (collecting) {
@@ -729,7 +749,7 @@ int fr_pcg_create(const fr_geom* g, fr_pcg** out) {
     FR_CHECK(hipMemset(*pp, 0, sizeof(double) * gs));
   }
   FR_CHECK(hipMalloc(&s->part, sizeof(double) * DNB));
-  FR_CHECK(hipMalloc(&s->dres, sizeof(double) * 4));
+  FR_CHECK(hipMalloc(&s->dres, sizeof(double) * 8));
   FR_CHECK(hipHostMalloc(&s->hres, sizeof(double) * 2, hipHostMallocDefault));
   *out = s;
   return FR_OK;
