@@ -348,5 +348,24 @@ class DuctWorkload:
         b[:, 28] = first_flag + np.arange(n)
         return b
 
+    def fill_particles(self, n: int, first_flag: int = 0) -> np.ndarray:
+        """n particles as the inlet would have delivered them over the last
+        n / inject_per_step steps, moving with the stream: x uniform over
+        the distance the stream covers in that time (capped at Lx), y and z
+        uniform, velocity u_in along x, T = T0, all four stages equal (ToBuf
+        layout, 29 values).  Handed over with the first step's inlet
+        particles, they load the duct to its working particle count."""
+        dt = self.params().dt
+        xf = min(self.L[0], n / self.inject_per_step * dt * self.u_in)
+        rng = np.random.default_rng((self.seed, 99))
+        b = np.zeros((n, 29))
+        pos = [rng.random(n) * xf, rng.random(n) * self.L[1], rng.random(n) * self.L[2]]
+        vals = pos + [np.full(n, self.u_in), np.zeros(n), np.zeros(n), np.ones(n)]
+        for q, v in enumerate(vals):
+            for s in range(4):
+                b[:, q * 4 + s] = v
+        b[:, 28] = first_flag + np.arange(n)
+        return b
+
 
 WORKLOADS["duct512"] = DuctWorkload()

@@ -336,13 +336,37 @@ __device__ __forceinline__ long bucket_cell(const Dev& d, int mode, double x, do
   return clampi((long)kk * d.NxNy + (long)jj * d.Nx + ii, d.size);
 }
 
-__global__ void k_bucket_count(Dev d, PartPtrs p, int st, int mode, int* __restrict__ cnt,
-                               int* __restrict__ key, int* __restrict__ slot) {
+// Storage is in cell order (re-sorted every step), so neighbouring lanes
+// mostly share a cell: each run of equal cells in consecutive lanes takes its
+// slots with one atomic (its first lane's), the others offset from it.  Slots
+// only have to be distinct within a cell; the placement orders by reference
+// index.  256-thread blocks of 64-lane waves; the valid lanes of a wave are a
+// prefix (np ends the last wave).
+__global__ void __launch_bounds__(256) k_bucket_count(Dev d, PartPtrs p, int st, int mode,
+                                                      int* __restrict__ cnt,
+                                                      int* __restrict__ key,
+                                                      int* __restrict__ slot) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.np) return;
-  const long c = bucket_cell(d, mode, p.s[0][st][i], p.s[1][st][i], p.s[2][st][i]);
-  key[i] = (int)c;
-  slot[i] = atomicAdd(&cnt[c], 1);
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < p.np;
+  int c = -1;
+  if (valid) c = (int)bucket_cell(d, mode, p.s[0][st][i], p.s[1][st][i], p.s[2][st][i]);
+  const int prev = __shfl_up(c, 1);
+  const bool head = valid && (lane == 0 || prev != c);
+  const unsigned long long heads = __ballot(head);
+  const int nvalid = __popcll(__ballot(valid));
+  const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+  const int leader = valid ? 63 - __clzll(heads & upto) : 0;
+  int base = 0;
+  if (head) {
+    const unsigned long long after = heads & ~upto;
+    const int end = min(after ? __ffsll((long long)after) - 1 : 64, nvalid);
+    base = atomicAdd(&cnt[c], end - lane);
+  }
+  base = __shfl(base, leader);
+  if (!valid) return;
+  key[i] = c;
+  slot[i] = base + (lane - leader);
 }
 
 // Buckets hold reference indices (ranks).
