@@ -398,15 +398,15 @@ def bench_duct(args):
         sm.call("Finish_Initialize")
         return sm
 
-    fill = w.fill_particles(args.duct_fill) if args.duct_fill > 0 else None
-
     def run(sm, nsteps, timed_from):
+        # the load stays within the capacity over the run's own inlet particles
+        nfill = max(0, min(args.duct_fill, w.capacity - nsteps * w.inject_per_step))
         flag0 = 0
         acc = [0.0, 0.0, 0.0, 0.0]  # wall, iters, poisson ms, particle ms
         for s in range(nsteps):
             inlet = w.inlet_particles(s, flag0)
-            if s == 0 and fill is not None:  # load the duct (untimed: a warm-up step)
-                inlet = np.concatenate([fill, inlet])
+            if s == 0 and nfill > 0:  # load the duct (untimed: a warm-up step)
+                inlet = np.concatenate([w.fill_particles(nfill), inlet])
                 inlet[:, 28] = np.arange(len(inlet))
             flag0 += len(inlet)
             sm.set_inflow(w.inflow(s), inlet)
@@ -460,6 +460,8 @@ def bench_duct(args):
     acc = run(sim, args.warmup + args.steps, args.warmup)
     wall, iters, t_poisson, t_part = acc
     ncell = w.n[0] * w.n[1] * w.n[2]
+    nfill_timed = max(0, min(args.duct_fill,
+                             w.capacity - (args.warmup + args.steps) * w.inject_per_step))
     diag = sim.diagnostics()
     out = {
         "metric": METRIC,
@@ -479,8 +481,8 @@ def bench_duct(args):
                                 if args.solver == "fft" else
                                 "PCG + FP32 multigrid pressure solve to ||r||/||b||<=1e-10") +
                                ", 2e4 inlet particles/step" +
-                               (f", loaded with {args.duct_fill} particles at the first step"
-                                if args.duct_fill > 0 else ""),
+                               (f", loaded with {nfill_timed} particles at the first step"
+                                if nfill_timed > 0 else ""),
                    "cells": ncell, "particles_end": sim.np_,
                    "particle_steps_per_s": round(sim.np_ * args.steps / wall, 1),
                    "poisson_iters_per_substep": round(iters / args.steps / 4, 3),
