@@ -182,6 +182,8 @@ class PartDev {
   int *evt = nullptr, *islo = nullptr, *plo = nullptr, *dst = nullptr, *mslot = nullptr,
       *mlv = nullptr;
   int* hbuf = nullptr;
+  // tiles of the trilinear deposits left to the dense kernel (count, ids)
+  int *dense_c = nullptr, *dense_w = nullptr;
   double* inj_buf = nullptr;
   int inj_cap = 0;
   bool flg_zero = false;  // the migration flags flg[0..cap] are all zero

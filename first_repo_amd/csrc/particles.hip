@@ -472,13 +472,23 @@ struct Merge {
         best = head[q];
         wb = q;
       }
+    // the chosen range's cursor by selects, so that each step issues one
+    // load (a load per range under divergent lanes would serialise them)
+    int c = 0, e = 0;
 #pragma unroll
     for (int q = 0; q < NB; q++)
       if (q == wb) {
-        pos = cur[q];
-        cur[q]++;
-        head[q] = cur[q] < end[q] ? list[cur[q]] : INT_MAX;
+        c = cur[q];
+        e = end[q];
       }
+    const int h = c + 1 < e ? list[c + 1] : INT_MAX;
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+      if (q == wb) {
+        cur[q] = c + 1;
+        head[q] = h;
+      }
+    pos = c;
     return wb;
   }
 };
@@ -505,6 +515,37 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wtot, int* total) {
   return base + inc - v;
 }
 
+// Per-target form of the trilinear deposit: an 8-way merge of the target's
+// source buckets (a..a+1, b..b+1, c..c+1) in ascending particle order.
+__device__ __forceinline__ double dep_cic_merge(const int (*ss)[DSR], const int* __restrict__ list,
+                                                const double* __restrict__ sxl,
+                                                const double* __restrict__ syl,
+                                                const double* __restrict__ szl,
+                                                const double* __restrict__ sw) {
+  Merge<8> m;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int r = threadIdx.y + ((q >> 1) & 1) + (DTY + 1) * (q >> 2), j = threadIdx.x + (q & 1);
+    m.cur[q] = ss[r][j];
+    m.end[q] = ss[r][j + 1];
+  }
+  m.init(list);
+  double acc = 0.0;
+  for (;;) {
+    int pos;
+    const int q = m.next(list, pos);
+    if (q < 0) break;
+    const double x_l = sxl[pos], y_l = syl[pos], z_l = szl[pos];
+    const double wx = (q & 1) ? (1 - x_l) : x_l;
+    const double wy = ((q >> 1) & 1) ? (1 - y_l) : y_l;
+    const double wz = (q >> 2) ? (1 - z_l) : z_l;
+    double t = wx * wy * wz;
+    if (sw) t = t * sw[pos];
+    acc = acc + t;
+  }
+  return acc;
+}
+
 // Trilinear deposit of particle.cpp:526-559 (sw == nullptr, concentration)
 // or 586-658 (weight RHS_T): raw per-cell sums for every ghosted cell before
 // the cumulative halo, in ascending particle order (the reference's scatter
@@ -518,7 +559,8 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wtot, int* total) {
 // from the bucket starts, offsets by a block scan); each contribution's
 // place in its segment is the number of smaller reference indices there;
 // each target then adds its segment in that order.  The same sum as the
-// reference's loop, with no per-target merge.
+// reference's loop, with no per-target merge.  A tile whose list exceeds
+// PCAP is left to k_dep_cic_dense: its id is appended to dl (dl[0] counts).
 template <typename T>
 __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ start,
                                                  const int* __restrict__ list,
@@ -526,7 +568,7 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
                                                  const double* __restrict__ syl,
                                                  const double* __restrict__ szl,
                                                  const double* __restrict__ sw,
-                                                 T* __restrict__ out) {
+                                                 T* __restrict__ out, int* __restrict__ dl) {
   __shared__ int ss[DROWS][DSR];
   __shared__ int seg[DTX * DTY + 1];
   __shared__ int cur[DTX * DTY];
@@ -621,29 +663,118 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
     }
     return;
   }
-  if (!live) return;
-  Merge<8> m;
+  // a dense tile: its id goes to the list of k_dep_cic_dense
+  if (tid == 0) dl[1 + atomicAdd(dl, 1)] = (int)(bk.x + gridDim.x * (bk.y + gridDim.y * bk.z));
+}
+
+// Dense tiles of k_dep_cic (ids in dl[1..dl[0]], grid-stride): the particles
+// of the tile's ten source rows are staged in LDS in bucket order (reference
+// index and operands), and every target merges its eight source buckets
+// there in ascending reference index.  A tile with more than DCAP source
+// particles merges from global memory.
+constexpr int DCAP = 2048;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dep_cic_dense(Dev d, const int* __restrict__ start,
+                                                       const int* __restrict__ list,
+                                                       const double* __restrict__ sxl,
+                                                       const double* __restrict__ syl,
+                                                       const double* __restrict__ szl,
+                                                       const double* __restrict__ sw,
+                                                       T* __restrict__ out,
+                                                       const int* __restrict__ dl, int nbx,
+                                                       int nby) {
+  __shared__ int ss[DROWS][DSR];
+  __shared__ int prk[DCAP];
+  __shared__ double px[DCAP], py[DCAP], pz[DCAP], pw[DCAP];
+  const int ntile = dl[0];
+  const int tid = threadIdx.y * DTX + threadIdx.x;
+  for (int it = blockIdx.x; it < ntile; it += gridDim.x) {
+    const int id = dl[1 + it];
+    const int a0 = (id % nbx) * DTX, b0 = ((id / nbx) % nby) * DTY, c = id / (nbx * nby);
+    __syncthreads();  // the previous tile is done with ss and the staged particles
+    stage_starts(d, start, a0, b0, c, ss);
+    const int a = a0 + threadIdx.x, b = b0 + threadIdx.y;
+    const bool live = a < d.Nx && b < d.Ny;
+    // LDS position of bucket entry u of row r: u + boff[r]
+    int rpre[DROWS + 1], boff[DROWS];
+    rpre[0] = 0;
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int r = threadIdx.y + ((q >> 1) & 1) + (DTY + 1) * (q >> 2), j = threadIdx.x + (q & 1);
-    m.cur[q] = ss[r][j];
-    m.end[q] = ss[r][j + 1];
+    for (int r = 0; r < DROWS; r++) {
+      rpre[r + 1] = rpre[r] + (ss[r][DSR - 1] - ss[r][0]);
+      boff[r] = rpre[r] - ss[r][0];
+    }
+    const int npart = rpre[DROWS];
+    if (npart > DCAP) {  // block-uniform
+      if (live) out[at(d, a, b, c)] = (T)dep_cic_merge(ss, list, sxl, syl, szl, sw);
+      continue;
+    }
+    // SU particles per thread per round, all loads issued before the stores
+    constexpr int SU = 8;
+    for (int f0 = tid; f0 < npart; f0 += SU * DTX * DTY) {
+      int uu[SU];
+#pragma unroll
+      for (int e = 0; e < SU; e++) {
+        const int f = f0 + e * DTX * DTY;
+        int r = 0, base = 0;
+#pragma unroll
+        for (int k = 1; k < DROWS; k++) r += f >= rpre[k];
+#pragma unroll
+        for (int k = 0; k < DROWS; k++)
+          if (k == r) base = rpre[k];
+        uu[e] = f < npart ? ss[r][0] + (f - base) : -1;
+      }
+      int vr[SU];
+      double vx[SU], vy[SU], vz[SU], vw[SU];
+#pragma unroll
+      for (int e = 0; e < SU; e++) {
+        if (uu[e] < 0) continue;
+        vr[e] = list[uu[e]];
+        vx[e] = sxl[uu[e]];
+        vy[e] = syl[uu[e]];
+        vz[e] = szl[uu[e]];
+        if (sw) vw[e] = sw[uu[e]];
+      }
+#pragma unroll
+      for (int e = 0; e < SU; e++) {
+        if (uu[e] < 0) continue;
+        const int f = f0 + e * DTX * DTY;
+        prk[f] = vr[e];
+        px[f] = vx[e];
+        py[f] = vy[e];
+        pz[f] = vz[e];
+        if (sw) pw[f] = vw[e];
+      }
+    }
+    __syncthreads();
+    if (!live) continue;
+    Merge<8> m;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int r = threadIdx.y + ((q >> 1) & 1) + (DTY + 1) * (q >> 2), j = threadIdx.x + (q & 1);
+      int o = 0;
+#pragma unroll
+      for (int k = 0; k < DROWS; k++)
+        if (k == r) o = boff[k];
+      m.cur[q] = ss[r][j] + o;
+      m.end[q] = ss[r][j + 1] + o;
+    }
+    m.init(prk);
+    double acc = 0.0;
+    for (;;) {
+      int pos;
+      const int q = m.next(prk, pos);
+      if (q < 0) break;
+      const double x_l = px[pos], y_l = py[pos], z_l = pz[pos];
+      const double wx = (q & 1) ? (1 - x_l) : x_l;
+      const double wy = ((q >> 1) & 1) ? (1 - y_l) : y_l;
+      const double wz = (q >> 2) ? (1 - z_l) : z_l;
+      double t = wx * wy * wz;
+      if (sw) t = t * pw[pos];
+      acc = acc + t;
+    }
+    out[at(d, a, b, c)] = (T)acc;
   }
-  m.init(list);
-  double acc = 0.0;
-  for (;;) {
-    int pos;
-    const int q = m.next(list, pos);
-    if (q < 0) break;
-    const double x_l = sxl[pos], y_l = syl[pos], z_l = szl[pos];
-    const double wx = (q & 1) ? (1 - x_l) : x_l;
-    const double wy = ((q >> 1) & 1) ? (1 - y_l) : y_l;
-    const double wz = (q >> 2) ? (1 - z_l) : z_l;
-    double t = wx * wy * wz;
-    if (sw) t = t * sw[pos];
-    acc = acc + t;
-  }
-  out[at(d, a, b, c)] = (T)acc;
 }
 
 // 1-D linear momentum deposit (particle.cpp:498-524) from the truncating-
@@ -1076,6 +1207,9 @@ int PartDev::alloc(int capacity, const Dev& d) {
   int** m[11] = {&flg, &nl, &pNL, &holepos, &tailnl, &evt, &islo, &plo, &dst, &mslot, &mlv};
   for (auto pp : m) FR_CHECK(hipMalloc(pp, sizeof(int) * (cap + 1)));
   FR_CHECK(hipHostMalloc(&hbuf, sizeof(int) * 4, hipHostMallocDefault));
+  const long ntile = (long)nblocks(d.Nx, DTX) * nblocks(d.Ny, DTY) * d.Nz;
+  FR_CHECK(hipMalloc(&dense_c, sizeof(int) * (ntile + 1)));
+  FR_CHECK(hipMalloc(&dense_w, sizeof(int) * (ntile + 1)));
   return FR_OK;
 }
 
@@ -1087,7 +1221,7 @@ void PartDev::release() {
     }
   void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, alt.flag, alt.rhs_u, alt.rhs_v, alt.rhs_w,
                  alt.rhs_T, rank, rank_alt, inv, scan_tmp, mig_tmp, rank_dep, flg, nl, pNL,
-                 holepos, tailnl, evt, islo, plo, dst, mslot, mlv, inj_buf};
+                 holepos, tailnl, evt, islo, plo, dst, mslot, mlv, inj_buf, dense_c, dense_w};
   for (void* p : all) (void)hipFree(p);
   free_buckets(R);
   free_buckets(T);
@@ -1156,8 +1290,17 @@ static dim3 grid3p(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 
 template <typename T>
 int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStream_t st) {
   if (!R.valid) return FR_EINVAL;
-  hipLaunchKernelGGL(k_dep_cic<T>, grid3p(d), dim3(64, 4), 0, st, d, R.start, R.list, R.s[0],
-                     R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out);
+  int* dl = weighted ? dense_w : dense_c;
+  FR_CHECK(hipMemsetAsync(dl, 0, sizeof(int), st));
+  const dim3 g = grid3p(d);
+  hipLaunchKernelGGL(k_dep_cic<T>, g, dim3(64, 4), 0, st, d, R.start, R.list, R.s[0], R.s[1],
+                     R.s[2], weighted ? R.s[3] : nullptr, out, dl);
+  FR_LAUNCH_CHECK();
+  // two workgroups of k_dep_cic_dense fit a CU (LDS): one resident round
+  const int nt = (int)(g.x * g.y * g.z);
+  hipLaunchKernelGGL(k_dep_cic_dense<T>, dim3(nt < 512 ? nt : 512), dim3(64, 4), 0, st, d,
+                     R.start, R.list, R.s[0], R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out, dl,
+                     (int)g.x, (int)g.y);
   FR_LAUNCH_CHECK();
   int rc = cum_fold(d, out, st);
   if (!rc && halo) rc = ghost_fill(d, out, st);
