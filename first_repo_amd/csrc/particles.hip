@@ -560,7 +560,8 @@ __device__ __forceinline__ double dep_cic_merge(const int (*ss)[DSR], const int*
 // place in its segment is the number of smaller reference indices there;
 // each target then adds its segment in that order.  The same sum as the
 // reference's loop, with no per-target merge.  A tile whose list exceeds
-// PCAP is left to k_dep_cic_dense: its id is appended to dl (dl[0] counts).
+// PCAP is left to k_dep_cic_dense (its id is appended to dl, dl[0] counts)
+// or, with dl null, merges per target.
 template <typename T>
 __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ start,
                                                  const int* __restrict__ list,
@@ -663,8 +664,13 @@ __global__ void __launch_bounds__(256) k_dep_cic(Dev d, const int* __restrict__ 
     }
     return;
   }
-  // a dense tile: its id goes to the list of k_dep_cic_dense
-  if (tid == 0) dl[1 + atomicAdd(dl, 1)] = (int)(bk.x + gridDim.x * (bk.y + gridDim.y * bk.z));
+  // a dense tile: its id goes to the list of k_dep_cic_dense, or without a
+  // list every target merges its buckets from global memory
+  if (dl) {
+    if (tid == 0) dl[1 + atomicAdd(dl, 1)] = (int)(bk.x + gridDim.x * (bk.y + gridDim.y * bk.z));
+    return;
+  }
+  if (live) out[at(d, a, b, c)] = (T)dep_cic_merge(ss, list, sxl, syl, szl, sw);
 }
 
 // Dense tiles of k_dep_cic (ids in dl[1..dl[0]], grid-stride): the particles
@@ -1290,18 +1296,26 @@ static dim3 grid3p(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 
 template <typename T>
 int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStream_t st) {
   if (!R.valid) return FR_EINVAL;
-  int* dl = weighted ? dense_w : dense_c;
-  FR_CHECK(hipMemsetAsync(dl, 0, sizeof(int), st));
+  // Dense tiles (over PCAP contributions) get their own kernel when the mean
+  // tile holds more than PCAP/2 contributions; at lower mean densities the
+  // few dense tiles merge in place, which spares every call a launch that
+  // reserves 76 KB of LDS per workgroup.
   const dim3 g = grid3p(d);
+  const double cells = (double)d.Nx * d.Ny * d.Nz;
+  const bool defer = (double)np * 8 * DTX * DTY > 0.5 * PCAP * cells;
+  int* dl = defer ? (weighted ? dense_w : dense_c) : nullptr;
+  if (dl) FR_CHECK(hipMemsetAsync(dl, 0, sizeof(int), st));
   hipLaunchKernelGGL(k_dep_cic<T>, g, dim3(64, 4), 0, st, d, R.start, R.list, R.s[0], R.s[1],
                      R.s[2], weighted ? R.s[3] : nullptr, out, dl);
   FR_LAUNCH_CHECK();
-  // two workgroups of k_dep_cic_dense fit a CU (LDS): one resident round
-  const int nt = (int)(g.x * g.y * g.z);
-  hipLaunchKernelGGL(k_dep_cic_dense<T>, dim3(nt < 512 ? nt : 512), dim3(64, 4), 0, st, d,
-                     R.start, R.list, R.s[0], R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out, dl,
-                     (int)g.x, (int)g.y);
-  FR_LAUNCH_CHECK();
+  if (dl) {
+    // two workgroups of k_dep_cic_dense fit a CU (LDS): one resident round
+    const int nt = (int)(g.x * g.y * g.z);
+    hipLaunchKernelGGL(k_dep_cic_dense<T>, dim3(nt < 512 ? nt : 512), dim3(64, 4), 0, st, d,
+                       R.start, R.list, R.s[0], R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out,
+                       dl, (int)g.x, (int)g.y);
+    FR_LAUNCH_CHECK();
+  }
   int rc = cum_fold(d, out, st);
   if (!rc && halo) rc = ghost_fill(d, out, st);
   return rc;

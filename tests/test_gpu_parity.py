@@ -338,3 +338,32 @@ def test_dense_deposits_match_oracle(tmp_path, npart):
                                stages=(0, 1, 2, 3))
     assert not bad, bad
     assert worst["C"] == 0.0, worst["C"]
+
+
+def test_clustered_deposits_match_oracle(tmp_path):
+    """A sparse particle set (mean tile below half the LDS list) with all of
+    its particles gathered into 3x3x3 cells, about 37 per cell: the few
+    dense tiles merge per target inside the sparse deposit kernel."""
+    params, d = make_box_case(tmp_path, n=16, n_particles=1000)
+    og = OL.OracleGrid(params, d)
+    og.initialize()
+    og.step()
+    rng = np.random.default_rng(5)
+    for q, L in enumerate((og.g.Lx, og.g.Ly, og.g.Lz)):
+        x = (0.3 + rng.random(og.np_) * 3.0 / 16.0) * L
+        for s in range(4):
+            og.part(q, s)[:] = x
+    sim = fr.Sim(params, d)
+    sim.initialize()
+    sync_oracle_to_device(og, sim)
+    og.call("update")
+    sim.call("update")
+    for m in ("Update_Rho", "Update_P0", "Update_Particle"):
+        print(f"clustered {m}", flush=True)
+        og.call(m, 0)
+        sim.call(m, 0)
+    f1, f0, parts = OUTPUTS["Update_Particle"]
+    worst, bad = compare_state(og, sim, tol=1e-12, fields1=f1, fields0=f0, particles=parts,
+                               stages=(0, 1, 2, 3))
+    assert not bad, bad
+    assert worst["C"] == 0.0, worst["C"]
