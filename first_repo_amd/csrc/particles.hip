@@ -754,7 +754,10 @@ __global__ void __launch_bounds__(256) k_dep_cic_dense(Dev d, const int* __restr
     }
     __syncthreads();
     if (!live) continue;
-    Merge<8> m;
+    // 8-way merge with packed state: key[q] = reference index * 8 + q of the
+    // bucket's head (INT_MAX when done; indices stay below 2^28), ce[q] =
+    // cursor | end << 16 (staged positions are below DCAP)
+    int key[8], ce[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int r = threadIdx.y + ((q >> 1) & 1) + (DTY + 1) * (q >> 2), j = threadIdx.x + (q & 1);
@@ -762,15 +765,29 @@ __global__ void __launch_bounds__(256) k_dep_cic_dense(Dev d, const int* __restr
 #pragma unroll
       for (int k = 0; k < DROWS; k++)
         if (k == r) o = boff[k];
-      m.cur[q] = ss[r][j] + o;
-      m.end[q] = ss[r][j + 1] + o;
+      const int c0 = ss[r][j] + o, e0 = ss[r][j + 1] + o;
+      ce[q] = c0 | (e0 << 16);
+      key[q] = c0 < e0 ? ((prk[c0] << 3) | q) : INT_MAX;
     }
-    m.init(prk);
     double acc = 0.0;
     for (;;) {
-      int pos;
-      const int q = m.next(prk, pos);
-      if (q < 0) break;
+      int best = key[0];
+#pragma unroll
+      for (int q = 1; q < 8; q++) best = min(best, key[q]);
+      if (best == INT_MAX) break;
+      const int q = best & 7;
+      int cq = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k == q) cq = ce[k];
+      const int pos = cq & 0xffff, nx = pos + 1;
+      const int h = nx < (cq >> 16) ? ((prk[nx] << 3) | q) : INT_MAX;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k == q) {
+          ce[k] = cq + 1;
+          key[k] = h;
+        }
       const double x_l = px[pos], y_l = py[pos], z_l = pz[pos];
       const double wx = (q & 1) ? (1 - x_l) : x_l;
       const double wy = ((q >> 1) & 1) ? (1 - y_l) : y_l;
@@ -1302,7 +1319,7 @@ int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStre
   // reserves 76 KB of LDS per workgroup.
   const dim3 g = grid3p(d);
   const double cells = (double)d.Nx * d.Ny * d.Nz;
-  const bool defer = (double)np * 8 * DTX * DTY > 0.5 * PCAP * cells;
+  const bool defer = (double)np * 8 * DTX * DTY > 0.5 * PCAP * cells && np < (1 << 28);
   int* dl = defer ? (weighted ? dense_w : dense_c) : nullptr;
   if (dl) FR_CHECK(hipMemsetAsync(dl, 0, sizeof(int), st));
   hipLaunchKernelGGL(k_dep_cic<T>, g, dim3(64, 4), 0, st, d, R.start, R.list, R.s[0], R.s[1],
