@@ -108,6 +108,21 @@ __device__ inline void load_tw(C2* tws, const C2* tw, int L) {
 // grid::Solve_Poisson passes (grid.cpp:641).
 // TC is the type of the coefficient arrays (float in the FP32 precision form,
 // promoted to double here); P and the right-hand side are always double.
+// The residual from its operands: face coefficients C (lower, upper face
+// per axis), P at the cell and its six neighbours, the right-hand side.
+__device__ __forceinline__ double residual_v(const Dev& d, double Cx0, double Cx1, double Cy0,
+                                             double Cy1, double Cz0, double Cz1, double pc,
+                                             double pxp, double pxm, double pyp, double pym,
+                                             double pzp, double pzm, double rhs) {
+  const double OneThird = 1. / 3.;
+  const double ccf = OneThird * ((Cx0 + Cx1) * 0.5 + (Cy0 + Cy1) * 0.5 + (Cz0 + Cz1) * 0.5);
+  const double cc = -1. / ccf;
+  const double D = (((pxp - pc) * d.idx) * Cx1 - ((pc - pxm) * d.idx) * Cx0) * d.idx +
+                   (((pyp - pc) * d.idy) * Cy1 - ((pc - pym) * d.idy) * Cy0) * d.idy +
+                   (((pzp - pc) * d.idz) * Cz1 - ((pc - pzm) * d.idz) * Cz0) * d.idz;
+  return (D - rhs) * cc;
+}
+
 template <bool INV, typename TC = double>
 __device__ __forceinline__ double residual_t(const Dev& d, const double* __restrict__ P,
                                              const TC* __restrict__ cx,
@@ -116,17 +131,9 @@ __device__ __forceinline__ double residual_t(const Dev& d, const double* __restr
                                              const double* __restrict__ RHS, long c) {
   const long sx = 1, sy = d.Nx, sz = d.NxNy;
   auto ld = [](const TC* a, long q) { return INV ? 1. / (double)a[q] : (double)a[q]; };
-  const double Cx0 = ld(cx, c), Cx1 = ld(cx, c + sx);
-  const double Cy0 = ld(cy, c), Cy1 = ld(cy, c + sy);
-  const double Cz0 = ld(cz, c), Cz1 = ld(cz, c + sz);
-  const double OneThird = 1. / 3.;
-  const double ccf = OneThird * ((Cx0 + Cx1) * 0.5 + (Cy0 + Cy1) * 0.5 + (Cz0 + Cz1) * 0.5);
-  const double cc = -1. / ccf;
-  const double pc = P[c];
-  const double D = (((P[c + sx] - pc) * d.idx) * Cx1 - ((pc - P[c - sx]) * d.idx) * Cx0) * d.idx +
-                   (((P[c + sy] - pc) * d.idy) * Cy1 - ((pc - P[c - sy]) * d.idy) * Cy0) * d.idy +
-                   (((P[c + sz] - pc) * d.idz) * Cz1 - ((pc - P[c - sz]) * d.idz) * Cz0) * d.idz;
-  return (D - RHS[c]) * cc;
+  return residual_v(d, ld(cx, c), ld(cx, c + sx), ld(cy, c), ld(cy, c + sy), ld(cz, c),
+                    ld(cz, c + sz), P[c], P[c + sx], P[c - sx], P[c + sy], P[c - sy], P[c + sz],
+                    P[c - sz], RHS[c]);
 }
 
 __device__ __forceinline__ double residual(const Dev& d, const double* __restrict__ P,

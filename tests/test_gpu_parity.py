@@ -90,7 +90,7 @@ def test_reductions(geo):
     assert rel(host(out), ref) < 1e-14
 
 
-@pytest.mark.parametrize("n", [(8, 8, 8), (16, 8, 32), (32, 32, 32)], ids=str)
+@pytest.mark.parametrize("n", [(8, 8, 8), (16, 8, 32), (32, 32, 32), (64, 64, 64)], ids=str)
 def test_box_poisson_matches_oracle(n):
     """fr_box_poisson_solve vs the oracle's poisson::FFT restatement on a
     variable-coefficient problem (density ratio ~2)."""
@@ -367,3 +367,26 @@ def test_clustered_deposits_match_oracle(tmp_path):
                                stages=(0, 1, 2, 3))
     assert not bad, bad
     assert worst["C"] == 0.0, worst["C"]
+
+
+def test_pressure_solve_64_matches_oracle(tmp_path):
+    """grid::Solve_Poisson on a 64^3 box (C = 1/Rho_face formed inside the
+    residual of the x pass) after the substep's earlier methods, from an
+    identical state."""
+    params, d = make_box_case(tmp_path, n=64, n_particles=20000)
+    og = OL.OracleGrid(params, d)
+    og.initialize()
+    og.step()
+    sim = fr.Sim(params, d)
+    sim.initialize()
+    sync_oracle_to_device(og, sim)
+    og.call("update")
+    sim.call("update")
+    for m in ("Update_Rho", "Update_P0", "Update_Particle", "Update_RU_WOP", "Compute_Div_U_new",
+              "Compute_RHS_Pois", "Solve_Poisson"):
+        print(f"64^3 {m}", flush=True)
+        og.call(m, 0)
+        sim.call(m, 0)
+    worst, bad = compare_state(og, sim, tol=1e-12, fields1=[], fields0=["P", "RHS_Pois"],
+                               particles=False)
+    assert not bad, bad
