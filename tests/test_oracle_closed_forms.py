@@ -401,3 +401,40 @@ def test_pcg_multigrid_beats_unpreconditioned_bound():
         rr = C.c_double()
         its.append(O.orc_pcg(g, rf[0], rf[1], rf[2], P, rhs, 200, 1e-10, C.byref(rr)))
     assert its[1] <= its[0] + 3, its
+
+
+def test_tke_and_dissipation_closed_forms(tmp_path):
+    """Grid diagnostics (grid.cpp:714-811) on fields set directly, density
+    rho0 everywhere, mean stream U0 along x:
+    - u_x = U0 + A sin(y): the plane-averaged profile is U0, so TKE =
+      rho0 A^2 / 4; the only strain is the centred ddy of u_x, averaged over
+      the two x-faces, so 2 nu S_ij S_ij averages to
+      (Mu0/rho0) A^2 sin^2(dy) / (2 dy^2);
+    - u_x = U0 + A sin(x): every plane is uniform, so TKE = 0; the face-to-
+      centre ddx gives 4 (Mu0/rho0) A^2 sin^2(dx/2) / dx^2.
+    Pins the plane-average subtraction, the density weight, the 1/8 shear
+    terms and the factor 2 nu."""
+    d = str(tmp_path)
+    mu, rho0, U0, A = 0.05, 1.7, 0.7, 0.4
+    _shear_case(d, mu=mu)
+    og = OL.OracleGrid(d + "/params.in", d)
+    og.initialize()
+    g = og.g
+    nz, ny, nx = g.Nz, g.Ny, g.Nx
+    dx, dy = g.dx, g.dy
+    cases = {
+        "y": ((U0 + A * np.sin((np.arange(ny) - 0.5) * dy))[None, :, None], rho0 * A * A / 4,
+              (mu / rho0) * A * A * math.sin(dy) ** 2 / (2 * dy * dy)),
+        "x": ((U0 + A * np.sin((np.arange(nx) - 1.0) * dx + 0.3))[None, None, :], 0.0,
+              4 * (mu / rho0) * A * A * math.sin(dx / 2) ** 2 / (dx * dx)),
+    }
+    for name, (ux, tke, diss) in cases.items():
+        ux = np.broadcast_to(ux, (nz, ny, nx))
+        og.field("Rho")[:] = rho0
+        for c in range(3):
+            og.field("Rho_face", c)[:] = rho0
+            for f in ("RU", "RU_np1"):
+                og.field(f, c)[:] = rho0 * ux if c == 0 else 0.0
+        out = og.diagnostics()
+        assert abs(out[0] - tke) <= 1e-12 * rho0 * A * A, (name, out[0], tke)
+        assert abs(out[7] - diss) <= 1e-12 * diss, (name, out[7], diss)
