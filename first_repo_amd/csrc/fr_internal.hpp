@@ -153,6 +153,10 @@ struct AdvCoef {
 struct Buckets {
   int *cnt = nullptr, *start = nullptr, *list = nullptr, *key = nullptr, *slot = nullptr;
   int* list2 = nullptr;  // unsorted bucket lists while a build places them
+  // scan scratch of this set's builds: the two sets are built on different
+  // streams (the heat branch's R beside the momentum deposit's T), so they
+  // share no scratch
+  int* scan_tmp = nullptr;
   double* s[6] = {};
   bool valid = false;
 };
@@ -172,7 +176,6 @@ class PartDev {
   long ncell = 0;
   Buckets R;  // rounded index, operands x_l,y_l,z_l,RHS_T
   Buckets T;  // truncating index, operands x_l,y_l,z_l,RHS_u,RHS_v,RHS_w
-  int* scan_tmp = nullptr;
   // the migration has its own scan scratch, and a copy of the reference
   // indices (rank_dep) lets a bucket build run beside a migration that
   // rewrites them

@@ -1191,11 +1191,12 @@ static int alloc_buckets(Buckets& B, long ncell, int cap, int nops) {
   FR_CHECK(hipMalloc(&B.key, sizeof(int) * cap));
   FR_CHECK(hipMalloc(&B.slot, sizeof(int) * cap));
   for (int q = 0; q < nops; q++) FR_CHECK(hipMalloc(&B.s[q], sizeof(double) * cap));
+  FR_CHECK(hipMalloc(&B.scan_tmp, sizeof(int) * (ncell / STILE + 2)));
   return FR_OK;
 }
 
 static void free_buckets(Buckets& B) {
-  void* all[] = {B.cnt, B.start, B.list, B.list2, B.key, B.slot};
+  void* all[] = {B.cnt, B.start, B.list, B.list2, B.key, B.slot, B.scan_tmp};
   for (void* p : all) (void)hipFree(p);
   for (double* p : B.s)
     if (p) (void)hipFree(p);
@@ -1224,7 +1225,6 @@ int PartDev::alloc(int capacity, const Dev& d) {
   if (!rc) rc = alloc_buckets(T, ncell, cap, 6);
   if (rc) return rc;
   const long big = ncell > cap ? ncell : cap;
-  FR_CHECK(hipMalloc(&scan_tmp, sizeof(int) * (big / STILE + 2)));
   FR_CHECK(hipMalloc(&mig_tmp, sizeof(int) * (big / STILE + 2)));
   FR_CHECK(hipMalloc(&rank_dep, sizeof(int) * cap));
   int** m[11] = {&flg, &nl, &pNL, &holepos, &tailnl, &evt, &islo, &plo, &dst, &mslot, &mlv};
@@ -1243,7 +1243,7 @@ void PartDev::release() {
       (void)hipFree(alt.s[q][t]);
     }
   void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, alt.flag, alt.rhs_u, alt.rhs_v, alt.rhs_w,
-                 alt.rhs_T, rank, rank_alt, inv, scan_tmp, mig_tmp, rank_dep, flg, nl, pNL,
+                 alt.rhs_T, rank, rank_alt, inv, mig_tmp, rank_dep, flg, nl, pNL,
                  holepos, tailnl, evt, islo, plo, dst, mslot, mlv, inj_buf, dense_c, dense_w};
   for (void* p : all) (void)hipFree(p);
   free_buckets(R);
@@ -1291,7 +1291,7 @@ int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool opera
   if (np > 0)
     hipLaunchKernelGGL(k_bucket_count, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), stage,
                        mode, B.cnt, B.key, B.slot);
-  int rc = scan_ex(B.cnt, ncell, B.start, scan_tmp, st);
+  int rc = scan_ex(B.cnt, ncell, B.start, B.scan_tmp, st);
   if (rc) return rc;
   if (np > 0) {
     hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,

@@ -221,4 +221,17 @@ void ref_part_op(void* h, int op, int k, double pre, double post, double* f0,
 }
 // particle::load_random on one rank (glibc rand, default seed).
 void ref_part_load_random(void* h) { ((RefCtx*)h)->part->load_random(); }
+
+// particle::Send_Recv(particle& pbox) (particle.cpp:860-865, 814-851) on a
+// duct rank that is both inlet and outlet.  A second particle object stands
+// in for the box grid's set: its Rbuf1/Rnum1 hold the n_in particles (29
+// values each, ToBuf order) that entered through the box's left face, which
+// the duct appends after its own z, y and x sorting.
+void ref_part_send_recv_duct(void* h, int n_in, const double* inbuf) {
+  RefCtx* c = (RefCtx*)h;
+  particle pbox(c->par, c->pc, c->gs);
+  pbox.Rnum1 = 29 * n_in;
+  if (n_in > 0) std::memcpy(pbox.Rbuf1, inbuf, sizeof(double) * 29 * (size_t)n_in);
+  c->part->Send_Recv(pbox);
+}
 }

@@ -284,7 +284,7 @@ double* orc_grid_field(void* h, const char* name, int comp) {
   if (!f) return nullptr;
   return comp == 0 ? f->x.v.data() : comp == 1 ? f->y.v.data() : f->z.v.data();
 }
-// scalars: P0, P0_int, P0_new, P0_np1, T_cur, C_BC, num_timestep
+// scalars: P0, P0_int, P0_new, P0_np1, T_cur, C_BC, U_avg_in, U_avg_out
 double* orc_grid_scalar(void* h, const char* name) {
   Grid* g = (Grid*)h;
   std::string n(name);
@@ -294,6 +294,8 @@ double* orc_grid_scalar(void* h, const char* name) {
   if (n == "P0_np1") return &g->P0_np1;
   if (n == "T_cur") return &g->T_cur;
   if (n == "C_BC") return &g->C_BC;
+  if (n == "U_avg_in") return &g->U_avg_in;
+  if (n == "U_avg_out") return &g->U_avg_out;
   return nullptr;
 }
 int orc_grid_np(void* h) { return ((Grid*)h)->part.Np; }

@@ -138,6 +138,7 @@ def ref():
         L.ref_part_get.argtypes = [vp, dp, ip, dp, vp, vp]
         L.ref_part_op.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, vp, vp, vp]
         L.ref_part_load_random.argtypes = [vp]
+        L.ref_part_send_recv_duct.argtypes = [vp, C.c_int, vp]
         _ref = L
     return _ref
 
@@ -174,8 +175,10 @@ class OracleGrid:
         self.L.orc_grid_step(self.h)
 
     def set_inflow(self, planes, pbuf=None):
-        """Duct inflow: planes (8, Nz, Ny) float64; pbuf (n, 29) particles."""
-        planes = np.ascontiguousarray(planes, dtype=np.float64)
+        """Duct inflow: planes (8, Nz, Ny) float64 or None (kept); pbuf (n, 29)
+        particles."""
+        if planes is not None:
+            planes = np.ascontiguousarray(planes, dtype=np.float64)
         n = 0 if pbuf is None else len(pbuf)
         pb = None if pbuf is None else np.ascontiguousarray(pbuf, dtype=np.float64)
         self.L.orc_grid_set_inflow(self.h, ptr(planes), n, ptr(pb))

@@ -234,3 +234,38 @@ def test_load_random_matches_reference(ctxs, tmp_path):
     for q in range(7):
         assert np.array_equal(sr[q * 4], so[q * 4]), q
     assert np.array_equal(flr, flo)
+
+
+@pytest.mark.parametrize("geo", [g for g in GEOMS if not g[1]], ids=lambda x: str(x))
+def test_duct_migration_with_inlet_matches_reference(ctxs, geo):
+    """particle::Send_Recv(particle& pbox) (particle.cpp:814-865) on a duct
+    rank that is inlet and outlet: y/z wraps, x leavers at both ends dropped,
+    the box's inlet particles (its Rbuf1) appended in the order handed over.
+    Several rounds, with and without an inlet buffer."""
+    h, og, p, _ = ctxs[tuple(geo)]
+    R = OL.ref()
+    g = og.g
+    rng = np.random.default_rng(23)
+    for trial, (n, n_in) in enumerate([(131, 17), (64, 0), (200, 45), (1, 3), (0, 9)]):
+        st, fl, rt = _random_particles(rng, g, p, n, spill=0.5)
+        R.ref_part_set(h, n, st, fl, rt)
+        _set_oracle(og, st, fl, rt)
+        # inlet particles: positions just inside the inlet, random state
+        inbuf = rng.standard_normal((n_in, 29))
+        for s in range(4):
+            inbuf[:, s] = rng.uniform(0, 0.3 * g.dx, n_in)
+            inbuf[:, 4 + s] = rng.uniform(0, g.Ly, n_in)
+            inbuf[:, 8 + s] = rng.uniform(0, g.Lz, n_in)
+        inbuf[:, 28] = 1000 + trial * 100 + np.arange(n_in)
+        inbuf = np.ascontiguousarray(inbuf)
+        R.ref_part_send_recv_duct(h, n_in, OL.ptr(inbuf))
+        og.set_inflow(None, inbuf if n_in else None)
+        og.call("Send_Recv")
+        sr, flr, rtr, _, _ = _get_ref(h)
+        so, flo, rto = _get_oracle(og)
+        assert sr.shape == so.shape, (trial, sr.shape, so.shape)
+        assert np.array_equal(flr, flo), f"order, trial {trial}"
+        assert np.array_equal(sr, so), f"state, trial {trial}"
+        # the set is smaller than the input when x leavers were dropped
+        x_new = st[2]
+        assert so.shape[1] == n_in + int(np.sum((x_new >= 0) & (x_new <= g.Lx)))
