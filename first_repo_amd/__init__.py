@@ -111,6 +111,7 @@ def lib():
             "fr_sim_particles_changed": ([vp], i),
             "fr_sim_set_inflow": ([vp, vp, i, vp], i),
             "fr_sim_reserve_particles": ([vp, i], i),
+            "fr_sim_set_resort_interval": ([vp, i], i),
             "fr_sim_diagnostics": ([vp, vp], i),
             "fr_sim_timing": ([vp, vp], i),
             "fr_sim_timing_ex": ([vp, vp, i], i),
@@ -214,6 +215,9 @@ class Sim:
 
     def reserve_particles(self, capacity):
         check(self.L.fr_sim_reserve_particles(self.h, int(capacity)), "reserve_particles")
+
+    def set_resort_interval(self, steps):
+        check(self.L.fr_sim_set_resort_interval(self.h, int(steps)), "set_resort_interval")
 
     def particles_changed(self):
         check(self.L.fr_sim_particles_changed(self.h))

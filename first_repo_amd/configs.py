@@ -25,6 +25,10 @@ _FIELDS = [
     ("TWC", "ParticleGravity"), ("A1", "A2"), ("Rho0",), ("U0",), ("T0",),
     ("Mu0", "k"), ("gx1", "gy1", "gz1", "gx2", "gy2", "gz2"), ("Cp", "Cv"),
     ("np0",), ("Dp", "Rhop"), ("Cvp",), ("epsilon_p",), ("Nu",), ("I0_1", "I0_2"),
+    # extension, after the reference's last field (its parser stops before
+    # it): per-particle response times uniform on [Tp_lo, Tp_hi]; 0 0 keeps
+    # the single Tp = Rhop Dp^2 / (18 Mu0)
+    ("Tp_lo", "Tp_hi"),
 ]
 
 
@@ -80,6 +84,8 @@ class Params:
     Nu: float = 2.0
     I0_1: float = 0.0
     I0_2: float = 0.0
+    Tp_lo: float = 0.0
+    Tp_hi: float = 0.0
 
     def text(self) -> str:
         d = asdict(self)
@@ -166,6 +172,9 @@ class BoxWorkload:
     re_lambda: float = 40.0
     prandtl: float = 0.7
     stokes: float = 1.0
+    # per-particle Stokes numbers uniform on this range (the params.in
+    # extension Tp_lo Tp_hi); None keeps one Stokes number for all
+    stokes_range: tuple | None = None
     mass_loading: float = 0.2
     cfl: float = 0.5
     heating_rate: float = 0.5     # initial particle dT/dt from radiation, units T0/time
@@ -224,6 +233,9 @@ class BoxWorkload:
             np0=(N + 0.5) / vol, Dp=Dp, Rhop=Rhop, Cvp=cvp, epsilon_p=eps_p,
             Nu=2.0, I0_1=I0, I0_2=I0, T_final=dt * self.steps * 1.5,
         )
+        if self.stokes_range is not None:
+            p.Tp_lo = self.stokes_range[0] * tau_eta
+            p.Tp_hi = self.stokes_range[1] * tau_eta
         for k, v in self.extra.items():
             setattr(p, k, v)
         assert p.N0 == N, (p.N0, N)
@@ -233,8 +245,10 @@ class BoxWorkload:
 # Named workloads (the measured configurations).
 WORKLOADS = {
     "box64": BoxWorkload(n=64, n_particles=100_000, steps=50),
-    "box256": BoxWorkload(n=256, n_particles=4_000_000, steps=200, mass_loading=0.5),
-    "box512": BoxWorkload(n=512, n_particles=16_000_000, steps=200, mass_loading=0.5),
+    "box256": BoxWorkload(n=256, n_particles=4_000_000, steps=200, mass_loading=0.5,
+                          stokes_range=(0.5, 4.0)),
+    "box512": BoxWorkload(n=512, n_particles=16_000_000, steps=200, mass_loading=0.5,
+                          stokes_range=(0.5, 4.0)),
 }
 
 

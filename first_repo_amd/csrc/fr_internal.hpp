@@ -141,6 +141,9 @@ struct PartPtrs {
 struct AdvCoef {
   double dt_post, dt_pre;      // dt*post, dt*pre
   double iTp;                  // 1/Tp
+  // per-particle response times (extension): 1/Tp from the particle's flag
+  int per_particle;
+  double tp_lo, tp_span;
   double gx, gy, gz;           // ParticleGravity * g
   double dtpost_iS, dtpre_iS;  // dt*post*iS, dt*pre*iS
   double alpha;

@@ -32,6 +32,20 @@ namespace fr {
 
 __device__ __forceinline__ long clampi(long q, long n) { return q < 0 ? 0 : (q >= n ? n - 1 : q); }
 
+// Response time of the particle with flag f under the per-particle extension:
+// the top 24 bits of the murmur3 32-bit finaliser of f as a fraction u in
+// [0, 1), Tp = lo + span * u.  Flags travel with the particles, so each keeps
+// its Tp through migration and storage re-sorts.
+__device__ __forceinline__ double tp_of_flag(int f, double lo, double span) {
+  unsigned h = (unsigned)f;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return lo + span * ((double)(h >> 8) * (1.0 / 16777216.0));
+}
+
 // particle.cpp:314-399, 480-496 (one RK4 stage) with the gather 340-360.
 template <typename T>
 __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict__ ux,
@@ -64,10 +78,11 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
       p.s[a][2][i] = c.last ? np1 : p.s[a][0][i] + c.dt_pre * X[a];
     }
   }
-  // velocities
-  const double ru = (double)((ug - (Cm)ui) * (Cm)c.iTp);
-  const double rv = (double)((vg - (Cm)vi) * (Cm)c.iTp);
-  const double rw = (double)((wg - (Cm)wi) * (Cm)c.iTp);
+  // velocities (per-particle extension: 1/Tp of the particle's own flag)
+  const double itp = c.per_particle ? 1. / tp_of_flag(p.flag[i], c.tp_lo, c.tp_span) : c.iTp;
+  const double ru = (double)((ug - (Cm)ui) * (Cm)itp);
+  const double rv = (double)((vg - (Cm)vi) * (Cm)itp);
+  const double rw = (double)((wg - (Cm)wi) * (Cm)itp);
   p.rhs_u[i] = ru;
   p.rhs_v[i] = rv;
   p.rhs_w[i] = rw;

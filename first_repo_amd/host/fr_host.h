@@ -32,6 +32,10 @@ class params {
   double Cp_ = 0, Cv_ = 0, R_ = 0, np0_ = 0, Dp_ = 0, Rhop_ = 0, Cvp_ = 0, epsilonp_ = 0;
   double Nu_ = 0, I0_ = 0, I01_ = 0, I02_ = 0, Tp_ = 0, mp_ = 0, P0_ = 0, W_ = 0;
   int N0_ = 0;
+  // extension fields appended after I0_1 I0_2 (absent in reference files):
+  // per-particle response times uniform on [Tp_lo, Tp_hi], drawn from the
+  // particle's flag; 0 0 keeps the single Tp of params.cpp:53
+  double Tp_lo_ = 0, Tp_hi_ = 0;
   bool ok = false;
 
   params() {}
@@ -273,6 +277,12 @@ class grid {
   std::vector<double> inlet;   // duct: inlet particles not yet received
   double P0 = 0, P0_int = 0, P0_new = 0, P0_np1 = 0, dP0_dt = 0, T_cur = 0;
   int num_timestep = 0, RK4_count = 0;
+  // Steps between re-sorts of the particle storage into cell order.  A
+  // re-sort costs about 1 ms at 256^3 / 4M and particles move well under a
+  // cell per step, so the order stays close to cell order in between; no
+  // result depends on it (the reference order lives in the ranks).  Set per
+  // solver (fr_sim_set_resort_interval) or by FR_RESORT_EVERY at creation.
+  int resort_every = 4;
   poisson PS_;
   std::string ic_dir;
   // mean of RHS_Part_Temp after its last deposit (tensor0::mean)

@@ -55,6 +55,13 @@ params::params(const char* path) {
   s >> Nu_;
   s >> I01_ >> I02_;
   if (s.fail()) return;
+  // extension fields after the reference's last one (its parser stops
+  // before them); a reference params.in has none and keeps the single Tp
+  double lo = 0, hi = 0;
+  if (s >> lo >> hi) {
+    Tp_lo_ = lo;
+    Tp_hi_ = hi;
+  }
   // params.cpp:50-63
   R_ = Cp_ - Cv_;
   P0_ = Rho0_ * T0_ * R_;
@@ -306,6 +313,9 @@ int particle::advance(int k, double pre, double post, tensor1& U) {
   c.dt_post = dt * post;
   c.dt_pre = dt * pre;
   c.iTp = (p.Tp() != 0) ? 1. / p.Tp() : 0.0;
+  c.per_particle = (p.Tp_lo_ != 0 || p.Tp_hi_ != 0);
+  c.tp_lo = p.Tp_lo_;
+  c.tp_span = p.Tp_hi_ - p.Tp_lo_;
   c.gx = p.ParticleGravity() * p.gx();
   c.gy = p.ParticleGravity() * p.gy();
   c.gz = p.ParticleGravity() * p.gz();
@@ -457,6 +467,7 @@ int grid::setup(const char* params_path, const char* ic) {
   if (f32 && (!param.IsBox() || param.Solver() != 0)) return FR_EINVAL;
   size = gridsize(param);
   ic_dir = ic ? ic : ".";
+  if (const char* e = getenv("FR_RESORT_EVERY")) resort_every = atoi(e) < 1 ? 1 : atoi(e);
   FR_CHECK(hipStreamCreate(&st));
   FR_CHECK(hipStreamCreate(&st2));
   FR_CHECK(hipStreamCreate(&st3));
@@ -659,25 +670,11 @@ int grid::Update_P0() {  // grid.cpp:539-554
   return FR_OK;
 }
 
-// Steps between re-sorts of the particle storage into cell order (measured
-// at 256^3 / 4M: every step 687, every 2 steps 694, every 4 steps 700
-// Mcell*steps/s; a re-sort costs about 1 ms, and particles move well under a
-// cell per step, so the order stays close to cell order in between);
-// FR_RESORT_EVERY overrides the default for tuning.
-static int resort_every() {
-  static const int v = [] {
-    const char* e = getenv("FR_RESORT_EVERY");
-    const int n = e ? atoi(e) : 4;
-    return n < 1 ? 1 : n;
-  }();
-  return v;
-}
-
 int grid::Update_Particle() {  // grid.cpp:659-680
   hipEventRecord(ev[2], st);
-  // storage back in cell order every resort_every() steps (reference order is
+  // storage back in cell order every resort_every steps (reference order is
   // kept in the particles' ranks, so no result depends on it)
-  if (RK4_count == 0 && num_timestep % resort_every() == 0)
+  if (RK4_count == 0 && num_timestep % resort_every == 0)
     FR_TRY(part.dev.resort(size.dev(), st));
   FR_TRY(part.advance(RK4_count, RK4_preCoeff[RK4_count], RK4_postCoeff[RK4_count], U));
   // The concentration deposit (rounded-index buckets, output C) and the
@@ -1299,6 +1296,12 @@ int fr_sim_set_collect(fr_sim* s, fr_unet_data* d) {
 int fr_sim_learned_stats(fr_sim* s, int* guesses_kept, int* fallbacks, double* log10_gain) {
   if (!s || !s->g.PS_.sp) return FR_EINVAL;
   return fr_pcg_learned_stats(s->g.PS_.sp, guesses_kept, fallbacks, log10_gain);
+}
+
+int fr_sim_set_resort_interval(fr_sim* s, int steps) {
+  if (!s || steps < 1) return FR_EINVAL;
+  s->g.resort_every = steps;
+  return FR_OK;
 }
 
 int fr_sim_timing(fr_sim* s, double* out) { return fr_sim_timing_ex(s, out, 6); }

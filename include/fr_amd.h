@@ -268,6 +268,11 @@ int fr_sim_reserve_particles(fr_sim* s, int capacity);
 int fr_sim_set_learned(fr_sim* s, fr_unet* net, int mode);
 int fr_sim_set_collect(fr_sim* s, fr_unet_data* d);
 int fr_sim_learned_stats(fr_sim* s, int* guesses_kept, int* fallbacks, double* log10_gain);
+/* Steps between re-sorts of the particle storage into cell order (default 4,
+ * or FR_RESORT_EVERY at creation).  A tuning knob with no counterpart in the
+ * reference: storage order never changes a result, which stays in the
+ * reference's particle order. */
+int fr_sim_set_resort_interval(fr_sim* s, int steps);
 /* Rebuild the particle cell lists after particle positions were set. */
 int fr_sim_particles_changed(fr_sim* s);
 /* Step diagnostics (grid.cpp:714-811 subset): out[0..9] = TKE, Rho_Mean,

@@ -235,3 +235,38 @@ void ref_part_send_recv_duct(void* h, int n_in, const double* inbuf) {
   c->part->Send_Recv(pbox);
 }
 }
+
+// Element-wise tensor0 operators (tensor0.cpp:93-241) under the op codes of
+// fr_t0_elementwise: 0 Equal_LinComb(a,A,b,B,c,C) / Equal_LinComb(a,A,b,B) /
+// Equal_Mult(a,A) as B and C are given, 1 PlusEqual_Mult(a,A), 2
+// Equal_LinComb(A,B,b,C), 3 Equal_Divide(a,A), 4 Equal_Divide(A,B), 5 =a,
+// 6 +=a, 7 *=a, 8 =A, 9 *=A, 10 -=A, 11 +=A.
+extern "C" void ref_elementwise(void* h, int op, double* out, double a, const double* A,
+                                double b, const double* B, double c, const double* Cc) {
+  RefCtx* ctx = (RefCtx*)h;
+  tensor0 O(ctx->gs, ctx->com), TA(ctx->gs, ctx->com), TB(ctx->gs, ctx->com),
+      TC(ctx->gs, ctx->com);
+  load(O, out);
+  if (A) load(TA, A);
+  if (B) load(TB, B);
+  if (Cc) load(TC, Cc);
+  switch (op) {
+    case 0:
+      if (B && Cc) O.Equal_LinComb(a, TA, b, TB, c, TC);
+      else if (B) O.Equal_LinComb(a, TA, b, TB);
+      else O.Equal_Mult(a, TA);
+      break;
+    case 1: O.PlusEqual_Mult(a, TA); break;
+    case 2: O.Equal_LinComb(TA, TB, b, TC); break;
+    case 3: O.Equal_Divide(a, TA); break;
+    case 4: O.Equal_Divide(TA, TB); break;
+    case 5: O = a; break;
+    case 6: O += a; break;
+    case 7: O *= a; break;
+    case 8: O = TA; break;
+    case 9: O *= TA; break;
+    case 10: O -= TA; break;
+    case 11: O += TA; break;
+  }
+  store(O, out);
+}

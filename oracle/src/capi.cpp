@@ -112,6 +112,40 @@ void orc_update_ghosts(const orc_geom* gg, double* f, int cumulative) {
     a.Update_Ghosts();
   store(a, f);
 }
+// Element-wise forms (tensor0.cpp:93-241) under the op codes of
+// fr_t0_elementwise (include/fr_amd.h); NULL operands are absent.
+void orc_elementwise(const orc_geom* gg, int op, double* out, double a, const double* A,
+                     double b, const double* B, double c, const double* Cc) {
+  Geom g = mk(gg);
+  F0 o, fa, fb, fc;
+  o.init(&g);
+  fa.init(&g);
+  fb.init(&g);
+  fc.init(&g);
+  load(o, out);
+  if (A) load(fa, A);
+  if (B) load(fb, B);
+  if (Cc) load(fc, Cc);
+  switch (op) {
+    case 0:
+      if (B && Cc) o.Equal_LinComb(a, fa, b, fb, c, fc);
+      else if (B) o.Equal_LinComb(a, fa, b, fb);
+      else o.Equal_Mult(a, fa);
+      break;
+    case 1: o.PlusEqual_Mult(a, fa); break;
+    case 2: o.Equal_LinComb(fa, fb, b, fc); break;
+    case 3: o.Equal_Divide(a, fa); break;
+    case 4: o.Equal_Divide(fa, fb); break;
+    case 5: o.set(a); break;
+    case 6: o.add(a); break;
+    case 7: o.mul(a); break;
+    case 8: o.copy(fa); break;
+    case 9: o.mul(fa); break;
+    case 10: o.sub(fa); break;
+    case 11: o.add(fa); break;
+  }
+  store(o, out);
+}
 // op: 0 sum, 1 mean, 2 max, 3 min, 4 max_abs, 5 sum_squares
 double orc_reduce(const orc_geom* gg, const double* f, int op) {
   Geom g = mk(gg);

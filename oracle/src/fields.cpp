@@ -48,6 +48,14 @@ bool Params::parse(const char* path) {
   s >> Nu;
   s >> I01 >> I02;
   if (s.fail()) return false;
+  // extension fields appended after the reference's last one (the
+  // reference's parser stops before them): per-particle response times
+  // uniform on [Tp_lo, Tp_hi]; absent or 0 0 = the single Tp below
+  double lo = 0, hi = 0;
+  if (s >> lo >> hi) {
+    Tp_lo = lo;
+    Tp_hi = hi;
+  }
   R = Cp - Cv;
   P0 = Rho0 * T0 * R;
   Tp = Rhop * Dp * Dp / (18 * Mu0);

@@ -45,10 +45,29 @@ struct Params {
   double I0 = 0, I01 = 0, I02 = 0;
   double Tp = 0, mp = 0, P0 = 0, W = 0;
   int N0 = 0;
+  // extension (no reference counterpart): per-particle response times
+  // uniform on [Tp_lo, Tp_hi], a function of the particle's flag (TpOfFlag);
+  // both 0 keeps the single Tp of params.cpp:53
+  double Tp_lo = 0, Tp_hi = 0;
+  bool per_particle_Tp() const { return Tp_lo != 0 || Tp_hi != 0; }
   double Vol() const { return Lx * Ly * Lz; }
   bool parse(const char* path);  // params.cpp:6-64
   void update(double time);      // params.cpp:66-77
 };
+
+// Response time of the particle with flag f under the per-particle
+// extension: u = the top 24 bits of the murmur3 32-bit finaliser of f, as a
+// fraction in [0, 1); Tp = lo + span * u.  Flags travel with particles, so a
+// particle keeps its Tp through migration, storage and restart.
+inline double TpOfFlag(int f, double lo, double span) {
+  uint32_t h = (uint32_t)f;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return lo + span * ((double)(h >> 8) * (1.0 / 16777216.0));
+}
 
 // ---------------------------------------------------------------- geometry
 // gridsize.cpp:4-88 on one rank: Nx = Nx_tot + 2 bs, dx = L/N, il = 0.

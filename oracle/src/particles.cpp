@@ -109,10 +109,14 @@ void Particles::gas2part_velocity(const F1& U) {
 void Particles::update_velocity(int k, double pre, double post) {
   double iTp = 0;
   if (par->Tp != 0) iTp = 1. / par->Tp;
+  const bool pp = par->per_particle_Tp();
+  const double span = par->Tp_hi - par->Tp_lo;
   for (int i = 0; i < Np; i++) {
-    RHS_u[i] = (ug[i] - u_int[i]) * iTp;
-    RHS_v[i] = (vg[i] - v_int[i]) * iTp;
-    RHS_w[i] = (wg[i] - w_int[i]) * iTp;
+    // per-particle extension: 1/Tp of this particle's flag
+    const double it = pp ? 1. / TpOfFlag(flag[i], par->Tp_lo, span) : iTp;
+    RHS_u[i] = (ug[i] - u_int[i]) * it;
+    RHS_v[i] = (vg[i] - v_int[i]) * it;
+    RHS_w[i] = (wg[i] - w_int[i]) * it;
   }
   double dt = par->dt;
   double gx = par->ParticleGravity * par->gx;

@@ -59,6 +59,8 @@ def orc():
         L.orc_interp.argtypes = [G, dp, dp, C.c_int, C.c_int]
         L.orc_i_f2c.argtypes = [G, dp, dp, dp, dp]
         L.orc_update_ghosts.argtypes = [G, dp, C.c_int]
+        L.orc_elementwise.argtypes = [G, C.c_int, dp, C.c_double, C.c_void_p, C.c_double,
+                                      C.c_void_p, C.c_double, C.c_void_p]
         L.orc_reduce.argtypes = [G, dp, C.c_int]
         L.orc_reduce.restype = C.c_double
         L.orc_reducex.argtypes = [G, dp, dp]
@@ -139,6 +141,8 @@ def ref():
         L.ref_part_op.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, vp, vp, vp]
         L.ref_part_load_random.argtypes = [vp]
         L.ref_part_send_recv_duct.argtypes = [vp, C.c_int, vp]
+        L.ref_elementwise.argtypes = [vp, C.c_int, dp, C.c_double, vp, C.c_double, vp,
+                                      C.c_double, vp]
         _ref = L
     return _ref
 

@@ -269,3 +269,30 @@ def test_duct_migration_with_inlet_matches_reference(ctxs, geo):
         # the set is smaller than the input when x leavers were dropped
         x_new = st[2]
         assert so.shape[1] == n_in + int(np.sum((x_new >= 0) & (x_new <= g.Lx)))
+
+
+# (op, uses A, uses B, uses C) of fr_t0_elementwise; op 0 three ways
+ELEM_CASES = [(0, 1, 1, 1), (0, 1, 1, 0), (0, 1, 0, 0), (1, 1, 0, 0), (2, 1, 1, 1),
+              (3, 1, 0, 0), (4, 1, 1, 0), (5, 0, 0, 0), (6, 0, 0, 0), (7, 0, 0, 0),
+              (8, 1, 0, 0), (9, 1, 0, 0), (10, 1, 0, 0), (11, 1, 0, 0)]
+
+
+@pytest.mark.parametrize("geo", GEOMS[:2], ids=lambda x: str(x))
+def test_elementwise_bit_exact(ctxs, geo):
+    """The tensor0 arithmetic operators (tensor0.cpp:93-241) over the whole
+    ghosted array, every form of fr_t0_elementwise."""
+    h, og, p, _ = ctxs[tuple(geo)]
+    R, O = OL.ref(), OL.orc()
+    g = og.g
+    rng = np.random.default_rng(17)
+    for op, ua, ub, uc in ELEM_CASES:
+        A, B, Cc = (1.5 + rng.random(OL.shape(g)) for _ in range(3))
+        prior = rng.standard_normal(OL.shape(g))
+        a, b, c = 0.37, -1.25, 2.5
+        r_out, o_out = prior.copy(), prior.copy()
+        args = [OL.ptr(A) if ua else None, b, OL.ptr(B) if ub else None, c,
+                OL.ptr(Cc) if uc else None]
+        R.ref_elementwise(h, op, r_out, a, *args)
+        O.orc_elementwise(g, op, o_out, a, *args)
+        assert np.array_equal(r_out, o_out), (op, ua, ub, uc)
+        assert not np.array_equal(o_out, prior) or op == 8
