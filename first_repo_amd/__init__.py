@@ -61,6 +61,7 @@ def lib():
             "fr_t0_update_ghosts": ([G, vp, i, vp], i),
             "fr_t0_elementwise": ([G, i, vp, d, vp, d, vp, d, vp, vp], i),
             "fr_t0_reduce": ([G, vp, i, C.POINTER(d), vp], i),
+            "fr_t0_reduce_noalloc": ([G, vp, i, C.POINTER(d), vp], i),
             "fr_t0_reducex": ([G, vp, vp, vp], i),
             "fr_box_poisson_create": ([G, C.POINTER(vp)], i),
             "fr_box_poisson_destroy": ([vp], i),

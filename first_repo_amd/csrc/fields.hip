@@ -10,6 +10,8 @@
 //         (i, img(j), img(k))                  (y, z wrap; x ghosts kept)
 // so every kernel writes the whole ghosted array in one pass and reads only
 // cells the reference reads.
+#include <mutex>
+
 #include "fr_internal.hpp"
 
 namespace fr {
@@ -433,6 +435,38 @@ int reduce_sync(const Dev& d, const T* f, int op, double* result, hipStream_t s)
 template int reduce_sync<double>(const Dev&, const double*, int, double*, hipStream_t);
 template int reduce_sync<float>(const Dev&, const float*, int, double*, hipStream_t);
 
+// ---- scratch of the allocation-free reduction (fr_t0_reduce_noalloc) ---------
+// The RED_PARTS partial results and the result of one reduction live in a
+// file-scope device array, `scratch`, which every GPU the library runs on
+// holds (8.3 KB) until the process ends.  Its name in host code is a host-side
+// stand-in, not a device address: host code uses only the address
+// hipGetSymbolAddress returns for it, looked up once per device with that
+// device current and kept per device (each GPU holds its own copy at its own
+// address).  Calls that overlap on one GPU share this array and race.
+constexpr int NOALLOC_DOUBLES = RED_PARTS + 8;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  FR_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    fprintf(stderr, "fr_amd: device %d beyond the %d the scratch table holds\n", dev,
+            NOALLOC_MAX_DEVICES);
+    return FR_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    FR_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FR_OK;
+}
+
 // communicator.cpp:635-646: one block per x index i.
 __global__ void __launch_bounds__(RB) k_reducex(Dev d, const double* __restrict__ f,
                                                 double denom, double* __restrict__ out) {
@@ -566,6 +600,26 @@ int fr_t0_reduce(const fr_geom* g, const double* f, int op, double* result, void
   if (!geom_ok(g) || !f || !result || op < 0 || op > 5) return FR_EINVAL;
   Dev d = make_dev(g);
   return reduce_sync(d, f, op, result, S(s));
+}
+
+int fr_t0_reduce_noalloc(const fr_geom* g, const double* f, int op, double* result, void* s) {
+  if (!geom_ok(g) || !f || !result || op < 0 || op > 5) {
+    fprintf(stderr,
+            "fr_amd: fr_t0_reduce_noalloc: bad argument (op %d; needs bs = 1, ghosted sizes "
+            ">= 3 and fewer than 2^31 ghosted cells)\n",
+            op);
+    return FR_EINVAL;
+  }
+  double* ws = nullptr;
+  FR_RET(fr::noalloc_scratch(&ws));
+  Dev d = make_dev(g);
+  // partial results in ws[0, RED_PARTS), the result at ws[RED_PARTS]; every
+  // partial is written by the call before it is read, so nothing is cleared
+  FR_RET(reduce_async_ws(d, f, op, ws + RED_PARTS, ws, S(s)));
+  FR_CHECK(hipMemcpyAsync(result, ws + RED_PARTS, sizeof(double), hipMemcpyDeviceToHost, S(s)));
+  FR_CHECK(hipStreamSynchronize(S(s)));
+  if (op == 1) *result = *result / ((double)(d.Nx - 2) * (d.Ny - 2) * (d.Nz - 2));
+  return FR_OK;
 }
 
 int fr_t0_reducex(const fr_geom* g, const double* f, double* out, void* s) {

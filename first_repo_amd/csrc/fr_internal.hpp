@@ -30,6 +30,10 @@ int reduce_async_ws(const Dev& d, const T* f, int op, double* dres, double* ws, 
 template <typename T>
 int make_mean_zero_async(const Dev& d, T* f, long n, double* ws, hipStream_t s);
 int reducex(const Dev& d, const double* f, double* out, hipStream_t s);
+// Device address, on the current device, of the file-scope scratch array of
+// the allocation-free reduction (RED_PARTS + 8 doubles); looked up once per
+// device.
+int noalloc_scratch(double** out);
 template <typename T>
 int elementwise(long n, int op, T* out, double a, const T* A, double b, const T* B, double c,
                 const T* C, hipStream_t s);

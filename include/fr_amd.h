@@ -90,6 +90,18 @@ int fr_t0_elementwise(const fr_geom* g, int op, double* out, double a,
  * i->j->k loop at the level of rounding). */
 int fr_t0_reduce(const fr_geom* g, const double* f, int op, double* result,
                  void* stream);
+/* fr_t0_reduce without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results as fr_t0_reduce, but the partial results live in a fixed 8.3 KB
+ * device array of the library (1024 partials and the result, the same for
+ * every field size) instead of a workspace allocated at first use, and the
+ * result is copied straight to *result.  Fields up to the limit every entry
+ * point has (fewer than 2^31 ghosted cells) are accepted; other arguments
+ * fail with FR_EINVAL and a message on stderr.  Calls on one GPU share that
+ * array: calls that overlap on one GPU race. */
+int fr_t0_reduce_noalloc(const fr_geom* g, const double* f, int op, double* result,
+                         void* stream);
 /* communicator::REDUCEX (communicator.cpp:635-646): out[i] = sum over
  * interior j,k of f(i,j,k) / (Ny_tot*Nz_tot), for every i; out is a device
  * array of Nx doubles. */
