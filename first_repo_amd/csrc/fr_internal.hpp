@@ -197,6 +197,11 @@ class PartDev {
   double* inj_buf = nullptr;
   int inj_cap = 0;
   bool flg_zero = false;  // the migration flags flg[0..cap] are all zero
+  // deposit kernels: 0 chosen by the mean particles per cell, 1 the pull
+  // form (one thread per target), 2 the tiled LDS form, 3 pull for the
+  // momentum deposit and tiled for the trilinear ones (FR_DEPOSIT=pull|tile|mixed)
+  int dep_form = 0;
+  bool use_pull(bool trilinear) const;
 
   int alloc(int capacity, const Dev& d);
   void release();

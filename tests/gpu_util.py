@@ -131,3 +131,18 @@ def compare_state(og, sim, tol=1e-12, fields1=FIELDS1, fields0=FIELDS0, particle
                 worst[f"part{q}.{s}"] = rel(sim.part(q, s), og.part(q, s))
     bad = {k: v for k, v in worst.items() if not v <= tol}
     return worst, bad
+
+
+def sim_with_deposit(params, d, form=None, **kw):
+    """fr.Sim whose particle deposits use `form` ("pull": one thread per
+    target cell; "tile": LDS tiles; None: chosen by density).  The form is
+    read from FR_DEPOSIT when the particle set is allocated."""
+    old = os.environ.pop("FR_DEPOSIT", None)
+    try:
+        if form:
+            os.environ["FR_DEPOSIT"] = form
+        return fr.Sim(params, d, **kw)
+    finally:
+        os.environ.pop("FR_DEPOSIT", None)
+        if old is not None:
+            os.environ["FR_DEPOSIT"] = old

@@ -13,7 +13,8 @@ import pytest
 
 import first_repo_amd as fr
 import oracle_lib as OL
-from gpu_util import dbuf, dptr, host, rel, make_box_case, sync_oracle_to_device, compare_state
+from gpu_util import (dbuf, dptr, host, rel, make_box_case, sync_oracle_to_device, compare_state,
+                      sim_with_deposit)
 
 pytestmark = pytest.mark.gpu
 
@@ -316,17 +317,18 @@ def test_steps_match_oracle(case):
         assert dg[9] == do[9]
 
 
-@pytest.mark.parametrize("interval", [1, 4])
-def test_lockstep_steps_match_oracle(case, interval):
+@pytest.mark.parametrize("interval,form", [(1, "pull"), (4, "pull"), (4, "tile")])
+def test_lockstep_steps_match_oracle(case, interval, form):
     """Six RK4 steps with no state copied between them: the device's own
     buckets, storage re-sorts (every `interval` steps, so the re-sort at step
     4 runs on ranks that migration rewrote) and migrations carried from step
-    to step.  Per step: fields and particles within 1e-12 relative per step
-    taken, particle order bit-exact, diagnostics likewise."""
+    to step, with either deposit form.  Per step: fields and particles within
+    1e-12 relative per step taken, particle order bit-exact, diagnostics
+    likewise."""
     params, d = case
     og = OL.OracleGrid(params, d)
     og.initialize()
-    sim = fr.Sim(params, d)
+    sim = sim_with_deposit(params, d, form)
     sim.set_resort_interval(interval)
     sim.initialize()
     for step in range(6):
@@ -385,19 +387,21 @@ def test_migration_order_many_leavers(case):
                 assert np.array_equal(sim.part(q, s), og.part(q, s)), (q, s)
 
 
+@pytest.mark.parametrize("form", ["tile", "pull"])
 @pytest.mark.parametrize("npart", [30000, 70000])
-def test_dense_deposits_match_oracle(tmp_path, npart):
+def test_dense_deposits_match_oracle(tmp_path, npart, form):
     """particle::Update_Particle at about 7 and 17 particles per cell: the
     trilinear deposits' tiles hold more contributions than their LDS list,
     so they take the dense path, which merges per target from the tile's
     particles staged in LDS (7 per cell) or, when they do not fit, from
-    global memory (17 per cell).  The concentration deposit is bit-exact;
+    global memory (17 per cell); the pull form merges every target's eight
+    buckets from global memory.  The concentration deposit is bit-exact;
     the rest within the substep bar."""
     params, d = make_box_case(tmp_path, n=16, n_particles=npart)
     og = OL.OracleGrid(params, d)
     og.initialize()
     og.step()
-    sim = fr.Sim(params, d)
+    sim = sim_with_deposit(params, d, form)
     sim.initialize()
     sync_oracle_to_device(og, sim)
     og.call("update")
@@ -413,10 +417,12 @@ def test_dense_deposits_match_oracle(tmp_path, npart):
     assert worst["C"] == 0.0, worst["C"]
 
 
-def test_clustered_deposits_match_oracle(tmp_path):
+@pytest.mark.parametrize("form", ["tile", "pull"])
+def test_clustered_deposits_match_oracle(tmp_path, form):
     """A sparse particle set (mean tile below half the LDS list) with all of
-    its particles gathered into 3x3x3 cells, about 37 per cell: the few
-    dense tiles merge per target inside the sparse deposit kernel."""
+    its particles gathered into 3x3x3 cells, about 37 per cell: in the tiled
+    form the few dense tiles merge per target inside the sparse deposit
+    kernel; in the pull form those targets merge long bucket ranges."""
     params, d = make_box_case(tmp_path, n=16, n_particles=1000)
     og = OL.OracleGrid(params, d)
     og.initialize()
@@ -426,7 +432,7 @@ def test_clustered_deposits_match_oracle(tmp_path):
         x = (0.3 + rng.random(og.np_) * 3.0 / 16.0) * L
         for s in range(4):
             og.part(q, s)[:] = x
-    sim = fr.Sim(params, d)
+    sim = sim_with_deposit(params, d, form)
     sim.initialize()
     sync_oracle_to_device(og, sim)
     og.call("update")
