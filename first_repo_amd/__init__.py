@@ -230,9 +230,11 @@ class Sim:
 
     def timing(self):
         """[step ms, Poisson ms, particle ms, Poisson iterations, momentum
-        kernel ms, its launches, forward x-pass ms, its launches]."""
-        out = np.zeros(8)
-        check(self.L.fr_sim_timing_ex(self.h, _p(out), 8))
+        kernel ms, its launches, forward x-pass ms, its launches, momentum
+        deposit kernel ms, its launches, trilinear deposit kernels ms, their
+        launches]."""
+        out = np.zeros(12)
+        check(self.L.fr_sim_timing_ex(self.h, _p(out), 12))
         return out
 
     def close(self):

@@ -154,6 +154,54 @@ struct AdvCoef {
   int last;
 };
 
+// HIP events bound to the launches of one kernel (hipExtLaunchKernelGGL
+// records them at the kernel's own start and end), summed after the stream
+// has been synchronised.  Launches beyond NE between two collects are not
+// timed.
+struct KTimer {
+  static constexpr int NE = 32;
+  hipEvent_t e0[NE] = {}, e1[NE] = {};
+  int used = 0;
+  double ms = 0;
+  long n = 0;
+  int init() {
+    for (int i = 0; i < NE; i++) {
+      if (hipEventCreate(&e0[i]) != hipSuccess || hipEventCreate(&e1[i]) != hipSuccess)
+        return FR_ENOMEM;
+    }
+    return FR_OK;
+  }
+  void release() {
+    for (int i = 0; i < NE; i++) {
+      if (e0[i]) (void)hipEventDestroy(e0[i]);
+      if (e1[i]) (void)hipEventDestroy(e1[i]);
+      e0[i] = e1[i] = nullptr;
+    }
+  }
+  // the event pair of the next launch, or false when none is left
+  bool next(hipEvent_t* a, hipEvent_t* b) {
+    if (used >= NE || !e0[used]) return false;
+    *a = e0[used];
+    *b = e1[used];
+    used++;
+    return true;
+  }
+  void collect() {
+    for (int i = 0; i < used; i++) {
+      float t = 0;
+      if (hipEventElapsedTime(&t, e0[i], e1[i]) == hipSuccess) {
+        ms += t;
+        n++;
+      }
+    }
+    used = 0;
+  }
+  void reset() {
+    ms = 0;
+    n = 0;
+  }
+};
+
 // Cell buckets of the particle set: particles grouped by cell index (rounded
 // or truncating), each bucket in ascending particle order, with the deposit
 // operands of each particle gathered into bucket order.
@@ -202,6 +250,8 @@ class PartDev {
   // momentum deposit and tiled for the trilinear ones (FR_DEPOSIT=pull|tile|mixed)
   int dep_form = 0;
   bool use_pull(bool trilinear) const;
+  // launch timers of the momentum and trilinear deposit kernels
+  KTimer t_vel, t_cic;
 
   int alloc(int capacity, const Dev& d);
   void release();

@@ -28,6 +28,8 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "fr_internal.hpp"
 
 namespace fr {
@@ -216,25 +218,27 @@ __global__ void __launch_bounds__(SB) k_scan_tiles(const int* __restrict__ in, l
 }
 
 // Exclusive scan of in[0..n) into out[0..n) (in == out allowed) in one
-// workgroup of SS threads, in rounds of SS consecutive elements: each round
+// workgroup of NT threads, in rounds of NT consecutive elements: each round
 // is one coalesced load per thread (the next round's issued before this
 // round's scan, so reads stay ahead of the writes), wave scans, wave totals
-// in LDS, and a running carry.  Returns the total (to every thread).
-constexpr int SS = 1024;
+// in LDS, and a running carry.  Returns the total (to every thread).  Kept
+// to small workgroups: a single workgroup of many waves waits for a whole
+// CU to drain when other streams' kernels fill the GPU.
+template <int NT>
 __device__ __forceinline__ int block_scan_rounds(const int* in, int n, int* out, int* wsum) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   int carry = 0;
   int next = tid < n ? in[tid] : 0;
-  for (int base = 0; base < n; base += SS) {
+  for (int base = 0; base < n; base += NT) {
     const int v = next;
-    const int i = base + tid, j = i + SS;
+    const int i = base + tid, j = i + NT;
     next = j < n ? in[j] : 0;
     const int inc = wave_incl_scan(v, lane);
     if (lane == 63) wsum[w] = inc;
     __syncthreads();
     int off = carry, tot = 0;
 #pragma unroll
-    for (int u = 0; u < SS / 64; u++) {
+    for (int u = 0; u < NT / 64; u++) {
       const int t = wsum[u];
       off += u < w ? t : 0;
       tot += t;
@@ -246,11 +250,36 @@ __device__ __forceinline__ int block_scan_rounds(const int* in, int n, int* out,
   return carry;
 }
 
-// The tile sums, in place; the grand total is stored at out_total.
-__global__ void __launch_bounds__(SS) k_scan_sums(int* tsum, int ntiles,
-                                                  int* __restrict__ out_total) {
-  __shared__ int wsum[SS / 64];
-  const int total = block_scan_rounds(tsum, ntiles, tsum, wsum);
+// The tile sums are scanned in two levels: blocks of MIDN tile sums in place
+// (each thread four consecutive ones), whose totals go to top[]; then top[]
+// in one small workgroup, which also stores the grand total.  k_scan_apply
+// adds both offsets.
+constexpr int MIDN = 1024, ST = 256;
+__global__ void __launch_bounds__(ST) k_scan_mid(int* __restrict__ tsum, int nt,
+                                                 int* __restrict__ top) {
+  __shared__ int wt[ST / 64];
+  const int base = blockIdx.x * MIDN + threadIdx.x * 4;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int v[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) v[q] = base + q < nt ? tsum[base + q] : 0;
+  const int s4 = v[0] + v[1] + v[2] + v[3];
+  const int inc = wave_incl_scan(s4, lane);
+  if (lane == 63) wt[w] = inc;
+  __syncthreads();
+  int run = inc - s4;
+  for (int u = 0; u < w; u++) run += wt[u];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    if (base + q < nt) tsum[base + q] = run;
+    run += v[q];
+  }
+  if (threadIdx.x == 0) top[blockIdx.x] = wt[0] + wt[1] + wt[2] + wt[3];
+}
+
+__global__ void __launch_bounds__(ST) k_scan_top(int* top, int nb, int* __restrict__ out_total) {
+  __shared__ int wsum[ST / 64];
+  const int total = block_scan_rounds<ST>(top, nb, top, wsum);
   if (threadIdx.x == 0) *out_total = total;
 }
 
@@ -260,6 +289,7 @@ __global__ void __launch_bounds__(SS) k_scan_sums(int* tsum, int ntiles,
 template <bool EQ0>
 __global__ void __launch_bounds__(SB) k_scan_apply(const int* __restrict__ in, long n,
                                                    const int* __restrict__ tsum,
+                                                   const int* __restrict__ top,
                                                    int* __restrict__ out) {
   __shared__ int wt[SV][SB / 64];
   const long base = (long)blockIdx.x * STILE + threadIdx.x * 4;
@@ -273,7 +303,7 @@ __global__ void __launch_bounds__(SB) k_scan_apply(const int* __restrict__ in, l
     if (lane == 63) wt[q][w] = inc[q];
   }
   __syncthreads();
-  int run = tsum[blockIdx.x];
+  int run = tsum[blockIdx.x] + top[blockIdx.x / MIDN];
 #pragma unroll
   for (int q = 0; q < SV; q++) {
     int x = run;
@@ -292,8 +322,15 @@ __global__ void __launch_bounds__(SB) k_scan_apply(const int* __restrict__ in, l
   }
 }
 
+// Scratch ints scan_ex_t needs for n elements: the tile sums and the totals
+// of their blocks.
+inline long scan_tmp_ints(long n) {
+  const long nt = (n + STILE - 1) / STILE;
+  return nt + (nt + MIDN - 1) / MIDN + 4;
+}
+
 // out[0..n) = exclusive scan of in (EQ0: of in[e] == 0), out[n] = total.
-// tmp holds n/STILE+1 ints; in and out are 16-byte aligned (whole
+// tmp holds scan_tmp_ints(n) ints; in and out are 16-byte aligned (whole
 // allocations).
 template <bool EQ0>
 int scan_ex_t(const int* in, long n, int* out, int* tmp, hipStream_t s) {
@@ -303,9 +340,12 @@ int scan_ex_t(const int* in, long n, int* out, int* tmp, hipStream_t s) {
     hipMemsetAsync(out, 0, sizeof(int), s);
     return FR_OK;
   }
+  const int nb = (nt + MIDN - 1) / MIDN;
+  int* top = tmp + nt;
   hipLaunchKernelGGL(k_scan_tiles<EQ0>, dim3(nt), dim3(SB), 0, s, in, n, tmp);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SS), 0, s, tmp, nt, out + n);
-  hipLaunchKernelGGL(k_scan_apply<EQ0>, dim3(nt), dim3(SB), 0, s, in, n, tmp, out);
+  hipLaunchKernelGGL(k_scan_mid, dim3(nb), dim3(ST), 0, s, tmp, nt, top);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(ST), 0, s, top, nb, out + n);
+  hipLaunchKernelGGL(k_scan_apply<EQ0>, dim3(nt), dim3(SB), 0, s, in, n, tmp, top, out);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
@@ -317,12 +357,12 @@ int scan_ex(const int* in, long n, int* out, int* tmp, hipStream_t s) {
 // Exclusive scan of in[0..n), n = np - *pK read on the device, in one
 // workgroup; out[n] = total.  For the per-axis leaver lists, whose length
 // the host does not wait for.
-__global__ void __launch_bounds__(SS) k_scan_small(const int* __restrict__ in, int np,
+__global__ void __launch_bounds__(ST) k_scan_small(const int* __restrict__ in, int np,
                                                    const int* __restrict__ pK,
                                                    int* __restrict__ out) {
-  __shared__ int wsum[SS / 64];
+  __shared__ int wsum[ST / 64];
   const int n = np - *pK;
-  const int total = block_scan_rounds(in, n, out, wsum);
+  const int total = block_scan_rounds<ST>(in, n, out, wsum);
   if (threadIdx.x == 0) out[n > 0 ? n : 0] = total;
 }
 
@@ -976,30 +1016,46 @@ __global__ void __launch_bounds__(256) k_dep_vel(Dev d, const int* __restrict__ 
 // cell) a target has a handful of contributions and the kernel streams its
 // output like a stencil.  The sums are the reference's, bit for bit.
 
-// Two-way merge of the target's own bucket [i0, e0) (weight (1-l) R) and the
-// bucket below it along the component [i1, e1) (weight l R).
-__device__ __forceinline__ double merge2_vel(const int* __restrict__ list, int i0, int e0, int i1,
-                                             int e1, const double* __restrict__ L,
-                                             const double* __restrict__ R) {
-  int k0 = i0 < e0 ? list[i0] : INT_MAX;
-  int k1 = i1 < e1 ? list[i1] : INT_MAX;
-  double acc = 0.0;
-  while (k0 != INT_MAX || k1 != INT_MAX) {
-    const bool own = k0 < k1;  // reference indices are distinct
-    const int pos = own ? i0 : i1;
-    const double l = L[pos], r = R[pos];
-    acc = acc + (own ? (1 - l) * r : l * r);
-    const int nx = pos + 1, en = own ? e0 : e1;
-    const int kn = nx < en ? list[nx] : INT_MAX;
-    if (own) {
-      i0 = nx;
-      k0 = kn;
-    } else {
-      i1 = nx;
-      k1 = kn;
-    }
+// Two-way merges of the target's own bucket (weight (1-l) R) with the
+// bucket below it along each component (weight l R), the three components
+// advanced together: each round takes one step of every unfinished merge,
+// so their loads are in flight at once and a wave waits for about as many
+// memory round trips as its longest merge, not their sum.  A merge whose
+// lower bucket is empty (most of them at box densities) reads no reference
+// indices.
+struct Merge2 {
+  int i0, e0, i1, e1, k0, k1;
+  double acc;
+};
+
+__device__ __forceinline__ void merge2_start(Merge2& m, const int* __restrict__ list) {
+  const bool both = m.i0 < m.e0 && m.i1 < m.e1;
+  // with one bucket empty the order is the other bucket's own order: keys
+  // 0 (own) or 1 (lower) stand in for the reference indices
+  m.k0 = m.i0 < m.e0 ? (both ? list[m.i0] : 0) : INT_MAX;
+  m.k1 = m.i1 < m.e1 ? (both ? list[m.i1] : 1) : INT_MAX;
+  m.acc = 0.0;
+}
+
+__device__ __forceinline__ void merge2_step(Merge2& m, const int* __restrict__ list,
+                                            const double* __restrict__ L,
+                                            const double* __restrict__ R) {
+  const bool own = m.k0 < m.k1;  // reference indices are distinct
+  const int pos = own ? m.i0 : m.i1;
+  const double l = L[pos], r = R[pos];
+  m.acc = m.acc + (own ? (1 - l) * r : l * r);
+  const int nx = pos + 1, en = own ? m.e0 : m.e1;
+  // the key of the next entry: its reference index while the other bucket
+  // still has entries, else a placeholder that keeps this bucket chosen
+  const int other = own ? m.k1 : m.k0;
+  const int kn = nx < en ? (other != INT_MAX ? list[nx] : (own ? 0 : 1)) : INT_MAX;
+  if (own) {
+    m.i0 = nx;
+    m.k0 = kn;
+  } else {
+    m.i1 = nx;
+    m.k1 = kn;
   }
-  return acc;
 }
 
 // particle.cpp:498-524 (before the cumulative halo): target t of component M
@@ -1022,22 +1078,43 @@ __global__ void __launch_bounds__(256) k_dep_vel_pull(Dev d, const int* __restri
   if (a >= d.Nx || b >= d.Ny) return;
   const int n = (c * d.Ny + b) * d.Nx + a;
   const int s0 = start[n], e0 = start[n + 1];
-  int sx = 0, ex = 0, sy = 0, ey = 0, sz = 0, ez = 0;
+  Merge2 m[3];
+#pragma unroll
+  for (int M = 0; M < 3; M++) {
+    m[M].i0 = s0;
+    m[M].e0 = e0;
+    m[M].i1 = m[M].e1 = 0;
+  }
   if (a > 0) {
-    sx = start[n - 1];
-    ex = s0;
+    m[0].i1 = start[n - 1];
+    m[0].e1 = s0;
   }
   if (b > 0) {
-    sy = start[n - d.Nx];
-    ey = start[n - d.Nx + 1];
+    m[1].i1 = start[n - d.Nx];
+    m[1].e1 = start[n - d.Nx + 1];
   }
   if (c > 0) {
-    sz = start[n - d.NxNy];
-    ez = start[n - d.NxNy + 1];
+    m[2].i1 = start[n - d.NxNy];
+    m[2].e1 = start[n - d.NxNy + 1];
   }
-  ox[n] = (T)merge2_vel(list, s0, e0, sx, ex, sxl, sru);
-  oy[n] = (T)merge2_vel(list, s0, e0, sy, ey, syl, srv);
-  oz[n] = (T)merge2_vel(list, s0, e0, sz, ez, szl, srw);
+#pragma unroll
+  for (int M = 0; M < 3; M++) merge2_start(m[M], list);
+  const double* L[3] = {sxl, syl, szl};
+  const double* R[3] = {sru, srv, srw};
+  for (;;) {
+    bool any = false;
+#pragma unroll
+    for (int M = 0; M < 3; M++) {
+      if (m[M].k0 != INT_MAX || m[M].k1 != INT_MAX) {
+        merge2_step(m[M], list, L[M], R[M]);
+        any = true;
+      }
+    }
+    if (!any) break;
+  }
+  ox[n] = (T)m[0].acc;
+  oy[n] = (T)m[1].acc;
+  oz[n] = (T)m[2].acc;
 }
 
 // particle.cpp:526-559 / 586-658 (before the cumulative halo): target
@@ -1324,7 +1401,7 @@ static int alloc_buckets(Buckets& B, long ncell, int cap, int nops) {
   FR_CHECK(hipMalloc(&B.key, sizeof(int) * cap));
   FR_CHECK(hipMalloc(&B.slot, sizeof(int) * cap));
   for (int q = 0; q < nops; q++) FR_CHECK(hipMalloc(&B.s[q], sizeof(double) * cap));
-  FR_CHECK(hipMalloc(&B.scan_tmp, sizeof(int) * (ncell / STILE + 2)));
+  FR_CHECK(hipMalloc(&B.scan_tmp, sizeof(int) * scan_tmp_ints(ncell)));
   return FR_OK;
 }
 
@@ -1377,7 +1454,7 @@ int PartDev::alloc(int capacity, const Dev& d) {
   if (!rc) rc = alloc_buckets(T, ncell, cap, 6);
   if (rc) return rc;
   const long big = ncell > cap ? ncell : cap;
-  FR_CHECK(hipMalloc(&mig_tmp, sizeof(int) * (big / STILE + 2)));
+  FR_CHECK(hipMalloc(&mig_tmp, sizeof(int) * scan_tmp_ints(big)));
   FR_CHECK(hipMalloc(&rank_dep, sizeof(int) * cap));
   int** m[11] = {&flg, &nl, &pNL, &holepos, &tailnl, &evt, &islo, &plo, &dst, &mslot, &mlv};
   for (auto pp : m) FR_CHECK(hipMalloc(pp, sizeof(int) * (cap + 1)));
@@ -1385,10 +1462,14 @@ int PartDev::alloc(int capacity, const Dev& d) {
   const long ntile = (long)nblocks(d.Nx, DTX) * nblocks(d.Ny, DTY) * d.Nz;
   FR_CHECK(hipMalloc(&dense_c, sizeof(int) * (ntile + 1)));
   FR_CHECK(hipMalloc(&dense_w, sizeof(int) * (ntile + 1)));
+  FR_RET(t_vel.init());
+  FR_RET(t_cic.init());
   return FR_OK;
 }
 
 void PartDev::release() {
+  t_vel.release();
+  t_cic.release();
   for (int q = 0; q < 7; q++)
     for (int t = 0; t < 4; t++) {
       (void)hipFree(s[q][t]);
@@ -1471,8 +1552,10 @@ int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStre
   // reserves 76 KB of LDS per workgroup.
   const dim3 g = grid3p(d);
   if (use_pull(true)) {
-    hipLaunchKernelGGL(k_dep_cic_pull<T>, g, dim3(DTX, DTY), 0, st, d, R.start, R.list, R.s[0],
-                       R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    t_cic.next(&e0, &e1);
+    hipExtLaunchKernelGGL(k_dep_cic_pull<T>, g, dim3(DTX, DTY), 0, st, e0, e1, 0, d, R.start,
+                          R.list, R.s[0], R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out);
     FR_LAUNCH_CHECK();
     int rc = cum_fold(d, out, st);
     if (!rc && halo) rc = ghost_fill(d, out, st);
@@ -1482,8 +1565,10 @@ int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStre
   const bool defer = (double)np * 8 * DTX * DTY > 0.5 * PCAP * cells && np < (1 << 28);
   int* dl = defer ? (weighted ? dense_w : dense_c) : nullptr;
   if (dl) FR_CHECK(hipMemsetAsync(dl, 0, sizeof(int), st));
-  hipLaunchKernelGGL(k_dep_cic<T>, g, dim3(64, 4), 0, st, d, R.start, R.list, R.s[0], R.s[1],
-                     R.s[2], weighted ? R.s[3] : nullptr, out, dl);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  t_cic.next(&e0, &e1);
+  hipExtLaunchKernelGGL(k_dep_cic<T>, g, dim3(64, 4), 0, st, e0, e1, 0, d, R.start, R.list,
+                        R.s[0], R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out, dl);
   FR_LAUNCH_CHECK();
   if (dl) {
     // two workgroups of k_dep_cic_dense fit a CU (LDS): one resident round
@@ -1501,12 +1586,16 @@ int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStre
 template <typename F>
 int PartDev::deposit_vel(const Dev& d, F* const out[3], hipStream_t st) {
   if (!T.valid) return FR_EINVAL;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  t_vel.next(&e0, &e1);
   if (use_pull(false))
-    hipLaunchKernelGGL(k_dep_vel_pull<F>, grid3p(d), dim3(DTX, DTY), 0, st, d, T.start, T.list,
-                       T.s[0], T.s[1], T.s[2], T.s[3], T.s[4], T.s[5], out[0], out[1], out[2]);
+    hipExtLaunchKernelGGL(k_dep_vel_pull<F>, grid3p(d), dim3(DTX, DTY), 0, st, e0, e1, 0, d,
+                          T.start, T.list, T.s[0], T.s[1], T.s[2], T.s[3], T.s[4], T.s[5], out[0],
+                          out[1], out[2]);
   else
-    hipLaunchKernelGGL(k_dep_vel<F>, grid3p(d), dim3(64, 4), 0, st, d, T.start, T.list, T.s[0],
-                       T.s[1], T.s[2], T.s[3], T.s[4], T.s[5], out[0], out[1], out[2]);
+    hipExtLaunchKernelGGL(k_dep_vel<F>, grid3p(d), dim3(64, 4), 0, st, e0, e1, 0, d, T.start,
+                          T.list, T.s[0], T.s[1], T.s[2], T.s[3], T.s[4], T.s[5], out[0], out[1],
+                          out[2]);
   FR_LAUNCH_CHECK();
   int rc = cum_fold_n(d, out, 3, st);  // the three components, one launch per phase
   return rc ? rc : ghost_fill_n(d, out, 3, st);
@@ -1530,7 +1619,7 @@ int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], 
     if (rc) return rc;
     hipLaunchKernelGGL(k_mig_rank, dim3(nb), dim3(256), 0, st, np, flg, pNL, holepos, tailnl);
     hipLaunchKernelGGL(k_mig_event, dim3(nb), dim3(256), 0, st, np, flg, pNL, tailnl, evt, islo);
-    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(SS), 0, st, islo, np, pNL + np, plo);
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(ST), 0, st, islo, np, pNL + np, plo);
     MigAxis m{ax, L[ax]};
     // h + nleave <= np moves (h <= K front holes, nleave = np - K leavers)
     const unsigned ng = std::min(nblocks(np, 256), 1024u);

@@ -962,6 +962,10 @@ int grid::step() {  // simulation.cpp:36-70 (box calls)
   t_poisson = t_particles = 0;
   iters_step = 0;
   box_poisson_xtimer(PS_.s, nullptr, nullptr, true);
+  part.dev.t_vel.collect();  // launches outside fr_sim_step are not counted
+  part.dev.t_cic.collect();
+  part.dev.t_vel.reset();
+  part.dev.t_cic.reset();
   auto t0 = std::chrono::steady_clock::now();
   for (RK4_count = 0; RK4_count < 4; RK4_count++) {
     FR_TRY(Update_Rho());
@@ -988,6 +992,8 @@ int grid::step() {  // simulation.cpp:36-70 (box calls)
   }
   n_momentum = 4;
   box_poisson_xtimer(PS_.s, &t_xpass, &n_xpass, false);
+  part.dev.t_vel.collect();
+  part.dev.t_cic.collect();
   return FR_OK;
 }
 
@@ -1308,10 +1314,12 @@ int fr_sim_timing(fr_sim* s, double* out) { return fr_sim_timing_ex(s, out, 6); 
 
 int fr_sim_timing_ex(fr_sim* s, double* out, int n) {
   if (!s || !out || n < 0) return FR_EINVAL;
-  const double v[8] = {s->g.t_step,     s->g.t_poisson, s->g.t_particles,
-                       (double)s->g.iters_step, s->g.t_momentum, (double)s->g.n_momentum,
-                       s->g.t_xpass,    (double)s->g.n_xpass};
-  for (int i = 0; i < n && i < 8; i++) out[i] = v[i];
+  const PartDev& p = s->g.part.dev;
+  const double v[12] = {s->g.t_step,     s->g.t_poisson, s->g.t_particles,
+                        (double)s->g.iters_step, s->g.t_momentum, (double)s->g.n_momentum,
+                        s->g.t_xpass,    (double)s->g.n_xpass, p.t_vel.ms, (double)p.t_vel.n,
+                        p.t_cic.ms,      (double)p.t_cic.n};
+  for (int i = 0; i < n && i < 12; i++) out[i] = v[i];
   return FR_OK;
 }
 

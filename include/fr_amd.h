@@ -299,8 +299,10 @@ int fr_sim_diagnostics(fr_sim* s, double* out);
 int fr_sim_timing(fr_sim* s, double* out);
 /* fr_sim_timing with out[6] the summed duration (ms, HIP events on the
  * solver's stream) of the box FFT solve's forward x pass (residual + real
- * FFT, k_pass_x_fwd) in the last step and out[7] its launch count; writes
- * the first min(n, 8) values. */
+ * FFT, k_pass_x_fwd) in the last step and out[7] its launch count, out[8]
+ * and out[9] the same for the momentum deposit kernel, out[10] and out[11]
+ * for the trilinear (concentration and heat) deposit kernels; writes the
+ * first min(n, 12) values. */
 int fr_sim_timing_ex(fr_sim* s, double* out, int n);
 
 #ifdef __cplusplus
