@@ -270,3 +270,76 @@ extern "C" void ref_elementwise(void* h, int op, double* out, double a, const do
   }
   store(O, out);
 }
+
+// Post-process operators of grid::Statistics (tensor0.cpp:575-663,
+// tensor1.cpp:318-439).  tensor0 forms (out holds the prior content, red
+// receives ptr_reduce_): 0 Equal_SijSij(U), 1 Equal_WijWij(U), 2
+// Equal_eta(a, b), 3 Equal_ST(s, a), 4 Equal_Reduced_sd(a), 5 Equal_nu(s, a),
+// 6 Equal_dissipation(a, b).  tensor1 forms (three outputs and their
+// ptr_reduce_): 7 Equal_TKE2(RU, U, R), 8 Equal_UiUi2(RU, U, R), 9
+// Equal_Cor_Rho_U(U, R), 10 Equal_TKE(U, R), 11 Equal_UiUi(U, R); RU, U and R
+// are reduced first, as grid::Statistics has them.  ref_post_scalars returns
+// tensor1::max, min and max_cfl(dt) of U.
+extern "C" void ref_post(void* h, int op, double* o0, double* o1, double* o2, double* red,
+                         const double* const* U3, const double* const* RU3, const double* a,
+                         const double* b, double s) {
+  RefCtx* c = (RefCtx*)h;
+  tensor1 U(c->gs, c->com), RU(c->gs, c->com), O(c->gs, c->com);
+  tensor0 A(c->gs, c->com), B(c->gs, c->com);
+  const int nx = c->gs->Nx();
+  if (U3) {
+    load(U.x, U3[0]);
+    load(U.y, U3[1]);
+    load(U.z, U3[2]);
+  }
+  if (RU3) {
+    load(RU.x, RU3[0]);
+    load(RU.y, RU3[1]);
+    load(RU.z, RU3[2]);
+  }
+  if (a) load(A, a);
+  if (b) load(B, b);
+  load(O.x, o0);
+  if (op >= 7) {
+    load(O.y, o1);
+    load(O.z, o2);
+  }
+  switch (op) {
+    case 0: O.x.Equal_SijSij(U); break;
+    case 1: O.x.Equal_WijWij(U); break;
+    case 2: O.x.Equal_eta(A, B); break;
+    case 3: O.x.Equal_ST(s, A); break;
+    case 4: O.x.Equal_Reduced_sd(A); break;
+    case 5: O.x.Equal_nu(s, A); break;
+    case 6: O.x.Equal_dissipation(A, B); break;
+    default:
+      A.reduceX();
+      if (op <= 8) RU.reduceX(); else U.reduceX();
+      if (op == 7) O.Equal_TKE2(RU, U, A);
+      if (op == 8) O.Equal_UiUi2(RU, U, A);
+      if (op == 9) O.Equal_Cor_Rho_U(U, A);
+      if (op == 10) O.Equal_TKE(U, A);
+      if (op == 11) O.Equal_UiUi(U, A);
+  }
+  store(O.x, o0);
+  if (red) std::memcpy(red, O.x.ptr_reduce_, sizeof(double) * nx);
+  if (op >= 7) {
+    store(O.y, o1);
+    store(O.z, o2);
+    if (red) {
+      std::memcpy(red + nx, O.y.ptr_reduce_, sizeof(double) * nx);
+      std::memcpy(red + 2 * nx, O.z.ptr_reduce_, sizeof(double) * nx);
+    }
+  }
+}
+
+extern "C" void ref_post_scalars(void* h, const double* const* U3, double dt, double* out3) {
+  RefCtx* c = (RefCtx*)h;
+  tensor1 U(c->gs, c->com);
+  load(U.x, U3[0]);
+  load(U.y, U3[1]);
+  load(U.z, U3[2]);
+  out3[0] = U.max();
+  out3[1] = U.min();
+  out3[2] = U.max_cfl(dt);
+}

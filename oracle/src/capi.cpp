@@ -256,7 +256,29 @@ void* orc_grid_create(const char* params_path) {
   g->setup(p);
   return g;
 }
+// The same with the reference's run files (touch.check and the stat files
+// now, info.txt at initialisation), relative to the working directory.
+void* orc_grid_create_ex(const char* params_path, int outputs) {
+  Params p;
+  if (!p.parse(params_path)) return nullptr;
+  Grid* g = new Grid();
+  g->setup(p);
+  g->outputs = outputs != 0;
+  if (g->outputs) g->open_outputs();
+  return g;
+}
 void orc_grid_destroy(void* h) { delete (Grid*)h; }
+// grid::Statistics, grid::Store (returns 0 on success, -1 when a file could
+// not be written, e.g. a particle file that exists), grid::Touch (returns
+// the stop flag), grid::Write_info
+void orc_grid_statistics(void* h) { ((Grid*)h)->Statistics(); }
+int orc_grid_store(void* h) { return ((Grid*)h)->Store() ? 0 : -1; }
+int orc_grid_touch(void* h) { return ((Grid*)h)->Touch() ? 1 : 0; }
+void orc_grid_write_info(void* h) { ((Grid*)h)->Write_info(); }
+int orc_grid_num_timestep_set(void* h, int n) {
+  ((Grid*)h)->num_timestep = n;
+  return 0;
+}
 void orc_grid_initialize(void* h, const double* U, const double* V,
                          const double* W) {
   ((Grid*)h)->Initialize(U, V, W);
@@ -405,3 +427,76 @@ double* orc_grid_part_aux(void* h, int which) {
 }
 }
 
+
+// ---- post-process operators of grid::Statistics (same op codes as the
+// partial reference build's ref_post; RU, U and R reduced first for 7-11)
+extern "C" void orc_post(const orc_geom* gg, int op, double* o0, double* o1, double* o2,
+                         double* red, const double* const* U3, const double* const* RU3,
+                         const double* a, const double* b, double s) {
+  Geom g = mk(gg);
+  F1 U, RU, O;
+  F0 A, B;
+  U.init(&g);
+  RU.init(&g);
+  O.init(&g);
+  A.init(&g);
+  B.init(&g);
+  const int nx = g.Nx;
+  if (U3) {
+    load(U.x, U3[0]);
+    load(U.y, U3[1]);
+    load(U.z, U3[2]);
+  }
+  if (RU3) {
+    load(RU.x, RU3[0]);
+    load(RU.y, RU3[1]);
+    load(RU.z, RU3[2]);
+  }
+  if (a) load(A, a);
+  if (b) load(B, b);
+  load(O.x, o0);
+  if (op >= 7) {
+    load(O.y, o1);
+    load(O.z, o2);
+  }
+  switch (op) {
+    case 0: O.x.Equal_SijSij(U); break;
+    case 1: O.x.Equal_WijWij(U); break;
+    case 2: O.x.Equal_eta(A, B); break;
+    case 3: O.x.Equal_ST(s, A); break;
+    case 4: O.x.Equal_Reduced_sd(A); break;
+    case 5: O.x.Equal_nu(s, A); break;
+    case 6: O.x.Equal_dissipation(A, B); break;
+    default:
+      A.reduceX();
+      if (op <= 8) RU.reduceX(); else U.reduceX();
+      if (op == 7) O.Equal_TKE2(RU, U, A);
+      if (op == 8) O.Equal_UiUi2(RU, U, A);
+      if (op == 9) O.Equal_Cor_Rho_U(U, A);
+      if (op == 10) O.Equal_TKE(U, A);
+      if (op == 11) O.Equal_UiUi(U, A);
+  }
+  store(O.x, o0);
+  if (red) std::memcpy(red, O.x.red.data(), sizeof(double) * nx);
+  if (op >= 7) {
+    store(O.y, o1);
+    store(O.z, o2);
+    if (red) {
+      std::memcpy(red + nx, O.y.red.data(), sizeof(double) * nx);
+      std::memcpy(red + 2 * nx, O.z.red.data(), sizeof(double) * nx);
+    }
+  }
+}
+
+extern "C" void orc_post_scalars(const orc_geom* gg, const double* const* U3, double dt,
+                                 double* out3) {
+  Geom g = mk(gg);
+  F1 U;
+  U.init(&g);
+  load(U.x, U3[0]);
+  load(U.y, U3[1]);
+  load(U.z, U3[2]);
+  out3[0] = U.max();
+  out3[1] = U.min();
+  out3[2] = U.max_cfl(dt);
+}

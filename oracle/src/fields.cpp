@@ -420,6 +420,94 @@ void F1::Equal_TKE(const F1& U, const F0& R) {
   }
   reduceX();
 }
+// tensor0.cpp:597-607: adds to the prior content (the reference never
+// clears it; grid::Statistics passes a field holding RU.y/Rho)
+void F0::Equal_WijWij(const F1& U) {
+  ORC_RANGE(bs, Nx - bs, bs, Ny - bs, bs, Nz - bs) {
+    double& r = (*this)(i, j, k);
+    r += 0.125 * pow(U.x.ddy_C2C(i, j, k) + U.x.ddy_C2C(i + 1, j, k) - U.y.ddx_C2C(i, j, k) -
+                         U.y.ddx_C2C(i, j + 1, k),
+                     2);
+    r += 0.125 * pow(U.x.ddz_C2C(i, j, k) + U.x.ddz_C2C(i + 1, j, k) - U.z.ddx_C2C(i, j, k) -
+                         U.z.ddx_C2C(i, j, k + 1),
+                     2);
+    r += 0.125 * pow(U.y.ddz_C2C(i, j, k) + U.y.ddz_C2C(i, j + 1, k) - U.z.ddy_C2C(i, j, k) -
+                         U.z.ddy_C2C(i, j, k + 1),
+                     2);
+  }
+}
+void F0::Equal_eta(const F0& nu, const F0& eps) {
+  ORC_RANGE(0, Nx, 0, Ny, 0, Nz) {
+    (*this)(i, j, k) = pow(nu(i, j, k) * nu(i, j, k) * nu(i, j, k) / eps(i, j, k), 0.25);
+  }
+  reduceX();
+}
+void F0::Equal_ST(double Tp, const F0& sijsij) {
+  const double Tp2 = 2 * Tp * Tp;
+  ORC_RANGE(0, Nx, 0, Ny, 0, Nz) { (*this)(i, j, k) = sqrt(Tp2 * sijsij(i, j, k)); }
+  reduceX();
+}
+void F0::Equal_Reduced_sd(F0& T) {
+  T.reduceX();
+  ORC_RANGE(0, Nx, 0, Ny, 0, Nz) {
+    (*this)(i, j, k) = (T(i, j, k) - T.red[i]) * (T(i, j, k) - T.red[i]);
+  }
+  reduceX();
+  for (int i = 0; i < Nx; i++) red[i] = sqrt(red[i]);
+}
+double F1::min() const {
+  double mx = x.min(), my = y.min(), mz = z.min();
+  mx = (my < mx) ? my : mx;
+  mx = (mz < mx) ? mz : mx;
+  return mx;
+}
+double F1::max_cfl(double dt) const {
+  double mx = x.max() * dt / x.g->dx;
+  double my = y.max() * dt / x.g->dy;
+  double mz = z.max() * dt / x.g->dz;
+  mx = (my > mx) ? my : mx;
+  mx = (mz > mx) ? mz : mx;
+  return mx;
+}
+void F1::Equal_TKE2(const F1& RU, const F1& U, const F0& R) {
+  const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
+  ORC_RANGE(0, Nx, 0, Ny, 0, Nz) {
+    x(i, j, k) = 0.5 * R(i, j, k) * (U.x(i, j, k) - RU.x.red[i] / R.red[i]) *
+                 (U.x(i, j, k) - RU.x.red[i] / R.red[i]);
+    y(i, j, k) = 0.5 * R(i, j, k) * (U.y(i, j, k) - RU.y.red[i] / R.red[i]) *
+                 (U.y(i, j, k) - RU.y.red[i] / R.red[i]);
+    z(i, j, k) = 0.5 * R(i, j, k) * (U.z(i, j, k) - RU.z.red[i] / R.red[i]) *
+                 (U.z(i, j, k) - RU.z.red[i] / R.red[i]);
+  }
+  reduceX();
+}
+void F1::Equal_UiUi2(const F1& RU, const F1& U, const F0& R) {
+  const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
+  ORC_RANGE(0, Nx, 0, Ny, 0, Nz) {
+    x(i, j, k) = (U.x(i, j, k) - RU.x.red[i] / R.red[i]) * (U.x(i, j, k) - RU.x.red[i] / R.red[i]);
+    y(i, j, k) = (U.y(i, j, k) - RU.y.red[i] / R.red[i]) * (U.y(i, j, k) - RU.y.red[i] / R.red[i]);
+    z(i, j, k) = (U.z(i, j, k) - RU.z.red[i] / R.red[i]) * (U.z(i, j, k) - RU.z.red[i] / R.red[i]);
+  }
+  reduceX();
+}
+void F1::Equal_Cor_Rho_U(const F1& U, const F0& R) {
+  const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
+  ORC_RANGE(0, Nx, 0, Ny, 0, Nz) {
+    x(i, j, k) = (U.x(i, j, k) - U.x.red[i]) * (R(i, j, k) - R.red[i]);
+    y(i, j, k) = (U.y(i, j, k) - U.y.red[i]) * (R(i, j, k) - R.red[i]);
+    z(i, j, k) = (U.z(i, j, k) - U.z.red[i]) * (R(i, j, k) - R.red[i]);
+  }
+  reduceX();
+}
+void F1::Equal_UiUi(const F1& U, const F0& R) {
+  const int Nx = R.Nx, Ny = R.Ny, Nz = R.Nz;
+  ORC_RANGE(0, Nx, 0, Ny, 0, Nz) {
+    x(i, j, k) = (U.x(i, j, k) - U.x.red[i]) * (U.x(i, j, k) - U.x.red[i]);
+    y(i, j, k) = (U.y(i, j, k) - U.y.red[i]) * (U.y(i, j, k) - U.y.red[i]);
+    z(i, j, k) = (U.z(i, j, k) - U.z.red[i]) * (U.z(i, j, k) - U.z.red[i]);
+  }
+  reduceX();
+}
 #undef ORC_RANGE
 
 }  // namespace orc

@@ -45,6 +45,11 @@ void Grid::Initialize(const double* Ub, const double* Vb, const double* Wb) {
       dst[c]->Update_Ghosts();
     }
     RU.x.add(par.Rho0 * par.U0);
+  } else if (par.Initial == 1) {  // grid.cpp:124-182
+    Restart();
+    Finish_Initialize();
+    if (outputs) Write_info();
+    return;
   } else {
     RU.set(0);
     RU.x.set(par.Rho0 * par.U0);
@@ -56,6 +61,7 @@ void Grid::Initialize(const double* Ub, const double* Vb, const double* Wb) {
   T_cur = 0;
   num_timestep = 0;
   Finish_Initialize();
+  if (outputs) Write_info();
 }
 
 // grid.cpp:197-229

@@ -212,6 +212,10 @@ struct F0 {
   void Equal_SijSij(const struct F1& U);
   void Equal_nu(double mu, const F0& Rho);
   void Equal_dissipation(const F0& nu, const F0& sijsij);
+  void Equal_WijWij(const struct F1& U);           // tensor0.cpp:597-607 (+=)
+  void Equal_eta(const F0& nu, const F0& eps);     // tensor0.cpp:633-640
+  void Equal_ST(double Tp, const F0& sijsij);      // tensor0.cpp:642-650
+  void Equal_Reduced_sd(F0& T);                    // tensor0.cpp:652-663
 };
 
 // ---------------------------------------------------------------- tensor1
@@ -279,7 +283,16 @@ struct F1 {
   void make_mean_zero() { x.make_mean_zero(); y.make_mean_zero(); z.make_mean_zero(); }
   void reduceX() { x.reduceX(); y.reduceX(); z.reduceX(); }
   double max() const;  // tensor1.cpp:318-326
+  double min() const;  // tensor1.cpp:328-336
+  double max_cfl(double dt) const;  // tensor1.cpp:356-365
+  void Equal_I_F2C(const F1& T) {  // tensor1.cpp:276-281
+    x.Equal_Ix_F2C(T.x); y.Equal_Iy_F2C(T.y); z.Equal_Iz_F2C(T.z);
+  }
   void Equal_TKE(const F1& U, const F0& R);  // tensor1.cpp:415-426
+  void Equal_TKE2(const F1& RU, const F1& U, const F0& R);  // tensor1.cpp:375-386
+  void Equal_UiUi2(const F1& RU, const F1& U, const F0& R);  // tensor1.cpp:388-399
+  void Equal_Cor_Rho_U(const F1& U, const F0& R);  // tensor1.cpp:401-412
+  void Equal_UiUi(const F1& U, const F0& R);  // tensor1.cpp:428-439
 };
 
 // ---------------------------------------------------------------- FFT
@@ -456,6 +469,19 @@ struct Grid {  // grid.h:16-115
   void duct_step();             // simulation.cpp:36-70, duct calls
   // step diagnostics (grid.cpp:714-866 subset) + mean particle T (extension)
   void diagnostics(double* out);
+  // run files, relative to the working directory (io.cpp): touch.check and
+  // the stat files at creation when `outputs` is set (grid.cpp:25-49),
+  // Statistics (714-866), Store (233-313), Touch (77-98), the Initial = 1
+  // restart (124-182), Write_info (869-973)
+  bool outputs = false;
+  std::string stat_files[15];
+  int Is_touch = 0;
+  void open_outputs();
+  void Statistics();
+  bool Store();
+  bool Touch();
+  bool Restart();
+  void Write_info();
   F0* field0(const std::string& name);
   F1* field1(const std::string& name);
 };

@@ -61,6 +61,9 @@ def orc():
         L.orc_update_ghosts.argtypes = [G, dp, C.c_int]
         L.orc_elementwise.argtypes = [G, C.c_int, dp, C.c_double, C.c_void_p, C.c_double,
                                       C.c_void_p, C.c_double, C.c_void_p]
+        L.orc_post.argtypes = [G, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
+        L.orc_post_scalars.argtypes = [G, C.c_void_p, C.c_double, C.c_void_p]
         L.orc_reduce.argtypes = [G, dp, C.c_int]
         L.orc_reduce.restype = C.c_double
         L.orc_reducex.argtypes = [G, dp, dp]
@@ -82,6 +85,16 @@ def orc():
         L.orc_conv3.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]
         L.orc_grid_create.argtypes = [C.c_char_p]
         L.orc_grid_create.restype = C.c_void_p
+        L.orc_grid_create_ex.argtypes = [C.c_char_p, C.c_int]
+        L.orc_grid_create_ex.restype = C.c_void_p
+        L.orc_grid_statistics.argtypes = [C.c_void_p]
+        L.orc_grid_store.argtypes = [C.c_void_p]
+        L.orc_grid_store.restype = C.c_int
+        L.orc_grid_touch.argtypes = [C.c_void_p]
+        L.orc_grid_touch.restype = C.c_int
+        L.orc_grid_write_info.argtypes = [C.c_void_p]
+        L.orc_grid_num_timestep.argtypes = [C.c_void_p]
+        L.orc_grid_num_timestep.restype = C.c_int
         L.orc_grid_destroy.argtypes = [C.c_void_p]
         L.orc_grid_initialize.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_grid_finish_initialize.argtypes = [C.c_void_p]
@@ -143,6 +156,8 @@ def ref():
         L.ref_part_send_recv_duct.argtypes = [vp, C.c_int, vp]
         L.ref_elementwise.argtypes = [vp, C.c_int, dp, C.c_double, vp, C.c_double, vp,
                                       C.c_double, vp]
+        L.ref_post.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double]
+        L.ref_post_scalars.argtypes = [vp, vp, C.c_double, vp]
         _ref = L
     return _ref
 
@@ -154,10 +169,12 @@ def ptr(a):
 class OracleGrid:
     """The oracle's box grid (grid.cpp restated) driven from a params file."""
 
-    def __init__(self, params_path, ic_dir=None):
+    def __init__(self, params_path, ic_dir=None, outputs=False):
+        """outputs: write the reference's run files (touch.check, stat files,
+        info.txt) relative to the working directory."""
         L = orc()
         self.L = L
-        self.h = L.orc_grid_create(params_path.encode())
+        self.h = L.orc_grid_create_ex(params_path.encode(), 1 if outputs else 0)
         if not self.h:
             raise RuntimeError("oracle: cannot parse " + params_path)
         self.g = Geom()
@@ -225,6 +242,19 @@ class OracleGrid:
     def part_aux(self, which):
         n = self.np_
         return np.ctypeslib.as_array(self.L.orc_grid_part_aux(self.h, which), shape=(n,))
+
+    def statistics(self):
+        self.L.orc_grid_statistics(self.h)
+
+    def store(self):
+        return self.L.orc_grid_store(self.h)
+
+    def touch(self):
+        return self.L.orc_grid_touch(self.h)
+
+    @property
+    def num_timestep(self):
+        return self.L.orc_grid_num_timestep(self.h)
 
     def diagnostics(self):
         out = np.zeros(10)

@@ -296,3 +296,43 @@ def test_elementwise_bit_exact(ctxs, geo):
         O.orc_elementwise(g, op, o_out, a, *args)
         assert np.array_equal(r_out, o_out), (op, ua, ub, uc)
         assert not np.array_equal(o_out, prior) or op == 8
+
+
+def _p3(arrs):
+    import ctypes as C
+    return (C.c_void_p * 3)(*[a.ctypes.data for a in arrs]) if arrs is not None else None
+
+
+@pytest.mark.parametrize("geo", GEOMS[:3], ids=lambda x: str(x))
+def test_statistics_postprocess_bit_exact(ctxs, geo):
+    """The post-process operators grid::Statistics uses (tensor0.cpp:575-663,
+    tensor1.cpp:318-439): every output field and its x-profile (ptr_reduce_),
+    bit-exact, including the += of Equal_WijWij onto prior content and the
+    unwritten ghosts of Equal_SijSij."""
+    import ctypes as C
+    h, og, p, _ = ctxs[tuple(geo)]
+    R, O = OL.ref(), OL.orc()
+    g = og.g
+    shp = OL.shape(g)
+    rng = np.random.default_rng(29)
+    for op in range(12):
+        U = [rng.standard_normal(shp) for _ in range(3)]
+        RU = [rng.standard_normal(shp) for _ in range(3)]
+        a = 0.5 + rng.random(shp)
+        b = 0.5 + rng.random(shp)
+        prior = [rng.standard_normal(shp) for _ in range(3)]
+        s = 0.37
+        ro, oo = [x.copy() for x in prior], [x.copy() for x in prior]
+        rr, orr = np.zeros(3 * g.Nx), np.zeros(3 * g.Nx)
+        args = (_p3(U), _p3(RU), OL.ptr(a), OL.ptr(b), C.c_double(s))
+        R.ref_post(h, op, *[OL.ptr(x) for x in ro], OL.ptr(rr), *args)
+        O.orc_post(g, op, *[OL.ptr(x) for x in oo], OL.ptr(orr), *args)
+        for i in range(3):
+            assert np.array_equal(ro[i], oo[i], equal_nan=True), (op, i)
+        if op >= 2:  # Equal_SijSij and Equal_WijWij leave ptr_reduce_ untouched
+            assert np.array_equal(rr, orr, equal_nan=True), op
+    U = [rng.standard_normal(shp) for _ in range(3)]
+    r3, o3 = np.zeros(3), np.zeros(3)
+    R.ref_post_scalars(h, _p3(U), C.c_double(0.01), OL.ptr(r3))
+    O.orc_post_scalars(g, _p3(U), C.c_double(0.01), OL.ptr(o3))
+    assert np.array_equal(r3, o3)
