@@ -113,6 +113,11 @@ def lib():
             "fr_sim_set_inflow": ([vp, vp, i, vp], i),
             "fr_sim_reserve_particles": ([vp, i], i),
             "fr_sim_set_resort_interval": ([vp, i], i),
+            "fr_sim_statistics": ([vp], i),
+            "fr_sim_store": ([vp], i),
+            "fr_sim_touch": ([vp, C.POINTER(i)], i),
+            "fr_sim_write_info": ([vp], i),
+            "fr_sim_num_timestep": ([vp, C.POINTER(i)], i),
             "fr_sim_diagnostics": ([vp, vp], i),
             "fr_sim_timing": ([vp, vp], i),
             "fr_sim_timing_ex": ([vp, vp, i], i),
@@ -135,18 +140,23 @@ def _p(a):
 
 
 FR_SIM_FP32 = 1
+FR_SIM_OUTPUT = 2
 
 
 class Sim:
     """The box solver held on the device (fr_sim_* of include/fr_amd.h)."""
 
-    def __init__(self, params_path: str, ic_dir: str | None = None, fp32: bool = False):
+    def __init__(self, params_path: str, ic_dir: str | None = None, fp32: bool = False,
+                 outputs: bool = False):
         """fp32 selects the precision form (FR_SIM_FP32): FP32 fields and
-        stencils, FP64 pressure, residual, sums and particles."""
+        stencils, FP64 pressure, residual, sums and particles.  outputs
+        (FR_SIM_OUTPUT) writes the reference's run files (touch.check, stat
+        files, info.txt) relative to the working directory."""
         self.L = lib()
         h = C.c_void_p()
-        check(self.L.fr_sim_create_ex(params_path.encode(), (ic_dir or ".").encode(),
-                                      FR_SIM_FP32 if fp32 else 0, C.byref(h)),
+        flags = (FR_SIM_FP32 if fp32 else 0) | (FR_SIM_OUTPUT if outputs else 0)
+        check(self.L.fr_sim_create_ex(params_path.encode(), (ic_dir or ".").encode(), flags,
+                                      C.byref(h)),
               "fr_sim_create_ex")
         self.h = h
         self.g = Geom()
@@ -216,6 +226,25 @@ class Sim:
 
     def reserve_particles(self, capacity):
         check(self.L.fr_sim_reserve_particles(self.h, int(capacity)), "reserve_particles")
+
+    def statistics(self):
+        check(self.L.fr_sim_statistics(self.h), "statistics")
+
+    def store(self):
+        """0 on success, nonzero (FR_EINVAL) when a file could not be
+        written, e.g. a particle snapshot that exists."""
+        return self.L.fr_sim_store(self.h)
+
+    def touch(self):
+        stop = C.c_int()
+        check(self.L.fr_sim_touch(self.h, C.byref(stop)), "touch")
+        return stop.value
+
+    @property
+    def num_timestep(self):
+        n = C.c_int()
+        check(self.L.fr_sim_num_timestep(self.h, C.byref(n)))
+        return n.value
 
     def set_resort_interval(self, steps):
         check(self.L.fr_sim_set_resort_interval(self.h, int(steps)), "set_resort_interval")

@@ -12,8 +12,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FR_OFFLOAD_ARCH", "gfx950")
 
 KERNELS = ["fields.hip", "box.hip", "duct.hip", "fft_poisson.hip", "duct_poisson.hip",
-           "mg_pcg.hip", "unet.hip", "particles.hip", "diag.hip"]
-HOST = ["host.cpp"]
+           "mg_pcg.hip", "unet.hip", "particles.hip", "diag.hip", "stats.hip"]
+HOST = ["host.cpp", "host_io.cpp"]
 
 # -ffp-contract=off: the FP64 path reproduces the reference's unfused
 # multiply-add sequences.  The packed-FP32 feature is disabled for all device

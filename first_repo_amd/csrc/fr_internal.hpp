@@ -134,6 +134,31 @@ int diag_diss(const Dev& d, const double* const u[3], const double* rho, double 
               double* out, hipStream_t s);
 int vsum(const double* v, int n, double* part257, double* host, hipStream_t s);
 
+// ---- statistics (stats.hip) ------------------------------------------------------------
+// Whole-array operators of grid::Statistics with per-x plane means (ra: three
+// profiles of Nx doubles, rr: one): o[m] receives, per ghosted cell,
+//   ST_RSD    (r - rr[i])^2                              (Equal_Reduced_sd)
+//   ST_TKE2   0.5 r (a_m - ra_m[i]/rr[i])^2               (Equal_TKE2)
+//   ST_UIUI2  (a_m - ra_m[i]/rr[i])^2                     (Equal_UiUi2)
+//   ST_CORRU  (a_m - ra_m[i]) (r - rr[i])                 (Equal_Cor_Rho_U)
+//   ST_TKE    0.5 r (a_m - ra_m[i])^2                     (Equal_TKE)
+//   ST_UIUI   (a_m - ra_m[i])^2                           (Equal_UiUi)
+//   ST_ST     sqrt(s a_0)                                 (Equal_ST, s = 2 Tp^2)
+//   ST_NU     s / r                                       (Equal_nu, s = Mu0)
+//   ST_DISS   2 a_0 b_0                                   (Equal_dissipation)
+//   ST_ETA    (a_0^3 / b_0)^0.25                          (Equal_eta)
+//   ST_DIV0   b_m / r                                     (tensor1 / tensor0)
+//   ST_DIV1   a_m / b_m                                   (tensor1 / tensor1)
+enum { ST_RSD, ST_TKE2, ST_UIUI2, ST_CORRU, ST_TKE, ST_UIUI, ST_ST, ST_NU, ST_DISS, ST_ETA,
+       ST_DIV0, ST_DIV1 };
+int stat_point(const Dev& d, int op, double* const o[3], const double* const a[3],
+               const double* const b[3], const double* r, const double* ra, const double* rr,
+               double s, hipStream_t st);
+// Equal_SijSij (rotation false) or Equal_WijWij (true, added to out) at
+// interior cells
+int stat_strain(const Dev& d, bool rotation, const double* const u[3], double* out,
+                hipStream_t st);
+
 // ---- particles (particles.hip) -------------------------------------------------------
 struct PartPtrs {
   double* s[7][4];  // x,y,z,u,v,w,T  x  base,_int,_new,_np1

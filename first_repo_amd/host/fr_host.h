@@ -321,6 +321,23 @@ class grid {
   long n_xpass = 0;
   int iters_step = 0;
   double* dred = nullptr;  // device scratch for diagnostics (Nx doubles x 4)
+  // ---- run files (host_io.cpp), relative to the working directory as in
+  // the reference: outputs enables touch.check at creation and info.txt at
+  // initialisation; Statistics writes the stat files when params.in asks
+  bool outputs = false;
+  int Is_touch = 0;
+  tensor1 st_a, st_b, st_c;  // Statistics scratch (U, dummy / dummy3, dummy2)
+  tensor0 io_wide[12];       // FP64 copies of FP32 fields for the run files
+  double* dprof = nullptr;   // device x-profiles of Statistics (10 x Nx)
+  std::vector<double> inflow_h;  // duct: host copy of the last inflow planes
+  int open_outputs();        // grid.cpp:25-49
+  int Statistics();          // grid.cpp:714-866
+  int Store();               // grid.cpp:233-313, particle.cpp:867-917
+  int Touch(bool* stop);     // grid.cpp:77-98
+  int Restart();             // grid.cpp:124-182, particle.cpp:961-1025
+  int Write_info();          // grid.cpp:869-973
+  const double* wide(tensor0& f, int slot);
+  int download(tensor0& f, std::vector<double>& h, int slot);
 
   grid() {}
   ~grid();

@@ -439,6 +439,7 @@ int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS) {
 
 grid::~grid() {
   if (dred) hipFree(dred);
+  if (dprof) hipFree(dprof);
   if (inflow) hipFree(inflow);
   // a grid whose setup stopped early holds null handles; destroying those
   // would leave an error behind for the next launch check
@@ -552,8 +553,13 @@ int grid::Initialize() {
   } else if (param.Initial() == 2) {
     RU.assign(0);
     RU.x.assign(param.Rho0() * param.U0());
+  } else if (param.Initial() == 1) {  // grid.cpp:124-182
+    FR_TRY(Restart());
+    FR_TRY(Finish_Initialize());
+    if (outputs) FR_TRY(Write_info());
+    return FR_OK;
   } else {
-    return FR_EINVAL;  // restart files: not built here
+    return FR_EINVAL;
   }
   Rho.assign(param.Rho0());
   P.assign(0);
@@ -565,7 +571,9 @@ int grid::Initialize() {
   P0 = param.P0();
   T_cur = 0;
   num_timestep = 0;
-  return Finish_Initialize();
+  FR_TRY(Finish_Initialize());
+  if (outputs) FR_TRY(Write_info());
+  return FR_OK;
 }
 
 // grid.cpp:197-229: the time-integration stages, Rho_face, the particle
@@ -601,9 +609,11 @@ int grid::Finish_Initialize() {
 
 int grid::set_inflow(const double* planes, int n, const double* particles) {
   if (param.IsBox() || n < 0) return FR_EINVAL;
-  if (planes)
-    FR_CHECK(hipMemcpy(inflow, planes, sizeof(double) * 8 * (size_t)size.Ny() * size.Nz(),
-                       hipMemcpyHostToDevice));
+  if (planes) {
+    const size_t n = 8 * (size_t)size.Ny() * size.Nz();
+    FR_CHECK(hipMemcpy(inflow, planes, sizeof(double) * n, hipMemcpyHostToDevice));
+    inflow_h.assign(planes, planes + n);
+  }
   if (n > 0 && particles) inlet.insert(inlet.end(), particles, particles + (size_t)n * 29);
   return FR_OK;
 }
@@ -1089,10 +1099,12 @@ int fr_sim_create(const char* params_path, const char* ic_dir, fr_sim** out) {
 }
 
 int fr_sim_create_ex(const char* params_path, const char* ic_dir, int flags, fr_sim** out) {
-  if (!params_path || !out || (flags & ~FR_SIM_FP32)) return FR_EINVAL;
+  if (!params_path || !out || (flags & ~(FR_SIM_FP32 | FR_SIM_OUTPUT))) return FR_EINVAL;
   fr_sim* s = new fr_sim();
   s->g.f32 = (flags & FR_SIM_FP32) != 0;
+  s->g.outputs = (flags & FR_SIM_OUTPUT) != 0;
   int rc = s->g.setup(params_path, ic_dir);
+  if (!rc && s->g.outputs) rc = s->g.open_outputs();
   if (rc) {
     delete s;
     return rc;
@@ -1302,6 +1314,26 @@ int fr_sim_set_collect(fr_sim* s, fr_unet_data* d) {
 int fr_sim_learned_stats(fr_sim* s, int* guesses_kept, int* fallbacks, double* log10_gain) {
   if (!s || !s->g.PS_.sp) return FR_EINVAL;
   return fr_pcg_learned_stats(s->g.PS_.sp, guesses_kept, fallbacks, log10_gain);
+}
+
+int fr_sim_statistics(fr_sim* s) { return s ? s->g.Statistics() : FR_EINVAL; }
+
+int fr_sim_store(fr_sim* s) { return s ? s->g.Store() : FR_EINVAL; }
+
+int fr_sim_touch(fr_sim* s, int* stop) {
+  if (!s || !stop) return FR_EINVAL;
+  bool b = false;
+  int rc = s->g.Touch(&b);
+  *stop = b ? 1 : 0;
+  return rc;
+}
+
+int fr_sim_write_info(fr_sim* s) { return s ? s->g.Write_info() : FR_EINVAL; }
+
+int fr_sim_num_timestep(fr_sim* s, int* n) {
+  if (!s || !n) return FR_EINVAL;
+  *n = s->g.num_timestep;
+  return FR_OK;
 }
 
 int fr_sim_set_resort_interval(fr_sim* s, int steps) {

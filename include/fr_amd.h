@@ -233,6 +233,11 @@ int fr_sim_create(const char* params_path, const char* ic_dir, fr_sim** out);
  * solver (Solver = 0) only, else FR_EINVAL.  No counterpart in the
  * reference, which runs FP64 throughout (grid.h:16-115). */
 #define FR_SIM_FP32 1
+/* FR_SIM_OUTPUT: the reference's run files, relative to the working
+ * directory as grid.cpp writes them: touch.check ("0") and, when params.in
+ * sets Statistics, the 15 stat .dat files (opened for append) at creation
+ * (grid.cpp:25-49); info.txt at fr_sim_initialize (grid.cpp:229, 869-973). */
+#define FR_SIM_OUTPUT 2
 int fr_sim_create_ex(const char* params_path, const char* ic_dir, int flags, fr_sim** out);
 int fr_sim_destroy(fr_sim* s);
 /* grid::Initialize (grid.cpp:110-231) */
@@ -280,6 +285,28 @@ int fr_sim_reserve_particles(fr_sim* s, int capacity);
 int fr_sim_set_learned(fr_sim* s, fr_unet* net, int mode);
 int fr_sim_set_collect(fr_sim* s, fr_unet_data* d);
 int fr_sim_learned_stats(fr_sim* s, int* guesses_kept, int* fallbacks, double* log10_gain);
+/* grid::Statistics (grid.cpp:714-866): when params.in sets Statistics, the 32
+ * x-profile files stat_dir/<name>.bin (Nx_tot doubles appended per call),
+ * the .dat lines "T_cur value" in the stream's default format and
+ * TIME.bin, computed on the device.  Paths relative to the working
+ * directory. */
+int fr_sim_statistics(fr_sim* s);
+/* grid::Store (grid.cpp:233-313) with particle::Store_All
+ * (particle.cpp:867-917): XYZ.bin at step 1; RU_<n>.bin (+ RU_S_<n>.bin in
+ * the duct) and numbers_<n>.dat every Data_freq_fast steps; Part_*_<n>.bin
+ * (base stage, reference order, created exclusively: FR_EINVAL if one
+ * exists), C_<n>.bin, Rho_<n>.bin (+ Rho_S_<n>.bin) every Data_freq_slow
+ * steps; both kinds after a touch stop.  Restart files with the prefix
+ * "Restart_" are read by fr_sim_initialize when params.in has Initial = 1
+ * (grid.cpp:124-182, particle.cpp:961-1025). */
+int fr_sim_store(fr_sim* s);
+/* grid::Touch (grid.cpp:77-98): reads touch.check; *stop = 1 (after a store)
+ * when it holds 1. */
+int fr_sim_touch(fr_sim* s, int* stop);
+/* grid::Write_info (grid.cpp:869-973): info.txt for one rank. */
+int fr_sim_write_info(fr_sim* s);
+/* Steps taken since the initial condition (grid::num_timestep). */
+int fr_sim_num_timestep(fr_sim* s, int* n);
 /* Steps between re-sorts of the particle storage into cell order (default 4,
  * or FR_RESORT_EVERY at creation).  A tuning knob with no counterpart in the
  * reference: storage order never changes a result, which stays in the

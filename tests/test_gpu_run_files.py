@@ -1,0 +1,167 @@
+"""Run files written by the device solver against the oracle's: snapshots
+(grid::Store, grid.cpp:233-313; particle::Store_All, particle.cpp:867-917),
+the Initial = 1 restart (grid.cpp:124-182; particle.cpp:961-1025), the
+statistics files (grid::Statistics, grid.cpp:714-866), info.txt
+(grid.cpp:869-973) and the touch.check stop (grid.cpp:77-98).
+
+Bars: particle flags, numbers files and info.txt identical; field and
+particle snapshots within the step bar (1e-12 normwise relative); a restart
+from the same files starts bit-identical to the oracle's; stat profiles
+within 1e-10 normwise per file (tree against serial plane sums) and the
+.dat values within their printed precision."""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import first_repo_amd as fr
+import oracle_lib as OL
+from first_repo_amd.configs import BoxWorkload, DuctWorkload
+from gpu_util import rel
+
+pytestmark = pytest.mark.gpu
+
+
+class Cwd:
+    def __init__(self, path):
+        self.path = str(path)
+
+    def __enter__(self):
+        self.old = os.getcwd()
+        os.makedirs(self.path, exist_ok=True)
+        os.chdir(self.path)
+        os.makedirs("data", exist_ok=True)
+        os.makedirs("stat", exist_ok=True)
+        return self
+
+    def __exit__(self, *a):
+        os.chdir(self.old)
+
+
+def _box(d, **extra):
+    w = BoxWorkload(n=16, n_particles=3000, steps=3, seed=8,
+                    extra={"Data_dir": "./data/", "Stat_dir": "./stat/", "Statistics": 1,
+                           "Data_freq_fast": 1, "Data_freq_slow": 1, **extra})
+    w.build(str(d))
+    return os.path.join(str(d), "params.in")
+
+
+def _run_pair(tmp_path, steps=2):
+    params = _box(tmp_path / "case")
+    gdir, odir = tmp_path / "gpu", tmp_path / "orc"
+    with Cwd(gdir):
+        sim = fr.Sim(params, str(tmp_path / "case"), outputs=True)
+        sim.initialize()
+        for _ in range(steps):
+            sim.step()
+            sim.statistics()
+            assert sim.store() == 0
+    with Cwd(odir):
+        og = OL.OracleGrid(params, str(tmp_path / "case"), outputs=True)
+        og.initialize()
+        for _ in range(steps):
+            og.step()
+            og.statistics()
+            assert og.store() == 0
+    return sim, og, gdir, odir, params
+
+
+def test_snapshots_statistics_and_info_match_oracle(tmp_path):
+    sim, og, gdir, odir, _ = _run_pair(tmp_path)
+    g = lambda f: os.path.join(gdir, f)  # noqa: E731
+    o = lambda f: os.path.join(odir, f)  # noqa: E731
+    assert open(g("info.txt")).read() == open(o("info.txt")).read()
+    assert open(g("touch.check")).read() == "0"
+    for n in (1, 2):
+        assert np.array_equal(np.fromfile(g(f"data/Part_flag_{n}.bin"), np.int32),
+                              np.fromfile(o(f"data/Part_flag_{n}.bin"), np.int32))
+        for q in "xyzuvwT":
+            a, b = np.fromfile(g(f"data/Part_{q}_{n}.bin")), np.fromfile(o(f"data/Part_{q}_{n}.bin"))
+            assert rel(a, b) <= 1e-12, (n, q)
+        for f in ("RU", "Rho", "C"):
+            a, b = np.fromfile(g(f"data/{f}_{n}.bin")), np.fromfile(o(f"data/{f}_{n}.bin"))
+            assert a.size == b.size and rel(a, b) <= 1e-12, (n, f)
+        gl = open(g(f"data/numbers_{n}.dat")).read().split()
+        ol = open(o(f"data/numbers_{n}.dat")).read().split()
+        assert gl[2:] == ol[2:]
+        for a, b in zip(gl[:2], ol[:2]):
+            assert abs(float(a) - float(b)) <= 1e-5 * abs(float(b))
+    assert np.array_equal(np.fromfile(g("data/XYZ.bin")), np.fromfile(o("data/XYZ.bin")))
+    names = sorted(f for f in os.listdir(o("stat")))
+    assert sorted(os.listdir(g("stat"))) == names
+    for f in names:
+        if f.endswith(".bin"):
+            a, b = np.fromfile(g("stat/" + f)), np.fromfile(o("stat/" + f))
+            assert a.size == b.size, f
+            assert rel(a, b) <= 1e-10, (f, rel(a, b))
+        else:
+            ga = open(g("stat/" + f)).read().split()
+            oa = open(o("stat/" + f)).read().split()
+            assert len(ga) == len(oa), f
+            for a, b in zip(ga, oa):
+                assert abs(float(a) - float(b)) <= 1e-5 * max(abs(float(b)), 1e-300), (f, a, b)
+    # a second store of the same snapshot fails on the exclusive particle files
+    with Cwd(gdir):
+        assert sim.store() != 0
+
+
+def test_restart_matches_oracle(tmp_path):
+    sim, og, gdir, odir, params = _run_pair(tmp_path)
+    rd = tmp_path / "restart"
+    os.makedirs(rd)
+    for src, dst in [("RU_2.bin", "Restart_RU.bin"), ("Rho_2.bin", "Restart_Rho.bin"),
+                     ("numbers_2.dat", "Restart_numbers.dat")] + \
+                    [(f"Part_{q}_2.bin", f"Restart_Part_{q}.bin") for q in
+                     ("x", "y", "z", "u", "v", "w", "T", "flag")]:
+        shutil.copy(os.path.join(gdir, "data", src), rd / dst)
+    txt = open(params).read().replace("# Initial\n0\n", "# Initial\n1\n")
+    open(rd / "params.in", "w").write(txt)
+    with Cwd(rd):
+        rs = fr.Sim("params.in", str(rd))
+        rs.initialize()
+        ro = OL.OracleGrid("params.in", None)  # no U/V/W.bin: the restart reads its files
+        ro.initialize()
+        assert rs.num_timestep == ro.num_timestep == 2
+        for c in range(3):
+            assert np.array_equal(rs.field("RU", c), ro.field("RU", c))
+        for f in ("Rho", "P", "Rho_int", "Rho_np1"):
+            assert np.array_equal(rs.field(f), ro.field(f)), f
+        assert rs.scalar("P0") == ro.scalar("P0")[0]
+        assert np.array_equal(rs.flags(), ro.flag())
+        for q in range(7):
+            for s in (0, 1, 3):
+                assert np.array_equal(rs.part(q, s), ro.part(q, s)), (q, s)
+        rs.step()
+        ro.step()
+        for f in ("Rho", "P"):
+            assert rel(rs.field(f), ro.field(f)) <= 1e-12, f
+        for c in range(3):
+            assert rel(rs.field("RU", c), ro.field("RU", c)) <= 1e-12
+        assert np.array_equal(rs.flags(), ro.flag())
+
+
+def test_duct_outlet_slices_and_touch(tmp_path):
+    w = DuctWorkload(n=(32, 16, 16), L=(4.0, 2.0, 2.0), inject_per_step=50, capacity=2000,
+                     extra={"Data_dir": "./data/", "Stat_dir": "./stat/",
+                            "Data_freq_fast": 1000, "Data_freq_slow": 1000})
+    case = tmp_path / "case"
+    w.build(str(case))
+    with Cwd(tmp_path / "gpu"):
+        sim = fr.Sim(str(case / "params.in"), str(case), outputs=True)
+        sim.reserve_particles(w.capacity)
+        sim.initialize()
+        sim.set_inflow(w.inflow(0), w.inlet_particles(0, 0))
+        sim.step()
+        assert sim.touch() == 0
+        assert not os.path.exists("data/RU_1.bin")
+        open("touch.check", "w").write("1")
+        assert sim.touch() == 1  # the stop stores both snapshot kinds
+        ru_s = np.fromfile("data/RU_S_1.bin")
+        rho_s = np.fromfile("data/Rho_S_1.bin")
+        assert ru_s.size == 3 * 16 * 16 and rho_s.size == 16 * 16
+        for c in range(3):
+            assert np.array_equal(ru_s[c * 256:(c + 1) * 256].reshape(16, 16),
+                                  sim.field("RU", c)[1:-1, 1:-1, -1])
+        assert np.array_equal(rho_s.reshape(16, 16), sim.field("Rho")[1:-1, 1:-1, -1])
+        assert np.fromfile("data/Part_flag_1.bin", np.int32).size == sim.np_ == 50
