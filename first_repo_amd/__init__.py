@@ -121,6 +121,11 @@ def lib():
             "fr_sim_diagnostics": ([vp, vp], i),
             "fr_sim_timing": ([vp, vp], i),
             "fr_sim_timing_ex": ([vp, vp, i], i),
+            "fr_pair_create": ([vp, vp, C.POINTER(vp)], i),
+            "fr_pair_initialize": ([vp], i),
+            "fr_pair_step": ([vp], i),
+            "fr_pair_touch": ([vp, C.POINTER(i)], i),
+            "fr_pair_destroy": ([vp], i),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -269,6 +274,41 @@ class Sim:
     def close(self):
         if getattr(self, "h", None):
             self.L.fr_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pair:
+    """The coupled run (fr_pair_* of include/fr_amd.h, simulation.cpp:19-73):
+    a periodic box Sim feeding a duct Sim of the same Ny, Nz, Ly, Lz.  The
+    pair keeps both Sims alive; close it before them."""
+
+    def __init__(self, box: Sim, duct: Sim):
+        self.L = lib()
+        self.box, self.duct = box, duct
+        h = C.c_void_p()
+        check(self.L.fr_pair_create(box.h, duct.h, C.byref(h)), "fr_pair_create")
+        self.h = h
+
+    def initialize(self):
+        check(self.L.fr_pair_initialize(self.h), "fr_pair_initialize")
+
+    def step(self):
+        check(self.L.fr_pair_step(self.h), "fr_pair_step")
+
+    def touch(self):
+        stop = C.c_int()
+        check(self.L.fr_pair_touch(self.h, C.byref(stop)), "fr_pair_touch")
+        return stop.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fr_pair_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -10,6 +10,7 @@ its discrete divergence (tensor0::Equal_Div_F2C) vanishes to round-off.
 """
 from __future__ import annotations
 
+import dataclasses
 import math
 import os
 from dataclasses import dataclass, field, asdict
@@ -383,3 +384,46 @@ class DuctWorkload:
 
 
 WORKLOADS["duct512"] = DuctWorkload()
+
+
+@dataclass
+class PairWorkload:
+    """The coupled run of the reference (simulation.cpp:19-73): a periodic
+    box of n^3 cells (the box workload's turbulence and particles, convected
+    at U0 = u_conv) feeding a duct of ratio*n x n x n cells and length
+    ratio*L (IsBox = 0, Initial = 2: uniform stream at U0, no particles)
+    through its x = Lx face.  Both grids take the box's dt, gas and particle
+    properties; the duct is not cooled and has no forcing, as in the
+    reference's params_IO.in, which also writes its run files to ./data_IO/
+    and ./stat_IO/.  Files: box/params.in with box/U,V,W.bin and
+    duct/params.in."""
+    box: BoxWorkload = field(default_factory=lambda: BoxWorkload(n=64, n_particles=100_000))
+    ratio: int = 4
+    u_conv: float = 1.0
+    cfl: float = 0.5
+    duct_capacity: int = 0  # 0: ratio x the box's particles x 1.5
+
+    def build(self, outdir: str) -> dict:
+        n, L = self.box.n, 2 * math.pi
+        bdir, ddir = os.path.join(outdir, "box"), os.path.join(outdir, "duct")
+        os.makedirs(bdir, exist_ok=True)
+        os.makedirs(ddir, exist_ok=True)
+        U, V, W = solenoidal_velocity((n, n, n), (L, L, L), self.box.seed)
+        for name, a in (("U.bin", U), ("V.bin", V), ("W.bin", W)):
+            a.astype("<f8").tofile(os.path.join(bdir, name))
+        pb = self.box.derive(U, V, W)
+        umax = float(max(np.abs(U).max(), np.abs(V).max(), np.abs(W).max()))
+        pb.U0 = self.u_conv
+        pb.dt = self.cfl * (L / n) / (umax + self.u_conv)
+        pb.T_final = pb.dt * self.box.steps * 1.5
+        pb.write(os.path.join(bdir, "params.in"))
+        pd = dataclasses.replace(pb)
+        pd.IsBox, pd.Nx_tot, pd.Lx = 0, self.ratio * n, self.ratio * L
+        pd.Initial, pd.Cooling, pd.A1, pd.A2 = 2, 0, 0.0, 0.0
+        pd.Data_dir, pd.Stat_dir = "./data_IO/", "./stat_IO/"
+        pd.write(os.path.join(ddir, "params.in"))
+        cap = self.duct_capacity or int(1.5 * self.ratio * self.box.n_particles)
+        return {"box": bdir, "duct": ddir, "dt": pb.dt, "duct_capacity": cap}
+
+
+WORKLOADS["pair64"] = PairWorkload()

@@ -309,6 +309,38 @@ int duct_rhs(const Dev& d, double sc, const double* const ru_new[3], const doubl
   return FR_OK;
 }
 
+// The duct inlet of a coupled run reads the box's first interior slice
+// (grid.cpp:352-363, 485-503, bs = 1): plane q of `out` ([8][Nz][Ny], the
+// order rho_new, rho_np1, RU_new x,y,z, RU_np1 x,y,z) is src[q] of the box at
+// x index xi[q] over the whole ghosted (j, k) plane.  One lane per (j, k).
+struct PlaneSrc {
+  const double* p[8];
+  int xi[8];
+};
+
+__global__ void __launch_bounds__(256) k_pair_planes(Dev b, PlaneSrc s, double* __restrict__ out) {
+  const long m = (long)b.Ny * b.Nz;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const int j = (int)(t % b.Ny), k = (int)(t / b.Ny);
+#pragma unroll
+  for (int q = 0; q < 8; q++) out[q * m + t] = s.p[q][at(b, s.xi[q], j, k)];
+}
+
+int pair_planes(const Dev& b, const double* const src[8], const int xi[8], double* out,
+                hipStream_t s) {
+  PlaneSrc ps;
+  for (int q = 0; q < 8; q++) {
+    if (xi[q] < 0 || xi[q] >= b.Nx) return FR_EINVAL;
+    ps.p[q] = src[q];
+    ps.xi[q] = xi[q];
+  }
+  hipLaunchKernelGGL(k_pair_planes, dim3(nblocks((long)b.Ny * b.Nz, 256)), dim3(256), 0, s, b,
+                     ps, out);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
 int duct_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, double* const ru_np1[3],
                       double* const ru_new[3], hipStream_t s) {
   hipLaunchKernelGGL(k_duct_update_ru_wp, grid3d(d), dim3(64, 4), 0, s, d, cf, P, ru_np1[0],

@@ -104,6 +104,10 @@ int duct_rhs(const Dev& d, double sc, const double* const ru_new[3], const doubl
              const double* dvg, double* rhs, hipStream_t s);
 int duct_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, double* const ru_np1[3],
                       double* const ru_new[3], hipStream_t s);
+// the duct's 8 inflow planes ([8][Nz][Ny]) from the box grid b: plane q is
+// src[q] at x index xi[q] over the ghosted (j, k) plane
+int pair_planes(const Dev& b, const double* const src[8], const int xi[8], double* out,
+                hipStream_t s);
 
 // ---- box pressure solve (fft_poisson.hip) -------------------------------------
 // fr_box_poisson_solve with, when inv is set, Rho_face passed in place of C
@@ -296,9 +300,15 @@ class PartDev {
   template <typename F>
   int deposit_vel(const Dev& d, F* const out[3], hipStream_t st);
   // periodic migration (box); with duct set, x leavers are dropped and the
-  // n_in inlet particles of `inj` (29 values each, host memory) appended
+  // n_in inlet particles of `inj` (29 values each; host memory, or device
+  // memory read on st when inj_dev) appended
   int migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st,
-              bool duct = false, const double* inj = nullptr, int n_in = 0);
+              bool duct = false, const double* inj = nullptr, int n_in = 0,
+              bool inj_dev = false);
+  // after a periodic migrate on st: the x pass's '>hi' leavers (the
+  // reference's Rbuf1) as 29-value rows into buf (at most cap rows), their
+  // count into *n_out (device)
+  int gather_hi(double* buf, int cap, int* n_out, hipStream_t st);
   void swap_stages(int a, int b);
   void swap_storage();  // exchange the storage set with `alt`
   int copy_stage(int dst_stage, int src_stage, hipStream_t st);

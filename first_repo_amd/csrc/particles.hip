@@ -19,9 +19,10 @@
 // Storage order is not the reference order.  Every storage slot carries the
 // particle's reference index (`rank`, with its inverse `inv`); the reference
 // order only decides the order of deposit sums and the outcome of migration,
-// which therefore rewrites ranks instead of moving particle data.  Once per
-// time step the slots are re-sorted by cell so gathers and bucket builds touch
-// memory in cell order.
+// which therefore rewrites ranks instead of moving particle data.  Every few
+// time steps (4 by default, fr_sim_set_resort_interval) the slots are
+// re-sorted by cell so gathers and bucket builds touch memory in nearly cell
+// order.
 #include <algorithm>
 #include <climits>
 #include <cstdint>
@@ -385,7 +386,7 @@ __device__ __forceinline__ long bucket_cell(const Dev& d, int mode, double x, do
   return clampi((long)kk * d.NxNy + (long)jj * d.Nx + ii, d.size);
 }
 
-// Storage is in cell order (re-sorted every step), so neighbouring lanes
+// Storage is in near cell order (re-sorted every few steps), so neighbouring lanes
 // mostly share a cell: each run of equal cells in consecutive lanes takes its
 // slots with one atomic (its first lane's), the others offset from it.  Slots
 // only have to be distinct within a cell; the placement orders by reference
@@ -1337,6 +1338,36 @@ __global__ void k_inject(int n, int np0, const double* __restrict__ buf, PartPtr
   inv[u] = u;
 }
 
+// The box's '>hi' list of its x pass, which the duct of a coupled run
+// receives as its Rbuf1 (particle.cpp:821, 836-838).  After the x pass the
+// leavers hold reference indices K..np-1, the '<lo' list first, so the '>hi'
+// list is the last n_hi = (np - K) - n_lo indices in packing order, with
+// K = pNL[np] and n_lo = plo[np - K] as the x pass left them.  Each row is
+// the ToBuf layout (particle.cpp:252-271): 7 quantities x 4 stages, then the
+// flag.  Grid-stride over the rows; block 0 lane 0 stores the count.
+__global__ void __launch_bounds__(256) k_gather_hi(int np, const int* __restrict__ pNL,
+                                                   const int* __restrict__ plo,
+                                                   const int* __restrict__ inv, PartPtrs p,
+                                                   double* __restrict__ buf, int cap,
+                                                   int* __restrict__ n_out) {
+  int nhi = 0;
+  if (np > 0) {
+    const int nleave = np - pNL[np];
+    nhi = nleave - plo[nleave];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *n_out = nhi;
+  const int n = nhi < cap ? nhi : cap;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int u = inv[np - nhi + t];
+    double* b = buf + (long)t * 29;
+#pragma unroll
+    for (int q = 0; q < 7; q++)
+#pragma unroll
+      for (int st = 0; st < 4; st++) b[q * 4 + st] = p.s[q][st][u];
+    b[28] = (double)p.flag[u];
+  }
+}
+
 // ---- storage order ------------------------------------------------------------------
 __global__ void k_iota2(int n, int* __restrict__ a, int* __restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1605,7 +1636,7 @@ int PartDev::deposit_vel(const Dev& d, F* const out[3], hipStream_t st) {
 // event order is computed on reference indices; only the reference indices of
 // the moved particles change, and the leavers' coordinates are wrapped.
 int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], hipStream_t st,
-                     bool duct, const double* inj, int n_in) {
+                     bool duct, const double* inj, int n_in, bool inj_dev) {
   // The leaver count K of each axis stays on the device: the kernels read it
   // from pNL[np] and loop over the moves, so the host does not wait for it
   // (the duct's outflow, which shrinks the set, is the exception).
@@ -1651,23 +1682,39 @@ int PartDev::migrate(const double lo[3], const double hi[3], const double L[3], 
   }
   flg_zero = true;
   if (duct && n_in > 0) {  // FromBuf of the inlet particles (particle.cpp:286-301)
-    if (np + n_in > cap) return FR_EOVERFLOW;
-    if (n_in > inj_cap) {
-      (void)hipFree(inj_buf);
-      inj_buf = nullptr;
-      FR_CHECK(hipMalloc(&inj_buf, sizeof(double) * 29 * (size_t)n_in));
-      inj_cap = n_in;
+    if (np + n_in > cap) {
+      fprintf(stderr, "fr_amd: duct particle capacity %d exceeded (%d + %d inlet)\n", cap, np,
+              n_in);
+      return FR_EOVERFLOW;
     }
-    FR_CHECK(hipMemcpyAsync(inj_buf, inj, sizeof(double) * 29 * (size_t)n_in,
-                            hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_inject, dim3(nblocks(n_in, 256)), dim3(256), 0, st, n_in, np,
-                       (const double*)inj_buf, ptrs(), rank, inv);
+    const double* src = inj;
+    if (!inj_dev) {
+      if (n_in > inj_cap) {
+        (void)hipFree(inj_buf);
+        inj_buf = nullptr;
+        FR_CHECK(hipMalloc(&inj_buf, sizeof(double) * 29 * (size_t)n_in));
+        inj_cap = n_in;
+      }
+      FR_CHECK(hipMemcpyAsync(inj_buf, inj, sizeof(double) * 29 * (size_t)n_in,
+                              hipMemcpyHostToDevice, st));
+      src = inj_buf;
+    }
+    hipLaunchKernelGGL(k_inject, dim3(nblocks(n_in, 256)), dim3(256), 0, st, n_in, np, src,
+                       ptrs(), rank, inv);
     FR_LAUNCH_CHECK();
     np += n_in;
-    FR_CHECK(hipStreamSynchronize(st));  // the host buffer may be reused
+    if (!inj_dev) FR_CHECK(hipStreamSynchronize(st));  // the host buffer may be reused
   }
   R.valid = false;
   T.valid = false;
+  return FR_OK;
+}
+
+int PartDev::gather_hi(double* buf, int cap, int* n_out, hipStream_t st) {
+  const unsigned nb = std::max(1u, std::min(nblocks(np, 256), 64u));
+  hipLaunchKernelGGL(k_gather_hi, dim3(nb), dim3(256), 0, st, np, (const int*)pNL,
+                     (const int*)plo, (const int*)inv, ptrs(), buf, cap, n_out);
+  FR_LAUNCH_CHECK();
   return FR_OK;
 }
 

@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -227,6 +228,8 @@ class particle {
   // (particle.cpp:860-865, 814-851) with the box's Rbuf1 given as n inlet
   // particles of 29 values each
   int Send_Recv_duct(const double* inlet, int n);
+  // the same with the n rows in device memory (the coupled box's '>hi' list)
+  int Send_Recv_duct_dev(const double* rows, int n);
   // gas2part_Temp_{new,int} + Compute_RHS_Temp_{new,int} (402-478)
   int temp(int stage, double c0, const tensor0& Rho, hipStream_t on = 0);
   int part2gas_Temp(int stage, tensor0& A, hipStream_t on = 0);  // particle.cpp:586-658
@@ -256,6 +259,22 @@ class poisson {
 // grid (grid.h:16-115): the box and duct time-step methods.  For the duct
 // the box grid g of the reference's grid::*(grid& g) methods is replaced by
 // the inflow planes and inlet particles set with set_inflow.
+// The coupling of a periodic box to a duct (simulation.cpp:19-73).  The
+// box writes the duct's inputs of each substep: the '>hi' list of its x pass
+// (device rows, 29 values each, at most cap; count on the device and in
+// pinned host memory) after its migration, and the duct's inflow planes after
+// its Update_RU_WP.  Both writes wait for ev_duct, which the duct records
+// when a substep no longer reads the previous ones.
+struct PairLink {
+  double* buf = nullptr;
+  int cap = 0;
+  int* d_n = nullptr;
+  int* h_n = nullptr;
+  hipEvent_t ev_rows = nullptr;    // box: rows and count written (its st3)
+  hipEvent_t ev_planes = nullptr;  // box: planes written (its st)
+  hipEvent_t ev_duct = nullptr;    // duct: substep done with both (its st)
+};
+
 class grid {
  public:
   params param;
@@ -275,6 +294,8 @@ class grid {
   tensor1 d64a, d64b;  // FP64 copies for the diagnostics of the FP32 form
   tensor0 d64r;
   std::vector<double> inlet;   // duct: inlet particles not yet received
+  PairLink* pair_out = nullptr;  // box of a coupled run: writes the duct's rows
+  PairLink* pair_in = nullptr;   // duct of a coupled run: its inlet rows
   double P0 = 0, P0_int = 0, P0_new = 0, P0_np1 = 0, dP0_dt = 0, T_cur = 0;
   int num_timestep = 0, RK4_count = 0;
   // Steps between re-sorts of the particle storage into cell order.  A
@@ -361,6 +382,12 @@ class grid {
   int Correct_mean_RU();
   int TimeAdvance();
   int step();  // simulation.cpp:36-70, box or duct (GRID_IO) calls
+  // step() in parts, for the coupled run: the parameter update and timers;
+  // one substep's Update_Rho .. Solve_Poisson; the end-of-step wait/timers
+  int begin_step();
+  int substep_front(int k);
+  int end_step();
+  std::chrono::steady_clock::time_point t0_step;
   int diagnostics(double* out);
   tensor0* field0(const std::string& name);
   tensor1* field1(const std::string& name);

@@ -3,6 +3,7 @@
 // the fr_sim_* C entry points.  Compiled by hipcc with the kernels into
 // libfr_amd.so.
 #include <array>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -368,6 +369,15 @@ int particle::Send_Recv_duct(const double* inlet, int n) {
   return rc;
 }
 
+int particle::Send_Recv_duct_dev(const double* rows, int n) {
+  const double lo[3] = {size_->xl_, size_->yl_, size_->zl_};
+  const double hi[3] = {size_->xh_, size_->yh_, size_->zh_};
+  const double L[3] = {size_->Lx(), size_->Ly(), size_->Lz()};
+  int rc = dev.migrate(lo, hi, L, st_, true, rows, n, true);
+  Np = dev.np;
+  return rc;
+}
+
 int particle::temp(int stage, double c0, const tensor0& Rho, hipStream_t on) {
   const double iTth = param_->Nu() * PI * param_->Dp() * param_->k();
   const hipStream_t s = on ? on : st_;
@@ -720,6 +730,13 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   if (box) {
     FR_CHECK(hipStreamWaitEvent(st3, evm0, 0));
     FR_TRY(part.Send_Recv(st3));
+    if (pair_out) {  // the coupled duct's Rbuf1 (particle.cpp:836-838)
+      PairLink& l = *pair_out;
+      FR_CHECK(hipStreamWaitEvent(st3, l.ev_duct, 0));
+      FR_TRY(part.dev.gather_hi(l.buf, l.cap, l.d_n, st3));
+      FR_CHECK(hipMemcpyAsync(l.h_n, l.d_n, sizeof(int), hipMemcpyDeviceToHost, st3));
+      FR_CHECK(hipEventRecord(l.ev_rows, st3));
+    }
     FR_CHECK(hipEventRecord(evm, st3));
     // box: it starts as soon as the migration is done, beside the momentum
     // deposit's tail and the momentum update
@@ -733,8 +750,19 @@ int grid::Update_Particle() {  // grid.cpp:659-680
     // kernel is queued first
     FR_CHECK(hipEventRecord(evj, st2));
     FR_CHECK(hipStreamWaitEvent(st, evj, 0));
-    FR_TRY(part.Send_Recv_duct(inlet.data(), (int)(inlet.size() / 29)));
-    inlet.clear();
+    if (pair_in) {  // the coupled box's '>hi' list of this substep
+      FR_CHECK(hipEventSynchronize(pair_in->ev_rows));
+      const int n = *pair_in->h_n;
+      if (n > pair_in->cap) {
+        fprintf(stderr, "fr_amd: %d box particles enter the duct, capacity %d\n", n,
+                pair_in->cap);
+        return FR_EOVERFLOW;
+      }
+      FR_TRY(part.Send_Recv_duct_dev(pair_in->buf, n));
+    } else {
+      FR_TRY(part.Send_Recv_duct(inlet.data(), (int)(inlet.size() / 29)));
+      inlet.clear();
+    }
     FR_CHECK(hipEventRecord(evf, st));
     FR_CHECK(hipStreamWaitEvent(st2, evf, 0));
     side_held = true;
@@ -967,33 +995,49 @@ int grid::TimeAdvance() {  // grid.cpp:315-325
   return FR_OK;
 }
 
-int grid::step() {  // simulation.cpp:36-70 (box calls)
+int grid::begin_step() {
   param.update(T_cur);
   t_poisson = t_particles = 0;
   iters_step = 0;
   box_poisson_xtimer(PS_.s, nullptr, nullptr, true);
-  part.dev.t_vel.collect();  // launches outside fr_sim_step are not counted
+  part.dev.t_vel.collect();  // launches outside a step are not counted
   part.dev.t_cic.collect();
   part.dev.t_vel.reset();
   part.dev.t_cic.reset();
-  auto t0 = std::chrono::steady_clock::now();
-  for (RK4_count = 0; RK4_count < 4; RK4_count++) {
-    FR_TRY(Update_Rho());
-    FR_TRY(Update_P0());
-    FR_TRY(Update_Particle());
-    FR_TRY(Update_RU_WOP());
-    FR_TRY(Compute_Div_U_new());
-    FR_TRY(Correct_OutFlow());  // duct only
-    FR_TRY(Compute_RHS_Pois());
-    FR_TRY(Solve_Poisson());
+  t0_step = std::chrono::steady_clock::now();
+  return FR_OK;
+}
+
+int grid::substep_front(int k) {
+  RK4_count = k;
+  FR_TRY(Update_Rho());
+  FR_TRY(Update_P0());
+  FR_TRY(Update_Particle());
+  FR_TRY(Update_RU_WOP());
+  FR_TRY(Compute_Div_U_new());
+  FR_TRY(Correct_OutFlow());  // duct only
+  FR_TRY(Compute_RHS_Pois());
+  FR_TRY(Solve_Poisson());
+  return FR_OK;
+}
+
+int grid::step() {  // simulation.cpp:36-70 (box calls)
+  FR_TRY(begin_step());
+  for (int k = 0; k < 4; k++) {
+    FR_TRY(substep_front(k));
     FR_TRY(Update_RU_WP());
     FR_TRY(TimeAdvance_RK4());
   }
   FR_TRY(Correct_mean_RU());
   FR_TRY(TimeAdvance());
+  return end_step();
+}
+
+int grid::end_step() {
   FR_CHECK(hipStreamSynchronize(st));
   FR_TRY(settle_side());
-  t_step = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  t_step = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_step)
+               .count();
   t_momentum = 0;
   for (int k = 0; k < 4; k++) {
     float ms = 0;
@@ -1120,7 +1164,10 @@ int fr_sim_destroy(fr_sim* s) {
 
 int fr_sim_initialize(fr_sim* s) { return s ? s->g.Initialize() : FR_EINVAL; }
 
-int fr_sim_step(fr_sim* s) { return s ? s->g.step() : FR_EINVAL; }
+// a grid of a coupled run steps only through fr_pair_step
+static bool coupled(const fr_sim* s) { return s->g.pair_in || s->g.pair_out; }
+
+int fr_sim_step(fr_sim* s) { return s && !coupled(s) ? s->g.step() : FR_EINVAL; }
 
 #define FR_TRY_S(x)     \
   do {                  \
@@ -1129,7 +1176,7 @@ int fr_sim_step(fr_sim* s) { return s ? s->g.step() : FR_EINVAL; }
   } while (0)
 
 int fr_sim_call(fr_sim* s, const char* m, int k) {
-  if (!s || !m) return FR_EINVAL;
+  if (!s || !m || coupled(s)) return FR_EINVAL;
   grid& g = s->g;
   std::string n(m);
   g.RK4_count = k;
@@ -1353,6 +1400,153 @@ int fr_sim_timing_ex(fr_sim* s, double* out, int n) {
                         p.t_cic.ms,      (double)p.t_cic.n};
   for (int i = 0; i < n && i < 12; i++) out[i] = v[i];
   return FR_OK;
+}
+
+// ---------------------------------------------------------------- coupled run
+// simulation.cpp:19-73 on one device: a periodic box feeding a duct.
+struct fr_pair {
+  fr_sim* box = nullptr;
+  fr_sim* duct = nullptr;
+  PairLink link;
+  bool ready = false;
+};
+
+// The duct's 8 inflow planes from the box's first interior slice (x index 1,
+// the RU x faces at 2).  At initialisation the box's RU and Rho stand in for
+// the _new/_np1 fields: the duct's restart reads the box's RU.y, RU.z there
+// (grid.cpp:155-164), which the ru_np1 y, z planes carry.
+static int pair_copy_planes(grid& b, grid& d, bool init) {
+  const double* now[8] = {b.Rho_new.pointer(),  b.Rho_np1.pointer(),  b.RU_new.x.pointer(),
+                          b.RU_new.y.pointer(), b.RU_new.z.pointer(), b.RU_np1.x.pointer(),
+                          b.RU_np1.y.pointer(), b.RU_np1.z.pointer()};
+  const double* ini[8] = {b.Rho.pointer(),  b.Rho.pointer(),  b.RU.x.pointer(),
+                          b.RU.y.pointer(), b.RU.z.pointer(), b.RU.x.pointer(),
+                          b.RU.y.pointer(), b.RU.z.pointer()};
+  const int xi[8] = {1, 1, 2, 1, 1, 2, 1, 1};
+  return pair_planes(b.size.dev(), init ? ini : now, xi, d.inflow, b.st);
+}
+
+int fr_pair_create(fr_sim* box, fr_sim* duct, fr_pair** out) {
+  if (!box || !duct || !out || box == duct) return FR_EINVAL;
+  grid& b = box->g;
+  grid& d = duct->g;
+  const auto same = [](double x, double y) { return std::fabs(x - y) <= 1e-12 * std::fabs(y); };
+  if (!b.param.IsBox() || d.param.IsBox() || b.f32 || d.f32 || coupled(box) || coupled(duct) ||
+      b.size.Ny() != d.size.Ny() || b.size.Nz() != d.size.Nz() ||
+      !same(b.size.Ly(), d.size.Ly()) || !same(b.size.Lz(), d.size.Lz())) {
+    fprintf(stderr,
+            "fr_amd: a coupled run takes an FP64 box and an FP64 duct of equal Ny, Nz, "
+            "Ly, Lz, neither already coupled\n");
+    return FR_EINVAL;
+  }
+  fr_pair* p = new fr_pair();
+  p->box = box;
+  p->duct = duct;
+  PairLink& l = p->link;
+  int rc = (int)hipMalloc(&l.d_n, sizeof(int));
+  if (!rc) rc = (int)hipHostMalloc(&l.h_n, sizeof(int), hipHostMallocDefault);
+  hipEvent_t* ev[3] = {&l.ev_rows, &l.ev_planes, &l.ev_duct};
+  for (int i = 0; i < 3 && !rc; i++) rc = (int)hipEventCreateWithFlags(ev[i], hipEventDisableTiming);
+  if (rc) {
+    fr_pair_destroy(p);
+    return rc;
+  }
+  *l.h_n = 0;
+  b.pair_out = &l;
+  d.pair_in = &l;
+  *out = p;
+  return FR_OK;
+}
+
+int fr_pair_destroy(fr_pair* p) {
+  if (!p) return FR_OK;
+  PairLink& l = p->link;
+  if (p->box) {
+    (void)hipStreamSynchronize(p->box->g.st3);
+    (void)hipStreamSynchronize(p->box->g.st);
+    if (p->box->g.pair_out == &l) p->box->g.pair_out = nullptr;
+  }
+  if (p->duct) {
+    (void)hipStreamSynchronize(p->duct->g.st);
+    if (p->duct->g.pair_in == &l) p->duct->g.pair_in = nullptr;
+  }
+  (void)hipFree(l.buf);
+  (void)hipFree(l.d_n);
+  (void)hipHostFree(l.h_n);
+  for (hipEvent_t e : {l.ev_rows, l.ev_planes, l.ev_duct})
+    if (e) (void)hipEventDestroy(e);
+  delete p;
+  return FR_OK;
+}
+
+// simulation.cpp:29-30: GRID.Initialize(GRID), GRID_IO.Initialize(GRID).
+// The inlet row buffer holds the duct's particle capacity (reserve it first).
+int fr_pair_initialize(fr_pair* p) {
+  if (!p) return FR_EINVAL;
+  grid& b = p->box->g;
+  grid& d = p->duct->g;
+  PairLink& l = p->link;
+  FR_TRY_S(b.Initialize());
+  const int cap = d.part.dev.cap;
+  if (cap > l.cap) {
+    (void)hipFree(l.buf);
+    l.buf = nullptr;
+    l.cap = 0;
+    FR_CHECK(hipMalloc(&l.buf, sizeof(double) * 29 * (size_t)cap));
+    l.cap = cap;
+  }
+  FR_TRY_S(pair_copy_planes(b, d, true));
+  const size_t n = 8 * (size_t)d.size.Ny() * d.size.Nz();
+  d.inflow_h.resize(n);
+  FR_CHECK(hipMemcpyAsync(d.inflow_h.data(), d.inflow, sizeof(double) * n,
+                          hipMemcpyDeviceToHost, b.st));
+  FR_CHECK(hipStreamSynchronize(b.st));
+  FR_TRY_S(d.Initialize());
+  FR_CHECK(hipEventRecord(l.ev_duct, d.st));
+  p->ready = true;
+  return FR_OK;
+}
+
+// simulation.cpp:34-62.  Per substep the box's eight methods, its planes
+// into the duct's inflow, the duct's eight (its migration takes the box's
+// '>hi' rows), then the box's TimeAdvance_RK4 and the duct's last two.
+int fr_pair_step(fr_pair* p) {
+  if (!p || !p->ready) return FR_EINVAL;
+  grid& b = p->box->g;
+  grid& d = p->duct->g;
+  PairLink& l = p->link;
+  FR_TRY_S(b.begin_step());
+  FR_TRY_S(d.begin_step());
+  for (int k = 0; k < 4; k++) {
+    FR_TRY_S(b.substep_front(k));
+    FR_TRY_S(b.Update_RU_WP());
+    FR_CHECK(hipStreamWaitEvent(b.st, l.ev_duct, 0));
+    FR_TRY_S(pair_copy_planes(b, d, false));
+    FR_CHECK(hipEventRecord(l.ev_planes, b.st));
+    FR_CHECK(hipStreamWaitEvent(d.st, l.ev_planes, 0));
+    FR_TRY_S(d.substep_front(k));
+    FR_TRY_S(b.TimeAdvance_RK4());
+    FR_TRY_S(d.Update_RU_WP());
+    FR_TRY_S(d.TimeAdvance_RK4());
+    FR_CHECK(hipEventRecord(l.ev_duct, d.st));
+  }
+  FR_TRY_S(b.Correct_mean_RU());
+  FR_TRY_S(d.Correct_mean_RU());
+  FR_TRY_S(b.TimeAdvance());
+  FR_TRY_S(d.TimeAdvance());
+  FR_TRY_S(b.end_step());
+  return d.end_step();
+}
+
+// grid::Touch(grid& g_IO) (grid.cpp:77-98): one read of touch.check; on a
+// stop both grids store.
+int fr_pair_touch(fr_pair* p, int* stop) {
+  if (!p || !stop) return FR_EINVAL;
+  bool s = false;
+  FR_TRY_S(p->box->g.Touch(&s));
+  p->duct->g.Is_touch = s;
+  *stop = s ? 1 : 0;
+  return s ? p->duct->g.Store() : FR_OK;
 }
 
 }  // extern "C"

@@ -226,7 +226,7 @@ int grid::Restart() {
           double& g0 = h[((size_t)k * s.Ny() + j) * s.Nx()];
           if (m == 3) g0 = param.Rho0();
           if ((m == 1 || m == 2) && !inflow_h.empty())
-            g0 = inflow_h[(size_t)(4 + m) * s.Ny() * s.Nz() + (size_t)k * s.Ny() + j];
+            g0 = inflow_h[(size_t)(5 + m) * s.Ny() * s.Nz() + (size_t)k * s.Ny() + j];
         }
     }
     tensor0& t = m < 3 ? RU.c(m) : Rho;

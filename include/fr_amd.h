@@ -332,6 +332,29 @@ int fr_sim_timing(fr_sim* s, double* out);
  * first min(n, 12) values. */
 int fr_sim_timing_ex(fr_sim* s, double* out, int n);
 
+/* ---- the coupled run (simulation.cpp:19-73): a periodic box (IsBox = 1)
+ * feeding a duct (IsBox = 0) of the same Ny, Nz, Ly, Lz on one device, both
+ * FP64.  Each substep the duct's inlet takes the box's first interior slice
+ * (grid::Update_Rho(grid&), Update_RU_WOP(grid&), grid.cpp:335-537) and the
+ * box's '>hi' particle list of its x pass (particle::Send_Recv(particle&),
+ * particle.cpp:814-851, 860-865), all in device memory.  While a pair
+ * exists its grids step only through fr_pair_step (fr_sim_step and
+ * fr_sim_call return FR_EINVAL); destroy the pair before either grid. */
+typedef struct fr_pair fr_pair;
+int fr_pair_create(fr_sim* box, fr_sim* duct, fr_pair** out);
+/* GRID.Initialize(GRID); GRID_IO.Initialize(GRID) (simulation.cpp:29-30):
+ * the duct's restart (Initial = 1) takes its inlet ghosts from the box's RU
+ * (grid.cpp:155-164).  Reserve the duct's particle capacity first: the box
+ * particles entering per substep must fit in it. */
+int fr_pair_initialize(fr_pair* p);
+/* One time step of both grids in the reference's call order
+ * (simulation.cpp:34-62). */
+int fr_pair_step(fr_pair* p);
+/* grid::Touch(grid& g_IO) (grid.cpp:77-98): one read of touch.check; on a
+ * stop (*stop = 1) both grids store. */
+int fr_pair_touch(fr_pair* p, int* stop);
+int fr_pair_destroy(fr_pair* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -234,6 +234,16 @@ void ref_part_send_recv_duct(void* h, int n_in, const double* inbuf) {
   if (n_in > 0) std::memcpy(pbox.Rbuf1, inbuf, sizeof(double) * 29 * (size_t)n_in);
   c->part->Send_Recv(pbox);
 }
+
+// The particle exchange of one coupled substep: GRID.Update_Particle's
+// part.Send_Recv() (grid.cpp:672), then GRID_IO.Update_Particle(GRID)'s
+// part.Send_Recv(g.part) (grid.cpp:696), which reads the box's Rbuf1/Rnum1.
+void ref_part_send_recv_pair(void* hbox, void* hduct) {
+  RefCtx* b = (RefCtx*)hbox;
+  RefCtx* d = (RefCtx*)hduct;
+  b->part->Send_Recv();
+  d->part->Send_Recv(*b->part);
+}
 }
 
 // Element-wise tensor0 operators (tensor0.cpp:93-241) under the op codes of

@@ -291,6 +291,18 @@ void orc_grid_step(void* h) {
   else
     g->duct_step();
 }
+// The coupled box `b` and duct `d` (simulation.cpp:19-73): initialise both
+// (each from its own U, V, W blocks, or nullptr where Initial != 0), one step
+// of both, and the shared touch.check stop (returns the flag).
+void orc_pair_initialize(void* b, void* d, const double* Ub, const double* Vb,
+                         const double* Wb, const double* Ud, const double* Vd,
+                         const double* Wd) {
+  const double* const ib[3] = {Ub, Vb, Wb};
+  const double* const id[3] = {Ud, Vd, Wd};
+  pair_initialize(*(Grid*)b, *(Grid*)d, ib, id);
+}
+void orc_pair_step(void* b, void* d) { pair_step(*(Grid*)b, *(Grid*)d); }
+int orc_pair_touch(void* b, void* d) { return pair_touch(*(Grid*)b, *(Grid*)d) ? 1 : 0; }
 // Duct inflow: planes = 8 ghosted [Nz][Ny] planes in the order rho_new,
 // rho_np1, ru_new x,y,z, ru_np1 x,y,z; pbuf = n_in particles x 29 values
 // (appended to any not yet consumed).
@@ -355,6 +367,13 @@ double* orc_grid_scalar(void* h, const char* name) {
   return nullptr;
 }
 int orc_grid_np(void* h) { return ((Grid*)h)->part.Np; }
+// The '>hi' list of the last migration's x pass (29 values per particle):
+// the box's Rbuf1 on one rank (particle.cpp:821, 836-838); *n its length.
+const double* orc_grid_sbuf0(void* h, int* n) {
+  Grid* g = (Grid*)h;
+  *n = (int)(g->part.Sbuf0.size() / 29);
+  return g->part.Sbuf0.data();
+}
 void orc_grid_set_np(void* h, int np) {
   Grid* g = (Grid*)h;
   g->part.Np = np;

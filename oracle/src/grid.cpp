@@ -423,6 +423,90 @@ void Grid::duct_step() {
   TimeAdvance();
 }
 
+// ---- the coupled box and duct (simulation.cpp:19-73) -----------------------
+
+// grid.cpp:352-363 and 485-503 (bs = 1): the duct's inlet reads the box's
+// first interior slice: Rho_new and Rho_np1 at i = 1, RU_new/RU_np1 x at the
+// face i = 2 and y, z at i = 1, over the whole ghosted (j, k) plane.  At
+// initialisation the restart reads the box's RU.y, RU.z at i = 1
+// (grid.cpp:155-164), which is what the ru_np1 y, z planes carry then.
+void pair_inflow(Grid& b, Grid& d, bool init) {
+  const int Ny = b.g.Ny, Nz = b.g.Nz;
+  const size_t m = (size_t)Ny * Nz;
+  std::vector<double>* dst[8] = {&d.in.rho_new,   &d.in.rho_np1,   &d.in.ru_new[0],
+                                 &d.in.ru_new[1], &d.in.ru_new[2], &d.in.ru_np1[0],
+                                 &d.in.ru_np1[1], &d.in.ru_np1[2]};
+  const F0* src[8] = {&b.Rho_new,  &b.Rho_np1,  &b.RU_new.x, &b.RU_new.y,
+                      &b.RU_new.z, &b.RU_np1.x, &b.RU_np1.y, &b.RU_np1.z};
+  if (init) {
+    const F0* s0[8] = {&b.Rho, &b.Rho, &b.RU.x, &b.RU.y, &b.RU.z, &b.RU.x, &b.RU.y, &b.RU.z};
+    for (int q = 0; q < 8; q++) src[q] = s0[q];
+  }
+  for (int q = 0; q < 8; q++) {
+    const int i = (q == 2 || q == 5) ? 2 : 1;
+    dst[q]->assign(m, 0.0);
+    for (int k = 0; k < Nz; k++)
+      for (int j = 0; j < Ny; j++) (*dst[q])[(size_t)k * Ny + j] = (*src[q])(i, j, k);
+  }
+}
+
+// simulation.cpp:27-30: GRID.Initialize(GRID), then GRID_IO.Initialize(GRID).
+// Each grid reads its own initial fields (the reference reads U.bin, V.bin,
+// W.bin from one directory for both).
+void pair_initialize(Grid& b, Grid& d, const double* const ic_b[3],
+                     const double* const ic_d[3]) {
+  b.Initialize(ic_b[0], ic_b[1], ic_b[2]);
+  pair_inflow(b, d, true);
+  d.Initialize(ic_d[0], ic_d[1], ic_d[2]);
+}
+
+// simulation.cpp:34-62: per substep the box's eight methods, then the duct's
+// with the box's planes and '>hi' list, then the box's TimeAdvance_RK4 and
+// the duct's last two; the mean corrections and time advances at the end.
+void pair_step(Grid& b, Grid& d) {
+  b.par.update(b.T_cur);
+  d.par.update(d.T_cur);
+  for (int k = 0; k < 4; k++) {
+    b.RK4_count = k;
+    b.Update_Rho();
+    b.Update_P0();
+    b.Update_Particle();
+    b.Update_RU_WOP();
+    b.Compute_Div_U_new();
+    b.Compute_RHS_Pois();
+    b.Solve_Poisson();
+    b.Update_RU_WP();
+    d.RK4_count = k;
+    pair_inflow(b, d, false);
+    // particle.cpp:821 and 836-838: the duct's Rbuf1 is the box's Rbuf1,
+    // which on one rank is the box's own '>hi' list of its x pass
+    d.in.pbuf = b.part.Sbuf0;
+    d.Update_Rho_duct();
+    d.Update_P0();
+    d.Update_Particle_duct();
+    d.Update_RU_WOP_duct();
+    d.Compute_Div_U_new();
+    d.Correct_OutFlow();
+    d.Compute_RHS_Pois();
+    d.Solve_Poisson_IO();
+    b.TimeAdvance_RK4();
+    d.Update_RU_WP();
+    d.TimeAdvance_RK4();
+  }
+  b.Correct_mean_RU();
+  d.Correct_mean_RU();
+  b.TimeAdvance();
+  d.TimeAdvance();
+}
+
+// grid.cpp:77-98: one read of touch.check stops both grids and stores both.
+bool pair_touch(Grid& b, Grid& d) {
+  const bool ans = b.Touch();
+  d.Is_touch = ans;
+  if (ans) d.Store();
+  return ans;
+}
+
 // grid.cpp:714-811 subset: out = {TKE, Rho_Mean, Rho_Max, Rho_Min, C_Max,
 // C_Min, C_Mean, mean dissipation (x-average of the DISSIPATION profile),
 // mean particle temperature (extension: sum of T in vector order / Np),

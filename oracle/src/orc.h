@@ -486,4 +486,13 @@ struct Grid {  // grid.h:16-115
   F1* field1(const std::string& name);
 };
 
+// The coupled run of simulation.cpp:19-73 on one rank: a periodic box `b`
+// feeding a duct `d` of the same Ny, Nz (grid.cpp:110-182 and 335-537 with
+// the box's planes, particle.cpp:814-851 with the box's '>hi' list).
+void pair_inflow(Grid& b, Grid& d, bool init);  // the duct's `in` from the box
+void pair_initialize(Grid& b, Grid& d, const double* const ic_b[3],
+                     const double* const ic_d[3]);
+void pair_step(Grid& b, Grid& d);   // simulation.cpp:34-62
+bool pair_touch(Grid& b, Grid& d);  // grid::Touch(grid& g_IO), grid.cpp:77-98
+
 }  // namespace orc

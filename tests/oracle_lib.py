@@ -99,6 +99,12 @@ def orc():
         L.orc_grid_initialize.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_grid_finish_initialize.argtypes = [C.c_void_p]
         L.orc_grid_step.argtypes = [C.c_void_p]
+        L.orc_grid_sbuf0.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.orc_grid_sbuf0.restype = C.POINTER(C.c_double)
+        L.orc_pair_initialize.argtypes = [C.c_void_p] * 8
+        L.orc_pair_step.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_pair_touch.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_pair_touch.restype = C.c_int
         L.orc_grid_set_inflow.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_grid_call.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.orc_grid_call.restype = C.c_int
@@ -154,6 +160,7 @@ def ref():
         L.ref_part_op.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, vp, vp, vp]
         L.ref_part_load_random.argtypes = [vp]
         L.ref_part_send_recv_duct.argtypes = [vp, C.c_int, vp]
+        L.ref_part_send_recv_pair.argtypes = [vp, vp]
         L.ref_elementwise.argtypes = [vp, C.c_int, dp, C.c_double, vp, C.c_double, vp,
                                       C.c_double, vp]
         L.ref_post.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double]
@@ -181,16 +188,18 @@ class OracleGrid:
         L.orc_grid_geom(self.h, C.byref(self.g))
         self.ic_dir = ic_dir
 
-    def initialize(self):
+    def ic_blocks(self):
+        """U.bin, V.bin, W.bin of ic_dir (kept alive on self), or Nones."""
         g = self.g
         n = (g.Nz - 2) * (g.Ny - 2) * (g.Nx - 2)
-        if self.ic_dir:
-            blks = [np.fromfile(os.path.join(self.ic_dir, f), dtype="<f8", count=n)
+        if not self.ic_dir:
+            return [None, None, None]
+        self._ic = [np.fromfile(os.path.join(self.ic_dir, f), dtype="<f8", count=n)
                     for f in ("U.bin", "V.bin", "W.bin")]
-            self._ic = blks
-            self.L.orc_grid_initialize(self.h, *[ptr(b) for b in blks])
-        else:
-            self.L.orc_grid_initialize(self.h, None, None, None)
+        return [ptr(b) for b in self._ic]
+
+    def initialize(self):
+        self.L.orc_grid_initialize(self.h, *self.ic_blocks())
 
     def step(self):
         self.L.orc_grid_step(self.h)
@@ -243,6 +252,14 @@ class OracleGrid:
         n = self.np_
         return np.ctypeslib.as_array(self.L.orc_grid_part_aux(self.h, which), shape=(n,))
 
+    def sbuf0(self):
+        """Copy of the last x pass's '>hi' list, shape (n, 29)."""
+        n = C.c_int()
+        p = self.L.orc_grid_sbuf0(self.h, C.byref(n))
+        if n.value == 0:
+            return np.zeros((0, 29))
+        return np.ctypeslib.as_array(p, shape=(n.value, 29)).copy()
+
     def statistics(self):
         self.L.orc_grid_statistics(self.h)
 
@@ -266,3 +283,23 @@ class OracleGrid:
             self.L.orc_grid_destroy(self.h)
         except Exception:
             pass
+
+
+class OraclePair:
+    """The oracle's coupled box and duct (simulation.cpp:19-73): `box` and
+    `duct` are OracleGrids of the same Ny, Nz; the duct's inflow planes and
+    inlet particles come from the box every substep."""
+
+    def __init__(self, box: OracleGrid, duct: OracleGrid):
+        self.box, self.duct = box, duct
+        self.L = box.L
+
+    def initialize(self):
+        self.L.orc_pair_initialize(self.box.h, self.duct.h, *self.box.ic_blocks(),
+                                   *self.duct.ic_blocks())
+
+    def step(self):
+        self.L.orc_pair_step(self.box.h, self.duct.h)
+
+    def touch(self):
+        return self.L.orc_pair_touch(self.box.h, self.duct.h)

@@ -165,3 +165,63 @@ def test_duct_outlet_slices_and_touch(tmp_path):
                                   sim.field("RU", c)[1:-1, 1:-1, -1])
         assert np.array_equal(rho_s.reshape(16, 16), sim.field("Rho")[1:-1, 1:-1, -1])
         assert np.fromfile("data/Part_flag_1.bin", np.int32).size == sim.np_ == 50
+
+
+def test_duct_restart_matches_oracle(tmp_path):
+    """The duct's Initial = 1 restart (grid.cpp:143-182): interior blocks,
+    the outlet slice files, and on the inlet ghost plane Rho0 and the box's
+    v, w, which the RU_np1 y and z inflow planes carry (they differ from the
+    x plane here, so a plane mix-up shows)."""
+    w = DuctWorkload(n=(32, 16, 16), L=(4.0, 2.0, 2.0), inject_per_step=50, capacity=2000,
+                     extra={"Data_dir": "./data/", "Stat_dir": "./stat/",
+                            "Data_freq_fast": 2, "Data_freq_slow": 2})
+    case = tmp_path / "case"
+    w.build(str(case))
+    gdir = tmp_path / "gpu"
+    with Cwd(gdir):
+        sim = fr.Sim(str(case / "params.in"), str(case), outputs=True)
+        sim.reserve_particles(w.capacity)
+        sim.initialize()
+        for s in range(2):
+            sim.set_inflow(w.inflow(s), w.inlet_particles(s, 50 * s))
+            sim.step()
+            assert sim.store() == 0
+    rd = tmp_path / "restart"
+    os.makedirs(rd)
+    pairs = [("RU_2.bin", "Restart_RU_IO.bin"), ("RU_S_2.bin", "Restart_RU_IO_S.bin"),
+             ("Rho_2.bin", "Restart_Rho_IO.bin"), ("Rho_S_2.bin", "Restart_Rho_IO_S.bin"),
+             ("numbers_2.dat", "Restart_numbers_IO.dat")]
+    pairs += [(f"Part_{q}_2.bin", f"Restart_Part_{q}_IO.bin") for q in
+              ("x", "y", "z", "u", "v", "w", "T", "flag")]
+    for src, dst in pairs:
+        shutil.copy(os.path.join(gdir, "data", src), rd / dst)
+    txt = open(case / "params.in").read().replace("# Initial\n0\n", "# Initial\n1\n")
+    open(rd / "params.in", "w").write(txt)
+    planes = w.inflow(7)
+    assert not np.array_equal(planes[5], planes[6])
+    with Cwd(rd):
+        rs = fr.Sim("params.in", str(rd))
+        rs.reserve_particles(w.capacity)
+        rs.set_inflow(planes, None)
+        rs.initialize()
+        ro = OL.OracleGrid("params.in", None)
+        ro.set_inflow(planes)
+        ro.initialize()
+        assert rs.num_timestep == ro.num_timestep == 2
+        for c in range(3):
+            assert np.array_equal(rs.field("RU", c), ro.field("RU", c)), c
+        for f in ("Rho", "Rho_int", "Rho_np1"):
+            assert np.array_equal(rs.field(f), ro.field(f)), f
+        assert rs.np_ == ro.np_ > 0
+        assert np.array_equal(rs.flags(), ro.flag())
+        for q in range(7):
+            assert np.array_equal(rs.part(q, 0), ro.part(q, 0)), q
+        rs.set_inflow(planes, w.inlet_particles(2, 100))
+        ro.set_inflow(planes, w.inlet_particles(2, 100))
+        rs.step()
+        ro.step()
+        for c in range(3):
+            a, b = rs.field("RU", c)[:, :, 1:], ro.field("RU", c)[:, :, 1:]
+            assert rel(a, b) <= 1e-12, c
+        assert rel(rs.field("Rho"), ro.field("Rho")) <= 1e-12
+        assert np.array_equal(rs.flags(), ro.flag())
