@@ -1,5 +1,6 @@
 """Builds first_repo_amd/libfr_amd.so (gfx950 kernels + host classes + C ABI)
-and the frsim driver with hipcc.  Usage: python -m first_repo_amd.build"""
+with hipcc, and the C++ test programs of tests/cpp against it.  Usage:
+python -m first_repo_amd.build"""
 from __future__ import annotations
 
 import os
@@ -13,7 +14,7 @@ ARCH = os.environ.get("FR_OFFLOAD_ARCH", "gfx950")
 
 KERNELS = ["fields.hip", "box.hip", "duct.hip", "fft_poisson.hip", "duct_poisson.hip",
            "mg_pcg.hip", "unet.hip", "particles.hip", "diag.hip", "stats.hip"]
-HOST = ["host.cpp", "host_io.cpp"]
+HOST = ["host.cpp", "host_io.cpp", "members.cpp"]
 
 # -ffp-contract=off: the FP64 path reproduces the reference's unfused
 # multiply-add sequences.  The packed-FP32 feature is disabled for all device
@@ -62,7 +63,23 @@ def build(jobs: int = 8) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)
+    build_test_programs()
     return LIB
+
+
+# C++ programs that drive the library through its host classes (tests/cpp),
+# linked against libfr_amd.so and found beside it at run time.
+TEST_PROGRAMS = ["member_substep.cpp"]
+
+
+def build_test_programs() -> None:
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    for f in TEST_PROGRAMS:
+        exe = os.path.join(out_dir, os.path.splitext(f)[0])
+        _run([HIPCC, *[x for x in FLAGS if x != "-fPIC"], "-x", "hip", os.path.join(src_dir, f),
+              "-x", "none", LIB, "-Wl,-rpath,$ORIGIN/../../../first_repo_amd", "-o", exe])
 
 
 if __name__ == "__main__":

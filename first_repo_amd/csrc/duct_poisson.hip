@@ -340,6 +340,12 @@ struct fr_duct_poisson {
   int* gate = nullptr;
   int* hgate = nullptr;
   int last_iters = 1;
+  // optional phase timers (poisson_IO's FFT_TIME / SOLVER_TIME): four events
+  // per iteration around the FFT passes and the Thomas sweeps
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  int tev_used = 0;
+  double fft_ms = 0, solver_ms = 0;
 };
 
 static int duct_poisson_iter(fr_duct_poisson* s, const double* cx, const double* cy,
@@ -372,6 +378,33 @@ int duct_poisson_solve(fr_duct_poisson* s, const double* cx, const double* cy, c
   }
   s->last_iters = ran > 0 ? ran : 1;
   if (num_iter) *num_iter = ran;
+  if (s->timing) {  // the stream is synchronised: every recorded event is done
+    for (int e = 0; e + 3 < s->tev_used; e += 4) {
+      float a = 0, b = 0, c = 0;
+      hipEventElapsedTime(&a, s->tev[e], s->tev[e + 1]);
+      hipEventElapsedTime(&b, s->tev[e + 1], s->tev[e + 2]);
+      hipEventElapsedTime(&c, s->tev[e + 2], s->tev[e + 3]);
+      s->fft_ms += a + c;
+      s->solver_ms += b;
+    }
+    s->tev_used = 0;
+  }
+  return FR_OK;
+}
+
+// Turns the phase timers on (creating 4 events per iteration up to
+// `iterations`) and returns the summed milliseconds so far.
+int duct_poisson_timing(fr_duct_poisson* s, int iterations, double* fft_ms, double* solver_ms) {
+  if (!s) return FR_EINVAL;
+  const size_t need = 4 * (size_t)std::max(iterations, 1);
+  while (s->tev.size() < need) {
+    hipEvent_t e;
+    FR_CHECK(hipEventCreate(&e));
+    s->tev.push_back(e);
+  }
+  s->timing = true;
+  if (fft_ms) *fft_ms = s->fft_ms;
+  if (solver_ms) *solver_ms = s->solver_ms;
   return FR_OK;
 }
 
@@ -460,6 +493,7 @@ int fr_duct_poisson_create(const fr_geom* g, fr_duct_poisson** out) {
 
 int fr_duct_poisson_destroy(fr_duct_poisson* s) {
   if (!s) return FR_OK;
+  for (hipEvent_t e : s->tev) (void)hipEventDestroy(e);
   (void)hipFree(s->A);
   (void)hipFree(s->T);
   (void)hipFree(s->cp);
@@ -487,6 +521,12 @@ static int duct_poisson_iter(fr_duct_poisson* s, const double* cx, const double*
                              double eps, hipStream_t st) {
   DuctDev& q = s->q;
   const int* g = s->gate;
+  hipEvent_t* ev = nullptr;
+  if (s->timing && s->tev_used + 4 <= (int)s->tev.size()) {
+    ev = &s->tev[s->tev_used];
+    s->tev_used += 4;
+    FR_CHECK(hipEventRecord(ev[0], st));
+  }
   if (inv)
     hipLaunchKernelGGL(k_duct_y_fwd<true>, dim3(s->nby), dim3(FT), s->lds_y, st, q, P, cx, cy,
                        cz, RHS, s->twfm, s->twfy, s->A, g);
@@ -495,12 +535,15 @@ static int duct_poisson_iter(fr_duct_poisson* s, const double* cx, const double*
                        cz, RHS, s->twfm, s->twfy, s->A, g);
   hipLaunchKernelGGL(k_duct_z<true>, dim3(s->nbz), dim3(FT), s->lds_z, st, q, s->twfz, s->A,
                      s->T, g);
+  if (ev) FR_CHECK(hipEventRecord(ev[1], st));
   hipLaunchKernelGGL(k_duct_thomas, dim3(nblocks(2 * q.nm, 64)), dim3(64), 0, st, q, s->lamy,
                      s->lamz, s->cp, (double*)s->T, g);
+  if (ev) FR_CHECK(hipEventRecord(ev[2], st));
   hipLaunchKernelGGL(k_duct_z<false>, dim3(s->nbz), dim3(FT), s->lds_z, st, q, s->twiz, s->T,
                      s->A, g);
   hipLaunchKernelGGL(k_duct_y_inv, dim3(s->nby), dim3(FT), s->lds_y, st, q, s->A, s->twim,
                      s->twiy, P, s->bmax, g);
+  if (ev) FR_CHECK(hipEventRecord(ev[3], st));
   FR_LAUNCH_CHECK();
   // the ghost copies of P below (periodic y,z; Neumann x) are idempotent
   int rc = max_final(s->bmax, s->nby, s->dres, st, s->gate, eps);

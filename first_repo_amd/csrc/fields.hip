@@ -502,6 +502,53 @@ template int elementwise<double>(long, int, double*, double, const double*, doub
 template int elementwise<float>(long, int, float*, double, const float*, double, const float*,
                                 double, const float*, hipStream_t);
 
+// The reference's element-wise members, one expression each with its own
+// operand order (tensor0.cpp:93-241; IEEE products commute, sums of two
+// terms commute, three terms add left to right).  Operands may alias out:
+// every lane reads its own element before writing it.
+__global__ void k_arith(long n, int op, double* out, double a, const double* A, const double* B,
+                        double b, const double* C, double c, const double* D) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double o = out[t];
+  double v;
+  switch (op) {
+    case AR_SET_S: v = a; break;
+    case AR_ADD_S: v = o + a; break;
+    case AR_SUB_S: v = o - a; break;
+    case AR_MUL_S: v = o * a; break;
+    case AR_DIV_S: v = o / a; break;
+    case AR_SET_T: v = A[t]; break;
+    case AR_ADD_T: v = o + A[t]; break;
+    case AR_SUB_T: v = o - A[t]; break;
+    case AR_MUL_T: v = o * A[t]; break;
+    case AR_DIV_T: v = o / A[t]; break;
+    case AR_PEM_TT: v = o + A[t] * B[t]; break;
+    case AR_PEM_ST: v = o + a * A[t]; break;
+    case AR_DIV_TT: v = A[t] / B[t]; break;
+    case AR_DIV_TS: v = A[t] / a; break;
+    case AR_DIV_ST: v = a / A[t]; break;
+    case AR_MUL_TT: v = A[t] * B[t]; break;
+    case AR_MUL_ST: v = a * A[t]; break;
+    case AR_LC_STST: v = a * A[t] + b * C[t]; break;
+    case AR_LC_TTST: v = A[t] * B[t] + b * C[t]; break;
+    case AR_LC_TTTT: v = A[t] * B[t] + C[t] * D[t]; break;
+    case AR_LC_STTT: v = a * A[t] + C[t] * D[t]; break;
+    case AR_LC_STSTST: v = a * A[t] + b * C[t] + c * D[t]; break;
+    default: v = o;
+  }
+  out[t] = v;
+}
+
+int t0_arith(long n, int op, double* out, double a, const double* A, const double* B, double b,
+             const double* C, double c, const double* D, hipStream_t s) {
+  if (op < 0 || op >= AR_NOPS || !out) return FR_EINVAL;
+  hipLaunchKernelGGL(k_arith, dim3(nblocks(n, 256)), dim3(256), 0, s, n, op, out, a, A, B, b, C,
+                     c, D);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
 __global__ void k_f2d(long n, const float* __restrict__ a, double* __restrict__ b) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) b[t] = (double)a[t];

@@ -34,6 +34,37 @@ int reducex(const Dev& d, const double* f, double* out, hipStream_t s);
 // the allocation-free reduction (RED_PARTS + 8 doubles); looked up once per
 // device.
 int noalloc_scratch(double** out);
+// The element-wise members of tensor0 (tensor0.cpp:93-241) over n values:
+// o = out's value, A B C D fields, a b c scalars.  Products and two-term sums
+// in the reference's own operand order; PlusEqual_Mult(T, s) is AR_PEM_ST and
+// Equal_Mult(T, s) AR_MUL_ST (products commute).
+enum ArithOp {
+  AR_SET_S,      // o = a
+  AR_ADD_S,      // o += a
+  AR_SUB_S,      // o -= a
+  AR_MUL_S,      // o *= a
+  AR_DIV_S,      // o /= a
+  AR_SET_T,      // o = A
+  AR_ADD_T,      // o += A
+  AR_SUB_T,      // o -= A
+  AR_MUL_T,      // o *= A
+  AR_DIV_T,      // o /= A
+  AR_PEM_TT,     // o += A*B
+  AR_PEM_ST,     // o += a*A
+  AR_DIV_TT,     // o = A/B
+  AR_DIV_TS,     // o = A/a
+  AR_DIV_ST,     // o = a/A
+  AR_MUL_TT,     // o = A*B
+  AR_MUL_ST,     // o = a*A
+  AR_LC_STST,    // o = a*A + b*C
+  AR_LC_TTST,    // o = A*B + b*C
+  AR_LC_TTTT,    // o = A*B + C*D
+  AR_LC_STTT,    // o = a*A + C*D
+  AR_LC_STSTST,  // o = a*A + b*C + c*D
+  AR_NOPS
+};
+int t0_arith(long n, int op, double* out, double a, const double* A, const double* B, double b,
+             const double* C, double c, const double* D, hipStream_t s);
 template <typename T>
 int elementwise(long n, int op, T* out, double a, const T* A, double b, const T* B, double c,
                 const T* C, hipStream_t s);
@@ -121,6 +152,10 @@ int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* c
 // of the box solve since the last reset.
 void box_poisson_xtimer(fr_box_poisson* s, double* ms, long* n, bool reset);
 
+// Phase timers of the duct solve (poisson_IO's FFT_TIME and SOLVER_TIME):
+// switched on by the first call, which sizes them for `iterations` per solve;
+// returns the summed FFT-pass and Thomas-sweep milliseconds so far.
+int duct_poisson_timing(fr_duct_poisson* s, int iterations, double* fft_ms, double* solver_ms);
 // fr_duct_poisson_solve with the same Rho_face option (duct_poisson.hip).
 int duct_poisson_solve(fr_duct_poisson* s, const double* cx, const double* cy, const double* cz,
                        bool inv, double* P, const double* RHS, int iterations, double epsilon,
@@ -299,6 +334,19 @@ class PartDev {
   // momentum deposit from T
   template <typename F>
   int deposit_vel(const Dev& d, F* const out[3], hipStream_t st);
+  // The particle members of the reference taken one at a time (FP64):
+  // phase 1 gas2part_velocity (gathered velocity kept per storage slot in
+  // gas[0..2]), 2 update_position, 4 update_velocity, 8 update_Temp
+  // (particle.cpp:314-399, 480-496); temp_split mode 0 gas2part_Temp_*
+  // (Tg from a temperature field into gas[3]), mode 1 Compute_RHS_Temp_*
+  // (particle.cpp:402-478).  The kept values follow storage slots: no
+  // migration or re-sort may come between a gather and its use.
+  double* gas[4] = {};
+  int gas_cap = 0;
+  int ensure_gas();
+  int advance_phase(const Dev& d, const AdvCoef& c, const double* const u[3], int phase,
+                    hipStream_t st);
+  int temp_split(const Dev& d, int stage, int mode, const double* T, double iTth, hipStream_t st);
   // periodic migration (box); with duct set, x leavers are dropped and the
   // n_in inlet particles of `inj` (29 values each; host memory, or device
   // memory read on st when inj_dev) appended

@@ -52,9 +52,17 @@ __device__ __forceinline__ double tp_of_flag(int f, double lo, double span) {
 }
 
 // particle.cpp:314-399, 480-496 (one RK4 stage) with the gather 340-360.
-template <typename T>
+// PH selects the phases of one launch: PH_ALL is the fused stage the time
+// step uses; a single phase is one member of the reference's surface
+// (gas2part_velocity stores the gathered velocity in gu/gv/gw for
+// update_velocity; update_position and update_Temp need no gather).
+constexpr int PH_GATHER = 1, PH_POS = 2, PH_VEL = 4, PH_TEMP = 8, PH_ALL = 15;
+
+template <typename T, int PH>
 __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict__ ux,
-                               const T* __restrict__ uy, const T* __restrict__ uz) {
+                               const T* __restrict__ uy, const T* __restrict__ uz,
+                               double* __restrict__ gu, double* __restrict__ gv,
+                               double* __restrict__ gw) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.np) return;
   // T = float (precision form): the gas-side arithmetic (interpolation
@@ -62,19 +70,26 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
   // particle state and its RK4 accumulation stay FP64.  T = double is the
   // reference arithmetic.
   using Cm = T;
-  const double xi = p.s[0][1][i], yi = p.s[1][1][i], zi = p.s[2][1][i];
-  const int ti = (int)(xi * d.idx), tj = (int)(yi * d.idy), tk = (int)(zi * d.idz);
-  const Cm x_l = (Cm)(xi * d.idx - ((int)(xi * d.idx)));
-  const Cm y_l = (Cm)(yi * d.idy - ((int)(yi * d.idy)));
-  const Cm z_l = (Cm)(zi * d.idz - ((int)(zi * d.idz)));
-  const int ii = ti + 1, jj = tj + 1, kk = tk + 1;
-  const long q = (long)kk * d.NxNy + (long)jj * d.Nx + ii;
-  const Cm ug = x_l * (Cm)ux[clampi(q + 1, d.size)] + (1 - x_l) * (Cm)ux[clampi(q, d.size)];
-  const Cm vg = y_l * (Cm)uy[clampi(q + d.Nx, d.size)] + (1 - y_l) * (Cm)uy[clampi(q, d.size)];
-  const Cm wg = z_l * (Cm)uz[clampi(q + d.NxNy, d.size)] + (1 - z_l) * (Cm)uz[clampi(q, d.size)];
-  // positions
+  Cm ug = 0, vg = 0, wg = 0;
+  if (PH & PH_GATHER) {
+    const double xi = p.s[0][1][i], yi = p.s[1][1][i], zi = p.s[2][1][i];
+    const int ti = (int)(xi * d.idx), tj = (int)(yi * d.idy), tk = (int)(zi * d.idz);
+    const Cm x_l = (Cm)(xi * d.idx - ((int)(xi * d.idx)));
+    const Cm y_l = (Cm)(yi * d.idy - ((int)(yi * d.idy)));
+    const Cm z_l = (Cm)(zi * d.idz - ((int)(zi * d.idz)));
+    const int ii = ti + 1, jj = tj + 1, kk = tk + 1;
+    const long q = (long)kk * d.NxNy + (long)jj * d.Nx + ii;
+    ug = x_l * (Cm)ux[clampi(q + 1, d.size)] + (1 - x_l) * (Cm)ux[clampi(q, d.size)];
+    vg = y_l * (Cm)uy[clampi(q + d.Nx, d.size)] + (1 - y_l) * (Cm)uy[clampi(q, d.size)];
+    wg = z_l * (Cm)uz[clampi(q + d.NxNy, d.size)] + (1 - z_l) * (Cm)uz[clampi(q, d.size)];
+    if (!(PH & PH_VEL)) {
+      gu[i] = (double)ug;
+      gv[i] = (double)vg;
+      gw[i] = (double)wg;
+    }
+  }
   const double ui = p.s[3][1][i], vi = p.s[4][1][i], wi = p.s[5][1][i];
-  {
+  if (PH & PH_POS) {
     const double X[3] = {ui, vi, wi};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -83,15 +98,20 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
       p.s[a][2][i] = c.last ? np1 : p.s[a][0][i] + c.dt_pre * X[a];
     }
   }
-  // velocities (per-particle extension: 1/Tp of the particle's own flag)
-  const double itp = c.per_particle ? 1. / tp_of_flag(p.flag[i], c.tp_lo, c.tp_span) : c.iTp;
-  const double ru = (double)((ug - (Cm)ui) * (Cm)itp);
-  const double rv = (double)((vg - (Cm)vi) * (Cm)itp);
-  const double rw = (double)((wg - (Cm)wi) * (Cm)itp);
-  p.rhs_u[i] = ru;
-  p.rhs_v[i] = rv;
-  p.rhs_w[i] = rw;
-  {
+  if (PH & PH_VEL) {
+    if (!(PH & PH_GATHER)) {
+      ug = (Cm)gu[i];
+      vg = (Cm)gv[i];
+      wg = (Cm)gw[i];
+    }
+    // per-particle extension: 1/Tp of the particle's own flag
+    const double itp = c.per_particle ? 1. / tp_of_flag(p.flag[i], c.tp_lo, c.tp_span) : c.iTp;
+    const double ru = (double)((ug - (Cm)ui) * (Cm)itp);
+    const double rv = (double)((vg - (Cm)vi) * (Cm)itp);
+    const double rw = (double)((wg - (Cm)wi) * (Cm)itp);
+    p.rhs_u[i] = ru;
+    p.rhs_v[i] = rv;
+    p.rhs_w[i] = rw;
     const double R[3] = {ru, rv, rw};
     const double G[3] = {c.gx, c.gy, c.gz};
 #pragma unroll
@@ -101,11 +121,33 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
       p.s[3 + a][2][i] = c.last ? np1 : p.s[3 + a][0][i] + c.dt_pre * (R[a] + G[a]);
     }
   }
-  // temperature (RHS_T from the previous stage)
-  const double rt = p.rhs_T[i];
-  const double tn = p.s[6][3][i] + c.dtpost_iS * (rt + c.alpha);
-  p.s[6][3][i] = tn;
-  p.s[6][2][i] = c.last ? tn : p.s[6][0][i] + c.dtpre_iS * (rt + c.alpha);
+  if (PH & PH_TEMP) {  // RHS_T from the previous stage
+    const double rt = p.rhs_T[i];
+    const double tn = p.s[6][3][i] + c.dtpost_iS * (rt + c.alpha);
+    p.s[6][3][i] = tn;
+    p.s[6][2][i] = c.last ? tn : p.s[6][0][i] + c.dtpre_iS * (rt + c.alpha);
+  }
+}
+
+// Trilinear gas temperature at position (X, Y, Z) from the rounded index of
+// the upper cell centre (particle.cpp:402-441), Tc(a, b, e) the cell values.
+template <typename Cm, typename F>
+__device__ __forceinline__ Cm interp_T(const Dev& d, double X, double Y, double Z, F Tc) {
+  int ii = (int)(round(X * d.idx));
+  int jj = (int)(round(Y * d.idy));
+  int kk = (int)(round(Z * d.idz));
+  const Cm x_l = (Cm)(X * d.idx - ii + 0.5);
+  const Cm y_l = (Cm)(Y * d.idy - jj + 0.5);
+  const Cm z_l = (Cm)(Z * d.idz - kk + 0.5);
+  ii += 1;
+  jj += 1;
+  kk += 1;
+  return x_l * y_l * z_l * Tc(ii, jj, kk) + (1 - x_l) * y_l * z_l * Tc(ii - 1, jj, kk) +
+         x_l * (1 - y_l) * z_l * Tc(ii, jj - 1, kk) + x_l * y_l * (1 - z_l) * Tc(ii, jj, kk - 1) +
+         (1 - x_l) * (1 - y_l) * z_l * Tc(ii - 1, jj - 1, kk) +
+         (1 - x_l) * y_l * (1 - z_l) * Tc(ii - 1, jj, kk - 1) +
+         x_l * (y_l - 1) * (z_l - 1) * Tc(ii, jj - 1, kk - 1) +
+         (1 - x_l) * (1 - y_l) * (1 - z_l) * Tc(ii - 1, jj - 1, kk - 1);
 }
 
 // particle.cpp:402-478: Tg at the stage `st` position (1 = _int, 2 = _new),
@@ -118,29 +160,30 @@ __global__ void k_part_temp(Dev d, PartPtrs p, int st, double c0, const F* __res
   // F = float (precision form): gas temperature interpolation and the heat
   // right-hand side in FP32; F = double is the reference arithmetic.
   using Cm = F;
-  const double X = p.s[0][st][i], Y = p.s[1][st][i], Z = p.s[2][st][i];
-  int ii = (int)(round(X * d.idx));
-  int jj = (int)(round(Y * d.idy));
-  int kk = (int)(round(Z * d.idz));
-  const Cm x_l = (Cm)(X * d.idx - ii + 0.5);
-  const Cm y_l = (Cm)(Y * d.idy - jj + 0.5);
-  const Cm z_l = (Cm)(Z * d.idz - kk + 0.5);
-  ii += 1;
-  jj += 1;
-  kk += 1;
   const Cm cc0 = (Cm)c0;
-  auto T = [&](int a, int b, int e) {
+  auto Tc = [&](int a, int b, int e) {
     return cc0 / (Cm)rho[clampi((long)e * d.NxNy + (long)b * d.Nx + a, d.size)];
   };
-  const Cm Tg = x_l * y_l * z_l * T(ii, jj, kk) +
-                (1 - x_l) * y_l * z_l * T(ii - 1, jj, kk) +
-                x_l * (1 - y_l) * z_l * T(ii, jj - 1, kk) +
-                x_l * y_l * (1 - z_l) * T(ii, jj, kk - 1) +
-                (1 - x_l) * (1 - y_l) * z_l * T(ii - 1, jj - 1, kk) +
-                (1 - x_l) * y_l * (1 - z_l) * T(ii - 1, jj, kk - 1) +
-                x_l * (y_l - 1) * (z_l - 1) * T(ii, jj - 1, kk - 1) +
-                (1 - x_l) * (1 - y_l) * (1 - z_l) * T(ii - 1, jj - 1, kk - 1);
+  const Cm Tg = interp_T<Cm>(d, p.s[0][st][i], p.s[1][st][i], p.s[2][st][i], Tc);
   p.rhs_T[i] = (double)((Tg - (Cm)p.s[6][st][i]) * (Cm)iTth);
+}
+
+// The same in the reference's two members: gas2part_Temp_{int,new} (mode 0:
+// Tg from a gas temperature field into tg) and Compute_RHS_Temp_{int,new}
+// (mode 1: RHS_T = (Tg - T_stage)*iTth).
+__global__ void k_part_temp_split(Dev d, PartPtrs p, int st, int mode,
+                                  const double* __restrict__ T, double* __restrict__ tg,
+                                  double iTth) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.np) return;
+  if (mode == 0) {
+    auto Tc = [&](int a, int b, int e) {
+      return T[clampi((long)e * d.NxNy + (long)b * d.Nx + a, d.size)];
+    };
+    tg[i] = interp_T<double>(d, p.s[0][st][i], p.s[1][st][i], p.s[2][st][i], Tc);
+  } else {
+    p.rhs_T[i] = (tg[i] - p.s[6][st][i]) * iTth;
+  }
 }
 
 // ---- device-wide exclusive scan of int32 ------------------------------------
@@ -1508,8 +1551,11 @@ void PartDev::release() {
     }
   void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, alt.flag, alt.rhs_u, alt.rhs_v, alt.rhs_w,
                  alt.rhs_T, rank, rank_alt, inv, mig_tmp, rank_dep, flg, nl, pNL,
-                 holepos, tailnl, evt, islo, plo, dst, mslot, mlv, inj_buf, dense_c, dense_w};
+                 holepos, tailnl, evt, islo, plo, dst, mslot, mlv, inj_buf, dense_c, dense_w,
+                 gas[0], gas[1], gas[2], gas[3]};
   for (void* p : all) (void)hipFree(p);
+  for (auto& g : gas) g = nullptr;
+  gas_cap = 0;
   free_buckets(R);
   free_buckets(T);
   (void)hipHostFree(hbuf);
@@ -1531,8 +1577,57 @@ PartPtrs PartDev::ptrs() const {
 template <typename T>
 int PartDev::advance(const Dev& d, const AdvCoef& c, const T* const u[3], hipStream_t st) {
   if (np == 0) return FR_OK;
-  hipLaunchKernelGGL(k_part_advance<T>, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), c,
-                     u[0], u[1], u[2]);
+  hipLaunchKernelGGL((k_part_advance<T, PH_ALL>), dim3(nblocks(np, 256)), dim3(256), 0, st, d,
+                     ptrs(), c, u[0], u[1], u[2], nullptr, nullptr, nullptr);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+int PartDev::ensure_gas() {
+  if (gas_cap >= cap && gas[0]) return FR_OK;
+  for (auto& g : gas) {
+    (void)hipFree(g);
+    g = nullptr;
+  }
+  gas_cap = 0;
+  for (auto& g : gas) FR_CHECK(hipMalloc(&g, sizeof(double) * cap));
+  gas_cap = cap;
+  return FR_OK;
+}
+
+int PartDev::advance_phase(const Dev& d, const AdvCoef& c, const double* const u[3], int phase,
+                           hipStream_t st) {
+  if (phase != PH_GATHER && phase != PH_POS && phase != PH_VEL && phase != PH_TEMP)
+    return FR_EINVAL;
+  FR_RET(ensure_gas());
+  if (np == 0) return FR_OK;
+  const dim3 g(nblocks(np, 256)), b(256);
+  const double* uu[3] = {u ? u[0] : nullptr, u ? u[1] : nullptr, u ? u[2] : nullptr};
+  if (phase == PH_GATHER && !(uu[0] && uu[1] && uu[2])) return FR_EINVAL;
+  if (phase == PH_GATHER)
+    hipLaunchKernelGGL((k_part_advance<double, PH_GATHER>), g, b, 0, st, d, ptrs(), c, uu[0],
+                       uu[1], uu[2], gas[0], gas[1], gas[2]);
+  else if (phase == PH_POS)
+    hipLaunchKernelGGL((k_part_advance<double, PH_POS>), g, b, 0, st, d, ptrs(), c, uu[0], uu[1],
+                       uu[2], gas[0], gas[1], gas[2]);
+  else if (phase == PH_VEL)
+    hipLaunchKernelGGL((k_part_advance<double, PH_VEL>), g, b, 0, st, d, ptrs(), c, uu[0], uu[1],
+                       uu[2], gas[0], gas[1], gas[2]);
+  else
+    hipLaunchKernelGGL((k_part_advance<double, PH_TEMP>), g, b, 0, st, d, ptrs(), c, uu[0],
+                       uu[1], uu[2], gas[0], gas[1], gas[2]);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+int PartDev::temp_split(const Dev& d, int stage, int mode, const double* T, double iTth,
+                        hipStream_t st) {
+  if ((stage != 1 && stage != 2) || (mode != 0 && mode != 1) || (mode == 0 && !T))
+    return FR_EINVAL;
+  FR_RET(ensure_gas());
+  if (np == 0) return FR_OK;
+  hipLaunchKernelGGL(k_part_temp_split, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(),
+                     stage, mode, T, gas[3], iTth);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }

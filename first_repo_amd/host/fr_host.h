@@ -108,6 +108,8 @@ class gridsize {
   fr::Dev dev() const { return fr::make_dev(&g); }
 };
 
+class tensor1;
+
 // tensor0: a ghosted scalar field in device memory (tensor0.h:6-132), FP64
 // or, in the FP32 precision form, FP32 storage.
 class tensor0 {
@@ -123,7 +125,6 @@ class tensor0 {
   tensor0(gridsize* p, hipStream_t st);
   ~tensor0();
   tensor0(const tensor0&) = delete;
-  tensor0& operator=(const tensor0&) = delete;
   void init(gridsize* p, hipStream_t st, bool fp32 = false);
   double* pointer() const { return (double*)ptr; }  // FP64 storage
   template <typename T>
@@ -131,6 +132,7 @@ class tensor0 {
     return (T*)ptr;
   }
   bool is_f32() const { return f32; }
+  hipStream_t stream() const { return st_; }
   int size() const { return p_->size(); }
   void swap(tensor0& o) {
     std::swap(ptr, o.ptr);
@@ -154,6 +156,57 @@ class tensor0 {
   double max_abs();
   void download(double* host) const;  // converted to double
   void upload(const double* host);
+
+  // ---- the reference's member surface (tensor0.h:17-131), FP64 storage,
+  // each call queued on the tensor's stream (members.cpp).  The reference's
+  // members return nothing: a call that fails leaves its status in
+  // fr::host_status().
+  int Nx() const { return p_->Nx(); }
+  int Ny() const { return p_->Ny(); }
+  int Nz() const { return p_->Nz(); }
+  int bs() const { return p_->bs(); }
+  double dx() const { return p_->dx(); }
+  double dy() const { return p_->dy(); }
+  double dz() const { return p_->dz(); }
+  double Lx() const { return p_->Lx(); }
+  double Ly() const { return p_->Ly(); }
+  double Lz() const { return p_->Lz(); }
+  gridsize* parameter() const { return p_; }
+  tensor0& operator=(double a);
+  tensor0& operator+=(double a);
+  tensor0& operator-=(double a);
+  tensor0& operator*=(double a);
+  tensor0& operator/=(double a);
+  tensor0& operator=(const tensor0& a);  // values; the storage stays this tensor's
+  tensor0& operator+=(const tensor0& a);
+  tensor0& operator-=(const tensor0& a);
+  tensor0& operator*=(const tensor0& a);
+  tensor0& operator/=(const tensor0& a);
+  void PlusEqual_Mult(tensor0& a, tensor0& b);
+  void PlusEqual_Mult(tensor0& a, double b);
+  void PlusEqual_Mult(double a, tensor0& b);
+  void Equal_Divide(tensor0& a, double b);
+  void Equal_Mult(tensor0& a, tensor0& b);
+  void Equal_Mult(tensor0& a, double b);
+  void Equal_Mult(double a, tensor0& b);
+  void Equal_LinComb(double a, tensor0& T, double b, tensor0& R);
+  void Equal_LinComb(tensor0& A, tensor0& T, double b, tensor0& R);
+  void Equal_LinComb(tensor0& A, tensor0& T, tensor0& B, tensor0& R);
+  void Equal_LinComb(double a, tensor0& T, tensor0& B, tensor0& R);
+  void Equal_LinComb(double a, tensor0& T, double b, tensor0& R, double c, tensor0& S);
+  void Equal_Div_F2C(tensor1& T);
+  void Equal_Del2(tensor0& T);
+  void Equal_Ix_C2F(tensor0& T);
+  void Equal_Iy_C2F(tensor0& T);
+  void Equal_Iz_C2F(tensor0& T);
+  void Equal_Ix_F2C(tensor0& T);
+  void Equal_Iy_F2C(tensor0& T);
+  void Equal_Iz_F2C(tensor0& T);
+  void Equal_I_F2C(tensor1& T);
+
+ private:
+  void arith(int op, double a, const tensor0* A, const tensor0* B, double b, const tensor0* C,
+             double c, const tensor0* D);
 };
 
 // tensor1: three staggered components (tensor1.h:6-90).
@@ -203,6 +256,62 @@ class tensor1 {
     out[2] = z.pointer();
     return out;
   }
+
+  // ---- the reference's member surface (tensor1.h:6-90), per component
+  // through the tensor0 members (members.cpp)
+  int Nx() const { return x.Nx(); }
+  int Ny() const { return x.Ny(); }
+  int Nz() const { return x.Nz(); }
+  int bs() const { return x.bs(); }
+  double dx() const { return x.dx(); }
+  double dy() const { return x.dy(); }
+  double dz() const { return x.dz(); }
+  double Lx() const { return x.Lx(); }
+  double Ly() const { return x.Ly(); }
+  double Lz() const { return x.Lz(); }
+  gridsize* parameter() const { return x.parameter(); }
+  tensor1& operator=(double a);
+  tensor1& operator+=(double a);
+  tensor1& operator-=(double a);
+  tensor1& operator*=(double a);
+  tensor1& operator/=(double a);
+  tensor1& operator=(const tensor1& a);
+  tensor1& operator+=(const tensor1& a);
+  tensor1& operator-=(const tensor1& a);
+  tensor1& operator*=(const tensor1& a);
+  tensor1& operator/=(const tensor1& a);
+  tensor1& operator=(const tensor0& a);
+  tensor1& operator+=(const tensor0& a);
+  tensor1& operator-=(const tensor0& a);
+  tensor1& operator*=(const tensor0& a);
+  tensor1& operator/=(const tensor0& a);
+  void Update_Ghosts();
+  void Update_Ghosts_CUM();
+  double sum();
+  double max();
+  double min();
+  void PlusEqual_Mult(tensor1& a, tensor1& b);
+  void PlusEqual_Mult(tensor1& a, double b);
+  void PlusEqual_Mult(double a, tensor1& b);
+  void Equal_Divide(tensor1& a, tensor1& b);
+  void Equal_Divide(tensor1& a, double b);
+  void Equal_Divide(tensor1& a, tensor0& b);
+  void Equal_Divide(tensor0& a, tensor1& b);
+  void Equal_Mult(tensor1& a, tensor1& b);
+  void Equal_Mult(tensor1& a, double b);
+  void Equal_Mult(double a, tensor1& b);
+  void Equal_LinComb(double a, tensor1& T, double b, tensor1& R);
+  void Equal_LinComb(tensor1& A, tensor1& T, double b, tensor1& R);
+  void Equal_LinComb(tensor1& A, tensor1& T, tensor1& B, tensor1& R);
+  void Equal_LinComb(double a, tensor1& T, tensor1& B, tensor1& R);
+  void Equal_LinComb(double a, tensor1& T, double b, tensor1& R, double c, tensor1& S);
+  void Equal_I_C2F(tensor0& T);
+  void Equal_Ix_C2F(tensor1& T);
+  void Equal_Iy_C2F(tensor1& T);
+  void Equal_Iz_C2F(tensor1& T);
+  void Equal_I_F2C(tensor1& T);
+  void Equal_Grad_C2F(tensor0& T);
+  void Equal_Del2(tensor1& T);
 };
 
 // particle (particle.h:12-155): device SoA set on one rank.
@@ -215,7 +324,7 @@ class particle {
   int Np = 0;
   particle() {}
   int init(params* p, gridsize* s, hipStream_t st, int capacity);
-  ~particle() { dev.release(); }
+  ~particle();
   int load_random();  // particle.cpp:919-958
   // particle.cpp:340-399 + 314-338 + 480-496 (gather and the three ODE updates)
   int advance(int k, double pre, double post, tensor1& U);
@@ -234,6 +343,28 @@ class particle {
   int temp(int stage, double c0, const tensor0& Rho, hipStream_t on = 0);
   int part2gas_Temp(int stage, tensor0& A, hipStream_t on = 0);  // particle.cpp:586-658
   int rebuild_cells(int stage, hipStream_t on = 0);
+
+  // ---- the reference's member surface (particle.h:126-154), one member at
+  // a time (FP64; members.cpp).  The time step itself runs them fused
+  // (advance, temp).  part2gas_velocity, part2gas_concentration, Send_Recv
+  // and load_random above keep the reference's names and arguments.
+  int update_position(int k, double pre, double post);
+  int update_velocity(int k, double pre, double post);
+  int update_Temp(int k, double pre, double post);
+  int gas2part_velocity(tensor1& U);
+  int gas2part_Temp_int(tensor0& T);
+  int gas2part_Temp_new(tensor0& T);
+  int Compute_RHS_Temp_int();
+  int Compute_RHS_Temp_new();
+  int part2gas_Temp_int(tensor0& A);
+  int part2gas_Temp_new(tensor0& A);
+  // the duct rank that is inlet and outlet: the box's '>hi' list of its last
+  // Send_Recv() is appended (particle.cpp:814-851, 860-865)
+  int Send_Recv(particle& pbox);
+  double* rows = nullptr;  // that list in device memory, 29 values a row
+  int rows_cap = 0;
+  int* rows_n = nullptr;
+  fr::AdvCoef adv_coef(int k, double pre, double post) const;
 };
 
 // poisson (poisson.h:19-81), FFT solver branch, and poisson_IO
@@ -255,6 +386,41 @@ class poisson {
   int Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS);  // P: FP64 storage
   int num_iteration() const { return num_iteration_; }
 };
+
+// poisson_IO (poisson_IO.h:20-88): the duct's pressure solve through the
+// reference's interface.  Solve(C, P, RHS) solves div(C grad P) = RHS with P
+// the initial guess (FFT in y, z and Thomas in x, defect correction to
+// max|dp| <= epsilon; or, with Solver = 2, PCG with the FP32 multigrid
+// preconditioner, C = 1/Rho_face given as Rho_face through Solve_face_rho).
+// num_iteration1() is the outer iteration count; num_iteration2() the mean
+// inner iterations per outer one: 0 for the direct FFT + Thomas inner solve
+// (the reference leaves it unset on that branch), the PCG count with
+// Solver = 2 (one outer iteration).  FFT_TIME and SOLVER_TIME sum seconds
+// of the FFT passes (with the residual formed in the first) and of the
+// Thomas sweeps, from HIP events.
+class poisson_IO {
+ public:
+  poisson PS;
+  double FFT_TIME = 0, SOLVER_TIME = 0;
+  int n1_ = 0;
+  double n2_ = 0;
+  poisson_IO() {}
+  int init(params* p, gridsize* g, hipStream_t st);
+  int Solve(tensor1& C, tensor0& P, tensor0& RHS);
+  int Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS);
+  int num_iteration1() const { return n1_; }
+  double num_iteration2() const { return n2_; }
+
+ private:
+  int account();
+};
+
+namespace fr {
+// First nonzero status of a reference-surface member that returns nothing
+// (tensor0/tensor1 members); reset clears it.
+int host_status(bool reset = false);
+void host_note(int rc);
+}  // namespace fr
 
 // grid (grid.h:16-115): the box and duct time-step methods.  For the duct
 // the box grid g of the reference's grid::*(grid& g) methods is replaced by
