@@ -475,7 +475,9 @@ int grid::setup(const char* params_path, const char* ic) {
   param = params(params_path);
   if (!param.ok) return FR_EINVAL;
   if (param.bs() != 1) return FR_EINVAL;
-  if (f32 && (!param.IsBox() || param.Solver() != 0)) return FR_EINVAL;
+  // the precision form: the box with the FFT solve (FP32 inner transform) or
+  // with PCG and the FP32 multigrid preconditioner (Solver = 2)
+  if (f32 && (!param.IsBox() || (param.Solver() != 0 && param.Solver() != 2))) return FR_EINVAL;
   size = gridsize(param);
   ic_dir = ic ? ic : ".";
   if (const char* e = getenv("FR_RESORT_EVERY")) resort_every = atoi(e) < 1 ? 1 : atoi(e);
@@ -940,7 +942,15 @@ int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
 int grid::Solve_Poisson() {  // grid.cpp:639-643
   // C = 1./Rho_face is formed inside the solver's residual
   hipEventRecord(ev[0], st);
-  FR_TRY(PS_.Solve_face_rho(Rho_face, P, RHS_Pois));
+  if (f32 && PS_.sp) {
+    // the PCG's FP64 operator reads the FP32 faces widened (the FFT solve
+    // widens them inside its residual); its V-cycles run in FP32 either way
+    for (int m = 0; m < 3; m++)
+      to_double(Rho_face.c(m).p<float>(), d64b.c(m).pointer(), size.size(), st);
+    FR_TRY(PS_.Solve_face_rho(d64b, P, RHS_Pois));
+  } else {
+    FR_TRY(PS_.Solve_face_rho(Rho_face, P, RHS_Pois));
+  }
   hipEventRecord(ev[1], st);
   hipEventSynchronize(ev[1]);
   float ms = 0;
@@ -1247,6 +1257,7 @@ static double* scal(fr_sim* s, const char* name) {
   if (n == "C_BC") return &g.C_BC;
   if (n == "U_avg_in") return &g.U_avg_in;
   if (n == "U_avg_out") return &g.U_avg_out;
+  if (n == "relres") return &g.PS_.relres_;
   return nullptr;
 }
 

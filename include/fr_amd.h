@@ -229,9 +229,11 @@ int fr_sim_create(const char* params_path, const char* ic_dir, fr_sim** out);
 /* fr_sim_create with options.  FR_SIM_FP32 selects the precision form: every
  * field but P held and advanced in FP32 (stencils, Rho_face, the Poisson
  * right-hand side, the particle deposits), while P, the Poisson residual and
- * every sum stay FP64; particles stay FP64.  Box geometry with the FFT
- * solver (Solver = 0) only, else FR_EINVAL.  No counterpart in the
- * reference, which runs FP64 throughout (grid.h:16-115). */
+ * every sum stay FP64; particles stay FP64.  The pressure solve is FP64
+ * defect correction around an FP32 inner FFT solve (Solver = 0) or FP64 PCG
+ * around FP32 multigrid V-cycles (Solver = 2).  Box geometry only, else
+ * FR_EINVAL.  No counterpart in the reference, which runs FP64 throughout
+ * (grid.h:16-115). */
 #define FR_SIM_FP32 1
 /* FR_SIM_OUTPUT: the reference's run files, relative to the working
  * directory as grid.cpp writes them: touch.check ("0") and, when params.in
@@ -254,7 +256,9 @@ int fr_sim_call(fr_sim* s, const char* method, int rk4_count);
 int fr_sim_get_field(fr_sim* s, const char* name, int comp, double* host);
 int fr_sim_set_field(fr_sim* s, const char* name, int comp,
                      const double* host);
-/* Scalars: P0 P0_int P0_new P0_np1 T_cur C_BC U_avg_in U_avg_out */
+/* Scalars: P0 P0_int P0_new P0_np1 T_cur C_BC U_avg_in U_avg_out, and
+ * relres: the relative residual ||r||/||b|| the last Solver = 2 pressure
+ * solve stopped at (setting it has no effect). */
 int fr_sim_get_scalar(fr_sim* s, const char* name, double* value);
 int fr_sim_set_scalar(fr_sim* s, const char* name, double value);
 int fr_sim_geom(fr_sim* s, fr_geom* out);
