@@ -1,6 +1,7 @@
 // C entry points of the oracle for the test suite and the CPU baseline in
 // bench.py.  TEST INFRASTRUCTURE: only tests/, the smoke check and the CPU
 // baseline leg load this library.
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -241,6 +242,29 @@ int orc_poisson(const orc_geom* gg, const double* cx, const double* cy,
   }
   store(p, P);
   return it;
+}
+// One V-cycle of the PCG preconditioner applied to the interior residual r
+// ([Nz][Ny][Nx] floats) with e = 0 on entry, the hierarchy cut at `levels`
+// (0: the solver's own), pre/post red-black sweeps and coarse_sweeps on the
+// coarsest level; e receives the fine-level correction.
+void orc_mg_vcycle(const orc_geom* gg, const double* rfx, const double* rfy, const double* rfz,
+                   const float* r, float* e, int levels, int pre, int post, int coarse_sweeps) {
+  Geom g = mk(gg);
+  F1 C;
+  C.init(&g);
+  load(C.x, rfx);
+  load(C.y, rfy);
+  load(C.z, rfz);
+  MGPCG m;
+  m.max_levels = levels;
+  m.pre = pre;
+  m.post = post;
+  m.coarse_sweeps = coarse_sweeps;
+  m.build(&g);
+  m.coefficients(C);
+  std::copy(r, r + m.L[0].r.size(), m.L[0].r.begin());
+  m.vcycle(0);
+  std::copy(m.L[0].e.begin(), m.L[0].e.end(), e);
 }
 void orc_glibc_rand(int n, int* out) {
   GlibcRand r;

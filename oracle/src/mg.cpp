@@ -65,6 +65,7 @@ void MGPCG::build(const Geom* gg) {
     lv.r.assign(n, 0.f);
     lv.t.assign(n, 0.f);
     L.push_back(lv);
+    if (max_levels && (int)L.size() >= max_levels) break;
     if (nx % 2 || ny % 2 || nz % 2 || nx < 8 || ny < 8 || nz < 8) break;
     nx /= 2;
     ny /= 2;

@@ -403,6 +403,7 @@ struct MGPCG {
   bool duct = false;
   std::vector<MGLevel> L;
   int pre = 2, post = 2, coarse_sweeps = 20;
+  int max_levels = 0;  // 0: coarsen while every size is even and >= 8
   void build(const Geom* gg);
   void coefficients(const F1& Rho_face);
   void smooth(MGLevel& v, int color);

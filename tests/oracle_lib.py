@@ -99,6 +99,7 @@ def orc():
         L.orc_grid_initialize.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_grid_finish_initialize.argtypes = [C.c_void_p]
         L.orc_grid_step.argtypes = [C.c_void_p]
+        L.orc_mg_vcycle.argtypes = [C.c_void_p] * 6 + [C.c_int] * 4
         L.orc_grid_sbuf0.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.orc_grid_sbuf0.restype = C.POINTER(C.c_double)
         L.orc_pair_initialize.argtypes = [C.c_void_p] * 8

@@ -3,6 +3,7 @@
 closed forms, brute force and the reference's own manufactured-solution tests
 (grid::Test_Poisson, grid.cpp:975-1023; grid::Test_Poisson_IO, 1025-1094)."""
 import ctypes as C
+import itertools
 import math
 import subprocess
 import sys
@@ -10,6 +11,7 @@ import sys
 import numpy as np
 import pytest
 
+import first_repo_amd as fr
 import oracle_lib as OL
 
 
@@ -482,3 +484,89 @@ def test_tke_and_dissipation_closed_forms(tmp_path):
         out = og.diagnostics()
         assert abs(out[0] - tke) <= 1e-12 * rho0 * A * A, (name, out[0], tke)
         assert abs(out[7] - diss) <= 1e-12 * diss, (name, out[7], diss)
+
+
+def _two_grid_lfa(N, pre, post):
+    """Two-grid convergence factor of the cycle on the periodic N^3 grid, by
+    local Fourier analysis: the 7-point operator (symbol sum 2(1-cos)),
+    red-black Gauss-Seidel half sweeps (red then black before the coarse
+    correction, black then red after) acting on the pairs of harmonics
+    theta, theta+(pi,pi,pi); restriction by the mean of the 8 children
+    (symbol prod (1+e^{i theta_d})/2), prolongation by injection (its
+    conjugate), the coarse operator rediscretised at spacing 2, solved
+    exactly.  The cycle couples the 8 harmonics theta + pi*alpha."""
+    alphas = list(itertools.product((0, 1), repeat=3))
+    best = 0.0
+    ks = range(-N // 4 + 1, N // 4 + 1)
+    for k in itertools.product(ks, repeat=3):
+        th = 2 * np.pi * np.array(k) / N
+        H = [th + np.pi * np.array(a) for a in alphas]
+        A = np.diag([np.sum(2 * (1 - np.cos(h))) for h in H]).astype(complex)
+        R = np.array([[np.prod((1 + np.exp(1j * h)) / 2) for h in H]])
+        P = np.array([[np.prod((1 + np.exp(-1j * h)) / 2)] for h in H])
+        Ac = np.sum(2 * (1 - np.cos(2 * th))) / 4.0
+        SR = np.zeros((8, 8), complex)
+        SB = np.zeros((8, 8), complex)
+        for i, a in enumerate(alphas):
+            j = alphas.index(tuple(1 - x for x in a))
+            if i > j:
+                continue
+            # red values are a+b, black a-b in the basis (harmonic i, j)
+            g = np.mean(np.cos(H[i]))
+            sr = [[(1 + g) / 2, -(1 + g) / 2], [(g - 1) / 2, -(g - 1) / 2]]
+            sb = [[(1 + g) / 2, (1 + g) / 2], [(1 - g) / 2, (1 - g) / 2]]
+            for p_ in range(2):
+                for q_ in range(2):
+                    SR[[i, j][p_], [i, j][q_]] = sr[p_][q_]
+                    SB[[i, j][p_], [i, j][q_]] = sb[p_][q_]
+        Spre = np.linalg.matrix_power(SB @ SR, pre)
+        Spost = np.linalg.matrix_power(SR @ SB, post)
+        if Ac == 0:  # theta = 0: no correction, the constant mode left out
+            keep = [i for i, a in enumerate(alphas) if a != (0, 0, 0)]
+            M = (Spost @ Spre)[np.ix_(keep, keep)]
+        else:
+            M = Spost @ (np.eye(8) - P @ R @ A / Ac) @ Spre
+        best = max(best, max(abs(np.linalg.eigvals(M))))
+    return best
+
+
+def _two_grid_measured(N, pre, post, its=40):
+    """The oracle's own V-cycle cut to two levels (coarse level swept to
+    convergence) as a stationary iteration e <- e + V(-A e) on zero-mean
+    errors, constant coefficients: the asymptotic error reduction per cycle."""
+    g = fr.make_geom(N, N, N, L=(N, N, N), box=True)
+    og = OL.Geom.from_buffer_copy(g)
+    one = np.ones((g.Nz, g.Ny, g.Nx))
+    O = OL.orc()
+    rng = np.random.default_rng(0)
+    e = rng.standard_normal((N, N, N))
+    e -= e.mean()
+    e /= np.linalg.norm(e)
+    logs = []
+    for _ in range(its):
+        Ae = sum(2 * e - np.roll(e, 1, a) - np.roll(e, -1, a) for a in range(3))
+        r = np.ascontiguousarray(-Ae, dtype=np.float32)
+        v = np.zeros_like(r)
+        O.orc_mg_vcycle(C.byref(og), OL.ptr(one), OL.ptr(one), OL.ptr(one),
+                        r.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), 2, pre,
+                        post, 300)
+        e = e + v.astype(np.float64)
+        e -= e.mean()
+        n = np.linalg.norm(e)
+        logs.append(np.log(n))
+        e /= n
+    return float(np.exp(np.mean(logs[its // 2:])))
+
+
+@pytest.mark.parametrize("N,pre,post,tol", [(8, 2, 2, 1e-4), (8, 1, 1, 1e-4), (16, 2, 2, 0.02)])
+def test_vcycle_two_grid_factor_matches_fourier_analysis(N, pre, post, tol):
+    """The FP32 multigrid preconditioner's cycle (smoother, restriction,
+    prolongation and coarse-operator scaling) on constant coefficients: its
+    measured two-grid convergence factor equals the closed-form Fourier
+    factor (0.0878 for two pre- and post-sweeps, 4/9 for one).  A sum in place of the
+    mean in the restriction, a wrong coarse scaling or a missing sweep moves
+    it far off."""
+    lfa = _two_grid_lfa(N, pre, post)
+    meas = _two_grid_measured(N, pre, post)
+    assert 0 < lfa < 0.5
+    assert abs(meas - lfa) <= tol * lfa, (meas, lfa)
