@@ -48,7 +48,11 @@ METRIC = ("Mcell·steps/s full RK4 step w/ particles; "
 # by what the counters read for a plain copy of known size: factors 2.0 and
 # 1.0), measured on the named workloads (profiles/hbm_traffic_*.md)
 TRAFFIC = {("k_pass_x_fwd", 256, 4_000_000, "fp64"): 1.051e9,
-           ("k_pass_x_fwd", 512, 16_000_000, "fp32"): 6.437e9}
+           ("k_pass_x_fwd", 512, 16_000_000, "fp32"): 6.437e9,
+           ("k_pass_x_fwd", 512, 16_000_000, "fp64"): 9.500e9,
+           ("k_update_ru_wop_z", 512, 16_000_000, "fp64"): 26.063e9,
+           ("k_dep_vel", 512, 16_000_000, "fp64"): 5.500e9,
+           ("k_dep_cic", 512, 16_000_000, "fp64"): 3.209e9}
 
 
 def box_workload(n, particles, solver="fft"):
@@ -87,7 +91,7 @@ def run_box(params, tmp, fp32, steps, warmup, resort=None):
     return out
 
 
-def kernel_table(n, particles, fp32, r, steps):
+def kernel_table(n, particles, fp32, r, steps, precision="fp64"):
     """Algorithmic bytes per launch and achieved bandwidth of the timed
     kernels.  Bytes count each distinct array element read or written once
     per launch (es = field element size, ghosted = (n+2)^3 cells):
@@ -124,7 +128,8 @@ def kernel_table(n, particles, fp32, r, steps):
         out.append({"kernel": name, "what": what, "alg_bytes_per_launch": int(nbytes),
                     "avg_launch_ms": round(per, 4), "launches_per_step": round(nl / steps, 2),
                     "ms_per_step": round(ms / steps, 3), "achieved_GBs": round(gbs, 1),
-                    "frac": round(gbs / HBM_PEAK_GBS, 4)})
+                    "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "traffic": TRAFFIC.get((name, n, particles, precision))})
     return out
 
 
@@ -473,7 +478,7 @@ def main():
         step_bytes = ncell * (4 * 208 + 4 * 80 * n_it) + particles * 4 * 264
     else:
         step_bytes = ncell * (4 * 416 + 4 * 96 * n_it) + particles * 4 * 288
-    kernels = kernel_table(n, particles, fp32, r, args.steps)
+    kernels = kernel_table(n, particles, fp32, r, args.steps, args.precision)
     top = max(kernels, key=lambda k: k["ms_per_step"])
     diag = r["diag"]
     form = ("precision form (FP32 fields/stencils, FP64 pressure and particles)" if fp32
