@@ -491,6 +491,14 @@ class grid {
   // joined by join_side() before anything reads RHS_Part_Temp or qmean
   bool side_pending = false;
   bool side_held = false;  // branch recorded but its launches not yet issued
+  // In a full time step of the box, a substep's concentration deposit is
+  // issued on st2 at the top of the substep, beside the density update and
+  // the particle advance (memory-bound kernels beside an issue-bound one):
+  // its inputs, the rounded-index buckets of the previous heat branch, are
+  // final there, and nothing reads C before the step ends.  conc_done marks
+  // it issued for the current substep; Update_Particle issues it otherwise.
+  bool conc_done = false;
+  hipEvent_t evc = nullptr;
   int launch_side();
   double* dq = nullptr;  // reduction workspace + result of that branch
   double* dmz = nullptr;  // workspace of the main stream's device-side means

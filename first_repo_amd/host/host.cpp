@@ -460,6 +460,7 @@ grid::~grid() {
   if (evf) hipEventDestroy(evf);
   if (evj) hipEventDestroy(evj);
   if (evj2) hipEventDestroy(evj2);
+  if (evc) hipEventDestroy(evc);
   if (evm) hipEventDestroy(evm);
   if (evm0) hipEventDestroy(evm0);
   if (st3) hipStreamDestroy(st3);
@@ -489,6 +490,7 @@ int grid::setup(const char* params_path, const char* ic) {
   FR_CHECK(hipEventCreateWithFlags(&evf, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evj, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evj2, hipEventDisableTiming));
+  FR_CHECK(hipEventCreateWithFlags(&evc, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evq, hipEventDisableTiming));
   for (auto& e : ev) FR_CHECK(hipEventCreate(&e));
   for (auto& e : evk) FR_CHECK(hipEventCreate(&e));
@@ -709,7 +711,8 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   // is the order of the reference, whose deposits precede Send_Recv.
   FR_CHECK(hipEventRecord(evf, st));
   FR_CHECK(hipStreamWaitEvent(st2, evf, 0));
-  FR_TRY(part.part2gas_concentration(C, st2));
+  if (!conc_done) FR_TRY(part.part2gas_concentration(C, st2));
+  conc_done = false;
   const bool box = param.IsBox();
   const int* rk = nullptr;
   if (box) {
@@ -1007,6 +1010,7 @@ int grid::TimeAdvance() {  // grid.cpp:315-325
 
 int grid::begin_step() {
   param.update(T_cur);
+  conc_done = false;
   t_poisson = t_particles = 0;
   iters_step = 0;
   box_poisson_xtimer(PS_.s, nullptr, nullptr, true);
@@ -1020,6 +1024,12 @@ int grid::begin_step() {
 
 int grid::substep_front(int k) {
   RK4_count = k;
+  if (param.IsBox()) {
+    FR_CHECK(hipEventRecord(evc, st));
+    FR_CHECK(hipStreamWaitEvent(st2, evc, 0));
+    FR_TRY(part.part2gas_concentration(C, st2));
+    conc_done = true;
+  }
   FR_TRY(Update_Rho());
   FR_TRY(Update_P0());
   FR_TRY(Update_Particle());
