@@ -333,7 +333,7 @@ class PartDev {
   int deposit_cic(const Dev& d, bool weighted, F* out, bool halo, hipStream_t st);
   // momentum deposit from T
   template <typename F>
-  int deposit_vel(const Dev& d, F* const out[3], hipStream_t st);
+  int deposit_vel(const Dev& d, F* const out[3], hipStream_t st, bool halo = true);
   // The particle members of the reference taken one at a time (FP64):
   // phase 1 gas2part_velocity (gathered velocity kept per storage slot in
   // gas[0..2]), 2 update_position, 4 update_velocity, 8 update_Temp

@@ -1710,7 +1710,7 @@ int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStre
 }
 
 template <typename F>
-int PartDev::deposit_vel(const Dev& d, F* const out[3], hipStream_t st) {
+int PartDev::deposit_vel(const Dev& d, F* const out[3], hipStream_t st, bool halo) {
   if (!T.valid) return FR_EINVAL;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   t_vel.next(&e0, &e1);
@@ -1724,7 +1724,7 @@ int PartDev::deposit_vel(const Dev& d, F* const out[3], hipStream_t st) {
                           out[2]);
   FR_LAUNCH_CHECK();
   int rc = cum_fold_n(d, out, 3, st);  // the three components, one launch per phase
-  return rc ? rc : ghost_fill_n(d, out, 3, st);
+  return rc || !halo ? rc : ghost_fill_n(d, out, 3, st);
 }
 
 // particle::Send_Recv on one rank: axes z, y, x (particle.cpp:853-858).  The
@@ -1911,7 +1911,7 @@ template int PartDev::temp<double>(const Dev&, int, double, const double*, doubl
 template int PartDev::temp<float>(const Dev&, int, double, const float*, double, hipStream_t);
 template int PartDev::deposit_cic<double>(const Dev&, bool, double*, bool, hipStream_t);
 template int PartDev::deposit_cic<float>(const Dev&, bool, float*, bool, hipStream_t);
-template int PartDev::deposit_vel<double>(const Dev&, double* const[3], hipStream_t);
-template int PartDev::deposit_vel<float>(const Dev&, float* const[3], hipStream_t);
+template int PartDev::deposit_vel<double>(const Dev&, double* const[3], hipStream_t, bool);
+template int PartDev::deposit_vel<float>(const Dev&, float* const[3], hipStream_t, bool);
 
 }  // namespace fr

@@ -331,7 +331,10 @@ class particle {
   int part2gas_concentration(tensor0& A, hipStream_t on = 0);  // particle.cpp:526-559
   // rk: reference indices to order the deposit by (a copy taken before a
   // migration that runs beside it); default the current ones
-  int part2gas_velocity(tensor1& A, const int* rk = nullptr);  // particle.cpp:498-524
+  // halo = false leaves the periodic ghost shell of A unformed (the fold into
+  // the interior is done either way)
+  int part2gas_velocity(tensor1& A, const int* rk = nullptr,
+                        bool halo = true);  // particle.cpp:498-524
   int Send_Recv(hipStream_t on = 0);                            // particle.cpp:853-858
   // duct rank that is inlet and outlet: Send_Recv(particle& pbox)
   // (particle.cpp:860-865, 814-851) with the box's Rbuf1 given as n inlet
@@ -491,13 +494,16 @@ class grid {
   // joined by join_side() before anything reads RHS_Part_Temp or qmean
   bool side_pending = false;
   bool side_held = false;  // branch recorded but its launches not yet issued
-  // In a full time step of the box, a substep's concentration deposit is
-  // issued on st2 at the top of the substep, beside the density update and
-  // the particle advance (memory-bound kernels beside an issue-bound one):
-  // its inputs, the rounded-index buckets of the previous heat branch, are
-  // final there, and nothing reads C before the step ends.  conc_done marks
-  // it issued for the current substep; Update_Particle issues it otherwise.
-  bool conc_done = false;
+  // In a full time step of the box (substep_front), Update_Particle runs as
+  // a part of the step: the substep's concentration deposit was issued on
+  // st2 at the top of the substep, beside the density update and the
+  // particle advance (memory-bound kernels beside an issue-bound one) -- its
+  // inputs, the rounded-index buckets of the previous heat branch, are final
+  // there, and nothing reads C before the step ends -- and the momentum
+  // deposit's periodic ghost shell is not formed: its one reader in the
+  // step, Update_RU_WOP, reads RHS_RU at interior faces only.  Called on its
+  // own, Update_Particle does both as the reference does.
+  bool step_front = false;
   hipEvent_t evc = nullptr;
   int launch_side();
   double* dq = nullptr;  // reduction workspace + result of that branch

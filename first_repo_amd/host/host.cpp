@@ -342,15 +342,15 @@ int particle::part2gas_concentration(tensor0& A, hipStream_t on) {
   return dev.deposit_cic(size_->dev(), false, A.pointer(), false, s);
 }
 
-int particle::part2gas_velocity(tensor1& A, const int* rk) {
+int particle::part2gas_velocity(tensor1& A, const int* rk, bool halo) {
   int rc = dev.build(size_->dev(), 1, 1, st_, true, rk);
   if (rc) return rc;
   if (A.x.is_f32()) {
     float* o[3] = {A.x.p<float>(), A.y.p<float>(), A.z.p<float>()};
-    return dev.deposit_vel(size_->dev(), o, st_);
+    return dev.deposit_vel(size_->dev(), o, st_, halo);
   }
   double* o[3] = {A.x.pointer(), A.y.pointer(), A.z.pointer()};
-  return dev.deposit_vel(size_->dev(), o, st_);
+  return dev.deposit_vel(size_->dev(), o, st_, halo);
 }
 
 int particle::Send_Recv(hipStream_t on) {
@@ -711,8 +711,9 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   // is the order of the reference, whose deposits precede Send_Recv.
   FR_CHECK(hipEventRecord(evf, st));
   FR_CHECK(hipStreamWaitEvent(st2, evf, 0));
-  if (!conc_done) FR_TRY(part.part2gas_concentration(C, st2));
-  conc_done = false;
+  const bool front = step_front;
+  step_front = false;
+  if (!front) FR_TRY(part.part2gas_concentration(C, st2));
   const bool box = param.IsBox();
   const int* rk = nullptr;
   if (box) {
@@ -725,7 +726,7 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   }
   // the momentum deposit is the longer branch: its launches go first
   if (param.TWC())
-    FR_TRY(part.part2gas_velocity(RHS_RU, rk));
+    FR_TRY(part.part2gas_velocity(RHS_RU, rk, !front));
   else
     RHS_RU.assign(0);
   // Heat branch on st2 (gas temperature at the particles, buckets of the new
@@ -1010,7 +1011,7 @@ int grid::TimeAdvance() {  // grid.cpp:315-325
 
 int grid::begin_step() {
   param.update(T_cur);
-  conc_done = false;
+  step_front = false;
   t_poisson = t_particles = 0;
   iters_step = 0;
   box_poisson_xtimer(PS_.s, nullptr, nullptr, true);
@@ -1028,7 +1029,7 @@ int grid::substep_front(int k) {
     FR_CHECK(hipEventRecord(evc, st));
     FR_CHECK(hipStreamWaitEvent(st2, evc, 0));
     FR_TRY(part.part2gas_concentration(C, st2));
-    conc_done = true;
+    step_front = true;
   }
   FR_TRY(Update_Rho());
   FR_TRY(Update_P0());
