@@ -102,6 +102,123 @@ __device__ inline void load_tw(C2* tws, const C2* tw, int L) {
   for (int t = threadIdx.x; t <= L / 2; t += FT) tws[t] = tw[t];
 }
 
+// ---- one pencil per 64-lane wave, in registers ---------------------------------
+// Length L = 2^LG (LG 8..10): each lane holds E = L/64 elements.  The same
+// radix-2 DIT butterflies and twiddles as fft_lds, grouped into passes of up
+// to log2(E) consecutive stages: a pass's groups (2^g elements at stride
+// 2^s0, closed under stages s0..s0+g-1) sit in the lanes' registers, and the
+// passes exchange elements through a wave-private LDS area, so the transform
+// has no workgroup barrier.  The result is bit-identical to fft_lds.
+template <int LG>
+struct WavePlan {
+  static constexpr int L = 1 << LG, LE = LG - 6, E = 1 << LE;
+  static constexpr int XS = L + (L >> 3);  // exchange entries per wave (padded)
+  static constexpr int S0(int p) { return p * LE; }
+  static constexpr int G(int p) { return LG - p * LE < LE ? LG - p * LE : LE; }
+  static constexpr int NPASS = (LG + LE - 1) / LE;
+};
+
+// padded LDS index: one spare entry every 8 keeps the strided phases of an
+// exchange off a single bank group
+__device__ __forceinline__ int wave_xi(int p) { return p + (p >> 3); }
+
+// Buffer position of register slot k of `lane` in the pass (S0, GG): slot
+// k = r*2^GG + m is element m of the lane's group r; group ids run lane-major
+// in the first pass (so the bit-reversed load is coalesced) and r-major
+// after it (so the natural-order result is).
+template <int LG, int S0, int GG>
+__device__ __forceinline__ int wave_pos(int lane, int k) {
+  constexpr int E = WavePlan<LG>::E, R = E >> GG;
+  const int r = k >> GG, m = k & ((1 << GG) - 1);
+  const int gid = S0 == 0 ? lane * R + r : r * 64 + lane;
+  const int pos = gid & ((1 << S0) - 1);
+  return ((gid >> S0) << (S0 + GG)) + pos + (m << S0);
+}
+
+template <int LG, int S0, int GG, typename C2>
+__device__ __forceinline__ void wave_stages(C2 (&x)[WavePlan<LG>::E], int lane, const C2* tw) {
+  constexpr int E = WavePlan<LG>::E, G = 1 << GG, R = E >> GG;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int gid = S0 == 0 ? lane * R + r : r * 64 + lane;
+    const int pos = gid & ((1 << S0) - 1);
+#pragma unroll
+    for (int t = 0; t < GG; t++) {
+      const int s = S0 + t, h = 1 << t;
+#pragma unroll
+      for (int m = 0; m < G; m++) {
+        if (m & h) continue;
+        bfly(x[r * G + m], x[r * G + m + h], tw[(pos + ((m & (h - 1)) << S0)) << (LG - 1 - s)]);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Registers of pass PA to those of pass PB through the wave's area ex.
+template <int LG, int PA, int PB, typename C2>
+__device__ __forceinline__ void wave_exchange(C2 (&x)[WavePlan<LG>::E], C2* ex, int lane) {
+  using P = WavePlan<LG>;
+#pragma unroll
+  for (int k = 0; k < P::E; k++) ex[wave_xi(wave_pos<LG, P::S0(PA), P::G(PA)>(lane, k))] = x[k];
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < P::E; k++) x[k] = ex[wave_xi(wave_pos<LG, P::S0(PB), P::G(PB)>(lane, k))];
+  wave_sync();
+}
+
+template <int LG, int PP, typename C2>
+__device__ __forceinline__ void wave_passes(C2 (&x)[WavePlan<LG>::E], C2* ex, int lane,
+                                            const C2* tw) {
+  using P = WavePlan<LG>;
+  if constexpr (PP < P::NPASS) {
+    if constexpr (PP > 0) wave_exchange<LG, PP - 1, PP>(x, ex, lane);
+    wave_stages<LG, P::S0(PP), P::G(PP)>(x, lane, tw);
+    wave_passes<LG, PP + 1>(x, ex, lane, tw);
+  }
+}
+
+// On entry x[k] is buffer element wave_pos<LG,0,LE>(lane,k) of the
+// bit-reversed input (wave_in_index gives its natural index); on exit x[k]
+// is output wave_out_index(lane, k), natural order.
+template <int LG, typename C2>
+__device__ __forceinline__ void wave_fft(C2 (&x)[WavePlan<LG>::E], C2* ex, int lane,
+                                         const C2* tw) {
+  wave_passes<LG, 0>(x, ex, lane, tw);
+}
+
+// natural index loaded into slot k: bitrev(lane*E + k) = bitrev_LE(k)*64 + bitrev6(lane)
+template <int LG>
+__device__ __forceinline__ int wave_in_index(int lane, int k) {
+  constexpr int LE = WavePlan<LG>::LE;
+  return (int)((__brev((unsigned)k) >> (32 - LE)) << 6) | (int)(__brev((unsigned)lane) >> 26);
+}
+
+template <int LG>
+__device__ __forceinline__ int wave_out_index(int lane, int k) {
+  using P = WavePlan<LG>;
+  return wave_pos<LG, P::S0(P::NPASS - 1), P::G(P::NPASS - 1)>(lane, k);
+}
+
+// Output registers (natural order) to the bit-reversed input registers of a
+// second transform, through ex.
+template <int LG, typename C2>
+__device__ __forceinline__ void wave_to_bitrev(C2 (&x)[WavePlan<LG>::E], C2* ex, int lane) {
+  using P = WavePlan<LG>;
+#pragma unroll
+  for (int k = 0; k < P::E; k++)
+    ex[wave_xi(bitrev(wave_out_index<LG>(lane, k), LG))] = x[k];
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < P::E; k++) x[k] = ex[wave_xi(lane * P::E + k)];
+  wave_sync();
+}
+
 // Residual r = (div(C grad P) - RHS) * cc, cc = -1/I_F2C(C)
 // (poisson.cpp:393-403, poisson_IO.cpp:90-97), at interior cell c.  With INV
 // the arrays hold Rho_face and C = 1./Rho_face is formed inline, the value

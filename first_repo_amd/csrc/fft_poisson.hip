@@ -210,6 +210,81 @@ __global__ void __launch_bounds__(FT) k_pass_strided(PoisDev q, int axis, int di
   }
 }
 
+// Passes Y, Y' and Z for pencils of 256 or 512 points: one pencil per
+// 64-lane wave held in registers (wave_fft), no workgroup barrier inside the
+// transforms.  The y pencils are contiguous, so each register slot's loads
+// and stores are 64 consecutive elements; the z pass takes WZ_WAVES
+// consecutive y pencils of one x frequency per workgroup, so its strided
+// rows are WZ_WAVES elements wide.  Bit-identical to k_pass_strided.
+constexpr int WY_WAVES = 4, WZ_WAVES = 8;
+
+template <typename R, int LG>
+__global__ void __launch_bounds__(64 * WY_WAVES) k_pass_y_wave(
+    PoisDev q, const typename Cx<R>::T* __restrict__ tw, typename Cx<R>::T* __restrict__ data,
+    const int* __restrict__ gate) {
+  using C2 = typename Cx<R>::T;
+  using P = WavePlan<LG>;
+  __shared__ C2 tws[P::L / 2 + 1];
+  __shared__ C2 exs[WY_WAVES][P::XS];
+  if (*gate) return;
+  for (int t = threadIdx.x; t <= P::L / 2; t += 64 * WY_WAVES) tws[t] = tw[t];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long pen = (long)blockIdx.x * WY_WAVES + wave;  // (kz, kx) pencil, P::L == ny
+  if (pen >= (long)q.nz * q.nxh) return;
+  C2* p = data + pen * P::L;
+  C2 x[P::E];
+#pragma unroll
+  for (int k = 0; k < P::E; k++) x[k] = p[wave_in_index<LG>(lane, k)];
+  wave_fft<LG>(x, exs[wave], lane, tws);
+#pragma unroll
+  for (int k = 0; k < P::E; k++) p[wave_out_index<LG>(lane, k)] = x[k];
+}
+
+template <typename R, int LG>
+__global__ void __launch_bounds__(64 * WZ_WAVES) k_pass_z_wave(
+    PoisDev q, const typename Cx<R>::T* __restrict__ tw, const typename Cx<R>::T* __restrict__ tw2,
+    const double* __restrict__ lamx, const double* __restrict__ lamy,
+    const double* __restrict__ lamz, typename Cx<R>::T* __restrict__ data,
+    const int* __restrict__ gate) {
+  using C2 = typename Cx<R>::T;
+  using P = WavePlan<LG>;
+  __shared__ C2 twf[P::L / 2 + 1], twi[P::L / 2 + 1];
+  __shared__ C2 exs[WZ_WAVES][P::XS];
+  if (*gate) return;
+  for (int t = threadIdx.x; t <= P::L / 2; t += 64 * WZ_WAVES) {
+    twf[t] = tw[t];
+    twi[t] = tw2[t];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tiles = q.ny / WZ_WAVES;
+  const unsigned bid = xcd_block().x;
+  const int kx = (int)(bid / (unsigned)tiles);
+  const int jy = ((int)bid - kx * tiles) * WZ_WAVES + wave;
+  const long stride = (long)q.nxh * q.ny;  // P::L == nz
+  C2* p = data + (long)kx * q.ny + jy;
+  C2 x[P::E];
+#pragma unroll
+  for (int k = 0; k < P::E; k++) x[k] = p[(long)wave_in_index<LG>(lane, k) * stride];
+  wave_fft<LG>(x, exs[wave], lane, twf);
+  // divide by lambda except the zero mode (poisson.cpp:438-451)
+  const double lxy = lamx[kx] + lamy[jy];
+#pragma unroll
+  for (int k = 0; k < P::E; k++) {
+    const int kz = wave_out_index<LG>(lane, k);
+    if (kx + jy + kz > 0) {
+      const R lam = (R)(lxy + lamz[kz]);
+      x[k].y /= lam;
+      x[k].x /= lam;
+    }
+  }
+  wave_to_bitrev<LG>(x, exs[wave], lane);
+  wave_fft<LG>(x, exs[wave], lane, twi);
+#pragma unroll
+  for (int k = 0; k < P::E; k++) p[(long)wave_out_index<LG>(lane, k) * stride] = x[k];
+}
+
 // Pass X': inverse real FFT along x: Z_k = (X_k + conj X_{M-k}) +
 // i w^k (X_k - conj X_{M-k}), k < M, an nx/2-point inverse FFT, x_2m = Re z_m,
 // x_2m+1 = Im z_m; dp = (-1)*x/N (poisson.cpp:455-461); P += dp at the
@@ -384,12 +459,31 @@ static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const
                           s->tx0[slot], s->tx1[slot], 0, q, P, cx, cy, cz, RHS, twfm, twf[0],
                           data, g);
   const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.ny + q.wz - 1) / q.wz;
-  hipLaunchKernelGGL(k_pass_strided<R>, dim3(ty * q.nz), dim3(FT), lds[1], st, q, 1, 0, twf[1],
-                     twf[1], s->lam[0], s->lam[1], s->lam[2], data, g);
-  hipLaunchKernelGGL(k_pass_strided<R>, dim3(tz * q.nxh), dim3(FT), lds[2], st, q, 2, 1, twf[2],
-                     twi[2], s->lam[0], s->lam[1], s->lam[2], data, g);
-  hipLaunchKernelGGL(k_pass_strided<R>, dim3(ty * q.nz), dim3(FT), lds[1], st, q, 1, 0, twi[1],
-                     twi[1], s->lam[0], s->lam[1], s->lam[2], data, g);
+  // 256- and 512-point y and z transforms one pencil per wave, other lengths
+  // in LDS workgroup tiles
+  const dim3 gyw(nblocks((long)q.nz * q.nxh, WY_WAVES)), byw(64 * WY_WAVES);
+  auto ypass = [&](const C2* tw) {
+    if (q.ly == 9)
+      hipLaunchKernelGGL((k_pass_y_wave<R, 9>), gyw, byw, 0, st, q, tw, data, g);
+    else if (q.ly == 8)
+      hipLaunchKernelGGL((k_pass_y_wave<R, 8>), gyw, byw, 0, st, q, tw, data, g);
+    else
+      hipLaunchKernelGGL(k_pass_strided<R>, dim3(ty * q.nz), dim3(FT), lds[1], st, q, 1, 0, tw,
+                         tw, s->lam[0], s->lam[1], s->lam[2], data, g);
+  };
+  ypass(twf[1]);
+  const bool zw = (q.lz == 9 || q.lz == 8) && q.ny % WZ_WAVES == 0;
+  const dim3 gzw(q.ny / WZ_WAVES * q.nxh), bzw(64 * WZ_WAVES);
+  if (zw && q.lz == 9)
+    hipLaunchKernelGGL((k_pass_z_wave<R, 9>), gzw, bzw, 0, st, q, twf[2], twi[2], s->lam[0],
+                       s->lam[1], s->lam[2], data, g);
+  else if (zw)
+    hipLaunchKernelGGL((k_pass_z_wave<R, 8>), gzw, bzw, 0, st, q, twf[2], twi[2], s->lam[0],
+                       s->lam[1], s->lam[2], data, g);
+  else
+    hipLaunchKernelGGL(k_pass_strided<R>, dim3(tz * q.nxh), dim3(FT), lds[2], st, q, 2, 1,
+                       twf[2], twi[2], s->lam[0], s->lam[1], s->lam[2], data, g);
+  ypass(twi[1]);
   hipLaunchKernelGGL(k_pass_x_inv<R>, dim3(nbx), dim3(FT), lds[0], st, q, data, twim, twi[0], P,
                      s->bmax, g);
   FR_LAUNCH_CHECK();

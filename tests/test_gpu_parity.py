@@ -118,10 +118,13 @@ def test_reductions(geo):
     assert rel(host(out), ref) < 1e-14
 
 
-@pytest.mark.parametrize("n", [(8, 8, 8), (16, 8, 32), (32, 32, 32), (64, 64, 64)], ids=str)
+@pytest.mark.parametrize("n", [(8, 8, 8), (16, 8, 32), (32, 32, 32), (64, 64, 64), (16, 256, 512),
+                               (8, 512, 256)], ids=str)
 def test_box_poisson_matches_oracle(n):
     """fr_box_poisson_solve vs the oracle's poisson::FFT restatement on a
-    variable-coefficient problem (density ratio ~2)."""
+    variable-coefficient problem (density ratio ~2).  The 256- and 512-point
+    y and z transforms run one pencil per wave (the last two shapes), the
+    others in LDS workgroup tiles."""
     import ctypes as C
     L = fr.lib()
     g = fr.make_geom(*n, L=(2 * np.pi, 1.0, 3.0))
