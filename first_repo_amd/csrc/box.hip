@@ -24,20 +24,29 @@ namespace fr {
 static dim3 grid3(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 4), d.Nz); }
 
 // grid::Update_Rho (grid.cpp:327-333) fused with U = RU_int / Rho_face
-// (grid.cpp:662, the first line of grid::Update_Particle).
-template <typename T>
+// (grid.cpp:662, the first line of grid::Update_Particle).  SET (last stage
+// of a full step only): the end-of-step copy Rho = Rho_np1 of
+// grid::TimeAdvance (grid.cpp:317) is made here, into rho_set; the last
+// stage does not read Rho.
+template <typename T, bool SET>
 __global__ void __launch_bounds__(256) k_update_rho(
     Dev d, BoxCoef cf, const T* __restrict__ rux, const T* __restrict__ ruy,
     const T* __restrict__ ruz, const T* __restrict__ rho, T* __restrict__ rho_np1,
     T* __restrict__ rho_new, const T* __restrict__ rfx, const T* __restrict__ rfy,
-    const T* __restrict__ rfz, T* __restrict__ ux, T* __restrict__ uy, T* __restrict__ uz) {
+    const T* __restrict__ rfz, T* __restrict__ ux, T* __restrict__ uy, T* __restrict__ uz,
+    T* __restrict__ rho_set) {
   FR_CELL_PROLOGUE
   // RHS_Rho = Div_F2C(RU_int)
   const T rhs = (rux[c + sx] - rux[c]) * (T)d.idx + (ruy[c + sy] - ruy[c]) * (T)d.idy +
                 (ruz[c + sz] - ruz[c]) * (T)d.idz;
   const T np1 = rho_np1[n] + (T)cf.cpost_neg * rhs;
   rho_np1[n] = np1;
-  rho_new[n] = cf.last ? np1 : 1 * rho[n] + (T)cf.cpre_neg * rhs;
+  if constexpr (SET) {
+    rho_new[n] = np1;
+    rho_set[n] = np1;
+  } else {
+    rho_new[n] = cf.last ? np1 : 1 * rho[n] + (T)cf.cpre_neg * rhs;
+  }
   ux[n] = rux[n] / rfx[n];
   uy[n] = ruy[n] / rfy[n];
   uz[n] = ruz[n] / rfz[n];
@@ -312,9 +321,17 @@ __global__ void __launch_bounds__(256) k_update_ru_wp(
 
 template <typename T>
 int box_update_rho(const Dev& d, const BoxCoef& cf, const T* const ru_int[3], const T* rho,
-                   T* rho_np1, T* rho_new, const T* const rf[3], T* const u[3], hipStream_t s) {
-  hipLaunchKernelGGL(k_update_rho<T>, grid3(d), dim3(64, 4), 0, s, d, cf, ru_int[0], ru_int[1],
-                     ru_int[2], rho, rho_np1, rho_new, rf[0], rf[1], rf[2], u[0], u[1], u[2]);
+                   T* rho_np1, T* rho_new, const T* const rf[3], T* const u[3], hipStream_t s,
+                   T* rho_set) {
+  if (rho_set && !cf.last) return FR_EINVAL;
+  if (rho_set)
+    hipLaunchKernelGGL((k_update_rho<T, true>), grid3(d), dim3(64, 4), 0, s, d, cf, ru_int[0],
+                       ru_int[1], ru_int[2], nullptr, rho_np1, rho_new, rf[0], rf[1], rf[2],
+                       u[0], u[1], u[2], rho_set);
+  else
+    hipLaunchKernelGGL((k_update_rho<T, false>), grid3(d), dim3(64, 4), 0, s, d, cf, ru_int[0],
+                       ru_int[1], ru_int[2], rho, rho_np1, rho_new, rf[0], rf[1], rf[2], u[0],
+                       u[1], u[2], nullptr);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
@@ -361,7 +378,7 @@ int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, T* const 
 
 #define FR_BOX_INST(T)                                                                        \
   template int box_update_rho<T>(const Dev&, const BoxCoef&, const T* const[3], const T*, T*, \
-                                 T*, const T* const[3], T* const[3], hipStream_t);            \
+                                 T*, const T* const[3], T* const[3], hipStream_t, T*);        \
   template int box_update_ru_wop<T>(const Dev&, const BoxCoef&, const WopCoef&,               \
                                     const T* const[3], const T* const[3], const T* const[3],  \
                                     const T* const[3], T* const[3], T* const[3],              \

@@ -420,6 +420,34 @@ template int make_mean_zero_async<double>(const Dev&, double*, long, double*, hi
 template int make_mean_zero_async<float>(const Dev&, float*, long, double*, hipStream_t);
 
 template <typename T>
+__global__ void k_add_neg_mean_set(long n, T* __restrict__ f, T* __restrict__ g,
+                                   const double* __restrict__ dsum, double denom, int has_c,
+                                   double c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const T a = (T)(-(dsum[0] / denom));
+  T v = f[t] + a;
+  if (has_c) v = v + (T)c;
+  f[t] = v;
+  g[t] = v;
+}
+
+template <typename T>
+int make_mean_zero_set_async(const Dev& d, T* f, T* g, long n, double* ws, bool has_c, double c,
+                             hipStream_t s) {
+  FR_RET(reduce_async_ws(d, (const T*)f, 0, ws + RED_PARTS, ws, s));
+  const double denom = (double)(d.Nx - 2) * (d.Ny - 2) * (d.Nz - 2);
+  hipLaunchKernelGGL(k_add_neg_mean_set<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, n, f, g,
+                     (const double*)(ws + RED_PARTS), denom, has_c ? 1 : 0, c);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+template int make_mean_zero_set_async<double>(const Dev&, double*, double*, long, double*, bool,
+                                              double, hipStream_t);
+template int make_mean_zero_set_async<float>(const Dev&, float*, float*, long, double*, bool,
+                                             double, hipStream_t);
+
+template <typename T>
 int reduce_sync(const Dev& d, const T* f, int op, double* result, hipStream_t s) {
   int rc = red_ws();
   if (rc) return rc;

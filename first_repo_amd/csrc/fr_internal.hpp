@@ -29,6 +29,11 @@ int reduce_async_ws(const Dev& d, const T* f, int op, double* dres, double* ws, 
 // RED_PARTS + 1 doubles); the same values as reduce_sync + elementwise add
 template <typename T>
 int make_mean_zero_async(const Dev& d, T* f, long n, double* ws, hipStream_t s);
+// The same followed by f += c when has_c, with the result also written to g
+// (one pass: the element values are those of the separate operations)
+template <typename T>
+int make_mean_zero_set_async(const Dev& d, T* f, T* g, long n, double* ws, bool has_c, double c,
+                             hipStream_t s);
 int reducex(const Dev& d, const double* f, double* out, hipStream_t s);
 // Device address, on the current device, of the file-scope scratch array of
 // the allocation-free reduction (RED_PARTS + 8 doubles); looked up once per
@@ -102,7 +107,8 @@ struct DivCoef {
 
 template <typename T>
 int box_update_rho(const Dev& d, const BoxCoef& cf, const T* const ru_int[3], const T* rho,
-                   T* rho_np1, T* rho_new, const T* const rf[3], T* const u[3], hipStream_t s);
+                   T* rho_np1, T* rho_new, const T* const rf[3], T* const u[3], hipStream_t s,
+                   T* rho_set = nullptr);
 template <typename T>
 int box_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w, const T* const u[3],
                       const T* const ru_int[3], const T* const S[3], const T* const ru[3],
@@ -216,6 +222,9 @@ struct AdvCoef {
   double dtpost_iS, dtpre_iS;  // dt*post*iS, dt*pre*iS
   double alpha;
   int last;
+  // last stage of a full step: the end-of-step copy of stage 3 into stage 0
+  // (grid::TimeAdvance, particle.cpp:1027-1040) made by the stage itself
+  int set0 = 0;
 };
 
 // HIP events bound to the launches of one kernel (hipExtLaunchKernelGGL

@@ -96,6 +96,7 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
       const double np1 = p.s[a][3][i] + c.dt_post * X[a];
       p.s[a][3][i] = np1;
       p.s[a][2][i] = c.last ? np1 : p.s[a][0][i] + c.dt_pre * X[a];
+      if (c.set0) p.s[a][0][i] = np1;
     }
   }
   if (PH & PH_VEL) {
@@ -119,6 +120,7 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
       const double np1 = p.s[3 + a][3][i] + c.dt_post * (R[a] + G[a]);
       p.s[3 + a][3][i] = np1;
       p.s[3 + a][2][i] = c.last ? np1 : p.s[3 + a][0][i] + c.dt_pre * (R[a] + G[a]);
+      if (c.set0) p.s[3 + a][0][i] = np1;
     }
   }
   if (PH & PH_TEMP) {  // RHS_T from the previous stage
@@ -126,6 +128,7 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
     const double tn = p.s[6][3][i] + c.dtpost_iS * (rt + c.alpha);
     p.s[6][3][i] = tn;
     p.s[6][2][i] = c.last ? tn : p.s[6][0][i] + c.dtpre_iS * (rt + c.alpha);
+    if (c.set0) p.s[6][0][i] = tn;
   }
 }
 

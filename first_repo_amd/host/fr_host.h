@@ -326,8 +326,9 @@ class particle {
   int init(params* p, gridsize* s, hipStream_t st, int capacity);
   ~particle();
   int load_random();  // particle.cpp:919-958
-  // particle.cpp:340-399 + 314-338 + 480-496 (gather and the three ODE updates)
-  int advance(int k, double pre, double post, tensor1& U);
+  // particle.cpp:340-399 + 314-338 + 480-496 (gather and the three ODE updates);
+  // set0 (last stage of a full step): stage 0 takes the stage-3 values too
+  int advance(int k, double pre, double post, tensor1& U, bool set0 = false);
   int part2gas_concentration(tensor0& A, hipStream_t on = 0);  // particle.cpp:526-559
   // rk: reference indices to order the deposit by (a copy taken before a
   // migration that runs beside it); default the current ones
@@ -504,6 +505,13 @@ class grid {
   // step, Update_RU_WOP, reads RHS_RU at interior faces only.  Called on its
   // own, Update_Particle does both as the reference does.
   bool step_front = false;
+  // Inside a full time step (begin_step .. end_step) of the box, the last
+  // stage's kernels also make grid::TimeAdvance's copies (Rho = Rho_np1 in
+  // the density update, RU = RU_np1 in Correct_mean_RU's mean removal,
+  // particle stage 0 = stage 3 in the advance), each in the pass that
+  // writes the source values; the flags tell TimeAdvance which are made.
+  bool in_step = false;
+  bool rho_set = false, ru_set = false, part_set = false;
   hipEvent_t evc = nullptr;
   int launch_side();
   double* dq = nullptr;  // reduction workspace + result of that branch

@@ -307,7 +307,7 @@ int particle::load_random() {
   return FR_OK;
 }
 
-int particle::advance(int k, double pre, double post, tensor1& U) {
+int particle::advance(int k, double pre, double post, tensor1& U, bool set0) {
   const params& p = *param_;
   AdvCoef c;
   const double dt = p.dt();
@@ -325,6 +325,7 @@ int particle::advance(int k, double pre, double post, tensor1& U) {
   c.dtpre_iS = dt * pre * iS;
   c.alpha = 0.25 * p.epsilonp() * PI * p.Dp() * p.Dp() * p.I0();
   c.last = (k == 3);
+  c.set0 = set0 && c.last;
   if (U.x.is_f32()) {
     const float* u[3] = {U.x.p<float>(), U.y.p<float>(), U.z.p<float>()};
     return dev.advance(size_->dev(), c, u, st_);
@@ -657,16 +658,19 @@ int grid::Update_Rho() {
                            inflow, inflow + m, st));
     return FR_OK;
   }
+  rho_set = in_step && RK4_count == 3;
   if (f32) {
     const float* ri_[3] = {RU_int.x.p<float>(), RU_int.y.p<float>(), RU_int.z.p<float>()};
     const float* rf_[3] = {Rho_face.x.p<float>(), Rho_face.y.p<float>(), Rho_face.z.p<float>()};
     float* u_[3] = {U.x.p<float>(), U.y.p<float>(), U.z.p<float>()};
     FR_TRY(box_update_rho<float>(size.dev(), box_coef(*this), ri_, Rho.p<float>(),
-                                 Rho_np1.p<float>(), Rho_new.p<float>(), rf_, u_, st));
+                                 Rho_np1.p<float>(), Rho_new.p<float>(), rf_, u_, st,
+                                 rho_set ? Rho.p<float>() : nullptr));
     return FR_OK;
   }
   FR_TRY(box_update_rho<double>(size.dev(), box_coef(*this), ri, Rho.pointer(),
-                                Rho_np1.pointer(), Rho_new.pointer(), rf, u, st));
+                                Rho_np1.pointer(), Rho_new.pointer(), rf, u, st,
+                                rho_set ? Rho.pointer() : nullptr));
   return FR_OK;
 }
 
@@ -700,7 +704,11 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   // kept in the particles' ranks, so no result depends on it)
   if (RK4_count == 0 && num_timestep % resort_every == 0)
     FR_TRY(part.dev.resort(size.dev(), st));
-  FR_TRY(part.advance(RK4_count, RK4_preCoeff[RK4_count], RK4_postCoeff[RK4_count], U));
+  // the coupled box's '>hi' rows carry stage 0 as the reference has it before
+  // TimeAdvance, so the copy stays in TimeAdvance there
+  part_set = in_step && param.IsBox() && !pair_out && RK4_count == 3;
+  FR_TRY(part.advance(RK4_count, RK4_preCoeff[RK4_count], RK4_postCoeff[RK4_count], U,
+                      part_set));
   // The concentration deposit (rounded-index buckets, output C) and the
   // momentum deposit (truncating-index buckets, output RHS_RU) read the same
   // particles and write disjoint data: both are latency-bound, so they run
@@ -989,7 +997,21 @@ int grid::TimeAdvance_RK4() {
 }
 
 int grid::Correct_mean_RU() {  // grid.cpp:565-577
-  if (param.IsBox()) {
+  if (param.IsBox() && in_step) {
+    // with RU = RU_np1 of TimeAdvance in the same pass
+    const Dev d = size.dev();
+    const long n = size.size();
+    for (int m = 0; m < 3; m++) {
+      tensor0 &f = RU_np1.c(m), &g = RU.c(m);
+      const double add = param.Rho0() * param.U0();
+      if (f.is_f32())
+        FR_TRY(make_mean_zero_set_async(d, f.p<float>(), g.p<float>(), n, dmz, m == 0, add, st));
+      else
+        FR_TRY(make_mean_zero_set_async(d, f.p<double>(), g.p<double>(), n, dmz, m == 0, add,
+                                        st));
+    }
+    ru_set = true;
+  } else if (param.IsBox()) {
     RU_np1.make_mean_zero_async(dmz);
     RU_np1.x.add(param.Rho0() * param.U0());
   } else {
@@ -1000,10 +1022,11 @@ int grid::Correct_mean_RU() {  // grid.cpp:565-577
 }
 
 int grid::TimeAdvance() {  // grid.cpp:315-325
-  Rho.copy(Rho_np1);
-  RU.copy(RU_np1);
+  if (!rho_set) Rho.copy(Rho_np1);
+  if (!ru_set) RU.copy(RU_np1);
   P0 = P0_np1;
-  FR_TRY(part.dev.copy_stage(0, 3, st));
+  if (!part_set) FR_TRY(part.dev.copy_stage(0, 3, st));
+  rho_set = ru_set = part_set = false;
   T_cur += param.dt();
   num_timestep++;
   return FR_OK;
@@ -1011,6 +1034,8 @@ int grid::TimeAdvance() {  // grid.cpp:315-325
 
 int grid::begin_step() {
   param.update(T_cur);
+  in_step = true;
+  rho_set = ru_set = part_set = false;
   step_front = false;
   t_poisson = t_particles = 0;
   iters_step = 0;
@@ -1055,6 +1080,7 @@ int grid::step() {  // simulation.cpp:36-70 (box calls)
 }
 
 int grid::end_step() {
+  in_step = false;
   FR_CHECK(hipStreamSynchronize(st));
   FR_TRY(settle_side());
   t_step = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_step)
@@ -1201,6 +1227,7 @@ int fr_sim_call(fr_sim* s, const char* m, int k) {
   grid& g = s->g;
   std::string n(m);
   g.RK4_count = k;
+  g.in_step = false;  // methods one by one: every copy where the reference makes it
   int rc;
   if (n == "update") {
     g.param.update(g.T_cur);
