@@ -251,18 +251,19 @@ __global__ void __launch_bounds__(256) k_update_ru_wop_z(
   }
 }
 
-// grid::Compute_Div_U_new (grid.cpp:556-563) + grid::Compute_RHS_Pois
-// (grid.cpp:629-637) up to the zero-mean shift: writes Rho_face = I_C2F(
-// Rho_new) and RHS_Pois = (Div_F2C(RU_new/Rho_face) - divergence)/(dt*pre).
+// One cell of k_div_rhs (any ghosted cell: a ghost takes its image's
+// values); returns the RHS_Pois value written.
 template <typename T>
-__global__ void __launch_bounds__(256) k_div_rhs(Dev d, DivCoef q, const T* __restrict__ rn,
-                                                 const T* __restrict__ Q,
-                                                 const T* __restrict__ rnx,
-                                                 const T* __restrict__ rny,
-                                                 const T* __restrict__ rnz, T* __restrict__ rfx,
-                                                 T* __restrict__ rfy, T* __restrict__ rfz,
-                                                 double* __restrict__ rhs) {
-  FR_CELL_PROLOGUE
+__device__ __forceinline__ double div_rhs_cell(const Dev& d, const DivCoef& q,
+                                               const T* __restrict__ rn, const T* __restrict__ Q,
+                                               const T* __restrict__ rnx,
+                                               const T* __restrict__ rny,
+                                               const T* __restrict__ rnz, T* __restrict__ rfx,
+                                               T* __restrict__ rfy, T* __restrict__ rfz,
+                                               double* __restrict__ rhs, int i, int j, int k) {
+  const long n = at(d, i, j, k);
+  const long c = at(d, img(i, d.Nx), img(j, d.Ny), img(k, d.Nz));
+  const long sx = 1, sy = d.Nx, sz = d.NxNy;
   // The right-hand side is formed in double from fields of either precision:
   // it is the small difference of large face fluxes, and the pressure solve
   // iterates to 1e-10 on it.
@@ -288,7 +289,41 @@ __global__ void __launch_bounds__(256) k_div_rhs(Dev d, DivCoef q, const T* __re
   double r = D - dvg;
   r = r * q.s;
   rhs[n] = r;
+  return r;
 }
+
+// grid::Compute_Div_U_new (grid.cpp:556-563) + grid::Compute_RHS_Pois
+// (grid.cpp:629-637) up to the zero-mean shift: writes Rho_face = I_C2F(
+// Rho_new) and RHS_Pois = (Div_F2C(RU_new/Rho_face) - divergence)/(dt*pre).
+// With part: the workgroup's sum of RHS_Pois over its interior cells (the
+// first level of the mean's reduction, in a fixed order) into part[tile].
+template <typename T>
+__global__ void __launch_bounds__(256) k_div_rhs(Dev d, DivCoef q, const T* __restrict__ rn,
+                                                 const T* __restrict__ Q,
+                                                 const T* __restrict__ rnx,
+                                                 const T* __restrict__ rny,
+                                                 const T* __restrict__ rnz, T* __restrict__ rfx,
+                                                 T* __restrict__ rfy, T* __restrict__ rfz,
+                                                 double* __restrict__ rhs,
+                                                 double* __restrict__ part) {
+  const uint3 bk = xcd_block();
+  const int i = bk.x * 64 + threadIdx.x, j = bk.y * 4 + threadIdx.y, k = bk.z;
+  double r = 0.0;
+  if (i < d.Nx && j < d.Ny)
+    r = div_rhs_cell(d, q, rn, Q, rnx, rny, rnz, rfx, rfy, rfz, rhs, i, j, k);
+  if (!part) return;
+  const bool in = i >= 1 && i <= d.Nx - 2 && j >= 1 && j <= d.Ny - 2 && k >= 1 && k <= d.Nz - 2;
+  double v = in ? r : 0.0;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __shared__ double wsum[4];
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  if ((tid & 63) == 0) wsum[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0)
+    part[((long)bk.z * gridDim.y + bk.y) * gridDim.x + bk.x] =
+        (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
 
 // grid::Update_RU_WP (grid.cpp:651-657)
 // The pressure gradient is formed from the FP64 pressure in double.
@@ -360,11 +395,19 @@ int box_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w, const T
 
 template <typename T>
 int box_div_rhs(const Dev& d, const DivCoef& q, const T* rho_new, const T* Q,
-                const T* const ru_new[3], T* const rf[3], double* rhs, hipStream_t s) {
-  hipLaunchKernelGGL(k_div_rhs<T>, grid3(d), dim3(64, 4), 0, s, d, q, rho_new, Q, ru_new[0],
-                     ru_new[1], ru_new[2], rf[0], rf[1], rf[2], rhs);
+                const T* const ru_new[3], T* const rf[3], double* rhs, hipStream_t s,
+                double* part, double* dsum) {
+  const dim3 g = grid3(d);
+  hipLaunchKernelGGL(k_div_rhs<T>, g, dim3(64, 4), 0, s, d, q, rho_new, Q, ru_new[0],
+                     ru_new[1], ru_new[2], rf[0], rf[1], rf[2], rhs, part);
   FR_LAUNCH_CHECK();
+  if (part && dsum) return sum_parts(part, (long)g.x * g.y * g.z, dsum, s);
   return FR_OK;
+}
+
+long box_div_parts(const Dev& d) {
+  const dim3 g = grid3(d);
+  return (long)g.x * g.y * g.z;
 }
 
 template <typename T>
@@ -384,7 +427,8 @@ int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, T* const 
                                     const T* const[3], T* const[3], T* const[3],              \
                                     const T* const[3], hipStream_t);                          \
   template int box_div_rhs<T>(const Dev&, const DivCoef&, const T*, const T*,                 \
-                              const T* const[3], T* const[3], double*, hipStream_t);          \
+                              const T* const[3], T* const[3], double*, hipStream_t, double*,  \
+                              double*);                                                       \
   template int box_update_ru_wp<T>(const Dev&, const BoxCoef&, const double*, T* const[3],    \
                                    T* const[3], hipStream_t);
 FR_BOX_INST(double)

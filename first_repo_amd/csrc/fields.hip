@@ -419,6 +419,47 @@ int make_mean_zero_async(const Dev& d, T* f, long n, double* ws, hipStream_t s) 
 template int make_mean_zero_async<double>(const Dev&, double*, long, double*, hipStream_t);
 template int make_mean_zero_async<float>(const Dev&, float*, long, double*, hipStream_t);
 
+constexpr int ST_SUM = 1024;
+__global__ void __launch_bounds__(ST_SUM) k_sum_parts(const double* __restrict__ part, long n,
+                                                      double* __restrict__ out) {
+  __shared__ double ws[ST_SUM / 64];
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long t = threadIdx.x;
+  for (; t + 7 * ST_SUM < n; t += 8 * ST_SUM) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = part[t + q * ST_SUM];
+#pragma unroll
+    for (int q = 0; q < 8; q++) a[q] += v[q];
+  }
+  for (; t < n; t += ST_SUM) a[0] += part[t];
+  double r = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  r = threadIdx.x < ST_SUM / 64 ? ws[threadIdx.x] : 0.0;
+  for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o);
+  if (threadIdx.x == 0) out[0] = r;
+}
+
+int sum_parts(const double* part, long n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(ST_SUM), 0, s, part, n, out);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+template <typename T>
+int add_neg_mean_async(const Dev& d, T* f, long n, const double* dsum, hipStream_t s) {
+  const double denom = (double)(d.Nx - 2) * (d.Ny - 2) * (d.Nz - 2);
+  hipLaunchKernelGGL(k_add_neg_mean<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, n, f, dsum,
+                     denom);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+template int add_neg_mean_async<double>(const Dev&, double*, long, const double*, hipStream_t);
+template int add_neg_mean_async<float>(const Dev&, float*, long, const double*, hipStream_t);
+
 template <typename T>
 __global__ void k_add_neg_mean_set(long n, T* __restrict__ f, T* __restrict__ g,
                                    const double* __restrict__ dsum, double denom, int has_c,

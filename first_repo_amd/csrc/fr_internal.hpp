@@ -34,6 +34,12 @@ int make_mean_zero_async(const Dev& d, T* f, long n, double* ws, hipStream_t s);
 template <typename T>
 int make_mean_zero_set_async(const Dev& d, T* f, T* g, long n, double* ws, bool has_c, double c,
                              hipStream_t s);
+// out[0] = the sum of part[0..n) in a fixed order (one workgroup)
+int sum_parts(const double* part, long n, double* out, hipStream_t s);
+// f[t] += (T)(-(dsum[0]/interior cells)) for the n elements of f: the shift
+// of make_mean_zero_async with the sum already formed
+template <typename T>
+int add_neg_mean_async(const Dev& d, T* f, long n, const double* dsum, hipStream_t s);
 int reducex(const Dev& d, const double* f, double* out, hipStream_t s);
 // Device address, on the current device, of the file-scope scratch array of
 // the allocation-free reduction (RED_PARTS + 8 doubles); looked up once per
@@ -116,7 +122,12 @@ int box_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w, const T
                       hipStream_t s);
 template <typename T>
 int box_div_rhs(const Dev& d, const DivCoef& q, const T* rho_new, const T* Q,
-                const T* const ru_new[3], T* const rf[3], double* rhs, hipStream_t s);
+                const T* const ru_new[3], T* const rf[3], double* rhs, hipStream_t s,
+                double* part = nullptr, double* dsum = nullptr);
+// With part (box_div_parts(d) doubles) and dsum: the interior sum of rhs is
+// formed in the same pass (per-workgroup sums, then one fixed-order
+// reduction into dsum[0]).
+long box_div_parts(const Dev& d);
 template <typename T>
 int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, T* const ru_np1[3],
                      T* const ru_new[3], hipStream_t s);

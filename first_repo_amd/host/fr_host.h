@@ -516,6 +516,7 @@ class grid {
   int launch_side();
   double* dq = nullptr;  // reduction workspace + result of that branch
   double* dmz = nullptr;  // workspace of the main stream's device-side means
+  double* dpart = nullptr;  // box: per-workgroup sums of RHS_Pois (k_div_rhs)
   // join_side() waits for the branch on the device and copies its sum to
   // pinned host memory (hq) behind an event; settle_side() also takes the
   // mean on the host, waiting for that copy only if it has not landed

@@ -468,6 +468,7 @@ grid::~grid() {
   if (evq) hipEventDestroy(evq);
   if (dq) hipFree(dq);
   if (dmz) hipFree(dmz);
+  if (dpart) hipFree(dpart);
   if (hq) hipHostFree(hq);
   if (st2) hipStreamDestroy(st2);
   if (st) hipStreamDestroy(st);
@@ -526,6 +527,7 @@ int grid::setup(const char* params_path, const char* ic) {
   FR_CHECK(hipMalloc(&dred, sizeof(double) * (4 * (size_t)size.Nx() + 512)));
   FR_CHECK(hipMalloc(&dq, sizeof(double) * (RED_PARTS + 8)));
   FR_CHECK(hipMalloc(&dmz, sizeof(double) * (RED_PARTS + 8)));
+  if (param.IsBox()) FR_CHECK(hipMalloc(&dpart, sizeof(double) * box_div_parts(size.dev())));
   FR_CHECK(hipHostMalloc(&hq, sizeof(double) * 2, hipHostMallocDefault));
   return FR_OK;
 }
@@ -939,15 +941,19 @@ int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
     const float* rn_[3] = {RU_new.x.p<float>(), RU_new.y.p<float>(), RU_new.z.p<float>()};
     float* rf_[3] = {Rho_face.x.p<float>(), Rho_face.y.p<float>(), Rho_face.z.p<float>()};
     FR_TRY(box_div_rhs<float>(size.dev(), q, Rho_new.p<float>(), RHS_Part_Temp.p<float>(), rn_,
-                              rf_, RHS_Pois.pointer(), st));
+                              rf_, RHS_Pois.pointer(), st, dpart, dmz + RED_PARTS));
   } else if (param.IsBox()) {
     FR_TRY(box_div_rhs<double>(size.dev(), q, Rho_new.pointer(), RHS_Part_Temp.pointer(), rn,
-                               rf, RHS_Pois.pointer(), st));
+                               rf, RHS_Pois.pointer(), st, dpart, dmz + RED_PARTS));
   } else {
     const double* rfc[3] = {rf[0], rf[1], rf[2]};
     FR_TRY(duct_rhs(size.dev(), q.s, rn, rfc, divergence.pointer(), RHS_Pois.pointer(), st));
+    RHS_Pois.make_mean_zero_async(dmz);
+    return FR_OK;
   }
-  RHS_Pois.make_mean_zero_async(dmz);
+  // box: the interior sum was formed by the kernel (tensor0::make_mean_zero)
+  FR_TRY(add_neg_mean_async(size.dev(), RHS_Pois.pointer(), (long)size.size(), dmz + RED_PARTS,
+                            st));
   return FR_OK;
 }
 
