@@ -297,6 +297,10 @@ struct Buckets {
   // share no scratch
   int* scan_tmp = nullptr;
   double* s[6] = {};
+  // the same operands as one record per bucket position (split-form
+  // deposits), written instead of s[] when rec_form is set at the build
+  double* rec = nullptr;
+  bool rec_form = false;
   bool valid = false;
 };
 
@@ -329,11 +333,13 @@ class PartDev {
   double* inj_buf = nullptr;
   int inj_cap = 0;
   bool flg_zero = false;  // the migration flags flg[0..cap] are all zero
-  // deposit kernels: 0 chosen by the mean particles per cell, 1 the pull
-  // form (one thread per target), 2 the tiled LDS form, 3 pull for the
-  // momentum deposit and tiled for the trilinear ones (FR_DEPOSIT=pull|tile|mixed)
+  // deposit kernels: 0 chosen by the mean particles per cell (split form for
+  // sparse sets, tiled for dense), 1 the pull form (one thread per target),
+  // 2 the tiled LDS form, 3 pull for the momentum deposit and tiled for the
+  // trilinear ones, 4 the split form (FR_DEPOSIT=pull|tile|mixed|split)
   int dep_form = 0;
-  bool use_pull(bool trilinear) const;
+  static constexpr int DEP_TILE = 0, DEP_PULL = 1, DEP_SPLIT = 2;
+  int dep_kind(bool trilinear) const;
   // launch timers of the momentum and trilinear deposit kernels
   KTimer t_vel, t_cic;
 

@@ -524,6 +524,31 @@ __global__ void k_bucket_gather(int np, int mode, PartPtrs p, int st,
   }
 }
 
+// The same operands as one record per bucket position (the split-form
+// deposits): mode 0 (x_l, y_l, z_l, RHS_T), mode 1 (x_l, RHS_u, y_l, RHS_v,
+// z_l, RHS_w); same arithmetic as k_bucket_gather.
+__global__ void k_bucket_gather_rec(int np, int mode, PartPtrs p, int st,
+                                    const int* __restrict__ posv, double idx, double idy,
+                                    double idz, double* __restrict__ rec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const long pos = posv[i];
+  const double X = p.s[0][st][i], Y = p.s[1][st][i], Z = p.s[2][st][i];
+  if (mode == 0) {
+    const int ii = (int)(round(X * idx));
+    const int jj = (int)(round(Y * idy));
+    const int kk = (int)(round(Z * idz));
+    double2* r = reinterpret_cast<double2*>(rec + 4 * pos);
+    r[0] = make_double2(X * idx - ii + 0.5, Y * idy - jj + 0.5);
+    r[1] = make_double2(Z * idz - kk + 0.5, p.rhs_T[i]);
+  } else {
+    double2* r = reinterpret_cast<double2*>(rec + 6 * pos);
+    r[0] = make_double2(X * idx - ((int)(X * idx)), p.rhs_u[i]);
+    r[1] = make_double2(Y * idy - ((int)(Y * idy)), p.rhs_v[i]);
+    r[2] = make_double2(Z * idz - ((int)(Z * idz)), p.rhs_w[i]);
+  }
+}
+
 // ---- deposits -----------------------------------------------------------------
 // Target tiles of DTX x DTY x 1 cells.  The bucket starts of the source cells
 // a tile reads (DTX+1 cells and their end, DTY+1 rows, 2 planes) are staged in
@@ -1109,20 +1134,14 @@ __device__ __forceinline__ void merge2_step(Merge2& m, const int* __restrict__ l
 // collects (1-l_M) R_M from the particles of bucket t and l_M R_M from those
 // of bucket t - e_M (truncating-index buckets).  Every ghosted cell is
 // written.
+// The three components of momentum target n = (a, b, c): two-way merges
+// (own bucket, bucket below along the component), advanced together.
 template <typename T>
-__global__ void __launch_bounds__(256) k_dep_vel_pull(Dev d, const int* __restrict__ start,
-                                                      const int* __restrict__ list,
-                                                      const double* __restrict__ sxl,
-                                                      const double* __restrict__ syl,
-                                                      const double* __restrict__ szl,
-                                                      const double* __restrict__ sru,
-                                                      const double* __restrict__ srv,
-                                                      const double* __restrict__ srw,
-                                                      T* __restrict__ ox, T* __restrict__ oy,
-                                                      T* __restrict__ oz) {
-  const uint3 bk = xcd_block();
-  const int a = bk.x * DTX + threadIdx.x, b = bk.y * DTY + threadIdx.y, c = bk.z;
-  if (a >= d.Nx || b >= d.Ny) return;
+__device__ __forceinline__ void dep_vel_target(const Dev& d, const int* __restrict__ start,
+                                               const int* __restrict__ list,
+                                               const double* const L[3],
+                                               const double* const R[3], T* const o[3], int a,
+                                               int b, int c) {
   const int n = (c * d.Ny + b) * d.Nx + a;
   const int s0 = start[n], e0 = start[n + 1];
   Merge2 m[3];
@@ -1146,8 +1165,6 @@ __global__ void __launch_bounds__(256) k_dep_vel_pull(Dev d, const int* __restri
   }
 #pragma unroll
   for (int M = 0; M < 3; M++) merge2_start(m[M], list);
-  const double* L[3] = {sxl, syl, szl};
-  const double* R[3] = {sru, srv, srw};
   for (;;) {
     bool any = false;
 #pragma unroll
@@ -1159,18 +1176,100 @@ __global__ void __launch_bounds__(256) k_dep_vel_pull(Dev d, const int* __restri
     }
     if (!any) break;
   }
-  ox[n] = (T)m[0].acc;
-  oy[n] = (T)m[1].acc;
-  oz[n] = (T)m[2].acc;
+#pragma unroll
+  for (int M = 0; M < 3; M++) o[M][n] = (T)m[M].acc;
+}
+
+// particle.cpp:498-524 (before the cumulative halo): target t of component M
+// collects (1-l_M) R_M from the particles of bucket t and l_M R_M from those
+// of bucket t - e_M (truncating-index buckets).  Every ghosted cell is
+// written.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dep_vel_pull(Dev d, const int* __restrict__ start,
+                                                      const int* __restrict__ list,
+                                                      const double* __restrict__ sxl,
+                                                      const double* __restrict__ syl,
+                                                      const double* __restrict__ szl,
+                                                      const double* __restrict__ sru,
+                                                      const double* __restrict__ srv,
+                                                      const double* __restrict__ srw,
+                                                      T* __restrict__ ox, T* __restrict__ oy,
+                                                      T* __restrict__ oz) {
+  const uint3 bk = xcd_block();
+  const int a = bk.x * DTX + threadIdx.x, b = bk.y * DTY + threadIdx.y, c = bk.z;
+  if (a >= d.Nx || b >= d.Ny) return;
+  const double* L[3] = {sxl, syl, szl};
+  const double* R[3] = {sru, srv, srw};
+  T* o[3] = {ox, oy, oz};
+  dep_vel_target(d, start, list, L, R, o, a, b, c);
+}
+
+// The eight bucket ranges of trilinear target (a, b, c): range 2*rq + ox is
+// source cell (a+ox, b+(rq&1), c+(rq>>1)).  Buckets outside the grid are
+// empty: columns clamp to the row end (column Nx is the start of the next
+// row), rows and planes past the grid get no range.
+__device__ __forceinline__ void cic_ranges(const Dev& d, const int* __restrict__ start, int a,
+                                           int b, int c, int* cur, int* end) {
+  const int a1 = min(a + 1, d.Nx), a2 = min(a + 2, d.Nx);
+#pragma unroll
+  for (int rq = 0; rq < 4; rq++) {
+    const int B = b + (rq & 1), Cz = c + (rq >> 1);
+    int v0 = 0, v1 = 0, v2 = 0;
+    if (B < d.Ny && Cz < d.Nz) {
+      const int rb = (Cz * d.Ny + B) * d.Nx;
+      v0 = start[rb + a];
+      v1 = start[rb + a1];
+      v2 = start[rb + a2];
+    }
+    cur[2 * rq] = v0;
+    end[2 * rq] = v1;
+    cur[2 * rq + 1] = v1;
+    end[2 * rq + 1] = v2;
+  }
+}
+
+// Weight of the particle at bucket position pos for the target it reaches
+// through range q (particle.cpp:536-556 / 597-655).
+__device__ __forceinline__ double cic_weight(int q, int pos, const double* __restrict__ sxl,
+                                             const double* __restrict__ syl,
+                                             const double* __restrict__ szl,
+                                             const double* __restrict__ sw) {
+  const double x_l = sxl[pos], y_l = syl[pos], z_l = szl[pos];
+  const double wx = (q & 1) ? (1 - x_l) : x_l;
+  const double wy = ((q >> 1) & 1) ? (1 - y_l) : y_l;
+  const double wz = (q >> 2) ? (1 - z_l) : z_l;
+  double t = wx * wy * wz;
+  if (sw) t = t * sw[pos];
+  return t;
+}
+
+// The ordered sum of trilinear target (a, b, c): an 8-way merge of its
+// source buckets in ascending reference index.
+__device__ __forceinline__ double dep_cic_target(const Dev& d, const int* __restrict__ start,
+                                                 const int* __restrict__ list,
+                                                 const double* __restrict__ sxl,
+                                                 const double* __restrict__ syl,
+                                                 const double* __restrict__ szl,
+                                                 const double* __restrict__ sw, int a, int b,
+                                                 int c) {
+  Merge<8> m;
+  cic_ranges(d, start, a, b, c, m.cur, m.end);
+  m.init(list);
+  double acc = 0.0;
+  for (;;) {
+    int pos;
+    const int q = m.next(list, pos);
+    if (q < 0) break;
+    acc = acc + cic_weight(q, pos, sxl, syl, szl, sw);
+  }
+  return acc;
 }
 
 // particle.cpp:526-559 / 586-658 (before the cumulative halo): target
 // (a,b,c) collects from the rounded-index bucket of cell (a+ox, b+oy, c+oz)
 // the weight with (1-x_l) where ox = 1 and x_l where ox = 0 (likewise y, z),
 // times RHS_T when sw is given; an 8-way merge in ascending reference index.
-// Buckets outside the grid are empty: columns clamp to the row end (column
-// Nx is the start of the next row), rows and planes past the grid get no
-// range.  Every ghosted cell is written.
+// Every ghosted cell is written.
 template <typename T>
 __global__ void __launch_bounds__(256) k_dep_cic_pull(Dev d, const int* __restrict__ start,
                                                       const int* __restrict__ list,
@@ -1182,38 +1281,323 @@ __global__ void __launch_bounds__(256) k_dep_cic_pull(Dev d, const int* __restri
   const uint3 bk = xcd_block();
   const int a = bk.x * DTX + threadIdx.x, b = bk.y * DTY + threadIdx.y, c = bk.z;
   if (a >= d.Nx || b >= d.Ny) return;
+  out[(c * d.Ny + b) * d.Nx + a] = (T)dep_cic_target(d, start, list, sxl, syl, szl, sw, a, b, c);
+}
+
+// ---- split-form deposits --------------------------------------------------------
+// The reference adds a target's contributions to 0.0 in ascending particle
+// order.  With at most two contributions that sum does not depend on the
+// order: 0.0 + t is t except that -0 becomes +0, and (0.0 + t1) + t2 equals
+// (0.0 + t2) + t1 in every case, signed zeros included.  The split form sums
+// such targets straight from their bucket ranges with no reference index
+// read; the targets with three or more contributions (at 0.12 particles per
+// cell about 7% of the trilinear and 0.2% of the momentum targets: Poisson
+// counts of mean 0.96 and 0.24) are compacted per workgroup in LDS and merged
+// in ascending reference index by the workgroup's first threads.
+//
+// These kernels are bound by instruction issue, not bytes, so the layout
+// serves the instruction count: a thread owns one (a, b) column of SKZ
+// consecutive target planes (the bucket starts of a source plane serve two
+// target planes from registers; cells a, a+1, a+2 of a row come from one
+// address with immediate offsets), and the bucket gather stores each
+// particle's operands as one record (array of structures) so that one
+// contribution is one or two 16-byte loads from one address.
+constexpr int SKZ = 4;    // target planes per thread
+constexpr int SNT = 256;  // threads per workgroup, consecutive cells of an (x, y) plane
+
+// Trilinear operand record: x_l, y_l, z_l, RHS_T (32 B).  Weight through
+// range q = ox + 2 oy + 4 oz (particle.cpp:536-556 / 597-655; the same
+// expression as cic_weight).
+template <bool W>
+__device__ __forceinline__ double cic_weight_rec(int q, const double* __restrict__ ops, int pos) {
+  const double* r = ops + 4 * (long)pos;
+  const double2 xy = *reinterpret_cast<const double2*>(r);
+  const double2 zw = *reinterpret_cast<const double2*>(r + 2);
+  const double wx = (q & 1) ? (1 - xy.x) : xy.x;
+  const double wy = ((q >> 1) & 1) ? (1 - xy.y) : xy.y;
+  const double wz = (q >> 2) ? (1 - zw.x) : zw.x;
+  double t = wx * wy * wz;
+  if (W) t = t * zw.y;
+  return t;
+}
+
+template <bool W>
+__device__ __forceinline__ double dep_cic_target_rec(const Dev& d, const int* __restrict__ start,
+                                                     const int* __restrict__ list,
+                                                     const double* __restrict__ ops, int a, int b,
+                                                     int c) {
   Merge<8> m;
-  const int a1 = min(a + 1, d.Nx), a2 = min(a + 2, d.Nx);
-#pragma unroll
-  for (int rq = 0; rq < 4; rq++) {  // source row (b + (rq & 1), c + (rq >> 1))
-    const int B = b + (rq & 1), Cz = c + (rq >> 1);
-    int v0 = 0, v1 = 0, v2 = 0;
-    if (B < d.Ny && Cz < d.Nz) {
-      const int rb = (Cz * d.Ny + B) * d.Nx;
-      v0 = start[rb + a];
-      v1 = start[rb + a1];
-      v2 = start[rb + a2];
-    }
-    m.cur[2 * rq] = v0;
-    m.end[2 * rq] = v1;
-    m.cur[2 * rq + 1] = v1;
-    m.end[2 * rq + 1] = v2;
-  }
+  cic_ranges(d, start, a, b, c, m.cur, m.end);
   m.init(list);
   double acc = 0.0;
   for (;;) {
     int pos;
     const int q = m.next(list, pos);
     if (q < 0) break;
-    const double x_l = sxl[pos], y_l = syl[pos], z_l = szl[pos];
-    const double wx = (q & 1) ? (1 - x_l) : x_l;
-    const double wy = ((q >> 1) & 1) ? (1 - y_l) : y_l;
-    const double wz = (q >> 2) ? (1 - z_l) : z_l;
-    double t = wx * wy * wz;
-    if (sw) t = t * sw[pos];
-    acc = acc + t;
+    acc = acc + cic_weight_rec<W>(q, ops, pos);
   }
-  out[(c * d.Ny + b) * d.Nx + a] = (T)acc;
+  return acc;
+}
+
+// Bucket starts of source rows b and b+1 of plane cz for column (a, b):
+// v[0..2] = start of cells a, a+1, a+2 of row b (a+2 clamped to the row end),
+// v[3..5] the same of row b+1; rows or planes outside the grid are empty.
+// start[] holds ncell + 2 entries, so the unclamped a+2 load of the last cell
+// stays inside it.
+__device__ __forceinline__ void cic_plane_starts(const Dev& d, const int* __restrict__ start,
+                                                 int ij, int cz, bool col, bool row1,
+                                                 bool lastx, int* v) {
+#pragma unroll
+  for (int k = 0; k < 6; k++) v[k] = 0;
+  if (col && cz < d.Nz) {
+    const int* p = start + ((long)cz * d.NxNy + ij);
+    v[0] = p[0];
+    v[1] = p[1];
+    v[2] = p[2];
+    if (row1) {
+      v[3] = p[d.Nx];
+      v[4] = p[d.Nx + 1];
+      v[5] = p[d.Nx + 2];
+    }
+    if (lastx) {
+      v[2] = v[1];
+      v[5] = v[4];
+    }
+  }
+}
+
+// Trilinear deposit of particle.cpp:526-559 / 586-658 (before the cumulative
+// halo), split form; every ghosted cell is written.  ops: trilinear records
+// in bucket order.
+template <typename T, bool W>
+__global__ void __launch_bounds__(SNT) k_dep_cic_split(Dev d, const int* __restrict__ start,
+                                                       const int* __restrict__ list,
+                                                       const double* __restrict__ ops,
+                                                       T* __restrict__ out) {
+  __shared__ int wt[4];
+  __shared__ int hl[SNT * SKZ];
+  const uint3 bk = xcd_block();
+  const int tid = threadIdx.x;
+  const int ij = bk.x * SNT + tid, c0 = bk.y * SKZ;
+  const bool col = ij < d.NxNy;
+  const int b = col ? ij / d.Nx : 0, a = ij - b * d.Nx;
+  const bool row1 = col && b + 1 < d.Ny, lastx = a == d.Nx - 1;
+  int lo[6], hi[6];
+  cic_plane_starts(d, start, ij, c0, col, row1, lastx, lo);
+  unsigned hard = 0;
+#pragma unroll
+  for (int k = 0; k < SKZ; k++) {
+    const int c = c0 + k;
+    cic_plane_starts(d, start, ij, c + 1, col, row1, lastx, hi);
+    if (col && c < d.Nz) {
+      // source rows rq = oy + 2 oz: (b, c), (b+1, c), (b, c+1), (b+1, c+1);
+      // each is the contiguous range [v0, v2) of cells a (ox = 0, below v1)
+      // and a+1 (ox = 1).  The first two contributions in range order:
+      int tot = 0, p1 = 0, p2 = 0, r1 = 0, r2 = 0, m1 = 0, m2 = 0;
+#pragma unroll
+      for (int rq = 3; rq >= 0; rq--) {
+        const int* v = rq < 2 ? lo + 3 * rq : hi + 3 * (rq - 2);
+        const int n = v[2] - v[0];
+        tot += n;
+        if (n > 0) {
+          p2 = n >= 2 ? v[0] + 1 : p1;
+          r2 = n >= 2 ? rq : r1;
+          m2 = n >= 2 ? v[1] : m1;
+          p1 = v[0];
+          r1 = rq;
+          m1 = v[1];
+        }
+      }
+      if (tot <= 2) {
+        double acc = 0.0;
+        if (tot > 0) {
+          const bool two = tot == 2;
+          const int pb = two ? p2 : p1, rb = two ? r2 : r1, mb = two ? m2 : m1;
+          const double t1 = cic_weight_rec<W>(2 * r1 + (p1 >= m1 ? 1 : 0), ops, p1);
+          const double t2 = cic_weight_rec<W>(2 * rb + (pb >= mb ? 1 : 0), ops, pb);
+          acc = acc + t1;
+          if (two) acc = acc + t2;
+        }
+        out[(long)c * d.NxNy + ij] = (T)acc;
+      } else {
+        hard |= 1u << k;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) lo[q] = hi[q];
+  }
+  int nh;
+  int o = block_excl_scan(__popc(hard), wt, &nh);
+#pragma unroll
+  for (int k = 0; k < SKZ; k++)
+    if (hard & (1u << k)) hl[o++] = (k << 8) | tid;
+  __syncthreads();
+  for (int i = tid; i < nh; i += SNT) {
+    const int e = hl[i];
+    const int ij2 = bk.x * SNT + (e & 255), c = c0 + (e >> 8);
+    const int b2 = ij2 / d.Nx, a2 = ij2 - b2 * d.Nx;
+    out[(long)c * d.NxNy + ij2] = (T)dep_cic_target_rec<W>(d, start, list, ops, a2, b2, c);
+  }
+}
+
+// Momentum operand record: (x_l, RHS_u), (y_l, RHS_v), (z_l, RHS_w) (48 B);
+// component M's pair at 16 M bytes.
+__device__ __forceinline__ double2 vel_pair(const double* __restrict__ ops, int pos, int M) {
+  return *reinterpret_cast<const double2*>(ops + 6 * (long)pos + 2 * M);
+}
+
+// merge2_step on the operand records
+__device__ __forceinline__ void merge2_step_rec(Merge2& m, const int* __restrict__ list,
+                                                const double* __restrict__ ops, int M) {
+  const bool own = m.k0 < m.k1;
+  const int pos = own ? m.i0 : m.i1;
+  const double2 lr = vel_pair(ops, pos, M);
+  m.acc = m.acc + (own ? (1 - lr.x) * lr.y : lr.x * lr.y);
+  const int nx = pos + 1, en = own ? m.e0 : m.e1;
+  const int other = own ? m.k1 : m.k0;
+  const int kn = nx < en ? (other != INT_MAX ? list[nx] : (own ? 0 : 1)) : INT_MAX;
+  if (own) {
+    m.i0 = nx;
+    m.k0 = kn;
+  } else {
+    m.i1 = nx;
+    m.k1 = kn;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void dep_vel_target_rec(const Dev& d, const int* __restrict__ start,
+                                                   const int* __restrict__ list,
+                                                   const double* __restrict__ ops,
+                                                   T* const o[3], int a, int b, int c) {
+  const long n = (long)c * d.NxNy + (long)b * d.Nx + a;
+  const int s0 = start[n], e0 = start[n + 1];
+  Merge2 m[3];
+#pragma unroll
+  for (int M = 0; M < 3; M++) {
+    m[M].i0 = s0;
+    m[M].e0 = e0;
+    m[M].i1 = m[M].e1 = 0;
+  }
+  if (a > 0) {
+    m[0].i1 = start[n - 1];
+    m[0].e1 = s0;
+  }
+  if (b > 0) {
+    m[1].i1 = start[n - d.Nx];
+    m[1].e1 = start[n - d.Nx + 1];
+  }
+  if (c > 0) {
+    m[2].i1 = start[n - d.NxNy];
+    m[2].e1 = start[n - d.NxNy + 1];
+  }
+#pragma unroll
+  for (int M = 0; M < 3; M++) merge2_start(m[M], list);
+  for (;;) {
+    bool any = false;
+#pragma unroll
+    for (int M = 0; M < 3; M++) {
+      if (m[M].k0 != INT_MAX || m[M].k1 != INT_MAX) {
+        merge2_step_rec(m[M], list, ops, M);
+        any = true;
+      }
+    }
+    if (!any) break;
+  }
+#pragma unroll
+  for (int M = 0; M < 3; M++) o[M][n] = (T)m[M].acc;
+}
+
+// Momentum deposit of particle.cpp:498-524 (before the cumulative halo),
+// split form: component M of target t takes (1-l_M) R_M from bucket t and
+// l_M R_M from bucket t - e_M; a target whose components each have at most
+// two contributions is summed directly, the rest merge all three.  The own
+// range of a plane is the z-lower range of the next.  ops: momentum records
+// in bucket order.
+template <typename T>
+__global__ void __launch_bounds__(SNT) k_dep_vel_split(Dev d, const int* __restrict__ start,
+                                                       const int* __restrict__ list,
+                                                       const double* __restrict__ ops,
+                                                       T* __restrict__ ox, T* __restrict__ oy,
+                                                       T* __restrict__ oz) {
+  __shared__ int wt[4];
+  __shared__ int hl[SNT * SKZ];
+  const uint3 bk = xcd_block();
+  const int tid = threadIdx.x;
+  const int ij = bk.x * SNT + tid, c0 = bk.y * SKZ;
+  const bool col = ij < d.NxNy;
+  const int b = col ? ij / d.Nx : 0, a = ij - b * d.Nx;
+  T* const o[3] = {ox, oy, oz};
+  int zs = 0, ze = 0;  // own range of plane c - 1
+  if (col && c0 > 0) {
+    const int* p = start + ((long)(c0 - 1) * d.NxNy + ij);
+    zs = p[0];
+    ze = p[1];
+  }
+  unsigned hard = 0;
+#pragma unroll
+  for (int k = 0; k < SKZ; k++) {
+    const int c = c0 + k;
+    if (col && c < d.Nz) {
+      const long n = (long)c * d.NxNy + ij;
+      const int* p = start + n;
+      const int s0 = p[0], e0 = p[1];
+      int i1[3] = {0, 0, zs}, e1[3] = {0, 0, ze};
+      if (a > 0) {
+        i1[0] = p[-1];
+        e1[0] = s0;
+      }
+      if (b > 0) {
+        i1[1] = p[-d.Nx];
+        e1[1] = p[-d.Nx + 1];
+      }
+      const int no = e0 - s0;
+      int nl[3];
+      bool hk = false;
+#pragma unroll
+      for (int M = 0; M < 3; M++) {
+        nl[M] = e1[M] - i1[M];
+        hk = hk || no + nl[M] > 2;
+      }
+      if (!hk) {
+        const int pb = no > 1 ? s0 + 1 : s0;
+#pragma unroll
+        for (int M = 0; M < 3; M++) {
+          double2 w0 = make_double2(0, 0), w1 = w0, u0 = w0, u1 = w0;
+          if (no > 0) {
+            w0 = vel_pair(ops, s0, M);
+            w1 = vel_pair(ops, pb, M);
+          }
+          if (nl[M] > 0) {
+            u0 = vel_pair(ops, i1[M], M);
+            u1 = vel_pair(ops, nl[M] > 1 ? i1[M] + 1 : i1[M], M);
+          }
+          double acc = 0.0;
+          if (no > 0) acc = acc + (1 - w0.x) * w0.y;
+          if (no > 1) acc = acc + (1 - w1.x) * w1.y;
+          if (nl[M] > 0) acc = acc + u0.x * u0.y;
+          if (nl[M] > 1) acc = acc + u1.x * u1.y;
+          o[M][n] = (T)acc;
+        }
+      } else {
+        hard |= 1u << k;
+      }
+      zs = s0;
+      ze = e0;
+    }
+  }
+  int nh;
+  int off = block_excl_scan(__popc(hard), wt, &nh);
+#pragma unroll
+  for (int k = 0; k < SKZ; k++)
+    if (hard & (1u << k)) hl[off++] = (k << 8) | tid;
+  __syncthreads();
+  for (int i = tid; i < nh; i += SNT) {
+    const int e = hl[i];
+    const int ij2 = bk.x * SNT + (e & 255), c = c0 + (e >> 8);
+    const int b2 = ij2 / d.Nx, a2 = ij2 - b2 * d.Nx;
+    dep_vel_target_rec(d, start, list, ops, o, a2, b2, c);
+  }
 }
 
 // ---- migration (one rank) -------------------------------------------------------
@@ -1472,7 +1856,11 @@ namespace fr {
 
 static int alloc_buckets(Buckets& B, long ncell, int cap, int nops) {
   FR_CHECK(hipMalloc(&B.cnt, sizeof(int) * (ncell + 1)));
-  FR_CHECK(hipMalloc(&B.start, sizeof(int) * (ncell + 1)));
+  // two entries past the scanned ncell + 1: the split-form deposits load the
+  // start of cell a+2 before clamping it to the row end
+  FR_CHECK(hipMalloc(&B.start, sizeof(int) * (ncell + 2)));
+  FR_CHECK(hipMemset(B.start, 0, sizeof(int) * (ncell + 2)));
+  FR_CHECK(hipMalloc(&B.rec, sizeof(double) * nops * (size_t)cap));
   FR_CHECK(hipMalloc(&B.list, sizeof(int) * cap));
   FR_CHECK(hipMalloc(&B.list2, sizeof(int) * cap));
   FR_CHECK(hipMalloc(&B.key, sizeof(int) * cap));
@@ -1483,33 +1871,39 @@ static int alloc_buckets(Buckets& B, long ncell, int cap, int nops) {
 }
 
 static void free_buckets(Buckets& B) {
-  void* all[] = {B.cnt, B.start, B.list, B.list2, B.key, B.slot, B.scan_tmp};
+  void* all[] = {B.cnt, B.start, B.list, B.list2, B.key, B.slot, B.scan_tmp, B.rec};
   for (void* p : all) (void)hipFree(p);
   for (double* p : B.s)
     if (p) (void)hipFree(p);
 }
 
-// Measured at 512^3 / 16M and 256^3 / 4M particles (0.12 and 0.24 per
-// cell): the pull form runs the momentum deposit 2.2x faster than the tiled
-// form (3.1 against 7.9 ms per launch at 512^3), but the trilinear deposits
-// 1.3x slower (its 8-way merges wait on dependent loads, and its light waves
-// crowd the momentum update that runs beside the heat deposit); whole steps:
-// 179.1 ms with the momentum deposit pulled and the trilinear ones tiled,
-// 180.7 all tiled, 184.4 all pulled.  So by default the momentum deposit
-// takes the pull form up to PULL_MAX_DENSITY particles per ghosted cell
-// (mean) and the trilinear deposits the tiled form.
-constexpr double PULL_MAX_DENSITY = 0.5;
+// Default forms by the mean particles per ghosted cell.  Up to
+// SPARSE_MAX_DENSITY both deposits take the split form: at 512^3 / 16M
+// particles (0.12 per cell) its kernels run alone in 1.20 ms (trilinear, tiled
+// form 1.90 ms) and 2.08 ms (momentum, pull form 2.42 ms, tiled 7.9 ms), and
+// the full step is 0.6% faster (interleaved runs on one box: 808.3, 808.7
+// against 802.7, 803.6 Mcell*steps/s with the momentum deposit pulled and the
+// trilinear ones tiled).  Denser sets (the duct's inlet third holds about 3
+// particles per cell, where nearly every target merges) take the tiled form.
+constexpr double SPARSE_MAX_DENSITY = 0.5;
 
-bool PartDev::use_pull(bool trilinear) const {
-  if (dep_form == 3) return !trilinear;
-  if (dep_form) return dep_form == 1;
-  return !trilinear && (double)np <= PULL_MAX_DENSITY * (double)ncell;
+int PartDev::dep_kind(bool trilinear) const {
+  if (dep_form == 3) return trilinear ? DEP_TILE : DEP_PULL;
+  if (dep_form == 1) return DEP_PULL;
+  if (dep_form == 2) return DEP_TILE;
+  if (dep_form == 4) return DEP_SPLIT;
+  (void)trilinear;
+  return (double)np <= SPARSE_MAX_DENSITY * (double)ncell ? DEP_SPLIT : DEP_TILE;
 }
 
 int PartDev::alloc(int capacity, const Dev& d) {
   flg_zero = false;
   if (const char* e = getenv("FR_DEPOSIT"))
-    dep_form = !strcmp(e, "pull") ? 1 : !strcmp(e, "tile") ? 2 : !strcmp(e, "mixed") ? 3 : 0;
+    dep_form = !strcmp(e, "pull")    ? 1
+               : !strcmp(e, "tile")  ? 2
+               : !strcmp(e, "mixed") ? 3
+               : !strcmp(e, "split") ? 4
+                                     : 0;
   cap = capacity < 1 ? 1 : capacity;
   ncell = d.size;
   for (int q = 0; q < 7; q++)
@@ -1649,6 +2043,9 @@ int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool opera
                    const int* rk) {
   Buckets& B = mode == 0 ? R : T;
   if (!rk) rk = rank;
+  // the operand layout follows the deposit form chosen now; the deposit
+  // reads whichever layout its buckets were built with
+  B.rec_form = dep_kind(mode == 0) == DEP_SPLIT;
   FR_CHECK(hipMemsetAsync(B.cnt, 0, sizeof(int) * ncell, st));
   if (np > 0)
     hipLaunchKernelGGL(k_bucket_count, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), stage,
@@ -1660,7 +2057,10 @@ int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool opera
                        B.slot, B.start, rk, B.list2);
     hipLaunchKernelGGL(k_bucket_place, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
                        B.start, rk, B.list2, B.list, B.slot);
-    if (operands)
+    if (operands && B.rec_form)
+      hipLaunchKernelGGL(k_bucket_gather_rec, dim3(nblocks(np, 256)), dim3(256), 0, st, np, mode,
+                         ptrs(), stage, B.slot, d.idx, d.idy, d.idz, B.rec);
+    else if (operands)
       hipLaunchKernelGGL(k_bucket_gather, dim3(nblocks(np, 256)), dim3(256), 0, st, np, mode,
                          ptrs(), stage, B.slot, d.idx, d.idy, d.idz, B.s[0], B.s[1], B.s[2],
                          B.s[3], B.s[4], B.s[5]);
@@ -1680,11 +2080,20 @@ int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStre
   // few dense tiles merge in place, which spares every call a launch that
   // reserves 76 KB of LDS per workgroup.
   const dim3 g = grid3p(d);
-  if (use_pull(true)) {
+  const int kind = R.rec_form ? DEP_SPLIT : dep_kind(true) == DEP_TILE ? DEP_TILE : DEP_PULL;
+  if (kind != DEP_TILE) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     t_cic.next(&e0, &e1);
-    hipExtLaunchKernelGGL(k_dep_cic_pull<T>, g, dim3(DTX, DTY), 0, st, e0, e1, 0, d, R.start,
-                          R.list, R.s[0], R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out);
+    const dim3 gs(nblocks(d.NxNy, SNT), nblocks(d.Nz, SKZ));
+    if (kind == DEP_SPLIT && weighted)
+      hipExtLaunchKernelGGL((k_dep_cic_split<T, true>), gs, dim3(SNT), 0, st, e0, e1, 0, d,
+                            R.start, R.list, R.rec, out);
+    else if (kind == DEP_SPLIT)
+      hipExtLaunchKernelGGL((k_dep_cic_split<T, false>), gs, dim3(SNT), 0, st, e0, e1, 0, d,
+                            R.start, R.list, R.rec, out);
+    else
+      hipExtLaunchKernelGGL(k_dep_cic_pull<T>, g, dim3(DTX, DTY), 0, st, e0, e1, 0, d, R.start,
+                            R.list, R.s[0], R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out);
     FR_LAUNCH_CHECK();
     int rc = cum_fold(d, out, st);
     if (!rc && halo) rc = ghost_fill(d, out, st);
@@ -1717,7 +2126,12 @@ int PartDev::deposit_vel(const Dev& d, F* const out[3], hipStream_t st, bool hal
   if (!T.valid) return FR_EINVAL;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   t_vel.next(&e0, &e1);
-  if (use_pull(false))
+  const int kind = T.rec_form ? DEP_SPLIT : dep_kind(false) == DEP_TILE ? DEP_TILE : DEP_PULL;
+  if (kind == DEP_SPLIT)
+    hipExtLaunchKernelGGL(k_dep_vel_split<F>, dim3(nblocks(d.NxNy, SNT), nblocks(d.Nz, SKZ)),
+                          dim3(SNT), 0, st, e0, e1, 0, d, T.start, T.list, T.rec, out[0], out[1],
+                          out[2]);
+  else if (kind == DEP_PULL)
     hipExtLaunchKernelGGL(k_dep_vel_pull<F>, grid3p(d), dim3(DTX, DTY), 0, st, e0, e1, 0, d,
                           T.start, T.list, T.s[0], T.s[1], T.s[2], T.s[3], T.s[4], T.s[5], out[0],
                           out[1], out[2]);

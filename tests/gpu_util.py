@@ -135,7 +135,8 @@ def compare_state(og, sim, tol=1e-12, fields1=FIELDS1, fields0=FIELDS0, particle
 
 def sim_with_deposit(params, d, form=None, **kw):
     """fr.Sim whose particle deposits use `form` ("pull": one thread per
-    target cell; "tile": LDS tiles; None: chosen by density).  The form is
+    target cell; "tile": LDS tiles; "split": direct sums for targets with at
+    most two contributions, merges for the rest; None: chosen by density).  The form is
     read from FR_DEPOSIT when the particle set is allocated."""
     old = os.environ.pop("FR_DEPOSIT", None)
     try:

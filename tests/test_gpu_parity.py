@@ -320,7 +320,8 @@ def test_steps_match_oracle(case):
         assert dg[9] == do[9]
 
 
-@pytest.mark.parametrize("interval,form", [(1, "pull"), (4, "pull"), (4, "tile")])
+@pytest.mark.parametrize("interval,form",
+                         [(1, "pull"), (4, "pull"), (4, "tile"), (1, "split"), (4, "split")])
 def test_lockstep_steps_match_oracle(case, interval, form):
     """Six RK4 steps with no state copied between them: the device's own
     buckets, storage re-sorts (every `interval` steps, so the re-sort at step
@@ -390,7 +391,7 @@ def test_migration_order_many_leavers(case):
                 assert np.array_equal(sim.part(q, s), og.part(q, s)), (q, s)
 
 
-@pytest.mark.parametrize("form", ["tile", "pull"])
+@pytest.mark.parametrize("form", ["tile", "pull", "split"])
 @pytest.mark.parametrize("npart", [30000, 70000])
 def test_dense_deposits_match_oracle(tmp_path, npart, form):
     """particle::Update_Particle at about 7 and 17 particles per cell: the
@@ -398,8 +399,9 @@ def test_dense_deposits_match_oracle(tmp_path, npart, form):
     so they take the dense path, which merges per target from the tile's
     particles staged in LDS (7 per cell) or, when they do not fit, from
     global memory (17 per cell); the pull form merges every target's eight
-    buckets from global memory.  The concentration deposit is bit-exact;
-    the rest within the substep bar."""
+    buckets from global memory, and the split form does so for nearly every
+    target (three or more contributions), compacted per workgroup.  The
+    concentration deposit is bit-exact; the rest within the substep bar."""
     params, d = make_box_case(tmp_path, n=16, n_particles=npart)
     og = OL.OracleGrid(params, d)
     og.initialize()
@@ -420,12 +422,14 @@ def test_dense_deposits_match_oracle(tmp_path, npart, form):
     assert worst["C"] == 0.0, worst["C"]
 
 
-@pytest.mark.parametrize("form", ["tile", "pull"])
+@pytest.mark.parametrize("form", ["tile", "pull", "split"])
 def test_clustered_deposits_match_oracle(tmp_path, form):
     """A sparse particle set (mean tile below half the LDS list) with all of
     its particles gathered into 3x3x3 cells, about 37 per cell: in the tiled
     form the few dense tiles merge per target inside the sparse deposit
-    kernel; in the pull form those targets merge long bucket ranges."""
+    kernel; in the pull and split forms those targets merge long bucket
+    ranges (the split form sums the lone and paired contributions at the
+    cluster's edge directly)."""
     params, d = make_box_case(tmp_path, n=16, n_particles=1000)
     og = OL.OracleGrid(params, d)
     og.initialize()
