@@ -135,7 +135,9 @@ __device__ __forceinline__ int wave_pos(int lane, int k) {
   return ((gid >> S0) << (S0 + GG)) + pos + (m << S0);
 }
 
-template <int LG, int S0, int GG, typename C2>
+// CJ: use the conjugates of tw (the inverse transform from the forward
+// table; negating the sine is exact, so the result equals a conjugate table)
+template <int LG, int S0, int GG, bool CJ, typename C2>
 __device__ __forceinline__ void wave_stages(C2 (&x)[WavePlan<LG>::E], int lane, const C2* tw) {
   constexpr int E = WavePlan<LG>::E, G = 1 << GG, R = E >> GG;
 #pragma unroll
@@ -148,7 +150,9 @@ __device__ __forceinline__ void wave_stages(C2 (&x)[WavePlan<LG>::E], int lane, 
 #pragma unroll
       for (int m = 0; m < G; m++) {
         if (m & h) continue;
-        bfly(x[r * G + m], x[r * G + m + h], tw[(pos + ((m & (h - 1)) << S0)) << (LG - 1 - s)]);
+        C2 w = tw[(pos + ((m & (h - 1)) << S0)) << (LG - 1 - s)];
+        if (CJ) w.y = -w.y;
+        bfly(x[r * G + m], x[r * G + m + h], w);
       }
     }
   }
@@ -172,24 +176,24 @@ __device__ __forceinline__ void wave_exchange(C2 (&x)[WavePlan<LG>::E], C2* ex, 
   wave_sync();
 }
 
-template <int LG, int PP, typename C2>
+template <int LG, int PP, bool CJ, typename C2>
 __device__ __forceinline__ void wave_passes(C2 (&x)[WavePlan<LG>::E], C2* ex, int lane,
                                             const C2* tw) {
   using P = WavePlan<LG>;
   if constexpr (PP < P::NPASS) {
     if constexpr (PP > 0) wave_exchange<LG, PP - 1, PP>(x, ex, lane);
-    wave_stages<LG, P::S0(PP), P::G(PP)>(x, lane, tw);
-    wave_passes<LG, PP + 1>(x, ex, lane, tw);
+    wave_stages<LG, P::S0(PP), P::G(PP), CJ>(x, lane, tw);
+    wave_passes<LG, PP + 1, CJ>(x, ex, lane, tw);
   }
 }
 
 // On entry x[k] is buffer element wave_pos<LG,0,LE>(lane,k) of the
 // bit-reversed input (wave_in_index gives its natural index); on exit x[k]
-// is output wave_out_index(lane, k), natural order.
-template <int LG, typename C2>
+// is output wave_out_index(lane, k), natural order.  CJ: twiddles conj(tw).
+template <int LG, bool CJ = false, typename C2>
 __device__ __forceinline__ void wave_fft(C2 (&x)[WavePlan<LG>::E], C2* ex, int lane,
                                          const C2* tw) {
-  wave_passes<LG, 0>(x, ex, lane, tw);
+  wave_passes<LG, 0, CJ>(x, ex, lane, tw);
 }
 
 // natural index loaded into slot k: bitrev(lane*E + k) = bitrev_LE(k)*64 + bitrev6(lane)

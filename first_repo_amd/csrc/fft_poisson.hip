@@ -241,32 +241,43 @@ __global__ void __launch_bounds__(64 * WY_WAVES) k_pass_y_wave(
   for (int k = 0; k < P::E; k++) p[wave_out_index<LG>(lane, k)] = x[k];
 }
 
+// The z pass stages its WZ_WAVES pencils through the waves' exchange areas:
+// the workgroup loads and stores (z, y) tiles whose WZ_WAVES consecutive y
+// elements are one contiguous 128-byte row segment (one memory request per
+// segment instead of one per element), and each wave then reads its pencil
+// from LDS.  The area pitch is XS + 2 elements, so the WZ_WAVES pencils of a
+// staged z start on different LDS banks.  The inverse transform reads the
+// forward twiddles conjugated (the host's inverse table is exactly that), so
+// one table and the areas fit two workgroups per CU.
 template <typename R, int LG>
 __global__ void __launch_bounds__(64 * WZ_WAVES) k_pass_z_wave(
-    PoisDev q, const typename Cx<R>::T* __restrict__ tw, const typename Cx<R>::T* __restrict__ tw2,
-    const double* __restrict__ lamx, const double* __restrict__ lamy,
-    const double* __restrict__ lamz, typename Cx<R>::T* __restrict__ data,
-    const int* __restrict__ gate) {
+    PoisDev q, const typename Cx<R>::T* __restrict__ tw, const double* __restrict__ lamx,
+    const double* __restrict__ lamy, const double* __restrict__ lamz,
+    typename Cx<R>::T* __restrict__ data, const int* __restrict__ gate) {
   using C2 = typename Cx<R>::T;
   using P = WavePlan<LG>;
-  __shared__ C2 twf[P::L / 2 + 1], twi[P::L / 2 + 1];
-  __shared__ C2 exs[WZ_WAVES][P::XS];
+  constexpr int XP = P::XS + 2;
+  __shared__ C2 twf[P::L / 2 + 1];
+  __shared__ C2 exs[WZ_WAVES][XP];
   if (*gate) return;
-  for (int t = threadIdx.x; t <= P::L / 2; t += 64 * WZ_WAVES) {
-    twf[t] = tw[t];
-    twi[t] = tw2[t];
-  }
-  __syncthreads();
+  for (int t = threadIdx.x; t <= P::L / 2; t += 64 * WZ_WAVES) twf[t] = tw[t];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tiles = q.ny / WZ_WAVES;
   const unsigned bid = xcd_block().x;
   const int kx = (int)(bid / (unsigned)tiles);
-  const int jy = ((int)bid - kx * tiles) * WZ_WAVES + wave;
+  const int j0 = ((int)bid - kx * tiles) * WZ_WAVES, jy = j0 + wave;
   const long stride = (long)q.nxh * q.ny;  // P::L == nz
-  C2* p = data + (long)kx * q.ny + jy;
+  C2* base = data + (long)kx * q.ny + j0;
+  static_assert(WZ_WAVES == 8, "staging maps 8 consecutive y per z");
+  for (int t = threadIdx.x; t < WZ_WAVES * P::L; t += 64 * WZ_WAVES) {
+    const int y = t & (WZ_WAVES - 1), z = t >> 3;
+    exs[y][wave_xi(z)] = base[(long)z * stride + y];
+  }
+  __syncthreads();
   C2 x[P::E];
 #pragma unroll
-  for (int k = 0; k < P::E; k++) x[k] = p[(long)wave_in_index<LG>(lane, k) * stride];
+  for (int k = 0; k < P::E; k++) x[k] = exs[wave][wave_xi(wave_in_index<LG>(lane, k))];
+  wave_sync();
   wave_fft<LG>(x, exs[wave], lane, twf);
   // divide by lambda except the zero mode (poisson.cpp:438-451)
   const double lxy = lamx[kx] + lamy[jy];
@@ -280,9 +291,14 @@ __global__ void __launch_bounds__(64 * WZ_WAVES) k_pass_z_wave(
     }
   }
   wave_to_bitrev<LG>(x, exs[wave], lane);
-  wave_fft<LG>(x, exs[wave], lane, twi);
+  wave_fft<LG, true>(x, exs[wave], lane, twf);
 #pragma unroll
-  for (int k = 0; k < P::E; k++) p[(long)wave_out_index<LG>(lane, k) * stride] = x[k];
+  for (int k = 0; k < P::E; k++) exs[wave][wave_xi(wave_out_index<LG>(lane, k))] = x[k];
+  __syncthreads();
+  for (int t = threadIdx.x; t < WZ_WAVES * P::L; t += 64 * WZ_WAVES) {
+    const int y = t & (WZ_WAVES - 1), z = t >> 3;
+    base[(long)z * stride + y] = exs[y][wave_xi(z)];
+  }
 }
 
 // Pass X': inverse real FFT along x: Z_k = (X_k + conj X_{M-k}) +
@@ -475,11 +491,11 @@ static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const
   const bool zw = (q.lz == 9 || q.lz == 8) && q.ny % WZ_WAVES == 0;
   const dim3 gzw(q.ny / WZ_WAVES * q.nxh), bzw(64 * WZ_WAVES);
   if (zw && q.lz == 9)
-    hipLaunchKernelGGL((k_pass_z_wave<R, 9>), gzw, bzw, 0, st, q, twf[2], twi[2], s->lam[0],
-                       s->lam[1], s->lam[2], data, g);
+    hipLaunchKernelGGL((k_pass_z_wave<R, 9>), gzw, bzw, 0, st, q, twf[2], s->lam[0], s->lam[1],
+                       s->lam[2], data, g);
   else if (zw)
-    hipLaunchKernelGGL((k_pass_z_wave<R, 8>), gzw, bzw, 0, st, q, twf[2], twi[2], s->lam[0],
-                       s->lam[1], s->lam[2], data, g);
+    hipLaunchKernelGGL((k_pass_z_wave<R, 8>), gzw, bzw, 0, st, q, twf[2], s->lam[0], s->lam[1],
+                       s->lam[2], data, g);
   else
     hipLaunchKernelGGL(k_pass_strided<R>, dim3(tz * q.nxh), dim3(FT), lds[2], st, q, 2, 1,
                        twf[2], twi[2], s->lam[0], s->lam[1], s->lam[2], data, g);
