@@ -39,6 +39,8 @@ struct PoisDev {
   int lxm, ly, lz;     // log2(nx/2), log2(ny), log2(nz)
   int wx, wy, wz;      // pencils per workgroup for each pass (powers of two)
   int lwx, lwy, lwz;   // their log2
+  int lxz;             // forward x pass: log2 of the z planes of its (y, z) row tile
+  int wxi, lwxi;       // inverse x pass: rows per workgroup, log2
   double tot;          // nx*ny*nz
   double itot;         // 1/tot, exact (tot is a power of two): x*itot == x/tot
 };
@@ -80,18 +82,21 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
   C2* twn = twm + (M / 2 + 1);
   load_tw(twm, twM, M);
   load_tw(twn, twN, q.nx);
-  const int rows = q.ny * q.nz, lm = q.lxm, lw = q.lwx;
-  // plain dispatch order: the neighbouring row blocks whose rows the residual
-  // stencil shares run at the same time across the XCDs (measured: an
-  // XCD-contiguous order fetches 16% more and runs 8% slower here)
+  const int lm = q.lxm, lw = q.lwx;
+  // The workgroup's W rows are a tile of 2^lwy_ consecutive y in 2^lxz
+  // consecutive planes: the residual's neighbour rows in y and z are shared
+  // inside the tile.  Plain dispatch order: the neighbouring tiles run at
+  // the same time across the XCDs (measured: an XCD-contiguous order fetches
+  // 16% more and runs 8% slower here).
+  const int lwy_ = q.lwx - q.lxz;
   const unsigned bid = blockIdx.x;
-  const int row0 = (int)bid * W;
+  const int j0 = (int)(bid & ((1u << (q.ly - lwy_)) - 1)) << lwy_;
+  const int k0 = (int)(bid >> (q.ly - lwy_)) << q.lxz;
   for (int t = threadIdx.x; t < W * M; t += FT) {
     const int p = t >> lm, m = t & (M - 1);
-    const int row = row0 + p;
+    const int jj = j0 + (p & ((1 << lwy_) - 1)), kk = k0 + (p >> lwy_);
     double r0 = 0.0, r1 = 0.0;
-    if (row < rows) {
-      const int jj = row & (q.ny - 1), kk = row >> q.ly;
+    if (kk < q.nz) {
       const long c = at(d, 2 * m + 1, jj + 1, kk + 1);
       r0 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c);
       r1 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c + 1);
@@ -104,8 +109,8 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
   fft_lds(buf, M, q.lxm, W, ps, twm);
   for (int t = threadIdx.x; t < W * (M + 1); t += FT) {
     const int k = t >> lw, p = t & (W - 1);  // pencil fastest: consecutive y in the spectrum
-    const int row = row0 + p;
-    if (row >= rows) continue;
+    const int jj = j0 + (p & ((1 << lwy_) - 1)), kk = k0 + (p >> lwy_);
+    if (kk >= q.nz) continue;
     const C2 A = buf[p * ps + (k == M ? 0 : k)];
     const C2 B = buf[p * ps + (k == 0 ? 0 : M - k)];
     const R Er = (R)0.5 * (A.x + B.x), Ei = (R)0.5 * (A.y - B.y);
@@ -116,7 +121,6 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
     C2 o;
     o.x = Er + Tr;
     o.y = Ei + Ti;
-    const int jj = row & (q.ny - 1), kk = row >> q.ly;
     out[((long)kk * (M + 1) + k) * q.ny + jj] = o;
   }
 }
@@ -319,14 +323,14 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q,
   __shared__ double red[FT];
   if (*gate) return;
   const Dev& d = q.d;
-  const int M = q.nx >> 1, W = q.wx, ps = M + 1;
+  const int M = q.nx >> 1, W = q.wxi, ps = M + 1;
   C2* buf = sm;
   C2* twm = sm + W * ps;
   C2* twn = twm + (M / 2 + 1);
   load_tw(twm, twM, M);
   load_tw(twn, twN, q.nx);
   __syncthreads();
-  const int rows = q.ny * q.nz, lm = q.lxm, lw = q.lwx;
+  const int rows = q.ny * q.nz, lm = q.lxm, lw = q.lwxi;
   const unsigned bid = blockIdx.x;
   const int row0 = (int)bid * W;
   for (int t = threadIdx.x; t < W * M; t += FT) {
@@ -423,8 +427,8 @@ struct fr_box_poisson {
   PoisDev q32;
   float2* data32 = nullptr;
   float2 *twf32[3] = {}, *twi32[3] = {}, *twfm32 = nullptr, *twim32 = nullptr;
-  size_t lds32[3] = {};
-  int nbx32 = 0;
+  size_t lds32[4] = {};
+  int nbx32 = 0, nbxi32 = 0;
   double2* data = nullptr;  // (nx/2+1)*ny*nz complex
   double2 *twf[3] = {}, *twi[3] = {};  // length nx, ny, nz
   double2 *twfm = nullptr, *twim = nullptr;  // length nx/2
@@ -432,8 +436,8 @@ struct fr_box_poisson {
   double* bmax = nullptr;
   double* dres = nullptr;
   double* hres = nullptr;
-  int nbx = 0;
-  size_t lds[3] = {};  // dynamic LDS bytes of the x, y and z passes
+  int nbx = 0, nbxi = 0;  // workgroups of the forward and inverse x passes
+  size_t lds[4] = {};  // dynamic LDS bytes of the x, y, z and inverse x passes
   // HIP events bound to each launch of the forward x pass (residual + real
   // FFT) of a batch of queued iterations, read after the batch's host
   // synchronisation for the iterations that ran: summed ms, count
@@ -463,7 +467,7 @@ static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const
   const C2* twfm = f ? (const C2*)s->twfm32 : (const C2*)s->twfm;
   const C2* twim = f ? (const C2*)s->twim32 : (const C2*)s->twim;
   const size_t* lds = f ? s->lds32 : s->lds;
-  const int nbx = f ? s->nbx32 : s->nbx;
+  const int nbx = f ? s->nbx32 : s->nbx, nbxi = f ? s->nbxi32 : s->nbxi;
   // the events are bound to the launch (start and end of the kernel itself)
   const int* g = s->gate;
   if (inv)
@@ -500,29 +504,41 @@ static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const
     hipLaunchKernelGGL(k_pass_strided<R>, dim3(tz * q.nxh), dim3(FT), lds[2], st, q, 2, 1,
                        twf[2], twi[2], s->lam[0], s->lam[1], s->lam[2], data, g);
   ypass(twi[1]);
-  hipLaunchKernelGGL(k_pass_x_inv<R>, dim3(nbx), dim3(FT), lds[0], st, q, data, twim, twi[0], P,
-                     s->bmax, g);
+  hipLaunchKernelGGL(k_pass_x_inv<R>, dim3(nbxi), dim3(FT), lds[3], st, q, data, twim, twi[0],
+                     P, s->bmax, g);
   FR_LAUNCH_CHECK();
   // the periodic ghost copy of P below is idempotent, so it needs no gate
-  int rc = max_final(s->bmax, nbx, s->dres, st, s->gate, eps);
+  int rc = max_final(s->bmax, nbxi, s->dres, st, s->gate, eps);
   return rc ? rc : ghost_fill(q.d, P, st);
 }
 
-// Pencil widths and LDS bytes of the three passes for complex elements of
-// `csize` bytes within `want` bytes of LDS.
-static void plan_passes(PoisDev& q, int csize, int want, size_t lds[3], int* nbx) {
+// Pencil widths and LDS bytes of the passes for complex elements of `csize`
+// bytes: the y, z and inverse x passes within `want` bytes of LDS, the
+// forward x pass within `want_f` (measured at 512^3: 8 rows per workgroup run
+// the forward pass 10% faster than 4, while the inverse pass runs 8% slower;
+// 16 rows slow both).  lxz: log2 of the forward pass's z planes per tile,
+// clamped to the tile and the grid.
+static void plan_passes(PoisDev& q, int csize, int want, int want_f, int lxz, size_t lds[4],
+                        int* nbx, int* nbxi) {
   const int nx = q.nx, ny = q.ny, nz = q.nz, M = nx / 2;
   const int budget = want / csize - 160 * 16 / csize;  // room for the static reduction array
-  q.wx = pencils(M, ny * nz, budget, (M / 2 + 1) + (nx / 2 + 1));
+  const int budget_f = want_f / csize - 160 * 16 / csize;
+  const int twx = (M / 2 + 1) + (nx / 2 + 1);
+  q.wx = pencils(M, ny * nz, budget_f, twx);
+  q.wxi = pencils(M, ny * nz, budget, twx);
   q.wy = pencils(ny, q.nxh, budget, 2 * (ny / 2 + 1));
   q.wz = pencils(nz, ny, budget, 2 * (nz / 2 + 1));
   q.lwx = log2i(q.wx);
+  q.lwxi = log2i(q.wxi);
   q.lwy = log2i(q.wy);
   q.lwz = log2i(q.wz);
-  lds[0] = (size_t)csize * (q.wx * (M + 1) + (M / 2 + 1) + (nx / 2 + 1));
+  q.lxz = std::max(std::min({lxz, q.lwx, q.lz}), q.lwx - q.ly);
+  lds[0] = (size_t)csize * (q.wx * (M + 1) + twx);
   lds[1] = (size_t)csize * (q.wy * (ny + 1) + 2 * (ny / 2 + 1));
   lds[2] = (size_t)csize * (q.wz * (nz + 1) + 2 * (nz / 2 + 1));
+  lds[3] = (size_t)csize * (q.wxi * (M + 1) + twx);
   *nbx = (int)(((long)ny * nz + q.wx - 1) / q.wx);
+  *nbxi = (int)(((long)ny * nz + q.wxi - 1) / q.wxi);
 }
 
 // LDS budget per workgroup: 32 KB by default (several workgroups per CU);
@@ -539,6 +555,27 @@ static int lds_budget() {
   if (want < 8192) want = 8192;
   if (want > maxsh) want = maxsh;
   return want;
+}
+
+// Forward x pass: 48 KB of LDS (8 rows of 512 points), all in one plane, by
+// default; FR_FFT_XF_LDS_BYTES and FR_FFT_XZ (log2 of the planes per tile)
+// override them for tuning.  Measured at 512^3 (8 rows per workgroup): one
+// plane 1.517 ms per launch, two 1.735, four 2.080.  In plain dispatch order
+// the tiles of neighbouring planes already run on the same XCD at the same
+// time and share its L2, so the refetched rows are the tiles' y neighbours,
+// which a taller tile in y halves.
+static int lds_budget_f() {
+  int want = 49152;
+  if (const char* e = getenv("FR_FFT_XF_LDS_BYTES")) want = atoi(e);
+  if (want < 8192) want = 8192;
+  if (want > 163840) want = 163840;
+  return want;
+}
+
+static int x_tile_planes() {
+  int l = 0;
+  if (const char* e = getenv("FR_FFT_XZ")) l = atoi(e);
+  return l < 0 ? 0 : l;
 }
 
 template <typename R>
@@ -562,8 +599,9 @@ static int ensure_f32(fr_box_poisson* s) {
   if (s->data32) return FR_OK;
   PoisDev& q = s->q32;
   q = s->q;
-  plan_passes(q, (int)sizeof(float2), lds_budget(), s->lds32, &s->nbx32);
-  FR_RET(allow_lds<float>(std::max({s->lds32[0], s->lds32[1], s->lds32[2]})));
+  plan_passes(q, (int)sizeof(float2), lds_budget(), lds_budget_f(), x_tile_planes(), s->lds32,
+              &s->nbx32, &s->nbxi32);
+  FR_RET(allow_lds<float>(std::max({s->lds32[0], s->lds32[1], s->lds32[2], s->lds32[3]})));
   const int n[3] = {q.nx, q.ny, q.nz};
   for (int a = 0; a < 3; a++) {
     FR_RET(upload_tw(n[a], -1, &s->twf32[a]));
@@ -571,7 +609,7 @@ static int ensure_f32(fr_box_poisson* s) {
   }
   FR_RET(upload_tw(n[0] / 2, -1, &s->twfm32));
   FR_RET(upload_tw(n[0] / 2, 1, &s->twim32));
-  if (s->nbx32 > s->nbx) return FR_EINVAL;  // bmax holds nbx entries
+  if (s->nbxi32 > s->nbxi) return FR_EINVAL;  // bmax holds nbxi entries
   FR_CHECK(hipMalloc(&s->data32, sizeof(float2) * (size_t)q.nxh * n[1] * n[2]));
   return FR_OK;
 }
@@ -688,12 +726,13 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
     if (!rc) rc = upload_tw(n[0] / 2, 1, &s->twim);
     if (rc) return rc;
   }
-  plan_passes(q, (int)sizeof(double2), lds_budget(), s->lds, &s->nbx);
+  plan_passes(q, (int)sizeof(double2), lds_budget(), lds_budget_f(), x_tile_planes(), s->lds,
+              &s->nbx, &s->nbxi);
   {
-    const int rc = allow_lds<double>(std::max({s->lds[0], s->lds[1], s->lds[2]}));
+    const int rc = allow_lds<double>(std::max({s->lds[0], s->lds[1], s->lds[2], s->lds[3]}));
     if (rc) return rc;
   }
-  FR_CHECK(hipMalloc(&s->bmax, sizeof(double) * (s->nbx + 1)));
+  FR_CHECK(hipMalloc(&s->bmax, sizeof(double) * (s->nbxi + 1)));
   FR_CHECK(hipMalloc(&s->dres, sizeof(double) * 2));
   FR_CHECK(hipHostMalloc(&s->hres, sizeof(double) * 2, hipHostMallocDefault));
   for (int j = 0; j < fr_box_poisson::QB; j++) {
