@@ -41,6 +41,7 @@ struct PoisDev {
   int lwx, lwy, lwz;   // their log2
   int lxz;             // forward x pass: log2 of the z planes of its (y, z) row tile
   int wxi, lwxi;       // inverse x pass: rows per workgroup, log2
+  int lgx, lgxi;       // log2 of the neighbouring tiles per XCD group (xcd_group), fwd/inv
   double tot;          // nx*ny*nz
   double itot;         // 1/tot, exact (tot is a power of two): x*itot == x/tot
 };
@@ -89,7 +90,7 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
   // the same time across the XCDs (measured: an XCD-contiguous order fetches
   // 16% more and runs 8% slower here).
   const int lwy_ = q.lwx - q.lxz;
-  const unsigned bid = blockIdx.x;
+  const unsigned bid = xcd_group(blockIdx.x, q.lgx, gridDim.x);
   const int j0 = (int)(bid & ((1u << (q.ly - lwy_)) - 1)) << lwy_;
   const int k0 = (int)(bid >> (q.ly - lwy_)) << q.lxz;
   for (int t = threadIdx.x; t < W * M; t += FT) {
@@ -331,7 +332,7 @@ __global__ void __launch_bounds__(FT) k_pass_x_inv(PoisDev q,
   load_tw(twn, twN, q.nx);
   __syncthreads();
   const int rows = q.ny * q.nz, lm = q.lxm, lw = q.lwxi;
-  const unsigned bid = blockIdx.x;
+  const unsigned bid = xcd_group(blockIdx.x, q.lgxi, gridDim.x);
   const int row0 = (int)bid * W;
   for (int t = threadIdx.x; t < W * M; t += FT) {
     const int k = t >> lw, p = t & (W - 1);  // pencil fastest: consecutive y in the spectrum
@@ -572,6 +573,19 @@ static int lds_budget_f() {
   return want;
 }
 
+// log2 of the neighbouring x-pass tiles that share an XCD (xcd_group): the
+// forward pass's tiles share their y-neighbour rows, the inverse pass's
+// 4-row tiles the 128-byte segments of the spectrum (4 rows read half a
+// segment each).  Measured at 512^3 (serialised launches): forward 1.529 ms
+// ungrouped, 1.494 in groups of 4 tiles, 1.568 of 16; inverse 0.798 ms
+// ungrouped, 0.717 in groups of 8, 0.701 of 32, 0.697 of 64.  FR_FFT_XG and
+// FR_FFT_XGI override the defaults (2 and 5) for tuning.
+static int env_int(const char* name, int dflt) {
+  const char* e = getenv(name);
+  const int v = e ? atoi(e) : dflt;
+  return v < 0 ? 0 : v > 6 ? 6 : v;
+}
+
 static int x_tile_planes() {
   int l = 0;
   if (const char* e = getenv("FR_FFT_XZ")) l = atoi(e);
@@ -601,6 +615,8 @@ static int ensure_f32(fr_box_poisson* s) {
   q = s->q;
   plan_passes(q, (int)sizeof(float2), lds_budget(), lds_budget_f(), x_tile_planes(), s->lds32,
               &s->nbx32, &s->nbxi32);
+  q.lgx = env_int("FR_FFT_XG", 2);
+  q.lgxi = env_int("FR_FFT_XGI", 5);
   FR_RET(allow_lds<float>(std::max({s->lds32[0], s->lds32[1], s->lds32[2], s->lds32[3]})));
   const int n[3] = {q.nx, q.ny, q.nz};
   for (int a = 0; a < 3; a++) {
@@ -728,6 +744,8 @@ int fr_box_poisson_create(const fr_geom* g, fr_box_poisson** out) {
   }
   plan_passes(q, (int)sizeof(double2), lds_budget(), lds_budget_f(), x_tile_planes(), s->lds,
               &s->nbx, &s->nbxi);
+  q.lgx = env_int("FR_FFT_XG", 2);
+  q.lgxi = env_int("FR_FFT_XGI", 5);
   {
     const int rc = allow_lds<double>(std::max({s->lds[0], s->lds[1], s->lds[2], s->lds[3]}));
     if (rc) return rc;

@@ -97,6 +97,19 @@ __device__ __forceinline__ uint3 xcd_block() {
   return make_uint3(lin % nbx, (lin / nbx) % nby, lin / (nbx * nby));
 }
 
+// Renumbers a linear workgroup id (of nb) so that 2^lg consecutive logical
+// ids run on one XCD at nearly the same time: hardware id b runs on XCD
+// b % 8, and within each full group of 8 * 2^lg ids, b maps to
+// (b % 8) * 2^lg + (b / 8) % 2^lg.  Logical ids a multiple of 8 * 2^lg apart
+// keep sharing an XCD, as in plain order; a last partial group keeps its
+// ids.  A bijection for every lg >= 0 (lg = 0 is the identity).
+__device__ __forceinline__ unsigned xcd_group(unsigned b, unsigned lg, unsigned nb) {
+  const unsigned g8 = 8u << lg;
+  const unsigned base = b & ~(g8 - 1);
+  if (base + g8 > nb) return b;
+  return base | ((b & 7u) << lg) | ((b >> 3) & ((1u << lg) - 1));
+}
+
 inline hipStream_t S(void* s) { return (hipStream_t)s; }
 
 inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
