@@ -251,52 +251,24 @@ __global__ void __launch_bounds__(256) k_update_ru_wop_z(
   }
 }
 
-// One cell of k_div_rhs (any ghosted cell: a ghost takes its image's
-// values); returns the RHS_Pois value written.
-template <typename T>
-__device__ __forceinline__ double div_rhs_cell(const Dev& d, const DivCoef& q,
-                                               const T* __restrict__ rn, const T* __restrict__ Q,
-                                               const T* __restrict__ rnx,
-                                               const T* __restrict__ rny,
-                                               const T* __restrict__ rnz, T* __restrict__ rfx,
-                                               T* __restrict__ rfy, T* __restrict__ rfz,
-                                               double* __restrict__ rhs, int i, int j, int k) {
-  const long n = at(d, i, j, k);
-  const long c = at(d, img(i, d.Nx), img(j, d.Ny), img(k, d.Nz));
-  const long sx = 1, sy = d.Nx, sz = d.NxNy;
-  // The right-hand side is formed in double from fields of either precision:
-  // it is the small difference of large face fluxes, and the pressure solve
-  // iterates to 1e-10 on it.
-  auto v = [](const T* a, long p) { return (double)a[p]; };
-  const double h = 0.5;
-  auto f = [&](long p) { return 1. / v(rn, p); };
-  const double fc = f(c);
-  const double del2 = (f(c + sx) - 2 * fc + f(c - sx)) * d.idx2 +
-                      (f(c + sy) - 2 * fc + f(c - sy)) * d.idy2 +
-                      (f(c + sz) - 2 * fc + f(c - sz)) * d.idz2;
-  const double qq = q.cooling ? v(Q, c) - q.qmean : v(Q, c);
-  const double dvg = q.kc * del2 + q.bq * qq;
-  // Rho_face = I_C2F(Rho_new) at the three lower faces and the upper ones
-  const double fx0 = (v(rn, c) + v(rn, c - sx)) * h, fx1 = (v(rn, c + sx) + v(rn, c)) * h;
-  const double fy0 = (v(rn, c) + v(rn, c - sy)) * h, fy1 = (v(rn, c + sy) + v(rn, c)) * h;
-  const double fz0 = (v(rn, c) + v(rn, c - sz)) * h, fz1 = (v(rn, c + sz) + v(rn, c)) * h;
-  rfx[n] = (T)fx0;
-  rfy[n] = (T)fy0;
-  rfz[n] = (T)fz0;
-  const double D = (v(rnx, c + sx) / fx1 - v(rnx, c) / fx0) * d.idx +
-                   (v(rny, c + sy) / fy1 - v(rny, c) / fy0) * d.idy +
-                   (v(rnz, c + sz) / fz1 - v(rnz, c) / fz0) * d.idz;
-  double r = D - dvg;
-  r = r * q.s;
-  rhs[n] = r;
-  return r;
-}
-
 // grid::Compute_Div_U_new (grid.cpp:556-563) + grid::Compute_RHS_Pois
 // (grid.cpp:629-637) up to the zero-mean shift: writes Rho_face = I_C2F(
-// Rho_new) and RHS_Pois = (Div_F2C(RU_new/Rho_face) - divergence)/(dt*pre).
-// With part: the workgroup's sum of RHS_Pois over its interior cells (the
-// first level of the mean's reduction, in a fixed order) into part[tile].
+// Rho_new) and RHS_Pois = (Div_F2C(RU_new/Rho_face) - divergence)/(dt*pre)
+// at every ghosted cell (a ghost takes its image's values).  The right-hand
+// side is formed in double from fields of either precision: it is the small
+// difference of large face fluxes, and the pressure solve iterates to 1e-10
+// on it.
+//
+// A workgroup is a 64x4 column tile marching through DKZ planes: Rho_new at
+// the planes below and above, its reciprocals, the upper z face value and
+// the z flux RU_new_z/Rho_face_z of the upper face are the next plane's
+// lower ones, so each is loaded or formed once (the same expressions, so
+// the same values).  The march reloads where the image plane sequence jumps
+// (the periodic ghost planes).  With part: per plane, the tile's sum of
+// RHS_Pois over its interior cells into part[(k*gy + ty)*gx + tx], summed
+// later in a fixed order.
+constexpr int DKZ = 8;
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_div_rhs(Dev d, DivCoef q, const T* __restrict__ rn,
                                                  const T* __restrict__ Q,
@@ -306,24 +278,86 @@ __global__ void __launch_bounds__(256) k_div_rhs(Dev d, DivCoef q, const T* __re
                                                  T* __restrict__ rfy, T* __restrict__ rfz,
                                                  double* __restrict__ rhs,
                                                  double* __restrict__ part) {
+  __shared__ double wsum[2][4];
   const uint3 bk = xcd_block();
-  const int i = bk.x * 64 + threadIdx.x, j = bk.y * 4 + threadIdx.y, k = bk.z;
-  double r = 0.0;
-  if (i < d.Nx && j < d.Ny)
-    r = div_rhs_cell(d, q, rn, Q, rnx, rny, rnz, rfx, rfy, rfz, rhs, i, j, k);
-  if (!part) return;
-  const bool in = i >= 1 && i <= d.Nx - 2 && j >= 1 && j <= d.Ny - 2 && k >= 1 && k <= d.Nz - 2;
-  double v = in ? r : 0.0;
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  __shared__ double wsum[4];
+  const int i = bk.x * 64 + threadIdx.x, j = bk.y * 4 + threadIdx.y;
+  const bool live = i < d.Nx && j < d.Ny;
+  const long sx = 1, sy = d.Nx, sz = d.NxNy;
+  const long cxy = live ? (long)img(j, d.Ny) * d.Nx + img(i, d.Nx) : 0;
   const int tid = threadIdx.y * 64 + threadIdx.x;
-  if ((tid & 63) == 0) wsum[tid >> 6] = v;
-  __syncthreads();
-  if (tid == 0)
-    part[((long)bk.z * gridDim.y + bk.y) * gridDim.x + bk.x] =
-        (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+  auto v = [](const T* a, long p) { return (double)a[p]; };
+  const double h = 0.5;
+  double rm = 0, rc = 0, rp = 0;  // Rho_new at c - sz, c, c + sz
+  double im = 0, ic = 0, ip = 0;  // their reciprocals
+  double fz1 = 0, qz1 = 0;        // upper z face value and RU_new_z / face there
+  int prev = -2;
+  const int k0 = bk.z * DKZ, k1 = min(k0 + DKZ, d.Nz);
+  for (int k = k0; k < k1; k++) {
+    const int ck = img(k, d.Nz);
+    const long c = (long)ck * sz + cxy;
+    double fz0 = 0, qz0 = 0;
+    if (ck == prev + 1) {
+      rm = rc;
+      rc = rp;
+      im = ic;
+      ic = ip;
+      fz0 = fz1;
+      qz0 = qz1;
+      if (live) {
+        rp = v(rn, c + sz);
+        ip = 1. / rp;
+        fz1 = (rp + rc) * h;
+        qz1 = v(rnz, c + sz) / fz1;
+      }
+    } else if (live) {
+      rm = v(rn, c - sz);
+      rc = v(rn, c);
+      rp = v(rn, c + sz);
+      im = 1. / rm;
+      ic = 1. / rc;
+      ip = 1. / rp;
+      fz0 = (rc + rm) * h;
+      qz0 = v(rnz, c) / fz0;
+      fz1 = (rp + rc) * h;
+      qz1 = v(rnz, c + sz) / fz1;
+    }
+    prev = ck;
+    double r = 0.0;
+    if (live) {
+      const long n = at(d, i, j, k);
+      const double rxp = v(rn, c + sx), rxm = v(rn, c - sx);
+      const double ryp = v(rn, c + sy), rym = v(rn, c - sy);
+      const double fc = ic;
+      const double del2 = (1. / rxp - 2 * fc + 1. / rxm) * d.idx2 +
+                          (1. / ryp - 2 * fc + 1. / rym) * d.idy2 +
+                          (ip - 2 * fc + im) * d.idz2;
+      const double qq = q.cooling ? v(Q, c) - q.qmean : v(Q, c);
+      const double dvg = q.kc * del2 + q.bq * qq;
+      // Rho_face = I_C2F(Rho_new) at the three lower faces and the upper ones
+      const double fx0 = (rc + rxm) * h, fx1 = (rxp + rc) * h;
+      const double fy0 = (rc + rym) * h, fy1 = (ryp + rc) * h;
+      rfx[n] = (T)fx0;
+      rfy[n] = (T)fy0;
+      rfz[n] = (T)fz0;
+      const double D = (v(rnx, c + sx) / fx1 - v(rnx, c) / fx0) * d.idx +
+                       (v(rny, c + sy) / fy1 - v(rny, c) / fy0) * d.idy + (qz1 - qz0) * d.idz;
+      r = D - dvg;
+      r = r * q.s;
+      rhs[n] = r;
+    }
+    if (part) {
+      const bool in = i >= 1 && i <= d.Nx - 2 && j >= 1 && j <= d.Ny - 2 && k >= 1 &&
+                      k <= d.Nz - 2;
+      double w = in ? r : 0.0;
+      for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+      if ((tid & 63) == 0) wsum[k & 1][tid >> 6] = w;
+      __syncthreads();
+      if (tid == 0)
+        part[((long)k * gridDim.y + bk.y) * gridDim.x + bk.x] =
+            (wsum[k & 1][0] + wsum[k & 1][1]) + (wsum[k & 1][2] + wsum[k & 1][3]);
+    }
+  }
 }
-
 
 // grid::Update_RU_WP (grid.cpp:651-657)
 // The pressure gradient is formed from the FP64 pressure in double.
@@ -398,8 +432,9 @@ int box_div_rhs(const Dev& d, const DivCoef& q, const T* rho_new, const T* Q,
                 const T* const ru_new[3], T* const rf[3], double* rhs, hipStream_t s,
                 double* part, double* dsum) {
   const dim3 g = grid3(d);
-  hipLaunchKernelGGL(k_div_rhs<T>, g, dim3(64, 4), 0, s, d, q, rho_new, Q, ru_new[0],
-                     ru_new[1], ru_new[2], rf[0], rf[1], rf[2], rhs, part);
+  hipLaunchKernelGGL(k_div_rhs<T>, dim3(g.x, g.y, nblocks(d.Nz, DKZ)), dim3(64, 4), 0, s, d, q,
+                     rho_new, Q, ru_new[0], ru_new[1], ru_new[2], rf[0], rf[1], rf[2], rhs,
+                     part);
   FR_LAUNCH_CHECK();
   if (part && dsum) return sum_parts(part, (long)g.x * g.y * g.z, dsum, s);
   return FR_OK;
