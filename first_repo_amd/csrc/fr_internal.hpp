@@ -328,6 +328,7 @@ class PartDev {
   int *evt = nullptr, *islo = nullptr, *plo = nullptr, *dst = nullptr, *mslot = nullptr,
       *mlv = nullptr;
   int* hbuf = nullptr;
+  int* dflag = nullptr;  // device flag of the re-sort's stage check
   // tiles of the trilinear deposits left to the dense kernel (count, ids)
   int *dense_c = nullptr, *dense_w = nullptr;
   double* inj_buf = nullptr;

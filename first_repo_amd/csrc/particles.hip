@@ -1824,24 +1824,60 @@ __global__ void k_from_ref(int n, const int* __restrict__ rank, const T* __restr
   if (i < n) dst[i] = src[rank[i]];
 }
 
+// Whether stage slots 1 and 3 equal slot 0 bit for bit for every particle
+// (diff stays 0 if so).  At the top of a time step they do on every path of
+// the step (the last stage's advance writes its result to slots 0 (in the
+// box's step), 2 and 3, migration wraps slots 0, 2 and 3 alike, and the stage
+// swap moves slot 2 to slot 1), but a state set from outside need not.  Reads
+// are in slot order.
+__global__ void __launch_bounds__(256) k_stage_eq(int np, PartPtrs a, int* __restrict__ diff) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool same = true;
+  if (i < np) {
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+      const long long v0 = __double_as_longlong(a.s[q][0][i]);
+      same = same && __double_as_longlong(a.s[q][1][i]) == v0 &&
+             __double_as_longlong(a.s[q][3][i]) == v0;
+    }
+  }
+  if (__ballot(!same) && (threadIdx.x & 63) == 0) atomicOr(diff, 1);
+}
+
 // New slot u takes the particle of reference index r = list[u] (cell
 // order) from slot inv[r]; inv[r] becomes u (list is a permutation, so each
-// thread reads and writes only its own inv entry).
+// thread reads and writes only its own inv entry).  The reads are scattered
+// (a particle that changed its cell row or plane since the last re-sort
+// comes from far away, and each value read costs a cache line): with *diff
+// clear, slot 0 is read once and written to slots 0, 1 and 3.  (Reading in
+// slot order and scattering the writes instead measured slower: 16.9 against
+// 10.3 ms at 512^3 / 16M.)
 // Runs only at the top of the first RK stage's particle update, right before
 // the advance.
 __global__ void k_resort(int np, const int* __restrict__ list, int* __restrict__ inv,
-                         PartPtrs a, PartPtrs b, int* __restrict__ rb) {
+                         PartPtrs a, PartPtrs b, int* __restrict__ rb,
+                         const int* __restrict__ diff) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= np) return;
   const int r = list[u];
   const int o = inv[r];
   // Stage slot 2 and rhs_u,v,w are written by the advance that follows
   // before anything reads them; slots 0, 1, 3 and rhs_T carry over.
+  if (*diff == 0) {
 #pragma unroll
-  for (int q = 0; q < 7; q++) {
-    b.s[q][0][u] = a.s[q][0][o];
-    b.s[q][1][u] = a.s[q][1][o];
-    b.s[q][3][u] = a.s[q][3][o];
+    for (int q = 0; q < 7; q++) {
+      const double v = a.s[q][0][o];
+      b.s[q][0][u] = v;
+      b.s[q][1][u] = v;
+      b.s[q][3][u] = v;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+      b.s[q][0][u] = a.s[q][0][o];
+      b.s[q][1][u] = a.s[q][1][o];
+      b.s[q][3][u] = a.s[q][3][o];
+    }
   }
   b.rhs_T[u] = a.rhs_T[o];
   b.flag[u] = a.flag[o];
@@ -1930,6 +1966,7 @@ int PartDev::alloc(int capacity, const Dev& d) {
   int** m[11] = {&flg, &nl, &pNL, &holepos, &tailnl, &evt, &islo, &plo, &dst, &mslot, &mlv};
   for (auto pp : m) FR_CHECK(hipMalloc(pp, sizeof(int) * (cap + 1)));
   FR_CHECK(hipHostMalloc(&hbuf, sizeof(int) * 4, hipHostMallocDefault));
+  FR_CHECK(hipMalloc(&dflag, sizeof(int) * 4));
   const long ntile = (long)nblocks(d.Nx, DTX) * nblocks(d.Ny, DTY) * d.Nz;
   FR_CHECK(hipMalloc(&dense_c, sizeof(int) * (ntile + 1)));
   FR_CHECK(hipMalloc(&dense_w, sizeof(int) * (ntile + 1)));
@@ -1949,7 +1986,7 @@ void PartDev::release() {
   void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, alt.flag, alt.rhs_u, alt.rhs_v, alt.rhs_w,
                  alt.rhs_T, rank, rank_alt, inv, mig_tmp, rank_dep, flg, nl, pNL,
                  holepos, tailnl, evt, islo, plo, dst, mslot, mlv, inj_buf, dense_c, dense_w,
-                 gas[0], gas[1], gas[2], gas[3]};
+                 gas[0], gas[1], gas[2], gas[3], dflag};
   for (void* p : all) (void)hipFree(p);
   for (auto& g : gas) g = nullptr;
   gas_cap = 0;
@@ -2269,8 +2306,11 @@ int PartDev::resort(const Dev& d, hipStream_t st) {
   if (rc) return rc;
   PartPtrs a = ptrs(), b = alt;
   b.np = np;
+  int* diff = dflag;
+  FR_CHECK(hipMemsetAsync(diff, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_stage_eq, dim3(nblocks(np, 256)), dim3(256), 0, st, np, a, diff);
   hipLaunchKernelGGL(k_resort, dim3(nblocks(np, 256)), dim3(256), 0, st, np, T.list, inv, a, b,
-                     rank_alt);
+                     rank_alt, diff);
   FR_LAUNCH_CHECK();
   swap_storage();
   T.valid = false;

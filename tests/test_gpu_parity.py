@@ -455,6 +455,42 @@ def test_clustered_deposits_match_oracle(tmp_path, form):
     assert worst["C"] == 0.0, worst["C"]
 
 
+def test_resort_moves_distinct_stage_slots(tmp_path):
+    """The storage re-sort at the top of a time step reads stage slot 0 only
+    when slots 1 and 3 equal it bit for bit (every step of the solver leaves
+    them so), and all three slots otherwise.  Here a synced state has slots 1
+    and 3 apart from slot 0; the first stage's particle update, which
+    re-sorts, still matches the oracle (particles bit-exact in order)."""
+    params, d = make_box_case(tmp_path, n=16, n_particles=3000, seed=11)
+    og = OL.OracleGrid(params, d)
+    og.initialize()
+    og.step()
+    rng = np.random.default_rng(4)
+    Ls = (og.g.Lx, og.g.Ly, og.g.Lz)
+    for q in range(7):
+        scale = 1e-3 * (Ls[q] if q < 3 else 1.0)
+        for s in (1, 3):
+            a = og.part(q, s)
+            # positions stay inside the box (a particle outside it has no
+            # defined gather cell)
+            keep = (a > 2 * scale) & (a < Ls[q] - 2 * scale) if q < 3 else np.ones(a.size, bool)
+            a[keep] += rng.uniform(-scale, scale, int(keep.sum()))
+    sim = fr.Sim(params, d)
+    sim.set_resort_interval(1)
+    sim.initialize()
+    sync_oracle_to_device(og, sim)
+    og.call("update")
+    sim.call("update")
+    for m in ("Update_Rho", "Update_P0", "Update_Particle"):
+        og.call(m, 0)
+        sim.call(m, 0)
+    f1, f0, parts = OUTPUTS["Update_Particle"]
+    worst, bad = compare_state(og, sim, tol=1e-12, fields1=f1, fields0=f0, particles=parts,
+                               stages=(0, 1, 2, 3))
+    assert not bad, bad
+    assert np.array_equal(sim.flags(), og.flag())
+
+
 def test_pressure_solve_64_matches_oracle(tmp_path):
     """grid::Solve_Poisson on a 64^3 box (C = 1/Rho_face formed inside the
     residual of the x pass) after the substep's earlier methods, from an
