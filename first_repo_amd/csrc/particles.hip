@@ -524,28 +524,38 @@ __global__ void k_bucket_gather(int np, int mode, PartPtrs p, int st,
   }
 }
 
-// The same operands as one record per bucket position (the split-form
-// deposits): mode 0 (x_l, y_l, z_l, RHS_T), mode 1 (x_l, RHS_u, y_l, RHS_v,
-// z_l, RHS_w); same arithmetic as k_bucket_gather.
-__global__ void k_bucket_gather_rec(int np, int mode, PartPtrs p, int st,
-                                    const int* __restrict__ posv, double idx, double idy,
-                                    double idz, double* __restrict__ rec) {
+// Split-form builds: k_bucket_place and the operand gather in one pass,
+// each particle's operand record (mode 0: x_l, y_l, z_l, RHS_T; mode 1:
+// x_l, RHS_u, y_l, RHS_v, z_l, RHS_w; the arithmetic of k_bucket_gather)
+// written at its placed position.  Without OPS only the placement.
+template <bool OPS>
+__global__ void k_bucket_place_rec(int np, int mode, PartPtrs p, int st,
+                                   const int* __restrict__ key, const int* __restrict__ start,
+                                   const int* __restrict__ rank, const int* __restrict__ unsorted,
+                                   int* __restrict__ list, double idx, double idy, double idz,
+                                   double* __restrict__ rec) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  const long pos = posv[i];
+  const int c = key[i], r = rank[i];
+  const int b = start[c], e = start[c + 1];
+  int n = 0;
+  for (int u = b; u < e; u++) n += unsorted[u] < r;
+  const long pos = b + n;
+  list[pos] = r;
+  if (!OPS) return;
   const double X = p.s[0][st][i], Y = p.s[1][st][i], Z = p.s[2][st][i];
   if (mode == 0) {
     const int ii = (int)(round(X * idx));
     const int jj = (int)(round(Y * idy));
     const int kk = (int)(round(Z * idz));
-    double2* r = reinterpret_cast<double2*>(rec + 4 * pos);
-    r[0] = make_double2(X * idx - ii + 0.5, Y * idy - jj + 0.5);
-    r[1] = make_double2(Z * idz - kk + 0.5, p.rhs_T[i]);
+    double2* q = reinterpret_cast<double2*>(rec + 4 * pos);
+    q[0] = make_double2(X * idx - ii + 0.5, Y * idy - jj + 0.5);
+    q[1] = make_double2(Z * idz - kk + 0.5, p.rhs_T[i]);
   } else {
-    double2* r = reinterpret_cast<double2*>(rec + 6 * pos);
-    r[0] = make_double2(X * idx - ((int)(X * idx)), p.rhs_u[i]);
-    r[1] = make_double2(Y * idy - ((int)(Y * idy)), p.rhs_v[i]);
-    r[2] = make_double2(Z * idz - ((int)(Z * idz)), p.rhs_w[i]);
+    double2* q = reinterpret_cast<double2*>(rec + 6 * pos);
+    q[0] = make_double2(X * idx - ((int)(X * idx)), p.rhs_u[i]);
+    q[1] = make_double2(Y * idy - ((int)(Y * idy)), p.rhs_v[i]);
+    q[2] = make_double2(Z * idz - ((int)(Z * idz)), p.rhs_w[i]);
   }
 }
 
@@ -2089,15 +2099,23 @@ int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool opera
                        mode, B.cnt, B.key, B.slot);
   int rc = scan_ex(B.cnt, ncell, B.start, B.scan_tmp, st);
   if (rc) return rc;
-  if (np > 0) {
+  if (np > 0 && B.rec_form) {
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
+                       B.slot, B.start, rk, B.list2);
+    if (operands)
+      hipLaunchKernelGGL(k_bucket_place_rec<true>, dim3(nblocks(np, 256)), dim3(256), 0, st, np,
+                         mode, ptrs(), stage, B.key, B.start, rk, B.list2, B.list, d.idx, d.idy,
+                         d.idz, B.rec);
+    else
+      hipLaunchKernelGGL(k_bucket_place_rec<false>, dim3(nblocks(np, 256)), dim3(256), 0, st, np,
+                         mode, ptrs(), stage, B.key, B.start, rk, B.list2, B.list, d.idx, d.idy,
+                         d.idz, B.rec);
+  } else if (np > 0) {
     hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
                        B.slot, B.start, rk, B.list2);
     hipLaunchKernelGGL(k_bucket_place, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
                        B.start, rk, B.list2, B.list, B.slot);
-    if (operands && B.rec_form)
-      hipLaunchKernelGGL(k_bucket_gather_rec, dim3(nblocks(np, 256)), dim3(256), 0, st, np, mode,
-                         ptrs(), stage, B.slot, d.idx, d.idy, d.idz, B.rec);
-    else if (operands)
+    if (operands)
       hipLaunchKernelGGL(k_bucket_gather, dim3(nblocks(np, 256)), dim3(256), 0, st, np, mode,
                          ptrs(), stage, B.slot, d.idx, d.idy, d.idz, B.s[0], B.s[1], B.s[2],
                          B.s[3], B.s[4], B.s[5]);
