@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) k_update_ru_wop(
   }
 }
 
-// The same update, z-marching: a workgroup owns a 64 x 4 column of interior
+// The same update, z-marching: a workgroup owns a 64 x 8 column of interior
 // cells over KZ planes and keeps planes k-1, k, k+1 of the six stencil
 // arrays (U and RU_int, with a one-cell x/y halo) in an LDS ring; plane k+2
 // is fetched into registers while plane k is computed, so every U/RU_int
@@ -127,12 +127,12 @@ __global__ void __launch_bounds__(256) k_update_ru_wop(
 // cells are written; the caller restores the periodic ghosts of the six
 // outputs afterwards (a ghost's value is its image's, as k_update_ru_wop
 // computes it).
-constexpr int WX = 64, WY = 4, WKZ = 16, WHX = WX + 2, WHY = WY + 2;
+constexpr int WX = 64, WY = 8, WKZ = 16, WHX = WX + 2, WHY = WY + 2;
 constexpr int WPL = 6 * WHY * WHX;               // values per plane of the ring
 constexpr int WPT = (WPL + WX * WY - 1) / (WX * WY);  // per thread
 
 template <int GRAV, typename T>
-__global__ void __launch_bounds__(256) k_update_ru_wop_z(
+__global__ void __launch_bounds__(WX * WY) k_update_ru_wop_z(
     Dev d, BoxCoef cf, WopCoef w, const T* __restrict__ ux, const T* __restrict__ uy,
     const T* __restrict__ uz, const T* __restrict__ rux, const T* __restrict__ ruy,
     const T* __restrict__ ruz, const T* __restrict__ sxp, const T* __restrict__ syp,
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(256) k_update_ru_wop_z(
   const int i0 = 1 + bk.x * WX, j0 = 1 + bk.y * WY, k0 = 1 + bk.z * WKZ;
   const int k1 = min(d.Nz - 1, k0 + WKZ);
   const int tid = threadIdx.y * WX + threadIdx.x;
-  // this thread's share of a plane: value t = tid + 256 q -> (array, y, x)
+  // this thread's share of a plane: value t = tid + WX*WY q -> (array, y, x)
   auto fetch = [&](int kk, T* reg) {
 #pragma unroll
     for (int q = 0; q < WPT; q++) {
