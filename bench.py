@@ -47,12 +47,12 @@ METRIC = ("Mcell·steps/s full RK4 step w/ particles; "
 # HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE (each scaled
 # by what the counters read for a plain copy of known size: factors 2.0 and
 # 1.0), measured on the named workloads (profiles/hbm_traffic_*.md)
-TRAFFIC = {("k_pass_x_fwd", 256, 4_000_000, "fp64"): 1.051e9,
-           ("k_pass_x_fwd", 512, 16_000_000, "fp32"): 6.437e9,
-           ("k_pass_x_fwd", 512, 16_000_000, "fp64"): 9.500e9,
-           ("k_update_ru_wop_z", 512, 16_000_000, "fp64"): 26.063e9,
-           ("k_dep_vel", 512, 16_000_000, "fp64"): 5.500e9,
-           ("k_dep_cic", 512, 16_000_000, "fp64"): 3.209e9}
+TRAFFIC = {("k_pass_x_fwd", 512, 16_000_000, "fp32"): 5.076e9,
+           ("k_update_ru_wop_z", 512, 16_000_000, "fp32"): 12.042e9,
+           ("k_pass_x_fwd", 512, 16_000_000, "fp64"): 7.974e9,
+           ("k_update_ru_wop_z", 512, 16_000_000, "fp64"): 24.793e9,
+           ("k_dep_vel", 512, 16_000_000, "fp64"): 5.248e9,
+           ("k_dep_cic", 512, 16_000_000, "fp64"): 3.340e9}
 
 
 def box_workload(n, particles, solver="fft"):
