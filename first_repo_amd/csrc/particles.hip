@@ -1386,6 +1386,9 @@ __global__ void __launch_bounds__(SNT) k_dep_cic_split(Dev d, const int* __restr
                                                        T* __restrict__ out) {
   __shared__ int wt[4];
   __shared__ int hl[SNT * SKZ];
+  // the tile's outputs, stored to memory together once its merged targets
+  // are in (whole cache lines, not lines with the merged targets as holes)
+  __shared__ T ob[SKZ][SNT];
   const uint3 bk = xcd_block();
   const int tid = threadIdx.x;
   const int ij = bk.x * SNT + tid, c0 = bk.y * SKZ;
@@ -1428,7 +1431,7 @@ __global__ void __launch_bounds__(SNT) k_dep_cic_split(Dev d, const int* __restr
           acc = acc + t1;
           if (two) acc = acc + t2;
         }
-        out[(long)c * d.NxNy + ij] = (T)acc;
+        ob[k][tid] = (T)acc;
       } else {
         hard |= 1u << k;
       }
@@ -1446,7 +1449,13 @@ __global__ void __launch_bounds__(SNT) k_dep_cic_split(Dev d, const int* __restr
     const int e = hl[i];
     const int ij2 = bk.x * SNT + (e & 255), c = c0 + (e >> 8);
     const int b2 = ij2 / d.Nx, a2 = ij2 - b2 * d.Nx;
-    out[(long)c * d.NxNy + ij2] = (T)dep_cic_target_rec<W>(d, start, list, ops, a2, b2, c);
+    ob[e >> 8][e & 255] = (T)dep_cic_target_rec<W>(d, start, list, ops, a2, b2, c);
+  }
+  __syncthreads();
+  if (col) {
+#pragma unroll
+    for (int k = 0; k < SKZ; k++)
+      if (c0 + k < d.Nz) out[(long)(c0 + k) * d.NxNy + ij] = ob[k][tid];
   }
 }
 
