@@ -436,7 +436,10 @@ int box_div_rhs(const Dev& d, const DivCoef& q, const T* rho_new, const T* Q,
                      rho_new, Q, ru_new[0], ru_new[1], ru_new[2], rf[0], rf[1], rf[2], rhs,
                      part);
   FR_LAUNCH_CHECK();
-  if (part && dsum) return sum_parts(part, (long)g.x * g.y * g.z, dsum, s);
+  // dsum[0]: the interior sum; dsum[1]: minus the interior mean
+  if (part && dsum)
+    return sum_parts(part, (long)g.x * g.y * g.z, dsum, s,
+                     (double)(d.Nx - 2) * (d.Ny - 2) * (d.Nz - 2));
   return FR_OK;
 }
 

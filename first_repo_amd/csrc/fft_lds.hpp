@@ -244,17 +244,20 @@ __device__ __forceinline__ double residual_v(const Dev& d, double Cx0, double Cx
   return (D - rhs) * cc;
 }
 
+// sh: the right-hand side is RHS[c] + a (a = minus its interior mean, not
+// yet applied to the field: the same value the field takes when it is).
 template <bool INV, typename TC = double>
 __device__ __forceinline__ double residual_t(const Dev& d, const double* __restrict__ P,
                                              const TC* __restrict__ cx,
                                              const TC* __restrict__ cy,
                                              const TC* __restrict__ cz,
-                                             const double* __restrict__ RHS, long c) {
+                                             const double* __restrict__ RHS, long c,
+                                             bool sh = false, double a = 0.0) {
   const long sx = 1, sy = d.Nx, sz = d.NxNy;
   auto ld = [](const TC* a, long q) { return INV ? 1. / (double)a[q] : (double)a[q]; };
   return residual_v(d, ld(cx, c), ld(cx, c + sx), ld(cy, c), ld(cy, c + sy), ld(cz, c),
                     ld(cz, c + sz), P[c], P[c + sx], P[c - sx], P[c + sy], P[c - sy], P[c + sz],
-                    P[c - sz], RHS[c]);
+                    P[c - sz], sh ? RHS[c] + a : RHS[c]);
 }
 
 __device__ __forceinline__ double residual(const Dev& d, const double* __restrict__ P,

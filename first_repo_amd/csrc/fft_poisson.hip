@@ -71,11 +71,16 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
                                                    const typename Cx<R>::T* __restrict__ twM,
                                                    const typename Cx<R>::T* __restrict__ twN,
                                                    typename Cx<R>::T* __restrict__ out,
-                                                   const int* __restrict__ gate) {
+                                                   const int* __restrict__ gate,
+                                                   const double* __restrict__ rsum) {
   using C2 = typename Cx<R>::T;
   extern __shared__ __align__(16) unsigned char smraw[];
   C2* sm = reinterpret_cast<C2*>(smraw);
   if (*gate) return;  // the solve has converged: a queued iteration does nothing
+  // rsum: the right-hand side's interior sum and minus its mean (rsum[1],
+  // the value make_mean_zero adds), the mean not yet removed from the field;
+  // the shift is read where it is added, after the residual's loads
+  const bool sh = rsum != nullptr;
   const Dev& d = q.d;
   const int M = q.nx >> 1, W = q.wx, ps = M + 1;
   C2* buf = sm;
@@ -99,8 +104,9 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
     double r0 = 0.0, r1 = 0.0;
     if (kk < q.nz) {
       const long c = at(d, 2 * m + 1, jj + 1, kk + 1);
-      r0 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c);
-      r1 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c + 1);
+      const double shift = sh ? rsum[1] : 0.0;
+      r0 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c, sh, shift);
+      r1 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c + 1, sh, shift);
     }
     C2 v;
     v.x = (R)r0;
@@ -458,7 +464,7 @@ struct fr_box_poisson {
 template <typename TC, typename R>
 static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz,
                             bool inv, double* P, const double* RHS, double eps, int slot,
-                            hipStream_t st) {
+                            hipStream_t st, const double* rsum) {
   using C2 = typename Cx<R>::T;
   const bool f = sizeof(R) == 4;
   PoisDev& q = f ? s->q32 : s->q;
@@ -474,11 +480,11 @@ static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const
   if (inv)
     hipExtLaunchKernelGGL((k_pass_x_fwd<true, TC, R>), dim3(nbx), dim3(FT), lds[0], st,
                           s->tx0[slot], s->tx1[slot], 0, q, P, cx, cy, cz, RHS, twfm, twf[0],
-                          data, g);
+                          data, g, rsum);
   else
     hipExtLaunchKernelGGL((k_pass_x_fwd<false, TC, R>), dim3(nbx), dim3(FT), lds[0], st,
                           s->tx0[slot], s->tx1[slot], 0, q, P, cx, cy, cz, RHS, twfm, twf[0],
-                          data, g);
+                          data, g, rsum);
   const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.ny + q.wz - 1) / q.wz;
   // 256- and 512-point y and z transforms one pencil per wave, other lengths
   // in LDS workgroup tiles
@@ -635,7 +641,7 @@ namespace fr {
 template <typename TC>
 int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz, bool inv,
                       double* P, const double* RHS, int iterations, double epsilon, int* num_iter,
-                      hipStream_t st) {
+                      hipStream_t st, const double* rsum) {
   if (!s || !cx || !cy || !cz || !P || !RHS) return FR_EINVAL;
   // FP32 coefficients (the precision form): FP32 inner spectral solve, with
   // the FP64 residual and P of the defect correction around it
@@ -651,8 +657,10 @@ int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* c
   while (launched < iterations) {
     const int k = std::min({batch, iterations - launched, (int)fr_box_poisson::QB});
     for (int j = 0; j < k; j++) {
-      int rc = f32 ? box_poisson_iter<TC, float>(s, cx, cy, cz, inv, P, RHS, epsilon, j, st)
-                   : box_poisson_iter<TC, double>(s, cx, cy, cz, inv, P, RHS, epsilon, j, st);
+      int rc = f32 ? box_poisson_iter<TC, float>(s, cx, cy, cz, inv, P, RHS, epsilon, j, st,
+                                                 rsum)
+                   : box_poisson_iter<TC, double>(s, cx, cy, cz, inv, P, RHS, epsilon, j, st,
+                                                  rsum);
       if (rc) return rc;
     }
     launched += k;
@@ -676,10 +684,10 @@ int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* c
 }
 template int box_poisson_solve<double>(fr_box_poisson*, const double*, const double*,
                                        const double*, bool, double*, const double*, int, double,
-                                       int*, hipStream_t);
+                                       int*, hipStream_t, const double*);
 template int box_poisson_solve<float>(fr_box_poisson*, const float*, const float*, const float*,
                                       bool, double*, const double*, int, double, int*,
-                                      hipStream_t);
+                                      hipStream_t, const double*);
 
 void box_poisson_xtimer(fr_box_poisson* s, double* ms, long* n, bool reset) {
   if (!s) return;
@@ -795,7 +803,7 @@ int fr_box_poisson_solve(fr_box_poisson* s, const double* cx, const double* cy,
                          const double* cz, double* P, const double* RHS, int iterations,
                          double epsilon, int* num_iter, void* stream) {
   return fr::box_poisson_solve(s, cx, cy, cz, false, P, RHS, iterations, epsilon, num_iter,
-                               S(stream));
+                               S(stream), nullptr);
 }
 
 }  // extern "C"

@@ -420,8 +420,10 @@ template int make_mean_zero_async<double>(const Dev&, double*, long, double*, hi
 template int make_mean_zero_async<float>(const Dev&, float*, long, double*, hipStream_t);
 
 constexpr int ST_SUM = 1024;
+// out[0] = the sum; with denom > 0 also out[1] = -(sum/denom), the value
+// make_mean_zero adds (k_add_neg_mean forms the same expression).
 __global__ void __launch_bounds__(ST_SUM) k_sum_parts(const double* __restrict__ part, long n,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out, double denom) {
   __shared__ double ws[ST_SUM / 64];
   double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   long t = threadIdx.x;
@@ -440,11 +442,14 @@ __global__ void __launch_bounds__(ST_SUM) k_sum_parts(const double* __restrict__
   if (threadIdx.x >= 64) return;
   r = threadIdx.x < ST_SUM / 64 ? ws[threadIdx.x] : 0.0;
   for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o);
-  if (threadIdx.x == 0) out[0] = r;
+  if (threadIdx.x == 0) {
+    out[0] = r;
+    if (denom > 0) out[1] = -(r / denom);
+  }
 }
 
-int sum_parts(const double* part, long n, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(ST_SUM), 0, s, part, n, out);
+int sum_parts(const double* part, long n, double* out, hipStream_t s, double denom) {
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(ST_SUM), 0, s, part, n, out, denom);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }

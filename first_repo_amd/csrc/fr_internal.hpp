@@ -35,7 +35,7 @@ template <typename T>
 int make_mean_zero_set_async(const Dev& d, T* f, T* g, long n, double* ws, bool has_c, double c,
                              hipStream_t s);
 // out[0] = the sum of part[0..n) in a fixed order (one workgroup)
-int sum_parts(const double* part, long n, double* out, hipStream_t s);
+int sum_parts(const double* part, long n, double* out, hipStream_t s, double denom = 0.0);
 // f[t] += (T)(-(dsum[0]/interior cells)) for the n elements of f: the shift
 // of make_mean_zero_async with the sum already formed
 template <typename T>
@@ -161,10 +161,14 @@ int pair_planes(const Dev& b, const double* const src[8], const int xi[8], doubl
 // fr_box_poisson_solve with, when inv is set, Rho_face passed in place of C
 // and C = 1./Rho_face formed inside the residual (grid.cpp:641).
 // TC: double, or float for the FP32 precision form (P and RHS stay double).
+// rsum (device, 2 values): when RHS still holds its mean, its interior sum
+// and minus its mean (sum_parts with a denominator); the residual then uses
+// RHS + rsum[1], the value the field takes when the mean is removed
+// (make_mean_zero).
 template <typename TC>
 int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz, bool inv,
                       double* P, const double* RHS, int iterations, double epsilon, int* num_iter,
-                      hipStream_t st);
+                      hipStream_t st, const double* rsum = nullptr);
 // Summed duration (ms, HIP events) and count of the forward x-pass launches
 // of the box solve since the last reset.
 void box_poisson_xtimer(fr_box_poisson* s, double* ms, long* n, bool reset);

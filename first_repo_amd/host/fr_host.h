@@ -386,8 +386,11 @@ class poisson {
   int init(params* p, gridsize* g, hipStream_t st);
   ~poisson();
   int Solve(tensor1& C, tensor0& P, tensor0& RHS);  // poisson.cpp:115-117
-  // Solve with C = 1./Rho_face (grid.cpp:641) without storing C
-  int Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS);  // P: FP64 storage
+  // Solve with C = 1./Rho_face (grid.cpp:641) without storing C.  rsum
+  // (device, box FFT solve only): RHS's interior sum while its mean is still
+  // in the field; the residual removes it as make_mean_zero would.
+  int Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS,
+                     const double* rsum = nullptr);  // P: FP64 storage
   int num_iteration() const { return num_iteration_; }
 };
 
@@ -511,6 +514,10 @@ class grid {
   // particle stage 0 = stage 3 in the advance), each in the pass that
   // writes the source values; the flags tell TimeAdvance which are made.
   bool in_step = false;
+  // RHS_Pois still holds its mean (removed inside the box FFT solve's
+  // residual in a time step); fill_rhs_pois() removes it from the field
+  bool rhs_mean_pending = false;
+  int fill_rhs_pois();
   bool rho_set = false, ru_set = false, part_set = false;
   hipEvent_t evc = nullptr;
   int launch_side();

@@ -420,7 +420,8 @@ int poisson::Solve(tensor1& C, tensor0& P, tensor0& RHS) {
                               RHS.pointer(), par->Iteration(), par->epsilon(),
                               &num_iteration_, st_);
 }
-int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS) {
+int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS, const double* rsum) {
+  if (rsum && !s) return FR_EINVAL;  // a pending mean shift only in the box FFT solve
   if (sp)
     return pcg_solve(sp, Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer(),
                      P.pointer(), RHS.pointer(), par->Iteration(), par->epsilon(),
@@ -428,14 +429,15 @@ int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS) {
   if (s && Rho_face.x.is_f32())
     return box_poisson_solve<float>(s, Rho_face.x.p<float>(), Rho_face.y.p<float>(),
                                     Rho_face.z.p<float>(), true, P.pointer(), RHS.pointer(),
-                                    par->Iteration(), par->epsilon(), &num_iteration_, st_);
+                                    par->Iteration(), par->epsilon(), &num_iteration_, st_,
+                                    rsum);
   if (sd)
     return duct_poisson_solve(sd, Rho_face.x.pointer(), Rho_face.y.pointer(),
                               Rho_face.z.pointer(), true, P.pointer(), RHS.pointer(),
                               par->Iteration(), par->epsilon(), &num_iteration_, st_);
   return box_poisson_solve<double>(s, Rho_face.x.pointer(), Rho_face.y.pointer(),
                                    Rho_face.z.pointer(), true, P.pointer(), RHS.pointer(),
-                                   par->Iteration(), par->epsilon(), &num_iteration_, st_);
+                                   par->Iteration(), par->epsilon(), &num_iteration_, st_, rsum);
 }
 
 // ---------------------------------------------------------------- grid
@@ -937,29 +939,45 @@ int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
   const DivCoef q = div_coef(*this);
   const double* rn[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
   double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
+  rhs_mean_pending = false;
+  // sum and minus the mean, in slots no other mean's reduction writes
+  double* rsum = dmz + RED_PARTS + 4;
   if (param.IsBox() && f32) {
     const float* rn_[3] = {RU_new.x.p<float>(), RU_new.y.p<float>(), RU_new.z.p<float>()};
     float* rf_[3] = {Rho_face.x.p<float>(), Rho_face.y.p<float>(), Rho_face.z.p<float>()};
     FR_TRY(box_div_rhs<float>(size.dev(), q, Rho_new.p<float>(), RHS_Part_Temp.p<float>(), rn_,
-                              rf_, RHS_Pois.pointer(), st, dpart, dmz + RED_PARTS));
+                              rf_, RHS_Pois.pointer(), st, dpart, rsum));
   } else if (param.IsBox()) {
     FR_TRY(box_div_rhs<double>(size.dev(), q, Rho_new.pointer(), RHS_Part_Temp.pointer(), rn,
-                               rf, RHS_Pois.pointer(), st, dpart, dmz + RED_PARTS));
+                               rf, RHS_Pois.pointer(), st, dpart, rsum));
   } else {
     const double* rfc[3] = {rf[0], rf[1], rf[2]};
     FR_TRY(duct_rhs(size.dev(), q.s, rn, rfc, divergence.pointer(), RHS_Pois.pointer(), st));
     RHS_Pois.make_mean_zero_async(dmz);
     return FR_OK;
   }
-  // box: the interior sum was formed by the kernel (tensor0::make_mean_zero)
-  FR_TRY(add_neg_mean_async(size.dev(), RHS_Pois.pointer(), (long)size.size(), dmz + RED_PARTS,
-                            st));
+  // box: the interior sum was formed by the kernel (tensor0::make_mean_zero).
+  // In the time step with the FFT solve the field keeps its mean: the
+  // solve's residual, the field's only reader there, adds the same shift
+  // itself, and the field takes it only when read (fill_rhs_pois).
+  rhs_mean_pending = true;
+  if (!(in_step && PS_.s)) return fill_rhs_pois();
   return FR_OK;
+}
+
+// RHS_Pois with its mean removed, as tensor0::make_mean_zero leaves it.
+int grid::fill_rhs_pois() {
+  if (!rhs_mean_pending) return FR_OK;
+  rhs_mean_pending = false;
+  return add_neg_mean_async(size.dev(), RHS_Pois.pointer(), (long)size.size(),
+                            dmz + RED_PARTS + 4, st);
 }
 
 int grid::Solve_Poisson() {  // grid.cpp:639-643
   // C = 1./Rho_face is formed inside the solver's residual
   hipEventRecord(ev[0], st);
+  if (rhs_mean_pending && !PS_.s) FR_TRY(fill_rhs_pois());
+  const double* rsum = rhs_mean_pending ? dmz + RED_PARTS + 4 : nullptr;
   if (f32 && PS_.sp) {
     // the PCG's FP64 operator reads the FP32 faces widened (the FFT solve
     // widens them inside its residual); its V-cycles run in FP32 either way
@@ -967,7 +985,7 @@ int grid::Solve_Poisson() {  // grid.cpp:639-643
       to_double(Rho_face.c(m).p<float>(), d64b.c(m).pointer(), size.size(), st);
     FR_TRY(PS_.Solve_face_rho(d64b, P, RHS_Pois));
   } else {
-    FR_TRY(PS_.Solve_face_rho(Rho_face, P, RHS_Pois));
+    FR_TRY(PS_.Solve_face_rho(Rho_face, P, RHS_Pois, rsum));
   }
   hipEventRecord(ev[1], st);
   hipEventSynchronize(ev[1]);
@@ -1162,7 +1180,10 @@ int grid::diagnostics(double* out) {
 
 tensor0* grid::field0(const std::string& n) {
   if (n == "P") return &P;
-  if (n == "RHS_Pois") return &RHS_Pois;
+  if (n == "RHS_Pois") {
+    (void)fill_rhs_pois();
+    return &RHS_Pois;
+  }
   if (n == "C") return &C;
   if (n == "Rho") return &Rho;
   if (n == "Rho_int") return &Rho_int;
