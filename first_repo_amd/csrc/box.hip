@@ -27,8 +27,12 @@ static dim3 grid3(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 4
 // (grid.cpp:662, the first line of grid::Update_Particle).  SET (last stage
 // of a full step only): the end-of-step copy Rho = Rho_np1 of
 // grid::TimeAdvance (grid.cpp:317) is made here, into rho_set; the last
-// stage does not read Rho.
-template <typename T, bool SET>
+// stage does not read Rho.  V (the BoxCoef flags, as template arguments so
+// that each form compiles to straight-line code): RHO_PLAIN; RHO_BASE (first
+// stage, Rho stands in for Rho_np1); RHO_SET (SET above); RHO_SET_LEAN (SET,
+// and Rho_np1 itself is not written: it equals Rho from there on).
+enum { RHO_PLAIN, RHO_BASE, RHO_SET, RHO_SET_LEAN };
+template <typename T, int V>
 __global__ void __launch_bounds__(256) k_update_rho(
     Dev d, BoxCoef cf, const T* __restrict__ rux, const T* __restrict__ ruy,
     const T* __restrict__ ruz, const T* __restrict__ rho, T* __restrict__ rho_np1,
@@ -39,8 +43,14 @@ __global__ void __launch_bounds__(256) k_update_rho(
   // RHS_Rho = Div_F2C(RU_int)
   const T rhs = (rux[c + sx] - rux[c]) * (T)d.idx + (ruy[c + sy] - ruy[c]) * (T)d.idy +
                 (ruz[c + sz] - ruz[c]) * (T)d.idz;
-  const T np1 = rho_np1[n] + (T)cf.cpost_neg * rhs;
-  rho_np1[n] = np1;
+  constexpr bool SET = V >= RHO_SET;
+  T base;
+  if constexpr (V == RHO_BASE)
+    base = rho[n];
+  else
+    base = rho_np1[n];
+  const T np1 = base + (T)cf.cpost_neg * rhs;
+  if constexpr (V != RHO_SET_LEAN) rho_np1[n] = np1;
   if constexpr (SET) {
     rho_new[n] = np1;
     rho_set[n] = np1;
@@ -131,7 +141,11 @@ constexpr int WX = 64, WY = 8, WKZ = 16, WHX = WX + 2, WHY = WY + 2;
 constexpr int WPL = 6 * WHY * WHX;               // values per plane of the ring
 constexpr int WPT = (WPL + WX * WY - 1) / (WX * WY);  // per thread
 
-template <int GRAV, typename T>
+// MODE (the BoxCoef flags, as template arguments): WOP_PLAIN; WOP_BASE
+// (first stage: RU stands in for RU_np1, which is not read); WOP_LEAN (last
+// stage of a full step: RU_np1 is not written, RU_new takes its value).
+enum { WOP_PLAIN, WOP_BASE, WOP_LEAN };
+template <int GRAV, int MODE, typename T>
 __global__ void __launch_bounds__(WX * WY) k_update_ru_wop_z(
     Dev d, BoxCoef cf, WopCoef w, const T* __restrict__ ux, const T* __restrict__ uy,
     const T* __restrict__ uz, const T* __restrict__ rux, const T* __restrict__ ruy,
@@ -194,7 +208,7 @@ __global__ void __launch_bounds__(WX * WY) k_update_ru_wop_z(
       o[m] = Sd[m][c];
       o[3 + m] = RI[m][c];
       o[6 + m] = R0[m][c];
-      o[9 + m] = N1[m][c];
+      if constexpr (MODE != WOP_BASE) o[9 + m] = N1[m][c];
       if constexpr (GRAV != 0) o[12 + m] = RF[m][c];
     }
   };
@@ -242,8 +256,8 @@ __global__ void __launch_bounds__(WX * WY) k_update_ru_wop_z(
       r = r + (T)w.A * op[3 + m];
       if (m == 0) r = r - (T)w.sx_forc;
       if constexpr (GRAV != 0) r = r + (op[12 + m] - (T)w.rho0) * gv[m];
-      const T np1 = op[9 + m] + (T)cf.cpost * r;
-      N1[m][c] = np1;
+      const T np1 = (MODE == WOP_BASE ? op[6 + m] : op[9 + m]) + (T)cf.cpost * r;
+      if constexpr (MODE != WOP_LEAN) N1[m][c] = np1;
       NW[m][c] = cf.last ? np1 : 1 * op[6 + m] + (T)cf.cpre * r;
     }
 #pragma unroll
@@ -388,19 +402,119 @@ __global__ void __launch_bounds__(256) k_update_ru_wp(
   }
 }
 
+// The last stage of a full box step (BoxCoef::lean): RU_new holds the stage's
+// RU_np1 (the momentum update wrote it there only) and takes the update of
+// k_update_ru_wp in place, the same expressions.  A workgroup is a 64x4
+// column tile through PKZ planes; it also sums each component over the
+// interior cells it covers (each thread in plane order, then the workgroup
+// in a fixed tree) into part[m * gridsize + block], which sum_parts reduces
+// to the means that grid.cpp:565-577 removes.
+constexpr int PKZ = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_update_ru_wp_lean(Dev d, BoxCoef cf,
+                                                           const double* __restrict__ P,
+                                                           T* __restrict__ nx, T* __restrict__ ny,
+                                                           T* __restrict__ nz,
+                                                           double* __restrict__ part) {
+  __shared__ double ws[3][4];
+  const uint3 bk = xcd_block();
+  const int i = bk.x * 64 + threadIdx.x, j = bk.y * 4 + threadIdx.y;
+  const bool live = i < d.Nx && j < d.Ny;
+  const bool inxy = i >= 1 && i <= d.Nx - 2 && j >= 1 && j <= d.Ny - 2;
+  const long sx = 1, sy = d.Nx, sz = d.NxNy;
+  const long cxy = live ? (long)img(j, d.Ny) * d.Nx + img(i, d.Nx) : 0;
+  const T cpn = (T)cf.cpost_neg;
+  double s[3] = {0.0, 0.0, 0.0};
+  const int k0 = bk.z * PKZ, k1 = min(k0 + PKZ, d.Nz);
+  if (live) {
+    for (int k = k0; k < k1; k++) {
+      const long n = at(d, i, j, k);
+      const long c = (long)img(k, d.Nz) * sz + cxy;
+      const T gx = (T)((P[c] - P[c - sx]) * d.idx);
+      const T gy = (T)((P[c] - P[c - sy]) * d.idy);
+      const T gz = (T)((P[c] - P[c - sz]) * d.idz);
+      const T ax = nx[n] + cpn * gx;
+      const T ay = ny[n] + cpn * gy;
+      const T az = nz[n] + cpn * gz;
+      nx[n] = ax;
+      ny[n] = ay;
+      nz[n] = az;
+      if (inxy && k >= 1 && k <= d.Nz - 2) {
+        s[0] += (double)ax;
+        s[1] += (double)ay;
+        s[2] += (double)az;
+      }
+    }
+  }
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    double w = s[m];
+    for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+    if ((tid & 63) == 0) ws[m][tid >> 6] = w;
+  }
+  __syncthreads();
+  if (tid < 3) {
+    const long nb = (long)gridDim.x * gridDim.y * gridDim.z;
+    const long b = ((long)bk.z * gridDim.y + bk.y) * gridDim.x + bk.x;
+    part[tid * nb + b] = (ws[tid][0] + ws[tid][1]) + (ws[tid][2] + ws[tid][3]);
+  }
+}
+
+// RU = RU_np1 with its mean removed, plus c on the x component
+// (grid.cpp:565-577 and the copy of grid.cpp:318), from src (the lean last
+// stage's RU_np1, which RU_int holds after the stage exchange) and the sums
+// sum_parts left in sums[2m], sums[2m+1] = -(sum/denom); the expressions of
+// k_add_neg_mean_set.
+template <typename T>
+__global__ void k_set_mean_free3(long n, T* __restrict__ d0, T* __restrict__ d1,
+                                 T* __restrict__ d2, const T* __restrict__ s0,
+                                 const T* __restrict__ s1, const T* __restrict__ s2,
+                                 const double* __restrict__ sums, double c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int m = blockIdx.y;
+  const T* src = m == 0 ? s0 : (m == 1 ? s1 : s2);
+  T* dst = m == 0 ? d0 : (m == 1 ? d1 : d2);
+  T v = src[t] + (T)sums[2 * m + 1];
+  if (m == 0) v = v + (T)c;
+  dst[t] = v;
+}
+
+long box_wp_parts(const Dev& d) {
+  return 3L * nblocks(d.Nx, 64) * nblocks(d.Ny, 4) * nblocks(d.Nz, PKZ);
+}
+
+template <typename T>
+int box_set_mean_free(const Dev& d, T* const ru[3], const T* const src[3], const double* sums,
+                      double c, hipStream_t s) {
+  const long n = (long)d.Nx * d.Ny * d.Nz;
+  hipLaunchKernelGGL(k_set_mean_free3<T>, dim3(nblocks(n, 256), 3), dim3(256), 0, s, n, ru[0],
+                     ru[1], ru[2], src[0], src[1], src[2], sums, c);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
 template <typename T>
 int box_update_rho(const Dev& d, const BoxCoef& cf, const T* const ru_int[3], const T* rho,
                    T* rho_np1, T* rho_new, const T* const rf[3], T* const u[3], hipStream_t s,
                    T* rho_set) {
   if (rho_set && !cf.last) return FR_EINVAL;
-  if (rho_set)
-    hipLaunchKernelGGL((k_update_rho<T, true>), grid3(d), dim3(64, 4), 0, s, d, cf, ru_int[0],
-                       ru_int[1], ru_int[2], nullptr, rho_np1, rho_new, rf[0], rf[1], rf[2],
-                       u[0], u[1], u[2], rho_set);
+  if ((cf.lean && !rho_set) || (cf.np1_base && (rho_set || cf.last))) return FR_EINVAL;
+  auto go = [&](auto kern, const T* r) {
+    hipLaunchKernelGGL(kern, grid3(d), dim3(64, 4), 0, s, d, cf, ru_int[0], ru_int[1],
+                       ru_int[2], r, rho_np1, rho_new, rf[0], rf[1], rf[2], u[0], u[1], u[2],
+                       rho_set);
+  };
+  if (rho_set && cf.lean)
+    go(k_update_rho<T, RHO_SET_LEAN>, nullptr);
+  else if (rho_set)
+    go(k_update_rho<T, RHO_SET>, nullptr);
+  else if (cf.np1_base)
+    go(k_update_rho<T, RHO_BASE>, rho);
   else
-    hipLaunchKernelGGL((k_update_rho<T, false>), grid3(d), dim3(64, 4), 0, s, d, cf, ru_int[0],
-                       ru_int[1], ru_int[2], rho, rho_np1, rho_new, rf[0], rf[1], rf[2], u[0],
-                       u[1], u[2], nullptr);
+    go(k_update_rho<T, RHO_PLAIN>, rho);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
@@ -412,17 +526,31 @@ int box_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w, const T
                       hipStream_t s) {
   // z-marching kernel over the interior, then the periodic ghosts of the outputs
   const dim3 g(nblocks(d.Nx - 2, WX), nblocks(d.Ny - 2, WY), nblocks(d.Nz - 2, WKZ));
-  if (w.gravity)
-    hipLaunchKernelGGL((k_update_ru_wop_z<1, T>), g, dim3(WX, WY), 0, s, d, cf, w, u[0], u[1],
-                       u[2], ru_int[0], ru_int[1], ru_int[2], S[0], S[1], S[2], ru[0], ru[1],
-                       ru[2], ru_np1[0], ru_np1[1], ru_np1[2], ru_new[0], ru_new[1], ru_new[2],
-                       rf[0], rf[1], rf[2]);
-  else
-    hipLaunchKernelGGL((k_update_ru_wop_z<0, T>), g, dim3(WX, WY), 0, s, d, cf, w, u[0], u[1],
-                       u[2], ru_int[0], ru_int[1], ru_int[2], S[0], S[1], S[2], ru[0], ru[1],
-                       ru[2], ru_np1[0], ru_np1[1], ru_np1[2], ru_new[0], ru_new[1], ru_new[2],
-                       rf[0], rf[1], rf[2]);
+  if ((cf.lean && !cf.last) || (cf.np1_base && cf.last)) return FR_EINVAL;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, g, dim3(WX, WY), 0, s, d, cf, w, u[0], u[1], u[2], ru_int[0],
+                       ru_int[1], ru_int[2], S[0], S[1], S[2], ru[0], ru[1], ru[2], ru_np1[0],
+                       ru_np1[1], ru_np1[2], ru_new[0], ru_new[1], ru_new[2], rf[0], rf[1],
+                       rf[2]);
+  };
+  const int mode = cf.lean ? WOP_LEAN : (cf.np1_base ? WOP_BASE : WOP_PLAIN);
+  if (w.gravity) {
+    if (mode == WOP_LEAN)
+      go(k_update_ru_wop_z<1, WOP_LEAN, T>);
+    else if (mode == WOP_BASE)
+      go(k_update_ru_wop_z<1, WOP_BASE, T>);
+    else
+      go(k_update_ru_wop_z<1, WOP_PLAIN, T>);
+  } else {
+    if (mode == WOP_LEAN)
+      go(k_update_ru_wop_z<0, WOP_LEAN, T>);
+    else if (mode == WOP_BASE)
+      go(k_update_ru_wop_z<0, WOP_BASE, T>);
+    else
+      go(k_update_ru_wop_z<0, WOP_PLAIN, T>);
+  }
   FR_LAUNCH_CHECK();
+  if (cf.lean) return ghost_fill_n(d, ru_new, 3, s);
   T* const out6[6] = {ru_np1[0], ru_new[0], ru_np1[1], ru_new[1], ru_np1[2], ru_new[2]};
   return ghost_fill_n(d, out6, 6, s);
 }
@@ -450,7 +578,17 @@ long box_div_parts(const Dev& d) {
 
 template <typename T>
 int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, T* const ru_np1[3],
-                     T* const ru_new[3], hipStream_t s) {
+                     T* const ru_new[3], hipStream_t s, double* part, double* sums) {
+  if (cf.lean) {
+    if (!part || !sums) return FR_EINVAL;
+    const dim3 g(nblocks(d.Nx, 64), nblocks(d.Ny, 4), nblocks(d.Nz, PKZ));
+    hipLaunchKernelGGL(k_update_ru_wp_lean<T>, g, dim3(64, 4), 0, s, d, cf, P, ru_new[0],
+                       ru_new[1], ru_new[2], part);
+    FR_LAUNCH_CHECK();
+    // sums[2m], sums[2m+1]: component m's interior sum and minus its mean
+    return sum_parts(part, (long)g.x * g.y * g.z, sums, s,
+                     (double)(d.Nx - 2) * (d.Ny - 2) * (d.Nz - 2), 3);
+  }
   hipLaunchKernelGGL(k_update_ru_wp<T>, grid3(d), dim3(64, 4), 0, s, d, cf, P, ru_np1[0],
                      ru_np1[1], ru_np1[2], ru_new[0], ru_new[1], ru_new[2]);
   FR_LAUNCH_CHECK();
@@ -468,7 +606,9 @@ int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, T* const 
                               const T* const[3], T* const[3], double*, hipStream_t, double*,  \
                               double*);                                                       \
   template int box_update_ru_wp<T>(const Dev&, const BoxCoef&, const double*, T* const[3],    \
-                                   T* const[3], hipStream_t);
+                                   T* const[3], hipStream_t, double*, double*);               \
+  template int box_set_mean_free<T>(const Dev&, T* const[3], const T* const[3],               \
+                                    const double*, double, hipStream_t);
 FR_BOX_INST(double)
 FR_BOX_INST(float)
 

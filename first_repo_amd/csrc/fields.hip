@@ -421,10 +421,13 @@ template int make_mean_zero_async<float>(const Dev&, float*, long, double*, hipS
 
 constexpr int ST_SUM = 1024;
 // out[0] = the sum; with denom > 0 also out[1] = -(sum/denom), the value
-// make_mean_zero adds (k_add_neg_mean forms the same expression).
+// make_mean_zero adds (k_add_neg_mean forms the same expression).  Workgroup
+// g sums part[g*n .. g*n+n) into out[2g], out[2g+1].
 __global__ void __launch_bounds__(ST_SUM) k_sum_parts(const double* __restrict__ part, long n,
                                                       double* __restrict__ out, double denom) {
   __shared__ double ws[ST_SUM / 64];
+  part += blockIdx.x * n;
+  out += 2 * blockIdx.x;
   double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   long t = threadIdx.x;
   for (; t + 7 * ST_SUM < n; t += 8 * ST_SUM) {
@@ -448,8 +451,8 @@ __global__ void __launch_bounds__(ST_SUM) k_sum_parts(const double* __restrict__
   }
 }
 
-int sum_parts(const double* part, long n, double* out, hipStream_t s, double denom) {
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(ST_SUM), 0, s, part, n, out, denom);
+int sum_parts(const double* part, long n, double* out, hipStream_t s, double denom, int sets) {
+  hipLaunchKernelGGL(k_sum_parts, dim3(sets), dim3(ST_SUM), 0, s, part, n, out, denom);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }

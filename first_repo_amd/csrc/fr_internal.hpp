@@ -35,7 +35,9 @@ template <typename T>
 int make_mean_zero_set_async(const Dev& d, T* f, T* g, long n, double* ws, bool has_c, double c,
                              hipStream_t s);
 // out[0] = the sum of part[0..n) in a fixed order (one workgroup)
-int sum_parts(const double* part, long n, double* out, hipStream_t s, double denom = 0.0);
+// sets > 1: `sets` arrays of n partials one after the other, into out[2g], out[2g+1]
+int sum_parts(const double* part, long n, double* out, hipStream_t s, double denom = 0.0,
+              int sets = 1);
 // f[t] += (T)(-(dsum[0]/interior cells)) for the n elements of f: the shift
 // of make_mean_zero_async with the sum already formed
 template <typename T>
@@ -90,6 +92,15 @@ struct BoxCoef {
   double cpost;      // dt*post
   double cpre;       // dt*pre
   int last;          // RK4_count == 3
+  // First stage only: Rho_np1 and RU_np1 hold the values of Rho and RU (a
+  // full step ends with them equal) without having been written; the
+  // kernels read Rho and RU in their place.
+  int np1_base = 0;
+  // Last stage of a full box step only: Rho_np1 and RU_np1 are not written
+  // (the step's end makes them equal to Rho and RU); the momentum update
+  // leaves the stage's RU_np1 in RU_new, which the pressure-gradient update
+  // finishes in place.
+  int lean = 0;
 };
 
 struct WopCoef {
@@ -128,9 +139,20 @@ int box_div_rhs(const Dev& d, const DivCoef& q, const T* rho_new, const T* Q,
 // formed in the same pass (per-workgroup sums, then one fixed-order
 // reduction into dsum[0]).
 long box_div_parts(const Dev& d);
+// With cf.lean: RU_new holds the stage's RU_np1 and is updated in place;
+// part (box_wp_parts(d) doubles) takes per-workgroup sums of the three
+// components, reduced into sums[2m] (the interior sum) and sums[2m+1] (minus
+// the interior mean).
 template <typename T>
 int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, T* const ru_np1[3],
-                     T* const ru_new[3], hipStream_t s);
+                     T* const ru_new[3], hipStream_t s, double* part = nullptr,
+                     double* sums = nullptr);
+long box_wp_parts(const Dev& d);
+// ru[m] = (src[m] + (T)sums[2m+1]) (+ c for m = 0): RU = RU_np1 after
+// Correct_mean_RU, from the sums of the lean last stage.
+template <typename T>
+int box_set_mean_free(const Dev& d, T* const ru[3], const T* const src[3], const double* sums,
+                      double c, hipStream_t s);
 
 // ---- duct substep kernels (duct.hip) ----------------------------------------------
 int duct_update_rho(const Dev& d, const BoxCoef& cf, double cbc, const double* const ru_int[3],

@@ -519,6 +519,17 @@ class grid {
   bool rhs_mean_pending = false;
   int fill_rhs_pois();
   bool rho_set = false, ru_set = false, part_set = false;
+  // A full box step (not coupled) ends with Rho_np1 = Rho and RU_np1 = RU
+  // without writing either: its last stage leaves them out (the density
+  // update writes Rho directly; the momentum updates finish RU_np1 in RU_new
+  // and take the sums of its means on the way), and the next first stage
+  // reads Rho and RU in their place.  The flags say which _np1 field has
+  // not been written; fill_np1() writes them before any other reader.
+  // ru_lean: from the lean last stage to Correct_mean_RU, RU_new holds the
+  // stage's RU_np1.
+  bool rho_np1_stale = false, ru_np1_stale = false, ru_lean = false;
+  bool lean_stage() const { return in_step && param.IsBox() && !pair_out && RK4_count == 3; }
+  int fill_np1();
   hipEvent_t evc = nullptr;
   int launch_side();
   double* dq = nullptr;  // reduction workspace + result of that branch

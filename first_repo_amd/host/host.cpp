@@ -528,8 +528,12 @@ int grid::setup(const char* params_path, const char* ic) {
   FR_TRY(PS_.init(&param, &size, st));
   FR_CHECK(hipMalloc(&dred, sizeof(double) * (4 * (size_t)size.Nx() + 512)));
   FR_CHECK(hipMalloc(&dq, sizeof(double) * (RED_PARTS + 8)));
-  FR_CHECK(hipMalloc(&dmz, sizeof(double) * (RED_PARTS + 8)));
-  if (param.IsBox()) FR_CHECK(hipMalloc(&dpart, sizeof(double) * box_div_parts(size.dev())));
+  FR_CHECK(hipMalloc(&dmz, sizeof(double) * (RED_PARTS + 16)));
+  // dpart: per-workgroup sums of k_div_rhs, or of the lean last stage's
+  // momentum update, one at a time on the main stream
+  if (param.IsBox())
+    FR_CHECK(hipMalloc(&dpart, sizeof(double) * std::max(box_div_parts(size.dev()),
+                                                         box_wp_parts(size.dev()))));
   FR_CHECK(hipHostMalloc(&hq, sizeof(double) * 2, hipHostMallocDefault));
   return FR_OK;
 }
@@ -598,6 +602,7 @@ int grid::Initialize() {
 // grid.cpp:197-229: the time-integration stages, Rho_face, the particle
 // stages and the first particle heat source, and C_BC.
 int grid::Finish_Initialize() {
+  rho_np1_stale = ru_np1_stale = ru_lean = false;
   RU_int.copy(RU);
   RU_np1.copy(RU);
   Rho_int.copy(Rho);
@@ -663,18 +668,33 @@ int grid::Update_Rho() {
     return FR_OK;
   }
   rho_set = in_step && RK4_count == 3;
+  BoxCoef cf = box_coef(*this);
+  if (rho_np1_stale && RK4_count == 0)
+    cf.np1_base = 1;  // Rho_np1 = Rho: read Rho in its place
+  else
+    FR_TRY(fill_np1());
+  cf.lean = lean_stage();
   if (f32) {
     const float* ri_[3] = {RU_int.x.p<float>(), RU_int.y.p<float>(), RU_int.z.p<float>()};
     const float* rf_[3] = {Rho_face.x.p<float>(), Rho_face.y.p<float>(), Rho_face.z.p<float>()};
     float* u_[3] = {U.x.p<float>(), U.y.p<float>(), U.z.p<float>()};
-    FR_TRY(box_update_rho<float>(size.dev(), box_coef(*this), ri_, Rho.p<float>(),
-                                 Rho_np1.p<float>(), Rho_new.p<float>(), rf_, u_, st,
+    FR_TRY(box_update_rho<float>(size.dev(), cf, ri_, Rho.p<float>(), Rho_np1.p<float>(),
+                                 Rho_new.p<float>(), rf_, u_, st,
                                  rho_set ? Rho.p<float>() : nullptr));
-    return FR_OK;
+  } else {
+    FR_TRY(box_update_rho<double>(size.dev(), cf, ri, Rho.pointer(), Rho_np1.pointer(),
+                                  Rho_new.pointer(), rf, u, st,
+                                  rho_set ? Rho.pointer() : nullptr));
   }
-  FR_TRY(box_update_rho<double>(size.dev(), box_coef(*this), ri, Rho.pointer(),
-                                Rho_np1.pointer(), Rho_new.pointer(), rf, u, st,
-                                rho_set ? Rho.pointer() : nullptr));
+  rho_np1_stale = cf.lean != 0;  // written unless lean (then equal to Rho)
+  return FR_OK;
+}
+
+int grid::fill_np1() {
+  if (ru_lean) return FR_EINVAL;  // inside a full step's last stage only
+  if (rho_np1_stale) Rho_np1.copy(Rho);
+  if (ru_np1_stale) RU_np1.copy(RU);
+  rho_np1_stale = ru_np1_stale = false;
   return FR_OK;
 }
 
@@ -855,6 +875,12 @@ int grid::Update_RU_WOP() {  // grid.cpp:387-425
   double* n1[3] = {RU_np1.x.pointer(), RU_np1.y.pointer(), RU_np1.z.pointer()};
   double* nw[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
   const double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
+  BoxCoef cf = box_coef(*this);
+  if (param.IsBox() && ru_np1_stale && RK4_count == 0)
+    cf.np1_base = 1;  // RU_np1 = RU: read RU in its place
+  else
+    FR_TRY(fill_np1());
+  if (param.IsBox()) cf.lean = lean_stage();
   hipEventRecord(evk[2 * (RK4_count & 3)], st);
   if (param.IsBox() && f32) {
     auto P3 = [](tensor1& t) { return std::array<float*, 3>{t.x.p<float>(), t.y.p<float>(), t.z.p<float>()}; };
@@ -867,17 +893,17 @@ int grid::Update_RU_WOP() {  // grid.cpp:387-425
     float* n1_[3] = {f[0], f[1], f[2]};
     float* nw_[3] = {g[0], g[1], g[2]};
     const float* rf_[3] = {h[0], h[1], h[2]};
-    FR_TRY(box_update_ru_wop<float>(size.dev(), box_coef(*this), w, u_, ri_, S_, r0_, n1_, nw_,
-                                    rf_, st));
+    FR_TRY(box_update_ru_wop<float>(size.dev(), cf, w, u_, ri_, S_, r0_, n1_, nw_, rf_, st));
   } else if (param.IsBox()) {
-    FR_TRY(box_update_ru_wop<double>(size.dev(), box_coef(*this), w, u, ri, S, r0, n1, nw, rf,
-                                     st));
+    FR_TRY(box_update_ru_wop<double>(size.dev(), cf, w, u, ri, S, r0, n1, nw, rf, st));
   } else {  // grid.cpp:427-537
     const size_t m = (size_t)size.Ny() * size.Nz();
     FR_TRY(duct_update_ru_wop(size.dev(), box_coef(*this), w, C_BC, u, ri, S, r0, n1, nw,
                               inflow + 2 * m, inflow + 5 * m, st));
   }
   hipEventRecord(evk[2 * (RK4_count & 3) + 1], st);
+  ru_np1_stale = false;
+  ru_lean = cf.lean != 0;
   // the particle heat branch, held by Update_Particle, now runs beside it
   FR_TRY(launch_side());
   return FR_OK;
@@ -999,12 +1025,20 @@ int grid::Solve_Poisson() {  // grid.cpp:639-643
 int grid::Update_RU_WP() {  // grid.cpp:651-657
   double* n1[3] = {RU_np1.x.pointer(), RU_np1.y.pointer(), RU_np1.z.pointer()};
   double* nw[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
+  BoxCoef cf = box_coef(*this);
+  if (ru_lean && RK4_count == 3)
+    cf.lean = 1;  // RU_new holds RU_np1; the means' sums go to dmz + RED_PARTS + 8
+  else if (ru_lean)
+    return FR_EINVAL;
+  else
+    FR_TRY(fill_np1());
+  double* sums = dmz + RED_PARTS + 8;
   if (param.IsBox() && f32) {
     float* n1_[3] = {RU_np1.x.p<float>(), RU_np1.y.p<float>(), RU_np1.z.p<float>()};
     float* nw_[3] = {RU_new.x.p<float>(), RU_new.y.p<float>(), RU_new.z.p<float>()};
-    FR_TRY(box_update_ru_wp<float>(size.dev(), box_coef(*this), P.pointer(), n1_, nw_, st));
+    FR_TRY(box_update_ru_wp<float>(size.dev(), cf, P.pointer(), n1_, nw_, st, dpart, sums));
   } else if (param.IsBox())
-    FR_TRY(box_update_ru_wp<double>(size.dev(), box_coef(*this), P.pointer(), n1, nw, st));
+    FR_TRY(box_update_ru_wp<double>(size.dev(), cf, P.pointer(), n1, nw, st, dpart, sums));
   else
     FR_TRY(duct_update_ru_wp(size.dev(), box_coef(*this), P.pointer(), n1, nw, st));
   return FR_OK;
@@ -1021,6 +1055,27 @@ int grid::TimeAdvance_RK4() {
 }
 
 int grid::Correct_mean_RU() {  // grid.cpp:565-577
+  if (ru_lean) {
+    // RU_int holds the last stage's RU_np1 after the stage exchange; RU takes
+    // it with the means removed, and RU_np1 equals RU from here on
+    const Dev d = size.dev();
+    const double add = param.Rho0() * param.U0();
+    const double* sums = dmz + RED_PARTS + 8;
+    if (f32) {
+      float* r[3] = {RU.x.p<float>(), RU.y.p<float>(), RU.z.p<float>()};
+      const float* s[3] = {RU_int.x.p<float>(), RU_int.y.p<float>(), RU_int.z.p<float>()};
+      FR_TRY(box_set_mean_free<float>(d, r, s, sums, add, st));
+    } else {
+      double* r[3] = {RU.x.pointer(), RU.y.pointer(), RU.z.pointer()};
+      const double* s[3] = {RU_int.x.pointer(), RU_int.y.pointer(), RU_int.z.pointer()};
+      FR_TRY(box_set_mean_free<double>(d, r, s, sums, add, st));
+    }
+    ru_lean = false;
+    ru_np1_stale = true;
+    ru_set = true;
+    return FR_OK;
+  }
+  FR_TRY(fill_np1());
   if (param.IsBox() && in_step) {
     // with RU = RU_np1 of TimeAdvance in the same pass
     const Dev d = size.dev();
@@ -1046,6 +1101,7 @@ int grid::Correct_mean_RU() {  // grid.cpp:565-577
 }
 
 int grid::TimeAdvance() {  // grid.cpp:315-325
+  if (!rho_set || !ru_set) FR_TRY(fill_np1());
   if (!rho_set) Rho.copy(Rho_np1);
   if (!ru_set) RU.copy(RU_np1);
   P0 = P0_np1;
@@ -1124,6 +1180,7 @@ int grid::end_step() {
 
 // grid.cpp:714-811 subset; see fr_sim_diagnostics.
 int grid::diagnostics(double* out) {
+  FR_TRY(fill_np1());
   const Dev d = size.dev();
   out[1] = Rho.mean();
   out[2] = Rho.max();
@@ -1179,6 +1236,7 @@ int grid::diagnostics(double* out) {
 }
 
 tensor0* grid::field0(const std::string& n) {
+  if (fill_np1()) return nullptr;
   if (n == "P") return &P;
   if (n == "RHS_Pois") {
     (void)fill_rhs_pois();
@@ -1195,6 +1253,7 @@ tensor0* grid::field0(const std::string& n) {
 }
 
 tensor1* grid::field1(const std::string& n) {
+  if (fill_np1()) return nullptr;
   if (n == "RU") return &RU;
   if (n == "RU_int") return &RU_int;
   if (n == "RU_new") return &RU_new;

@@ -275,6 +275,7 @@ int grid::Restart() {
 // the reference's expressions (stats.hip); plane means are tree sums, so a
 // profile agrees with the reference's loop to rounding.
 int grid::Statistics() {
+  FR_TRY(fill_np1());
   if (!param.Statistics_) return FR_OK;
   FR_TRY(settle_side());
   const Dev d = size.dev();
