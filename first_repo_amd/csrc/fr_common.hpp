@@ -88,6 +88,17 @@ __device__ __forceinline__ long at(const Dev& d, int i, int j, int k) {
 // one contiguous run of the grid (neighbouring tiles and planes share an L2);
 // a tail of fewer than 8 groups keeps its id.  Returns (x, y, z) block
 // coordinates of the renumbered id.
+// The same renumbering for a one-dimensional grid: the linear id of the
+// workgroup's run position (particle kernels: particles stored in cell
+// order gather from neighbouring cells, whose rows then share one L2).
+__device__ __forceinline__ unsigned xcd_lin() {
+  const unsigned nb = gridDim.x;
+  unsigned lin = blockIdx.x;
+  const unsigned per = nb >> 3;
+  if (lin < (per << 3)) lin = (lin & 7u) * per + (lin >> 3);
+  return lin;
+}
+
 __device__ __forceinline__ uint3 xcd_block() {
   const unsigned nbx = gridDim.x, nby = gridDim.y;
   const unsigned nb = nbx * nby * gridDim.z;

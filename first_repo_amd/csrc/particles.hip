@@ -63,7 +63,7 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
                                const T* __restrict__ uy, const T* __restrict__ uz,
                                double* __restrict__ gu, double* __restrict__ gv,
                                double* __restrict__ gw) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_lin() * blockDim.x + threadIdx.x;
   if (i >= p.np) return;
   // T = float (precision form): the gas-side arithmetic (interpolation
   // weights, interpolated velocity, drag right-hand side) runs in FP32; the
@@ -158,7 +158,7 @@ __device__ __forceinline__ Cm interp_T(const Dev& d, double X, double Y, double 
 template <typename F>
 __global__ void k_part_temp(Dev d, PartPtrs p, int st, double c0, const F* __restrict__ rho,
                             double iTth) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_lin() * blockDim.x + threadIdx.x;
   if (i >= p.np) return;
   // F = float (precision form): gas temperature interpolation and the heat
   // right-hand side in FP32; F = double is the reference arithmetic.
@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(256) k_bucket_count(Dev d, PartPtrs p, int st,
                                                       int* __restrict__ cnt,
                                                       int* __restrict__ key,
                                                       int* __restrict__ slot) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_lin() * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool valid = i < p.np;
   int c = -1;
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(256) k_bucket_count(Dev d, PartPtrs p, int st,
 __global__ void k_bucket_scatter(int np, const int* __restrict__ key, const int* __restrict__ slot,
                                  const int* __restrict__ start, const int* __restrict__ rank,
                                  int* __restrict__ list) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_lin() * blockDim.x + threadIdx.x;
   if (i >= np) return;
   list[start[key[i]] + slot[i]] = rank[i];
 }
@@ -534,7 +534,7 @@ __global__ void k_bucket_place_rec(int np, int mode, PartPtrs p, int st,
                                    const int* __restrict__ rank, const int* __restrict__ unsorted,
                                    int* __restrict__ list, double idx, double idy, double idz,
                                    double* __restrict__ rec) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_lin() * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const int c = key[i], r = rank[i];
   const int b = start[c], e = start[c + 1];
