@@ -260,6 +260,25 @@ __device__ __forceinline__ double residual_t(const Dev& d, const double* __restr
                     P[c - sz], sh ? RHS[c] + a : RHS[c]);
 }
 
+// The same residual from the cell-centred density rn instead of the face
+// arrays: the face value at the lower face of q along e is the one the
+// divergence kernel stores, I_C2F = (rn[q] + rn[q - e]) * 0.5 in double,
+// rounded to TC, and C = 1./face.  Equal to residual_t<true> on faces formed
+// from the same rn (ghost values of rn equal their images).
+template <typename TC>
+__device__ __forceinline__ double residual_cell(const Dev& d, const double* __restrict__ P,
+                                                const TC* __restrict__ rn,
+                                                const double* __restrict__ RHS, long c, bool sh,
+                                                double a) {
+  const long sx = 1, sy = d.Nx, sz = d.NxNy;
+  auto v = [&](long q) { return (double)rn[q]; };
+  auto C = [](double hi, double lo) { return 1. / (double)(TC)((hi + lo) * 0.5); };
+  const double r = v(c);
+  return residual_v(d, C(r, v(c - sx)), C(v(c + sx), r), C(r, v(c - sy)), C(v(c + sy), r),
+                    C(r, v(c - sz)), C(v(c + sz), r), P[c], P[c + sx], P[c - sx], P[c + sy],
+                    P[c - sy], P[c + sz], P[c - sz], sh ? RHS[c] + a : RHS[c]);
+}
+
 __device__ __forceinline__ double residual(const Dev& d, const double* __restrict__ P,
                                            const double* __restrict__ cx,
                                            const double* __restrict__ cy,

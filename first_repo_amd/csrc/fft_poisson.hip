@@ -62,7 +62,11 @@ struct Cx<float> {
   using T = float2;
 };
 
-template <bool INV, typename TC, typename R>
+// CF, the coefficient form: CF_C (cx, cy, cz hold C), CF_FACE (they hold
+// Rho_face; C = 1./Rho_face), CF_CELL (cx holds the cell-centred density the
+// face values are interpolated from; cy and cz are not read).
+enum { CF_C = 0, CF_FACE = 1, CF_CELL = 2 };
+template <int CF, typename TC, typename R>
 __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __restrict__ P,
                                                    const TC* __restrict__ cx,
                                                    const TC* __restrict__ cy,
@@ -105,8 +109,13 @@ __global__ void __launch_bounds__(FT) k_pass_x_fwd(PoisDev q, const double* __re
     if (kk < q.nz) {
       const long c = at(d, 2 * m + 1, jj + 1, kk + 1);
       const double shift = sh ? rsum[1] : 0.0;
-      r0 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c, sh, shift);
-      r1 = residual_t<INV, TC>(d, P, cx, cy, cz, RHS, c + 1, sh, shift);
+      if constexpr (CF == CF_CELL) {
+        r0 = residual_cell<TC>(d, P, cx, RHS, c, sh, shift);
+        r1 = residual_cell<TC>(d, P, cx, RHS, c + 1, sh, shift);
+      } else {
+        r0 = residual_t<CF == CF_FACE, TC>(d, P, cx, cy, cz, RHS, c, sh, shift);
+        r1 = residual_t<CF == CF_FACE, TC>(d, P, cx, cy, cz, RHS, c + 1, sh, shift);
+      }
     }
     C2 v;
     v.x = (R)r0;
@@ -458,12 +467,12 @@ struct fr_box_poisson {
   int last_iters = 1;  // iterations of the previous solve: the next batch size
 };
 
-// One defect-correction iteration; leaves max|dp| in s->dres[0].  With inv
-// the coefficient arrays hold Rho_face and C = 1./Rho_face.  R is the real
-// type of the spectral solve (the residual and P stay double).
+// One defect-correction iteration; leaves max|dp| in s->dres[0].  cf: the
+// coefficient form of k_pass_x_fwd.  R is the real type of the spectral
+// solve (the residual and P stay double).
 template <typename TC, typename R>
 static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz,
-                            bool inv, double* P, const double* RHS, double eps, int slot,
+                            int cf, double* P, const double* RHS, double eps, int slot,
                             hipStream_t st, const double* rsum) {
   using C2 = typename Cx<R>::T;
   const bool f = sizeof(R) == 4;
@@ -477,14 +486,16 @@ static int box_poisson_iter(fr_box_poisson* s, const TC* cx, const TC* cy, const
   const int nbx = f ? s->nbx32 : s->nbx, nbxi = f ? s->nbxi32 : s->nbxi;
   // the events are bound to the launch (start and end of the kernel itself)
   const int* g = s->gate;
-  if (inv)
-    hipExtLaunchKernelGGL((k_pass_x_fwd<true, TC, R>), dim3(nbx), dim3(FT), lds[0], st,
-                          s->tx0[slot], s->tx1[slot], 0, q, P, cx, cy, cz, RHS, twfm, twf[0],
-                          data, g, rsum);
+  auto xf = [&](auto kern) {
+    hipExtLaunchKernelGGL(kern, dim3(nbx), dim3(FT), lds[0], st, s->tx0[slot], s->tx1[slot], 0,
+                          q, P, cx, cy, cz, RHS, twfm, twf[0], data, g, rsum);
+  };
+  if (cf == CF_CELL)
+    xf(k_pass_x_fwd<CF_CELL, TC, R>);
+  else if (cf == CF_FACE)
+    xf(k_pass_x_fwd<CF_FACE, TC, R>);
   else
-    hipExtLaunchKernelGGL((k_pass_x_fwd<false, TC, R>), dim3(nbx), dim3(FT), lds[0], st,
-                          s->tx0[slot], s->tx1[slot], 0, q, P, cx, cy, cz, RHS, twfm, twf[0],
-                          data, g, rsum);
+    xf(k_pass_x_fwd<CF_C, TC, R>);
   const int ty = (q.nxh + q.wy - 1) / q.wy, tz = (q.ny + q.wz - 1) / q.wz;
   // 256- and 512-point y and z transforms one pencil per wave, other lengths
   // in LDS workgroup tiles
@@ -601,12 +612,13 @@ static int x_tile_planes() {
 template <typename R>
 static int allow_lds(size_t need) {
   if (need <= 65536) return FR_OK;
-  FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<false, double, R>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-  FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<true, double, R>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-  FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_fwd<true, float, R>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  const void* xf[5] = {(const void*)k_pass_x_fwd<CF_C, double, R>,
+                       (const void*)k_pass_x_fwd<CF_FACE, double, R>,
+                       (const void*)k_pass_x_fwd<CF_FACE, float, R>,
+                       (const void*)k_pass_x_fwd<CF_CELL, double, R>,
+                       (const void*)k_pass_x_fwd<CF_CELL, float, R>};
+  for (const void* f : xf)
+    FR_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
   FR_CHECK(hipFuncSetAttribute((const void*)k_pass_strided<R>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
   FR_CHECK(hipFuncSetAttribute((const void*)k_pass_x_inv<R>,
@@ -639,10 +651,11 @@ static int ensure_f32(fr_box_poisson* s) {
 namespace fr {
 
 template <typename TC>
-int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz, bool inv,
+int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz, int cf,
                       double* P, const double* RHS, int iterations, double epsilon, int* num_iter,
                       hipStream_t st, const double* rsum) {
-  if (!s || !cx || !cy || !cz || !P || !RHS) return FR_EINVAL;
+  if (!s || !cx || !P || !RHS || cf < CF_C || cf > CF_CELL) return FR_EINVAL;
+  if (cf != CF_CELL && (!cy || !cz)) return FR_EINVAL;
   // FP32 coefficients (the precision form): FP32 inner spectral solve, with
   // the FP64 residual and P of the defect correction around it
   constexpr bool f32 = sizeof(TC) == 4;
@@ -657,9 +670,9 @@ int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* c
   while (launched < iterations) {
     const int k = std::min({batch, iterations - launched, (int)fr_box_poisson::QB});
     for (int j = 0; j < k; j++) {
-      int rc = f32 ? box_poisson_iter<TC, float>(s, cx, cy, cz, inv, P, RHS, epsilon, j, st,
+      int rc = f32 ? box_poisson_iter<TC, float>(s, cx, cy, cz, cf, P, RHS, epsilon, j, st,
                                                  rsum)
-                   : box_poisson_iter<TC, double>(s, cx, cy, cz, inv, P, RHS, epsilon, j, st,
+                   : box_poisson_iter<TC, double>(s, cx, cy, cz, cf, P, RHS, epsilon, j, st,
                                                   rsum);
       if (rc) return rc;
     }
@@ -683,10 +696,10 @@ int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* c
   return FR_OK;
 }
 template int box_poisson_solve<double>(fr_box_poisson*, const double*, const double*,
-                                       const double*, bool, double*, const double*, int, double,
+                                       const double*, int, double*, const double*, int, double,
                                        int*, hipStream_t, const double*);
 template int box_poisson_solve<float>(fr_box_poisson*, const float*, const float*, const float*,
-                                      bool, double*, const double*, int, double, int*,
+                                      int, double*, const double*, int, double, int*,
                                       hipStream_t, const double*);
 
 void box_poisson_xtimer(fr_box_poisson* s, double* ms, long* n, bool reset) {

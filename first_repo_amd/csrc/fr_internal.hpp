@@ -180,15 +180,18 @@ int pair_planes(const Dev& b, const double* const src[8], const int xi[8], doubl
                 hipStream_t s);
 
 // ---- box pressure solve (fft_poisson.hip) -------------------------------------
-// fr_box_poisson_solve with, when inv is set, Rho_face passed in place of C
-// and C = 1./Rho_face formed inside the residual (grid.cpp:641).
+// fr_box_poisson_solve with the coefficients in one of three forms (cf):
+// 0: C itself; 1: Rho_face in place of C, C = 1./Rho_face formed inside the
+// residual (grid.cpp:641); 2: the cell-centred density Rho_new in cx (cy, cz
+// unused), the face values interpolated from it inside the residual as the
+// divergence kernel forms Rho_face (one array read instead of three).
 // TC: double, or float for the FP32 precision form (P and RHS stay double).
 // rsum (device, 2 values): when RHS still holds its mean, its interior sum
 // and minus its mean (sum_parts with a denominator); the residual then uses
 // RHS + rsum[1], the value the field takes when the mean is removed
 // (make_mean_zero).
 template <typename TC>
-int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz, bool inv,
+int box_poisson_solve(fr_box_poisson* s, const TC* cx, const TC* cy, const TC* cz, int cf,
                       double* P, const double* RHS, int iterations, double epsilon, int* num_iter,
                       hipStream_t st, const double* rsum = nullptr);
 // Summed duration (ms, HIP events) and count of the forward x-pass launches

@@ -391,6 +391,10 @@ class poisson {
   // in the field; the residual removes it as make_mean_zero would.
   int Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS,
                      const double* rsum = nullptr);  // P: FP64 storage
+  // The same (box FFT solve only) when Rho_face = I_C2F(Rho_new), as
+  // Compute_Div_U_new leaves it: the residual forms the face values from
+  // Rho_new itself (one array read per iteration instead of three).
+  int Solve_cell_rho(tensor0& Rho_new, tensor0& P, tensor0& RHS, const double* rsum = nullptr);
   int num_iteration() const { return num_iteration_; }
 };
 
@@ -528,6 +532,9 @@ class grid {
   // ru_lean: from the lean last stage to Correct_mean_RU, RU_new holds the
   // stage's RU_np1.
   bool rho_np1_stale = false, ru_np1_stale = false, ru_lean = false;
+  // Rho_face = I_C2F(Rho_new) as the box's Compute_RHS_Pois formed it, with
+  // neither field changed since (field accessors and Update_Rho clear it)
+  bool face_of_rho_new = false;
   bool lean_stage() const { return in_step && param.IsBox() && !pair_out && RK4_count == 3; }
   int fill_np1();
   hipEvent_t evc = nullptr;

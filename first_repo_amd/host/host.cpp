@@ -439,6 +439,16 @@ int poisson::Solve_face_rho(tensor1& Rho_face, tensor0& P, tensor0& RHS, const d
                                    Rho_face.z.pointer(), true, P.pointer(), RHS.pointer(),
                                    par->Iteration(), par->epsilon(), &num_iteration_, st_, rsum);
 }
+int poisson::Solve_cell_rho(tensor0& Rho_new, tensor0& P, tensor0& RHS, const double* rsum) {
+  if (!s) return FR_EINVAL;
+  if (Rho_new.is_f32())
+    return box_poisson_solve<float>(s, Rho_new.p<float>(), nullptr, nullptr, 2, P.pointer(),
+                                    RHS.pointer(), par->Iteration(), par->epsilon(),
+                                    &num_iteration_, st_, rsum);
+  return box_poisson_solve<double>(s, Rho_new.pointer(), nullptr, nullptr, 2, P.pointer(),
+                                   RHS.pointer(), par->Iteration(), par->epsilon(),
+                                   &num_iteration_, st_, rsum);
+}
 
 // ---------------------------------------------------------------- grid
 #define FR_TRY(x)                \
@@ -668,6 +678,7 @@ int grid::Update_Rho() {
     return FR_OK;
   }
   rho_set = in_step && RK4_count == 3;
+  face_of_rho_new = false;
   BoxCoef cf = box_coef(*this);
   if (rho_np1_stale && RK4_count == 0)
     cf.np1_base = 1;  // Rho_np1 = Rho: read Rho in its place
@@ -982,6 +993,7 @@ int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
     RHS_Pois.make_mean_zero_async(dmz);
     return FR_OK;
   }
+  face_of_rho_new = true;
   // box: the interior sum was formed by the kernel (tensor0::make_mean_zero).
   // In the time step with the FFT solve the field keeps its mean: the
   // solve's residual, the field's only reader there, adds the same shift
@@ -1010,6 +1022,8 @@ int grid::Solve_Poisson() {  // grid.cpp:639-643
     for (int m = 0; m < 3; m++)
       to_double(Rho_face.c(m).p<float>(), d64b.c(m).pointer(), size.size(), st);
     FR_TRY(PS_.Solve_face_rho(d64b, P, RHS_Pois));
+  } else if (face_of_rho_new && PS_.s) {
+    FR_TRY(PS_.Solve_cell_rho(Rho_new, P, RHS_Pois, rsum));
   } else {
     FR_TRY(PS_.Solve_face_rho(Rho_face, P, RHS_Pois, rsum));
   }
@@ -1237,6 +1251,7 @@ int grid::diagnostics(double* out) {
 
 tensor0* grid::field0(const std::string& n) {
   if (fill_np1()) return nullptr;
+  face_of_rho_new = false;  // the caller may change the fields
   if (n == "P") return &P;
   if (n == "RHS_Pois") {
     (void)fill_rhs_pois();
@@ -1254,6 +1269,7 @@ tensor0* grid::field0(const std::string& n) {
 
 tensor1* grid::field1(const std::string& n) {
   if (fill_np1()) return nullptr;
+  face_of_rho_new = false;
   if (n == "RU") return &RU;
   if (n == "RU_int") return &RU_int;
   if (n == "RU_new") return &RU_new;

@@ -542,3 +542,34 @@ def test_pressure_solve_64_matches_oracle(tmp_path):
     worst, bad = compare_state(og, sim, tol=1e-12, fields1=[], fields0=["P", "RHS_Pois"],
                                particles=False)
     assert not bad, bad
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fp32", [False, True], ids=["fp64", "fp32"])
+def test_cell_density_residual_equals_face_form(tmp_path, fp32):
+    """Inside the box's method sequence the FFT solve's residual forms the face
+    densities from Rho_new (as Compute_RHS_Pois forms Rho_face); a field read
+    in between sends the next solve through the stored Rho_face instead.  Both
+    must give the same P bit for bit and the same iteration count."""
+    params, d = make_box_case(tmp_path, n=32, n_particles=4000)
+    sim = fr.Sim(params, d, fp32=fp32)
+    sim.initialize()
+    sim.step()
+    sim.call("update")
+    for m in ("Update_Rho", "Update_P0", "Update_Particle", "Update_RU_WOP", "Compute_Div_U_new"):
+        sim.call(m, 0)
+    p0 = sim.field("P")
+    sim.call("Compute_RHS_Pois", 0)
+    it0 = sim.timing()[3]  # iterations counted so far (not a field read)
+    sim.call("Solve_Poisson", 0)  # cell-density form
+    it1 = sim.timing()[3]
+    p_cell = sim.field("P")
+    sim.set_field("P", p0)
+    sim.call("Solve_Poisson", 0)  # the field reads above: stored Rho_face
+    it2 = sim.timing()[3]
+    p_face = sim.field("P")
+    print(f"32^3 fp32={fp32} max|P_cell - P_face| = {np.max(np.abs(p_cell - p_face))}, "
+          f"iterations {it1 - it0} and {it2 - it1}", flush=True)
+    assert np.array_equal(p_cell, p_face)
+    assert not np.array_equal(p_cell, p0)
+    assert it1 - it0 == it2 - it1 > 0
