@@ -95,8 +95,9 @@ def kernel_table(n, particles, fp32, r, steps, precision="fp64"):
     """Algorithmic bytes per launch and achieved bandwidth of the timed
     kernels.  Bytes count each distinct array element read or written once
     per launch (es = field element size, ghosted = (n+2)^3 cells):
-      k_pass_x_fwd: reads P, RHS (FP64) and the three Rho_face arrays once per
-        interior cell, writes the half spectrum (n/2+1) n^2 complex;
+      k_pass_x_fwd: reads P, RHS (FP64) and Rho_new (the face densities of
+        the residual are interpolated from it) once per interior cell, writes
+        the half spectrum (n/2+1) n^2 complex;
       momentum deposit: writes 3 components and reads a bucket start per
         ghosted cell; per particle its reference index and 6 operands;
       trilinear deposits (concentration and heat alternate): 1 output and a
@@ -113,7 +114,7 @@ def kernel_table(n, particles, fp32, r, steps, precision="fp64"):
     nspec = (n // 2 + 1) * n * n
     rows = [
         ("k_pass_x_fwd", "pressure residual + real FFT along x (events bound to the launch)",
-         ncell * (28 if fp32 else 40) + nspec * (8 if fp32 else 16), t[6], t[7]),
+         ncell * (20 if fp32 else 24) + nspec * (8 if fp32 else 16), t[6], t[7]),
         ("k_dep_vel", "momentum deposit, particle.cpp:498-524 (events bound to the launch)",
          g * (3 * es + 4) + particles * 52, t[8], t[9]),
         ("k_dep_cic", "concentration and heat deposits, particle.cpp:526-658 (events bound "
