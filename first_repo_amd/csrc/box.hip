@@ -6,6 +6,8 @@
 // so no value changes), and writes every cell including the ghosts (the
 // ghost of a box field is the value at its periodic image).  Inputs must
 // have valid periodic ghosts, which every kernel here maintains.
+#include <type_traits>
+
 #include "fr_internal.hpp"
 
 namespace fr {
@@ -30,9 +32,13 @@ static dim3 grid3(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 4
 // stage does not read Rho.  V (the BoxCoef flags, as template arguments so
 // that each form compiles to straight-line code): RHO_PLAIN; RHO_BASE (first
 // stage, Rho stands in for Rho_np1); RHO_SET (SET above); RHO_SET_LEAN (SET,
-// and Rho_np1 itself is not written: it equals Rho from there on).
+// and Rho_np1 itself is not written: it equals Rho from there on).  FI: rfx
+// holds Rho_int and Rho_face = I_C2F(Rho_int) (the divergence kernel of the
+// previous stage formed it from the same values, which the stage exchange
+// has moved into Rho_int): the face values are formed here as that kernel
+// forms them (at the image cell, in double, rounded to T); rfy, rfz unused.
 enum { RHO_PLAIN, RHO_BASE, RHO_SET, RHO_SET_LEAN };
-template <typename T, int V>
+template <typename T, int V, bool FI>
 __global__ void __launch_bounds__(256) k_update_rho(
     Dev d, BoxCoef cf, const T* __restrict__ rux, const T* __restrict__ ruy,
     const T* __restrict__ ruz, const T* __restrict__ rho, T* __restrict__ rho_np1,
@@ -57,9 +63,30 @@ __global__ void __launch_bounds__(256) k_update_rho(
   } else {
     rho_new[n] = cf.last ? np1 : 1 * rho[n] + (T)cf.cpre_neg * rhs;
   }
-  ux[n] = rux[n] / rfx[n];
-  uy[n] = ruy[n] / rfy[n];
-  uz[n] = ruz[n] / rfz[n];
+  if constexpr (FI) {
+    const double r0 = (double)rfx[c];
+    ux[n] = rux[n] / (T)((r0 + (double)rfx[c - sx]) * 0.5);
+    uy[n] = ruy[n] / (T)((r0 + (double)rfx[c - sy]) * 0.5);
+    uz[n] = ruz[n] / (T)((r0 + (double)rfx[c - sz]) * 0.5);
+  } else {
+    ux[n] = rux[n] / rfx[n];
+    uy[n] = ruy[n] / rfy[n];
+    uz[n] = ruz[n] / rfz[n];
+  }
+}
+
+// Rho_face = I_C2F(rn) at every ghosted cell, the values k_div_rhs stores
+// (at the image cell, in double, rounded to T): the faces of a step that
+// left them unstored, for a reader outside the kernels that rebuild them.
+template <typename T>
+__global__ void __launch_bounds__(256) k_faces_of(Dev d, const T* __restrict__ rn,
+                                                  T* __restrict__ rfx, T* __restrict__ rfy,
+                                                  T* __restrict__ rfz) {
+  FR_CELL_PROLOGUE
+  const double r0 = (double)rn[c];
+  rfx[n] = (T)((r0 + (double)rn[c - sx]) * 0.5);
+  rfy[n] = (T)((r0 + (double)rn[c - sy]) * 0.5);
+  rfz[n] = (T)((r0 + (double)rn[c - sz]) * 0.5);
 }
 
 // Div_F2C(U) at cell q (any q with q+e in range), grid.cpp:390.
@@ -283,7 +310,7 @@ __global__ void __launch_bounds__(WX * WY) k_update_ru_wop_z(
 // later in a fixed order.
 constexpr int DKZ = 8;
 
-template <typename T>
+template <typename T, bool SF>
 __global__ void __launch_bounds__(256) k_div_rhs(Dev d, DivCoef q, const T* __restrict__ rn,
                                                  const T* __restrict__ Q,
                                                  const T* __restrict__ rnx,
@@ -350,9 +377,11 @@ __global__ void __launch_bounds__(256) k_div_rhs(Dev d, DivCoef q, const T* __re
       // Rho_face = I_C2F(Rho_new) at the three lower faces and the upper ones
       const double fx0 = (rc + rxm) * h, fx1 = (rxp + rc) * h;
       const double fy0 = (rc + rym) * h, fy1 = (ryp + rc) * h;
-      rfx[n] = (T)fx0;
-      rfy[n] = (T)fy0;
-      rfz[n] = (T)fz0;
+      if constexpr (SF) {
+        rfx[n] = (T)fx0;
+        rfy[n] = (T)fy0;
+        rfz[n] = (T)fz0;
+      }
       const double D = (v(rnx, c + sx) / fx1 - v(rnx, c) / fx0) * d.idx +
                        (v(rny, c + sy) / fy1 - v(rny, c) / fy0) * d.idy + (qz1 - qz0) * d.idz;
       r = D - dvg;
@@ -499,22 +528,39 @@ int box_set_mean_free(const Dev& d, T* const ru[3], const T* const src[3], const
 template <typename T>
 int box_update_rho(const Dev& d, const BoxCoef& cf, const T* const ru_int[3], const T* rho,
                    T* rho_np1, T* rho_new, const T* const rf[3], T* const u[3], hipStream_t s,
-                   T* rho_set) {
+                   T* rho_set, const T* rho_int) {
   if (rho_set && !cf.last) return FR_EINVAL;
   if ((cf.lean && !rho_set) || (cf.np1_base && (rho_set || cf.last))) return FR_EINVAL;
+  if (!rho_int && (!rf[0] || !rf[1] || !rf[2])) return FR_EINVAL;
+  const T* f[3] = {rho_int ? rho_int : rf[0], rho_int ? nullptr : rf[1],
+                   rho_int ? nullptr : rf[2]};
   auto go = [&](auto kern, const T* r) {
     hipLaunchKernelGGL(kern, grid3(d), dim3(64, 4), 0, s, d, cf, ru_int[0], ru_int[1],
-                       ru_int[2], r, rho_np1, rho_new, rf[0], rf[1], rf[2], u[0], u[1], u[2],
+                       ru_int[2], r, rho_np1, rho_new, f[0], f[1], f[2], u[0], u[1], u[2],
                        rho_set);
   };
-  if (rho_set && cf.lean)
-    go(k_update_rho<T, RHO_SET_LEAN>, nullptr);
-  else if (rho_set)
-    go(k_update_rho<T, RHO_SET>, nullptr);
-  else if (cf.np1_base)
-    go(k_update_rho<T, RHO_BASE>, rho);
+  auto pick = [&](auto fi) {
+    constexpr bool FI = decltype(fi)::value;
+    if (rho_set && cf.lean)
+      go(k_update_rho<T, RHO_SET_LEAN, FI>, nullptr);
+    else if (rho_set)
+      go(k_update_rho<T, RHO_SET, FI>, nullptr);
+    else if (cf.np1_base)
+      go(k_update_rho<T, RHO_BASE, FI>, rho);
+    else
+      go(k_update_rho<T, RHO_PLAIN, FI>, rho);
+  };
+  if (rho_int)
+    pick(std::true_type{});
   else
-    go(k_update_rho<T, RHO_PLAIN>, rho);
+    pick(std::false_type{});
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+template <typename T>
+int box_faces_of(const Dev& d, const T* rn, T* const rf[3], hipStream_t s) {
+  hipLaunchKernelGGL(k_faces_of<T>, grid3(d), dim3(64, 4), 0, s, d, rn, rf[0], rf[1], rf[2]);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
@@ -560,9 +606,16 @@ int box_div_rhs(const Dev& d, const DivCoef& q, const T* rho_new, const T* Q,
                 const T* const ru_new[3], T* const rf[3], double* rhs, hipStream_t s,
                 double* part, double* dsum) {
   const dim3 g = grid3(d);
-  hipLaunchKernelGGL(k_div_rhs<T>, dim3(g.x, g.y, nblocks(d.Nz, DKZ)), dim3(64, 4), 0, s, d, q,
-                     rho_new, Q, ru_new[0], ru_new[1], ru_new[2], rf[0], rf[1], rf[2], rhs,
-                     part);
+  // rf[0] null: the face values are formed but not stored
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(g.x, g.y, nblocks(d.Nz, DKZ)), dim3(64, 4), 0, s, d, q,
+                       rho_new, Q, ru_new[0], ru_new[1], ru_new[2], rf[0], rf[1], rf[2], rhs,
+                       part);
+  };
+  if (rf[0])
+    go(k_div_rhs<T, true>);
+  else
+    go(k_div_rhs<T, false>);
   FR_LAUNCH_CHECK();
   // dsum[0]: the interior sum; dsum[1]: minus the interior mean
   if (part && dsum)
@@ -597,7 +650,9 @@ int box_update_ru_wp(const Dev& d, const BoxCoef& cf, const double* P, T* const 
 
 #define FR_BOX_INST(T)                                                                        \
   template int box_update_rho<T>(const Dev&, const BoxCoef&, const T* const[3], const T*, T*, \
-                                 T*, const T* const[3], T* const[3], hipStream_t, T*);        \
+                                 T*, const T* const[3], T* const[3], hipStream_t, T*,         \
+                                 const T*);                                                   \
+  template int box_faces_of<T>(const Dev&, const T*, T* const[3], hipStream_t);               \
   template int box_update_ru_wop<T>(const Dev&, const BoxCoef&, const WopCoef&,               \
                                     const T* const[3], const T* const[3], const T* const[3],  \
                                     const T* const[3], T* const[3], T* const[3],              \

@@ -125,7 +125,12 @@ struct DivCoef {
 template <typename T>
 int box_update_rho(const Dev& d, const BoxCoef& cf, const T* const ru_int[3], const T* rho,
                    T* rho_np1, T* rho_new, const T* const rf[3], T* const u[3], hipStream_t s,
-                   T* rho_set = nullptr);
+                   T* rho_set = nullptr, const T* rho_int = nullptr);
+// With rho_int: Rho_face = I_C2F(Rho_int) holds, and U = RU_int/Rho_face
+// forms the face values from Rho_int (rf is not read).
+// Rho_face = I_C2F(rn) at every ghosted cell, as box_div_rhs stores it.
+template <typename T>
+int box_faces_of(const Dev& d, const T* rn, T* const rf[3], hipStream_t s);
 template <typename T>
 int box_update_ru_wop(const Dev& d, const BoxCoef& cf, const WopCoef& w, const T* const u[3],
                       const T* const ru_int[3], const T* const S[3], const T* const ru[3],
@@ -135,6 +140,7 @@ template <typename T>
 int box_div_rhs(const Dev& d, const DivCoef& q, const T* rho_new, const T* Q,
                 const T* const ru_new[3], T* const rf[3], double* rhs, hipStream_t s,
                 double* part = nullptr, double* dsum = nullptr);
+// rf[0] null: the face values are used but not stored.
 // With part (box_div_parts(d) doubles) and dsum: the interior sum of rhs is
 // formed in the same pass (per-workgroup sums, then one fixed-order
 // reduction into dsum[0]).

@@ -532,9 +532,16 @@ class grid {
   // ru_lean: from the lean last stage to Correct_mean_RU, RU_new holds the
   // stage's RU_np1.
   bool rho_np1_stale = false, ru_np1_stale = false, ru_lean = false;
-  // Rho_face = I_C2F(Rho_new) as the box's Compute_RHS_Pois formed it, with
-  // neither field changed since (field accessors and Update_Rho clear it)
-  bool face_of_rho_new = false;
+  // face_src: Rho_face = I_C2F(Rho_new) (1) or I_C2F(Rho_int) (2) in value,
+  // as the box's Compute_RHS_Pois formed it (1; the stage exchange makes it
+  // 2), with neither field changed since; 0: only the stored Rho_face.
+  // face_stored: Rho_face's memory holds it.  Inside a full box step with
+  // the FFT solve and no gravity the faces are not stored: their readers
+  // there (the solve's residual, the density update's U = RU_int/Rho_face)
+  // form them from the density; fill_faces() stores them for any other.
+  int face_src = 0;
+  bool face_stored = true;
+  int fill_faces();
   bool lean_stage() const { return in_step && param.IsBox() && !pair_out && RK4_count == 3; }
   int fill_np1();
   hipEvent_t evc = nullptr;

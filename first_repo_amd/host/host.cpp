@@ -613,6 +613,8 @@ int grid::Initialize() {
 // stages and the first particle heat source, and C_BC.
 int grid::Finish_Initialize() {
   rho_np1_stale = ru_np1_stale = ru_lean = false;
+  face_src = 0;
+  face_stored = true;
   RU_int.copy(RU);
   RU_np1.copy(RU);
   Rho_int.copy(Rho);
@@ -678,7 +680,14 @@ int grid::Update_Rho() {
     return FR_OK;
   }
   rho_set = in_step && RK4_count == 3;
-  face_of_rho_new = false;
+  // Rho_new is rewritten: faces formed from it must be stored first; faces
+  // formed from Rho_int are formed here from Rho_int
+  if (face_src == 1) {
+    FR_TRY(fill_faces());
+    face_src = 0;
+  }
+  const bool fi = face_src == 2;
+  if (!fi) FR_TRY(fill_faces());
   BoxCoef cf = box_coef(*this);
   if (rho_np1_stale && RK4_count == 0)
     cf.np1_base = 1;  // Rho_np1 = Rho: read Rho in its place
@@ -691,11 +700,13 @@ int grid::Update_Rho() {
     float* u_[3] = {U.x.p<float>(), U.y.p<float>(), U.z.p<float>()};
     FR_TRY(box_update_rho<float>(size.dev(), cf, ri_, Rho.p<float>(), Rho_np1.p<float>(),
                                  Rho_new.p<float>(), rf_, u_, st,
-                                 rho_set ? Rho.p<float>() : nullptr));
+                                 rho_set ? Rho.p<float>() : nullptr,
+                                 fi ? Rho_int.p<float>() : nullptr));
   } else {
     FR_TRY(box_update_rho<double>(size.dev(), cf, ri, Rho.pointer(), Rho_np1.pointer(),
                                   Rho_new.pointer(), rf, u, st,
-                                  rho_set ? Rho.pointer() : nullptr));
+                                  rho_set ? Rho.pointer() : nullptr,
+                                  fi ? Rho_int.pointer() : nullptr));
   }
   rho_np1_stale = cf.lean != 0;  // written unless lean (then equal to Rho)
   return FR_OK;
@@ -706,6 +717,21 @@ int grid::fill_np1() {
   if (rho_np1_stale) Rho_np1.copy(Rho);
   if (ru_np1_stale) RU_np1.copy(RU);
   rho_np1_stale = ru_np1_stale = false;
+  return FR_OK;
+}
+
+int grid::fill_faces() {
+  if (face_stored) return FR_OK;
+  if (face_src != 1 && face_src != 2) return FR_EINVAL;
+  tensor0& src = face_src == 1 ? Rho_new : Rho_int;
+  if (f32) {
+    float* rf[3] = {Rho_face.x.p<float>(), Rho_face.y.p<float>(), Rho_face.z.p<float>()};
+    FR_TRY(box_faces_of<float>(size.dev(), src.p<float>(), rf, st));
+  } else {
+    double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
+    FR_TRY(box_faces_of<double>(size.dev(), src.pointer(), rf, st));
+  }
+  face_stored = true;
   return FR_OK;
 }
 
@@ -886,6 +912,7 @@ int grid::Update_RU_WOP() {  // grid.cpp:387-425
   double* n1[3] = {RU_np1.x.pointer(), RU_np1.y.pointer(), RU_np1.z.pointer()};
   double* nw[3] = {RU_new.x.pointer(), RU_new.y.pointer(), RU_new.z.pointer()};
   const double* rf[3] = {Rho_face.x.pointer(), Rho_face.y.pointer(), Rho_face.z.pointer()};
+  if (w.gravity) FR_TRY(fill_faces());
   BoxCoef cf = box_coef(*this);
   if (param.IsBox() && ru_np1_stale && RK4_count == 0)
     cf.np1_base = 1;  // RU_np1 = RU: read RU in its place
@@ -979,9 +1006,14 @@ int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
   rhs_mean_pending = false;
   // sum and minus the mean, in slots no other mean's reduction writes
   double* rsum = dmz + RED_PARTS + 4;
+  // in a full step with the FFT solve and no gravity the faces stay unstored
+  const bool keep = !(param.IsBox() && in_step && PS_.s && param.gx() == 0 &&
+                      param.gy() == 0 && param.gz() == 0);
+  if (param.IsBox() && !keep) rf[0] = nullptr;
   if (param.IsBox() && f32) {
     const float* rn_[3] = {RU_new.x.p<float>(), RU_new.y.p<float>(), RU_new.z.p<float>()};
-    float* rf_[3] = {Rho_face.x.p<float>(), Rho_face.y.p<float>(), Rho_face.z.p<float>()};
+    float* rf_[3] = {keep ? Rho_face.x.p<float>() : nullptr, Rho_face.y.p<float>(),
+                     Rho_face.z.p<float>()};
     FR_TRY(box_div_rhs<float>(size.dev(), q, Rho_new.p<float>(), RHS_Part_Temp.p<float>(), rn_,
                               rf_, RHS_Pois.pointer(), st, dpart, rsum));
   } else if (param.IsBox()) {
@@ -993,7 +1025,8 @@ int grid::Compute_RHS_Pois() {  // grid.cpp:556-563 + 629-637
     RHS_Pois.make_mean_zero_async(dmz);
     return FR_OK;
   }
-  face_of_rho_new = true;
+  face_src = 1;
+  face_stored = keep;
   // box: the interior sum was formed by the kernel (tensor0::make_mean_zero).
   // In the time step with the FFT solve the field keeps its mean: the
   // solve's residual, the field's only reader there, adds the same shift
@@ -1017,14 +1050,16 @@ int grid::Solve_Poisson() {  // grid.cpp:639-643
   if (rhs_mean_pending && !PS_.s) FR_TRY(fill_rhs_pois());
   const double* rsum = rhs_mean_pending ? dmz + RED_PARTS + 4 : nullptr;
   if (f32 && PS_.sp) {
+    FR_TRY(fill_faces());
     // the PCG's FP64 operator reads the FP32 faces widened (the FFT solve
     // widens them inside its residual); its V-cycles run in FP32 either way
     for (int m = 0; m < 3; m++)
       to_double(Rho_face.c(m).p<float>(), d64b.c(m).pointer(), size.size(), st);
     FR_TRY(PS_.Solve_face_rho(d64b, P, RHS_Pois));
-  } else if (face_of_rho_new && PS_.s) {
+  } else if (face_src == 1 && PS_.s) {
     FR_TRY(PS_.Solve_cell_rho(Rho_new, P, RHS_Pois, rsum));
   } else {
+    FR_TRY(fill_faces());
     FR_TRY(PS_.Solve_face_rho(Rho_face, P, RHS_Pois, rsum));
   }
   hipEventRecord(ev[1], st);
@@ -1062,6 +1097,7 @@ int grid::Update_RU_WP() {  // grid.cpp:651-657
 // next read, so exchanging buffers equals the copies.
 int grid::TimeAdvance_RK4() {
   Rho_int.swap(Rho_new);
+  if (face_src) face_src = 3 - face_src;  // the faces' source moved with the buffer
   RU_int.swap(RU_new);
   P0_int = P0_new;
   part.dev.swap_stages(1, 2);
@@ -1195,6 +1231,7 @@ int grid::end_step() {
 // grid.cpp:714-811 subset; see fr_sim_diagnostics.
 int grid::diagnostics(double* out) {
   FR_TRY(fill_np1());
+  FR_TRY(fill_faces());
   const Dev d = size.dev();
   out[1] = Rho.mean();
   out[2] = Rho.max();
@@ -1250,8 +1287,8 @@ int grid::diagnostics(double* out) {
 }
 
 tensor0* grid::field0(const std::string& n) {
-  if (fill_np1()) return nullptr;
-  face_of_rho_new = false;  // the caller may change the fields
+  if (fill_np1() || fill_faces()) return nullptr;
+  face_src = 0;  // the caller may change the fields
   if (n == "P") return &P;
   if (n == "RHS_Pois") {
     (void)fill_rhs_pois();
@@ -1268,8 +1305,8 @@ tensor0* grid::field0(const std::string& n) {
 }
 
 tensor1* grid::field1(const std::string& n) {
-  if (fill_np1()) return nullptr;
-  face_of_rho_new = false;
+  if (fill_np1() || fill_faces()) return nullptr;
+  face_src = 0;
   if (n == "RU") return &RU;
   if (n == "RU_int") return &RU_int;
   if (n == "RU_new") return &RU_new;

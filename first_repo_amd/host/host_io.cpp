@@ -276,6 +276,7 @@ int grid::Restart() {
 // profile agrees with the reference's loop to rounding.
 int grid::Statistics() {
   FR_TRY(fill_np1());
+  FR_TRY(fill_faces());
   if (!param.Statistics_) return FR_OK;
   FR_TRY(settle_side());
   const Dev d = size.dev();
