@@ -390,6 +390,13 @@ class PartDev {
   // rk: reference indices to order the buckets by (default: rank)
   int build(const Dev& d, int stage, int mode, hipStream_t st, bool operands = true,
             const int* rk = nullptr);
+  // build = build_head (count, scan, unsorted lists: the positions and the
+  // reference indices only) + build_tail (placement by reference index and
+  // the operands), which may follow on another stream behind an event;
+  // nothing may change the positions, ranks or storage order in between
+  int build_head(const Dev& d, int stage, int mode, hipStream_t st, const int* rk = nullptr);
+  int build_tail(const Dev& d, int stage, int mode, hipStream_t st, bool operands = true,
+                 const int* rk = nullptr);
   // trilinear deposit from R: weighted by RHS_T (heat) or not (concentration)
   template <typename F>
   int deposit_cic(const Dev& d, bool weighted, F* out, bool halo, hipStream_t st);

@@ -2097,20 +2097,35 @@ int PartDev::temp(const Dev& d, int stage, double c0, const T* rho, double iTth,
 
 int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool operands,
                    const int* rk) {
+  FR_RET(build_head(d, stage, mode, st, rk));
+  return build_tail(d, stage, mode, st, operands, rk);
+}
+
+int PartDev::build_head(const Dev& d, int stage, int mode, hipStream_t st, const int* rk) {
   Buckets& B = mode == 0 ? R : T;
   if (!rk) rk = rank;
   // the operand layout follows the deposit form chosen now; the deposit
   // reads whichever layout its buckets were built with
   B.rec_form = dep_kind(mode == 0) == DEP_SPLIT;
+  B.valid = false;
   FR_CHECK(hipMemsetAsync(B.cnt, 0, sizeof(int) * ncell, st));
   if (np > 0)
     hipLaunchKernelGGL(k_bucket_count, dim3(nblocks(np, 256)), dim3(256), 0, st, d, ptrs(), stage,
                        mode, B.cnt, B.key, B.slot);
   int rc = scan_ex(B.cnt, ncell, B.start, B.scan_tmp, st);
   if (rc) return rc;
-  if (np > 0 && B.rec_form) {
+  if (np > 0)
     hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
                        B.slot, B.start, rk, B.list2);
+  FR_LAUNCH_CHECK();
+  return FR_OK;
+}
+
+int PartDev::build_tail(const Dev& d, int stage, int mode, hipStream_t st, bool operands,
+                        const int* rk) {
+  Buckets& B = mode == 0 ? R : T;
+  if (!rk) rk = rank;
+  if (np > 0 && B.rec_form) {
     if (operands)
       hipLaunchKernelGGL(k_bucket_place_rec<true>, dim3(nblocks(np, 256)), dim3(256), 0, st, np,
                          mode, ptrs(), stage, B.key, B.start, rk, B.list2, B.list, d.idx, d.idy,
@@ -2120,8 +2135,6 @@ int PartDev::build(const Dev& d, int stage, int mode, hipStream_t st, bool opera
                          mode, ptrs(), stage, B.key, B.start, rk, B.list2, B.list, d.idx, d.idy,
                          d.idz, B.rec);
   } else if (np > 0) {
-    hipLaunchKernelGGL(k_bucket_scatter, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
-                       B.slot, B.start, rk, B.list2);
     hipLaunchKernelGGL(k_bucket_place, dim3(nblocks(np, 256)), dim3(256), 0, st, np, B.key,
                        B.start, rk, B.list2, B.list, B.slot);
     if (operands)

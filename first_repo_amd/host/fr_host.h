@@ -334,8 +334,10 @@ class particle {
   // migration that runs beside it); default the current ones
   // halo = false leaves the periodic ghost shell of A unformed (the fold into
   // the interior is done either way)
-  int part2gas_velocity(tensor1& A, const int* rk = nullptr,
-                        bool halo = true);  // particle.cpp:498-524
+  // head: the bucket build's first half (PartDev::build_head of the stage-1
+  // positions) was issued on another stream and recorded in this event
+  int part2gas_velocity(tensor1& A, const int* rk = nullptr, bool halo = true,
+                        hipEvent_t head = nullptr);  // particle.cpp:498-524
   int Send_Recv(hipStream_t on = 0);                            // particle.cpp:853-858
   // duct rank that is inlet and outlet: Send_Recv(particle& pbox)
   // (particle.cpp:860-865, 814-851) with the box's Rbuf1 given as n inlet
@@ -496,6 +498,10 @@ class grid {
   // third stream (default priority): the box migration beside the deposits
   hipStream_t st3 = 0;
   hipEvent_t evm = nullptr, evm0 = nullptr;
+  // the momentum deposit's bucket head, issued on st3 at the top of a box
+  // substep beside the density update and the advance (vel_head: pending)
+  hipEvent_t evh = nullptr;
+  bool vel_head = false;
   hipEvent_t evf = nullptr, evj = nullptr, evj2 = nullptr;
   // the heat branch of Update_Particle (gas temperature at the particles,
   // buckets, heat deposit, its mean) runs on st2 beside Update_RU_WOP and is

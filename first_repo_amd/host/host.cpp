@@ -343,8 +343,14 @@ int particle::part2gas_concentration(tensor0& A, hipStream_t on) {
   return dev.deposit_cic(size_->dev(), false, A.pointer(), false, s);
 }
 
-int particle::part2gas_velocity(tensor1& A, const int* rk, bool halo) {
-  int rc = dev.build(size_->dev(), 1, 1, st_, true, rk);
+int particle::part2gas_velocity(tensor1& A, const int* rk, bool halo, hipEvent_t head) {
+  int rc;
+  if (head) {
+    FR_CHECK(hipStreamWaitEvent(st_, head, 0));
+    rc = dev.build_tail(size_->dev(), 1, 1, st_, true, rk);
+  } else {
+    rc = dev.build(size_->dev(), 1, 1, st_, true, rk);
+  }
   if (rc) return rc;
   if (A.x.is_f32()) {
     float* o[3] = {A.x.p<float>(), A.y.p<float>(), A.z.p<float>()};
@@ -476,6 +482,7 @@ grid::~grid() {
   if (evc) hipEventDestroy(evc);
   if (evm) hipEventDestroy(evm);
   if (evm0) hipEventDestroy(evm0);
+  if (evh) hipEventDestroy(evh);
   if (st3) hipStreamDestroy(st3);
   if (evq) hipEventDestroy(evq);
   if (dq) hipFree(dq);
@@ -501,6 +508,7 @@ int grid::setup(const char* params_path, const char* ic) {
   FR_CHECK(hipStreamCreate(&st3));
   FR_CHECK(hipEventCreateWithFlags(&evm, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evm0, hipEventDisableTiming));
+  FR_CHECK(hipEventCreateWithFlags(&evh, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evf, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evj, hipEventDisableTiming));
   FR_CHECK(hipEventCreateWithFlags(&evj2, hipEventDisableTiming));
@@ -795,9 +803,10 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   }
   // the momentum deposit is the longer branch: its launches go first
   if (param.TWC())
-    FR_TRY(part.part2gas_velocity(RHS_RU, rk, !front));
+    FR_TRY(part.part2gas_velocity(RHS_RU, rk, !front, vel_head ? evh : nullptr));
   else
     RHS_RU.assign(0);
+  vel_head = false;
   // Heat branch on st2 (gas temperature at the particles, buckets of the new
   // positions, heat deposit, its sum): it follows the concentration deposit
   // on st2 and needs the migrated positions and reference indices.  Nothing
@@ -1185,6 +1194,20 @@ int grid::substep_front(int k) {
     FR_CHECK(hipStreamWaitEvent(st2, evc, 0));
     FR_TRY(part.part2gas_concentration(C, st2));
     step_front = true;
+    // The momentum deposit's buckets are of the stage-1 positions, which
+    // this substep does not change before the deposit (the advance writes
+    // the other stages; the storage re-sort, when due here, reorders, so
+    // then the whole build waits for it): count, scan and unsorted lists go
+    // beside the density update and the advance on st3, ahead of the
+    // migration there; the placement, which takes the advance's forces,
+    // follows the advance on st.
+    const bool resort_now = k == 0 && num_timestep % resort_every == 0;
+    vel_head = param.TWC() && part.dev.np > 0 && !resort_now;
+    if (vel_head) {
+      FR_CHECK(hipStreamWaitEvent(st3, evc, 0));
+      FR_TRY(part.dev.build_head(size.dev(), 1, 1, st3));
+      FR_CHECK(hipEventRecord(evh, st3));
+    }
   }
   FR_TRY(Update_Rho());
   FR_TRY(Update_P0());
