@@ -366,6 +366,9 @@ class PartDev {
   int* dflag = nullptr;  // device flag of the re-sort's stage check
   // tiles of the trilinear deposits left to the dense kernel (count, ids)
   int *dense_c = nullptr, *dense_w = nullptr;
+  double* cic_part = nullptr;  // per-workgroup sums of the weighted split deposit
+  long cic_nparts = 0;
+  bool heat_sum_field = false;  // FR_HEAT_SUM=field: never use them
   double* inj_buf = nullptr;
   int inj_cap = 0;
   bool flg_zero = false;  // the migration flags flg[0..cap] are all zero
@@ -398,8 +401,12 @@ class PartDev {
   int build_tail(const Dev& d, int stage, int mode, hipStream_t st, bool operands = true,
                  const int* rk = nullptr);
   // trilinear deposit from R: weighted by RHS_T (heat) or not (concentration)
+  // sum, summed: when the weighted deposit can form it in the same pass (the
+  // split form), sum[0] takes the interior sum of out after the fold and
+  // *summed is set; otherwise *summed is cleared and sum is not written
   template <typename F>
-  int deposit_cic(const Dev& d, bool weighted, F* out, bool halo, hipStream_t st);
+  int deposit_cic(const Dev& d, bool weighted, F* out, bool halo, hipStream_t st,
+                  double* sum = nullptr, bool* summed = nullptr);
   // momentum deposit from T
   template <typename F>
   int deposit_vel(const Dev& d, F* const out[3], hipStream_t st, bool halo = true);

@@ -1379,12 +1379,19 @@ __device__ __forceinline__ void cic_plane_starts(const Dev& d, const int* __rest
 // Trilinear deposit of particle.cpp:526-559 / 586-658 (before the cumulative
 // halo), split form; every ghosted cell is written.  ops: trilinear records
 // in bucket order.
+// part (W only, may be null): per workgroup, the sum of its outputs (each
+// thread over its planes, then a fixed tree) at part[bk.y * gridDim.x + bk.x].
+// Every deposited value lands in exactly one interior cell after the
+// cumulative ghost fold, so these partials sum to the field's interior sum
+// after the fold (in another order).
 template <typename T, bool W>
 __global__ void __launch_bounds__(SNT) k_dep_cic_split(Dev d, const int* __restrict__ start,
                                                        const int* __restrict__ list,
                                                        const double* __restrict__ ops,
-                                                       T* __restrict__ out) {
+                                                       T* __restrict__ out,
+                                                       double* __restrict__ part) {
   __shared__ int wt[4];
+  __shared__ double wsum[SNT / 64];
   __shared__ int hl[SNT * SKZ];
   // the tile's outputs, stored to memory together once its merged targets
   // are in (whole cache lines, not lines with the merged targets as holes)
@@ -1452,10 +1459,21 @@ __global__ void __launch_bounds__(SNT) k_dep_cic_split(Dev d, const int* __restr
     ob[e >> 8][e & 255] = (T)dep_cic_target_rec<W>(d, start, list, ops, a2, b2, c);
   }
   __syncthreads();
+  double ts = 0.0;
   if (col) {
 #pragma unroll
     for (int k = 0; k < SKZ; k++)
-      if (c0 + k < d.Nz) out[(long)(c0 + k) * d.NxNy + ij] = ob[k][tid];
+      if (c0 + k < d.Nz) {
+        const T v = ob[k][tid];
+        out[(long)(c0 + k) * d.NxNy + ij] = v;
+        ts += (double)v;
+      }
+  }
+  if (W && part) {
+    for (int o = 32; o > 0; o >>= 1) ts += __shfl_xor(ts, o);
+    if ((tid & 63) == 0) wsum[tid >> 6] = ts;
+    __syncthreads();
+    if (tid == 0) part[(long)bk.y * gridDim.x + bk.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
   }
 }
 
@@ -1959,6 +1977,10 @@ int PartDev::alloc(int capacity, const Dev& d) {
                : !strcmp(e, "mixed") ? 3
                : !strcmp(e, "split") ? 4
                                      : 0;
+  // FR_HEAT_SUM=field: the heat sum always from a reduction of the folded
+  // field (the split deposit's own partials add the same values in another
+  // order; the switch lets the deposit forms be compared bit for bit)
+  if (const char* e = getenv("FR_HEAT_SUM")) heat_sum_field = !strcmp(e, "field");
   cap = capacity < 1 ? 1 : capacity;
   ncell = d.size;
   for (int q = 0; q < 7; q++)
@@ -1989,6 +2011,8 @@ int PartDev::alloc(int capacity, const Dev& d) {
   const long ntile = (long)nblocks(d.Nx, DTX) * nblocks(d.Ny, DTY) * d.Nz;
   FR_CHECK(hipMalloc(&dense_c, sizeof(int) * (ntile + 1)));
   FR_CHECK(hipMalloc(&dense_w, sizeof(int) * (ntile + 1)));
+  cic_nparts = (long)nblocks(d.NxNy, SNT) * nblocks(d.Nz, SKZ);
+  FR_CHECK(hipMalloc(&cic_part, sizeof(double) * cic_nparts));
   FR_RET(t_vel.init());
   FR_RET(t_cic.init());
   return FR_OK;
@@ -2005,7 +2029,7 @@ void PartDev::release() {
   void* all[] = {flag, rhs_u, rhs_v, rhs_w, rhs_T, alt.flag, alt.rhs_u, alt.rhs_v, alt.rhs_w,
                  alt.rhs_T, rank, rank_alt, inv, mig_tmp, rank_dep, flg, nl, pNL,
                  holepos, tailnl, evt, islo, plo, dst, mslot, mlv, inj_buf, dense_c, dense_w,
-                 gas[0], gas[1], gas[2], gas[3], dflag};
+                 gas[0], gas[1], gas[2], gas[3], dflag, cic_part};
   for (void* p : all) (void)hipFree(p);
   for (auto& g : gas) g = nullptr;
   gas_cap = 0;
@@ -2150,8 +2174,10 @@ int PartDev::build_tail(const Dev& d, int stage, int mode, hipStream_t st, bool 
 static dim3 grid3p(const Dev& d) { return dim3(nblocks(d.Nx, 64), nblocks(d.Ny, 4), d.Nz); }
 
 template <typename T>
-int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStream_t st) {
+int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStream_t st,
+                         double* sum, bool* summed) {
   if (!R.valid) return FR_EINVAL;
+  if (summed) *summed = false;
   // Dense tiles (over PCAP contributions) get their own kernel when the mean
   // tile holds more than PCAP/2 contributions; at lower mean densities the
   // few dense tiles merge in place, which spares every call a launch that
@@ -2162,16 +2188,23 @@ int PartDev::deposit_cic(const Dev& d, bool weighted, T* out, bool halo, hipStre
     hipEvent_t e0 = nullptr, e1 = nullptr;
     t_cic.next(&e0, &e1);
     const dim3 gs(nblocks(d.NxNy, SNT), nblocks(d.Nz, SKZ));
+    // the weighted deposit's interior sum after the fold, from its partials
+    const bool part = kind == DEP_SPLIT && weighted && sum && summed && !heat_sum_field &&
+                      (long)gs.x * gs.y <= cic_nparts;
     if (kind == DEP_SPLIT && weighted)
       hipExtLaunchKernelGGL((k_dep_cic_split<T, true>), gs, dim3(SNT), 0, st, e0, e1, 0, d,
-                            R.start, R.list, R.rec, out);
+                            R.start, R.list, R.rec, out, part ? cic_part : nullptr);
     else if (kind == DEP_SPLIT)
       hipExtLaunchKernelGGL((k_dep_cic_split<T, false>), gs, dim3(SNT), 0, st, e0, e1, 0, d,
-                            R.start, R.list, R.rec, out);
+                            R.start, R.list, R.rec, out, nullptr);
     else
       hipExtLaunchKernelGGL(k_dep_cic_pull<T>, g, dim3(DTX, DTY), 0, st, e0, e1, 0, d, R.start,
                             R.list, R.s[0], R.s[1], R.s[2], weighted ? R.s[3] : nullptr, out);
     FR_LAUNCH_CHECK();
+    if (part) {
+      FR_RET(sum_parts(cic_part, (long)gs.x * gs.y, sum, st));
+      *summed = true;
+    }
     int rc = cum_fold(d, out, st);
     if (!rc && halo) rc = ghost_fill(d, out, st);
     return rc;
@@ -2406,8 +2439,10 @@ template int PartDev::advance<float>(const Dev&, const AdvCoef&, const float* co
                                      hipStream_t);
 template int PartDev::temp<double>(const Dev&, int, double, const double*, double, hipStream_t);
 template int PartDev::temp<float>(const Dev&, int, double, const float*, double, hipStream_t);
-template int PartDev::deposit_cic<double>(const Dev&, bool, double*, bool, hipStream_t);
-template int PartDev::deposit_cic<float>(const Dev&, bool, float*, bool, hipStream_t);
+template int PartDev::deposit_cic<double>(const Dev&, bool, double*, bool, hipStream_t, double*,
+                                          bool*);
+template int PartDev::deposit_cic<float>(const Dev&, bool, float*, bool, hipStream_t, double*,
+                                         bool*);
 template int PartDev::deposit_vel<double>(const Dev&, double* const[3], hipStream_t, bool);
 template int PartDev::deposit_vel<float>(const Dev&, float* const[3], hipStream_t, bool);
 

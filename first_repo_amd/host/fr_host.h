@@ -347,7 +347,10 @@ class particle {
   int Send_Recv_duct_dev(const double* rows, int n);
   // gas2part_Temp_{new,int} + Compute_RHS_Temp_{new,int} (402-478)
   int temp(int stage, double c0, const tensor0& Rho, hipStream_t on = 0);
-  int part2gas_Temp(int stage, tensor0& A, hipStream_t on = 0);  // particle.cpp:586-658
+  // sum, summed: see PartDev::deposit_cic (the interior sum of A after the
+  // fold, formed by the deposit itself where its form allows)
+  int part2gas_Temp(int stage, tensor0& A, hipStream_t on = 0, double* sum = nullptr,
+                    bool* summed = nullptr);  // particle.cpp:586-658
   int rebuild_cells(int stage, hipStream_t on = 0);
 
   // ---- the reference's member surface (particle.h:126-154), one member at

@@ -393,11 +393,11 @@ int particle::temp(int stage, double c0, const tensor0& Rho, hipStream_t on) {
 }
 
 // Requires rebuild_cells(stage) after the RHS_T of that stage was formed.
-int particle::part2gas_Temp(int stage, tensor0& A, hipStream_t on) {
+int particle::part2gas_Temp(int stage, tensor0& A, hipStream_t on, double* sum, bool* summed) {
   (void)stage;
   const hipStream_t s = on ? on : st_;
-  if (A.is_f32()) return dev.deposit_cic(size_->dev(), true, A.p<float>(), true, s);
-  return dev.deposit_cic(size_->dev(), true, A.pointer(), true, s);
+  if (A.is_f32()) return dev.deposit_cic(size_->dev(), true, A.p<float>(), true, s, sum, summed);
+  return dev.deposit_cic(size_->dev(), true, A.pointer(), true, s, sum, summed);
 }
 
 int particle::rebuild_cells(int stage, hipStream_t on) {
@@ -862,11 +862,14 @@ int grid::launch_side() {
   side_held = false;
   FR_TRY(part.temp(2, P0_new / param.R(), Rho_new, st2));
   FR_TRY(part.rebuild_cells(2, st2));
-  FR_TRY(part.part2gas_Temp(2, RHS_Part_Temp, st2));
+  // the heat sum (Update_P0's mean) from the deposit's own partials where
+  // its form provides them, else a reduction of the folded field
+  bool summed = false;
+  FR_TRY(part.part2gas_Temp(2, RHS_Part_Temp, st2, dq + RED_PARTS, &summed));
   FR_CHECK(hipEventRecord(ev[3], st2));
-  if (RHS_Part_Temp.is_f32())
+  if (!summed && RHS_Part_Temp.is_f32())
     FR_TRY(reduce_async_ws(size.dev(), RHS_Part_Temp.p<float>(), 0, dq + RED_PARTS, dq, st2));
-  else
+  else if (!summed)
     FR_TRY(reduce_async_ws(size.dev(), RHS_Part_Temp.pointer(), 0, dq + RED_PARTS, dq, st2));
   FR_CHECK(hipEventRecord(evj2, st2));
   return FR_OK;

@@ -133,17 +133,22 @@ def compare_state(og, sim, tol=1e-12, fields1=FIELDS1, fields0=FIELDS0, particle
     return worst, bad
 
 
-def sim_with_deposit(params, d, form=None, **kw):
+def sim_with_deposit(params, d, form=None, heat_sum=None, **kw):
     """fr.Sim whose particle deposits use `form` ("pull": one thread per
     target cell; "tile": LDS tiles; "split": direct sums for targets with at
     most two contributions, merges for the rest; None: chosen by density).  The form is
-    read from FR_DEPOSIT when the particle set is allocated."""
-    old = os.environ.pop("FR_DEPOSIT", None)
+    read from FR_DEPOSIT when the particle set is allocated.  heat_sum="field":
+    the heat deposit's sum always from a reduction of the folded field
+    (FR_HEAT_SUM), not from the split deposit's own partials."""
+    env = {"FR_DEPOSIT": form, "FR_HEAT_SUM": heat_sum}
+    old = {k: os.environ.pop(k, None) for k in env}
     try:
-        if form:
-            os.environ["FR_DEPOSIT"] = form
+        for k, v in env.items():
+            if v:
+                os.environ[k] = v
         return fr.Sim(params, d, **kw)
     finally:
-        os.environ.pop("FR_DEPOSIT", None)
-        if old is not None:
-            os.environ["FR_DEPOSIT"] = old
+        for k, v in old.items():
+            os.environ.pop(k, None)
+            if v is not None:
+                os.environ[k] = v

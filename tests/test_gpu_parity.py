@@ -352,11 +352,13 @@ def test_deposit_forms_agree_bit_for_bit(tmp_path):
     merges of the rest, operand records) and the pull form (every target
     merged) give the reference's ordered sums, so three full steps agree bit
     for bit in the momentum and density fields and the particles, and the
-    last momentum deposit RHS_RU agrees over the interior."""
+    last momentum deposit RHS_RU agrees over the interior.  The heat sum comes
+    from the folded field in both (the split deposit's own partials add the
+    same values in another order)."""
     params, d = make_box_case(tmp_path, n=16, n_particles=4000, seed=13)
-    a = sim_with_deposit(params, d, "split")
+    a = sim_with_deposit(params, d, "split", heat_sum="field")
     a.initialize()
-    b = sim_with_deposit(params, d, "pull")
+    b = sim_with_deposit(params, d, "pull", heat_sum="field")
     b.initialize()
     for step in range(3):
         a.step()
@@ -573,3 +575,25 @@ def test_cell_density_residual_equals_face_form(tmp_path, fp32):
     assert np.array_equal(p_cell, p_face)
     assert not np.array_equal(p_cell, p0)
     assert it1 - it0 == it2 - it1 > 0
+
+
+@pytest.mark.gpu
+def test_heat_sum_from_deposit_partials(tmp_path):
+    """The split heat deposit sums its own outputs before the cumulative fold
+    (each deposited value lands in one interior cell after it); the steps
+    agree with those that reduce the folded field to rounding."""
+    params, d = make_box_case(tmp_path, n=16, n_particles=4000, seed=5)
+    a = sim_with_deposit(params, d, "split")
+    a.initialize()
+    b = sim_with_deposit(params, d, "split", heat_sum="field")
+    b.initialize()
+    for _ in range(3):
+        a.step()
+        b.step()
+    for f in ("Rho", "RHS_Part_Temp", "P"):
+        e = rel(a.field(f), b.field(f))
+        print(f"heat sum partials vs field: {f} {e:.3e}", flush=True)
+        assert e <= 1e-12, f
+    for c in range(3):
+        assert rel(a.field("RU", c), b.field("RU", c)) <= 1e-12
+    assert np.array_equal(a.flags(), b.flags())
