@@ -105,8 +105,9 @@ def kernel_table(n, particles, fp32, r, steps, precision="fp64"):
         weights, plus RHS_T for the heat deposit (mean of the two per launch);
       k_update_ru_wop_z: 21 values per ghosted cell (reads U, RU_int, S, RU,
         RU_np1; writes RU_np1, RU_new) in the middle stages of a step, 18 in
-        the first (RU stands in for RU_np1, which is not read) and the last
-        (RU_np1 is not written): 19.5 per launch over a step's four."""
+        the first (RU stands in for RU_np1, which is not read), 15 in the last
+        (RU_np1 is not written, RU is not read): 18.75 per launch over a
+        step's four."""
     es = 4 if fp32 else 8
     g = r["ghosted"]
     t = r["timing"]
@@ -120,7 +121,7 @@ def kernel_table(n, particles, fp32, r, steps, precision="fp64"):
         ("k_dep_cic", "concentration and heat deposits, particle.cpp:526-658 (events bound "
          "to the launch)", g * (es + 4) + particles * 32, t[10], t[11]),
         ("k_update_ru_wop_z", "fused momentum update, grid.cpp:387-425 (events around the "
-         "launch; runs beside the particle heat branch)", 78 * es * g // 4, t[4], t[5]),
+         "launch; runs beside the particle heat branch)", 75 * es * g // 4, t[4], t[5]),
     ]
     out = []
     for name, what, nbytes, ms, nl in rows:

@@ -170,7 +170,8 @@ constexpr int WPT = (WPL + WX * WY - 1) / (WX * WY);  // per thread
 
 // MODE (the BoxCoef flags, as template arguments): WOP_PLAIN; WOP_BASE
 // (first stage: RU stands in for RU_np1, which is not read); WOP_LEAN (last
-// stage of a full step: RU_np1 is not written, RU_new takes its value).
+// stage of a full step: RU_np1 is not written, RU_new takes its value, and
+// RU, which the last stage does not use, is not read).
 enum { WOP_PLAIN, WOP_BASE, WOP_LEAN };
 template <int GRAV, int MODE, typename T>
 __global__ void __launch_bounds__(WX * WY) k_update_ru_wop_z(
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(WX * WY) k_update_ru_wop_z(
     for (int m = 0; m < 3; m++) {
       o[m] = Sd[m][c];
       o[3 + m] = RI[m][c];
-      o[6 + m] = R0[m][c];
+      if constexpr (MODE != WOP_LEAN) o[6 + m] = R0[m][c];  // the last stage does not use RU
       if constexpr (MODE != WOP_BASE) o[9 + m] = N1[m][c];
       if constexpr (GRAV != 0) o[12 + m] = RF[m][c];
     }
@@ -285,7 +286,10 @@ __global__ void __launch_bounds__(WX * WY) k_update_ru_wop_z(
       if constexpr (GRAV != 0) r = r + (op[12 + m] - (T)w.rho0) * gv[m];
       const T np1 = (MODE == WOP_BASE ? op[6 + m] : op[9 + m]) + (T)cf.cpost * r;
       if constexpr (MODE != WOP_LEAN) N1[m][c] = np1;
-      NW[m][c] = cf.last ? np1 : 1 * op[6 + m] + (T)cf.cpre * r;
+      if constexpr (MODE == WOP_LEAN)
+        NW[m][c] = np1;
+      else
+        NW[m][c] = cf.last ? np1 : 1 * op[6 + m] + (T)cf.cpre * r;
     }
 #pragma unroll
     for (int q = 0; q < NOP; q++) op[q] = opn[q];
