@@ -552,7 +552,9 @@ class grid {
   bool face_stored = true;
   int fill_faces();
   bool lean_stage() const { return in_step && param.IsBox() && !pair_out && RK4_count == 3; }
-  int fill_np1();
+  int fill_np1();  // both; each method fills only the one it reads
+  int fill_rho_np1();
+  int fill_ru_np1();
   hipEvent_t evc = nullptr;
   int launch_side();
   double* dq = nullptr;  // reduction workspace + result of that branch

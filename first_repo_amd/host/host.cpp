@@ -700,7 +700,7 @@ int grid::Update_Rho() {
   if (rho_np1_stale && RK4_count == 0)
     cf.np1_base = 1;  // Rho_np1 = Rho: read Rho in its place
   else
-    FR_TRY(fill_np1());
+    FR_TRY(fill_rho_np1());
   cf.lean = lean_stage();
   if (f32) {
     const float* ri_[3] = {RU_int.x.p<float>(), RU_int.y.p<float>(), RU_int.z.p<float>()};
@@ -721,10 +721,18 @@ int grid::Update_Rho() {
 }
 
 int grid::fill_np1() {
-  if (ru_lean) return FR_EINVAL;  // inside a full step's last stage only
+  FR_TRY(fill_rho_np1());
+  return fill_ru_np1();
+}
+int grid::fill_rho_np1() {
   if (rho_np1_stale) Rho_np1.copy(Rho);
+  rho_np1_stale = false;
+  return FR_OK;
+}
+int grid::fill_ru_np1() {
+  if (ru_lean) return FR_EINVAL;  // inside a full step's last stage only
   if (ru_np1_stale) RU_np1.copy(RU);
-  rho_np1_stale = ru_np1_stale = false;
+  ru_np1_stale = false;
   return FR_OK;
 }
 
@@ -929,7 +937,7 @@ int grid::Update_RU_WOP() {  // grid.cpp:387-425
   if (param.IsBox() && ru_np1_stale && RK4_count == 0)
     cf.np1_base = 1;  // RU_np1 = RU: read RU in its place
   else
-    FR_TRY(fill_np1());
+    FR_TRY(fill_ru_np1());
   if (param.IsBox()) cf.lean = lean_stage();
   hipEventRecord(evk[2 * (RK4_count & 3)], st);
   if (param.IsBox() && f32) {
@@ -1092,7 +1100,7 @@ int grid::Update_RU_WP() {  // grid.cpp:651-657
   else if (ru_lean)
     return FR_EINVAL;
   else
-    FR_TRY(fill_np1());
+    FR_TRY(fill_ru_np1());
   double* sums = dmz + RED_PARTS + 8;
   if (param.IsBox() && f32) {
     float* n1_[3] = {RU_np1.x.p<float>(), RU_np1.y.p<float>(), RU_np1.z.p<float>()};

@@ -275,9 +275,9 @@ int grid::Restart() {
 // the reference's expressions (stats.hip); plane means are tree sums, so a
 // profile agrees with the reference's loop to rounding.
 int grid::Statistics() {
+  if (!param.Statistics_) return FR_OK;
   FR_TRY(fill_np1());
   FR_TRY(fill_faces());
-  if (!param.Statistics_) return FR_OK;
   FR_TRY(settle_side());
   const Dev d = size.dev();
   const int Nx = size.Nx();
