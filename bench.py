@@ -46,13 +46,14 @@ METRIC = ("Mcell·steps/s full RK4 step w/ particles; "
           "Poisson iters+ms to 1e-10 vs plain")
 # HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE (each scaled
 # by what the counters read for a plain copy of known size: factors 2.0 and
-# 1.0), measured on the named workloads (profiles/hbm_traffic_*.md)
-TRAFFIC = {("k_pass_x_fwd", 512, 16_000_000, "fp32"): 5.076e9,
-           ("k_update_ru_wop_z", 512, 16_000_000, "fp32"): 12.042e9,
-           ("k_pass_x_fwd", 512, 16_000_000, "fp64"): 7.974e9,
-           ("k_update_ru_wop_z", 512, 16_000_000, "fp64"): 24.793e9,
+# 1.0), measured on the named workloads (profiles/hbm_traffic_*.md); the
+# momentum update's is the mean over a step's four stage forms
+TRAFFIC = {("k_pass_x_fwd", 512, 16_000_000, "fp32"): 3.775e9,
+           ("k_update_ru_wop_z", 512, 16_000_000, "fp32"): 10.991e9,
+           ("k_pass_x_fwd", 512, 16_000_000, "fp64"): 5.488e9,
+           ("k_update_ru_wop_z", 512, 16_000_000, "fp64"): 22.150e9,
            ("k_dep_vel", 512, 16_000_000, "fp64"): 5.248e9,
-           ("k_dep_cic", 512, 16_000_000, "fp64"): 3.340e9}
+           ("k_dep_cic", 512, 16_000_000, "fp64"): 2.682e9}
 
 
 def box_workload(n, particles, solver="fft"):
