@@ -386,7 +386,9 @@ class PartDev {
   void release();
   PartPtrs ptrs() const;
   template <typename F>
-  int advance(const Dev& d, const AdvCoef& c, const F* const u[3], hipStream_t st);
+  // eq: stage slots 1 and 3 hold slot 0's values (the top of a full step)
+  int advance(const Dev& d, const AdvCoef& c, const F* const u[3], hipStream_t st,
+              bool eq = false);
   template <typename F>
   int temp(const Dev& d, int stage, double c0, const F* rho, double iTth, hipStream_t st);
   // buckets of the `stage` positions; mode 0 rounded (R), 1 truncating (T)

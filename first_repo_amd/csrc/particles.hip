@@ -58,13 +58,29 @@ __device__ __forceinline__ double tp_of_flag(int f, double lo, double span) {
 // update_velocity; update_position and update_Temp need no gather).
 constexpr int PH_GATHER = 1, PH_POS = 2, PH_VEL = 4, PH_TEMP = 8, PH_ALL = 15;
 
-template <typename T, int PH>
+// EQ (first stage of a step whose stage slots 1 and 3 equal slot 0, as a
+// full step leaves them): slot 0 is read once in place of all three.
+template <typename T, int PH, bool EQ = false>
 __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict__ ux,
                                const T* __restrict__ uy, const T* __restrict__ uz,
                                double* __restrict__ gu, double* __restrict__ gv,
                                double* __restrict__ gw) {
   const int i = xcd_lin() * blockDim.x + threadIdx.x;
   if (i >= p.np) return;
+  double e0[7];
+  if constexpr (EQ) {
+#pragma unroll
+    for (int q = 0; q < 7; q++) e0[q] = p.s[q][0][i];
+  }
+  auto S0 = [&](int q) -> double {
+    if constexpr (EQ) return e0[q]; else return p.s[q][0][i];
+  };
+  auto S1 = [&](int q) -> double {
+    if constexpr (EQ) return e0[q]; else return p.s[q][1][i];
+  };
+  auto S3 = [&](int q) -> double {
+    if constexpr (EQ) return e0[q]; else return p.s[q][3][i];
+  };
   // T = float (precision form): the gas-side arithmetic (interpolation
   // weights, interpolated velocity, drag right-hand side) runs in FP32; the
   // particle state and its RK4 accumulation stay FP64.  T = double is the
@@ -72,7 +88,7 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
   using Cm = T;
   Cm ug = 0, vg = 0, wg = 0;
   if (PH & PH_GATHER) {
-    const double xi = p.s[0][1][i], yi = p.s[1][1][i], zi = p.s[2][1][i];
+    const double xi = S1(0), yi = S1(1), zi = S1(2);
     const int ti = (int)(xi * d.idx), tj = (int)(yi * d.idy), tk = (int)(zi * d.idz);
     const Cm x_l = (Cm)(xi * d.idx - ((int)(xi * d.idx)));
     const Cm y_l = (Cm)(yi * d.idy - ((int)(yi * d.idy)));
@@ -88,14 +104,14 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
       gw[i] = (double)wg;
     }
   }
-  const double ui = p.s[3][1][i], vi = p.s[4][1][i], wi = p.s[5][1][i];
+  const double ui = S1(3), vi = S1(4), wi = S1(5);
   if (PH & PH_POS) {
     const double X[3] = {ui, vi, wi};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      const double np1 = p.s[a][3][i] + c.dt_post * X[a];
+      const double np1 = S3(a) + c.dt_post * X[a];
       p.s[a][3][i] = np1;
-      p.s[a][2][i] = c.last ? np1 : p.s[a][0][i] + c.dt_pre * X[a];
+      p.s[a][2][i] = c.last ? np1 : S0(a) + c.dt_pre * X[a];
       if (c.set0) p.s[a][0][i] = np1;
     }
   }
@@ -117,17 +133,17 @@ __global__ void k_part_advance(Dev d, PartPtrs p, AdvCoef c, const T* __restrict
     const double G[3] = {c.gx, c.gy, c.gz};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      const double np1 = p.s[3 + a][3][i] + c.dt_post * (R[a] + G[a]);
+      const double np1 = S3(3 + a) + c.dt_post * (R[a] + G[a]);
       p.s[3 + a][3][i] = np1;
-      p.s[3 + a][2][i] = c.last ? np1 : p.s[3 + a][0][i] + c.dt_pre * (R[a] + G[a]);
+      p.s[3 + a][2][i] = c.last ? np1 : S0(3 + a) + c.dt_pre * (R[a] + G[a]);
       if (c.set0) p.s[3 + a][0][i] = np1;
     }
   }
   if (PH & PH_TEMP) {  // RHS_T from the previous stage
     const double rt = p.rhs_T[i];
-    const double tn = p.s[6][3][i] + c.dtpost_iS * (rt + c.alpha);
+    const double tn = S3(6) + c.dtpost_iS * (rt + c.alpha);
     p.s[6][3][i] = tn;
-    p.s[6][2][i] = c.last ? tn : p.s[6][0][i] + c.dtpre_iS * (rt + c.alpha);
+    p.s[6][2][i] = c.last ? tn : S0(6) + c.dtpre_iS * (rt + c.alpha);
     if (c.set0) p.s[6][0][i] = tn;
   }
 }
@@ -2052,10 +2068,16 @@ PartPtrs PartDev::ptrs() const {
 }
 
 template <typename T>
-int PartDev::advance(const Dev& d, const AdvCoef& c, const T* const u[3], hipStream_t st) {
+int PartDev::advance(const Dev& d, const AdvCoef& c, const T* const u[3], hipStream_t st,
+                     bool eq) {
   if (np == 0) return FR_OK;
-  hipLaunchKernelGGL((k_part_advance<T, PH_ALL>), dim3(nblocks(np, 256)), dim3(256), 0, st, d,
-                     ptrs(), c, u[0], u[1], u[2], nullptr, nullptr, nullptr);
+  if (eq && (c.last || c.set0)) return FR_EINVAL;
+  if (eq)
+    hipLaunchKernelGGL((k_part_advance<T, PH_ALL, true>), dim3(nblocks(np, 256)), dim3(256), 0,
+                       st, d, ptrs(), c, u[0], u[1], u[2], nullptr, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((k_part_advance<T, PH_ALL>), dim3(nblocks(np, 256)), dim3(256), 0, st, d,
+                       ptrs(), c, u[0], u[1], u[2], nullptr, nullptr, nullptr);
   FR_LAUNCH_CHECK();
   return FR_OK;
 }
@@ -2434,9 +2456,9 @@ int PartDev::set_flags(const int* host, hipStream_t st) {
 }
 
 template int PartDev::advance<double>(const Dev&, const AdvCoef&, const double* const[3],
-                                      hipStream_t);
+                                      hipStream_t, bool);
 template int PartDev::advance<float>(const Dev&, const AdvCoef&, const float* const[3],
-                                     hipStream_t);
+                                     hipStream_t, bool);
 template int PartDev::temp<double>(const Dev&, int, double, const double*, double, hipStream_t);
 template int PartDev::temp<float>(const Dev&, int, double, const float*, double, hipStream_t);
 template int PartDev::deposit_cic<double>(const Dev&, bool, double*, bool, hipStream_t, double*,

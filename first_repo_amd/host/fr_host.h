@@ -328,7 +328,7 @@ class particle {
   int load_random();  // particle.cpp:919-958
   // particle.cpp:340-399 + 314-338 + 480-496 (gather and the three ODE updates);
   // set0 (last stage of a full step): stage 0 takes the stage-3 values too
-  int advance(int k, double pre, double post, tensor1& U, bool set0 = false);
+  int advance(int k, double pre, double post, tensor1& U, bool set0 = false, bool eq = false);
   int part2gas_concentration(tensor0& A, hipStream_t on = 0);  // particle.cpp:526-559
   // rk: reference indices to order the deposit by (a copy taken before a
   // migration that runs beside it); default the current ones
@@ -532,6 +532,9 @@ class grid {
   bool rhs_mean_pending = false;
   int fill_rhs_pois();
   bool rho_set = false, ru_set = false, part_set = false;
+  // particle stage slots 1 and 3 hold slot 0's values (set by a full box
+  // step's end, cleared by the next advance and by any outside change)
+  bool stages_equal = false;
   // A full box step (not coupled) ends with Rho_np1 = Rho and RU_np1 = RU
   // without writing either: its last stage leaves them out (the density
   // update writes Rho directly; the momentum updates finish RU_np1 in RU_new

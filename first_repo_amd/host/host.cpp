@@ -307,7 +307,7 @@ int particle::load_random() {
   return FR_OK;
 }
 
-int particle::advance(int k, double pre, double post, tensor1& U, bool set0) {
+int particle::advance(int k, double pre, double post, tensor1& U, bool set0, bool eq) {
   const params& p = *param_;
   AdvCoef c;
   const double dt = p.dt();
@@ -328,10 +328,10 @@ int particle::advance(int k, double pre, double post, tensor1& U, bool set0) {
   c.set0 = set0 && c.last;
   if (U.x.is_f32()) {
     const float* u[3] = {U.x.p<float>(), U.y.p<float>(), U.z.p<float>()};
-    return dev.advance(size_->dev(), c, u, st_);
+    return dev.advance(size_->dev(), c, u, st_, eq);
   }
   const double* u[3] = {U.x.pointer(), U.y.pointer(), U.z.pointer()};
-  return dev.advance(size_->dev(), c, u, st_);
+  return dev.advance(size_->dev(), c, u, st_, eq);
 }
 
 // The rounded-index buckets were built on the _new positions at the end of
@@ -784,8 +784,11 @@ int grid::Update_Particle() {  // grid.cpp:659-680
   // the coupled box's '>hi' rows carry stage 0 as the reference has it before
   // TimeAdvance, so the copy stays in TimeAdvance there
   part_set = in_step && param.IsBox() && !pair_out && RK4_count == 3;
+  // the first stage of a step after a full step: slots 1 and 3 hold slot 0
+  const bool eq = in_step && RK4_count == 0 && stages_equal;
+  stages_equal = false;
   FR_TRY(part.advance(RK4_count, RK4_preCoeff[RK4_count], RK4_postCoeff[RK4_count], U,
-                      part_set));
+                      part_set, eq));
   // The concentration deposit (rounded-index buckets, output C) and the
   // momentum deposit (truncating-index buckets, output RHS_RU) read the same
   // particles and write disjoint data: both are latency-bound, so they run
@@ -1176,6 +1179,9 @@ int grid::TimeAdvance() {  // grid.cpp:315-325
   if (!ru_set) RU.copy(RU_np1);
   P0 = P0_np1;
   if (!part_set) FR_TRY(part.dev.copy_stage(0, 3, st));
+  // a full step's last stage wrote its result to slots 0, 2 and 3, the
+  // migration wrapped them alike and the stage exchange made slot 1 slot 2
+  stages_equal = part_set;
   rho_set = ru_set = part_set = false;
   T_cur += param.dt();
   num_timestep++;
@@ -1514,6 +1520,7 @@ int fr_sim_get_particles(fr_sim* s, int q, int st, double* host) {
 int fr_sim_set_particles(fr_sim* s, int np, int q, int st, const double* host) {
   if (!s || !host || q < 0 || q > 7 || st < 0 || st > 3) return FR_EINVAL;
   FR_TRY_S(s->g.settle_side());
+  s->g.stages_equal = false;
   PartDev& p = s->g.part.dev;
   if (np > p.cap || np < 0) return FR_EOVERFLOW;
   if (np != p.np) {  // a new particle set: storage in reference order
@@ -1535,6 +1542,7 @@ int fr_sim_get_flags(fr_sim* s, int* host) {
 int fr_sim_set_flags(fr_sim* s, int np, const int* host) {
   if (!s || !host) return FR_EINVAL;
   FR_TRY_S(s->g.settle_side());
+  s->g.stages_equal = false;
   PartDev& p = s->g.part.dev;
   if (np > p.cap || np < 0) return FR_EOVERFLOW;
   if (np != p.np) {
@@ -1549,6 +1557,7 @@ int fr_sim_set_flags(fr_sim* s, int np, const int* host) {
 int fr_sim_particles_changed(fr_sim* s) {
   if (!s) return FR_EINVAL;
   FR_TRY_S(s->g.settle_side());
+  s->g.stages_equal = false;
   int rc = s->g.part.rebuild_cells(1);
   if (!rc) rc = (int)hipStreamSynchronize(s->g.st);
   return rc;
